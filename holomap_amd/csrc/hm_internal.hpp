@@ -1,0 +1,137 @@
+// Internal definitions shared by the HIP kernels and the host runtime of the
+// holomap kNN-graph library (gfx950 / CDNA4, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HM_WAVE 64
+
+namespace hm {
+
+// ---------------------------------------------------------------------------
+// Per-dataset descriptor, resident in device memory. One per dataset of a
+// batch; a single fit uses a batch of one. Row ids inside a dataset are local
+// (0..n-1); `row_off` places the dataset's rows in the batch-global row space
+// that per-row arrays (flags, sigma, rho, ...) are indexed by.
+// ---------------------------------------------------------------------------
+struct DsDesc {
+  const float* x;        // raw input, n x d, row-major (caller's layout)
+  const float* xr;       // raw input padded: n_pad x kp f32 (zero-filled)
+  const float* op32;     // candidate-selection operand, f32: n_pad x kp
+                         //   (euclidean: == xr; cosine: rows / ||row||)
+  const uint16_t* op16;  // same operand in bf16 (RNE), n_pad x kp, or null
+  const float* colbias;  // n_pad: euclid ||x_j||^2 (f32); cosine 0 / -inf
+  const double* rnorm;   // n: cosine sqrt(sum x^2) (f64, reference fold)
+  const float* rstat;    // n x HM_NSTAT per-row bound statistics
+  const float* smax;     // HM_NSTAT maxima over rows (per dataset)
+  uint64_t* cand;        // candidate keys: n x nsplit x C
+  uint64_t* ctau;        // thresholds: n x nsplit
+  uint32_t* knn_idx;     // n x k (output)
+  float* knn_dist;       // n x k (output)
+  int64_t n;             // rows
+  int64_t n_pad;         // rows padded to the column-block size
+  int64_t row_off;       // first global row id
+  int64_t edge_off;      // first global edge id (= sum of n*k before)
+  int32_t d;             // features
+  int32_t kp;            // padded feature stride (elements)
+  int32_t k;             // neighbours
+  int32_t metric;        // 0 euclidean, 1 cosine
+  int32_t C;             // candidates kept per (row, split)
+  int32_t nsplit;        // column splits of the candidate sweep
+  int32_t x_aligned;     // unused padding
+  int32_t pad_;
+};
+
+// Number of float statistics kept per row for the candidate error bound.
+#define HM_NSTAT 8
+// Stat slots.
+enum : int {
+  ST_NZ = 0,   // f32 ||x||^2 as used in keys (euclid), 0 for cosine
+  ST_XN = 1,   // ||xhat|| of the bf16 (or f32) representation
+  ST_EN = 2,   // ||x - xhat||  (representation error, rounded up)
+  ST_XO = 3,   // ||x|| of the (normalised) f32 operand
+  ST_HN = 4,   // bf16x3: ||hi||
+  ST_LN = 5,   // bf16x3: ||lo||
+  ST_HLN = 6,  // bf16x3: ||hi + lo||
+  ST_RN = 7,   // bf16x3: ||x - hi - lo||
+};
+
+// Work item of the candidate sweep / exact scan: a block of query rows of
+// one dataset against one column split.
+struct WorkItem {
+  int32_t ds;      // dataset
+  int32_t split;   // column split
+  int32_t qbeg;    // first query (row id, or index into a row list)
+  int32_t qcnt;    // number of queries in the block
+};
+
+// Precision / mode codes (mirrored in the public header).
+enum : int { PREC_EXACT = 0, PREC_FP32 = 1, PREC_BF16 = 2, PREC_BF16X3 = 3 };
+enum : int { CALIB_REF_F32 = 0, CALIB_F64 = 1 };
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t f2u(float x) { return __float_as_uint(x); }
+__device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
+
+// Order-preserving map of a float to uint32 (total order, -0 < +0).
+__device__ __forceinline__ uint32_t ordered_u32(float x) {
+  uint32_t u = __float_as_uint(x);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unordered_f32(uint32_t o) {
+  uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+  return __uint_as_float(u);
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  lo = __shfl_xor(lo, m, 64);
+  hi = __shfl_xor(hi, m, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
+  uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+  uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Ascending bitonic sort of one u64 per lane across the 64-lane wave.
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      uint64_t o = shfl_xor_u64(v, j);
+      bool up = (lane & k) == 0;
+      bool lower = (lane & j) == 0;
+      uint64_t mn = v < o ? v : o, mx = v < o ? o : v;
+      v = (lower == up) ? mn : mx;
+    }
+  }
+  return v;
+}
+
+// Ascending bitonic sort of `len` (power of two, 64..1024) u64 keys held in
+// LDS by one wave. Caller provides lane-synchronous access (single wave).
+__device__ __forceinline__ void wave_sort_lds(uint64_t* a, int len) {
+  const int lane = threadIdx.x & 63;
+  for (int k = 2; k <= len; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < len; i += 64) {
+        int p = i ^ j;
+        if (p > i) {
+          uint64_t x = a[i], y = a[p];
+          bool up = (i & k) == 0;
+          if ((x > y) == up) { a[i] = y; a[p] = x; }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+  }
+}
+
+}  // namespace hm

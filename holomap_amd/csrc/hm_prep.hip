@@ -1,0 +1,188 @@
+// Data preparation for the candidate sweep.
+//
+// For every row of every dataset of a batch:
+//   * rownorm_kernel  (thread per row): the reference's sequential f64 fold
+//     sum_f x_f^2 (src/metric.rs:61-66 keeps na/nb as such folds), giving the
+//     cosine denominator factor sqrt(na) bit-exactly and the Euclidean key
+//     bias ||x||^2 rounded to f32;
+//   * prep_kernel     (wave per row): a zero-padded f32 copy (n_pad x kp),
+//     the candidate-selection operand (euclid: the copy itself; cosine: the
+//     row scaled to unit length), its bf16 (RNE) image, and the statistics
+//     the certification bound needs (representation error norms);
+//   * statmax_kernel: per-dataset maxima of the statistics (max is exact and
+//     order-free, computed by a fixed two-level tree).
+#include "hm_internal.hpp"
+
+namespace hm {
+
+__device__ __forceinline__ int find_ds(const DsDesc* ds, int nds, int64_t grow) {
+  int lo = 0, hi = nds - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (ds[mid].row_off <= grow) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  uint32_t r = ((u >> 16) & 1u) + 0x7fffu;
+  return (uint16_t)((u + r) >> 16);  // inputs are finite
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) {
+  return __uint_as_float((uint32_t)h << 16);
+}
+
+// Thread per global row: sequential f64 fold of x_f^2 in feature order.
+__global__ void rownorm_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
+                               float* __restrict__ nz_out /* global rows */,
+                               double* __restrict__ rn_out /* global rows */) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total_rows) return;
+  int di = find_ds(ds, nds, g);
+  const DsDesc& D = ds[di];
+  int64_t i = g - D.row_off;
+  const float* x = D.x + i * D.d;
+  double s = 0.0;
+  for (int f = 0; f < D.d; ++f) {
+    double v = (double)x[f];
+    s = s + v * v;
+  }
+  nz_out[g] = (float)s;
+  rn_out[g] = sqrt(s);
+}
+
+// Wave per (dataset, padded row). Writes xr, op32 (cosine), op16, colbias and
+// rstat. mode_bits: 1 = produce op16 (bf16 sweep), 2 = bf16x3 stats.
+__global__ __launch_bounds__(256) void prep_kernel(
+    const DsDesc* __restrict__ ds, int nds, int64_t total_pad_rows,
+    const int64_t* __restrict__ pad_off /* nds+1 prefix of n_pad */,
+    const float* __restrict__ nz_g, const double* __restrict__ rn_g, int mode_bits) {
+  const int lane = threadIdx.x & 63;
+  int64_t pr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (pr >= total_pad_rows) return;
+  // dataset by padded-row prefix
+  int lo = 0, hi = nds - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (pad_off[mid] <= pr) lo = mid; else hi = mid - 1;
+  }
+  const DsDesc& D = ds[lo];
+  const int64_t i = pr - pad_off[lo];
+  const int kp = D.kp, d = D.d;
+  float* xr = const_cast<float*>(D.xr) + i * kp;
+  float* op32 = const_cast<float*>(D.op32) + i * kp;
+  uint16_t* op16 = D.op16 ? const_cast<uint16_t*>(D.op16) + i * kp : nullptr;
+  const bool cosine = D.metric == 1;
+  const bool real = i < D.n;
+  double rn = 0.0;
+  float nz = 0.0f;
+  if (real) {
+    rn = rn_g[D.row_off + i];
+    nz = nz_g[D.row_off + i];
+  }
+  const bool zero_row = real && rn == 0.0;
+  // statistics accumulated per lane, reduced by a fixed butterfly
+  double s_xn = 0, s_en = 0, s_xo = 0, s_hn = 0, s_ln = 0, s_hln = 0, s_rn = 0;
+  for (int f = lane; f < kp; f += 64) {
+    float v = (real && f < d) ? D.x[i * d + f] : 0.0f;
+    xr[f] = v;
+    float w = v;
+    double wexact = (double)v;
+    if (cosine) {
+      if (real && !zero_row) {
+        wexact = (double)v / rn;
+        w = (float)wexact;
+      } else {
+        wexact = 0.0;
+        w = 0.0f;
+      }
+      op32[f] = w;
+    }
+    s_xo += (double)w * (double)w;
+    uint16_t h = f32_to_bf16_rne(w);
+    float hf = bf16_to_f32(h);
+    if (op16) op16[f] = h;
+    if (mode_bits & 1) {
+      double e = wexact - (double)hf;
+      s_xn += (double)hf * (double)hf;
+      s_en += e * e;
+    } else {
+      double e = wexact - (double)w;
+      s_xn += (double)w * (double)w;
+      s_en += e * e;
+    }
+    if (mode_bits & 2) {
+      float lof = w - hf;  // exact in f32
+      float lf = bf16_to_f32(f32_to_bf16_rne(lof));
+      double r = wexact - (double)hf - (double)lf;
+      s_hn += (double)hf * hf;
+      s_ln += (double)lf * lf;
+      double hl = (double)hf + (double)lf;
+      s_hln += hl * hl;
+      s_rn += r * r;
+    }
+  }
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    s_xn += __shfl_xor(s_xn, m, 64);
+    s_en += __shfl_xor(s_en, m, 64);
+    s_xo += __shfl_xor(s_xo, m, 64);
+    s_hn += __shfl_xor(s_hn, m, 64);
+    s_ln += __shfl_xor(s_ln, m, 64);
+    s_hln += __shfl_xor(s_hln, m, 64);
+    s_rn += __shfl_xor(s_rn, m, 64);
+  }
+  if (lane == 0) {
+    float* cb = const_cast<float*>(D.colbias);
+    if (!real) {
+      cb[i] = 0.0f;
+    } else if (cosine) {
+      cb[i] = zero_row ? -INFINITY : 0.0f;
+    } else {
+      cb[i] = nz;
+    }
+    if (real) {
+      // norms rounded up by a relative 2^-20 so the stored f32 bounds the
+      // f64 value; the f64 partial sums themselves are within d*2^-53.
+      const double up = 1.0 + 0x1p-20;
+      float* st = const_cast<float*>(D.rstat) + i * HM_NSTAT;
+      st[ST_NZ] = cosine ? 0.0f : nz;
+      st[ST_XN] = (float)(sqrt(s_xn) * up);
+      st[ST_EN] = (float)(sqrt(s_en) * up);
+      st[ST_XO] = (float)(sqrt(s_xo) * up);
+      st[ST_HN] = (float)(sqrt(s_hn) * up);
+      st[ST_LN] = (float)(sqrt(s_ln) * up);
+      st[ST_HLN] = (float)(sqrt(s_hln) * up);
+      st[ST_RN] = (float)(sqrt(s_rn) * up);
+    }
+  }
+}
+
+// Per-dataset maxima of rstat. One block per dataset, fixed reduction tree.
+__global__ __launch_bounds__(256) void statmax_kernel(const DsDesc* __restrict__ ds) {
+  const DsDesc& D = ds[blockIdx.x];
+  __shared__ float red[256][HM_NSTAT];
+  float m[HM_NSTAT];
+#pragma unroll
+  for (int s = 0; s < HM_NSTAT; ++s) m[s] = 0.0f;
+  for (int64_t i = threadIdx.x; i < D.n; i += 256) {
+    const float* st = D.rstat + i * HM_NSTAT;
+#pragma unroll
+    for (int s = 0; s < HM_NSTAT; ++s) m[s] = fmaxf(m[s], st[s]);
+  }
+#pragma unroll
+  for (int s = 0; s < HM_NSTAT; ++s) red[threadIdx.x][s] = m[s];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int s = 0; s < HM_NSTAT; ++s)
+        red[threadIdx.x][s] = fmaxf(red[threadIdx.x][s], red[threadIdx.x + w][s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < HM_NSTAT) const_cast<float*>(D.smax)[threadIdx.x] = red[0][threadIdx.x];
+}
+
+}  // namespace hm

@@ -1,0 +1,55 @@
+/* CPU oracle for the holomap kNN-graph path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker that the GPU path is
+ * compared against; it is loaded by tests/, by __graft_entry__.smoke() and by
+ * bench.py's cpu_baseline leg, and by nothing in holomap_amd/.
+ *
+ * Every function restates the reference's arithmetic in the reference's
+ * order (Rust float semantics: no contraction, sequential folds, IEEE
+ * round-to-nearest-even), so its outputs are the reference's outputs:
+ *   hmo_euclidean        src/metric.rs:42-53
+ *   hmo_cosine           src/metric.rs:58-70
+ *   hmo_exact_knn        src/knn.rs:22-55
+ *   hmo_expf             src/fmath.rs:119-122 -> libm 0.2.16 expf
+ *                        (FreeBSD msun e_expf.c method, restated)
+ *   hmo_smooth_knn       src/smooth_knn.rs:12-84
+ *   hmo_fuzzy            src/fuzzy.rs:25-85
+ *   hmo_csr_indptr       src/sparse.rs:21-35
+ * hmo_smooth_knn_f64 is an FP64 bisection run to convergence: the FP64
+ * oracle for the GPU's FP64 calibration mode (no reference counterpart).
+ */
+#ifndef HM_ORACLE_H
+#define HM_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+float hmo_euclidean(const float* a, const float* b, size_t d);
+float hmo_cosine(const float* a, const float* b, size_t d);
+/* metric: 0 = euclidean, 1 = cosine. Full (dist, idx) sort per row, as knn.rs. */
+void hmo_exact_knn(const float* data, size_t n, size_t d, size_t k, int metric,
+                   uint32_t* indices, float* dists);
+/* Same semantics for a subset of query rows (row ids in `rows`), output
+ * row-major nrows x k. nthreads > 1 splits query rows over OpenMP threads
+ * (rows are independent, so results are bit-identical to nthreads = 1). */
+void hmo_exact_knn_rows(const float* data, size_t n, size_t d, size_t k, int metric,
+                        const uint32_t* rows, size_t nrows, uint32_t* indices,
+                        float* dists, int nthreads);
+float hmo_expf(float x);
+void hmo_smooth_knn(const float* knn_dists, size_t n, size_t k, float* sigmas,
+                    float* rhos);
+void hmo_smooth_knn_f64(const float* knn_dists, size_t n, size_t k, double* sigmas,
+                        double* rhos);
+/* rows/cols/vals capacity must be >= 2*n*k. Returns nnz. */
+size_t hmo_fuzzy(const uint32_t* indices, const float* dists, const float* sigmas,
+                 const float* rhos, size_t n, size_t k, uint32_t* rows,
+                 uint32_t* cols, float* vals);
+void hmo_csr_indptr(const uint32_t* rows, size_t nnz, size_t n, uint64_t* indptr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
