@@ -1,0 +1,262 @@
+// Smooth-kNN calibration (src/smooth_knn.rs:12-84) on the device.
+//
+// Thread per row. Mode CALIB_REF_F32 repeats the reference's float32
+// bisection step for step (state in f32, psum folded left to right over
+// row[1..], the f64 comparisons against log2(k), f32 mid-point updates,
+// hi seeded at f32::MAX with doubling), with expf evaluated by the same
+// algorithm the reference's libm crate uses (see hm_expf). The result is
+// bit-identical to the reference arithmetic. Mode CALIB_F64 runs the same
+// bracket search in f64 with f64 exp until the bracket stops moving.
+//
+// The floor for rows without a non-zero distance uses mean_all, a
+// sequential f64 mean over all n*k distances (src/smooth_knn.rs:26). That
+// fold is inherently serial, so it is evaluated lazily by one lane in
+// calib_fixup_kernel, and only when such a row exists.
+#include "hm_internal.hpp"
+
+namespace hm {
+
+// f32 exp by the method of libm 0.2.16 expf (src/fmath.rs:119-122): argument
+// reduction by k*ln2 (ln2 split hi/lo), the degree-2 rational Remez form
+// exp(r) = 1 + (r*c/(2-c) - lo + hi), c = r - r^2 (P1 + r^2 P2), then an exact
+// power-of-two scaling. Every operation is a single f32 rounding (this file
+// is compiled without contraction).
+__device__ __forceinline__ float hm_scalbnf(float x, int n) {
+  float y = x;
+  if (n > 127) {
+    y *= 0x1p127f;
+    n -= 127;
+    if (n > 127) {
+      y *= 0x1p127f;
+      n -= 127;
+      if (n > 127) n = 127;
+    }
+  } else if (n < -126) {
+    y *= 0x1p-126f * 0x1p24f;
+    n += 126 - 24;
+    if (n < -126) {
+      y *= 0x1p-126f * 0x1p24f;
+      n += 126 - 24;
+      if (n < -126) n = -126;
+    }
+  }
+  return y * __uint_as_float((uint32_t)(0x7f + n) << 23);
+}
+
+__device__ __forceinline__ float hm_expf(float x) {
+  const float ln2hi = 6.9314575195e-01f, ln2lo = 1.4286067653e-06f;
+  const float invln2 = 1.4426950216e+00f;
+  const float P1 = 1.6666625440e-1f, P2 = -2.7667332906e-3f;
+  uint32_t hx = __float_as_uint(x);
+  const int sign = (int)(hx >> 31);
+  hx &= 0x7fffffffu;
+  if (hx >= 0x42aeac50u) {
+    if (hx > 0x7f800000u) return x;
+    if (hx >= 0x42b17218u && !sign) return x * 0x1p127f;
+    if (sign && hx >= 0x42cff1b5u) return 0.0f;
+  }
+  int k;
+  float hi, lo;
+  if (hx > 0x3eb17218u) {
+    if (hx > 0x3f851592u)
+      k = (int)(invln2 * x + (sign ? -0.5f : 0.5f));
+    else
+      k = 1 - sign - sign;
+    const float kf = (float)k;
+    hi = x - kf * ln2hi;
+    lo = kf * ln2lo;
+    x = hi - lo;
+  } else if (hx > 0x39000000u) {
+    k = 0;
+    hi = x;
+    lo = 0.0f;
+  } else {
+    return 1.0f + x;
+  }
+  const float xx = x * x;
+  const float c = x - xx * (P1 + xx * P2);
+  const float y = 1.0f + (x * c / (2.0f - c) - lo + hi);
+  return k == 0 ? y : hm_scalbnf(y, k);
+}
+
+__device__ __forceinline__ int calib_find_ds(const DsDesc* ds, int nds, int64_t grow) {
+  int lo = 0, hi = nds - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (ds[mid].row_off <= grow) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// target_log2k[ds] is log2(k) computed on the host (glibc), as the
+// reference's std f64::log2 does.
+__global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
+                                  const double* __restrict__ target_log2k, int mode,
+                                  float* __restrict__ sigma, float* __restrict__ rho,
+                                  double* __restrict__ sigma64, uint32_t* __restrict__ zrow) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total_rows) return;
+  const int di = calib_find_ds(ds, nds, g);
+  const DsDesc& D = ds[di];
+  const int64_t i = g - D.row_off;
+  const int k = D.k;
+  const float* row = D.knn_dist + i * k;
+  const double target = target_log2k[di];
+
+  float r = 0.0f;
+  for (int t = 0; t < k; ++t) {
+    if (row[t] > 0.0f) { r = row[t]; break; }
+  }
+  rho[g] = r;
+  if (mode == CALIB_REF_F32) {
+    const float fmax = 3.40282347e+38f;
+    float lo = 0.0f, hi = fmax, mid = 1.0f;
+    for (int it = 0; it < 64; ++it) {
+      float psum = 0.0f;
+      for (int t = 1; t < k; ++t) {
+        const float dd = row[t] - r;
+        if (dd > 0.0f)
+          psum += hm_expf(-(dd / mid));
+        else
+          psum += 1.0f;
+      }
+      if (fabs((double)psum - target) < 1e-5) break;
+      if ((double)psum > target) {
+        hi = mid;
+        mid = (lo + hi) / 2.0f;
+      } else {
+        lo = mid;
+        if (hi >= fmax)
+          mid *= 2.0f;
+        else
+          mid = (lo + hi) / 2.0f;
+      }
+    }
+    float s = mid;
+    if (r > 0.0f) {
+      double acc = 0.0;
+      for (int t = 0; t < k; ++t) acc = acc + (double)row[t];
+      const float fl = 1e-3f * (float)(acc / (double)k);
+      if (s < fl) s = fl;
+      zrow[g] = 0;
+    } else {
+      zrow[g] = 1;  // floor needs mean_all: applied by calib_fixup_kernel
+    }
+    sigma[g] = s;
+    if (sigma64) sigma64[g] = (double)s;
+  } else {
+    const double rr = (double)r;
+    double lo = 0.0, hi = INFINITY, mid = 1.0;
+    for (int it = 0; it < 2200; ++it) {
+      double psum = 0.0;
+      for (int t = 1; t < k; ++t) {
+        const double dd = (double)row[t] - rr;
+        psum += dd > 0.0 ? exp(-(dd / mid)) : 1.0;
+      }
+      if (psum == target) break;
+      double nmid;
+      if (psum > target) {
+        hi = mid;
+        nmid = 0.5 * (lo + hi);
+      } else {
+        lo = mid;
+        nmid = isinf(hi) ? mid * 2.0 : 0.5 * (lo + hi);
+      }
+      if (nmid == mid || isinf(nmid)) break;
+      mid = nmid;
+    }
+    double s = mid;
+    if (r > 0.0f) {
+      double acc = 0.0;
+      for (int t = 0; t < k; ++t) acc += (double)row[t];
+      const double fl = 1e-3 * (acc / (double)k);
+      if (s < fl) s = fl;
+      zrow[g] = 0;
+    } else {
+      zrow[g] = 1;
+    }
+    if (sigma64) sigma64[g] = s;
+    sigma[g] = (float)s;
+  }
+}
+
+// Applies the global-mean floor to rows with rho == 0. One workgroup per
+// dataset; the mean is the reference's left-to-right f64 fold, computed by
+// one lane and only if the dataset has such a row.
+__global__ __launch_bounds__(256) void calib_fixup_kernel(const DsDesc* __restrict__ ds, int mode,
+                                                          float* __restrict__ sigma,
+                                                          double* __restrict__ sigma64,
+                                                          const uint32_t* __restrict__ zrow) {
+  const DsDesc& D = ds[blockIdx.x];
+  __shared__ int any;
+  __shared__ double mean_s;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int64_t i = threadIdx.x; i < D.n; i += 256) mine |= (int)zrow[D.row_off + i];
+  if (__syncthreads_or(mine)) {
+    if (threadIdx.x == 0) {
+      const int64_t m = D.n * D.k;
+      double s = 0.0;
+      for (int64_t t = 0; t < m; ++t) s = s + (double)D.knn_dist[t];
+      mean_s = s / (double)m;
+    }
+    __syncthreads();
+    const double mean_all = mean_s;
+    for (int64_t i = threadIdx.x; i < D.n; i += 256) {
+      const int64_t g = D.row_off + i;
+      if (!zrow[g]) continue;
+      if (mode == CALIB_REF_F32) {
+        const float fl = 1e-3f * (float)mean_all;
+        float s = sigma[g];
+        if (s < fl) s = fl;
+        sigma[g] = s;
+        if (sigma64) sigma64[g] = (double)s;
+      } else {
+        double s = sigma64 ? sigma64[g] : (double)sigma[g];
+        const double fl = 1e-3 * mean_all;
+        if (s < fl) s = fl;
+        if (sigma64) sigma64[g] = s;
+        sigma[g] = (float)s;
+      }
+    }
+  }
+  (void)any;
+}
+
+// Directed membership strengths (src/fuzzy.rs:31-48), one thread per edge.
+// key1[e] = global column id, or `sentinel` for dropped edges (self, w == 0).
+__global__ void directed_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_edges,
+                                const float* __restrict__ sigma, const float* __restrict__ rho,
+                                uint32_t sentinel, uint32_t* __restrict__ key1,
+                                uint32_t* __restrict__ pay1, float* __restrict__ wval,
+                                uint32_t* __restrict__ srcrow) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total_edges) return;
+  int lo = 0, hi = nds - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (ds[mid].edge_off <= e) lo = mid; else hi = mid - 1;
+  }
+  const DsDesc& D = ds[lo];
+  const int64_t le = e - D.edge_off;
+  const int64_t i = le / D.k;
+  const int64_t g = D.row_off + i;
+  const uint32_t col = D.knn_idx[le];
+  const float dd = D.knn_dist[le] - rho[g];
+  const float s = sigma[g];
+  const float w = (dd <= 0.0f || s == 0.0f) ? 1.0f : hm_expf(-(dd / s));
+  const bool keep = (int64_t)col != i && w != 0.0f;
+  key1[e] = keep ? (uint32_t)(D.row_off + col) : sentinel;
+  pay1[e] = (uint32_t)e;
+  wval[e] = w;
+  srcrow[e] = (uint32_t)g;
+}
+
+// Standalone expf over an array (parity tests of the device expf).
+__global__ void expf_kernel(const float* __restrict__ x, float* __restrict__ y, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = hm_expf(x[t]);
+}
+
+}  // namespace hm

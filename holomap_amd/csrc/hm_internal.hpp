@@ -24,8 +24,8 @@ struct DsDesc {
   const double* rnorm;   // n: cosine sqrt(sum x^2) (f64, reference fold)
   const float* rstat;    // n x HM_NSTAT per-row bound statistics
   const float* smax;     // HM_NSTAT maxima over rows (per dataset)
-  uint64_t* cand;        // candidate keys: n x nsplit x C
-  uint64_t* ctau;        // thresholds: n x nsplit
+  uint64_t* cand;        // candidate keys: n x nlist x cout
+  uint64_t* ctau;        // list thresholds: n x nlist
   uint32_t* knn_idx;     // n x k (output)
   float* knn_dist;       // n x k (output)
   int64_t n;             // rows
@@ -36,10 +36,11 @@ struct DsDesc {
   int32_t kp;            // padded feature stride (elements)
   int32_t k;             // neighbours
   int32_t metric;        // 0 euclidean, 1 cosine
-  int32_t C;             // candidates kept per (row, split)
+  int32_t C;             // candidates targeted per (row, split)
   int32_t nsplit;        // column splits of the candidate sweep
-  int32_t x_aligned;     // unused padding
-  int32_t pad_;
+  int32_t nlist;         // candidate lists per row = 2 * nsplit (one per
+                         //   half of the 64-lane wave, see hm_gram.hip)
+  int32_t cout;          // key slots per list
 };
 
 // Number of float statistics kept per row for the candidate error bound.

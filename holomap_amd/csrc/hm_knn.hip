@@ -1,0 +1,468 @@
+// Exact kNN in the reference's arithmetic.
+//
+//  * exact_scan_kernel  -- brute force over a column split for a block of up
+//    to 32 query rows: every pair distance is the reference's sequential f64
+//    fold (src/metric.rs:42-53 / 58-70), self forced to 0 (src/knn.rs:38-40),
+//    and each row keeps its k smallest (f32 dist, idx) keys -- the order of
+//    src/knn.rs:48, since dist >= +0 makes the f32 bit pattern monotone.
+//    Serves the exact precision mode and the fallback for uncertified rows.
+//  * exact_merge_kernel -- merges the per-split top-k lists of a row.
+//  * rerank_kernel      -- after the MFMA candidate sweep: recomputes the
+//    plausible candidates exactly, takes the top-k and certifies that no
+//    column outside the candidate set can enter it.
+#include "hm_internal.hpp"
+
+namespace hm {
+
+__device__ __forceinline__ double f64max0(double v) {
+  // f64::max(v, 0.0): a NaN operand yields the other one
+  return (v != v) ? 0.0 : (v > 0.0 ? v : 0.0);
+}
+
+// Reference distance of query row a (uniform) to row b, over the zero-padded
+// copies (padding adds exact zeros: (0-0)^2 and 0*0 change no partial sum).
+__device__ __forceinline__ float ref_dist(const float* __restrict__ a, const float* __restrict__ b,
+                                          int d4, int metric, double rna, double rnb) {
+  double s = 0.0;
+  if (metric == 0) {
+    for (int f = 0; f < d4; f += 4) {
+      const float4 x = *(const float4*)(a + f);
+      const float4 y = *(const float4*)(b + f);
+      double t;
+      t = (double)x.x - (double)y.x; s = s + t * t;
+      t = (double)x.y - (double)y.y; s = s + t * t;
+      t = (double)x.z - (double)y.z; s = s + t * t;
+      t = (double)x.w - (double)y.w; s = s + t * t;
+    }
+    return (float)sqrt(s);
+  } else {
+    for (int f = 0; f < d4; f += 4) {
+      const float4 x = *(const float4*)(a + f);
+      const float4 y = *(const float4*)(b + f);
+      s = s + (double)x.x * (double)y.x;
+      s = s + (double)x.y * (double)y.y;
+      s = s + (double)x.z * (double)y.z;
+      s = s + (double)x.w * (double)y.w;
+    }
+    return (float)f64max0(1.0 - s / (rna * rnb));
+  }
+}
+
+__device__ __forceinline__ uint64_t dist_key(float dist, uint32_t j) {
+  return ((uint64_t)__float_as_uint(dist) << 32) | j;
+}
+
+// Ascending bitonic sort of 128 keys, two per lane (indices lane, lane+64).
+__device__ __forceinline__ void sort128(uint64_t& v0, uint64_t& v1) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 2; k <= 128; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j == 64) {
+        uint64_t mn = v0 < v1 ? v0 : v1, mx = v0 < v1 ? v1 : v0;
+        v0 = mn;
+        v1 = mx;
+      } else {
+        uint64_t o0 = shfl_xor_u64(v0, j), o1 = shfl_xor_u64(v1, j);
+        bool up0 = (lane & k) == 0, up1 = ((lane + 64) & k) == 0;
+        bool lw = (lane & j) == 0;
+        uint64_t mn0 = v0 < o0 ? v0 : o0, mx0 = v0 < o0 ? o0 : v0;
+        uint64_t mn1 = v1 < o1 ? v1 : o1, mx1 = v1 < o1 ? o1 : v1;
+        v0 = (lw == up0) ? mn0 : mx0;
+        v1 = (lw == up1) ? mn1 : mx1;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exact_scan_kernel
+// Work item: (ds, split, qbeg, qcnt<=32). Queries are qlist[qbeg + r] (global
+// row ids) or, without a list, local rows qbeg + r. Output: for query slot p
+// (= list position, or local row in identity mode) the split's k smallest
+// keys, sorted, at xpart[(p*xs + split)*kmax ...].
+// ---------------------------------------------------------------------------
+#define XS_QB 32   // queries per work item
+#define XS_CB 64   // candidate columns per step
+#define XS_KF 64   // features per LDS chunk
+#define XS_QCAP 128
+
+__global__ __launch_bounds__(256) void exact_scan_kernel(
+    const DsDesc* __restrict__ ds_arr, const WorkItem* __restrict__ items,
+    const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist,
+    int xs, int kmax, uint64_t* __restrict__ xpart) {
+  __shared__ double qf[XS_QB][XS_KF];
+  __shared__ float cf[XS_CB][XS_KF + 1];
+  __shared__ uint64_t buf[XS_QB][XS_QCAP];
+  __shared__ int64_t qrow_s[XS_QB];
+  __shared__ double qrn_s[XS_QB];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nitems = nitems_dev ? *nitems_dev : nitems_host;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem W = items[it];
+    const DsDesc& D = ds_arr[W.ds];
+    const int64_t n = D.n;
+    const int kp = D.kp, d = D.d, k = D.k, metric = D.metric;
+    const int64_t span = ((n + (int64_t)xs * XS_CB - 1) / ((int64_t)xs * XS_CB)) * XS_CB;
+    const int64_t cbeg = (int64_t)W.split * span;
+    const int64_t cend = cbeg + span < n ? cbeg + span : n;
+    __syncthreads();  // previous item fully done with the shared arrays
+    if (tid < XS_QB) {
+      int64_t row = -1;
+      if (tid < W.qcnt) {
+        const int p = W.qbeg + tid;
+        row = qlist ? (int64_t)qlist[p] - D.row_off : (int64_t)p;
+      }
+      qrow_s[tid] = row;
+      qrn_s[tid] = (row >= 0 && metric == 1) ? D.rnorm[row] : 0.0;
+    }
+    __syncthreads();
+    uint64_t tau[8];
+    int cnt[8];
+    int64_t myq[8];
+    double myrn[8];
+#pragma unroll
+    for (int qq = 0; qq < 8; ++qq) {
+      tau[qq] = ~0ull;
+      cnt[qq] = 0;
+      myq[qq] = qrow_s[wave * 8 + qq];
+      myrn[qq] = qrn_s[wave * 8 + qq];
+    }
+    for (int64_t cb = cbeg; cb < cend; cb += XS_CB) {
+      double acc[8];
+#pragma unroll
+      for (int qq = 0; qq < 8; ++qq) acc[qq] = 0.0;
+      for (int f0 = 0; f0 < d; f0 += XS_KF) {
+        __syncthreads();
+        {  // candidates: 64 rows x 64 features, 16 per thread
+          const int r = tid >> 2, c = (tid & 3) * 16;
+          const int64_t j = cb + r;
+#pragma unroll
+          for (int e = 0; e < 16; e += 4) {
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (j < n && f0 + c + e < kp) v = *(const float4*)(D.xr + (size_t)j * kp + f0 + c + e);
+            cf[r][c + e] = v.x;
+            cf[r][c + e + 1] = v.y;
+            cf[r][c + e + 2] = v.z;
+            cf[r][c + e + 3] = v.w;
+          }
+        }
+        {  // queries: 32 rows x 64 features as f64, 8 per thread
+          const int r = tid >> 3, c = (tid & 7) * 8;
+          const int64_t row = qrow_s[r];
+#pragma unroll
+          for (int e = 0; e < 8; e += 4) {
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (row >= 0 && f0 + c + e < kp) v = *(const float4*)(D.xr + (size_t)row * kp + f0 + c + e);
+            qf[r][c + e] = (double)v.x;
+            qf[r][c + e + 1] = (double)v.y;
+            qf[r][c + e + 2] = (double)v.z;
+            qf[r][c + e + 3] = (double)v.w;
+          }
+        }
+        __syncthreads();
+        const int nf = d - f0 < XS_KF ? d - f0 : XS_KF;
+        if (metric == 0) {
+          for (int f = 0; f < nf; ++f) {
+            const double b = (double)cf[lane][f];
+#pragma unroll
+            for (int qq = 0; qq < 8; ++qq) {
+              const double t = qf[wave * 8 + qq][f] - b;
+              acc[qq] = acc[qq] + t * t;
+            }
+          }
+        } else {
+          for (int f = 0; f < nf; ++f) {
+            const double b = (double)cf[lane][f];
+#pragma unroll
+            for (int qq = 0; qq < 8; ++qq) acc[qq] = acc[qq] + qf[wave * 8 + qq][f] * b;
+          }
+        }
+      }
+      const int64_t j = cb + lane;
+      const double rnc = (metric == 1 && j < n) ? D.rnorm[j] : 0.0;
+#pragma unroll
+      for (int qq = 0; qq < 8; ++qq) {
+        const int slot = wave * 8 + qq;
+        if (myq[qq] < 0) continue;  // uniform
+        float dist;
+        if (metric == 0) dist = (float)sqrt(acc[qq]);
+        else dist = (float)f64max0(1.0 - acc[qq] / (myrn[qq] * rnc));
+        if (j == myq[qq]) dist = 0.0f;
+        const uint64_t key = dist_key(dist, (uint32_t)j);
+        bool hit = (j < cend) && key < tau[qq];
+        uint64_t mask = __ballot(hit);
+        if (mask == 0ull) continue;
+        if (cnt[qq] + __popcll(mask) > XS_QCAP) {
+          uint64_t v0 = lane < cnt[qq] ? buf[slot][lane] : ~0ull;
+          uint64_t v1 = lane + 64 < cnt[qq] ? buf[slot][lane + 64] : ~0ull;
+          sort128(v0, v1);
+          if (lane < k) buf[slot][lane] = v0;
+          if (lane + 64 < k) buf[slot][lane + 64] = v1;
+          tau[qq] = (k - 1) < 64 ? readlane_u64(v0, k - 1) : readlane_u64(v1, k - 65);
+          cnt[qq] = k;
+          hit = (j < cend) && key < tau[qq];
+          mask = __ballot(hit);
+        }
+        const int pos = cnt[qq] + (int)__popcll(mask & ((1ull << lane) - 1ull));
+        if (hit) buf[slot][pos] = key;
+        cnt[qq] += (int)__popcll(mask);
+      }
+    }
+    // final: sort each row's buffer, emit its k smallest
+#pragma unroll
+    for (int qq = 0; qq < 8; ++qq) {
+      const int slot = wave * 8 + qq;
+      if (myq[qq] < 0) continue;
+      uint64_t v0 = lane < cnt[qq] ? buf[slot][lane] : ~0ull;
+      uint64_t v1 = lane + 64 < cnt[qq] ? buf[slot][lane + 64] : ~0ull;
+      sort128(v0, v1);
+      const int p = W.qbeg + slot;
+      uint64_t* dst = xpart + ((size_t)p * xs + W.split) * kmax;
+      if (lane < k) dst[lane] = v0;
+      if (lane + 64 < k) dst[lane + 64] = v1;
+    }
+  }
+}
+
+// Merge the xs sorted per-split lists of every listed query; write knn rows.
+// One wave per list position p < *count (or per local row in identity mode).
+__global__ __launch_bounds__(256) void exact_merge_kernel(
+    const DsDesc* __restrict__ ds_arr, int nds, const uint32_t* __restrict__ qlist,
+    const int* __restrict__ count_dev, int64_t count_host, int xs, int kmax,
+    const uint64_t* __restrict__ xpart) {
+  __shared__ uint64_t s[4][1024];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t count = count_dev ? (int64_t)*count_dev : count_host;
+  for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < count; p += (int64_t)gridDim.x * 4) {
+    int64_t grow = qlist ? (int64_t)qlist[p] : p;
+    int lo = 0, hi = nds - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (ds_arr[mid].row_off <= grow) lo = mid; else hi = mid - 1;
+    }
+    const DsDesc& D = ds_arr[lo];
+    const int64_t row = grow - D.row_off;
+    const int k = D.k;
+    int len = 64;
+    while (len < xs * k) len <<= 1;
+    uint64_t* a = s[wave];
+    for (int e = lane; e < len; e += 64) {
+      const int sp = e / k, t = e % k;
+      a[e] = (sp < xs) ? xpart[((size_t)p * xs + sp) * kmax + t] : ~0ull;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    wave_sort_lds(a, len);
+    for (int t = lane; t < k; t += 64) {
+      const uint64_t key = a[t];
+      D.knn_idx[row * k + t] = (uint32_t)key;
+      D.knn_dist[row * k + t] = __uint_as_float((uint32_t)(key >> 32));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// rerank_kernel: one wave per global row (all rows of all datasets).
+//   mode: 0 bf16, 1 fp32, 2 bf16x3 -- selects the error bound of the keys.
+// Writes knn rows for every row, and flag[grow] = 1 where the candidate set
+// could not be certified to contain the reference's top-k (those rows are
+// recomputed by the exact scan).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float next_up(float x) {
+  // next float toward +inf for finite x >= 0
+  return __uint_as_float(__float_as_uint(x) + 1u);
+}
+__device__ __forceinline__ float f32_round_up(double v) {
+  // smallest float >= v, for v >= 0
+  float f = (float)v;
+  if ((double)f < v) f = next_up(f);
+  return f;
+}
+
+__global__ __launch_bounds__(256) void rerank_kernel(const DsDesc* __restrict__ ds_arr, int nds,
+                                                     int64_t total_rows, int mode,
+                                                     uint32_t* __restrict__ flag) {
+  __shared__ uint64_t cs[4][512];
+  __shared__ uint64_t es[4][512];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t grow = (int64_t)blockIdx.x * 4 + wave;
+  if (grow >= total_rows) return;
+  int lo = 0, hi = nds - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (ds_arr[mid].row_off <= grow) lo = mid; else hi = mid - 1;
+  }
+  const DsDesc& D = ds_arr[lo];
+  const int64_t i = grow - D.row_off;
+  const int k = D.k, L = D.nlist, CO = D.cout, d = D.d, kp = D.kp, metric = D.metric;
+  const int d4 = (d + 3) & ~3;
+  uint32_t* oidx = D.knn_idx + i * k;
+  float* odst = D.knn_dist + i * k;
+  const double rna = metric == 1 ? D.rnorm[i] : 0.0;
+
+  if (metric == 1 && rna == 0.0) {
+    // zero query: every cosine distance is max(NaN, 0) = 0, so the list is
+    // the k lowest indices (self included) at distance 0
+    for (int t = lane; t < k; t += 64) { oidx[t] = (uint32_t)t; odst[t] = 0.0f; }
+    if (lane == 0) flag[grow] = 0;
+    return;
+  }
+
+  // ---- gather and sort the candidate keys of all lists ----
+  const int m = L * CO;
+  int len = 64;
+  while (len < m) len <<= 1;
+  uint64_t* a = cs[wave];
+  const uint64_t* src = D.cand + (size_t)i * L * CO;
+  for (int e = lane; e < len; e += 64) a[e] = e < m ? src[e] : ~0ull;
+  uint64_t taumin = ~0ull;
+  for (int s = 0; s < L; ++s) {
+    const uint64_t t = D.ctau[(size_t)i * L + s];
+    taumin = t < taumin ? t : taumin;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  wave_sort_lds(a, len);
+  int nvalid = 0;
+  for (int e = lane; e < len; e += 64) nvalid += (a[e] != ~0ull) ? 1 : 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nvalid += __shfl_xor(nvalid, o, 64);
+  if (nvalid < k) {
+    // cannot happen for n >= k (the lists keep min(C, columns) keys);
+    // leave the row to the exact scan rather than decode an empty slot
+    if (lane == 0) flag[grow] = 1;
+    return;
+  }
+
+  // ---- error bound of the candidate keys for this row ----
+  const float* st = D.rstat + i * HM_NSTAT;
+  const float* mx = D.smax;
+  const double u = 0x1p-24;
+  double E;
+  double dotmax;
+  if (mode == 2) {
+    const double gam = 2.0 * 3.0 * kp * u;
+    const double dot_err = (double)st[ST_LN] * mx[ST_LN] + (double)st[ST_HLN] * mx[ST_RN] +
+                           (double)st[ST_RN] * mx[ST_XO];
+    const double acc_err = gam * ((double)st[ST_HN] * mx[ST_HN] + (double)st[ST_HN] * mx[ST_LN] +
+                                  (double)st[ST_LN] * mx[ST_HN]);
+    E = dot_err + acc_err;
+    dotmax = (double)st[ST_XO] * mx[ST_XO] * 1.01;
+  } else {
+    const double gam = 2.0 * kp * u;
+    const double dot_err = (double)st[ST_EN] * ((double)mx[ST_XN] + mx[ST_EN]) +
+                           (double)st[ST_XN] * mx[ST_EN];
+    const double acc_err = gam * (double)st[ST_XN] * mx[ST_XN];
+    E = dot_err + acc_err;
+    dotmax = (double)st[ST_XN] * mx[ST_XN] * (1.0 + gam);
+  }
+  const double eps_exact = (d + 8) * 0x1p-52;  // f64 fold of the exact distance
+  double nzi = 0.0;
+  if (metric == 0) {
+    nzi = (double)st[ST_NZ];
+    E = 2.0 * E + (double)st[ST_NZ] * 0x1p-23 + (double)mx[ST_NZ] * 0x1p-23 +
+        u * ((double)mx[ST_NZ] + 2.0 * dotmax);
+    E = E * (1.0 + 1e-3) + 1e-30;
+  } else {
+    E = E * (1.0 + 1e-3) + 1e-12;
+  }
+
+  // ---- prune: candidates that cannot beat the k key-best ones ----
+  const double kk = (double)unordered_f32((uint32_t)(a[k - 1] >> 32));
+  double thr;  // a candidate with lower bound >= thr is strictly beyond the top-k
+  if (metric == 0) {
+    double ub = nzi + kk + E;
+    ub = ub > 0.0 ? ub : 0.0;
+    const float dup = next_up(f32_round_up(sqrt(ub * (1.0 + eps_exact))));
+    thr = (double)dup * (double)dup * (1.0 + eps_exact) + 1e-300;
+  } else {
+    double ub = 1.0 + kk + E + eps_exact;
+    ub = ub > 0.0 ? ub : 0.0;
+    thr = (double)next_up(f32_round_up(ub));
+  }
+  int np = 0;
+  for (int e = lane; e < nvalid; e += 64) {
+    const double key = (double)unordered_f32((uint32_t)(a[e] >> 32));
+    double lb = metric == 0 ? (nzi + key - E) * (1.0 - eps_exact) : 1.0 + key - E - eps_exact;
+    np += (lb < thr) ? 1 : 0;  // monotone in e: a prefix
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) np += __shfl_xor(np, o, 64);
+  if (np < k) np = k;
+
+  // ---- exact distances of the plausible prefix ----
+  int elen = 64;
+  while (elen < np) elen <<= 1;
+  uint64_t* ex = es[wave];
+  const float* qa = D.xr + (size_t)i * kp;
+  bool badid = false;
+  for (int e = lane; e < elen; e += 64) {
+    uint64_t key = ~0ull;
+    if (e < np) {
+      const uint32_t j = (uint32_t)a[e];
+      if ((int64_t)j >= D.n) {
+        badid = true;  // never produced by the sweep; guard the row read
+      } else {
+        float dist;
+        if ((int64_t)j == i) {
+          dist = 0.0f;
+        } else {
+          const double rnb = metric == 1 ? D.rnorm[j] : 0.0;
+          dist = ref_dist(qa, D.xr + (size_t)j * kp, d4, metric, rna, rnb);
+        }
+        key = dist_key(dist, j);
+      }
+    }
+    ex[e] = key;
+  }
+  if (__ballot(badid) != 0ull) {
+    if (lane == 0) flag[grow] = 1;  // recomputed by the exact scan
+    return;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  wave_sort_lds(ex, elen);
+  for (int t = lane; t < k; t += 64) {
+    oidx[t] = (uint32_t)ex[t];
+    odst[t] = __uint_as_float((uint32_t)(ex[t] >> 32));
+  }
+
+  // ---- certification against every column outside the candidate set ----
+  bool cert = true;
+  if (taumin != ~0ull) {
+    const float dk = __uint_as_float((uint32_t)(ex[k - 1] >> 32));
+    const double tf = (double)unordered_f32((uint32_t)(taumin >> 32));
+    const double nu = (double)next_up(dk);
+    if (metric == 0) {
+      const double lb = (nzi + tf - E) * (1.0 - eps_exact);
+      cert = lb >= nu * nu * (1.0 + eps_exact) + 1e-300;
+    } else {
+      cert = (1.0 + tf - E - eps_exact) >= nu;
+    }
+  }
+  if (lane == 0) flag[grow] = cert ? 0u : 1u;
+}
+
+// Fallback work list: for each dataset, blocks of <= XS_QB flagged rows
+// (from the sorted global list) x xs column splits. Single thread; the
+// dataset count is small.
+__global__ void fallback_items_kernel(const DsDesc* __restrict__ ds_arr, int nds,
+                                      const uint32_t* __restrict__ list, const int* __restrict__ count,
+                                      int xs, WorkItem* __restrict__ items, int* __restrict__ nitems) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int cnt = *count;
+  int p = 0, w = 0;
+  for (int di = 0; di < nds; ++di) {
+    const int64_t end = ds_arr[di].row_off + ds_arr[di].n;
+    int q = p;
+    while (q < cnt && (int64_t)list[q] < end) ++q;
+    for (int b = p; b < q; b += XS_QB) {
+      const int qc = q - b < XS_QB ? q - b : XS_QB;
+      for (int s = 0; s < xs; ++s) items[w++] = WorkItem{di, s, b, qc};
+    }
+    p = q;
+  }
+  *nitems = w;
+}
+
+}  // namespace hm

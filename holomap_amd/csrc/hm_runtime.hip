@@ -1,0 +1,1042 @@
+// Host runtime of the holomap kNN-graph library: contexts, job planning and
+// allocation, the device-driven pipeline (no host synchronisation between
+// stages, so a job can be captured into a HIP graph and replayed), and the
+// C ABI declared in include/holomap_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/holomap_gpu.h"
+#include "hm_internal.hpp"
+
+namespace hm {
+// kernels (defined in the other translation units)
+__global__ void rownorm_kernel(const DsDesc*, int, int64_t, float*, double*);
+__global__ void prep_kernel(const DsDesc*, int, int64_t, const int64_t*, const float*,
+                            const double*, int);
+__global__ void statmax_kernel(const DsDesc*);
+template <int MODE, int QCAP>
+__global__ void gram_topc_kernel(const DsDesc*, const WorkItem*, const int*, int, const uint32_t*);
+__global__ void exact_scan_kernel(const DsDesc*, const WorkItem*, const int*, int,
+                                  const uint32_t*, int, int, uint64_t*);
+__global__ void exact_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t, int,
+                                   int, const uint64_t*);
+__global__ void rerank_kernel(const DsDesc*, int, int64_t, int, uint32_t*);
+__global__ void fallback_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int,
+                                      WorkItem*, int*);
+__global__ void smooth_knn_kernel(const DsDesc*, int, int64_t, const double*, int, float*, float*,
+                                  double*, uint32_t*);
+__global__ void calib_fixup_kernel(const DsDesc*, int, float*, double*, const uint32_t*);
+__global__ void directed_kernel(const DsDesc*, int, int64_t, const float*, const float*, uint32_t,
+                                uint32_t*, uint32_t*, float*, uint32_t*);
+__global__ void expf_kernel(const float*, float*, int64_t);
+template <typename T>
+__global__ void scan_tile_sums(const T*, int64_t, T*);
+template <typename T>
+__global__ void scan_sums(T*, int64_t, T*);
+template <typename T>
+__global__ void scan_tiles(const T*, int64_t, const T*, T*);
+__global__ void radix_hist(const uint32_t*, int64_t, int, int64_t, uint32_t*);
+__global__ void radix_scatter(const uint32_t*, const uint32_t*, int64_t, int, int64_t,
+                              const uint32_t*, uint32_t*, uint32_t*);
+__global__ void lower_bounds(const uint32_t*, int64_t, int64_t, uint32_t*);
+__global__ void compact_kernel(const uint32_t*, const uint32_t*, int64_t, uint32_t*, int*);
+__global__ void fill_u32(uint32_t*, int64_t, uint32_t);
+__global__ void gather_key2(const uint32_t*, const uint32_t*, const uint32_t*, int64_t, uint32_t,
+                            uint32_t*);
+__global__ void union_kernel(int, const DsDesc*, int, int64_t, const uint32_t*, const uint32_t*,
+                             const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*,
+                             const float*, uint64_t*, const uint64_t*, uint32_t*, uint32_t*, float*);
+__global__ void local_indptr_kernel(const DsDesc*, int, int64_t, const uint64_t*, const uint64_t*,
+                                    const int64_t*, uint64_t*);
+}  // namespace hm
+
+using namespace hm;
+
+struct hm_ctx {
+  int device = 0;
+  int num_cus = 256;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0;
+  uint64_t in_use = 0;
+  std::string err;
+};
+
+namespace {
+
+constexpr int kScanTile = 4096;
+constexpr int kRadixTile = 4096;
+
+struct DsHost {
+  int64_t n, n_pad, row_off, edge_off;
+  int d, kp, k, metric, C, nsplit, nlist, cout;
+};
+
+int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+int bitlen(uint64_t v) {
+  int b = 0;
+  while (v) { ++b; v >>= 1; }
+  return b;
+}
+
+}  // namespace
+
+struct hm_job {
+  hm_ctx* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  hm_options opt{};
+  uint32_t stages = HM_STAGE_ALL;
+  int nds = 0;
+  std::vector<DsHost> dsh;
+  std::vector<DsDesc> hdesc;
+  int64_t total_rows = 0, total_pad = 0, total_edges = 0;
+  int kmax = 0, ql = 48, mode = 0, xs = 1, fb_cap = 0, n_items_l1 = 0;
+  std::vector<WorkItem> items_l1_h;
+  std::vector<void*> allocs;
+  uint64_t bytes = 0;
+  // device buffers
+  DsDesc* d_desc = nullptr;
+  float *x = nullptr, *xr = nullptr, *op32c = nullptr, *colbias = nullptr, *rstat = nullptr,
+        *smax = nullptr, *nz = nullptr, *knn_dist = nullptr, *sigma = nullptr, *rho = nullptr,
+        *wval = nullptr, *vals = nullptr;
+  uint16_t* op16 = nullptr;
+  double *rnorm = nullptr, *sigma64 = nullptr, *target = nullptr;
+  uint64_t *cand = nullptr, *ctau = nullptr, *xpart = nullptr, *cntu = nullptr,
+           *indptr_g = nullptr, *total_nnz = nullptr, *indptr_l = nullptr, *sc64 = nullptr;
+  uint32_t *knn_idx = nullptr, *flag = nullptr, *flagpos = nullptr, *list = nullptr,
+           *zrow = nullptr, *key1 = nullptr, *pay1 = nullptr, *srcrow = nullptr, *k1s = nullptr,
+           *p1s = nullptr, *key2 = nullptr, *k2s = nullptr, *p2s = nullptr, *tmpk = nullptr,
+           *tmpv = nullptr, *inptr = nullptr, *outptr = nullptr, *rows = nullptr, *cols = nullptr,
+           *table = nullptr, *toffs = nullptr, *sc32 = nullptr, *sc32t = nullptr;
+  int *count = nullptr, *nitems_fb = nullptr;
+  WorkItem *items_l1 = nullptr, *items_fb = nullptr;
+  int64_t *pad_off = nullptr, *iptr_off = nullptr;
+  std::vector<int64_t> iptr_off_h;
+  // graph and timing
+  hipGraphExec_t gexec = nullptr;
+  bool captured = false;
+  hipEvent_t ev[HM_NTIMES + 1] = {};
+  float times[HM_NTIMES] = {};
+  bool timed = false;
+};
+
+namespace {
+
+hm_status fail(hm_ctx* ctx, hm_status st, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return st;
+}
+
+#define HM_HIP(ctx, call)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail((ctx), HM_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define HM_LAUNCH(ctx)                                                                    \
+  do {                                                                                    \
+    hipError_t e_ = hipGetLastError();                                                    \
+    if (e_ != hipSuccess)                                                                 \
+      return fail((ctx), HM_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// per-lane list capacity of the sweep: keeps ceil(C/2) .. QL-16 keys
+int pick_ql(int cmax) { return cmax <= 64 ? 48 : 64; }
+
+int gram_mode(int precision) {
+  switch (precision) {
+    case HM_PREC_FP32: return 1;
+    case HM_PREC_BF16: return 0;
+    case HM_PREC_BF16X3: return 2;
+    default: return -1;
+  }
+}
+
+int choose_nsplit(int64_t n, int C, int cus, int64_t other_items) {
+  const int64_t rb = cdiv(n, 128);
+  int best = 1;
+  double bestc = 1e300;
+  const int smax = std::max(1, std::min<int>(8, std::min<int>(512 / std::max(C, 1), (int)(n / 256))));
+  for (int s = 1; s <= smax; ++s) {
+    const double rounds = (double)cdiv(rb * s + other_items, cus);
+    const double c = rounds / s + 0.02 * s;
+    if (c < bestc - 1e-12) { bestc = c; best = s; }
+  }
+  return best;
+}
+
+struct Plan {
+  std::vector<DsHost> dsh;
+  int64_t total_rows = 0, total_pad = 0, total_edges = 0, sum_xr = 0, sum_cos = 0, sum_cand = 0,
+          fb_cap = 0;
+  int kmax = 0, cmax = 0, xs = 1;
+};
+
+Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_t* k,
+               const int32_t* metric, const hm_options& opt, int cus) {
+  Plan P;
+  P.dsh.resize(nds);
+  int64_t nmax = 0;
+  for (uint32_t i = 0; i < nds; ++i) {
+    DsHost& h = P.dsh[i];
+    h.n = (int64_t)n[i];
+    h.d = (int)d[i];
+    h.k = (int)k[i];
+    h.metric = metric ? metric[i] : 0;
+    h.kp = (int)cdiv(h.d, 32) * 32;
+    h.n_pad = cdiv(h.n, 128) * 128;
+    int margin = opt.margin < 0 ? 2 * h.k : opt.margin;
+    int C = h.k + margin;
+    C = std::min(C, 96);  // two lane lists of up to 48 keys
+    C = std::max(C, h.k);
+    h.C = C;
+    h.row_off = P.total_rows;
+    h.edge_off = P.total_edges;
+    P.total_rows += h.n;
+    P.total_pad += h.n_pad;
+    P.total_edges += h.n * h.k;
+    P.sum_xr += h.n_pad * h.kp;
+    if (h.metric == 1) P.sum_cos += h.n_pad * h.kp;
+    P.kmax = std::max(P.kmax, h.k);
+    P.cmax = std::max(P.cmax, C);
+    nmax = std::max(nmax, h.n);
+  }
+  // sweep splits: a lone dataset (or a small batch) splits its columns so
+  // the launch has enough work items for every CU
+  const int ql = pick_ql(P.cmax);
+  const int cout = ql - 16;
+  int64_t rb_total = 0;
+  for (auto& h : P.dsh) rb_total += cdiv(h.n, 128);
+  for (auto& h : P.dsh) {
+    const int smax = std::max(1, 256 / cout);  // re-rank holds <= 512 keys per row
+    if (opt.nsplit > 0) {
+      h.nsplit = std::max(1, std::min<int>(opt.nsplit, smax));
+    } else if (rb_total >= 2 * cus) {
+      h.nsplit = 1;
+    } else {
+      h.nsplit = std::min(smax, choose_nsplit(h.n, h.C, cus, rb_total - cdiv(h.n, 128)));
+    }
+    h.nlist = 2 * h.nsplit;
+    h.cout = cout;
+    P.sum_cand += h.n * h.nlist * h.cout;
+  }
+  P.xs = (int)std::max<int64_t>(1, std::min<int64_t>(8, nmax / 2048));
+  for (auto& h : P.dsh) P.fb_cap += cdiv(h.n, 32) * P.xs;
+  return P;
+}
+
+uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
+  const int mode = gram_mode(opt.precision);
+  const int64_t R = P.total_rows, E = P.total_edges, nds = (int64_t)P.dsh.size();
+  uint64_t b = 0;
+  int64_t sum_nd = 0;
+  for (auto& h : P.dsh) sum_nd += h.n * h.d;
+  b += sum_nd * 4;                                   // x
+  if (stages & HM_STAGE_KNN) {
+    b += P.sum_xr * 4 + P.sum_cos * 4;               // xr, op32 (cosine)
+    if (mode == 0) b += P.sum_xr * 2;                // op16
+    b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + R * 12;
+    b += P.sum_cand * 8 + R * 16 * 8;                // cand, ctau (<= 16 lists)
+    b += R * 12 + P.fb_cap * 16 + R * P.xs * P.kmax * 8;
+  }
+  b += E * 8;                                        // knn idx + dist
+  b += R * 24 + nds * 8;                             // sigma, rho, sigma64, zrow
+  if (stages & HM_STAGE_FUZZY) {
+    b += E * 11 * 4 + (R + 1) * 8 + R * 16 + 16 + 2 * E * 12 + (R + nds) * 8;
+    b += 2 * 256 * (cdiv(E, kRadixTile) + 1) * 4;
+  }
+  b += 2 * (cdiv(std::max<int64_t>(256 * (cdiv(E, kRadixTile) + 1), R + 1), kScanTile) + 2) * 8;
+  return b + (1u << 20);
+}
+
+template <typename T>
+hm_status dalloc(hm_job* J, T** p, int64_t count) {
+  if (count <= 0) count = 1;
+  const size_t bytes = (size_t)count * sizeof(T);
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(J->ctx, HM_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  J->allocs.push_back(q);
+  J->bytes += bytes;
+  *p = (T*)q;
+  return HM_OK;
+}
+
+#define HM_TRY(x)                 \
+  do {                            \
+    hm_status s_ = (x);           \
+    if (s_ != HM_OK) return s_;   \
+  } while (0)
+
+// ---- device building blocks ----
+
+template <typename T>
+hm_status scan_excl(hm_job* J, const T* in, int64_t n, T* out, T* total, T* tmp) {
+  if (n <= 0) return HM_OK;
+  const int64_t nt = cdiv(n, kScanTile);
+  hipLaunchKernelGGL(scan_tile_sums<T>, dim3((unsigned)nt), dim3(256), 0, J->stream, in, n, tmp);
+  hipLaunchKernelGGL(scan_sums<T>, dim3(1), dim3(1024), 0, J->stream, tmp, nt, total);
+  hipLaunchKernelGGL(scan_tiles<T>, dim3((unsigned)nt), dim3(256), 0, J->stream, in, n,
+                     (const T*)tmp, out);
+  HM_LAUNCH(J->ctx);
+  return HM_OK;
+}
+
+hm_status radix_sort(hm_job* J, const uint32_t* kin, const uint32_t* vin, int64_t n, int bits,
+                     uint32_t* kout, uint32_t* vout) {
+  if (n <= 0) return HM_OK;
+  const int passes = std::max(1, (bits + 7) / 8);
+  const int64_t nt = cdiv(n, kRadixTile);
+  const uint32_t* sk = kin;
+  const uint32_t* sv = vin;
+  uint32_t* dk = (passes & 1) ? kout : J->tmpk;
+  uint32_t* dv = (passes & 1) ? vout : J->tmpv;
+  for (int p = 0; p < passes; ++p) {
+    hipLaunchKernelGGL(radix_hist, dim3((unsigned)nt), dim3(256), 0, J->stream, sk, n, 8 * p, nt,
+                       J->table);
+    HM_TRY(scan_excl<uint32_t>(J, J->table, 256 * nt, J->toffs, (uint32_t*)nullptr, J->sc32));
+    hipLaunchKernelGGL(radix_scatter, dim3((unsigned)nt), dim3(256), 0, J->stream, sk, sv, n,
+                       8 * p, nt, (const uint32_t*)J->toffs, dk, dv);
+    HM_LAUNCH(J->ctx);
+    sk = dk;
+    sv = dv;
+    if (dk == kout) { dk = J->tmpk; dv = J->tmpv; } else { dk = kout; dv = vout; }
+  }
+  return HM_OK;
+}
+
+unsigned grid_for(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, cdiv(n, per)); }
+
+hm_status launch_sweep(hm_job* J) {
+  const unsigned grid = (unsigned)std::max(1, std::min(J->n_items_l1, J->ctx->num_cus));
+#define HM_GRAM(M, Q)                                                                       \
+  hipLaunchKernelGGL((gram_topc_kernel<M, Q>), dim3(grid), dim3(256), 0, J->stream, J->d_desc, \
+                     (const WorkItem*)J->items_l1, (const int*)nullptr, J->n_items_l1,       \
+                     (const uint32_t*)nullptr)
+  const int q = J->ql;
+  switch (J->mode) {
+    case 0: if (q == 48) HM_GRAM(0, 48); else HM_GRAM(0, 64); break;
+    case 1: if (q == 48) HM_GRAM(1, 48); else HM_GRAM(1, 64); break;
+    case 2: if (q == 48) HM_GRAM(2, 48); else HM_GRAM(2, 64); break;
+    default: break;
+  }
+#undef HM_GRAM
+  HM_LAUNCH(J->ctx);
+  return HM_OK;
+}
+
+void mark(hm_job* J, int i) {
+  if (J->timed) (void)hipEventRecord(J->ev[i], J->stream);
+}
+
+hm_status enqueue_pipeline(hm_job* J) {
+  hm_ctx* ctx = J->ctx;
+  const int nds = J->nds;
+  const int64_t R = J->total_rows, E = J->total_edges;
+  const int cus = ctx->num_cus;
+  mark(J, 0);
+  if (J->stages & HM_STAGE_KNN) {
+    // ---- prep ----
+    hipLaunchKernelGGL(rownorm_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, J->d_desc,
+                       nds, R, J->nz, J->rnorm);
+    const int mode_bits = (J->mode == 0 ? 1 : 0) | 2;
+    hipLaunchKernelGGL(prep_kernel, dim3(grid_for(J->total_pad, 4)), dim3(256), 0, J->stream,
+                       J->d_desc, nds, J->total_pad, (const int64_t*)J->pad_off,
+                       (const float*)J->nz, (const double*)J->rnorm, mode_bits);
+    hipLaunchKernelGGL(statmax_kernel, dim3(nds), dim3(256), 0, J->stream, J->d_desc);
+    HM_LAUNCH(ctx);
+    mark(J, 1);
+    // ---- candidate sweep + re-rank ----
+    if (J->mode >= 0) {
+      HM_TRY(launch_sweep(J));
+      mark(J, 2);
+      hipLaunchKernelGGL(rerank_kernel, dim3(grid_for(R, 4)), dim3(256), 0, J->stream, J->d_desc,
+                         nds, R, J->mode, J->flag);
+      HM_LAUNCH(ctx);
+    } else {
+      mark(J, 2);
+      hipLaunchKernelGGL(fill_u32, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, J->flag, R, 1u);
+      HM_LAUNCH(ctx);
+    }
+    mark(J, 3);
+    // ---- exact scan of the uncertified rows ----
+    HM_TRY(scan_excl<uint32_t>(J, J->flag, R, J->flagpos, (uint32_t*)nullptr, J->sc32));
+    hipLaunchKernelGGL(compact_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream,
+                       (const uint32_t*)J->flag, (const uint32_t*)J->flagpos, R, J->list, J->count);
+    hipLaunchKernelGGL(fallback_items_kernel, dim3(1), dim3(64), 0, J->stream, J->d_desc, nds,
+                       (const uint32_t*)J->list, (const int*)J->count, J->xs, J->items_fb,
+                       J->nitems_fb);
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(J->fb_cap, 2 * cus));
+    hipLaunchKernelGGL(exact_scan_kernel, dim3(gx), dim3(256), 0, J->stream, J->d_desc,
+                       (const WorkItem*)J->items_fb, (const int*)J->nitems_fb, 0,
+                       (const uint32_t*)J->list, J->xs, J->kmax, J->xpart);
+    const unsigned gm = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(R, 4), 8 * cus));
+    hipLaunchKernelGGL(exact_merge_kernel, dim3(gm), dim3(256), 0, J->stream, J->d_desc, nds,
+                       (const uint32_t*)J->list, (const int*)J->count, (int64_t)0, J->xs, J->kmax,
+                       (const uint64_t*)J->xpart);
+    HM_LAUNCH(ctx);
+    mark(J, 4);
+  } else {
+    for (int i = 1; i <= 4; ++i) mark(J, i);
+  }
+  if (J->stages & HM_STAGE_CALIB) {
+    hipLaunchKernelGGL(smooth_knn_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream,
+                       J->d_desc, nds, R, (const double*)J->target, J->opt.calib, J->sigma, J->rho,
+                       J->sigma64, J->zrow);
+    hipLaunchKernelGGL(calib_fixup_kernel, dim3(nds), dim3(256), 0, J->stream, J->d_desc,
+                       J->opt.calib, J->sigma, J->sigma64, (const uint32_t*)J->zrow);
+    HM_LAUNCH(ctx);
+  }
+  mark(J, 5);
+  if (J->stages & HM_STAGE_FUZZY) {
+    const uint32_t sent = (uint32_t)R;
+    const int bits = bitlen((uint64_t)R);
+    hipLaunchKernelGGL(directed_kernel, dim3(grid_for(E, 256)), dim3(256), 0, J->stream, J->d_desc,
+                       nds, E, (const float*)J->sigma, (const float*)J->rho, sent, J->key1, J->pay1,
+                       J->wval, J->srcrow);
+    HM_LAUNCH(ctx);
+    HM_TRY(radix_sort(J, J->key1, J->pay1, E, bits, J->k1s, J->p1s));
+    hipLaunchKernelGGL(lower_bounds, dim3(grid_for(R + 1, 256)), dim3(256), 0, J->stream,
+                       (const uint32_t*)J->k1s, E, R, J->inptr);
+    hipLaunchKernelGGL(gather_key2, dim3(grid_for(E, 256)), dim3(256), 0, J->stream,
+                       (const uint32_t*)J->p1s, (const uint32_t*)J->key1,
+                       (const uint32_t*)J->srcrow, E, sent, J->key2);
+    HM_LAUNCH(ctx);
+    HM_TRY(radix_sort(J, J->key2, J->p1s, E, bits, J->k2s, J->p2s));
+    hipLaunchKernelGGL(lower_bounds, dim3(grid_for(R + 1, 256)), dim3(256), 0, J->stream,
+                       (const uint32_t*)J->k2s, E, R, J->outptr);
+    hipLaunchKernelGGL(union_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, 0, J->d_desc,
+                       nds, R, (const uint32_t*)J->inptr, (const uint32_t*)J->p1s,
+                       (const uint32_t*)J->outptr, (const uint32_t*)J->p2s,
+                       (const uint32_t*)J->key1, (const uint32_t*)J->srcrow, (const float*)J->wval,
+                       J->cntu, (const uint64_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                       (float*)nullptr);
+    HM_LAUNCH(ctx);
+    HM_TRY(scan_excl<uint64_t>(J, J->cntu, R, J->indptr_g, J->total_nnz, J->sc64));
+    hipLaunchKernelGGL(union_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, 1, J->d_desc,
+                       nds, R, (const uint32_t*)J->inptr, (const uint32_t*)J->p1s,
+                       (const uint32_t*)J->outptr, (const uint32_t*)J->p2s,
+                       (const uint32_t*)J->key1, (const uint32_t*)J->srcrow, (const float*)J->wval,
+                       J->cntu, (const uint64_t*)J->indptr_g, J->rows, J->cols, J->vals);
+    hipLaunchKernelGGL(local_indptr_kernel, dim3(grid_for(R + nds, 256)), dim3(256), 0, J->stream,
+                       J->d_desc, nds, R, (const uint64_t*)J->indptr_g,
+                       (const uint64_t*)J->total_nnz, (const int64_t*)J->iptr_off, J->indptr_l);
+    HM_LAUNCH(ctx);
+  }
+  mark(J, 6);
+  return HM_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+void hm_default_options(hm_options* o) {
+  if (!o) return;
+  o->precision = HM_PREC_BF16;
+  o->calib = HM_CALIB_REF_F32;
+  o->margin = -1;
+  o->nsplit = 0;
+  o->use_graph = 0;
+  o->autotune = 0;
+  o->timing = 0;
+  o->reserved = 0;
+}
+
+const char* hm_version(void) { return "holomap_gpu 0.1 gfx950"; }
+
+int hm_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+hm_status hm_ctx_create(hm_ctx** out, int device, uint64_t budget) {
+  if (!out) return HM_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return HM_EDEVICE;
+  if (device < 0 || device >= n) return HM_EINVAL;
+  hm_ctx* c = new hm_ctx();
+  c->device = device;
+  c->budget = budget;
+  if (hipSetDevice(device) != hipSuccess) { delete c; return HM_EDEVICE; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return HM_EDEVICE;
+  }
+  *out = c;
+  return HM_OK;
+}
+
+void hm_ctx_destroy(hm_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* hm_last_error(const hm_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+hm_status hm_validate(const float* data, uint64_t len, uint32_t d, uint32_t k, uint64_t* bad) {
+  if (d == 0 || len == 0 || len % d != 0) return HM_EBADSHAPE;
+  if (k < 2) return HM_EINVAL;
+  const uint64_t n = len / d;
+  if (n <= k) return HM_ETOOFEW;
+  if (data) {
+    for (uint64_t i = 0; i < len; ++i) {
+      if (!std::isfinite(data[i])) {
+        if (bad) *bad = i;
+        return HM_ENONFINITE;
+      }
+    }
+  }
+  return HM_OK;
+}
+
+uint64_t hm_job_estimate_bytes(uint32_t nds, const uint64_t* n, const uint32_t* d,
+                               const uint32_t* k, const hm_options* opt) {
+  hm_options o;
+  if (opt) o = *opt; else hm_default_options(&o);
+  Plan P = make_plan(nds, n, d, k, nullptr, o, 256);
+  return plan_bytes(P, o, HM_STAGE_ALL);
+}
+
+hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint32_t* d,
+                        const uint32_t* k, const int32_t* metric, const hm_options* opt,
+                        uint32_t stages, hm_job** out) {
+  if (!ctx || !out || nds == 0 || !n || !d || !k) return fail(ctx, HM_EINVAL, "bad job arguments");
+  *out = nullptr;
+  hm_options o;
+  if (opt) o = *opt; else hm_default_options(&o);
+  const bool knn_stage = (stages & HM_STAGE_KNN) != 0;
+  for (uint32_t i = 0; i < nds; ++i) {
+    if (d[i] == 0) return fail(ctx, HM_EBADSHAPE, "n_features must be > 0");
+    if (n[i] == 0 || k[i] == 0) return fail(ctx, HM_EBADSHAPE, "empty input");
+    if (knn_stage) {
+      // stage-level kNN (src/knn.rs:22) needs k <= n; fit_transform's
+      // stricter n > k >= 2 is checked by hm_validate in hm_knn_graph
+      if (n[i] < k[i]) return fail(ctx, HM_ETOOFEW, "need at least n_neighbors points");
+      if (k[i] > 64)
+        return fail(ctx, HM_EUNSUPPORTED, "n_neighbors > 64 is not supported on the GPU path");
+    }
+    if (n[i] >= (1ull << 31)) return fail(ctx, HM_EUNSUPPORTED, "n >= 2^31");
+    if (metric && metric[i] != HM_EUCLIDEAN && metric[i] != HM_COSINE)
+      return fail(ctx, HM_EINVAL, "metric must be \"euclidean\" or \"cosine\"");
+  }
+  if (o.precision < HM_PREC_EXACT || o.precision > HM_PREC_BF16X3)
+    return fail(ctx, HM_EINVAL, "unknown precision");
+  if (o.calib != HM_CALIB_REF_F32 && o.calib != HM_CALIB_F64)
+    return fail(ctx, HM_EINVAL, "unknown calibration mode");
+  HM_HIP(ctx, hipSetDevice(ctx->device));
+  Plan P = make_plan(nds, n, d, k, metric, o, ctx->num_cus);
+  if (P.total_rows >= 0xffffffffll || P.total_edges >= 0xffffffffll)
+    return fail(ctx, HM_EUNSUPPORTED, "batch too large for 32-bit row/edge ids");
+  const uint64_t need = plan_bytes(P, o, stages);
+  if (ctx->budget && ctx->in_use + need > ctx->budget)
+    return fail(ctx, HM_EBUDGET, "job needs " + std::to_string(need) + " bytes, budget left " +
+                                     std::to_string(ctx->budget - ctx->in_use));
+
+  hm_job* J = new hm_job();
+  J->ctx = ctx;
+  J->stream = ctx->stream;
+  J->opt = o;
+  J->stages = stages;
+  J->nds = (int)nds;
+  J->dsh = P.dsh;
+  J->total_rows = P.total_rows;
+  J->total_pad = P.total_pad;
+  J->total_edges = P.total_edges;
+  J->kmax = P.kmax;
+  J->ql = pick_ql(P.cmax);
+  J->mode = gram_mode(o.precision);
+  J->xs = P.xs;
+  J->fb_cap = (int)P.fb_cap;
+  auto bail = [&](hm_status s) {
+    hm_job_destroy(J);
+    return s;
+  };
+#define HM_A(ptr, cnt)                          \
+  do {                                          \
+    hm_status s_ = dalloc(J, &(ptr), (cnt));    \
+    if (s_ != HM_OK) return bail(s_);           \
+  } while (0)
+  const int64_t R = P.total_rows, E = P.total_edges;
+  int64_t sum_nd = 0;
+  for (auto& h : P.dsh) sum_nd += h.n * h.d;
+  HM_A(J->d_desc, nds);
+  HM_A(J->x, sum_nd);
+  const bool knn = stages & HM_STAGE_KNN;
+  if (knn) {
+    HM_A(J->xr, P.sum_xr);
+    if (P.sum_cos > 0) HM_A(J->op32c, P.sum_cos);
+    if (J->mode == 0) HM_A(J->op16, P.sum_xr);
+    HM_A(J->colbias, P.total_pad);
+    HM_A(J->rstat, R * HM_NSTAT);
+    HM_A(J->smax, (int64_t)nds * HM_NSTAT);
+    HM_A(J->nz, R);
+    HM_A(J->rnorm, R);
+    HM_A(J->cand, std::max<int64_t>(P.sum_cand, 1));
+    int64_t sum_tau = 0;
+    for (auto& h : P.dsh) sum_tau += h.n * h.nlist;
+    HM_A(J->ctau, sum_tau);
+    HM_A(J->flag, R);
+    HM_A(J->flagpos, R);
+    HM_A(J->list, R);
+    HM_A(J->count, 1);
+    HM_A(J->nitems_fb, 1);
+    HM_A(J->items_fb, std::max<int64_t>(P.fb_cap, 1));
+    HM_A(J->xpart, R * P.xs * P.kmax);
+    HM_A(J->pad_off, nds + 1);
+  }
+  HM_A(J->knn_idx, E);
+  HM_A(J->knn_dist, E);
+  HM_A(J->sigma, R);
+  HM_A(J->rho, R);
+  HM_A(J->sigma64, R);
+  HM_A(J->zrow, R);
+  HM_A(J->target, nds);
+  const int64_t ntr = cdiv(E, kRadixTile) + 1;
+  if (stages & HM_STAGE_FUZZY) {
+    HM_A(J->key1, E); HM_A(J->pay1, E); HM_A(J->srcrow, E); HM_A(J->wval, E);
+    HM_A(J->k1s, E); HM_A(J->p1s, E); HM_A(J->key2, E); HM_A(J->k2s, E); HM_A(J->p2s, E);
+    HM_A(J->tmpk, E); HM_A(J->tmpv, E);
+    HM_A(J->inptr, R + 1); HM_A(J->outptr, R + 1);
+    HM_A(J->cntu, R); HM_A(J->indptr_g, R + 1); HM_A(J->total_nnz, 1);
+    HM_A(J->rows, 2 * E); HM_A(J->cols, 2 * E); HM_A(J->vals, 2 * E);
+    HM_A(J->indptr_l, R + nds); HM_A(J->iptr_off, nds);
+    HM_A(J->table, 256 * ntr); HM_A(J->toffs, 256 * ntr);
+  }
+  const int64_t scan_max = std::max<int64_t>(256 * ntr, R + 1);
+  HM_A(J->sc32, cdiv(scan_max, kScanTile) + 2);
+  HM_A(J->sc64, cdiv(scan_max, kScanTile) + 2);
+#undef HM_A
+
+  // descriptors
+  J->hdesc.resize(nds);
+  int64_t off_x = 0, off_xr = 0, off_cos = 0, off_pad = 0, off_cand = 0, off_tau = 0;
+  std::vector<int64_t> pad_off_h(nds + 1, 0);
+  std::vector<double> target_h(nds);
+  J->iptr_off_h.resize(nds);
+  for (uint32_t i = 0; i < nds; ++i) {
+    const DsHost& h = P.dsh[i];
+    DsDesc& D = J->hdesc[i];
+    memset(&D, 0, sizeof(D));
+    D.x = J->x + off_x;
+    if (knn) {
+      D.xr = J->xr + off_xr;
+      if (h.metric == 1) { D.op32 = J->op32c + off_cos; off_cos += h.n_pad * h.kp; }
+      else D.op32 = D.xr;
+      D.op16 = J->op16 ? J->op16 + off_xr : nullptr;
+      D.colbias = J->colbias + off_pad;
+      D.rnorm = J->rnorm + h.row_off;
+      D.rstat = J->rstat + h.row_off * HM_NSTAT;
+      D.smax = J->smax + (int64_t)i * HM_NSTAT;
+      D.cand = J->cand + off_cand;
+      D.ctau = J->ctau + off_tau;
+    }
+    D.knn_idx = J->knn_idx + h.edge_off;
+    D.knn_dist = J->knn_dist + h.edge_off;
+    D.n = h.n;
+    D.n_pad = h.n_pad;
+    D.row_off = h.row_off;
+    D.edge_off = h.edge_off;
+    D.d = h.d;
+    D.kp = h.kp;
+    D.k = h.k;
+    D.metric = h.metric;
+    D.C = h.C;
+    D.nsplit = h.nsplit;
+    D.nlist = h.nlist;
+    D.cout = h.cout;
+    pad_off_h[i] = off_pad;
+    off_x += h.n * h.d;
+    off_xr += h.n_pad * h.kp;
+    off_pad += h.n_pad;
+    off_cand += h.n * h.nlist * h.cout;
+    off_tau += h.n * h.nlist;
+    target_h[i] = std::log2((double)h.k);  // std f64::log2 (glibc), src/smooth_knn.rs:25
+    J->iptr_off_h[i] = h.row_off + i;
+  }
+  pad_off_h[nds] = off_pad;
+  hm_status st = HM_OK;
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    if (st == HM_OK && hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+      st = fail(ctx, HM_EDEVICE, "descriptor upload failed");
+  };
+  up(J->d_desc, J->hdesc.data(), sizeof(DsDesc) * nds);
+  up(J->target, target_h.data(), sizeof(double) * nds);
+  if (knn) up(J->pad_off, pad_off_h.data(), sizeof(int64_t) * (nds + 1));
+  if (stages & HM_STAGE_FUZZY) up(J->iptr_off, J->iptr_off_h.data(), sizeof(int64_t) * nds);
+  // sweep work items: (dataset, split, 128-row block), longest first
+  if (knn && J->mode >= 0) {
+    std::vector<std::pair<int64_t, WorkItem>> it;
+    for (uint32_t i = 0; i < nds; ++i) {
+      const DsHost& h = P.dsh[i];
+      const int64_t cost = cdiv(h.n, h.nsplit) * h.kp;
+      for (int64_t rb = 0; rb < h.n; rb += 128)
+        for (int s = 0; s < h.nsplit; ++s)
+          it.push_back({cost, WorkItem{(int)i, s, (int)rb, (int)std::min<int64_t>(128, h.n - rb)}});
+    }
+    std::stable_sort(it.begin(), it.end(),
+                     [](const auto& a, const auto& b) { return a.first > b.first; });
+    J->items_l1_h.resize(it.size());
+    for (size_t t = 0; t < it.size(); ++t) J->items_l1_h[t] = it[t].second;
+    J->n_items_l1 = (int)it.size();
+    hm_status s2 = dalloc(J, &J->items_l1, (int64_t)it.size());
+    if (s2 != HM_OK) return bail(s2);
+    up(J->items_l1, J->items_l1_h.data(), sizeof(WorkItem) * it.size());
+  }
+  if (st != HM_OK) return bail(st);
+  for (auto& e : J->ev) {
+    if (hipEventCreate(&e) != hipSuccess) return bail(fail(ctx, HM_EDEVICE, "event create"));
+  }
+  ctx->in_use += J->bytes;
+  *out = J;
+  return HM_OK;
+}
+
+void hm_job_destroy(hm_job* J) {
+  if (!J) return;
+  (void)hipSetDevice(J->ctx->device);
+  if (J->stream) (void)hipStreamSynchronize(J->stream);
+  if (J->gexec) (void)hipGraphExecDestroy(J->gexec);
+  for (auto e : J->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (void* p : J->allocs) (void)hipFree(p);
+  if (J->ctx->in_use >= J->bytes) J->ctx->in_use -= J->bytes;
+  delete J;
+}
+
+uint64_t hm_job_bytes(const hm_job* J) { return J ? J->bytes : 0; }
+
+hm_status hm_job_upload(hm_job* J, uint32_t ds, const float* data) {
+  if (!J || ds >= (uint32_t)J->nds || !data) return HM_EINVAL;
+  const DsHost& h = J->dsh[ds];
+  HM_HIP(J->ctx, hipMemcpyAsync((void*)J->hdesc[ds].x, data, (size_t)h.n * h.d * 4,
+                                hipMemcpyHostToDevice, J->stream));
+  return HM_OK;
+}
+
+hm_status hm_job_upload_device(hm_job* J, uint32_t ds, const float* dev) {
+  if (!J || ds >= (uint32_t)J->nds || !dev) return HM_EINVAL;
+  const DsHost& h = J->dsh[ds];
+  HM_HIP(J->ctx, hipMemcpyAsync((void*)J->hdesc[ds].x, dev, (size_t)h.n * h.d * 4,
+                                hipMemcpyDeviceToDevice, J->stream));
+  return HM_OK;
+}
+
+hm_status hm_job_set_knn(hm_job* J, uint32_t ds, const uint32_t* idx, const float* dist) {
+  if (!J || ds >= (uint32_t)J->nds || !idx || !dist) return HM_EINVAL;
+  const DsHost& h = J->dsh[ds];
+  const size_t m = (size_t)h.n * h.k;
+  HM_HIP(J->ctx, hipMemcpyAsync(J->knn_idx + h.edge_off, idx, m * 4, hipMemcpyHostToDevice, J->stream));
+  HM_HIP(J->ctx, hipMemcpyAsync(J->knn_dist + h.edge_off, dist, m * 4, hipMemcpyHostToDevice, J->stream));
+  return HM_OK;
+}
+
+hm_status hm_job_set_calib(hm_job* J, uint32_t ds, const float* sig, const float* rh) {
+  if (!J || ds >= (uint32_t)J->nds || !sig || !rh) return HM_EINVAL;
+  const DsHost& h = J->dsh[ds];
+  HM_HIP(J->ctx, hipMemcpyAsync(J->sigma + h.row_off, sig, h.n * 4, hipMemcpyHostToDevice, J->stream));
+  HM_HIP(J->ctx, hipMemcpyAsync(J->rho + h.row_off, rh, h.n * 4, hipMemcpyHostToDevice, J->stream));
+  return HM_OK;
+}
+
+hm_status hm_job_run(hm_job* J) {
+  if (!J) return HM_EINVAL;
+  hm_ctx* ctx = J->ctx;
+  HM_HIP(ctx, hipSetDevice(ctx->device));
+  J->timed = J->opt.timing && !J->opt.use_graph;
+  if (J->opt.use_graph) {
+    if (!J->captured) {
+      hipGraph_t g = nullptr;
+      HM_HIP(ctx, hipStreamBeginCapture(J->stream, hipStreamCaptureModeThreadLocal));
+      hm_status s = enqueue_pipeline(J);
+      hipError_t e = hipStreamEndCapture(J->stream, &g);
+      if (s != HM_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return s;
+      }
+      if (e != hipSuccess) return fail(ctx, HM_EDEVICE, std::string("capture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&J->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(ctx, HM_EDEVICE, std::string("instantiate: ") + hipGetErrorString(e));
+      J->captured = true;
+    }
+    HM_HIP(ctx, hipGraphLaunch(J->gexec, J->stream));
+    return HM_OK;
+  }
+  return enqueue_pipeline(J);
+}
+
+hm_status hm_job_sync(hm_job* J) {
+  if (!J) return HM_EINVAL;
+  HM_HIP(J->ctx, hipStreamSynchronize(J->stream));
+  if (J->timed) {
+    // stage boundaries: 0 start,1 prep,2 sweep,3 rerank,4 fallback,5 calib,6 fuzzy
+    const int map[6][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {5, 6}};
+    for (int s = 0; s < 6; ++s) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, J->ev[map[s][0]], J->ev[map[s][1]]);
+      J->times[s] = ms;
+    }
+    float tot = 0.f;
+    (void)hipEventElapsedTime(&tot, J->ev[0], J->ev[6]);
+    J->times[HM_T_TOTAL] = tot;
+  }
+  return HM_OK;
+}
+
+int hm_job_done(hm_job* J) {
+  if (!J) return HM_EINVAL;
+  hipError_t e = hipStreamQuery(J->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return HM_EDEVICE;
+}
+
+hm_status hm_job_times(const hm_job* J, float* ms) {
+  if (!J || !ms) return HM_EINVAL;
+  for (int i = 0; i < HM_NTIMES; ++i) ms[i] = J->times[i];
+  return HM_OK;
+}
+
+hm_status hm_job_counters(hm_job* J, uint64_t* c) {
+  if (!J || !c) return HM_EINVAL;
+  for (int i = 0; i < HM_NCOUNTERS; ++i) c[i] = 0;
+  if (J->stages & HM_STAGE_KNN) {
+    int cnt = 0;
+    HM_HIP(J->ctx, hipMemcpy(&cnt, J->count, sizeof(int), hipMemcpyDeviceToHost));
+    c[HM_C_FALLBACK_ROWS] = (uint64_t)cnt;
+    c[HM_C_CANDIDATES] = (uint64_t)J->dsh[0].nlist * J->dsh[0].cout;
+    c[HM_C_NSPLIT] = (uint64_t)J->dsh[0].nsplit;
+  }
+  if (J->stages & HM_STAGE_FUZZY) {
+    uint64_t nnz = 0;
+    HM_HIP(J->ctx, hipMemcpy(&nnz, J->total_nnz, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    c[HM_C_NNZ] = nnz;
+  }
+  return HM_OK;
+}
+
+hm_status hm_job_download(hm_job* J, uint32_t ds, hm_graph_out* o) {
+  if (!J || !o || ds >= (uint32_t)J->nds) return HM_EINVAL;
+  hm_ctx* ctx = J->ctx;
+  HM_HIP(ctx, hipStreamSynchronize(J->stream));
+  const DsHost& h = J->dsh[ds];
+  const size_t m = (size_t)h.n * h.k;
+  if (o->indices) HM_HIP(ctx, hipMemcpy(o->indices, J->knn_idx + h.edge_off, m * 4, hipMemcpyDeviceToHost));
+  if (o->dists) HM_HIP(ctx, hipMemcpy(o->dists, J->knn_dist + h.edge_off, m * 4, hipMemcpyDeviceToHost));
+  if (o->sigmas) HM_HIP(ctx, hipMemcpy(o->sigmas, J->sigma + h.row_off, h.n * 4, hipMemcpyDeviceToHost));
+  if (o->rhos) HM_HIP(ctx, hipMemcpy(o->rhos, J->rho + h.row_off, h.n * 4, hipMemcpyDeviceToHost));
+  if (o->sigmas64) HM_HIP(ctx, hipMemcpy(o->sigmas64, J->sigma64 + h.row_off, h.n * 8, hipMemcpyDeviceToHost));
+  o->nnz = 0;
+  if (J->stages & HM_STAGE_FUZZY) {
+    uint64_t se[2];
+    HM_HIP(ctx, hipMemcpy(se, J->indptr_g + h.row_off, 8, hipMemcpyDeviceToHost));
+    if (ds + 1 < (uint32_t)J->nds)
+      HM_HIP(ctx, hipMemcpy(se + 1, J->indptr_g + h.row_off + h.n, 8, hipMemcpyDeviceToHost));
+    else
+      HM_HIP(ctx, hipMemcpy(se + 1, J->total_nnz, 8, hipMemcpyDeviceToHost));
+    const uint64_t nnz = se[1] - se[0];
+    o->nnz = nnz;
+    if ((o->rows || o->cols || o->vals) && o->cap < nnz)
+      return fail(ctx, HM_EINVAL, "COO capacity " + std::to_string(o->cap) + " < nnz " + std::to_string(nnz));
+    if (o->rows) HM_HIP(ctx, hipMemcpy(o->rows, J->rows + se[0], nnz * 4, hipMemcpyDeviceToHost));
+    if (o->cols) HM_HIP(ctx, hipMemcpy(o->cols, J->cols + se[0], nnz * 4, hipMemcpyDeviceToHost));
+    if (o->vals) HM_HIP(ctx, hipMemcpy(o->vals, J->vals + se[0], nnz * 4, hipMemcpyDeviceToHost));
+    if (o->indptr)
+      HM_HIP(ctx, hipMemcpy(o->indptr, J->indptr_l + J->iptr_off_h[ds], (h.n + 1) * 8, hipMemcpyDeviceToHost));
+  }
+  return HM_OK;
+}
+
+hm_status hm_job_device_outputs(hm_job* J, uint32_t ds, const uint32_t** idx, const float** dist,
+                                const uint32_t** rows, const uint32_t** cols, const float** vals,
+                                const uint64_t** indptr) {
+  if (!J || ds >= (uint32_t)J->nds) return HM_EINVAL;
+  const DsHost& h = J->dsh[ds];
+  if (idx) *idx = J->knn_idx + h.edge_off;
+  if (dist) *dist = J->knn_dist + h.edge_off;
+  uint64_t s0 = 0;
+  if ((rows || cols || vals) && (J->stages & HM_STAGE_FUZZY)) {
+    HM_HIP(J->ctx, hipStreamSynchronize(J->stream));
+    HM_HIP(J->ctx, hipMemcpy(&s0, J->indptr_g + h.row_off, 8, hipMemcpyDeviceToHost));
+  }
+  if (rows) *rows = J->rows ? J->rows + s0 : nullptr;
+  if (cols) *cols = J->cols ? J->cols + s0 : nullptr;
+  if (vals) *vals = J->vals ? J->vals + s0 : nullptr;
+  if (indptr) *indptr = J->indptr_l ? J->indptr_l + J->iptr_off_h[ds] : nullptr;
+  return HM_OK;
+}
+
+// ---- stage-level and fused host-buffer entry points ----
+
+static hm_status one_job(hm_ctx* ctx, uint64_t n, uint32_t d, uint32_t k, int32_t metric,
+                         const hm_options* opt, uint32_t stages, hm_job** J) {
+  const uint64_t nn[1] = {n};
+  const uint32_t dd[1] = {d}, kk[1] = {k};
+  const int32_t mm[1] = {metric};
+  return hm_job_create(ctx, 1, nn, dd, kk, mm, opt, stages, J);
+}
+
+hm_status hm_exact_knn(hm_ctx* ctx, const float* data, uint64_t n, uint32_t d, uint32_t k,
+                       int32_t metric, const hm_options* opt, uint32_t* indices, float* dists) {
+  if (!ctx || !data || !indices || !dists) return fail(ctx, HM_EINVAL, "null argument");
+  if (d == 0 || n == 0) return fail(ctx, HM_EBADSHAPE, "empty input");
+  if (k == 0) return fail(ctx, HM_EINVAL, "n_neighbors must be >= 1");
+  if (n < k) return fail(ctx, HM_ETOOFEW, "need at least n_neighbors points");
+  for (uint64_t t = 0; t < n * d; ++t)
+    if (!std::isfinite(data[t])) return fail(ctx, HM_ENONFINITE, "non-finite input");
+  hm_job* J = nullptr;
+  HM_TRY(one_job(ctx, n, d, k, metric, opt, HM_STAGE_KNN, &J));
+  hm_status s = hm_job_upload(J, 0, data);
+  if (s == HM_OK) s = hm_job_run(J);
+  if (s == HM_OK) s = hm_job_sync(J);
+  hm_graph_out o{};
+  o.indices = indices;
+  o.dists = dists;
+  if (s == HM_OK) s = hm_job_download(J, 0, &o);
+  hm_job_destroy(J);
+  return s;
+}
+
+hm_status hm_smooth_knn(hm_ctx* ctx, const float* knn_dists, uint64_t n, uint32_t k, int32_t calib,
+                        float* sigmas, float* rhos) {
+  if (!ctx || !knn_dists || !sigmas || !rhos || n == 0 || k == 0)
+    return fail(ctx, HM_EINVAL, "null argument");
+  hm_options o;
+  hm_default_options(&o);
+  o.calib = calib;
+  hm_job* J = nullptr;
+  // a calibration-only job: the kNN distances are uploaded, not computed
+  const uint64_t nn[1] = {n};
+  const uint32_t dd[1] = {1}, kk[1] = {k};
+  hm_status s = hm_job_create(ctx, 1, nn, dd, kk, nullptr, &o, HM_STAGE_CALIB, &J);
+  if (s != HM_OK) return s;
+  std::vector<uint32_t> dummy((size_t)n * k, 0);
+  s = hm_job_set_knn(J, 0, dummy.data(), knn_dists);
+  if (s == HM_OK) s = hm_job_run(J);
+  if (s == HM_OK) s = hm_job_sync(J);
+  hm_graph_out out{};
+  out.sigmas = sigmas;
+  out.rhos = rhos;
+  if (s == HM_OK) s = hm_job_download(J, 0, &out);
+  hm_job_destroy(J);
+  return s;
+}
+
+hm_status hm_fuzzy_simplicial_set(hm_ctx* ctx, const uint32_t* idx, const float* dists,
+                                  const float* sigmas, const float* rhos, uint64_t n, uint32_t k,
+                                  uint32_t* rows, uint32_t* cols, float* vals, uint64_t cap,
+                                  uint64_t* nnz) {
+  if (!ctx || !idx || !dists || !sigmas || !rhos || !nnz) return fail(ctx, HM_EINVAL, "null argument");
+  for (uint64_t t = 0; t < n * k; ++t)
+    if (idx[t] >= n) return fail(ctx, HM_EINVAL, "neighbour index out of range");
+  hm_options o;
+  hm_default_options(&o);
+  const uint64_t nn[1] = {n};
+  const uint32_t dd[1] = {1}, kk[1] = {k};
+  hm_job* J = nullptr;
+  HM_TRY(hm_job_create(ctx, 1, nn, dd, kk, nullptr, &o, HM_STAGE_FUZZY, &J));
+  hm_status s = hm_job_set_knn(J, 0, idx, dists);
+  if (s == HM_OK) s = hm_job_set_calib(J, 0, sigmas, rhos);
+  if (s == HM_OK) s = hm_job_run(J);
+  if (s == HM_OK) s = hm_job_sync(J);
+  hm_graph_out out{};
+  out.rows = rows;
+  out.cols = cols;
+  out.vals = vals;
+  out.cap = cap;
+  if (s == HM_OK) s = hm_job_download(J, 0, &out);
+  *nnz = out.nnz;
+  hm_job_destroy(J);
+  return s;
+}
+
+hm_status hm_csr_from_fuzzy(hm_ctx* ctx, const uint32_t* rows, uint64_t nnz, uint64_t n,
+                            uint64_t* indptr) {
+  if (!ctx || !indptr || (!rows && nnz)) return fail(ctx, HM_EINVAL, "null argument");
+  HM_HIP(ctx, hipSetDevice(ctx->device));
+  uint32_t* drows = nullptr;
+  uint32_t* dptr = nullptr;
+  HM_HIP(ctx, hipMalloc(&drows, std::max<uint64_t>(nnz, 1) * 4));
+  if (hipMalloc(&dptr, (n + 1) * 4) != hipSuccess) {
+    (void)hipFree(drows);
+    return fail(ctx, HM_ENOMEM, "hipMalloc failed");
+  }
+  hm_status s = HM_OK;
+  if (nnz && hipMemcpy(drows, rows, nnz * 4, hipMemcpyHostToDevice) != hipSuccess)
+    s = fail(ctx, HM_EDEVICE, "upload failed");
+  if (s == HM_OK) {
+    // rows are sorted: indptr[r] = lower_bound(rows, r), indptr[n] = nnz
+    hipLaunchKernelGGL(lower_bounds, dim3(grid_for((int64_t)n + 1, 256)), dim3(256), 0, ctx->stream,
+                       (const uint32_t*)drows, (int64_t)nnz, (int64_t)n, dptr);
+    std::vector<uint32_t> h(n + 1);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+        hipMemcpy(h.data(), dptr, (n + 1) * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      s = fail(ctx, HM_EDEVICE, "csr kernel failed");
+    else
+      for (uint64_t i = 0; i <= n; ++i) indptr[i] = h[i];
+  }
+  (void)hipFree(drows);
+  (void)hipFree(dptr);
+  return s;
+}
+
+hm_status hm_knn_graph(hm_ctx* ctx, const float* data, uint64_t n, uint32_t d, uint32_t k,
+                       int32_t metric, const hm_options* opt, hm_graph_out* out) {
+  if (!ctx || !data || !out) return fail(ctx, HM_EINVAL, "null argument");
+  uint64_t bad = 0;
+  hm_status v = hm_validate(data, n * d, d, k, &bad);
+  if (v != HM_OK) return fail(ctx, v, "input validation failed");
+  hm_job* J = nullptr;
+  HM_TRY(one_job(ctx, n, d, k, metric, opt, HM_STAGE_ALL, &J));
+  hm_status s = hm_job_upload(J, 0, data);
+  if (s == HM_OK) s = hm_job_run(J);
+  if (s == HM_OK) s = hm_job_sync(J);
+  if (s == HM_OK) s = hm_job_download(J, 0, out);
+  hm_job_destroy(J);
+  return s;
+}
+
+hm_status hm_expf_array(hm_ctx* ctx, const float* x, float* y, uint64_t n) {
+  if (!ctx || !x || !y) return fail(ctx, HM_EINVAL, "null argument");
+  if (n == 0) return HM_OK;
+  HM_HIP(ctx, hipSetDevice(ctx->device));
+  float *dx = nullptr, *dy = nullptr;
+  HM_HIP(ctx, hipMalloc(&dx, n * 4));
+  if (hipMalloc(&dy, n * 4) != hipSuccess) {
+    (void)hipFree(dx);
+    return fail(ctx, HM_ENOMEM, "hipMalloc failed");
+  }
+  hm_status s = HM_OK;
+  if (hipMemcpy(dx, x, n * 4, hipMemcpyHostToDevice) != hipSuccess) s = fail(ctx, HM_EDEVICE, "upload");
+  if (s == HM_OK) {
+    hipLaunchKernelGGL(expf_kernel, dim3(grid_for((int64_t)n, 256)), dim3(256), 0, ctx->stream,
+                       (const float*)dx, dy, (int64_t)n);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+        hipMemcpy(y, dy, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      s = fail(ctx, HM_EDEVICE, "expf kernel failed");
+  }
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  return s;
+}
+
+}  // extern "C"

@@ -1,0 +1,214 @@
+// Deterministic, atomic-free building blocks for the symmetrisation:
+//   * exclusive prefix scan (three-phase: tile sums, single-workgroup scan of
+//     the sums, tile scan + offset) for u32 and u64;
+//   * stable LSD radix sort of (u32 key, u32 payload) with 8-bit digits.
+//     Tile digit counts and in-tile ranks come from wave ballots (a lane's
+//     peers are the lanes with the same digit), never from atomics, so the
+//     output permutation is a fixed function of the input.
+#include "hm_internal.hpp"
+
+namespace hm {
+
+#define SC_TILE 4096  // elements per scan tile (256 threads x 16)
+
+template <typename T>
+__global__ __launch_bounds__(256) void scan_tile_sums(const T* __restrict__ in, int64_t n,
+                                                      T* __restrict__ sums) {
+  __shared__ T red[256];
+  const int64_t base = (int64_t)blockIdx.x * SC_TILE + threadIdx.x * 16;
+  T s = 0;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) s += (base + e < n) ? in[base + e] : (T)0;
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+// Exclusive scan of the tile sums in place; also writes the grand total.
+template <typename T>
+__global__ __launch_bounds__(1024) void scan_sums(T* __restrict__ sums, int64_t ntiles,
+                                                  T* __restrict__ total) {
+  __shared__ T part[1024];
+  __shared__ T carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t b = 0; b < ntiles; b += 1024) {
+    const int64_t idx = b + threadIdx.x;
+    const T v = idx < ntiles ? sums[idx] : (T)0;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
+      T add = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : (T)0;
+      __syncthreads();
+      part[threadIdx.x] += add;
+      __syncthreads();
+    }
+    const T incl = part[threadIdx.x];
+    const T c = carry;
+    if (idx < ntiles) sums[idx] = c + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = c + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scan_tiles(const T* __restrict__ in, int64_t n,
+                                                  const T* __restrict__ offs, T* __restrict__ out) {
+  __shared__ T part[256];
+  const int64_t base = (int64_t)blockIdx.x * SC_TILE + threadIdx.x * 16;
+  T v[16];
+  T s = 0;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    v[e] = (base + e < n) ? in[base + e] : (T)0;
+    s += v[e];
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    T add = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : (T)0;
+    __syncthreads();
+    part[threadIdx.x] += add;
+    __syncthreads();
+  }
+  T run = offs[blockIdx.x] + part[threadIdx.x] - s;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    if (base + e < n) out[base + e] = run;
+    run += v[e];
+  }
+}
+
+template __global__ void scan_tile_sums<uint32_t>(const uint32_t*, int64_t, uint32_t*);
+template __global__ void scan_tile_sums<uint64_t>(const uint64_t*, int64_t, uint64_t*);
+template __global__ void scan_sums<uint32_t>(uint32_t*, int64_t, uint32_t*);
+template __global__ void scan_sums<uint64_t>(uint64_t*, int64_t, uint64_t*);
+template __global__ void scan_tiles<uint32_t>(const uint32_t*, int64_t, const uint32_t*, uint32_t*);
+template __global__ void scan_tiles<uint64_t>(const uint64_t*, int64_t, const uint64_t*, uint64_t*);
+
+// ---------------------------------------------------------------------------
+// Radix sort. Tile = 4096 elements = 4 waves x 16 rounds x 64 lanes; element
+// order inside a tile is (wave, round, lane), i.e. index order, so ranks
+// computed in that order keep the sort stable.
+// ---------------------------------------------------------------------------
+#define RX_TILE 4096
+
+__device__ __forceinline__ uint64_t match_digit(uint32_t dig, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (dig >> b) & 1u;
+    const uint64_t bb = __ballot(bit);
+    peers &= bit ? bb : ~bb;
+  }
+  return valid ? peers : 0ull;
+}
+
+// table[digit * ntiles + tile] = count of `digit` in the tile
+__global__ __launch_bounds__(256) void radix_hist(const uint32_t* __restrict__ keys, int64_t n,
+                                                  int shift, int64_t ntiles,
+                                                  uint32_t* __restrict__ table) {
+  __shared__ uint32_t cnt[4][256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = threadIdx.x; t < 1024; t += 256) (&cnt[0][0])[t] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RX_TILE + wave * 1024;
+  for (int r = 0; r < 16; ++r) {
+    const int64_t idx = base + r * 64 + lane;
+    const bool valid = idx < n;
+    const uint32_t dig = valid ? (keys[idx] >> shift) & 255u : 0u;
+    const uint64_t peers = match_digit(dig, valid);
+    const int leader = peers ? (int)__ffsll((long long)peers) - 1 : 64;
+    if (valid && lane == leader) cnt[wave][dig] += (uint32_t)__popcll(peers);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  __syncthreads();
+  const int dg = threadIdx.x;
+  table[(int64_t)dg * ntiles + blockIdx.x] = cnt[0][dg] + cnt[1][dg] + cnt[2][dg] + cnt[3][dg];
+}
+
+// offs = exclusive scan of table (digit-major)
+__global__ __launch_bounds__(256) void radix_scatter(const uint32_t* __restrict__ kin,
+                                                     const uint32_t* __restrict__ vin, int64_t n,
+                                                     int shift, int64_t ntiles,
+                                                     const uint32_t* __restrict__ offs,
+                                                     uint32_t* __restrict__ kout,
+                                                     uint32_t* __restrict__ vout) {
+  __shared__ uint32_t cnt[4][256];
+  __shared__ uint32_t wbase[4][256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = threadIdx.x; t < 1024; t += 256) (&cnt[0][0])[t] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RX_TILE + wave * 1024;
+  uint32_t key[16], val[16], rank[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int64_t idx = base + r * 64 + lane;
+    const bool valid = idx < n;
+    key[r] = valid ? kin[idx] : 0u;
+    val[r] = valid ? vin[idx] : 0u;
+    const uint32_t dig = (key[r] >> shift) & 255u;
+    const uint64_t peers = match_digit(dig, valid);
+    const uint32_t before = cnt[wave][dig];  // earlier rounds of this wave
+    rank[r] = before + (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const int leader = peers ? (int)__ffsll((long long)peers) - 1 : 64;
+    if (valid && lane == leader) cnt[wave][dig] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  __syncthreads();
+  {
+    const int dg = threadIdx.x;
+    uint32_t run = offs[(int64_t)dg * ntiles + blockIdx.x];
+    for (int w = 0; w < 4; ++w) {
+      wbase[w][dg] = run;
+      run += cnt[w][dg];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int64_t idx = base + r * 64 + lane;
+    if (idx < n) {
+      const uint32_t dig = (key[r] >> shift) & 255u;
+      const uint32_t pos = wbase[wave][dig] + rank[r];
+      kout[pos] = key[r];
+      vout[pos] = val[r];
+    }
+  }
+}
+
+// lower_bound of every value v in [0, nv) in a sorted key array
+__global__ void lower_bounds(const uint32_t* __restrict__ sorted, int64_t n, int64_t nv,
+                             uint32_t* __restrict__ out) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > nv) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)sorted[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  out[v] = (uint32_t)lo;
+}
+
+// list[pos[i]] = i for flagged rows (pos = exclusive scan of flag); count.
+__global__ void compact_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                               int64_t n, uint32_t* __restrict__ list, int* __restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flag[i]) list[pos[i]] = (uint32_t)i;
+  if (i == n - 1) *count = (int)(pos[i] + flag[i]);
+}
+
+__global__ void fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace hm

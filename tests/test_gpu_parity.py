@@ -1,0 +1,253 @@
+"""GPU parity: the HIP path against the oracle and the reference's golden
+fixtures. kNN indices/distances, rho, sigma (ref_f32 mode) and the fuzzy
+graph must be bit-identical; FP64 calibration within the stated tolerance."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import holomap_amd as H
+import oracle as O
+from holomap_amd import datasets as DS
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+FIXTURES = ["blobs_euclidean", "cosine", "duplicates_euclidean"]
+PRECS = ["exact", "fp32", "bf16", "bf16x3"]
+
+
+def load_fixture(name):
+    with open(os.path.join(GOLD, f"stage2_{name}.json")) as f:
+        fx = json.load(f)
+    return fx, np.array(fx["data"], np.float64).astype(np.float32)
+
+
+def assert_graph_equal(res, ref_knn, ref_cal, ref_g):
+    assert np.array_equal(res.knn.indices, ref_knn.indices)
+    assert np.array_equal(res.knn.dists.view(np.uint32), ref_knn.dists.view(np.uint32))
+    assert np.array_equal(res.calib.rhos.view(np.uint32), ref_cal.rhos.view(np.uint32))
+    assert np.array_equal(res.calib.sigmas.view(np.uint32), ref_cal.sigmas.view(np.uint32))
+    assert np.array_equal(res.graph.rows, ref_g.rows)
+    assert np.array_equal(res.graph.cols, ref_g.cols)
+    assert np.array_equal(res.graph.vals.view(np.uint32), ref_g.vals.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("prec", PRECS)
+def test_fixtures_all_precisions(ctx, name, prec):
+    fx, data = load_fixture(name)
+    d, k = fx["n_features"], fx["n_neighbors"]
+    res = H.knn_graph(data, d, k, fx["metric"], precision=prec, ctx=ctx)
+    assert np.array_equal(res.knn.indices, np.array(fx["knn_indices"], np.uint32))
+    assert np.array_equal(res.knn.dists, np.array(fx["knn_dists"], np.float64).astype(np.float32))
+    assert np.array_equal(res.graph.rows, np.array(fx["graph_rows"], np.uint32))
+    assert np.array_equal(res.graph.cols, np.array(fx["graph_cols"], np.uint32))
+    e = np.array(fx["graph_vals"])
+    assert np.all(np.abs(res.graph.vals - e) <= 1e-5 * (1 + np.abs(e)))
+    ref = O.knn_graph(data, d, k, fx["metric"])
+    assert_graph_equal(res, *ref)
+    ip = O.csr_indptr(ref[2].rows, data.shape[0])
+    assert np.array_equal(res.indptr, ip)
+
+
+def test_known_answers(ctx):
+    # src/knn.rs:64-104, src/smooth_knn.rs:103-124, src/fuzzy.rs:111-153
+    knn = H.exact_knn(np.array([0.0, 1.0, 3.0, 7.0], np.float32), 1, 3, ctx=ctx)
+    assert knn.indices[1].tolist() == [1, 0, 2] and knn.dists[3].tolist() == [0.0, 4.0, 6.0]
+    dup = H.exact_knn(np.array([5.0, 9.0, 5.0], np.float32), 1, 2, ctx=ctx)
+    assert dup.indices[2].tolist() == [0, 2]
+    c = H.exact_knn(np.array([0.3, 0.7, 0.31, 0.69], np.float32), 2, 2, "cosine", ctx=ctx)
+    assert c.dists[0, 0] == 0.0 and c.dists[0, 1] > 0.0
+    a = H.exact_knn(np.array([1.0, 0.0, 100.0, 1.0, 0.1, 0.1], np.float32), 2, 3, "cosine", ctx=ctx)
+    assert a.indices[0].tolist() == [0, 1, 2]
+    cal = H.smooth_knn(np.array([0, 1, 3, 0, 1, 2, 0, 2, 3], np.float32), 3, ctx=ctx)
+    assert cal.sigmas.tolist() == [3.72998046875, 1.864990234375, 1.864990234375]
+    assert cal.rhos.tolist() == [1.0, 1.0, 2.0]
+    z = H.smooth_knn(np.zeros(4, np.float32), 2, ctx=ctx)
+    assert z.sigmas.tolist() == [1.0, 1.0]
+    k2 = H.Knn(2, np.array([0, 1, 1, 2, 2, 1], np.uint32).reshape(3, 2),
+               np.array([0, 1, 0, 0.1, 0, 0.1], np.float32).reshape(3, 2))
+    c2 = H.SmoothKnn(np.ones(3, np.float32), np.array([0.5, 0.1, 0.1], np.float32))
+    g = H.fuzzy_simplicial_set(k2, c2, ctx=ctx)
+    w = O.expf(-0.5)
+    assert g.rows.tolist() == [0, 1, 1, 2] and g.cols.tolist() == [1, 0, 2, 1]
+    assert g.vals.tolist() == [w, w, 1.0, 1.0]
+    csr = H.Csr.from_fuzzy(H.FuzzyGraph(3, np.array([0, 1, 1, 2], np.uint32),
+                                        np.array([1, 0, 2, 1], np.uint32),
+                                        np.ones(4, np.float32)), ctx=ctx)
+    assert csr.indptr.tolist() == [0, 1, 3, 4]
+    idx, vals = csr.row(1)
+    assert idx.tolist() == [0, 2] and vals.tolist() == [1.0, 1.0]
+
+
+def test_device_expf_matches_oracle(ctx):
+    rng = np.random.default_rng(1)
+    bits = rng.integers(0x80000000, 0xC2D00000, size=200000, dtype=np.uint64).astype(np.uint32)
+    x = np.concatenate([bits.view(np.float32), np.linspace(-110, 0, 5001, dtype=np.float32),
+                        np.array([0.0, -0.0, -1e-30, -87.5, -103.9, -104.0], np.float32)])
+    y = H.expf_device(x, ctx=ctx)
+    L = O.lib()
+    ref = np.array([L.hmo_expf(float(v)) for v in x], np.float32)
+    assert np.array_equal(y.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("metric", ["euclidean", "cosine"])
+def test_c1_full_graph(ctx, prec, metric):
+    x = DS.c1()
+    res = H.knn_graph(x, 50, 15, metric, precision=prec, ctx=ctx)
+    assert_graph_equal(res, *O.knn_graph(x, 50, 15, metric))
+
+
+@pytest.mark.parametrize("shape", [(257, 3, 5), (130, 33, 7), (129, 1, 2), (200, 50, 64),
+                                   (65, 130, 64), (1000, 17, 30)])
+@pytest.mark.parametrize("metric", ["euclidean", "cosine"])
+def test_ragged_shapes(ctx, shape, metric):
+    n, d, k = shape
+    rng = np.random.default_rng(n * 7 + d)
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    ref = O.knn_graph(x, d, k, metric)
+    for prec in ["bf16", "fp32"]:
+        res = H.knn_graph(x, d, k, metric, precision=prec, ctx=ctx)
+        assert_graph_equal(res, *ref)
+
+
+def test_duplicates_zero_rows_and_ties(ctx):
+    rng = np.random.default_rng(3)
+    n, d, k = 600, 12, 15
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    x[100:130] = x[5]          # 31 identical rows: rho = 0 rows, mean_all floor
+    x[200] = 0.0
+    x[201] = 0.0               # zero vectors (cosine distance 0 to all)
+    x[300:310] = np.round(x[300:310])  # many exact distance ties
+    x[301] = x[300] * 2.0      # parallel pair
+    for metric in ["euclidean", "cosine"]:
+        ref = O.knn_graph(x, d, k, metric)
+        for prec in PRECS:
+            res = H.knn_graph(x, d, k, metric, precision=prec, ctx=ctx)
+            assert_graph_equal(res, *ref)
+
+
+def test_c2_sampled_rows_and_mode_agreement(ctx):
+    x = DS.c2()
+    n, d, k = x.shape[0], 128, 15
+    out = {}
+    for prec in ["bf16", "fp32", "exact"]:
+        job = H.Job(ctx, [n], [d], [k], ["euclidean"], precision=prec)
+        job.upload(0, x)
+        job.run()
+        job.sync()
+        out[prec] = (job.download(0), job.counters())
+        job.close()
+    rows = np.random.default_rng(0).choice(n, 256, replace=False).astype(np.uint32)
+    sub = O.exact_knn_rows(x, d, k, "euclidean", rows, nthreads=8)
+    for prec, (res, cnt) in out.items():
+        assert np.array_equal(res.knn.indices[rows], sub.indices), prec
+        assert np.array_equal(res.knn.dists[rows].view(np.uint32), sub.dists.view(np.uint32)), prec
+    base = out["exact"][0]
+    for prec in ["bf16", "fp32"]:
+        r = out[prec][0]
+        assert np.array_equal(r.knn.indices, base.knn.indices)
+        assert np.array_equal(r.knn.dists.view(np.uint32), base.knn.dists.view(np.uint32))
+        assert np.array_equal(r.graph.vals.view(np.uint32), base.graph.vals.view(np.uint32))
+    # the downstream stages against the oracle, on the GPU's kNN
+    cal = O.smooth_knn(base.knn.dists, k)
+    assert np.array_equal(base.calib.sigmas.view(np.uint32), cal.sigmas.view(np.uint32))
+    g = O.fuzzy_simplicial_set(O.Knn(k, base.knn.indices, base.knn.dists), cal)
+    assert np.array_equal(base.graph.rows, g.rows) and np.array_equal(base.graph.cols, g.cols)
+    assert np.array_equal(base.graph.vals.view(np.uint32), g.vals.view(np.uint32))
+
+
+def test_determinism_byte_identical(ctx):
+    x = DS.c2(n=6000)
+    a = H.knn_graph(x, 128, 15, precision="bf16", ctx=ctx)
+    b = H.knn_graph(x, 128, 15, precision="bf16", ctx=ctx)
+    for f in ["indices", "dists"]:
+        assert getattr(a.knn, f).tobytes() == getattr(b.knn, f).tobytes()
+    assert a.graph.vals.tobytes() == b.graph.vals.tobytes()
+    assert a.graph.cols.tobytes() == b.graph.cols.tobytes()
+
+
+def test_batch_equals_alone(ctx):
+    rng = np.random.default_rng(4)
+    specs = [(700, 32, 10, "euclidean"), (333, 64, 15, "cosine"), (1500, 128, 30, "euclidean"),
+             (260, 256, 15, "cosine")]
+    data = [rng.normal(size=(n, d)).astype(np.float32) for n, d, _, _ in specs]
+    job = H.Job(ctx, [s[0] for s in specs], [s[1] for s in specs], [s[2] for s in specs],
+                [s[3] for s in specs])
+    for i, x in enumerate(data):
+        job.upload(i, x)
+    job.run()
+    job.sync()
+    for i, (n, d, k, m) in enumerate(specs):
+        r = job.download(i)
+        assert_graph_equal(r, *O.knn_graph(data[i], d, k, m))
+    job.close()
+
+
+def test_graph_replay(ctx):
+    x = DS.c2(n=3000, d=64)
+    ref = O.knn_graph(x, 64, 15, "euclidean")
+    job = H.Job(ctx, [3000], [64], [15], ["euclidean"], use_graph=True)
+    job.upload(0, x)
+    for _ in range(3):
+        job.run()
+        job.sync()
+        assert_graph_equal(job.download(0), *ref)
+    y = DS.c2(seed=8, n=3000, d=64)
+    job.upload(0, y)
+    job.run()
+    job.sync()
+    assert_graph_equal(job.download(0), *O.knn_graph(y, 64, 15, "euclidean"))
+    job.close()
+
+
+def test_f64_calibration_vs_fp64_oracle(ctx):
+    x = DS.c1()
+    job = H.Job(ctx, [1000], [50], [15], ["euclidean"], calib="f64")
+    job.upload(0, x)
+    job.run()
+    job.sync()
+    r = job.download(0)
+    job.close()
+    c64 = O.smooth_knn_f64(r.knn.dists, 15)
+    rel = np.abs(r.calib.sigmas64 - c64.sigmas) / c64.sigmas
+    assert rel.max() < 1e-12
+    ref32 = O.smooth_knn(r.knn.dists, 15)
+    # distance of the reference's f32 search from the FP64 root
+    assert (np.abs(ref32.sigmas - c64.sigmas) / c64.sigmas).max() < 1e-3
+
+
+@pytest.mark.parametrize("metric", ["euclidean", "cosine"])
+def test_heavy_ties_in_candidate_lists(ctx, metric):
+    # 300 identical rows make equal selection keys straddle the lane-list
+    # window; in cosine, 120 zero rows give -inf keys to every query
+    rng = np.random.default_rng(12)
+    n, d, k = 3000, 24, 15
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    x[1000:1300] = x[17]
+    if metric == "cosine":
+        x[2000:2120] = 0.0
+    ref = O.knn_graph(x, d, k, metric)
+    for prec in ["bf16", "fp32", "bf16x3"]:
+        res = H.knn_graph(x, d, k, metric, precision=prec, ctx=ctx)
+        assert_graph_equal(res, *ref)
+
+
+def test_margin_and_split_invariance(ctx):
+    # the candidate configuration changes only which keys are examined:
+    # outputs are identical for every margin / split count
+    x = DS.c2(n=5000, d=64)
+    ref = H.knn_graph(x, 64, 15, precision="exact", ctx=ctx)
+    for margin in [0, 5, 30, 60]:
+        for nsplit in [1, 3, 5]:
+            job = H.Job(ctx, [5000], [64], [15], ["euclidean"], margin=margin, nsplit=nsplit)
+            job.upload(0, x)
+            job.run()
+            job.sync()
+            r = job.download(0)
+            job.close()
+            assert np.array_equal(r.knn.indices, ref.knn.indices), (margin, nsplit)
+            assert np.array_equal(r.graph.vals.view(np.uint32), ref.graph.vals.view(np.uint32))

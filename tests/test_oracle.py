@@ -1,0 +1,237 @@
+"""Pins the CPU oracle (oracle/hm_oracle.c) before it is trusted as the
+checker of the GPU path.
+
+Each oracle function is checked against something that would catch a
+plausible mistake (dropped term, wrong sign/index, transposed operand):
+  * the reference's golden fixtures (tests/golden/stage2_*.json),
+  * the known answers of the reference's unit tests,
+  * an independent numpy restatement / brute force on tiny inputs,
+  * mathematical invariants (expf error bound vs. correctly rounded exp,
+    the calibration equation, t-conorm symmetry).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+FIXTURES = ["blobs_euclidean", "cosine", "duplicates_euclidean"]
+
+
+def load_fixture(name):
+    with open(os.path.join(GOLD, f"stage2_{name}.json")) as f:
+        fx = json.load(f)
+    data = np.array(fx["data"], np.float64).astype(np.float32)
+    return fx, data
+
+
+def close(a, e):
+    # src/fixture_parity.rs:18-24
+    a = np.asarray(a, np.float64)
+    e = np.asarray(e, np.float64)
+    return np.all(np.abs(a - e) <= 1e-5 * (1.0 + np.abs(e)))
+
+
+# ---------------------------------------------------------------- fixtures
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixture_knn_bit_exact(name):
+    fx, data = load_fixture(name)
+    knn = O.exact_knn(data, fx["n_features"], fx["n_neighbors"], fx["metric"])
+    assert np.array_equal(knn.indices, np.array(fx["knn_indices"], np.uint32))
+    # the fixture distances are float32 values written as decimals
+    assert np.array_equal(knn.dists, np.array(fx["knn_dists"], np.float64).astype(np.float32))
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixture_calibration(name):
+    fx, data = load_fixture(name)
+    k = fx["n_neighbors"]
+    knn = O.exact_knn(data, fx["n_features"], k, fx["metric"])
+    cal = O.smooth_knn(knn.dists, k)
+    assert np.array_equal(cal.rhos, np.array(fx["rhos"], np.float64).astype(np.float32))
+    assert close(cal.sigmas, fx["sigmas"])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixture_graph(name):
+    fx, data = load_fixture(name)
+    k = fx["n_neighbors"]
+    knn, cal, g = O.knn_graph(data, fx["n_features"], k, fx["metric"])
+    assert np.array_equal(g.rows, np.array(fx["graph_rows"], np.uint32))
+    assert np.array_equal(g.cols, np.array(fx["graph_cols"], np.uint32))
+    assert close(g.vals, fx["graph_vals"])
+
+
+# ---------------------------------------------------------------- known answers
+def test_metric_known_values():
+    # src/metric.rs:91-101
+    assert O.euclidean([0.0, 0.0], [3.0, 4.0]) == 5.0
+    assert O.euclidean([1.0, 2.0, 3.0], [1.0, 2.0, 3.0]) == 0.0
+    assert O.cosine([1.0, 0.0], [0.0, 1.0]) == 1.0
+    assert O.cosine([1.0, 0.0], [2.0, 0.0]) == 0.0
+    assert O.cosine([1.0, 0.0], [-3.0, 0.0]) == 2.0
+    assert O.cosine([0.0, 0.0], [1.0, 2.0]) == 0.0  # NaN.max(0.0) = 0
+
+
+def test_knn_known_answers():
+    # src/knn.rs:64-104
+    line = np.array([0.0, 1.0, 3.0, 7.0], np.float32)
+    knn = O.exact_knn(line, 1, 3, "euclidean")
+    assert knn.indices[1].tolist() == [1, 0, 2]
+    assert knn.dists[1].tolist() == [0.0, 1.0, 2.0]
+    assert knn.indices[3].tolist() == [3, 2, 1]
+    assert knn.dists[3].tolist() == [0.0, 4.0, 6.0]
+    dup = O.exact_knn(np.array([5.0, 9.0, 5.0], np.float32), 1, 2, "euclidean")
+    assert dup.indices[0].tolist() == [0, 2]
+    assert dup.indices[2].tolist() == [0, 2]
+    assert dup.dists[2].tolist() == [0.0, 0.0]
+    c = O.exact_knn(np.array([0.3, 0.7, 0.31, 0.69], np.float32), 2, 2, "cosine")
+    assert c.dists[0, 0] == 0.0 and c.dists[1, 0] == 0.0
+    assert c.dists[0, 1] > 0.0 and c.dists[1, 1] > 0.0
+    a = O.exact_knn(np.array([1.0, 0.0, 100.0, 1.0, 0.1, 0.1], np.float32), 2, 3, "cosine")
+    assert a.indices[0].tolist() == [0, 1, 2]
+
+
+def test_smooth_knn_known_answers():
+    # src/smooth_knn.rs:103-124: exact float32 values of the reference
+    cal = O.smooth_knn(np.array([0, 1, 3, 0, 1, 2, 0, 2, 3], np.float32), 3)
+    assert cal.rhos.tolist() == [1.0, 1.0, 2.0]
+    assert cal.sigmas.tolist() == [3.72998046875, 1.864990234375, 1.864990234375]
+    z = O.smooth_knn(np.zeros(4, np.float32), 2)
+    assert z.rhos.tolist() == [0.0, 0.0] and z.sigmas.tolist() == [1.0, 1.0]
+
+
+def test_fuzzy_known_answers():
+    # src/fuzzy.rs:111-153
+    knn = O.Knn(3, np.array([0, 1, 2, 1, 0, 2, 2, 1, 0], np.uint32).reshape(3, 3),
+                np.array([0, 1, 3, 0, 1, 2, 0, 2, 3], np.float32).reshape(3, 3))
+    cal = O.SmoothKnn(np.array([3.72998046875, 1.864990234375, 1.864990234375], np.float32),
+                      np.array([1, 1, 2], np.float32))
+    g = O.fuzzy_simplicial_set(knn, cal)
+    w = O.expf(np.float32(-2.0) / np.float32(3.72998046875))
+    fused = np.float32(np.float32(w + w) - np.float32(w * w))
+    assert g.rows.tolist() == [0, 0, 1, 1, 2, 2]
+    assert g.cols.tolist() == [1, 2, 0, 2, 0, 1]
+    assert np.allclose(g.vals, [1.0, fused, 1.0, 1.0, fused, 1.0], atol=1e-6)
+    knn2 = O.Knn(2, np.array([0, 1, 1, 2, 2, 1], np.uint32).reshape(3, 2),
+                 np.array([0, 1, 0, 0.1, 0, 0.1], np.float32).reshape(3, 2))
+    cal2 = O.SmoothKnn(np.ones(3, np.float32), np.array([0.5, 0.1, 0.1], np.float32))
+    g2 = O.fuzzy_simplicial_set(knn2, cal2)
+    w2 = O.expf(-0.5)
+    assert g2.rows.tolist() == [0, 1, 1, 2]
+    assert g2.cols.tolist() == [1, 0, 2, 1]
+    assert g2.vals.tolist() == [w2, w2, 1.0, 1.0]
+
+
+def test_csr_known_answer():
+    # src/sparse.rs:117-124 (path graph 0-1-2)
+    ip = O.csr_indptr(np.array([0, 1, 1, 2], np.uint32), 3)
+    assert ip.tolist() == [0, 1, 3, 4]
+
+
+# ---------------------------------------------------------------- independent restatements
+def np_euclid(a, b):
+    s = 0.0
+    for x, y in zip(a.astype(np.float64), b.astype(np.float64)):
+        t = x - y
+        s = s + t * t
+    return np.float32(np.sqrt(s))
+
+
+def np_cosine(a, b):
+    dot = na = nb = 0.0
+    for x, y in zip(a.astype(np.float64), b.astype(np.float64)):
+        dot += x * y
+        na += x * x
+        nb += y * y
+    with np.errstate(invalid="ignore", divide="ignore"):
+        v = 1.0 - dot / (np.sqrt(na) * np.sqrt(nb))
+    return np.float32(0.0 if np.isnan(v) else max(v, 0.0))
+
+
+@pytest.mark.parametrize("metric", ["euclidean", "cosine"])
+def test_brute_force_tiny(metric):
+    rng = np.random.default_rng(5)
+    n, d, k = 24, 7, 6
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    x[5] = x[2]          # exact duplicate: tie broken by index
+    x[9] = 0.0           # zero vector (cosine: distance 0 to all)
+    x[11] = 3.0 * x[4]   # parallel (cosine distance 0)
+    f = np_euclid if metric == "euclidean" else np_cosine
+    knn = O.exact_knn(x, d, k, metric)
+    for i in range(n):
+        dist = np.array([0.0 if j == i else f(x[i], x[j]) for j in range(n)], np.float32)
+        order = np.lexsort((np.arange(n), dist))[:k]
+        assert knn.indices[i].tolist() == order.tolist()
+        assert np.array_equal(knn.dists[i], dist[order])
+
+
+def test_knn_rows_matches_full():
+    rng = np.random.default_rng(6)
+    x = rng.normal(size=(300, 9)).astype(np.float32)
+    full = O.exact_knn(x, 9, 8, "euclidean")
+    rows = np.array([0, 17, 299, 150], np.uint32)
+    sub = O.exact_knn_rows(x, 9, 8, "euclidean", rows, nthreads=2)
+    assert np.array_equal(sub.indices, full.indices[rows])
+    assert np.array_equal(sub.dists, full.dists[rows])
+
+
+def test_expf_error_bound_and_specials():
+    # The libm/FreeBSD method is documented < 1 ulp; a wrong coefficient,
+    # constant or reduction step breaks this bound by orders of magnitude.
+    rng = np.random.default_rng(7)
+    bits = rng.integers(0x80000000, 0xC2D00000, size=20000, dtype=np.uint64).astype(np.uint32)
+    xs = bits.view(np.float32)
+    xs = np.concatenate([xs, np.linspace(-103.9, 0, 2001, dtype=np.float32)])
+    L = O.lib()
+    worst = 0.0
+    for x in xs:
+        y = np.float32(L.hmo_expf(float(x)))
+        ref = np.exp(np.float64(x))
+        if ref < 1.2e-38:  # subnormal results: absolute check
+            assert abs(float(y) - ref) <= 1.5e-45
+            continue
+        ulp = np.spacing(np.float32(ref))
+        worst = max(worst, abs(float(y) - ref) / float(ulp))
+    assert worst < 1.0
+    assert O.expf(0.0) == 1.0
+    assert O.expf(-200.0) == 0.0
+    # src/fmath.rs:155-161
+    for x in [0.0, -1.0, 1.0, -7.5]:
+        assert abs(float(O.expf(x)) - np.exp(x)) < 1e-6
+
+
+def test_smooth_knn_solves_calibration_equation():
+    rng = np.random.default_rng(8)
+    x = rng.normal(size=(400, 6)).astype(np.float32)
+    k = 10
+    knn = O.exact_knn(x, 6, k, "euclidean")
+    cal = O.smooth_knn(knn.dists, k)
+    c64 = O.smooth_knn_f64(knn.dists, k)
+    target = np.log2(k)
+    for i in range(400):
+        dd = knn.dists[i, 1:].astype(np.float64) - c64.rhos[i]
+        ps = np.where(dd > 0, np.exp(-dd / c64.sigmas[i]), 1.0).sum()
+        assert abs(ps - target) < 1e-9
+    # the reference's float32 search stops within 1e-5 of the target
+    rel = np.abs(cal.sigmas - c64.sigmas) / c64.sigmas
+    assert rel.max() < 1e-3
+    assert np.array_equal(cal.rhos.astype(np.float64), c64.rhos)
+
+
+def test_fuzzy_invariants():
+    rng = np.random.default_rng(9)
+    x = rng.normal(size=(500, 5)).astype(np.float32)
+    knn, cal, g = O.knn_graph(x, 5, 12, "euclidean")
+    key = g.rows.astype(np.int64) * 500 + g.cols
+    assert np.all(np.diff(key) > 0)                 # sorted, no duplicates
+    assert np.all(g.rows != g.cols)                 # no self edges
+    assert np.all((g.vals > 0) & (g.vals <= 1))
+    m = {(int(r), int(c)): v for r, c, v in zip(g.rows, g.cols, g.vals)}
+    for (r, c), v in m.items():
+        assert m[(c, r)] == v                       # bitwise symmetric
+    ip = O.csr_indptr(g.rows, 500)
+    assert ip[-1] == g.rows.size and np.all(np.diff(ip.astype(np.int64)) >= 0)
