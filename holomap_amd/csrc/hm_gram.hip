@@ -20,14 +20,15 @@
 // Selection. Lanes l and l+32 hold the same query but disjoint columns, so
 // each lane keeps its own list: an LDS buffer of QL (key, j) entries and a
 // threshold tau. A value is appended (a masked store, no cross-lane work)
-// only if its key is below tau. Before each 32-column subtile (at most 16
+// only if its key is below min(tau, tau of the partner lane). Before each 32-column subtile (at most 16
 // values per lane) a lane with fewer than 16 free slots is compacted: a
 // bisection on the key value finds a cut p keeping between ceil(C/2) and
 // QL-16 entries, entries above p are dropped and tau := the largest kept
 // key. Invariant: every column of the lane's half that is not in its buffer
-// has key > tau. A row's two lists together hold at least C keys and every
-// column missing from both has key > min(tau_0, tau_1): the candidate set
-// and the threshold the certification bound uses. No atomics anywhere.
+// has key > min(tau_0, tau_1) (both only decrease). A row's two lists hold
+// at least C keys between them, every column missing from both has key >
+// min(tau_0, tau_1): the candidate set and the threshold the certification
+// bound uses. No atomics anywhere.
 #include "hm_internal.hpp"
 #include "hm_select.hpp"
 
@@ -48,11 +49,11 @@ __device__ __forceinline__ float bf16_f32(uint16_t h) { return __uint_as_float((
 // MODE 0: bf16 operand (op16); MODE 1: f32 operand (op32, fp32 MFMA);
 // MODE 2: bf16x3 -- op32 split on load into hi + lo bf16, three MFMAs per
 //         step (hi.hi + lo.hi + hi.lo).
-// LDS budget (160 KB): the lane lists take 256*QL*8 bytes (96 KB for QL=48,
-// 128 KB for QL=64), so the QL=64 variants stage half-size feature chunks.
+// LDS budget (160 KB): the lane lists take 256*QL*6 bytes (f32 key + u16
+// column offset; 96 KB for QL=64) and the two operand tiles ~36-40 KB.
 template <int MODE, int QL>
 struct GramCfg {
-  static constexpr int KC = (MODE == 0) ? (QL <= 48 ? 64 : 32) : (QL <= 48 ? 32 : 16);
+  static constexpr int KC = (MODE == 0) ? 64 : 32;
   static constexpr int GES = (MODE == 0) ? 2 : 4;          // global element bytes
   static constexpr int ES = (MODE == 1) ? 4 : 2;           // LDS element bytes
   static constexpr int LD = KC + (MODE == 1 ? 4 : 8);      // LDS row stride (elements)
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(256, 1) void gram_topc_kernel(
   __shared__ __attribute__((aligned(16))) uint8_t c_s[NT][128 * LD * Cfg::ES];
   __shared__ float colb_s[128];
   __shared__ float lkf[QL][256];
-  __shared__ uint32_t lkj[QL][256];
+  __shared__ uint16_t lkj[QL][256];  // column - cbeg (split span <= 65536)
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q = lane & 31, half = lane >> 5;
@@ -143,7 +144,12 @@ __global__ __launch_bounds__(256, 1) void gram_topc_kernel(
     const DsDesc& D = ds_arr[W.ds];
     const int64_t n = D.n;
     const int kp = D.kp;
-    const int keep_lo = (D.C + 1) / 2, keep_hi = D.cout;  // cout == QL - 16
+    // list sizes: a compaction triggered by a full lane list (> QL - 16)
+    // takes every lane holding more than keep_lo + win down to
+    // [keep_lo, keep_lo + win] (so lanes stay in step and a compaction buys
+    // many free slots); the final trim keeps at most cout = QL - 16
+    const int keep_lo = (D.C + 1) / 2, keep_out = D.cout;
+    const int win = (keep_out - keep_lo) / 2;
     const float scale = D.metric == 1 ? -1.0f : -2.0f;
     // column range of this split (aligned to 128 columns)
     const int64_t span = ((n + (int64_t)D.nsplit * 128 - 1) / ((int64_t)D.nsplit * 128)) * 128;
@@ -222,8 +228,15 @@ __global__ __launch_bounds__(256, 1) void gram_topc_kernel(
         }
       }
 
-      // ---- epilogue: keys, fast threshold filter, ballot insertion ----
-      const float tauf = tau == ~0ull ? INFINITY : unordered_f32((uint32_t)(tau >> 32));
+      // ---- epilogue: keys, fast threshold filter, list appends ----
+      // The two lanes of a row filter with the smaller of their thresholds:
+      // certification uses that minimum anyway (see the file header).
+      uint64_t teff = tau;
+      {
+        const uint64_t tp = shfl_xor_u64(tau, 32);
+        teff = tp < teff ? tp : teff;
+      }
+      const float tauf = teff == ~0ull ? INFINITY : unordered_f32((uint32_t)(teff >> 32));
       bool anyhit = false;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
@@ -237,21 +250,31 @@ __global__ __launch_bounds__(256, 1) void gram_topc_kernel(
       }
       if (__ballot(anyhit) != 0ull) {
         // Each 32-column subtile adds at most 16 values to a lane's list;
-        // lanes with fewer than 16 free slots are compacted first.
+        // before it, a lane list with fewer than 16 free slots triggers a
+        // compaction of every lane list holding more than keep_lo + win.
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          if (__ballot(cnt > QL - 16) != 0ull)
-            lane_compact<QL>(&lkf[0][tid], &lkj[0][tid], 256, cnt, tau, keep_lo, keep_hi);
+          if (__ballot(cnt > QL - 16) != 0ull) {
+            lane_compact<QL>(&lkf[0][tid], &lkj[0][tid], 256, cnt, tau, keep_lo + win, keep_lo,
+                             keep_lo + win);
+            const uint64_t tp = shfl_xor_u64(tau, 32);
+            teff = tp < tau ? tp : tau;
+          }
 #pragma unroll
           for (int a = 0; a < 16; ++a) {
             const int jl = t * 32 + 8 * (a >> 2) + 4 * half + (a & 3);
             const int64_t j = cb + jl;
             const float key = acc[t][a];
-            const uint64_t k64 = ((uint64_t)ordered_u32(key) << 32) | (uint64_t)(uint32_t)j;
-            if (j < cend && k64 < tau) {
-              lkf[cnt][tid] = key;
-              lkj[cnt][tid] = (uint32_t)j;
-              ++cnt;
+            // list keys carry the column offset within the split; teff is
+            // in the same space (both lists of a row share cbeg)
+            const uint64_t k64 = ((uint64_t)ordered_u32(key) << 32) | (uint64_t)(uint32_t)(j - cbeg);
+            const bool hit = j < cend && k64 < teff;
+            if (__ballot(hit) != 0ull) {
+              if (hit) {
+                lkf[cnt][tid] = key;
+                lkj[cnt][tid] = (uint16_t)(j - cbeg);
+                ++cnt;
+              }
             }
           }
         }
@@ -259,8 +282,8 @@ __global__ __launch_bounds__(256, 1) void gram_topc_kernel(
     }
 
     // ---- final: trim each lane list to <= cout keys, write it with tau ----
-    if (__ballot(cnt > keep_hi) != 0ull)
-      lane_compact<QL>(&lkf[0][tid], &lkj[0][tid], 256, cnt, tau, keep_lo, keep_hi);
+    if (__ballot(cnt > keep_out) != 0ull)
+      lane_compact<QL>(&lkf[0][tid], &lkj[0][tid], 256, cnt, tau, keep_out, keep_lo, keep_out);
     const int slot = wave * 32 + q;
     if (slot < W.qcnt) {
       const int p = W.qbeg + slot;
@@ -269,10 +292,12 @@ __global__ __launch_bounds__(256, 1) void gram_topc_kernel(
       uint64_t* dst = D.cand + ((size_t)row * D.nlist + list) * D.cout;
       for (int e = 0; e < D.cout; ++e) {
         uint64_t v = ~0ull;
-        if (e < cnt) v = ((uint64_t)ordered_u32(lkf[e][tid]) << 32) | lkj[e][tid];
+        if (e < cnt)
+          v = ((uint64_t)ordered_u32(lkf[e][tid]) << 32) | (uint32_t)(lkj[e][tid] + cbeg);
         dst[e] = v;
       }
-      D.ctau[(size_t)row * D.nlist + list] = tau;
+      // back to absolute column ids (order preserved: same offset)
+      D.ctau[(size_t)row * D.nlist + list] = tau == ~0ull ? tau : tau + (uint64_t)cbeg;
     }
     cnt = 0;
   }
@@ -281,11 +306,8 @@ __global__ __launch_bounds__(256, 1) void gram_topc_kernel(
 #define HM_INST(MODE, QL)                                                          \
   template __global__ void gram_topc_kernel<MODE, QL>(const DsDesc*, const WorkItem*,    \
                                                       const int*, int, const uint32_t*);
-HM_INST(0, 48)
 HM_INST(0, 64)
-HM_INST(1, 48)
 HM_INST(1, 64)
-HM_INST(2, 48)
 HM_INST(2, 64)
 
 }  // namespace hm

@@ -147,7 +147,7 @@ hm_status fail(hm_ctx* ctx, hm_status st, const std::string& msg) {
   } while (0)
 
 // per-lane list capacity of the sweep: keeps ceil(C/2) .. QL-16 keys
-int pick_ql(int cmax) { return cmax <= 64 ? 48 : 64; }
+int pick_ql(int) { return 64; }
 
 int gram_mode(int precision) {
   switch (precision) {
@@ -222,6 +222,8 @@ Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_
     } else {
       h.nsplit = std::min(smax, choose_nsplit(h.n, h.C, cus, rb_total - cdiv(h.n, 128)));
     }
+    // the sweep stores column offsets within a split as u16
+    h.nsplit = std::max<int>(h.nsplit, (int)cdiv(h.n, 65536));
     h.nlist = 2 * h.nsplit;
     h.cout = cout;
     P.sum_cand += h.n * h.nlist * h.cout;
@@ -322,11 +324,10 @@ hm_status launch_sweep(hm_job* J) {
   hipLaunchKernelGGL((gram_topc_kernel<M, Q>), dim3(grid), dim3(256), 0, J->stream, J->d_desc, \
                      (const WorkItem*)J->items_l1, (const int*)nullptr, J->n_items_l1,       \
                      (const uint32_t*)nullptr)
-  const int q = J->ql;
   switch (J->mode) {
-    case 0: if (q == 48) HM_GRAM(0, 48); else HM_GRAM(0, 64); break;
-    case 1: if (q == 48) HM_GRAM(1, 48); else HM_GRAM(1, 64); break;
-    case 2: if (q == 48) HM_GRAM(2, 48); else HM_GRAM(2, 64); break;
+    case 0: HM_GRAM(0, 64); break;
+    case 1: HM_GRAM(1, 64); break;
+    case 2: HM_GRAM(2, 64); break;
     default: break;
   }
 #undef HM_GRAM
@@ -535,6 +536,8 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
         return fail(ctx, HM_EUNSUPPORTED, "n_neighbors > 64 is not supported on the GPU path");
     }
     if (n[i] >= (1ull << 31)) return fail(ctx, HM_EUNSUPPORTED, "n >= 2^31");
+    if (knn_stage && n[i] > 5ull * 65536ull)
+      return fail(ctx, HM_EUNSUPPORTED, "n > 327680 rows per dataset on the GPU path");
     if (metric && metric[i] != HM_EUCLIDEAN && metric[i] != HM_COSINE)
       return fail(ctx, HM_EINVAL, "metric must be \"euclidean\" or \"cosine\"");
   }

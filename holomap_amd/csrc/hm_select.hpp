@@ -29,18 +29,20 @@ HM_HD uint32_t sel_ordered(float x) {
 
 HM_HD uint64_t sel_key(float v, uint32_t j) { return ((uint64_t)sel_ordered(v) << 32) | j; }
 
-// A lane's list: entry e is (kf[e*stride], kj[e*stride]). Lanes with
-// cnt > keep_hi are compacted to a count in [keep_lo, keep_hi]; the kept
-// entries are exactly the smallest ones in the u64 key order
+// A lane's list: entry e is (kf[e*stride], kj[e*stride]); J is the column
+// id type (the sweep stores u16 offsets within its column split). Lanes with
+// cnt > act are compacted to a count in [keep_lo, keep_hi] (act >= keep_hi);
+// the kept entries are exactly the smallest ones in the u64 key order
 // (ordered(value) << 32 | column), and tau becomes the largest kept key.
 // Every dropped entry is larger than the new tau.
-template <int QL>
-HM_HD void lane_compact(float* kf, uint32_t* kj, int stride, int& cnt, uint64_t& tau, int keep_lo,
-                        int keep_hi) {
-  const bool active = cnt > keep_hi;
+template <int QL, typename J>
+HM_HD void lane_compact(float* kf, J* kj, int stride, int& cnt, uint64_t& tau, int act,
+                        int keep_lo, int keep_hi) {
+  const bool active = cnt > act;
   // f(v) = #{entries with value <= v}
   float mx = -INFINITY, mn = INFINITY;
   int ninf = 0;
+#pragma unroll 16
   for (int e = 0; e < QL; ++e) {
     if (e < cnt) {
       const float v = kf[e * stride];
@@ -64,6 +66,7 @@ HM_HD void lane_compact(float* kf, uint32_t* kj, int stride, int& cnt, uint64_t&
     if (!HM_ANY(!done)) break;
     const float mid = lo * 0.5f + hi * 0.5f;
     int c = 0;
+#pragma unroll 16
     for (int e = 0; e < QL; ++e) c += (e < cnt && kf[e * stride] <= mid) ? 1 : 0;
     if (!done) {
       if (!(mid > lo && mid < hi)) {
@@ -85,6 +88,7 @@ HM_HD void lane_compact(float* kf, uint32_t* kj, int stride, int& cnt, uint64_t&
   // ties: keep every value < tie and the smallest columns among value == tie
   if (HM_ANY(active && tiep)) {
     int c_lt = 0;
+#pragma unroll 16
     for (int e = 0; e < QL; ++e) c_lt += (e < cnt && kf[e * stride] < tie) ? 1 : 0;
     const int need = keep_lo - c_lt;
     uint32_t jl = 0, jh = 0xffffffffu;
@@ -94,8 +98,9 @@ HM_HD void lane_compact(float* kf, uint32_t* kj, int stride, int& cnt, uint64_t&
       if (go) {
         const uint32_t jm = jl + (jh - jl) / 2;
         int c = 0;
+#pragma unroll 16
         for (int e = 0; e < QL; ++e)
-          c += (e < cnt && kf[e * stride] == tie && kj[e * stride] <= jm) ? 1 : 0;
+          c += (e < cnt && kf[e * stride] == tie && (uint32_t)kj[e * stride] <= jm) ? 1 : 0;
         if (c >= need) jh = jm; else jl = jm + 1;
       }
     }
@@ -111,8 +116,9 @@ HM_HD void lane_compact(float* kf, uint32_t* kj, int stride, int& cnt, uint64_t&
       if (go) {
         const uint64_t m64 = l64 + (h64 - l64) / 2;
         int c = 0;
+#pragma unroll 16
         for (int e = 0; e < QL; ++e)
-          c += (e < cnt && sel_key(kf[e * stride], kj[e * stride]) <= m64) ? 1 : 0;
+          c += (e < cnt && sel_key(kf[e * stride], (uint32_t)kj[e * stride]) <= m64) ? 1 : 0;
         if (c >= keep_lo) h64 = m64; else l64 = m64 + 1;
       }
     }
@@ -124,8 +130,8 @@ HM_HD void lane_compact(float* kf, uint32_t* kj, int stride, int& cnt, uint64_t&
     for (int e = 0; e < QL; ++e) {
       if (e < cnt) {
         const float v = kf[e * stride];
-        const uint32_t j = kj[e * stride];
-        const uint64_t key = sel_key(v, j);
+        const J j = kj[e * stride];
+        const uint64_t key = sel_key(v, (uint32_t)j);
         if (key <= T) {
           kf[w * stride] = v;
           kj[w * stride] = j;

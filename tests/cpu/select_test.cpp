@@ -19,11 +19,11 @@ static int fails = 0;
     }                                         \
   } while (0)
 
-template <int QL>
+template <int QL, typename J = uint32_t>
 void check_compact(std::mt19937_64& rng, int mode, int keep_lo, int keep_hi) {
   const int stride = 3;  // any stride: the kernel uses 256 (lane-minor)
   std::vector<float> kf(QL * stride, 0.f);
-  std::vector<uint32_t> kj(QL * stride, 0);
+  std::vector<J> kj(QL * stride, 0);
   std::uniform_int_distribution<int> cd(keep_hi + 1, QL);
   const int cnt0 = cd(rng);
   std::normal_distribution<float> nd(0.f, 1.f);
@@ -39,23 +39,24 @@ void check_compact(std::mt19937_64& rng, int mode, int keep_lo, int keep_hi) {
       case 5: v = nd(rng) * 1e30f; break;                         // huge magnitudes
       default: v = std::ldexp(1.0f, -140) * (float)(rng() % 5); break;  // subnormals
     }
-    uint32_t j = (uint32_t)(rng() % 100000);
+    const uint32_t jmod = sizeof(J) == 2 ? 65536u : 100000u;
+    uint32_t j = (uint32_t)(rng() % jmod);
     // keep columns distinct, as in the sweep
     while (std::find_if(keys.begin(), keys.end(), [&](uint64_t k) { return (uint32_t)k == j; }) !=
            keys.end())
-      j = (uint32_t)(rng() % 100000);
+      j = (uint32_t)(rng() % jmod);
     kf[e * stride] = v;
-    kj[e * stride] = j;
+    kj[e * stride] = (J)j;
     keys.push_back(sel_key(v, j));
   }
   int cnt = cnt0;
   uint64_t tau = ~0ull;
-  lane_compact<QL>(kf.data(), kj.data(), stride, cnt, tau, keep_lo, keep_hi);
+  lane_compact<QL>(kf.data(), kj.data(), stride, cnt, tau, keep_hi, keep_lo, keep_hi);
   std::sort(keys.begin(), keys.end());
   CHECK(cnt >= keep_lo && cnt <= keep_hi, "mode %d: cnt %d outside [%d,%d]\n", mode, cnt, keep_lo,
         keep_hi);
   std::vector<uint64_t> kept;
-  for (int e = 0; e < cnt; ++e) kept.push_back(sel_key(kf[e * stride], kj[e * stride]));
+  for (int e = 0; e < cnt; ++e) kept.push_back(sel_key(kf[e * stride], (uint32_t)kj[e * stride]));
   std::sort(kept.begin(), kept.end());
   bool prefix = true;
   for (int e = 0; e < cnt; ++e) prefix &= kept[e] == keys[e];
@@ -79,7 +80,7 @@ void check_stream(std::mt19937_64& rng, int mode, int m, int keep_lo) {
   int cnt = 0;
   uint64_t tau = ~0ull;
   for (int g = 0; g < m; g += 16) {
-    if (cnt > QL - 16) lane_compact<QL>(f, jj, 256, cnt, tau, keep_lo, keep_hi);
+    if (cnt > QL - 16) lane_compact<QL>(f, jj, 256, cnt, tau, keep_lo + 4, keep_lo, keep_lo + 4);
     for (int t = 0; t < 16 && g + t < m; ++t) {
       const uint32_t j = (uint32_t)(g + t);
       float v = mode == 0 ? nd(rng) : (mode == 1 ? (float)(rng() % 7) : -(float)j * 0.001f);
@@ -93,7 +94,7 @@ void check_stream(std::mt19937_64& rng, int mode, int m, int keep_lo) {
       }
     }
   }
-  if (cnt > keep_hi) lane_compact<QL>(f, jj, 256, cnt, tau, keep_lo, keep_hi);
+  if (cnt > keep_hi) lane_compact<QL>(f, jj, 256, cnt, tau, keep_hi, keep_lo, keep_hi);
   std::sort(all.begin(), all.end());
   std::vector<uint64_t> kept;
   for (int e = 0; e < cnt; ++e) kept.push_back(sel_key(f[e * 256], jj[e * 256]));
@@ -116,6 +117,8 @@ int main() {
     check_compact<48>(rng, mode, 32, 32);
     check_compact<64>(rng, mode, 40, 48);
     check_compact<64>(rng, mode, 8, 48);
+    check_compact<64, uint16_t>(rng, mode, 23, 35);
+    check_compact<64, uint16_t>(rng, mode, 48, 48);
   }
   for (int rep = 0; rep < 60; ++rep) {
     check_stream<48>(rng, rep % 3, 50 + rep * 97, 23);
