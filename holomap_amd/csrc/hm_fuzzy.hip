@@ -4,15 +4,22 @@
 // de-duplicated union of A and A^T coordinates and fuses each pair with the
 // probabilistic t-conorm w = (v + vt) - v*vt in f32. Here the same COO comes
 // from two stable radix sorts (hm_sort.hip):
-//   sort 1 by global column -> for every row c, its in-edges (sources
-//            ascending, since edges enter in source order);
-//   sort 2 of that sequence by source row -> every row's out-edges with
-//            columns ascending;
-// then one thread per row merges its out-list and in-list by column id,
-// counting (pass 0) or writing (pass 1, at the exclusive-scan offset) the
-// fused entries. The merge visits exactly the union coordinates in (row,
-// col) order and w is evaluated with the reference's operand order, so the
-// output is the reference's COO entry for entry.
+//   sort 1 by global column -> every row c's in-edges, sources ascending;
+//   sort 2 of that sequence by source row -> every row's out-edges, columns
+//            ascending (list A_r);
+// then, load-balanced (one thread per element, no per-row loops, so rows of
+// very high in-degree cost nothing extra):
+//   * each out-edge r->c looks up its reverse c->r in A_c by binary search
+//     (vt, or 0 if absent);
+//   * the in-edges i->c without a reverse form list B_c (sources ascending);
+//   * row r of the union is A_r merged with B_r (disjoint column sets), and
+//     every element finds its output slot as its rank in its own list plus
+//     the number of smaller columns in the other list ("merge path").
+// For an entry (r, c): v = A[r][c], vt = A[c][r] (0 if absent), w evaluated
+// with the reference's operand order. With v, vt in [0, 1] and v + vt > 0,
+// w >= max(v, vt) - min(v, vt) and w == 0 would need v == vt == v*vt, so
+// w > 0 always and the reference's drop of zero entries (src/fuzzy.rs:73)
+// never applies.
 #include "hm_internal.hpp"
 
 namespace hm {
@@ -26,57 +33,128 @@ __global__ void gather_key2(const uint32_t* __restrict__ pay1s, const uint32_t* 
   key2[p] = key1[e] != sentinel ? srcrow[e] : sentinel;
 }
 
-// pass 0: cnt[r] = number of union entries of row r
-// pass 1: write rows/cols (dataset-local ids) and vals at indptr_g[r]
-__global__ void union_kernel(int pass, const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
-                             const uint32_t* __restrict__ inptr, const uint32_t* __restrict__ pay1s,
-                             const uint32_t* __restrict__ outptr, const uint32_t* __restrict__ pay2s,
-                             const uint32_t* __restrict__ key1, const uint32_t* __restrict__ srcrow,
-                             const float* __restrict__ wval, uint64_t* __restrict__ cnt,
-                             const uint64_t* __restrict__ indptr_g, uint32_t* __restrict__ rows,
-                             uint32_t* __restrict__ cols, float* __restrict__ vals) {
+__device__ __forceinline__ uint32_t lb_u32(const uint32_t* __restrict__ a, uint32_t lo, uint32_t hi,
+                                           uint32_t v) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// A lists: column and weight of every sort-2 position (valid prefix only).
+__global__ void union_gather_a(const uint32_t* __restrict__ p2s, const uint32_t* __restrict__ key1,
+                               const float* __restrict__ wval, const uint32_t* __restrict__ outptr,
+                               int64_t total_rows, uint32_t* __restrict__ ocol,
+                               float* __restrict__ oval) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= (int64_t)outptr[total_rows]) return;
+  const uint32_t e = p2s[a];
+  ocol[a] = key1[e];
+  oval[a] = wval[e];
+}
+
+// Reverse lookup of every out-edge r->c in A_c; marks mutual edges by id.
+__global__ void union_mutual(const uint32_t* __restrict__ k2s, const uint32_t* __restrict__ p2s,
+                             const uint32_t* __restrict__ ocol, const float* __restrict__ oval,
+                             const uint32_t* __restrict__ outptr, int64_t total_rows,
+                             float* __restrict__ orev, uint32_t* __restrict__ mflag) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= (int64_t)outptr[total_rows]) return;
+  const uint32_t r = k2s[a], c = ocol[a];
+  const uint32_t lo = outptr[c], hi = outptr[c + 1];
+  const uint32_t f = lb_u32(ocol, lo, hi, r);
+  const bool found = f < hi && ocol[f] == r;
+  orev[a] = found ? oval[f] : 0.0f;
+  mflag[p2s[a]] = found ? 1u : 0u;
+}
+
+// nm[p] = 1 for a valid in-edge (sort-1 position p) without a reverse edge.
+__global__ void union_nonmutual(const uint32_t* __restrict__ p1s, const uint32_t* __restrict__ mflag,
+                                const uint32_t* __restrict__ inptr, int64_t total_rows,
+                                int64_t total_edges, uint32_t* __restrict__ nm) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > total_edges) return;
+  nm[p] = (p < (int64_t)inptr[total_rows] && !mflag[p1s[p]]) ? 1u : 0u;
+}
+
+// B lists (order kept) and their row pointers.
+__global__ void union_gather_b(const uint32_t* __restrict__ k1s, const uint32_t* __restrict__ p1s,
+                               const uint32_t* __restrict__ srcrow, const float* __restrict__ wval,
+                               const uint32_t* __restrict__ nm, const uint32_t* __restrict__ bpos,
+                               const uint32_t* __restrict__ inptr, int64_t total_rows,
+                               uint32_t* __restrict__ bsrc, uint32_t* __restrict__ brow,
+                               float* __restrict__ bval) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (int64_t)inptr[total_rows] || !nm[p]) return;
+  const uint32_t b = bpos[p], e = p1s[p];
+  bsrc[b] = srcrow[e];
+  brow[b] = k1s[p];
+  bval[b] = wval[e];
+}
+
+// bptr[c] = first B slot of row c; cnt[r] = |A_r| + |B_r|.
+__global__ void union_rowcount(const uint32_t* __restrict__ inptr, const uint32_t* __restrict__ outptr,
+                               const uint32_t* __restrict__ bpos, int64_t total_rows,
+                               uint32_t* __restrict__ bptr, uint64_t* __restrict__ cnt) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= total_rows) return;
-  uint32_t a = outptr[r], ae = outptr[r + 1];
-  uint32_t b = inptr[r], be = inptr[r + 1];
-  int64_t roff = 0;
-  uint64_t o = 0;
-  if (pass == 1) {
-    int lo = 0, hi = nds - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (ds[mid].row_off <= r) lo = mid; else hi = mid - 1;
-    }
-    roff = ds[lo].row_off;
-    o = indptr_g[r];
+  if (r > total_rows) return;
+  const uint32_t b0 = bpos[inptr[r]];
+  bptr[r] = b0;
+  if (r < total_rows) {
+    const uint32_t b1 = bpos[inptr[r + 1]];
+    cnt[r] = (uint64_t)(outptr[r + 1] - outptr[r]) + (uint64_t)(b1 - b0);
   }
-  uint64_t c_out = 0;
-  const uint32_t INF = 0xffffffffu;
-  while (a < ae || b < be) {
-    const uint32_t ea = a < ae ? pay2s[a] : 0u;
-    const uint32_t eb = b < be ? pay1s[b] : 0u;
-    const uint32_t ca = a < ae ? key1[ea] : INF;
-    const uint32_t cb = b < be ? srcrow[eb] : INF;
-    uint32_t c;
-    float v = 0.0f, vt = 0.0f;
-    if (ca < cb) {
-      c = ca; v = wval[ea]; ++a;
-    } else if (cb < ca) {
-      c = cb; vt = wval[eb]; ++b;
-    } else {
-      c = ca; v = wval[ea]; vt = wval[eb]; ++a; ++b;
-    }
-    const float w = (v + vt) - v * vt;  // f32, the reference's operand order
-    if (w != 0.0f) {
-      if (pass == 1) {
-        rows[o + c_out] = (uint32_t)(r - roff);
-        cols[o + c_out] = (uint32_t)((int64_t)c - roff);
-        vals[o + c_out] = w;
-      }
-      ++c_out;
-    }
+}
+
+__device__ __forceinline__ int64_t row_offset_of(const DsDesc* __restrict__ ds, int nds, int64_t r) {
+  int lo = 0, hi = nds - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ds[mid].row_off <= r) lo = mid; else hi = mid - 1;
   }
-  if (pass == 0) cnt[r] = c_out;
+  return ds[lo].row_off;
+}
+
+// A elements: entry (r, c) with v = A[r][c], vt = A[c][r].
+__global__ void union_scatter_a(const DsDesc* __restrict__ ds, int nds,
+                                const uint32_t* __restrict__ k2s, const uint32_t* __restrict__ ocol,
+                                const float* __restrict__ oval, const float* __restrict__ orev,
+                                const uint32_t* __restrict__ outptr, const uint32_t* __restrict__ bsrc,
+                                const uint32_t* __restrict__ bptr,
+                                const uint64_t* __restrict__ indptr_g, int64_t total_rows,
+                                uint32_t* __restrict__ rows, uint32_t* __restrict__ cols,
+                                float* __restrict__ vals) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= (int64_t)outptr[total_rows]) return;
+  const uint32_t r = k2s[a], c = ocol[a];
+  const uint32_t nb = lb_u32(bsrc, bptr[r], bptr[r + 1], c) - bptr[r];
+  const uint64_t pos = indptr_g[r] + (uint64_t)(a - outptr[r]) + nb;
+  const float v = oval[a], vt = orev[a];
+  const int64_t roff = row_offset_of(ds, nds, r);
+  rows[pos] = (uint32_t)(r - roff);
+  cols[pos] = (uint32_t)((int64_t)c - roff);
+  vals[pos] = (v + vt) - v * vt;  // f32, the reference's operand order
+}
+
+// B elements: entry (c, i) for an edge i->c without reverse: v = 0, vt = w.
+__global__ void union_scatter_b(const DsDesc* __restrict__ ds, int nds,
+                                const uint32_t* __restrict__ brow, const uint32_t* __restrict__ bsrc,
+                                const float* __restrict__ bval, const uint32_t* __restrict__ bptr,
+                                const uint32_t* __restrict__ ocol, const uint32_t* __restrict__ outptr,
+                                const uint64_t* __restrict__ indptr_g, int64_t total_rows,
+                                uint32_t* __restrict__ rows, uint32_t* __restrict__ cols,
+                                float* __restrict__ vals) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= (int64_t)bptr[total_rows]) return;
+  const uint32_t c = brow[b], i = bsrc[b];
+  const uint32_t na = lb_u32(ocol, outptr[c], outptr[c + 1], i) - outptr[c];
+  const uint64_t pos = indptr_g[c] + (uint64_t)(b - bptr[c]) + na;
+  const float v = 0.0f, vt = bval[b];
+  const int64_t roff = row_offset_of(ds, nds, c);
+  rows[pos] = (uint32_t)(c - roff);
+  cols[pos] = (uint32_t)((int64_t)i - roff);
+  vals[pos] = (v + vt) - v * vt;
 }
 
 // Per-dataset CSR row pointer (dataset-local) from the global offsets.

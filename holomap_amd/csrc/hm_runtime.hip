@@ -49,9 +49,23 @@ __global__ void compact_kernel(const uint32_t*, const uint32_t*, int64_t, uint32
 __global__ void fill_u32(uint32_t*, int64_t, uint32_t);
 __global__ void gather_key2(const uint32_t*, const uint32_t*, const uint32_t*, int64_t, uint32_t,
                             uint32_t*);
-__global__ void union_kernel(int, const DsDesc*, int, int64_t, const uint32_t*, const uint32_t*,
-                             const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*,
-                             const float*, uint64_t*, const uint64_t*, uint32_t*, uint32_t*, float*);
+__global__ void union_gather_a(const uint32_t*, const uint32_t*, const float*, const uint32_t*,
+                               int64_t, uint32_t*, float*);
+__global__ void union_mutual(const uint32_t*, const uint32_t*, const uint32_t*, const float*,
+                             const uint32_t*, int64_t, float*, uint32_t*);
+__global__ void union_nonmutual(const uint32_t*, const uint32_t*, const uint32_t*, int64_t, int64_t,
+                                uint32_t*);
+__global__ void union_gather_b(const uint32_t*, const uint32_t*, const uint32_t*, const float*,
+                               const uint32_t*, const uint32_t*, const uint32_t*, int64_t, uint32_t*,
+                               uint32_t*, float*);
+__global__ void union_rowcount(const uint32_t*, const uint32_t*, const uint32_t*, int64_t, uint32_t*,
+                               uint64_t*);
+__global__ void union_scatter_a(const DsDesc*, int, const uint32_t*, const uint32_t*, const float*,
+                                const float*, const uint32_t*, const uint32_t*, const uint32_t*,
+                                const uint64_t*, int64_t, uint32_t*, uint32_t*, float*);
+__global__ void union_scatter_b(const DsDesc*, int, const uint32_t*, const uint32_t*, const float*,
+                                const uint32_t*, const uint32_t*, const uint32_t*, const uint64_t*,
+                                int64_t, uint32_t*, uint32_t*, float*);
 __global__ void local_indptr_kernel(const DsDesc*, int, int64_t, const uint64_t*, const uint64_t*,
                                     const int64_t*, uint64_t*);
 }  // namespace hm
@@ -112,7 +126,10 @@ struct hm_job {
            *zrow = nullptr, *key1 = nullptr, *pay1 = nullptr, *srcrow = nullptr, *k1s = nullptr,
            *p1s = nullptr, *key2 = nullptr, *k2s = nullptr, *p2s = nullptr, *tmpk = nullptr,
            *tmpv = nullptr, *inptr = nullptr, *outptr = nullptr, *rows = nullptr, *cols = nullptr,
-           *table = nullptr, *toffs = nullptr, *sc32 = nullptr, *sc32t = nullptr;
+           *table = nullptr, *toffs = nullptr, *sc32 = nullptr, *sc32t = nullptr,
+           *ocol = nullptr, *mflag = nullptr, *nmf = nullptr, *bpos = nullptr, *bsrc = nullptr,
+           *brow = nullptr, *bptr = nullptr;
+  float *oval = nullptr, *orev = nullptr, *bval = nullptr;
   int *count = nullptr, *nitems_fb = nullptr;
   WorkItem *items_l1 = nullptr, *items_fb = nullptr;
   int64_t *pad_off = nullptr, *iptr_off = nullptr;
@@ -250,7 +267,7 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
   b += E * 8;                                        // knn idx + dist
   b += R * 24 + nds * 8;                             // sigma, rho, sigma64, zrow
   if (stages & HM_STAGE_FUZZY) {
-    b += E * 11 * 4 + (R + 1) * 8 + R * 16 + 16 + 2 * E * 12 + (R + nds) * 8;
+    b += E * 20 * 4 + (R + 1) * 12 + R * 16 + 16 + 2 * E * 12 + (R + nds) * 8;
     b += 2 * 256 * (cdiv(E, kRadixTile) + 1) * 4;
   }
   b += 2 * (cdiv(std::max<int64_t>(256 * (cdiv(E, kRadixTile) + 1), R + 1), kScanTile) + 2) * 8;
@@ -415,19 +432,39 @@ hm_status enqueue_pipeline(hm_job* J) {
     HM_TRY(radix_sort(J, J->key2, J->p1s, E, bits, J->k2s, J->p2s));
     hipLaunchKernelGGL(lower_bounds, dim3(grid_for(R + 1, 256)), dim3(256), 0, J->stream,
                        (const uint32_t*)J->k2s, E, R, J->outptr);
-    hipLaunchKernelGGL(union_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, 0, J->d_desc,
-                       nds, R, (const uint32_t*)J->inptr, (const uint32_t*)J->p1s,
-                       (const uint32_t*)J->outptr, (const uint32_t*)J->p2s,
-                       (const uint32_t*)J->key1, (const uint32_t*)J->srcrow, (const float*)J->wval,
-                       J->cntu, (const uint64_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                       (float*)nullptr);
+    // load-balanced union: reverse lookups, non-mutual in-edges, merge path
+    const unsigned ge = grid_for(E, 256);
+    hipLaunchKernelGGL(union_gather_a, dim3(ge), dim3(256), 0, J->stream, (const uint32_t*)J->p2s,
+                       (const uint32_t*)J->key1, (const float*)J->wval,
+                       (const uint32_t*)J->outptr, R, J->ocol, J->oval);
+    hipLaunchKernelGGL(union_mutual, dim3(ge), dim3(256), 0, J->stream, (const uint32_t*)J->k2s,
+                       (const uint32_t*)J->p2s, (const uint32_t*)J->ocol, (const float*)J->oval,
+                       (const uint32_t*)J->outptr, R, J->orev, J->mflag);
+    hipLaunchKernelGGL(union_nonmutual, dim3(grid_for(E + 1, 256)), dim3(256), 0, J->stream,
+                       (const uint32_t*)J->p1s, (const uint32_t*)J->mflag,
+                       (const uint32_t*)J->inptr, R, E, J->nmf);
+    HM_LAUNCH(ctx);
+    HM_TRY(scan_excl<uint32_t>(J, J->nmf, E + 1, J->bpos, (uint32_t*)nullptr, J->sc32));
+    hipLaunchKernelGGL(union_gather_b, dim3(ge), dim3(256), 0, J->stream, (const uint32_t*)J->k1s,
+                       (const uint32_t*)J->p1s, (const uint32_t*)J->srcrow, (const float*)J->wval,
+                       (const uint32_t*)J->nmf, (const uint32_t*)J->bpos,
+                       (const uint32_t*)J->inptr, R, J->bsrc, J->brow, J->bval);
+    hipLaunchKernelGGL(union_rowcount, dim3(grid_for(R + 1, 256)), dim3(256), 0, J->stream,
+                       (const uint32_t*)J->inptr, (const uint32_t*)J->outptr,
+                       (const uint32_t*)J->bpos, R, J->bptr, J->cntu);
     HM_LAUNCH(ctx);
     HM_TRY(scan_excl<uint64_t>(J, J->cntu, R, J->indptr_g, J->total_nnz, J->sc64));
-    hipLaunchKernelGGL(union_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, 1, J->d_desc,
-                       nds, R, (const uint32_t*)J->inptr, (const uint32_t*)J->p1s,
-                       (const uint32_t*)J->outptr, (const uint32_t*)J->p2s,
-                       (const uint32_t*)J->key1, (const uint32_t*)J->srcrow, (const float*)J->wval,
-                       J->cntu, (const uint64_t*)J->indptr_g, J->rows, J->cols, J->vals);
+    hipLaunchKernelGGL(union_scatter_a, dim3(ge), dim3(256), 0, J->stream, J->d_desc, nds,
+                       (const uint32_t*)J->k2s, (const uint32_t*)J->ocol, (const float*)J->oval,
+                       (const float*)J->orev, (const uint32_t*)J->outptr, (const uint32_t*)J->bsrc,
+                       (const uint32_t*)J->bptr, (const uint64_t*)J->indptr_g, R, J->rows, J->cols,
+                       J->vals);
+    hipLaunchKernelGGL(union_scatter_b, dim3(ge), dim3(256), 0, J->stream, J->d_desc, nds,
+                       (const uint32_t*)J->brow, (const uint32_t*)J->bsrc, (const float*)J->bval,
+                       (const uint32_t*)J->bptr, (const uint32_t*)J->ocol,
+                       (const uint32_t*)J->outptr, (const uint64_t*)J->indptr_g, R, J->rows,
+                       J->cols, J->vals);
+    HM_LAUNCH(ctx);
     hipLaunchKernelGGL(local_indptr_kernel, dim3(grid_for(R + nds, 256)), dim3(256), 0, J->stream,
                        J->d_desc, nds, R, (const uint64_t*)J->indptr_g,
                        (const uint64_t*)J->total_nnz, (const int64_t*)J->iptr_off, J->indptr_l);
@@ -623,8 +660,11 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     HM_A(J->rows, 2 * E); HM_A(J->cols, 2 * E); HM_A(J->vals, 2 * E);
     HM_A(J->indptr_l, R + nds); HM_A(J->iptr_off, nds);
     HM_A(J->table, 256 * ntr); HM_A(J->toffs, 256 * ntr);
+    HM_A(J->ocol, E); HM_A(J->oval, E); HM_A(J->orev, E); HM_A(J->mflag, E);
+    HM_A(J->nmf, E + 1); HM_A(J->bpos, E + 1); HM_A(J->bsrc, E); HM_A(J->brow, E);
+    HM_A(J->bval, E); HM_A(J->bptr, R + 1);
   }
-  const int64_t scan_max = std::max<int64_t>(256 * ntr, R + 1);
+  const int64_t scan_max = std::max<int64_t>(std::max<int64_t>(256 * ntr, R + 1), E + 1);
   HM_A(J->sc32, cdiv(scan_max, kScanTile) + 2);
   HM_A(J->sc64, cdiv(scan_max, kScanTile) + 2);
 #undef HM_A
