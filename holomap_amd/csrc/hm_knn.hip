@@ -428,8 +428,8 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DsDesc* __restrict__ 
   }
 
   // ---- certification against every column outside the candidate set ----
-  bool cert = true;
-  if (taumin != ~0ull) {
+  bool cert = taumin != 0ull;  // 0: a candidate list overflowed (incomplete)
+  if (cert && taumin != ~0ull) {
     const float dk = __uint_as_float((uint32_t)(ex[k - 1] >> 32));
     const double tf = (double)unordered_f32((uint32_t)(taumin >> 32));
     const double nu = (double)next_up(dk);

@@ -20,8 +20,8 @@ __global__ void rownorm_kernel(const DsDesc*, int, int64_t, float*, double*);
 __global__ void prep_kernel(const DsDesc*, int, int64_t, const int64_t*, const float*,
                             const double*, int);
 __global__ void statmax_kernel(const DsDesc*);
-template <int MODE, int QCAP>
-__global__ void gram_topc_kernel(const DsDesc*, const WorkItem*, const int*, int, const uint32_t*);
+template <int MODE, int PASS>
+__global__ void gram_sweep_kernel(const DsDesc*, const WorkItem*, const int*, int, const uint32_t*);
 __global__ void exact_scan_kernel(const DsDesc*, const WorkItem*, const int*, int,
                                   const uint32_t*, int, int, uint64_t*);
 __global__ void exact_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t, int,
@@ -109,13 +109,14 @@ struct hm_job {
   std::vector<DsHost> dsh;
   std::vector<DsDesc> hdesc;
   int64_t total_rows = 0, total_pad = 0, total_edges = 0;
-  int kmax = 0, ql = 48, mode = 0, xs = 1, fb_cap = 0, n_items_l1 = 0;
+  int kmax = 0, mode = 0, xs = 1, fb_cap = 0, n_items_l1 = 0;
   std::vector<WorkItem> items_l1_h;
   std::vector<void*> allocs;
   uint64_t bytes = 0;
   // device buffers
   DsDesc* d_desc = nullptr;
   float *x = nullptr, *xr = nullptr, *op32c = nullptr, *colbias = nullptr, *rstat = nullptr,
+        *thr = nullptr,
         *smax = nullptr, *nz = nullptr, *knn_dist = nullptr, *sigma = nullptr, *rho = nullptr,
         *wval = nullptr, *vals = nullptr;
   uint16_t* op16 = nullptr;
@@ -163,8 +164,6 @@ hm_status fail(hm_ctx* ctx, hm_status st, const std::string& msg) {
       return fail((ctx), HM_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e_)); \
   } while (0)
 
-// per-lane list capacity of the sweep: keeps ceil(C/2) .. QL-16 keys
-int pick_ql(int) { return 64; }
 
 int gram_mode(int precision) {
   switch (precision) {
@@ -210,7 +209,7 @@ Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_
     h.n_pad = cdiv(h.n, 128) * 128;
     int margin = opt.margin < 0 ? 2 * h.k : opt.margin;
     int C = h.k + margin;
-    C = std::min(C, 96);  // two lane lists of up to 48 keys
+    C = std::min(C, 64);  // two collect lists of 48 keys hold ~C + 8 candidates
     C = std::max(C, h.k);
     h.C = C;
     h.row_off = P.total_rows;
@@ -226,8 +225,7 @@ Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_
   }
   // sweep splits: a lone dataset (or a small batch) splits its columns so
   // the launch has enough work items for every CU
-  const int ql = pick_ql(P.cmax);
-  const int cout = ql - 16;
+  const int cout = 48;  // collect-list slots per (row, split, half)
   int64_t rb_total = 0;
   for (auto& h : P.dsh) rb_total += cdiv(h.n, 128);
   for (auto& h : P.dsh) {
@@ -261,7 +259,7 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
     b += P.sum_xr * 4 + P.sum_cos * 4;               // xr, op32 (cosine)
     if (mode == 0) b += P.sum_xr * 2;                // op16
     b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + R * 12;
-    b += P.sum_cand * 8 + R * 16 * 8;                // cand, ctau (<= 16 lists)
+    b += P.sum_cand * 8 + R * 16 * 8 + R * 8 * 4;    // cand, ctau, thr
     b += R * 12 + P.fb_cap * 16 + R * P.xs * P.kmax * 8;
   }
   b += E * 8;                                        // knn idx + dist
@@ -336,15 +334,17 @@ hm_status radix_sort(hm_job* J, const uint32_t* kin, const uint32_t* vin, int64_
 unsigned grid_for(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, cdiv(n, per)); }
 
 hm_status launch_sweep(hm_job* J) {
-  const unsigned grid = (unsigned)std::max(1, std::min(J->n_items_l1, J->ctx->num_cus));
-#define HM_GRAM(M, Q)                                                                       \
-  hipLaunchKernelGGL((gram_topc_kernel<M, Q>), dim3(grid), dim3(256), 0, J->stream, J->d_desc, \
-                     (const WorkItem*)J->items_l1, (const int*)nullptr, J->n_items_l1,       \
+  const unsigned g0 = (unsigned)std::max(1, std::min(J->n_items_l1, J->ctx->num_cus));
+  const unsigned g1 = g0;
+#define HM_GRAM(M, P)                                                                        \
+  hipLaunchKernelGGL((gram_sweep_kernel<M, P>), dim3(P == 0 ? g0 : g1), dim3(256), 0, J->stream, \
+                     J->d_desc,                                                               \
+                     (const WorkItem*)J->items_l1, (const int*)nullptr, J->n_items_l1,        \
                      (const uint32_t*)nullptr)
   switch (J->mode) {
-    case 0: HM_GRAM(0, 64); break;
-    case 1: HM_GRAM(1, 64); break;
-    case 2: HM_GRAM(2, 64); break;
+    case 0: HM_GRAM(0, 0); HM_GRAM(0, 1); break;
+    case 1: HM_GRAM(1, 0); HM_GRAM(1, 1); break;
+    case 2: HM_GRAM(2, 0); HM_GRAM(2, 1); break;
     default: break;
   }
 #undef HM_GRAM
@@ -602,7 +602,6 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
   J->total_pad = P.total_pad;
   J->total_edges = P.total_edges;
   J->kmax = P.kmax;
-  J->ql = pick_ql(P.cmax);
   J->mode = gram_mode(o.precision);
   J->xs = P.xs;
   J->fb_cap = (int)P.fb_cap;
@@ -634,6 +633,9 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     int64_t sum_tau = 0;
     for (auto& h : P.dsh) sum_tau += h.n * h.nlist;
     HM_A(J->ctau, sum_tau);
+    int64_t sum_thr = 0;
+    for (auto& h : P.dsh) sum_thr += h.n * h.nsplit;
+    HM_A(J->thr, sum_thr);
     HM_A(J->flag, R);
     HM_A(J->flagpos, R);
     HM_A(J->list, R);
@@ -671,7 +673,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
 
   // descriptors
   J->hdesc.resize(nds);
-  int64_t off_x = 0, off_xr = 0, off_cos = 0, off_pad = 0, off_cand = 0, off_tau = 0;
+  int64_t off_x = 0, off_xr = 0, off_cos = 0, off_pad = 0, off_cand = 0, off_tau = 0, off_thr = 0;
   std::vector<int64_t> pad_off_h(nds + 1, 0);
   std::vector<double> target_h(nds);
   J->iptr_off_h.resize(nds);
@@ -691,6 +693,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
       D.smax = J->smax + (int64_t)i * HM_NSTAT;
       D.cand = J->cand + off_cand;
       D.ctau = J->ctau + off_tau;
+      D.thr = J->thr + off_thr;
     }
     D.knn_idx = J->knn_idx + h.edge_off;
     D.knn_dist = J->knn_dist + h.edge_off;
@@ -712,6 +715,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     off_pad += h.n_pad;
     off_cand += h.n * h.nlist * h.cout;
     off_tau += h.n * h.nlist;
+    off_thr += h.n * h.nsplit;
     target_h[i] = std::log2((double)h.k);  // std f64::log2 (glibc), src/smooth_knn.rs:25
     J->iptr_off_h[i] = h.row_off + i;
   }

@@ -35,20 +35,30 @@ HM_HD uint64_t sel_key(float v, uint32_t j) { return ((uint64_t)sel_ordered(v) <
 // the kept entries are exactly the smallest ones in the u64 key order
 // (ordered(value) << 32 | column), and tau becomes the largest kept key.
 // Every dropped entry is larger than the new tau.
+// The entry loops load 16 entries at a time unconditionally (the whole QL
+// region is allocated) and mask by e < cnt afterwards, so the loads of a
+// batch are in flight together instead of each waiting behind a guard.
+#define HM_SEL_B 16
+
 template <int QL, typename J>
 HM_HD void lane_compact(float* kf, J* kj, int stride, int& cnt, uint64_t& tau, int act,
                         int keep_lo, int keep_hi) {
+  static_assert(QL % HM_SEL_B == 0, "QL must be a multiple of the load batch");
   const bool active = cnt > act;
   // f(v) = #{entries with value <= v}
   float mx = -INFINITY, mn = INFINITY;
   int ninf = 0;
-#pragma unroll 16
-  for (int e = 0; e < QL; ++e) {
-    if (e < cnt) {
-      const float v = kf[e * stride];
-      mx = v > mx ? v : mx;
-      if (v == -INFINITY) ++ninf;
-      else mn = v < mn ? v : mn;
+  for (int e0 = 0; e0 < QL; e0 += HM_SEL_B) {
+    float v[HM_SEL_B];
+#pragma unroll
+    for (int u = 0; u < HM_SEL_B; ++u) v[u] = kf[(e0 + u) * stride];
+#pragma unroll
+    for (int u = 0; u < HM_SEL_B; ++u) {
+      if (e0 + u < cnt) {
+        mx = v[u] > mx ? v[u] : mx;
+        if (v[u] == -INFINITY) ++ninf;
+        else mn = v[u] < mn ? v[u] : mn;
+      }
     }
   }
   uint64_t T = ~0ull;  // keep the entries with key <= T
@@ -66,8 +76,13 @@ HM_HD void lane_compact(float* kf, J* kj, int stride, int& cnt, uint64_t& tau, i
     if (!HM_ANY(!done)) break;
     const float mid = lo * 0.5f + hi * 0.5f;
     int c = 0;
-#pragma unroll 16
-    for (int e = 0; e < QL; ++e) c += (e < cnt && kf[e * stride] <= mid) ? 1 : 0;
+    for (int e0 = 0; e0 < QL; e0 += HM_SEL_B) {
+      float v[HM_SEL_B];
+#pragma unroll
+      for (int u = 0; u < HM_SEL_B; ++u) v[u] = kf[(e0 + u) * stride];
+#pragma unroll
+      for (int u = 0; u < HM_SEL_B; ++u) c += (e0 + u < cnt && v[u] <= mid) ? 1 : 0;
+    }
     if (!done) {
       if (!(mid > lo && mid < hi)) {
         // lo and hi are adjacent floats: the ties at value hi straddle the
@@ -88,19 +103,32 @@ HM_HD void lane_compact(float* kf, J* kj, int stride, int& cnt, uint64_t& tau, i
   // ties: keep every value < tie and the smallest columns among value == tie
   if (HM_ANY(active && tiep)) {
     int c_lt = 0;
-#pragma unroll 16
-    for (int e = 0; e < QL; ++e) c_lt += (e < cnt && kf[e * stride] < tie) ? 1 : 0;
+    for (int e0 = 0; e0 < QL; e0 += HM_SEL_B) {
+      float v[HM_SEL_B];
+#pragma unroll
+      for (int u = 0; u < HM_SEL_B; ++u) v[u] = kf[(e0 + u) * stride];
+#pragma unroll
+      for (int u = 0; u < HM_SEL_B; ++u) c_lt += (e0 + u < cnt && v[u] < tie) ? 1 : 0;
+    }
     const int need = keep_lo - c_lt;
     uint32_t jl = 0, jh = 0xffffffffu;
     for (int it = 0; it < 34; ++it) {
       const bool go = active && tiep && jl < jh;
       if (!HM_ANY(go)) break;
+      const uint32_t jm = jl + (jh - jl) / 2;
+      int c = 0;
+      for (int e0 = 0; e0 < QL; e0 += HM_SEL_B) {
+        float v[HM_SEL_B];
+        uint32_t jv[HM_SEL_B];
+#pragma unroll
+        for (int u = 0; u < HM_SEL_B; ++u) {
+          v[u] = kf[(e0 + u) * stride];
+          jv[u] = (uint32_t)kj[(e0 + u) * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < HM_SEL_B; ++u) c += (e0 + u < cnt && v[u] == tie && jv[u] <= jm) ? 1 : 0;
+      }
       if (go) {
-        const uint32_t jm = jl + (jh - jl) / 2;
-        int c = 0;
-#pragma unroll 16
-        for (int e = 0; e < QL; ++e)
-          c += (e < cnt && kf[e * stride] == tie && (uint32_t)kj[e * stride] <= jm) ? 1 : 0;
         if (c >= need) jh = jm; else jl = jm + 1;
       }
     }
@@ -113,12 +141,20 @@ HM_HD void lane_compact(float* kf, J* kj, int stride, int& cnt, uint64_t& tau, i
     for (int it = 0; it < 66; ++it) {
       const bool go = fb && l64 < h64;
       if (!HM_ANY(go)) break;
+      const uint64_t m64 = l64 + (h64 - l64) / 2;
+      int c = 0;
+      for (int e0 = 0; e0 < QL; e0 += HM_SEL_B) {
+        float v[HM_SEL_B];
+        uint32_t jv[HM_SEL_B];
+#pragma unroll
+        for (int u = 0; u < HM_SEL_B; ++u) {
+          v[u] = kf[(e0 + u) * stride];
+          jv[u] = (uint32_t)kj[(e0 + u) * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < HM_SEL_B; ++u) c += (e0 + u < cnt && sel_key(v[u], jv[u]) <= m64) ? 1 : 0;
+      }
       if (go) {
-        const uint64_t m64 = l64 + (h64 - l64) / 2;
-        int c = 0;
-#pragma unroll 16
-        for (int e = 0; e < QL; ++e)
-          c += (e < cnt && sel_key(kf[e * stride], (uint32_t)kj[e * stride]) <= m64) ? 1 : 0;
         if (c >= keep_lo) h64 = m64; else l64 = m64 + 1;
       }
     }
@@ -127,14 +163,20 @@ HM_HD void lane_compact(float* kf, J* kj, int stride, int& cnt, uint64_t& tau, i
   if (active) {
     int w = 0;
     uint64_t tmax = 0;
-    for (int e = 0; e < QL; ++e) {
-      if (e < cnt) {
-        const float v = kf[e * stride];
-        const J j = kj[e * stride];
-        const uint64_t key = sel_key(v, (uint32_t)j);
-        if (key <= T) {
-          kf[w * stride] = v;
-          kj[w * stride] = j;
+    for (int e0 = 0; e0 < QL; e0 += HM_SEL_B) {
+      float v[HM_SEL_B];
+      J jv[HM_SEL_B];
+#pragma unroll
+      for (int u = 0; u < HM_SEL_B; ++u) {
+        v[u] = kf[(e0 + u) * stride];
+        jv[u] = kj[(e0 + u) * stride];
+      }
+#pragma unroll
+      for (int u = 0; u < HM_SEL_B; ++u) {
+        const uint64_t key = sel_key(v[u], (uint32_t)jv[u]);
+        if (e0 + u < cnt && key <= T) {
+          kf[w * stride] = v[u];
+          kj[w * stride] = jv[u];
           ++w;
           tmax = key > tmax ? key : tmax;
         }
