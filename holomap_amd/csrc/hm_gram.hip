@@ -6,12 +6,22 @@
 // hm_knn.hip), so the selection precision never reaches the output.
 //
 // Work item = 128 query rows of one dataset x one column split. Workgroup =
-// 4 waves; wave w owns queries [32w, 32w+32). Each step the workgroup stages
-// a 128-column block and the 128 query rows (KC features at a time) in LDS
-// and every wave computes its 32 queries x 128 columns with four 32x32 MFMA
-// tiles (candidates as the M operand, queries as N, so an output lane holds a
-// single query: lane l -> query l%32, register a of tile t -> column
-// 32t + 8*(a/4) + 4*(l/32) + a%4 of the block).
+// 4 waves; wave w owns queries [32w, 32w+32). The split's columns are swept in
+// 128-column blocks; every wave computes its 32 queries x 128 columns with
+// four 32x32 MFMA tiles (candidates as the M operand, queries as N, so an
+// output lane holds a single query: lane l -> query l%32, register a of tile
+// t -> column 32t + 8*(a/4) + 4*(l/32) + a%4 of the block).
+//
+// Operand staging: a "stage" is one 128-byte slice of every operand row of a
+// block (64 bf16 or 32 f32 features) for the 128 query rows and the 128
+// candidate rows, plus the block's 128 column biases. Stages are copied
+// global -> LDS by the LDS-DMA path (global_load_lds, 16 bytes per lane, no
+// registers held while in flight) into a ring of NSTAGE slots, NSTAGE-1
+// stages ahead of the MFMAs; a wave waits for its own copies with
+// s_waitcnt vmcnt and a barrier publishes them. The ring runs across block
+// boundaries, so the pipeline never drains inside an item. Rows in LDS are
+// 128 bytes with the eight 16-byte chunks XOR-swizzled per row, which keeps
+// the MFMA operand reads (16 bytes per lane, 32 rows) free of bank conflicts.
 //
 // Selection key per (query i, column j):
 //   euclidean  key = ||x_j||^2 - 2 <x_i, x_j>   (row constant ||x_i||^2 dropped)
@@ -23,145 +33,295 @@
 //    column subset. T = the C-th smallest group minimum. C distinct groups
 //    each contain a key <= T, so at least C keys of the split are <= T.
 //  PASS 1 (collect): every column with key <= T is appended to its lane's
-//    list (f32 key, u16 column offset in LDS). All other columns have
-//    key > T: that is the threshold the certification bound uses. A list
-//    that overflows is reported (threshold 0) and its row is recomputed by
-//    the exact scan. The keys of both passes come from identical MFMA
-//    sequences; correctness only relies on pass 1.
+//    list in global memory (the row's candidate slots; a few dozen appends
+//    per lane over a whole split). All other columns have key > T: that is
+//    the threshold the certification bound uses. A list that overflows is
+//    reported (threshold 0) and its row is recomputed by the exact scan.
+//    The keys of both passes come from identical MFMA sequences;
+//    correctness only relies on pass 1.
 // No atomics; nothing depends on scheduling.
 #include "hm_internal.hpp"
 
 namespace hm {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) void lds_void_t;
 
-#define HM_QB 48  // pass-1 list capacity per lane
+// Workgroup barrier that orders LDS only. __syncthreads() also waits for
+// every outstanding global access (vmcnt 0), which would drain the copy
+// pipeline at each barrier; the copies are waited for explicitly instead.
+#define HM_LDS_BARRIER() asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory")
 
-__device__ __forceinline__ uint16_t bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  uint32_t r = ((u >> 16) & 1u) + 0x7fffu;
-  return (uint16_t)((u + r) >> 16);
+template <int N>
+__device__ __forceinline__ void vm_wait() {
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
 }
-__device__ __forceinline__ float bf16_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // MODE 0: bf16 operand (op16); MODE 1: f32 operand (op32, fp32 MFMA);
-// MODE 2: bf16x3 -- op32 split on load into hi + lo bf16, three MFMAs per
+// MODE 2: bf16x3 -- hi (op16) and lo (op16lo) bf16 parts, three MFMAs per
 //         step (hi.hi + lo.hi + hi.lo).
-// LDS: the two operand tiles take ~70-74 KB, the pass-1 lists 72 KB.
+// HM_SWEEP_OCC workgroups share a CU in modes 0 and 1 (the LDS ring is split
+// between them).
 template <int MODE>
-struct GramCfg {
-  static constexpr int KC = (MODE == 0) ? 128 : 64;     // features per chunk
-  static constexpr int GES = (MODE == 0) ? 2 : 4;       // global element bytes
-  static constexpr int ES = (MODE == 1) ? 4 : 2;        // LDS element bytes
-  static constexpr int LD = KC + (MODE == 1 ? 4 : 8);   // LDS row stride (elements)
-  static constexpr int NT = (MODE == 2) ? 2 : 1;        // tiles per operand (hi/lo)
-  static constexpr int U = KC * GES / 16;               // 16-byte loads per row
-  static constexpr int S = 128 * U / 256;               // loads per thread per operand
+struct SweepCfg {
+  static constexpr int NT = MODE == 2 ? 2 : 1;   // parts per operand (hi, lo)
+  static constexpr int ES = MODE == 1 ? 4 : 2;   // element bytes
+  static constexpr int KC = 128 / ES;            // features per stage
+  static constexpr int TILE = 128 * 128;         // bytes of one 128-row slice
+  static constexpr int NI = NT * 8;              // operand copies per wave per stage
+  static constexpr int COLB = 2 * 512;           // column biases (two copies)
+  static constexpr int STAGE = 2 * NT * TILE + COLB;
+  static constexpr int OCC = MODE == 2 ? 1 : HM_SWEEP_OCC;  // workgroups per CU
+  static constexpr int NSTAGE = (MODE == 2 ? 2 : 4) / OCC;
+  static constexpr int VM = NI + 1;              // vmcnt entries per stage and wave
+  static_assert(NSTAGE >= 2, "the ring needs two slots");
 };
 
-// Global -> register prefetch of one KC-feature chunk of the 128 query rows
-// and the 128 candidate rows (16-byte loads, zero beyond the padded width).
-template <int S, int U, int KC, int GES>
-__device__ __forceinline__ void gram_gload(u32x4 (&gq)[S], u32x4 (&gc)[S], const void* opbase,
-                                           const int64_t (&qrow)[S], int64_t cb, int kp, int k0,
-                                           int tid) {
-  const int valid_bytes = (kp - k0 < KC ? kp - k0 : KC) * GES;
-  const size_t rowbytes = (size_t)kp * GES;
+// Chunk slot of logical 16-byte chunk c in LDS row r: c ^ swz(r).
+__device__ __forceinline__ int row_swz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
+
+// Per-item copy plan of one lane. Tile order in a stage: 0 query hi,
+// 1 candidate hi, 2 query lo, 3 candidate lo, then the biases. Wave w copies
+// the 8-row groups rb = w*NI + i, i < NI (1 KB each: lane l -> row rb*8 + l/8
+// of the stage, chunk slot l%8), and 64 biases. src[i] is the lane's source
+// for feature slice 0 (and, for candidate rows, block 0 of the split); a
+// stage adds a uniform byte offset.
+template <int MODE>
+struct CopyPlan {
+  const uint8_t* src[SweepCfg<MODE>::NI];
+  const float* bias;
+};
+
+template <int MODE>
+__device__ __forceinline__ void sweep_issue(uint8_t* stage, const CopyPlan<MODE>& P, int64_t uoff,
+                                            int64_t boff, bool partial, int kbyte, int rowbytes,
+                                            int cb_even, const uint8_t* __restrict__ zero,
+                                            int wave) {
+  using Cfg = SweepCfg<MODE>;
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int idx = tid + 256 * s;
-    const int r = idx / U, c16 = idx % U;
-    const bool inb = c16 * 16 < valid_bytes;
-    const size_t koff = (size_t)k0 * GES + (size_t)c16 * 16;
-    const u32x4 zero = {0u, 0u, 0u, 0u};
-    gq[s] = (qrow[s] >= 0 && inb)
-                ? *(const u32x4*)((const uint8_t*)opbase + (size_t)qrow[s] * rowbytes + koff)
-                : zero;
-    gc[s] = inb ? *(const u32x4*)((const uint8_t*)opbase + (size_t)(cb + r) * rowbytes + koff)
-                : zero;
+  for (int i = 0; i < Cfg::NI; ++i) {
+    const int rb = wave * Cfg::NI + i;
+    const uint8_t* src = P.src[i] + uoff;
+    if (partial) {
+      // last slice of a width that is not a multiple of 128 bytes: chunks
+      // beyond the padded width read zeros (exact zero products)
+      const int cbyte = (i & 1) ? (cb_even ^ 16) : cb_even;
+      src = kbyte + cbyte < rowbytes ? src : zero;
+    }
+    __builtin_amdgcn_global_load_lds(src, (lds_void_t*)(stage + rb * 1024), 16, 0, 0);
+  }
+  __builtin_amdgcn_global_load_lds(
+      P.bias + boff,
+      (lds_void_t*)(stage + 2 * Cfg::NT * Cfg::TILE + (wave >> 1) * 512 + (wave & 1) * 256), 4, 0,
+      0);
+}
+
+// Wait until this wave's copies of the stage computed next have landed;
+// `after` stages were issued after it.
+template <int MODE>
+__device__ __forceinline__ void sweep_wait(int after) {
+  constexpr int V = SweepCfg<MODE>::VM;
+  if constexpr (SweepCfg<MODE>::NSTAGE == 4) {
+    if (after >= 3) vm_wait<3 * V>();
+    else if (after == 2) vm_wait<2 * V>();
+    else if (after == 1) vm_wait<V>();
+    else vm_wait<0>();
+  } else {
+    if (after >= 1) vm_wait<V>();
+    else vm_wait<0>();
   }
 }
 
-__device__ __forceinline__ uint32_t pack_hi(uint32_t a, uint32_t b, uint32_t& lo_pack) {
-  // hi/lo bf16 split of two f32 bit patterns; returns packed hi, sets packed lo
-  const float fa = __uint_as_float(a), fb = __uint_as_float(b);
-  const uint16_t ha = bf16_rne(fa), hb = bf16_rne(fb);
-  const uint16_t la = bf16_rne(fa - bf16_f32(ha)), lb = bf16_rne(fb - bf16_f32(hb));
-  lo_pack = (uint32_t)la | ((uint32_t)lb << 16);
-  return (uint32_t)ha | ((uint32_t)hb << 16);
+// Accumulate one stage: 32 queries x 128 columns x one 128-byte slice.
+// Operand reads go in two batches of 10 ahead of the MFMAs that use them
+// (sched_group_barrier keeps that order; the default schedule interleaves
+// reads and MFMAs one by one and exposes the LDS latency each time).
+// LOWREG: the second batch is read after the first batch's MFMAs (40
+// operand VGPRs instead of 80; pass 0 also holds 64 group minima).
+template <int MODE, bool LOWREG>
+__device__ __forceinline__ void sweep_mma(floatx16 (&acc)[4], const uint8_t* st,
+                                          const uint8_t* qrow_p, int q, int half, int swz) {
+  constexpr int TILE = SweepCfg<MODE>::TILE;
+  if constexpr (MODE == 1) {
+    // chunk c = 2m + half holds features 8m + 4*half + j (j < 4); MFMA j
+    // of step m sums features 8m + j and 8m + 4 + j over the two halves
+    floatx4 b4[4], a4[4][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int cs = ((2 * m + half) ^ swz) * 16;
+      b4[m] = *(const floatx4*)(qrow_p + cs);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) a4[m][t] = *(const floatx4*)(st + TILE + (t * 32 + q) * 128 + cs);
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[m][t][j], b4[m][j], acc[t], 0, 0, 0);
+    if constexpr (LOWREG) {
+      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
+    } else {
+      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
+    }
+  } else if constexpr (MODE == 0) {
+    bf16x8_t bh[4], ah[4][4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int cs = ((2 * ks + half) ^ swz) * 16;
+      bh[ks] = *(const bf16x8_t*)(qrow_p + cs);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) ah[ks][t] = *(const bf16x8_t*)(st + TILE + (t * 32 + q) * 128 + cs);
+    }
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks][t], bh[ks], acc[t], 0, 0, 0);
+    if constexpr (LOWREG) {
+      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);  // reads of steps 0-1
+      __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);   // MFMAs of steps 0-1
+      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);  // reads of steps 2-3
+      __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);   // MFMAs of steps 2-3
+    } else {
+      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);  // reads of steps 0-1
+      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);  // reads of steps 2-3
+      __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);   // MFMAs of steps 0-1
+      __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);   // MFMAs of steps 2-3
+    }
+  } else {
+    // two halves of two k-steps: hi and lo parts of both operands
+#pragma unroll
+    for (int kh = 0; kh < 2; ++kh) {
+      bf16x8_t bh[2], bl[2], ah[2][4], al[2][4];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int cs = ((2 * (2 * kh + u) + half) ^ swz) * 16;
+        bh[u] = *(const bf16x8_t*)(qrow_p + cs);
+        bl[u] = *(const bf16x8_t*)(qrow_p + 2 * TILE + cs);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          ah[u][t] = *(const bf16x8_t*)(st + TILE + (t * 32 + q) * 128 + cs);
+          al[u][t] = *(const bf16x8_t*)(st + 3 * TILE + (t * 32 + q) * 128 + cs);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[u][t], bh[u], acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[u][t], bh[u], acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[u][t], bl[u], acc[t], 0, 0, 0);
+        }
+      __builtin_amdgcn_sched_group_barrier(0x100, 20, 0);  // 20 LDS reads
+      __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);  // 24 MFMAs
+    }
+  }
 }
 
-// Register -> LDS store of the prefetched chunk (MODE 2 splits into hi/lo).
-template <int MODE, int S, int U, int LD>
-__device__ __forceinline__ void gram_lstore(const u32x4 (&gq)[S], const u32x4 (&gc)[S],
-                                            uint8_t* qh, uint8_t* ql, uint8_t* ch,
-                                            uint8_t* cl, int tid) {
+// Epilogue of one 128-column block. FULL: every column of the block is in
+// the split (no validity test). Pass 1 appends hits to dst[min(cnt, cout-1)]
+// and counts them all: cnt > cout afterwards means the list overflowed (its
+// contents are then unused). T is NaN for lanes without a query, so they
+// never hit. A position costs fma + compare + a skipped branch when no lane
+// of the wave hits.
+template <int PASS, bool FULL>
+__device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[4], float (&gmin)[4][16],
+                                               const float* colb, int rem, uint32_t colbase,
+                                               float scale, float T, int cout, uint64_t* dst,
+                                               int& cnt) {
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int idx = tid + 256 * s;
-    const int r = idx / U, c16 = idx % U;
-    if constexpr (MODE == 0) {
-      *(u32x4*)(qh + (r * LD + c16 * 8) * 2) = gq[s];
-      *(u32x4*)(ch + (r * LD + c16 * 8) * 2) = gc[s];
-    } else if constexpr (MODE == 1) {
-      *(u32x4*)(qh + (r * LD + c16 * 4) * 4) = gq[s];
-      *(u32x4*)(ch + (r * LD + c16 * 4) * 4) = gc[s];
-    } else {
-      uint32_t l0, l1, l2, l3;
-      const uint32_t h0 = pack_hi(gq[s].x, gq[s].y, l0), h1 = pack_hi(gq[s].z, gq[s].w, l1);
-      const uint32_t h2 = pack_hi(gc[s].x, gc[s].y, l2), h3 = pack_hi(gc[s].z, gc[s].w, l3);
-      *(u32x2*)(qh + (r * LD + c16 * 4) * 2) = u32x2{h0, h1};
-      *(u32x2*)(ql + (r * LD + c16 * 4) * 2) = u32x2{l0, l1};
-      *(u32x2*)(ch + (r * LD + c16 * 4) * 2) = u32x2{h2, h3};
-      *(u32x2*)(cl + (r * LD + c16 * 4) * 2) = u32x2{l2, l3};
+  for (int t = 0; t < 4; ++t) {
+#pragma unroll
+    for (int a4 = 0; a4 < 4; ++a4) {
+      const int jc = t * 32 + 8 * a4;  // column of register a4*4 at half 0
+      const floatx4 cbv = *(const floatx4*)(colb + jc);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int a = a4 * 4 + e;
+        const bool valid = FULL || jc + e < rem;
+        const float key = __builtin_fmaf(scale, acc[t][a], cbv[e]);
+        if constexpr (PASS == 0) {
+          gmin[t][a] = valid ? fminf(gmin[t][a], key) : gmin[t][a];
+        } else {
+          if (valid && key <= T) {
+            dst[cnt < cout ? cnt : cout - 1] =
+                ((uint64_t)ordered_u32(key) << 32) | (colbase + jc + e);
+            ++cnt;
+          }
+        }
+      }
     }
   }
 }
 
 template <int MODE, int PASS>
-__global__ __launch_bounds__(256, 1) void gram_sweep_kernel(
+__global__ __launch_bounds__(256, SweepCfg<MODE>::OCC) void gram_sweep_kernel(
     const DsDesc* __restrict__ ds_arr, const WorkItem* __restrict__ items,
     const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist) {
-  using Cfg = GramCfg<MODE>;
-  constexpr int KC = Cfg::KC, LD = Cfg::LD, NT = Cfg::NT, U = Cfg::U, S = Cfg::S,
-                GES = Cfg::GES;
-  constexpr int QB = (PASS == 1) ? HM_QB : 1;
-  __shared__ __attribute__((aligned(16))) uint8_t q_s[NT][128 * LD * Cfg::ES];
-  __shared__ __attribute__((aligned(16))) uint8_t c_s[NT][128 * LD * Cfg::ES];
-  __shared__ float colb_s[128];
-  __shared__ float lkf[QB][256];
-  __shared__ uint16_t lkj[QB][256];  // column - cbeg (split span <= 65536)
+  using Cfg = SweepCfg<MODE>;
+  constexpr int NI = Cfg::NI, NSTAGE = Cfg::NSTAGE, STAGE = Cfg::STAGE, TILE = Cfg::TILE;
+  __shared__ __attribute__((aligned(1024))) uint8_t smem[NSTAGE * STAGE];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q = lane & 31, half = lane >> 5;
   const int nitems = nitems_dev ? *nitems_dev : nitems_host;
+  // copy side: query rows for waves whose slices are query tiles; the
+  // swizzled global chunk of copy i is c_even (i even) or c_even ^ 1
+  const bool qwave = (((wave * NI) >> 4) & 1) == 0;
+  const int cb_even = ((lane & 7) ^ (lane >> 3)) * 16;  // byte offset of the chunk (rb even)
+  // read side: rows 32t + q and 32w + q share swz(q)
+  const int swz = row_swz(q);
 
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const WorkItem W = items[it];
     const DsDesc& D = ds_arr[W.ds];
     const int64_t n = D.n;
     const int kp = D.kp;
+    const int rowbytes = kp * Cfg::ES;
     const float scale = D.metric == 1 ? -1.0f : -2.0f;
     // column range of this split (aligned to 128 columns)
     const int64_t span = ((n + (int64_t)D.nsplit * 128 - 1) / ((int64_t)D.nsplit * 128)) * 128;
     const int64_t cbeg = (int64_t)W.split * span;
     const int64_t cend = cbeg + span < n ? cbeg + span : n;
+    const int nchunk = (rowbytes + 127) / 128;
+    const bool ragged = (rowbytes & 127) != 0;  // last slice partly beyond the width
+    const int nblk = cend > cbeg ? (int)((cend - cbeg + 127) / 128) : 0;
+    const int total = nblk * nchunk;
+    const int64_t blkbytes = (int64_t)128 * rowbytes;
 
-    // row ids of this item's queries, for the loader (S rows per thread)
-    int64_t qrow[S];
+    const uint8_t* ophi = (MODE == 1) ? (const uint8_t*)D.op32 : (const uint8_t*)D.op16;
+    const uint8_t* oplo = (MODE == 2) ? (const uint8_t*)D.op16lo : ophi;
+    const uint8_t* zero = (const uint8_t*)D.zero16;
+    CopyPlan<MODE> P;
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      const int r = (tid + 256 * s) / U;
-      if (r < W.qcnt) {
-        const int p = W.qbeg + r;
-        qrow[s] = qlist ? (int64_t)qlist[p] - D.row_off : (int64_t)p;
+    for (int i = 0; i < NI; ++i) {
+      const int rb = wave * NI + i;
+      const int r = ((rb & 15) << 3) + (lane >> 3);  // row within its tile
+      int64_t row;
+      if (qwave) {
+        row = 0;  // missing queries read row 0 (results unused)
+        if (r < W.qcnt) {
+          const int p = W.qbeg + r;
+          row = qlist ? (int64_t)qlist[p] - D.row_off : (int64_t)p;
+        }
       } else {
-        qrow[s] = -1;
+        row = cbeg + r;
       }
+      const int cbyte = (i & 1) ? (cb_even ^ 16) : cb_even;
+      P.src[i] = ((rb >> 4) >= 2 ? oplo : ophi) + row * rowbytes + cbyte;
     }
+    P.bias = D.colbias + cbeg + (wave & 1) * 64 + lane;
+
     // this lane's own query
     const int slot = wave * 32 + q;
     const bool myvalid = slot < W.qcnt;
@@ -170,9 +330,10 @@ __global__ __launch_bounds__(256, 1) void gram_sweep_kernel(
                 : 0;
 
     float gmin[4][16];
-    float T = -INFINITY;
+    float T = __builtin_nanf("");  // no query: compares false
     int cnt = 0;
-    bool ovf = false;
+    const int list = W.split * 2 + half;
+    uint64_t* dst = D.cand + ((size_t)myrow * D.nlist + list) * D.cout;
     if constexpr (PASS == 0) {
 #pragma unroll
       for (int t = 0; t < 4; ++t)
@@ -182,86 +343,55 @@ __global__ __launch_bounds__(256, 1) void gram_sweep_kernel(
       if (myvalid) T = D.thr[(size_t)myrow * D.nsplit + W.split];
     }
 
-    const void* opbase = (MODE == 0) ? (const void*)D.op16 : (const void*)D.op32;
-    const int nchunk = (kp + KC - 1) / KC;
-
-    for (int64_t cb = cbeg; cb < cend; cb += 128) {
-      floatx16 acc[4];
+    HM_LDS_BARRIER();  // the previous item's last stage is no longer read
+    // prologue: stages 0 .. NSTAGE-2
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int e = 0; e < 16; ++e) acc[t][e] = 0.0f;
-
-      u32x4 gq[S], gc[S];
-      gram_gload<S, U, KC, GES>(gq, gc, opbase, qrow, cb, kp, 0, tid);
-      for (int ch = 0; ch < nchunk; ++ch) {
-        const int k0 = ch * KC;
-        __syncthreads();  // all waves are done with the previous tiles / colb_s
-        gram_lstore<MODE, S, U, LD>(gq, gc, q_s[0], q_s[NT - 1], c_s[0], c_s[NT - 1], tid);
-        if (ch == 0 && tid < 128) {
-          const int64_t j = cb + tid;
-          colb_s[tid] = j < n ? D.colbias[j] : 0.0f;
-        }
-        __syncthreads();
-        if (ch + 1 < nchunk) gram_gload<S, U, KC, GES>(gq, gc, opbase, qrow, cb, kp, k0 + KC, tid);
-        const int kvalid = kp - k0 < KC ? kp - k0 : KC;
-        if constexpr (MODE == 1) {
-          const float* qf = (const float*)q_s[0];
-          const float* cf = (const float*)c_s[0];
-          for (int ks = 0; ks < kvalid / 2; ++ks) {
-            const int kk = ks * 2 + half;
-            const float b = qf[(wave * 32 + q) * LD + kk];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-              const float a = cf[(t * 32 + q) * LD + kk];
-              acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[t], 0, 0, 0);
-            }
-          }
-        } else {
-          for (int ks = 0; ks < kvalid / 16; ++ks) {
-            const int kk = ks * 16 + half * 8;
-            const bf16x8_t bh = *(const bf16x8_t*)(q_s[0] + ((wave * 32 + q) * LD + kk) * 2);
-            bf16x8_t bl;
-            if constexpr (MODE == 2) bl = *(const bf16x8_t*)(q_s[1] + ((wave * 32 + q) * LD + kk) * 2);
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-              const bf16x8_t ah = *(const bf16x8_t*)(c_s[0] + ((t * 32 + q) * LD + kk) * 2);
-              acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[t], 0, 0, 0);
-              if constexpr (MODE == 2) {
-                const bf16x8_t al = *(const bf16x8_t*)(c_s[1] + ((t * 32 + q) * LD + kk) * 2);
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[t], 0, 0, 0);
-              }
-            }
-          }
-        }
+    for (int g = 0; g < NSTAGE - 1; ++g) {
+      if (g < total) {
+        const int b = g / nchunk, c = g - b * nchunk;
+        sweep_issue<MODE>(smem + g * STAGE, P, (qwave ? 0 : b * blkbytes) + c * 128,
+                          (int64_t)b * 128, ragged && c == nchunk - 1, c * 128, rowbytes, cb_even,
+                          zero, wave);
       }
+    }
 
-      // ---- epilogue ----
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-#pragma unroll
-        for (int a = 0; a < 16; ++a) {
-          const int jl = t * 32 + 8 * (a >> 2) + 4 * half + (a & 3);
-          const bool valid = cb + jl < cend;
-          const float key = __builtin_fmaf(scale, acc[t][a], colb_s[jl]);
-          if constexpr (PASS == 0) {
-            gmin[t][a] = valid ? fminf(gmin[t][a], key) : gmin[t][a];
-          } else {
-            const bool hit = valid && key <= T;
-            if (__ballot(hit) != 0ull) {
-              if (hit) {
-                if (cnt < QB) {
-                  lkf[cnt][tid] = key;
-                  lkj[cnt][tid] = (uint16_t)(cb + jl - cbeg);
-                  ++cnt;
-                } else {
-                  ovf = true;
-                }
-              }
-            }
-          }
+    floatx16 acc[4];
+    for (int g = 0; g < total; ++g) {
+      const int blk = g / nchunk, ch = g - blk * nchunk;
+      HM_LDS_BARRIER();  // every wave is done with stage g-1 (its slot is reused now)
+      {
+        const int gi = g + NSTAGE - 1;
+        if (gi < total) {
+          const int b = gi / nchunk, c = gi - b * nchunk;
+          sweep_issue<MODE>(smem + (gi % NSTAGE) * STAGE, P, (qwave ? 0 : b * blkbytes) + c * 128,
+                            (int64_t)b * 128, ragged && c == nchunk - 1, c * 128, rowbytes,
+                            cb_even, zero, wave);
         }
+        const int after = total - 1 - g < NSTAGE - 1 ? total - 1 - g : NSTAGE - 1;
+        sweep_wait<MODE>(after);
+      }
+      HM_LDS_BARRIER();  // stage g is in LDS for every wave
+      const uint8_t* st = smem + (g % NSTAGE) * STAGE;
+      if (ch == 0) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int e = 0; e < 16; ++e) acc[t][e] = 0.0f;
+      }
+      sweep_mma<MODE, PASS == 0 && Cfg::OCC == 2>(acc, st, st + (wave * 32 + q) * 128, q, half,
+                                                  swz);
+      if (ch != nchunk - 1) continue;
+
+      // ---- epilogue of block blk (its biases ride in this stage) ----
+      // 32-bit per-lane bases; position offsets are compile-time constants
+      const int64_t cb = cbeg + (int64_t)blk * 128;
+      const float* colb = (const float*)(st + 2 * Cfg::NT * TILE) + 4 * half;
+      const uint32_t colbase = (uint32_t)cb + 4 * half;
+      if (cend - cb >= 128) {
+        sweep_epilogue<PASS, true>(acc, gmin, colb, 128, colbase, scale, T, D.cout, dst, cnt);
+      } else {
+        const int rem = (int)(cend - cb) - 4 * half;
+        sweep_epilogue<PASS, false>(acc, gmin, colb, rem, colbase, scale, T, D.cout, dst, cnt);
       }
     }
 
@@ -290,18 +420,11 @@ __global__ __launch_bounds__(256, 1) void gram_sweep_kernel(
       if (myvalid && half == 0) D.thr[(size_t)myrow * D.nsplit + W.split] = unordered_f32(lo);
     } else {
       if (myvalid) {
-        const int list = W.split * 2 + half;
-        uint64_t* dst = D.cand + ((size_t)myrow * D.nlist + list) * D.cout;
-        for (int e = 0; e < D.cout; ++e) {
-          uint64_t v = ~0ull;
-          if (e < cnt)
-            v = ((uint64_t)ordered_u32(lkf[e][tid]) << 32) | (uint32_t)(lkj[e][tid] + cbeg);
-          dst[e] = v;
-        }
+        for (int e = cnt; e < D.cout; ++e) dst[e] = ~0ull;
         // every column not listed has key > T; an overflowing list reports 0
         // (no threshold) so the row is left to the exact scan
         D.ctau[(size_t)myrow * D.nlist + list] =
-            ovf ? 0ull : (((uint64_t)ordered_u32(T) << 32) | 0xffffffffull);
+            cnt > D.cout ? 0ull : (((uint64_t)ordered_u32(T) << 32) | 0xffffffffull);
       }
     }
   }

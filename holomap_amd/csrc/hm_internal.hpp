@@ -20,6 +20,7 @@ struct DsDesc {
   const float* op32;     // candidate-selection operand, f32: n_pad x kp
                          //   (euclidean: == xr; cosine: rows / ||row||)
   const uint16_t* op16;  // same operand in bf16 (RNE), n_pad x kp, or null
+  const uint16_t* op16lo;  // bf16 of (operand - op16), n_pad x kp, or null
   const float* colbias;  // n_pad: euclid ||x_j||^2 (f32); cosine 0 / -inf
   const double* rnorm;   // n: cosine sqrt(sum x^2) (f64, reference fold)
   const float* rstat;    // n x HM_NSTAT per-row bound statistics
@@ -27,6 +28,7 @@ struct DsDesc {
   uint64_t* cand;        // candidate keys: n x nlist x cout
   uint64_t* ctau;        // list thresholds: n x nlist
   float* thr;            // sweep pass-0 thresholds: n x nsplit
+  const uint64_t* zero16;  // >= 16 zero bytes (sweep copies beyond the padded width)
   uint32_t* knn_idx;     // n x k (output)
   float* knn_dist;       // n x k (output)
   int64_t n;             // rows
@@ -43,6 +45,12 @@ struct DsDesc {
                          //   half of the 64-lane wave, see hm_gram.hip)
   int32_t cout;          // key slots per list
 };
+
+// Workgroups of the candidate sweep per CU (its LDS ring is split between
+// them).
+#ifndef HM_SWEEP_OCC
+#define HM_SWEEP_OCC 2
+#endif
 
 // Number of float statistics kept per row for the candidate error bound.
 #define HM_NSTAT 8

@@ -91,7 +91,7 @@ __device__ __forceinline__ void sort128(uint64_t& v0, uint64_t& v1) {
 __global__ __launch_bounds__(256) void exact_scan_kernel(
     const DsDesc* __restrict__ ds_arr, const WorkItem* __restrict__ items,
     const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist,
-    int xs, int kmax, uint64_t* __restrict__ xpart) {
+    const int* __restrict__ xs_dev, int xs_host, int kmax, uint64_t* __restrict__ xpart) {
   __shared__ double qf[XS_QB][XS_KF];
   __shared__ float cf[XS_CB][XS_KF + 1];
   __shared__ uint64_t buf[XS_QB][XS_QCAP];
@@ -100,11 +100,14 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nitems = nitems_dev ? *nitems_dev : nitems_host;
+  const int xs = xs_dev ? *xs_dev : xs_host;
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const WorkItem W = items[it];
     const DsDesc& D = ds_arr[W.ds];
     const int64_t n = D.n;
     const int kp = D.kp, d = D.d, k = D.k, metric = D.metric;
+    // waves whose 8 query slots are all empty skip the arithmetic
+    const bool wact = wave * 8 < W.qcnt;
     const int64_t span = ((n + (int64_t)xs * XS_CB - 1) / ((int64_t)xs * XS_CB)) * XS_CB;
     const int64_t cbeg = (int64_t)W.split * span;
     const int64_t cend = cbeg + span < n ? cbeg + span : n;
@@ -164,7 +167,8 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(
         }
         __syncthreads();
         const int nf = d - f0 < XS_KF ? d - f0 : XS_KF;
-        if (metric == 0) {
+        if (!wact) {
+        } else if (metric == 0) {
           for (int f = 0; f < nf; ++f) {
             const double b = (double)cf[lane][f];
 #pragma unroll
@@ -231,11 +235,12 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(
 // One wave per list position p < *count (or per local row in identity mode).
 __global__ __launch_bounds__(256) void exact_merge_kernel(
     const DsDesc* __restrict__ ds_arr, int nds, const uint32_t* __restrict__ qlist,
-    const int* __restrict__ count_dev, int64_t count_host, int xs, int kmax,
-    const uint64_t* __restrict__ xpart) {
+    const int* __restrict__ count_dev, int64_t count_host, const int* __restrict__ xs_dev,
+    int xs_host, int kmax, const uint64_t* __restrict__ xpart) {
   __shared__ uint64_t s[4][1024];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t count = count_dev ? (int64_t)*count_dev : count_host;
+  const int xs = xs_dev ? *xs_dev : xs_host;
   for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < count; p += (int64_t)gridDim.x * 4) {
     int64_t grow = qlist ? (int64_t)qlist[p] : p;
     int lo = 0, hi = nds - 1;
@@ -246,15 +251,31 @@ __global__ __launch_bounds__(256) void exact_merge_kernel(
     const DsDesc& D = ds_arr[lo];
     const int64_t row = grow - D.row_off;
     const int k = D.k;
-    int len = 64;
-    while (len < xs * k) len <<= 1;
+    const int tot = xs * k;
+    const uint64_t* src = xpart + (size_t)p * xs * kmax;
     uint64_t* a = s[wave];
-    for (int e = lane; e < len; e += 64) {
-      const int sp = e / k, t = e % k;
-      a[e] = (sp < xs) ? xpart[((size_t)p * xs + sp) * kmax + t] : ~0ull;
+    if (tot <= 1024) {
+      int len = 64;
+      while (len < tot) len <<= 1;
+      for (int e = lane; e < len; e += 64) {
+        const int sp = e / k, t = e % k;
+        a[e] = (sp < xs) ? src[(size_t)sp * kmax + t] : ~0ull;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      wave_sort_lds(a, len);
+    } else {
+      // rounds of 960 new keys on top of the 64 smallest so far (k <= 64)
+      a[lane] = ~0ull;
+      for (int base = 0; base < tot; base += 960) {
+        for (int e = 64 + lane; e < 1024; e += 64) {
+          const int g = base + e - 64;
+          a[e] = g < tot ? src[(size_t)(g / k) * kmax + g % k] : ~0ull;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        wave_sort_lds(a, 1024);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    wave_sort_lds(a, len);
     for (int t = lane; t < k; t += 64) {
       const uint64_t key = a[t];
       D.knn_idx[row * k + t] = (uint32_t)key;
@@ -446,11 +467,37 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DsDesc* __restrict__ 
 // Fallback work list: for each dataset, blocks of <= XS_QB flagged rows
 // (from the sorted global list) x xs column splits. Single thread; the
 // dataset count is small.
+// The column split count starts at the planned xs and grows when few rows are
+// flagged (a handful of rows would otherwise run on a handful of CUs), up to
+// XS_MAX and within the item and partial-list capacities, which hold
+// item_cap items and row_cap * xs lists.
+#define XS_MAX 512
 __global__ void fallback_items_kernel(const DsDesc* __restrict__ ds_arr, int nds,
                                       const uint32_t* __restrict__ list, const int* __restrict__ count,
-                                      int xs, WorkItem* __restrict__ items, int* __restrict__ nitems) {
+                                      int xs, int64_t item_cap, int64_t row_cap, int target,
+                                      WorkItem* __restrict__ items, int* __restrict__ nitems,
+                                      int* __restrict__ xs_out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int cnt = *count;
+  int64_t nqb = 0;
+  {
+    int p = 0;
+    for (int di = 0; di < nds; ++di) {
+      const int64_t end = ds_arr[di].row_off + ds_arr[di].n;
+      int q = p;
+      while (q < cnt && (int64_t)list[q] < end) ++q;
+      nqb += (q - p + XS_QB - 1) / XS_QB;
+      p = q;
+    }
+  }
+  int64_t xd = xs;
+  if (nqb > 0 && nqb * xs < target) {
+    int64_t g = target / nqb;
+    g = g < XS_MAX ? g : XS_MAX;
+    g = g < item_cap / nqb ? g : item_cap / nqb;
+    g = g < row_cap * xs / cnt ? g : row_cap * xs / cnt;
+    xd = g > xd ? g : xd;
+  }
   int p = 0, w = 0;
   for (int di = 0; di < nds; ++di) {
     const int64_t end = ds_arr[di].row_off + ds_arr[di].n;
@@ -458,11 +505,12 @@ __global__ void fallback_items_kernel(const DsDesc* __restrict__ ds_arr, int nds
     while (q < cnt && (int64_t)list[q] < end) ++q;
     for (int b = p; b < q; b += XS_QB) {
       const int qc = q - b < XS_QB ? q - b : XS_QB;
-      for (int s = 0; s < xs; ++s) items[w++] = WorkItem{di, s, b, qc};
+      for (int s = 0; s < (int)xd; ++s) items[w++] = WorkItem{di, s, b, qc};
     }
     p = q;
   }
   *nitems = w;
+  *xs_out = (int)xd;
 }
 
 }  // namespace hm

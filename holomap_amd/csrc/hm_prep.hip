@@ -52,8 +52,8 @@ __global__ void rownorm_kernel(const DsDesc* __restrict__ ds, int nds, int64_t t
   rn_out[g] = sqrt(s);
 }
 
-// Wave per (dataset, padded row). Writes xr, op32 (cosine), op16, colbias and
-// rstat. mode_bits: 1 = produce op16 (bf16 sweep), 2 = bf16x3 stats.
+// Wave per (dataset, padded row). Writes xr, op32 (cosine), op16 (+ op16lo),
+// colbias and rstat. mode_bits: 1 = produce op16 (bf16 sweep), 2 = bf16x3 stats.
 __global__ __launch_bounds__(256) void prep_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_pad_rows,
     const int64_t* __restrict__ pad_off /* nds+1 prefix of n_pad */,
@@ -73,6 +73,7 @@ __global__ __launch_bounds__(256) void prep_kernel(
   float* xr = const_cast<float*>(D.xr) + i * kp;
   float* op32 = const_cast<float*>(D.op32) + i * kp;
   uint16_t* op16 = D.op16 ? const_cast<uint16_t*>(D.op16) + i * kp : nullptr;
+  uint16_t* op16lo = D.op16lo ? const_cast<uint16_t*>(D.op16lo) + i * kp : nullptr;
   const bool cosine = D.metric == 1;
   const bool real = i < D.n;
   double rn = 0.0;
@@ -103,6 +104,7 @@ __global__ __launch_bounds__(256) void prep_kernel(
     uint16_t h = f32_to_bf16_rne(w);
     float hf = bf16_to_f32(h);
     if (op16) op16[f] = h;
+    if (op16lo) op16lo[f] = f32_to_bf16_rne(w - hf);  // w - hf is exact in f32
     if (mode_bits & 1) {
       double e = wexact - (double)hf;
       s_xn += (double)hf * (double)hf;

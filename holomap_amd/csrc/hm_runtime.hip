@@ -23,12 +23,12 @@ __global__ void statmax_kernel(const DsDesc*);
 template <int MODE, int PASS>
 __global__ void gram_sweep_kernel(const DsDesc*, const WorkItem*, const int*, int, const uint32_t*);
 __global__ void exact_scan_kernel(const DsDesc*, const WorkItem*, const int*, int,
-                                  const uint32_t*, int, int, uint64_t*);
-__global__ void exact_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t, int,
-                                   int, const uint64_t*);
+                                  const uint32_t*, const int*, int, int, uint64_t*);
+__global__ void exact_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t,
+                                   const int*, int, int, const uint64_t*);
 __global__ void rerank_kernel(const DsDesc*, int, int64_t, int, uint32_t*);
 __global__ void fallback_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int,
-                                      WorkItem*, int*);
+                                      int64_t, int64_t, int, WorkItem*, int*, int*);
 __global__ void smooth_knn_kernel(const DsDesc*, int, int64_t, const double*, int, float*, float*,
                                   double*, uint32_t*);
 __global__ void calib_fixup_kernel(const DsDesc*, int, float*, double*, const uint32_t*);
@@ -119,8 +119,9 @@ struct hm_job {
         *thr = nullptr,
         *smax = nullptr, *nz = nullptr, *knn_dist = nullptr, *sigma = nullptr, *rho = nullptr,
         *wval = nullptr, *vals = nullptr;
-  uint16_t* op16 = nullptr;
+  uint16_t *op16 = nullptr, *op16lo = nullptr;
   double *rnorm = nullptr, *sigma64 = nullptr, *target = nullptr;
+  uint64_t *zero16 = nullptr;  // 512 zero bytes: sweep source beyond the padded width
   uint64_t *cand = nullptr, *ctau = nullptr, *xpart = nullptr, *cntu = nullptr,
            *indptr_g = nullptr, *total_nnz = nullptr, *indptr_l = nullptr, *sc64 = nullptr;
   uint32_t *knn_idx = nullptr, *flag = nullptr, *flagpos = nullptr, *list = nullptr,
@@ -131,7 +132,7 @@ struct hm_job {
            *ocol = nullptr, *mflag = nullptr, *nmf = nullptr, *bpos = nullptr, *bsrc = nullptr,
            *brow = nullptr, *bptr = nullptr;
   float *oval = nullptr, *orev = nullptr, *bval = nullptr;
-  int *count = nullptr, *nitems_fb = nullptr;
+  int *count = nullptr, *nitems_fb = nullptr, *xs_fb = nullptr;
   WorkItem *items_l1 = nullptr, *items_fb = nullptr;
   int64_t *pad_off = nullptr, *iptr_off = nullptr;
   std::vector<int64_t> iptr_off_h;
@@ -258,6 +259,7 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
   if (stages & HM_STAGE_KNN) {
     b += P.sum_xr * 4 + P.sum_cos * 4;               // xr, op32 (cosine)
     if (mode == 0) b += P.sum_xr * 2;                // op16
+    if (mode == 2) b += P.sum_xr * 4;                // op16 + op16lo
     b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + R * 12;
     b += P.sum_cand * 8 + R * 16 * 8 + R * 8 * 4;    // cand, ctau, thr
     b += R * 12 + P.fb_cap * 16 + R * P.xs * P.kmax * 8;
@@ -331,14 +333,20 @@ hm_status radix_sort(hm_job* J, const uint32_t* kin, const uint32_t* vin, int64_
   return HM_OK;
 }
 
+// Persistent grid of the candidate sweep: HM_SWEEP_OCC workgroups per CU
+// (one in bf16x3 mode), at most one per item.
+int64_t sweep_grid(const hm_ctx* ctx, int mode, int64_t items) {
+  const int64_t occ = mode == 2 ? 1 : HM_SWEEP_OCC;
+  return std::max<int64_t>(1, std::min<int64_t>(items, occ * ctx->num_cus));
+}
+
 unsigned grid_for(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, cdiv(n, per)); }
 
 hm_status launch_sweep(hm_job* J) {
-  const unsigned g0 = (unsigned)std::max(1, std::min(J->n_items_l1, J->ctx->num_cus));
-  const unsigned g1 = g0;
+  // grid and item order go together (see the item permutation in hm_job_create)
+  const unsigned g = (unsigned)sweep_grid(J->ctx, J->mode, J->n_items_l1);
 #define HM_GRAM(M, P)                                                                        \
-  hipLaunchKernelGGL((gram_sweep_kernel<M, P>), dim3(P == 0 ? g0 : g1), dim3(256), 0, J->stream, \
-                     J->d_desc,                                                               \
+  hipLaunchKernelGGL((gram_sweep_kernel<M, P>), dim3(g), dim3(256), 0, J->stream, J->d_desc, \
                      (const WorkItem*)J->items_l1, (const int*)nullptr, J->n_items_l1,        \
                      (const uint32_t*)nullptr)
   switch (J->mode) {
@@ -390,17 +398,17 @@ hm_status enqueue_pipeline(hm_job* J) {
     HM_TRY(scan_excl<uint32_t>(J, J->flag, R, J->flagpos, (uint32_t*)nullptr, J->sc32));
     hipLaunchKernelGGL(compact_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream,
                        (const uint32_t*)J->flag, (const uint32_t*)J->flagpos, R, J->list, J->count);
-    hipLaunchKernelGGL(fallback_items_kernel, dim3(1), dim3(64), 0, J->stream, J->d_desc, nds,
-                       (const uint32_t*)J->list, (const int*)J->count, J->xs, J->items_fb,
-                       J->nitems_fb);
     const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(J->fb_cap, 2 * cus));
+    hipLaunchKernelGGL(fallback_items_kernel, dim3(1), dim3(64), 0, J->stream, J->d_desc, nds,
+                       (const uint32_t*)J->list, (const int*)J->count, J->xs, (int64_t)J->fb_cap,
+                       R, (int)gx, J->items_fb, J->nitems_fb, J->xs_fb);
     hipLaunchKernelGGL(exact_scan_kernel, dim3(gx), dim3(256), 0, J->stream, J->d_desc,
                        (const WorkItem*)J->items_fb, (const int*)J->nitems_fb, 0,
-                       (const uint32_t*)J->list, J->xs, J->kmax, J->xpart);
+                       (const uint32_t*)J->list, (const int*)J->xs_fb, J->xs, J->kmax, J->xpart);
     const unsigned gm = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(R, 4), 8 * cus));
     hipLaunchKernelGGL(exact_merge_kernel, dim3(gm), dim3(256), 0, J->stream, J->d_desc, nds,
-                       (const uint32_t*)J->list, (const int*)J->count, (int64_t)0, J->xs, J->kmax,
-                       (const uint64_t*)J->xpart);
+                       (const uint32_t*)J->list, (const int*)J->count, (int64_t)0,
+                       (const int*)J->xs_fb, J->xs, J->kmax, (const uint64_t*)J->xpart);
     HM_LAUNCH(ctx);
     mark(J, 4);
   } else {
@@ -623,7 +631,8 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
   if (knn) {
     HM_A(J->xr, P.sum_xr);
     if (P.sum_cos > 0) HM_A(J->op32c, P.sum_cos);
-    if (J->mode == 0) HM_A(J->op16, P.sum_xr);
+    if (J->mode == 0 || J->mode == 2) HM_A(J->op16, P.sum_xr);
+    if (J->mode == 2) HM_A(J->op16lo, P.sum_xr);
     HM_A(J->colbias, P.total_pad);
     HM_A(J->rstat, R * HM_NSTAT);
     HM_A(J->smax, (int64_t)nds * HM_NSTAT);
@@ -636,11 +645,15 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     int64_t sum_thr = 0;
     for (auto& h : P.dsh) sum_thr += h.n * h.nsplit;
     HM_A(J->thr, sum_thr);
+    HM_A(J->zero16, 64);
+    if (hipMemset(J->zero16, 0, 64 * sizeof(uint64_t)) != hipSuccess)
+      return bail(fail(ctx, HM_EDEVICE, "hipMemset failed"));
     HM_A(J->flag, R);
     HM_A(J->flagpos, R);
     HM_A(J->list, R);
     HM_A(J->count, 1);
     HM_A(J->nitems_fb, 1);
+    HM_A(J->xs_fb, 1);
     HM_A(J->items_fb, std::max<int64_t>(P.fb_cap, 1));
     HM_A(J->xpart, R * P.xs * P.kmax);
     HM_A(J->pad_off, nds + 1);
@@ -687,6 +700,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
       if (h.metric == 1) { D.op32 = J->op32c + off_cos; off_cos += h.n_pad * h.kp; }
       else D.op32 = D.xr;
       D.op16 = J->op16 ? J->op16 + off_xr : nullptr;
+      D.op16lo = J->op16lo ? J->op16lo + off_xr : nullptr;
       D.colbias = J->colbias + off_pad;
       D.rnorm = J->rnorm + h.row_off;
       D.rstat = J->rstat + h.row_off * HM_NSTAT;
@@ -694,6 +708,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
       D.cand = J->cand + off_cand;
       D.ctau = J->ctau + off_tau;
       D.thr = J->thr + off_thr;
+      D.zero16 = J->zero16;
     }
     D.knn_idx = J->knn_idx + h.edge_off;
     D.knn_dist = J->knn_dist + h.edge_off;
@@ -741,8 +756,25 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     }
     std::stable_sort(it.begin(), it.end(),
                      [](const auto& a, const auto& b) { return a.first > b.first; });
+    // The sweep grid is G persistent workgroups; workgroup b takes items b,
+    // b + G, ... and runs on XCD b % X (round-robin dispatch). Item slots are
+    // permuted so that in every round each XCD receives a contiguous run of
+    // the (row block, split) order above: its workgroups then stream the same
+    // few row blocks and column splits at the same time and share them in
+    // that XCD's L2.
+    const int64_t ni = (int64_t)it.size();
+    const int64_t G = sweep_grid(ctx, J->mode, ni);
+    const int64_t X = (ctx->num_cus % 8 == 0) ? 8 : 1;
     J->items_l1_h.resize(it.size());
-    for (size_t t = 0; t < it.size(); ++t) J->items_l1_h[t] = it[t].second;
+    for (int64_t r0 = 0; r0 < ni; r0 += G) {
+      const int64_t rl = std::min(G, ni - r0);  // items in this round
+      const bool perm = X > 1 && rl % X == 0;
+      const int64_t per = rl / X;
+      for (int64_t l = 0; l < rl; ++l) {
+        const int64_t p = perm ? (l % per) * X + l / per : l;
+        J->items_l1_h[r0 + p] = it[r0 + l].second;
+      }
+    }
     J->n_items_l1 = (int)it.size();
     hm_status s2 = dalloc(J, &J->items_l1, (int64_t)it.size());
     if (s2 != HM_OK) return bail(s2);

@@ -251,3 +251,32 @@ def test_margin_and_split_invariance(ctx):
             job.close()
             assert np.array_equal(r.knn.indices, ref.knn.indices), (margin, nsplit)
             assert np.array_equal(r.graph.vals.view(np.uint32), ref.graph.vals.view(np.uint32))
+
+
+def test_few_uncertified_rows_use_wide_fallback(ctx):
+    # Row 0 has 200 neighbours at distance 1 (up to rounding) and those 200
+    # rows are 1.41 +- 0.2 apart: their k-th and C-th selection keys differ
+    # by less than the key error bound (which scales with the largest norms
+    # of the dataset, here ~60), so these rows alone are not certified and
+    # are recomputed by the exact scan. With so few rows the scan spreads
+    # them over many column splits (73 here) and merges more than 1024
+    # partial keys per row in rounds.
+    rng = np.random.default_rng(21)
+    n, d, k = 10000, 32, 15
+    x = (rng.normal(size=(n, d)) * 3.0 + 10.0).astype(np.float32)
+    c = np.zeros(d, np.float32)
+    u = rng.normal(size=(200, d))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    x[0] = c
+    x[1:201] = (c + u).astype(np.float32)
+    ref = O.knn_graph(x, d, k, "euclidean")
+    for prec in ["bf16", "fp32"]:
+        job = H.Job(ctx, [n], [d], [k], ["euclidean"], precision=prec)
+        job.upload(0, x)
+        job.run()
+        job.sync()
+        r = job.download(0)
+        fb = job.counters()["fallback_rows"]
+        job.close()
+        assert_graph_equal(r, *ref)
+        assert 1 <= fb <= 201, fb
