@@ -62,59 +62,58 @@ __device__ __forceinline__ void vm_wait() {
 // MODE 0: bf16 operand (op16); MODE 1: f32 operand (op32, fp32 MFMA);
 // MODE 2: bf16x3 -- hi (op16) and lo (op16lo) bf16 parts, three MFMAs per
 //         step (hi.hi + lo.hi + hi.lo).
-// HM_SWEEP_OCC workgroups share a CU in modes 0 and 1 (the LDS ring is split
-// between them).
+// A workgroup has NW = QW/32 waves: QW query rows share every candidate
+// slice (256 in modes 0 and 1; 128 in mode 2, whose stages are twice as big).
 template <int MODE>
 struct SweepCfg {
   static constexpr int NT = MODE == 2 ? 2 : 1;   // parts per operand (hi, lo)
   static constexpr int ES = MODE == 1 ? 4 : 2;   // element bytes
   static constexpr int KC = 128 / ES;            // features per stage
-  static constexpr int TILE = 128 * 128;         // bytes of one 128-row slice
-  static constexpr int NI = NT * 8;              // operand copies per wave per stage
-  static constexpr int COLB = 2 * 512;           // column biases (two copies)
-  static constexpr int STAGE = 2 * NT * TILE + COLB;
-  static constexpr int OCC = MODE == 2 ? 1 : HM_SWEEP_OCC;  // workgroups per CU
-  static constexpr int NSTAGE = (MODE == 2 ? 2 : 4) / OCC;
+  static constexpr int QW = MODE == 2 ? 128 : 256;  // query rows per item
+  static constexpr int NW = QW / 32;             // waves per workgroup
+  static constexpr int QG = QW / 8;              // 8-row copy groups of a query slice
+  static constexpr int RG = NT * (QG + 16);      // copy groups per stage
+  static constexpr int NI = RG / NW;             // operand copies per wave per stage
+  static constexpr int QT = 0;                   // stage offsets: query hi,
+  static constexpr int CT = QW * 128;            //   candidate hi,
+  static constexpr int QL = (QW + 128) * 128;    //   query lo, candidate lo
+  static constexpr int CL = (2 * QW + 128) * 128;
+  static constexpr int BIAS = NT * (QW + 128) * 128;  // NW/2 copies of the 128 biases
+  static constexpr int STAGE = BIAS + NW / 2 * 512;
+  static constexpr int NSTAGE = MODE == 2 ? 2 : 3;
   static constexpr int VM = NI + 1;              // vmcnt entries per stage and wave
-  static_assert(NSTAGE >= 2, "the ring needs two slots");
+  static_assert(RG % NW == 0, "copy groups split evenly over the waves");
+  static_assert(NSTAGE * STAGE <= 160 * 1024, "LDS");
 };
 
 // Chunk slot of logical 16-byte chunk c in LDS row r: c ^ swz(r).
 __device__ __forceinline__ int row_swz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
 
-// Per-item copy plan of one lane. Tile order in a stage: 0 query hi,
-// 1 candidate hi, 2 query lo, 3 candidate lo, then the biases. Wave w copies
-// the 8-row groups rb = w*NI + i, i < NI (1 KB each: lane l -> row rb*8 + l/8
-// of the stage, chunk slot l%8), and 64 biases. src[i] is the lane's source
-// for feature slice 0 (and, for candidate rows, block 0 of the split); a
-// stage adds a uniform byte offset.
+// Per-item copy plan of one lane. A stage is QG + 16 (x NT) groups of 8 rows
+// in the order query hi, candidate hi, query lo, candidate lo, then the
+// biases. Wave w copies the groups rb = w*NI + i, i < NI (1 KB each: lane l
+// -> row rb*8 + l/8 of the stage, chunk slot l%8) and 64 biases. src[i] is
+// the lane's source for feature slice 0 (and, for candidate rows, block 0 of
+// the split); cand[i] marks candidate groups, which also advance by block.
 template <int MODE>
 struct CopyPlan {
   const uint8_t* src[SweepCfg<MODE>::NI];
+  bool cand[SweepCfg<MODE>::NI];
   const float* bias;
 };
 
 template <int MODE>
-__device__ __forceinline__ void sweep_issue(uint8_t* stage, const CopyPlan<MODE>& P, int64_t uoff,
-                                            int64_t boff, bool partial, int kbyte, int rowbytes,
-                                            int cb_even, const uint8_t* __restrict__ zero,
-                                            int wave) {
+__device__ __forceinline__ void sweep_issue(uint8_t* stage, const CopyPlan<MODE>& P, int64_t boff_bytes,
+                                            int koff, int64_t boff, int wave) {
   using Cfg = SweepCfg<MODE>;
 #pragma unroll
   for (int i = 0; i < Cfg::NI; ++i) {
     const int rb = wave * Cfg::NI + i;
-    const uint8_t* src = P.src[i] + uoff;
-    if (partial) {
-      // last slice of a width that is not a multiple of 128 bytes: chunks
-      // beyond the padded width read zeros (exact zero products)
-      const int cbyte = (i & 1) ? (cb_even ^ 16) : cb_even;
-      src = kbyte + cbyte < rowbytes ? src : zero;
-    }
+    const uint8_t* src = P.src[i] + (P.cand[i] ? boff_bytes + koff : (int64_t)koff);
     __builtin_amdgcn_global_load_lds(src, (lds_void_t*)(stage + rb * 1024), 16, 0, 0);
   }
   __builtin_amdgcn_global_load_lds(
-      P.bias + boff,
-      (lds_void_t*)(stage + 2 * Cfg::NT * Cfg::TILE + (wave >> 1) * 512 + (wave & 1) * 256), 4, 0,
+      P.bias + boff, (lds_void_t*)(stage + Cfg::BIAS + (wave >> 1) * 512 + (wave & 1) * 256), 4, 0,
       0);
 }
 
@@ -123,15 +122,10 @@ __device__ __forceinline__ void sweep_issue(uint8_t* stage, const CopyPlan<MODE>
 template <int MODE>
 __device__ __forceinline__ void sweep_wait(int after) {
   constexpr int V = SweepCfg<MODE>::VM;
-  if constexpr (SweepCfg<MODE>::NSTAGE == 4) {
-    if (after >= 3) vm_wait<3 * V>();
-    else if (after == 2) vm_wait<2 * V>();
-    else if (after == 1) vm_wait<V>();
-    else vm_wait<0>();
-  } else {
-    if (after >= 1) vm_wait<V>();
-    else vm_wait<0>();
-  }
+  static_assert(SweepCfg<MODE>::NSTAGE <= 3, "waits for up to two later stages");
+  if (after >= 2) vm_wait<2 * V>();
+  else if (after == 1) vm_wait<V>();
+  else vm_wait<0>();
 }
 
 // Accumulate one stage: 32 queries x 128 columns x one 128-byte slice.
@@ -143,7 +137,7 @@ __device__ __forceinline__ void sweep_wait(int after) {
 template <int MODE, bool LOWREG>
 __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[4], const uint8_t* st,
                                           const uint8_t* qrow_p, int q, int half, int swz) {
-  constexpr int TILE = SweepCfg<MODE>::TILE;
+  using Cfg = SweepCfg<MODE>;
   if constexpr (MODE == 1) {
     // chunk c = 2m + half holds features 8m + 4*half + j (j < 4); MFMA j
     // of step m sums features 8m + j and 8m + 4 + j over the two halves
@@ -153,7 +147,7 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[4], const uint8_t* st,
       const int cs = ((2 * m + half) ^ swz) * 16;
       b4[m] = *(const floatx4*)(qrow_p + cs);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) a4[m][t] = *(const floatx4*)(st + TILE + (t * 32 + q) * 128 + cs);
+      for (int t = 0; t < 4; ++t) a4[m][t] = *(const floatx4*)(st + Cfg::CT + (t * 32 + q) * 128 + cs);
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m)
@@ -180,7 +174,7 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[4], const uint8_t* st,
       const int cs = ((2 * ks + half) ^ swz) * 16;
       bh[ks] = *(const bf16x8_t*)(qrow_p + cs);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) ah[ks][t] = *(const bf16x8_t*)(st + TILE + (t * 32 + q) * 128 + cs);
+      for (int t = 0; t < 4; ++t) ah[ks][t] = *(const bf16x8_t*)(st + Cfg::CT + (t * 32 + q) * 128 + cs);
     }
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
@@ -207,11 +201,11 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[4], const uint8_t* st,
       for (int u = 0; u < 2; ++u) {
         const int cs = ((2 * (2 * kh + u) + half) ^ swz) * 16;
         bh[u] = *(const bf16x8_t*)(qrow_p + cs);
-        bl[u] = *(const bf16x8_t*)(qrow_p + 2 * TILE + cs);
+        bl[u] = *(const bf16x8_t*)(qrow_p + Cfg::QL + cs);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          ah[u][t] = *(const bf16x8_t*)(st + TILE + (t * 32 + q) * 128 + cs);
-          al[u][t] = *(const bf16x8_t*)(st + 3 * TILE + (t * 32 + q) * 128 + cs);
+          ah[u][t] = *(const bf16x8_t*)(st + Cfg::CT + (t * 32 + q) * 128 + cs);
+          al[u][t] = *(const bf16x8_t*)(st + Cfg::CL + (t * 32 + q) * 128 + cs);
         }
       }
 #pragma unroll
@@ -265,19 +259,19 @@ __device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[4], float (
 }
 
 template <int MODE, int PASS>
-__global__ __launch_bounds__(256, SweepCfg<MODE>::OCC) void gram_sweep_kernel(
+__global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
     const DsDesc* __restrict__ ds_arr, const WorkItem* __restrict__ items,
     const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist) {
   using Cfg = SweepCfg<MODE>;
-  constexpr int NI = Cfg::NI, NSTAGE = Cfg::NSTAGE, STAGE = Cfg::STAGE, TILE = Cfg::TILE;
+  constexpr int NI = Cfg::NI, NSTAGE = Cfg::NSTAGE, STAGE = Cfg::STAGE, QG = Cfg::QG;
   __shared__ __attribute__((aligned(1024))) uint8_t smem[NSTAGE * STAGE];
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: LDS bases stay scalar
   const int q = lane & 31, half = lane >> 5;
   const int nitems = nitems_dev ? *nitems_dev : nitems_host;
-  // copy side: query rows for waves whose slices are query tiles; the
-  // swizzled global chunk of copy i is c_even (i even) or c_even ^ 1
-  const bool qwave = (((wave * NI) >> 4) & 1) == 0;
+  // copy side: the swizzled global chunk of a copy is cb_even (even group)
+  // or cb_even ^ 16 (odd group); tiles start at even groups
   const int cb_even = ((lane & 7) ^ (lane >> 3)) * 16;  // byte offset of the chunk (rb even)
   // read side: rows 32t + q and 32w + q share swz(q)
   const int swz = row_swz(q);
@@ -287,28 +281,30 @@ __global__ __launch_bounds__(256, SweepCfg<MODE>::OCC) void gram_sweep_kernel(
     const DsDesc& D = ds_arr[W.ds];
     const int64_t n = D.n;
     const int kp = D.kp;
-    const int rowbytes = kp * Cfg::ES;
+    // operand rows are a multiple of 128 bytes (bf16 rows padded to kp16)
+    const int rowbytes = MODE == 1 ? kp * 4 : D.kp16 * 2;
     const float scale = D.metric == 1 ? -1.0f : -2.0f;
     // column range of this split (aligned to 128 columns)
     const int64_t span = ((n + (int64_t)D.nsplit * 128 - 1) / ((int64_t)D.nsplit * 128)) * 128;
     const int64_t cbeg = (int64_t)W.split * span;
     const int64_t cend = cbeg + span < n ? cbeg + span : n;
-    const int nchunk = (rowbytes + 127) / 128;
-    const bool ragged = (rowbytes & 127) != 0;  // last slice partly beyond the width
+    const int nchunk = rowbytes / 128;
     const int nblk = cend > cbeg ? (int)((cend - cbeg + 127) / 128) : 0;
     const int total = nblk * nchunk;
     const int64_t blkbytes = (int64_t)128 * rowbytes;
 
     const uint8_t* ophi = (MODE == 1) ? (const uint8_t*)D.op32 : (const uint8_t*)D.op16;
     const uint8_t* oplo = (MODE == 2) ? (const uint8_t*)D.op16lo : ophi;
-    const uint8_t* zero = (const uint8_t*)D.zero16;
     CopyPlan<MODE> P;
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
-      const int rb = wave * NI + i;
-      const int r = ((rb & 15) << 3) + (lane >> 3);  // row within its tile
+      const int rb = wave * NI + i;  // uniform
+      const bool lo = rb >= QG + 16;
+      const int rl = lo ? rb - (QG + 16) : rb;  // group within the hi or lo part
+      const bool cand = rl >= QG;
+      const int r = ((cand ? rl - QG : rl) << 3) + (lane >> 3);  // row within its tile
       int64_t row;
-      if (qwave) {
+      if (!cand) {
         row = 0;  // missing queries read row 0 (results unused)
         if (r < W.qcnt) {
           const int p = W.qbeg + r;
@@ -318,7 +314,8 @@ __global__ __launch_bounds__(256, SweepCfg<MODE>::OCC) void gram_sweep_kernel(
         row = cbeg + r;
       }
       const int cbyte = (i & 1) ? (cb_even ^ 16) : cb_even;
-      P.src[i] = ((rb >> 4) >= 2 ? oplo : ophi) + row * rowbytes + cbyte;
+      P.src[i] = (lo ? oplo : ophi) + row * rowbytes + cbyte;
+      P.cand[i] = cand;
     }
     P.bias = D.colbias + cbeg + (wave & 1) * 64 + lane;
 
@@ -344,28 +341,26 @@ __global__ __launch_bounds__(256, SweepCfg<MODE>::OCC) void gram_sweep_kernel(
     }
 
     HM_LDS_BARRIER();  // the previous item's last stage is no longer read
-    // prologue: stages 0 .. NSTAGE-2
+    // (bi, ci): block and slice of the next stage to issue
+    int bi = 0, ci = 0;
 #pragma unroll
-    for (int g = 0; g < NSTAGE - 1; ++g) {
+    for (int g = 0; g < NSTAGE - 1; ++g) {  // prologue: stages 0 .. NSTAGE-2
       if (g < total) {
-        const int b = g / nchunk, c = g - b * nchunk;
-        sweep_issue<MODE>(smem + g * STAGE, P, (qwave ? 0 : b * blkbytes) + c * 128,
-                          (int64_t)b * 128, ragged && c == nchunk - 1, c * 128, rowbytes, cb_even,
-                          zero, wave);
+        sweep_issue<MODE>(smem + g * STAGE, P, bi * blkbytes, ci * 128, (int64_t)bi * 128, wave);
+        if (++ci == nchunk) { ci = 0; ++bi; }
       }
     }
 
     floatx16 acc[4];
+    int blk = 0, ch = 0;  // stage g
     for (int g = 0; g < total; ++g) {
-      const int blk = g / nchunk, ch = g - blk * nchunk;
       HM_LDS_BARRIER();  // every wave is done with stage g-1 (its slot is reused now)
       {
         const int gi = g + NSTAGE - 1;
         if (gi < total) {
-          const int b = gi / nchunk, c = gi - b * nchunk;
-          sweep_issue<MODE>(smem + (gi % NSTAGE) * STAGE, P, (qwave ? 0 : b * blkbytes) + c * 128,
-                            (int64_t)b * 128, ragged && c == nchunk - 1, c * 128, rowbytes,
-                            cb_even, zero, wave);
+          sweep_issue<MODE>(smem + (gi % NSTAGE) * STAGE, P, bi * blkbytes, ci * 128,
+                            (int64_t)bi * 128, wave);
+          if (++ci == nchunk) { ci = 0; ++bi; }
         }
         const int after = total - 1 - g < NSTAGE - 1 ? total - 1 - g : NSTAGE - 1;
         sweep_wait<MODE>(after);
@@ -378,14 +373,17 @@ __global__ __launch_bounds__(256, SweepCfg<MODE>::OCC) void gram_sweep_kernel(
 #pragma unroll
           for (int e = 0; e < 16; ++e) acc[t][e] = 0.0f;
       }
-      sweep_mma<MODE, PASS == 0 && Cfg::OCC == 2>(acc, st, st + (wave * 32 + q) * 128, q, half,
-                                                  swz);
-      if (ch != nchunk - 1) continue;
+      // two waves per SIMD in modes 0 and 1: pass 0 (64 group minima) reads
+      // in smaller batches to stay within 256 registers
+      sweep_mma<MODE, PASS == 0 && MODE != 2>(acc, st, st + Cfg::QT + (wave * 32 + q) * 128, q,
+                                              half, swz);
+      if (++ch != nchunk) continue;
+      ch = 0;
 
       // ---- epilogue of block blk (its biases ride in this stage) ----
       // 32-bit per-lane bases; position offsets are compile-time constants
       const int64_t cb = cbeg + (int64_t)blk * 128;
-      const float* colb = (const float*)(st + 2 * Cfg::NT * TILE) + 4 * half;
+      const float* colb = (const float*)(st + Cfg::BIAS) + 4 * half;
       const uint32_t colbase = (uint32_t)cb + 4 * half;
       if (cend - cb >= 128) {
         sweep_epilogue<PASS, true>(acc, gmin, colb, 128, colbase, scale, T, D.cout, dst, cnt);
@@ -393,6 +391,7 @@ __global__ __launch_bounds__(256, SweepCfg<MODE>::OCC) void gram_sweep_kernel(
         const int rem = (int)(cend - cb) - 4 * half;
         sweep_epilogue<PASS, false>(acc, gmin, colb, rem, colbase, scale, T, D.cout, dst, cnt);
       }
+      ++blk;
     }
 
     if constexpr (PASS == 0) {

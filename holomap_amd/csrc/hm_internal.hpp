@@ -19,8 +19,8 @@ struct DsDesc {
   const float* xr;       // raw input padded: n_pad x kp f32 (zero-filled)
   const float* op32;     // candidate-selection operand, f32: n_pad x kp
                          //   (euclidean: == xr; cosine: rows / ||row||)
-  const uint16_t* op16;  // same operand in bf16 (RNE), n_pad x kp, or null
-  const uint16_t* op16lo;  // bf16 of (operand - op16), n_pad x kp, or null
+  const uint16_t* op16;  // same operand in bf16 (RNE), n_pad x kp16, or null
+  const uint16_t* op16lo;  // bf16 of (operand - op16), n_pad x kp16, or null
   const float* colbias;  // n_pad: euclid ||x_j||^2 (f32); cosine 0 / -inf
   const double* rnorm;   // n: cosine sqrt(sum x^2) (f64, reference fold)
   const float* rstat;    // n x HM_NSTAT per-row bound statistics
@@ -28,7 +28,6 @@ struct DsDesc {
   uint64_t* cand;        // candidate keys: n x nlist x cout
   uint64_t* ctau;        // list thresholds: n x nlist
   float* thr;            // sweep pass-0 thresholds: n x nsplit
-  const uint64_t* zero16;  // >= 16 zero bytes (sweep copies beyond the padded width)
   uint32_t* knn_idx;     // n x k (output)
   float* knn_dist;       // n x k (output)
   int64_t n;             // rows
@@ -36,7 +35,8 @@ struct DsDesc {
   int64_t row_off;       // first global row id
   int64_t edge_off;      // first global edge id (= sum of n*k before)
   int32_t d;             // features
-  int32_t kp;            // padded feature stride (elements)
+  int32_t kp;            // padded feature stride (elements, multiple of 32)
+  int32_t kp16;          // op16 / op16lo row stride (multiple of 64: 128-byte rows)
   int32_t k;             // neighbours
   int32_t metric;        // 0 euclidean, 1 cosine
   int32_t C;             // candidates targeted per (row, split)
@@ -46,11 +46,9 @@ struct DsDesc {
   int32_t cout;          // key slots per list
 };
 
-// Workgroups of the candidate sweep per CU (its LDS ring is split between
-// them).
-#ifndef HM_SWEEP_OCC
-#define HM_SWEEP_OCC 2
-#endif
+// Query rows per work item of the candidate sweep (hm_gram.hip): 256 in the
+// bf16 and fp32 modes, 128 in bf16x3.
+__host__ __device__ constexpr int sweep_rows(int mode) { return mode == 2 ? 128 : 256; }
 
 // Number of float statistics kept per row for the candidate error bound.
 #define HM_NSTAT 8

@@ -72,8 +72,12 @@ __global__ __launch_bounds__(256) void prep_kernel(
   const int kp = D.kp, d = D.d;
   float* xr = const_cast<float*>(D.xr) + i * kp;
   float* op32 = const_cast<float*>(D.op32) + i * kp;
-  uint16_t* op16 = D.op16 ? const_cast<uint16_t*>(D.op16) + i * kp : nullptr;
-  uint16_t* op16lo = D.op16lo ? const_cast<uint16_t*>(D.op16lo) + i * kp : nullptr;
+  uint16_t* op16 = D.op16 ? const_cast<uint16_t*>(D.op16) + i * D.kp16 : nullptr;
+  uint16_t* op16lo = D.op16lo ? const_cast<uint16_t*>(D.op16lo) + i * D.kp16 : nullptr;
+  for (int f = kp + lane; f < D.kp16; f += 64) {  // bf16 rows are padded further
+    if (op16) op16[f] = 0;
+    if (op16lo) op16lo[f] = 0;
+  }
   const bool cosine = D.metric == 1;
   const bool real = i < D.n;
   double rn = 0.0;

@@ -88,7 +88,7 @@ constexpr int kRadixTile = 4096;
 
 struct DsHost {
   int64_t n, n_pad, row_off, edge_off;
-  int d, kp, k, metric, C, nsplit, nlist, cout;
+  int d, kp, kp16, k, metric, C, nsplit, nlist, cout;
 };
 
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -121,7 +121,6 @@ struct hm_job {
         *wval = nullptr, *vals = nullptr;
   uint16_t *op16 = nullptr, *op16lo = nullptr;
   double *rnorm = nullptr, *sigma64 = nullptr, *target = nullptr;
-  uint64_t *zero16 = nullptr;  // 512 zero bytes: sweep source beyond the padded width
   uint64_t *cand = nullptr, *ctau = nullptr, *xpart = nullptr, *cntu = nullptr,
            *indptr_g = nullptr, *total_nnz = nullptr, *indptr_l = nullptr, *sc64 = nullptr;
   uint32_t *knn_idx = nullptr, *flag = nullptr, *flagpos = nullptr, *list = nullptr,
@@ -175,8 +174,8 @@ int gram_mode(int precision) {
   }
 }
 
-int choose_nsplit(int64_t n, int C, int cus, int64_t other_items) {
-  const int64_t rb = cdiv(n, 128);
+int choose_nsplit(int64_t n, int qw, int C, int cus, int64_t other_items) {
+  const int64_t rb = cdiv(n, qw);
   int best = 1;
   double bestc = 1e300;
   const int smax = std::max(1, std::min<int>(8, std::min<int>(512 / std::max(C, 1), (int)(n / 256))));
@@ -190,8 +189,8 @@ int choose_nsplit(int64_t n, int C, int cus, int64_t other_items) {
 
 struct Plan {
   std::vector<DsHost> dsh;
-  int64_t total_rows = 0, total_pad = 0, total_edges = 0, sum_xr = 0, sum_cos = 0, sum_cand = 0,
-          fb_cap = 0;
+  int64_t total_rows = 0, total_pad = 0, total_edges = 0, sum_xr = 0, sum_x16 = 0, sum_cos = 0,
+          sum_cand = 0, fb_cap = 0;
   int kmax = 0, cmax = 0, xs = 1;
 };
 
@@ -207,6 +206,7 @@ Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_
     h.k = (int)k[i];
     h.metric = metric ? metric[i] : 0;
     h.kp = (int)cdiv(h.d, 32) * 32;
+    h.kp16 = (int)cdiv(h.d, 64) * 64;
     h.n_pad = cdiv(h.n, 128) * 128;
     int margin = opt.margin < 0 ? 2 * h.k : opt.margin;
     int C = h.k + margin;
@@ -219,6 +219,7 @@ Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_
     P.total_pad += h.n_pad;
     P.total_edges += h.n * h.k;
     P.sum_xr += h.n_pad * h.kp;
+    P.sum_x16 += h.n_pad * h.kp16;
     if (h.metric == 1) P.sum_cos += h.n_pad * h.kp;
     P.kmax = std::max(P.kmax, h.k);
     P.cmax = std::max(P.cmax, C);
@@ -227,8 +228,9 @@ Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_
   // sweep splits: a lone dataset (or a small batch) splits its columns so
   // the launch has enough work items for every CU
   const int cout = 48;  // collect-list slots per (row, split, half)
+  const int qw = sweep_rows(gram_mode(opt.precision));
   int64_t rb_total = 0;
-  for (auto& h : P.dsh) rb_total += cdiv(h.n, 128);
+  for (auto& h : P.dsh) rb_total += cdiv(h.n, qw);
   for (auto& h : P.dsh) {
     const int smax = std::max(1, 256 / cout);  // re-rank holds <= 512 keys per row
     if (opt.nsplit > 0) {
@@ -236,7 +238,7 @@ Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_
     } else if (rb_total >= 2 * cus) {
       h.nsplit = 1;
     } else {
-      h.nsplit = std::min(smax, choose_nsplit(h.n, h.C, cus, rb_total - cdiv(h.n, 128)));
+      h.nsplit = std::min(smax, choose_nsplit(h.n, qw, h.C, cus, rb_total - cdiv(h.n, qw)));
     }
     // the sweep stores column offsets within a split as u16
     h.nsplit = std::max<int>(h.nsplit, (int)cdiv(h.n, 65536));
@@ -258,8 +260,8 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
   b += sum_nd * 4;                                   // x
   if (stages & HM_STAGE_KNN) {
     b += P.sum_xr * 4 + P.sum_cos * 4;               // xr, op32 (cosine)
-    if (mode == 0) b += P.sum_xr * 2;                // op16
-    if (mode == 2) b += P.sum_xr * 4;                // op16 + op16lo
+    if (mode == 0) b += P.sum_x16 * 2;               // op16
+    if (mode == 2) b += P.sum_x16 * 4;               // op16 + op16lo
     b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + R * 12;
     b += P.sum_cand * 8 + R * 16 * 8 + R * 8 * 4;    // cand, ctau, thr
     b += R * 12 + P.fb_cap * 16 + R * P.xs * P.kmax * 8;
@@ -333,11 +335,11 @@ hm_status radix_sort(hm_job* J, const uint32_t* kin, const uint32_t* vin, int64_
   return HM_OK;
 }
 
-// Persistent grid of the candidate sweep: HM_SWEEP_OCC workgroups per CU
-// (one in bf16x3 mode), at most one per item.
+// Persistent grid of the candidate sweep: one workgroup per CU, at most one
+// per item.
 int64_t sweep_grid(const hm_ctx* ctx, int mode, int64_t items) {
-  const int64_t occ = mode == 2 ? 1 : HM_SWEEP_OCC;
-  return std::max<int64_t>(1, std::min<int64_t>(items, occ * ctx->num_cus));
+  (void)mode;
+  return std::max<int64_t>(1, std::min<int64_t>(items, ctx->num_cus));
 }
 
 unsigned grid_for(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, cdiv(n, per)); }
@@ -346,7 +348,8 @@ hm_status launch_sweep(hm_job* J) {
   // grid and item order go together (see the item permutation in hm_job_create)
   const unsigned g = (unsigned)sweep_grid(J->ctx, J->mode, J->n_items_l1);
 #define HM_GRAM(M, P)                                                                        \
-  hipLaunchKernelGGL((gram_sweep_kernel<M, P>), dim3(g), dim3(256), 0, J->stream, J->d_desc, \
+  hipLaunchKernelGGL((gram_sweep_kernel<M, P>), dim3(g), dim3(sweep_rows(M) / 32 * 64), 0,     \
+                     J->stream, J->d_desc,                                                    \
                      (const WorkItem*)J->items_l1, (const int*)nullptr, J->n_items_l1,        \
                      (const uint32_t*)nullptr)
   switch (J->mode) {
@@ -631,8 +634,8 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
   if (knn) {
     HM_A(J->xr, P.sum_xr);
     if (P.sum_cos > 0) HM_A(J->op32c, P.sum_cos);
-    if (J->mode == 0 || J->mode == 2) HM_A(J->op16, P.sum_xr);
-    if (J->mode == 2) HM_A(J->op16lo, P.sum_xr);
+    if (J->mode == 0 || J->mode == 2) HM_A(J->op16, P.sum_x16);
+    if (J->mode == 2) HM_A(J->op16lo, P.sum_x16);
     HM_A(J->colbias, P.total_pad);
     HM_A(J->rstat, R * HM_NSTAT);
     HM_A(J->smax, (int64_t)nds * HM_NSTAT);
@@ -645,9 +648,6 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     int64_t sum_thr = 0;
     for (auto& h : P.dsh) sum_thr += h.n * h.nsplit;
     HM_A(J->thr, sum_thr);
-    HM_A(J->zero16, 64);
-    if (hipMemset(J->zero16, 0, 64 * sizeof(uint64_t)) != hipSuccess)
-      return bail(fail(ctx, HM_EDEVICE, "hipMemset failed"));
     HM_A(J->flag, R);
     HM_A(J->flagpos, R);
     HM_A(J->list, R);
@@ -686,7 +686,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
 
   // descriptors
   J->hdesc.resize(nds);
-  int64_t off_x = 0, off_xr = 0, off_cos = 0, off_pad = 0, off_cand = 0, off_tau = 0, off_thr = 0;
+  int64_t off_x16 = 0, off_x = 0, off_xr = 0, off_cos = 0, off_pad = 0, off_cand = 0, off_tau = 0, off_thr = 0;
   std::vector<int64_t> pad_off_h(nds + 1, 0);
   std::vector<double> target_h(nds);
   J->iptr_off_h.resize(nds);
@@ -699,8 +699,8 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
       D.xr = J->xr + off_xr;
       if (h.metric == 1) { D.op32 = J->op32c + off_cos; off_cos += h.n_pad * h.kp; }
       else D.op32 = D.xr;
-      D.op16 = J->op16 ? J->op16 + off_xr : nullptr;
-      D.op16lo = J->op16lo ? J->op16lo + off_xr : nullptr;
+      D.op16 = J->op16 ? J->op16 + off_x16 : nullptr;
+      D.op16lo = J->op16lo ? J->op16lo + off_x16 : nullptr;
       D.colbias = J->colbias + off_pad;
       D.rnorm = J->rnorm + h.row_off;
       D.rstat = J->rstat + h.row_off * HM_NSTAT;
@@ -708,7 +708,6 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
       D.cand = J->cand + off_cand;
       D.ctau = J->ctau + off_tau;
       D.thr = J->thr + off_thr;
-      D.zero16 = J->zero16;
     }
     D.knn_idx = J->knn_idx + h.edge_off;
     D.knn_dist = J->knn_dist + h.edge_off;
@@ -718,6 +717,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     D.edge_off = h.edge_off;
     D.d = h.d;
     D.kp = h.kp;
+    D.kp16 = h.kp16;
     D.k = h.k;
     D.metric = h.metric;
     D.C = h.C;
@@ -727,6 +727,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     pad_off_h[i] = off_pad;
     off_x += h.n * h.d;
     off_xr += h.n_pad * h.kp;
+    off_x16 += h.n_pad * h.kp16;
     off_pad += h.n_pad;
     off_cand += h.n * h.nlist * h.cout;
     off_tau += h.n * h.nlist;
@@ -747,12 +748,13 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
   // sweep work items: (dataset, split, 128-row block), longest first
   if (knn && J->mode >= 0) {
     std::vector<std::pair<int64_t, WorkItem>> it;
+    const int qw = sweep_rows(J->mode);
     for (uint32_t i = 0; i < nds; ++i) {
       const DsHost& h = P.dsh[i];
       const int64_t cost = cdiv(h.n, h.nsplit) * h.kp;
-      for (int64_t rb = 0; rb < h.n; rb += 128)
+      for (int64_t rb = 0; rb < h.n; rb += qw)
         for (int s = 0; s < h.nsplit; ++s)
-          it.push_back({cost, WorkItem{(int)i, s, (int)rb, (int)std::min<int64_t>(128, h.n - rb)}});
+          it.push_back({cost, WorkItem{(int)i, s, (int)rb, (int)std::min<int64_t>(qw, h.n - rb)}});
     }
     std::stable_sort(it.begin(), it.end(),
                      [](const auto& a, const auto& b) { return a.first > b.first; });
