@@ -103,6 +103,35 @@ def cpu_baseline(x, d, k, metric, res, budget_s=12.0):
                       f"({t_rest:.2f} s); pts/s = 1/(t_knn/{m} + t_rest/{n})"}
 
 
+def sweep_traffic(workload_name, precision):
+    """HBM bytes per sweep launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload_name}_{precision}_gram_sweep_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def target_check(ctx, steps):
+    """The N=50,000 envelope workload: BF16 vs FP32 GPU path and the CPU port."""
+    x, d, k, metric, desc = workload("c3")
+    n = x.shape[0]
+    dt, stage, cnt, res = run_path(ctx, x, d, k, metric, "bf16", steps, 2)
+    dt32, st32, c32, res32 = run_path(ctx, x, d, k, metric, "fp32", max(3, steps // 2), 1)
+    same = bool(np.array_equal(res32.knn.indices, res.knn.indices) and
+                np.array_equal(res32.graph.vals.view(np.uint32), res.graph.vals.view(np.uint32)))
+    cb = cpu_baseline(x, d, k, metric, res)
+    return {"workload": f"c3: {desc}", "bf16_ms_per_step": dt * 1e3, "bf16_pts_per_s": n / dt,
+            "fp32_ms_per_step": dt32 * 1e3, "speedup_vs_fp32": dt32 / dt,
+            "identical_graph": same, "fallback_rows": cnt["fallback_rows"],
+            "stage_ms": {kk: round(v, 4) for kk, v in stage.items()},
+            "cpu_baseline": cb, "speedup_vs_cpu": (n / dt) / cb["value"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,7 +141,21 @@ def main():
     ap.add_argument("--precision", default="bf16")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-target", action="store_true",
+                    help="skip the N=50,000 (d=784) BF16 vs FP32 vs CPU check")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain elementwise copy of MIB MiB (reads and writes that "
+                         "many bytes once) to calibrate the HBM byte counters, then exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe * (1 << 20)}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the kNN-graph path here runs on a single device "
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
@@ -143,14 +186,18 @@ def main():
         "fallback_rows": cnt["fallback_rows"],
         "nnz": cnt["nnz"],
     }
-    # roofline of the dominant kernel (the candidate sweep: MFMA-bound)
+    # roofline of the dominant kernel, the candidate sweep (MFMA-bound). It is
+    # launched twice per step (threshold pass, collect pass); each launch does
+    # the algorithmic 2*N*N*d flops of one Gram matrix. The sweep stage time is
+    # HIP events on the job's stream around both launches.
     flops = 2.0 * n * n * d
-    t_sweep = stage["sweep"] * 1e-3
+    t_launch = stage["sweep"] * 1e-3 / 2
     peak = PEAK.get(args.precision, PEAK["bf16"])
-    ach = flops / t_sweep if t_sweep > 0 else 0.0
-    out["roofline"] = {"bound": "mfma", "kernel": "gram_topc_kernel", "achieved": ach / 1e12,
-                       "peak": peak / 1e12, "unit": "TFLOP/s", "frac": ach / peak,
-                       "traffic": None, "flops_per_launch": flops}
+    ach = flops / t_launch if t_launch > 0 else 0.0
+    out["roofline"] = {"bound": "mfma", "kernel": "hm::gram_sweep_kernel (2 launches/step)",
+                       "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                       "frac": ach / peak, "traffic": sweep_traffic(args.workload, args.precision),
+                       "flops_per_launch": flops, "launch_ms": t_launch * 1e3}
     if not args.no_fp32 and args.precision != "fp32":
         dt32, st32, c32, res32 = run_path(ctx, x, d, k, metric, "fp32", max(3, args.steps // 2), 2)
         same = bool(np.array_equal(res32.knn.indices, res.knn.indices) and
@@ -163,6 +210,8 @@ def main():
         cb = cpu_baseline(x, d, k, metric, res)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = out["value"] / cb["value"]
+    if not args.no_target and args.workload != "c3":
+        out["target_n50k"] = target_check(ctx, 10)
     print(json.dumps(out))
 
 

@@ -6,7 +6,8 @@
 //    and each row keeps its k smallest (f32 dist, idx) keys -- the order of
 //    src/knn.rs:48, since dist >= +0 makes the f32 bit pattern monotone.
 //    Serves the exact precision mode and the fallback for uncertified rows.
-//  * exact_merge_kernel -- merges the per-split top-k lists of a row.
+//  * exact_group_merge_kernel, exact_merge_kernel -- merge the per-split
+//    top-k lists of a row (two levels when there are many splits).
 //  * rerank_kernel      -- after the MFMA candidate sweep: recomputes the
 //    plausible candidates exactly, takes the top-k and certifies that no
 //    column outside the candidate set can enter it.
@@ -231,18 +232,59 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(
   }
 }
 
-// Merge the xs sorted per-split lists of every listed query; write knn rows.
-// One wave per list position p < *count (or per local row in identity mode).
-__global__ __launch_bounds__(256) void exact_merge_kernel(
+// Merging the xs sorted per-split lists of every listed query (or local
+// row in identity mode) into its knn row, in two levels so a wave never
+// sorts more than 1024 keys: with xs * k > 1024, exact_group_merge_kernel
+// first reduces each group of G consecutive splits (G * k <= 1024) to its k
+// smallest keys, written over the group's first list; exact_merge_kernel
+// then merges the remaining lists (every G-th, or all when xs * k <= 1024).
+__global__ __launch_bounds__(256) void exact_group_merge_kernel(
     const DsDesc* __restrict__ ds_arr, int nds, const uint32_t* __restrict__ qlist,
     const int* __restrict__ count_dev, int64_t count_host, const int* __restrict__ xs_dev,
-    int xs_host, int kmax, const uint64_t* __restrict__ xpart) {
+    int xs_host, int kmax, int G, uint64_t* __restrict__ xpart) {
   __shared__ uint64_t s[4][1024];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t count = count_dev ? (int64_t)*count_dev : count_host;
   const int xs = xs_dev ? *xs_dev : xs_host;
+  const int ngrp = (xs + G - 1) / G;
+  if (xs * kmax <= 1024) return;  // a single merge suffices
+  for (int64_t u = (int64_t)blockIdx.x * 4 + wave; u < count * ngrp; u += (int64_t)gridDim.x * 4) {
+    const int64_t p = u / ngrp;
+    const int grp = (int)(u - p * ngrp);
+    const int64_t grow = qlist ? (int64_t)qlist[p] : p;
+    int lo = 0, hi = nds - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (ds_arr[mid].row_off <= grow) lo = mid; else hi = mid - 1;
+    }
+    const int k = ds_arr[lo].k;
+    const int s0 = grp * G, ns = xs - s0 < G ? xs - s0 : G;
+    uint64_t* src = xpart + ((size_t)p * xs + s0) * kmax;
+    uint64_t* a = s[wave];
+    int len = 64;
+    while (len < ns * k) len <<= 1;
+    for (int e = lane; e < len; e += 64) {
+      const int sp = e / k, t = e % k;
+      a[e] = (sp < ns) ? src[(size_t)sp * kmax + t] : ~0ull;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    wave_sort_lds(a, len);
+    for (int t = lane; t < k; t += 64) src[t] = a[t];
+  }
+}
+
+__global__ __launch_bounds__(256) void exact_merge_kernel(
+    const DsDesc* __restrict__ ds_arr, int nds, const uint32_t* __restrict__ qlist,
+    const int* __restrict__ count_dev, int64_t count_host, const int* __restrict__ xs_dev,
+    int xs_host, int kmax, int G, const uint64_t* __restrict__ xpart) {
+  __shared__ uint64_t s[4][1024];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t count = count_dev ? (int64_t)*count_dev : count_host;
+  const int xs = xs_dev ? *xs_dev : xs_host;
+  const int stride = xs * kmax <= 1024 ? 1 : G;  // split slots holding a list to merge
+  const int nl = (xs + stride - 1) / stride;
   for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < count; p += (int64_t)gridDim.x * 4) {
-    int64_t grow = qlist ? (int64_t)qlist[p] : p;
+    const int64_t grow = qlist ? (int64_t)qlist[p] : p;
     int lo = 0, hi = nds - 1;
     while (lo < hi) {
       int mid = (lo + hi + 1) >> 1;
@@ -251,31 +293,16 @@ __global__ __launch_bounds__(256) void exact_merge_kernel(
     const DsDesc& D = ds_arr[lo];
     const int64_t row = grow - D.row_off;
     const int k = D.k;
-    const int tot = xs * k;
     const uint64_t* src = xpart + (size_t)p * xs * kmax;
     uint64_t* a = s[wave];
-    if (tot <= 1024) {
-      int len = 64;
-      while (len < tot) len <<= 1;
-      for (int e = lane; e < len; e += 64) {
-        const int sp = e / k, t = e % k;
-        a[e] = (sp < xs) ? src[(size_t)sp * kmax + t] : ~0ull;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      wave_sort_lds(a, len);
-    } else {
-      // rounds of 960 new keys on top of the 64 smallest so far (k <= 64)
-      a[lane] = ~0ull;
-      for (int base = 0; base < tot; base += 960) {
-        for (int e = 64 + lane; e < 1024; e += 64) {
-          const int g = base + e - 64;
-          a[e] = g < tot ? src[(size_t)(g / k) * kmax + g % k] : ~0ull;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        wave_sort_lds(a, 1024);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      }
+    int len = 64;
+    while (len < nl * k) len <<= 1;
+    for (int e = lane; e < len; e += 64) {
+      const int l = e / k, t = e % k;
+      a[e] = (l < nl) ? src[(size_t)l * stride * kmax + t] : ~0ull;
     }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    wave_sort_lds(a, len);
     for (int t = lane; t < k; t += 64) {
       const uint64_t key = a[t];
       D.knn_idx[row * k + t] = (uint32_t)key;
@@ -469,14 +496,13 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DsDesc* __restrict__ 
 // dataset count is small.
 // The column split count starts at the planned xs and grows when few rows are
 // flagged (a handful of rows would otherwise run on a handful of CUs), up to
-// XS_MAX and within the item and partial-list capacities, which hold
-// item_cap items and row_cap * xs lists.
-#define XS_MAX 512
+// xs_cap (which keeps the merge at two levels) and within the item and
+// partial-list capacities, which hold item_cap items and row_cap * xs lists.
 __global__ void fallback_items_kernel(const DsDesc* __restrict__ ds_arr, int nds,
                                       const uint32_t* __restrict__ list, const int* __restrict__ count,
-                                      int xs, int64_t item_cap, int64_t row_cap, int target,
-                                      WorkItem* __restrict__ items, int* __restrict__ nitems,
-                                      int* __restrict__ xs_out) {
+                                      int xs, int xs_cap, int64_t item_cap, int64_t row_cap,
+                                      int target, WorkItem* __restrict__ items,
+                                      int* __restrict__ nitems, int* __restrict__ xs_out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int cnt = *count;
   int64_t nqb = 0;
@@ -493,7 +519,7 @@ __global__ void fallback_items_kernel(const DsDesc* __restrict__ ds_arr, int nds
   int64_t xd = xs;
   if (nqb > 0 && nqb * xs < target) {
     int64_t g = target / nqb;
-    g = g < XS_MAX ? g : XS_MAX;
+    g = g < xs_cap ? g : xs_cap;
     g = g < item_cap / nqb ? g : item_cap / nqb;
     g = g < row_cap * xs / cnt ? g : row_cap * xs / cnt;
     xd = g > xd ? g : xd;

@@ -24,10 +24,12 @@ template <int MODE, int PASS>
 __global__ void gram_sweep_kernel(const DsDesc*, const WorkItem*, const int*, int, const uint32_t*);
 __global__ void exact_scan_kernel(const DsDesc*, const WorkItem*, const int*, int,
                                   const uint32_t*, const int*, int, int, uint64_t*);
+__global__ void exact_group_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t,
+                                         const int*, int, int, int, uint64_t*);
 __global__ void exact_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t,
-                                   const int*, int, int, const uint64_t*);
+                                   const int*, int, int, int, const uint64_t*);
 __global__ void rerank_kernel(const DsDesc*, int, int64_t, int, uint32_t*);
-__global__ void fallback_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int,
+__global__ void fallback_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int, int,
                                       int64_t, int64_t, int, WorkItem*, int*, int*);
 __global__ void smooth_knn_kernel(const DsDesc*, int, int64_t, const double*, int, float*, float*,
                                   double*, uint32_t*);
@@ -402,16 +404,25 @@ hm_status enqueue_pipeline(hm_job* J) {
     hipLaunchKernelGGL(compact_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream,
                        (const uint32_t*)J->flag, (const uint32_t*)J->flagpos, R, J->list, J->count);
     const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(J->fb_cap, 2 * cus));
+    // merge groups: G splits of k keys fit one 1024-key sort; two merge
+    // levels cover up to G * G splits
+    int kpow = 1;
+    while (kpow < J->kmax) kpow <<= 1;
+    const int G = std::max(1, 1024 / kpow);
+    const int xs_cap = std::max(J->xs, std::min(512, G * G));
     hipLaunchKernelGGL(fallback_items_kernel, dim3(1), dim3(64), 0, J->stream, J->d_desc, nds,
-                       (const uint32_t*)J->list, (const int*)J->count, J->xs, (int64_t)J->fb_cap,
-                       R, (int)gx, J->items_fb, J->nitems_fb, J->xs_fb);
+                       (const uint32_t*)J->list, (const int*)J->count, J->xs, xs_cap,
+                       (int64_t)J->fb_cap, R, (int)gx, J->items_fb, J->nitems_fb, J->xs_fb);
     hipLaunchKernelGGL(exact_scan_kernel, dim3(gx), dim3(256), 0, J->stream, J->d_desc,
                        (const WorkItem*)J->items_fb, (const int*)J->nitems_fb, 0,
                        (const uint32_t*)J->list, (const int*)J->xs_fb, J->xs, J->kmax, J->xpart);
     const unsigned gm = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(R, 4), 8 * cus));
+    hipLaunchKernelGGL(exact_group_merge_kernel, dim3(gm), dim3(256), 0, J->stream, J->d_desc,
+                       nds, (const uint32_t*)J->list, (const int*)J->count, (int64_t)0,
+                       (const int*)J->xs_fb, J->xs, J->kmax, G, J->xpart);
     hipLaunchKernelGGL(exact_merge_kernel, dim3(gm), dim3(256), 0, J->stream, J->d_desc, nds,
                        (const uint32_t*)J->list, (const int*)J->count, (int64_t)0,
-                       (const int*)J->xs_fb, J->xs, J->kmax, (const uint64_t*)J->xpart);
+                       (const int*)J->xs_fb, J->xs, J->kmax, G, (const uint64_t*)J->xpart);
     HM_LAUNCH(ctx);
     mark(J, 4);
   } else {
