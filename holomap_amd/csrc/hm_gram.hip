@@ -419,11 +419,11 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
       if (myvalid && half == 0) D.thr[(size_t)myrow * D.nsplit + W.split] = unordered_f32(lo);
     } else {
       if (myvalid) {
-        for (int e = cnt; e < D.cout; ++e) dst[e] = ~0ull;
         // every column not listed has key > T; an overflowing list reports 0
         // (no threshold) so the row is left to the exact scan
         D.ctau[(size_t)myrow * D.nlist + list] =
             cnt > D.cout ? 0ull : (((uint64_t)ordered_u32(T) << 32) | 0xffffffffull);
+        D.ccnt[(size_t)myrow * D.nlist + list] = (uint32_t)(cnt < D.cout ? cnt : D.cout);
       }
     }
   }

@@ -27,6 +27,7 @@ struct DsDesc {
   const float* smax;     // HM_NSTAT maxima over rows (per dataset)
   uint64_t* cand;        // candidate keys: n x nlist x cout
   uint64_t* ctau;        // list thresholds: n x nlist
+  uint32_t* ccnt;        // keys stored per list (<= cout): n x nlist
   float* thr;            // sweep pass-0 thresholds: n x nsplit
   uint32_t* knn_idx;     // n x k (output)
   float* knn_dist;       // n x k (output)

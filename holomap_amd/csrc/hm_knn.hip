@@ -364,7 +364,11 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DsDesc* __restrict__ 
   while (len < m) len <<= 1;
   uint64_t* a = cs[wave];
   const uint64_t* src = D.cand + (size_t)i * L * CO;
-  for (int e = lane; e < len; e += 64) a[e] = e < m ? src[e] : ~0ull;
+  const uint32_t* cnt = D.ccnt + (size_t)i * L;
+  for (int e = lane; e < len; e += 64) {
+    const int l = e / CO;
+    a[e] = (e < m && (uint32_t)(e - l * CO) < cnt[l]) ? src[e] : ~0ull;
+  }
   uint64_t taumin = ~0ull;
   for (int s = 0; s < L; ++s) {
     const uint64_t t = D.ctau[(size_t)i * L + s];

@@ -125,6 +125,7 @@ struct hm_job {
   double *rnorm = nullptr, *sigma64 = nullptr, *target = nullptr;
   uint64_t *cand = nullptr, *ctau = nullptr, *xpart = nullptr, *cntu = nullptr,
            *indptr_g = nullptr, *total_nnz = nullptr, *indptr_l = nullptr, *sc64 = nullptr;
+  uint32_t *ccnt = nullptr;
   uint32_t *knn_idx = nullptr, *flag = nullptr, *flagpos = nullptr, *list = nullptr,
            *zrow = nullptr, *key1 = nullptr, *pay1 = nullptr, *srcrow = nullptr, *k1s = nullptr,
            *p1s = nullptr, *key2 = nullptr, *k2s = nullptr, *p2s = nullptr, *tmpk = nullptr,
@@ -265,7 +266,7 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
     if (mode == 0) b += P.sum_x16 * 2;               // op16
     if (mode == 2) b += P.sum_x16 * 4;               // op16 + op16lo
     b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + R * 12;
-    b += P.sum_cand * 8 + R * 16 * 8 + R * 8 * 4;    // cand, ctau, thr
+    b += P.sum_cand * 8 + R * 16 * 12 + R * 8 * 4;   // cand, ctau + ccnt, thr
     b += R * 12 + P.fb_cap * 16 + R * P.xs * P.kmax * 8;
   }
   b += E * 8;                                        // knn idx + dist
@@ -656,6 +657,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     int64_t sum_tau = 0;
     for (auto& h : P.dsh) sum_tau += h.n * h.nlist;
     HM_A(J->ctau, sum_tau);
+    HM_A(J->ccnt, sum_tau);
     int64_t sum_thr = 0;
     for (auto& h : P.dsh) sum_thr += h.n * h.nsplit;
     HM_A(J->thr, sum_thr);
@@ -718,6 +720,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
       D.smax = J->smax + (int64_t)i * HM_NSTAT;
       D.cand = J->cand + off_cand;
       D.ctau = J->ctau + off_tau;
+      D.ccnt = J->ccnt + off_tau;
       D.thr = J->thr + off_thr;
     }
     D.knn_idx = J->knn_idx + h.edge_off;
