@@ -11,7 +11,11 @@
  *   hmo_cosine           src/metric.rs:58-70
  *   hmo_exact_knn        src/knn.rs:22-55
  *   hmo_expf             src/fmath.rs:119-122 -> libm 0.2.16 expf
- *                        (FreeBSD msun e_expf.c method, restated)
+ *                        (FreeBSD msun e_expf.c method, restated).
+ *                        Pinned to < 1 ulp against the correctly rounded
+ *                        exp and through the golden sigmas; bit-level
+ *                        agreement with libm 0.2.16 itself (a dependency
+ *                        absent here) is parity unpinned.
  *   hmo_smooth_knn       src/smooth_knn.rs:12-84
  *   hmo_fuzzy            src/fuzzy.rs:25-85
  *   hmo_csr_indptr       src/sparse.rs:21-35
