@@ -132,6 +132,79 @@ def target_check(ctx, steps):
             "cpu_baseline": cb, "speedup_vs_cpu": (n / dt) / cb["value"]}
 
 
+def batch_bench(ctx, steps):
+    """C4: 256 datasets (N log-uniform in [200, 8000], d in {32, 64, 128,
+    256}, k in {10, 15, 30}, euclidean/cosine) in one job: one launch per
+    stage for the whole batch, replayed from a captured hipGraph."""
+    import holomap_amd as H
+    from holomap_amd import datasets as DS
+    sets = DS.c4()
+    job = H.Job(ctx, [s.n for s in sets], [s.d for s in sets], [s.k for s in sets],
+                [s.metric for s in sets], use_graph=True)
+    for i, s in enumerate(sets):
+        job.upload(i, s.data)
+    job.run()
+    job.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        job.run()
+    job.sync()
+    dt = (time.perf_counter() - t0) / steps
+    fb = job.counters()["fallback_rows"]
+    job.close()
+    rows = sum(s.n for s in sets)
+    return {"workload": "c4: 256 datasets, N log-uniform [200,8000], d in {32,64,128,256}, "
+                        "k in {10,15,30}, metric 50/50, seed 11",
+            "datasets": len(sets), "rows": rows, "ms_per_step": dt * 1e3,
+            "graphs_per_s": len(sets) / dt, "pts_per_s": rows / dt, "fallback_rows": fb}
+
+
+def serve_bench(ctx, lambdas, n_req, pool=16, seed=0):
+    """Serving loop (hm_server_*): Poisson arrivals at each rate, requests of
+    the mixed C5 shapes (N log-uniform in [500, 50000], d in {64, 384, 768},
+    k in {15, 30}, Gaussian-mixture euclidean or unit-normalised cosine),
+    drawn from a pool of `pool` seeded datasets. Latency = arrival -> result
+    on the host (fetch done)."""
+    import holomap_amd as H
+    from holomap_amd import datasets as DS
+    reqs = [DS.c5_request(1000 + i) for i in range(pool)]
+    rng = np.random.default_rng(seed)
+    out = []
+    for lam in lambdas:
+        srv = H.Server(ctx, max_jobs=4, max_batch=16)
+        arr = np.cumsum(rng.exponential(1.0 / lam, n_req))
+        pick = rng.integers(0, pool, n_req)
+        lat, pend, pts = [], {}, 0
+        i = 0
+        t0 = time.perf_counter()
+        while len(lat) < n_req:
+            now = time.perf_counter() - t0
+            while i < n_req and arr[i] <= now:
+                r = reqs[pick[i]]
+                pend[srv.submit(r.data, r.d, r.k, r.metric)] = (arr[i], r.n)
+                i += 1
+            srv.progress()
+            for rid in list(pend):
+                if srv.state(rid) == srv.DONE:
+                    srv.fetch(rid)
+                    a, n = pend.pop(rid)
+                    lat.append(time.perf_counter() - t0 - a)
+                    pts += n
+            if not pend and i < n_req:
+                time.sleep(max(0.0, min(arr[i] - (time.perf_counter() - t0), 0.002)))
+        span = time.perf_counter() - t0
+        srv.close()
+        lat = np.array(lat)
+        out.append({"lambda_req_s": lam, "requests": n_req,
+                    "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                    "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                    "throughput_req_s": n_req / span, "throughput_pts_s": pts / span})
+    return {"workload": "c5 serving: Poisson arrivals, N log-uniform [500,50000], d in "
+                        "{64,384,768}, k in {15,30}, euclidean mixture / cosine unit rows; "
+                        f"{pool} pooled seeded datasets; max_jobs 4, max_batch 16",
+            "rates": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +216,12 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-target", action="store_true",
                     help="skip the N=50,000 (d=784) BF16 vs FP32 vs CPU check")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop figures")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-dataset batch figures")
+    ap.add_argument("--serve-rates", default="50,200",
+                    help="Poisson arrival rates (req/s) of the serving figures")
+    ap.add_argument("--serve-requests", type=int, default=120,
+                    help="requests per arrival rate")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB (reads and writes that "
                          "many bytes once) to calibrate the HBM byte counters, then exit")
@@ -212,6 +291,11 @@ def main():
         out["speedup_vs_cpu"] = out["value"] / cb["value"]
     if not args.no_target and args.workload != "c3":
         out["target_n50k"] = target_check(ctx, 10)
+    if not args.no_batch:
+        out["batch_c4"] = batch_bench(ctx, 10)
+    if not args.no_serve:
+        rates = [float(v) for v in args.serve_rates.split(",") if v]
+        out["serving"] = serve_bench(ctx, rates, args.serve_requests)
     print(json.dumps(out))
 
 

@@ -167,6 +167,16 @@ class Context:
             raise DeviceError(f"hm_ctx_create failed ({st}): no usable GPU")
         self.handle = h
 
+    @property
+    def budget(self) -> int:
+        """HBM budget in bytes (0 = none)."""
+        return int(_lib.load().hm_ctx_budget(self.handle))
+
+    @property
+    def in_use(self) -> int:
+        """Device bytes held by this context's live jobs."""
+        return int(_lib.load().hm_ctx_in_use(self.handle))
+
     @classmethod
     def default(cls) -> "Context":
         if cls._default is None:
@@ -199,6 +209,8 @@ class Context:
             raise InvalidParameter(msg)
         if st == _lib.HM_ENONFINITE:
             raise NonFiniteInput(-1)
+        if st == _lib.HM_EBUDGET:
+            raise MemoryError(msg)
         raise DeviceError(f"status {st}: {msg}")
 
 
@@ -340,6 +352,7 @@ class Job:
         ctx._check(st, n=int(self.n[0]), k=int(self.k[0]))
         self.handle = h
         self.stages = stages
+        self._keep = []
 
     @property
     def bytes(self) -> int:
@@ -349,7 +362,7 @@ class Job:
         x = np.ascontiguousarray(data, np.float32).reshape(-1)
         assert x.size == int(self.n[ds]) * int(self.d[ds])
         self.ctx._check(_lib.load().hm_job_upload(self.handle, ds, x.ctypes.data))
-        self._keep = x  # the copy is asynchronous; keep the source alive until sync
+        self._keep.append(x)  # the copy is asynchronous; keep the sources alive until sync
 
     def upload_device(self, ds: int, ptr: int) -> None:
         self.ctx._check(_lib.load().hm_job_upload_device(self.handle, ds, ctypes.c_void_p(ptr)))
@@ -359,7 +372,7 @@ class Job:
 
     def sync(self) -> None:
         self.ctx._check(_lib.load().hm_job_sync(self.handle))
-        self._keep = None
+        self._keep = []
 
     def done(self) -> bool:
         r = _lib.load().hm_job_done(self.handle)
@@ -376,7 +389,7 @@ class Job:
         a = np.zeros(_lib.HM_NCOUNTERS, np.uint64)
         self.ctx._check(_lib.load().hm_job_counters(self.handle, a.ctypes.data))
         return {"fallback_rows": int(a[0]), "nnz": int(a[1]), "candidates": int(a[2]),
-                "nsplit": int(a[3])}
+                "nsplit": int(a[3]), "escalated_rows": int(a[4])}
 
     def download(self, ds: int, with_sigma64: bool = False) -> GraphResult:
         n, k = int(self.n[ds]), int(self.k[ds])
@@ -406,6 +419,90 @@ class Job:
     def close(self):
         if getattr(self, "handle", None):
             _lib.load().hm_job_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- serving
+class Server:
+    """Fit-request serving loop (hm_server_*): requests queue in arrival
+    order; progress() retires finished jobs and admits the longest queue
+    prefix that fits max_batch / max_batch_rows and the context's free HBM
+    budget into a new job, with up to max_jobs jobs running at once on their
+    own streams. Results stay on the device until fetch()."""
+
+    QUEUED, RUNNING, DONE = 0, 1, 2
+
+    def __init__(self, ctx: Context, *, max_jobs=4, max_batch=32, max_batch_rows=1 << 20,
+                 precision="bf16", calib="ref_f32", margin=-1):
+        self.ctx = ctx
+        L = _lib.load()
+        cfg = _lib.ServerConfig()
+        L.hm_server_default_config(ctypes.byref(cfg))
+        cfg.max_jobs, cfg.max_batch, cfg.max_batch_rows = max_jobs, max_batch, max_batch_rows
+        cfg.opt = _opts(precision, calib, margin, 0, False, False)
+        h = ctypes.c_void_p()
+        ctx._check(L.hm_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self._shape = {}
+
+    def submit(self, data, n_features: int, k: int, metric="euclidean") -> int:
+        x = np.ascontiguousarray(data, np.float32).reshape(-1)
+        _validate(x, n_features, k)
+        rid = ctypes.c_uint64()
+        self.ctx._check(_lib.load().hm_server_submit(self.handle, x.ctypes.data,
+                                                     x.size // n_features, n_features, k,
+                                                     _metric(metric), ctypes.byref(rid)))
+        self._shape[rid.value] = (x.size // n_features, k)
+        return rid.value
+
+    def progress(self) -> int:
+        c = ctypes.c_uint32()
+        self.ctx._check(_lib.load().hm_server_progress(self.handle, ctypes.byref(c)))
+        return c.value
+
+    def state(self, rid: int) -> int:
+        return int(_lib.load().hm_server_state(self.handle, rid))
+
+    def pending(self) -> int:
+        return int(_lib.load().hm_server_pending(self.handle))
+
+    def fetch(self, rid: int) -> GraphResult:
+        """Results of a done request (raises its error for a failed one)."""
+        n, k = self._shape.pop(rid)
+        L = _lib.load()
+        st = self.state(rid)
+        if st < 0:
+            self.ctx._check(L.hm_server_fetch(self.handle, rid, None), n=n, k=k)
+        nnz = ctypes.c_uint64()
+        self.ctx._check(L.hm_server_result_info(self.handle, rid, None, None, ctypes.byref(nnz)))
+        m = int(nnz.value)
+        idx = np.empty((n, k), np.uint32)
+        dist = np.empty((n, k), np.float32)
+        sig = np.empty(n, np.float32)
+        rho = np.empty(n, np.float32)
+        rows = np.empty(max(m, 1), np.uint32)
+        cols = np.empty(max(m, 1), np.uint32)
+        vals = np.empty(max(m, 1), np.float32)
+        indptr = np.empty(n + 1, np.uint64)
+        o = _lib.GraphOut()
+        o.indices, o.dists = idx.ctypes.data, dist.ctypes.data
+        o.sigmas, o.rhos = sig.ctypes.data, rho.ctypes.data
+        o.rows, o.cols, o.vals = rows.ctypes.data, cols.ctypes.data, vals.ctypes.data
+        o.indptr = indptr.ctypes.data
+        o.cap = max(m, 1)
+        self.ctx._check(L.hm_server_fetch(self.handle, rid, ctypes.byref(o)))
+        g = FuzzyGraph(n, rows[:m].copy(), cols[:m].copy(), vals[:m].copy())
+        return GraphResult(Knn(k, idx, dist), SmoothKnn(sig, rho, None), g, indptr)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().hm_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -34,7 +34,7 @@ HM_STAGE_CALIB = 2
 HM_STAGE_FUZZY = 4
 HM_STAGE_ALL = 7
 HM_NTIMES = 8
-HM_NCOUNTERS = 4
+HM_NCOUNTERS = 5
 TIME_NAMES = ["prep", "sweep", "rerank", "fallback", "calib", "fuzzy", "total"]
 
 # every symbol include/holomap_gpu.h declares
@@ -45,7 +45,10 @@ EXPORTS = [
     "hm_job_destroy", "hm_job_bytes", "hm_job_estimate_bytes", "hm_job_upload",
     "hm_job_upload_device", "hm_job_set_knn", "hm_job_set_calib", "hm_job_run", "hm_job_sync",
     "hm_job_done", "hm_job_download", "hm_job_times", "hm_job_counters",
-    "hm_job_device_outputs", "hm_expf_array",
+    "hm_job_device_outputs", "hm_expf_array", "hm_ctx_budget", "hm_ctx_in_use",
+    "hm_server_default_config", "hm_server_create", "hm_server_destroy", "hm_server_submit",
+    "hm_server_progress", "hm_server_state", "hm_server_pending", "hm_server_result_info",
+    "hm_server_fetch",
 ]
 
 
@@ -63,6 +66,11 @@ class GraphOut(ctypes.Structure):
                 ("vals", ctypes.c_void_p), ("indptr", ctypes.c_void_p),
                 ("sigmas64", ctypes.c_void_p), ("cap", ctypes.c_uint64),
                 ("nnz", ctypes.c_uint64)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_jobs", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
+                ("max_batch_rows", ctypes.c_uint64), ("opt", Options)]
 
 
 _lib = None
@@ -107,6 +115,17 @@ def load():
         "hm_job_counters": (i32, [vp, vp]),
         "hm_job_device_outputs": (i32, [vp, u32, P(vp), P(vp), P(vp), P(vp), P(vp), P(vp)]),
         "hm_expf_array": (i32, [vp, vp, vp, u64]),
+        "hm_ctx_budget": (u64, [vp]),
+        "hm_ctx_in_use": (u64, [vp]),
+        "hm_server_default_config": (None, [P(ServerConfig)]),
+        "hm_server_create": (i32, [vp, P(ServerConfig), P(vp)]),
+        "hm_server_destroy": (None, [vp]),
+        "hm_server_submit": (i32, [vp, vp, u64, u32, u32, i32, P(u64)]),
+        "hm_server_progress": (i32, [vp, P(u32)]),
+        "hm_server_state": (ctypes.c_int, [vp, u64]),
+        "hm_server_pending": (u32, [vp]),
+        "hm_server_result_info": (i32, [vp, u64, P(u64), P(u32), P(u64)]),
+        "hm_server_fetch": (i32, [vp, u64, P(GraphOut)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -329,14 +329,11 @@ __device__ __forceinline__ float f32_round_up(double v) {
   return f;
 }
 
-__global__ __launch_bounds__(256) void rerank_kernel(const DsDesc* __restrict__ ds_arr, int nds,
-                                                     int64_t total_rows, int mode,
-                                                     uint32_t* __restrict__ flag) {
-  __shared__ uint64_t cs[4][512];
-  __shared__ uint64_t es[4][512];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t grow = (int64_t)blockIdx.x * 4 + wave;
-  if (grow >= total_rows) return;
+// One row (global id grow) by one wave; a, ex: the wave's 512-key LDS areas.
+__device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, int nds,
+                                           int64_t grow, int mode, uint32_t* __restrict__ flag,
+                                           uint64_t* a, uint64_t* ex) {
+  const int lane = threadIdx.x & 63;
   int lo = 0, hi = nds - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
@@ -362,7 +359,6 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DsDesc* __restrict__ 
   const int m = L * CO;
   int len = 64;
   while (len < m) len <<= 1;
-  uint64_t* a = cs[wave];
   const uint64_t* src = D.cand + (size_t)i * L * CO;
   const uint32_t* cnt = D.ccnt + (size_t)i * L;
   for (int e = lane; e < len; e += 64) {
@@ -401,6 +397,17 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DsDesc* __restrict__ 
                                   (double)st[ST_LN] * mx[ST_HN]);
     E = dot_err + acc_err;
     dotmax = (double)st[ST_XO] * mx[ST_XO] * 1.01;
+  } else if (mode == 1) {
+    // f32 operand w: ||w|| = XO (rounded up), its rounding error
+    // ||w_exact - w|| <= u/(1-u) ||w|| componentwise (whatever the job's
+    // primary mode stored in the XN/EN slots)
+    const double gam = 2.0 * kp * u;
+    const double xn = (double)st[ST_XO], mxn = (double)mx[ST_XO];
+    const double en = xn * u * 1.0001, men = mxn * u * 1.0001;
+    const double dot_err = en * (mxn + men) + xn * men;
+    const double acc_err = gam * xn * mxn;
+    E = dot_err + acc_err;
+    dotmax = xn * mxn * (1.0 + gam);
   } else {
     const double gam = 2.0 * kp * u;
     const double dot_err = (double)st[ST_EN] * ((double)mx[ST_XN] + mx[ST_EN]) +
@@ -446,7 +453,6 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DsDesc* __restrict__ 
   // ---- exact distances of the plausible prefix ----
   int elen = 64;
   while (elen < np) elen <<= 1;
-  uint64_t* ex = es[wave];
   const float* qa = D.xr + (size_t)i * kp;
   bool badid = false;
   for (int e = lane; e < elen; e += 64) {
@@ -495,9 +501,86 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DsDesc* __restrict__ 
   if (lane == 0) flag[grow] = cert ? 0u : 1u;
 }
 
+// All rows (rlist == nullptr: rows 0 .. total_rows-1, one per wave), or the
+// *rcount rows listed in rlist (global ids), grid-stride.
+__global__ __launch_bounds__(256) void rerank_kernel(const DsDesc* __restrict__ ds_arr, int nds,
+                                                     int64_t total_rows, int mode,
+                                                     uint32_t* __restrict__ flag,
+                                                     const uint32_t* __restrict__ rlist,
+                                                     const int* __restrict__ rcount) {
+  __shared__ uint64_t cs[4][512];
+  __shared__ uint64_t es[4][512];
+  const int wave = threadIdx.x >> 6;
+  const int64_t cnt = rlist ? (int64_t)*rcount : total_rows;
+  for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < cnt; p += (int64_t)gridDim.x * 4) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // previous row's LDS use is over
+    rerank_row(ds_arr, nds, rlist ? (int64_t)rlist[p] : p, mode, flag, cs[wave], es[wave]);
+  }
+}
+
 // Fallback work list: for each dataset, blocks of <= XS_QB flagged rows
 // (from the sorted global list) x xs column splits. Single thread; the
 // dataset count is small.
+// First position in the sorted list [0, cnt) holding a row id >= v.
+__device__ __forceinline__ int list_lower_bound(const uint32_t* __restrict__ list, int cnt,
+                                                int64_t v) {
+  int lo = 0, hi = cnt;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((int64_t)list[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Work items of a candidate sweep over the listed rows only (sorted global
+// row ids): per dataset, blocks of qw listed rows x its column splits.
+// Single thread; datasets are found by binary search in the list.
+//
+// The FP32 re-sweep pays a whole item (qw rows x a column split, both
+// passes) even for one listed row, while the exact scan pays per row, so a
+// handful of rows goes straight to the exact scan: the kernel compares the
+// two estimated times (FP32 MFMA sweep ~0.45 TFLOP/s per CU in
+// rounds of `cus` items, exact f64 scan ~2e13 flop/s over the device) and
+// emits no items (*nrows = 0: the re-rank of the escalated rows is skipped
+// too) when the scan is cheaper.
+__global__ void sweep_items_kernel(const DsDesc* __restrict__ ds_arr, int nds,
+                                   const uint32_t* __restrict__ list, const int* __restrict__ count,
+                                   int qw, int cus, WorkItem* __restrict__ items,
+                                   int* __restrict__ nitems, int* __restrict__ nrows) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int cnt = *count;
+  double t_exact = 0.0, t_item = 0.0;
+  int64_t nit = 0;
+  for (int di = 0; di < nds; ++di) {
+    const DsDesc& D = ds_arr[di];
+    const int p = list_lower_bound(list, cnt, D.row_off);
+    const int q = list_lower_bound(list, cnt, D.row_off + D.n);
+    if (q == p) continue;
+    t_exact += (double)(q - p) * (double)D.n * D.kp * 3.0 / 2e13;
+    const double span = (double)D.n / D.nsplit;
+    const double ti = 4.0 * span * D.kp * qw / 0.45e12;
+    t_item = ti > t_item ? ti : t_item;
+    nit += (int64_t)((q - p + qw - 1) / qw) * D.nsplit;
+  }
+  const double t_sweep = (double)((nit + cus - 1) / cus) * t_item;
+  if (cnt == 0 || t_exact <= t_sweep) {
+    *nitems = 0;
+    *nrows = 0;
+    return;
+  }
+  *nrows = cnt;
+  int w = 0;
+  for (int di = 0; di < nds; ++di) {
+    const int p = list_lower_bound(list, cnt, ds_arr[di].row_off);
+    const int q = list_lower_bound(list, cnt, ds_arr[di].row_off + ds_arr[di].n);
+    for (int b = p; b < q; b += qw) {
+      const int qc = q - b < qw ? q - b : qw;
+      for (int s = 0; s < ds_arr[di].nsplit; ++s) items[w++] = WorkItem{di, s, b, qc};
+    }
+  }
+  *nitems = w;
+}
+
 // The column split count starts at the planned xs and grows when few rows are
 // flagged (a handful of rows would otherwise run on a handful of CUs), up to
 // xs_cap (which keeps the merge at two levels) and within the item and
@@ -510,15 +593,10 @@ __global__ void fallback_items_kernel(const DsDesc* __restrict__ ds_arr, int nds
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int cnt = *count;
   int64_t nqb = 0;
-  {
-    int p = 0;
-    for (int di = 0; di < nds; ++di) {
-      const int64_t end = ds_arr[di].row_off + ds_arr[di].n;
-      int q = p;
-      while (q < cnt && (int64_t)list[q] < end) ++q;
-      nqb += (q - p + XS_QB - 1) / XS_QB;
-      p = q;
-    }
+  for (int di = 0; di < nds; ++di) {
+    const int p = list_lower_bound(list, cnt, ds_arr[di].row_off);
+    const int q = list_lower_bound(list, cnt, ds_arr[di].row_off + ds_arr[di].n);
+    nqb += (q - p + XS_QB - 1) / XS_QB;
   }
   int64_t xd = xs;
   if (nqb > 0 && nqb * xs < target) {
@@ -528,16 +606,14 @@ __global__ void fallback_items_kernel(const DsDesc* __restrict__ ds_arr, int nds
     g = g < row_cap * xs / cnt ? g : row_cap * xs / cnt;
     xd = g > xd ? g : xd;
   }
-  int p = 0, w = 0;
+  int w = 0;
   for (int di = 0; di < nds; ++di) {
-    const int64_t end = ds_arr[di].row_off + ds_arr[di].n;
-    int q = p;
-    while (q < cnt && (int64_t)list[q] < end) ++q;
+    const int p = list_lower_bound(list, cnt, ds_arr[di].row_off);
+    const int q = list_lower_bound(list, cnt, ds_arr[di].row_off + ds_arr[di].n);
     for (int b = p; b < q; b += XS_QB) {
       const int qc = q - b < XS_QB ? q - b : XS_QB;
       for (int s = 0; s < (int)xd; ++s) items[w++] = WorkItem{di, s, b, qc};
     }
-    p = q;
   }
   *nitems = w;
   *xs_out = (int)xd;

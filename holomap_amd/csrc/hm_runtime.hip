@@ -28,7 +28,10 @@ __global__ void exact_group_merge_kernel(const DsDesc*, int, const uint32_t*, co
                                          const int*, int, int, int, uint64_t*);
 __global__ void exact_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t,
                                    const int*, int, int, int, const uint64_t*);
-__global__ void rerank_kernel(const DsDesc*, int, int64_t, int, uint32_t*);
+__global__ void rerank_kernel(const DsDesc*, int, int64_t, int, uint32_t*, const uint32_t*,
+                              const int*);
+__global__ void sweep_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int, int,
+                                   WorkItem*, int*, int*);
 __global__ void fallback_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int, int,
                                       int64_t, int64_t, int, WorkItem*, int*, int*);
 __global__ void smooth_knn_kernel(const DsDesc*, int, int64_t, const double*, int, float*, float*,
@@ -112,6 +115,7 @@ struct hm_job {
   std::vector<DsDesc> hdesc;
   int64_t total_rows = 0, total_pad = 0, total_edges = 0;
   int kmax = 0, mode = 0, xs = 1, fb_cap = 0, n_items_l1 = 0;
+  bool counted = false;  // bytes added to the context's in_use
   std::vector<WorkItem> items_l1_h;
   std::vector<void*> allocs;
   uint64_t bytes = 0;
@@ -135,7 +139,9 @@ struct hm_job {
            *brow = nullptr, *bptr = nullptr;
   float *oval = nullptr, *orev = nullptr, *bval = nullptr;
   int *count = nullptr, *nitems_fb = nullptr, *xs_fb = nullptr;
-  WorkItem *items_l1 = nullptr, *items_fb = nullptr;
+  WorkItem *items_l1 = nullptr, *items_fb = nullptr, *items_l2 = nullptr;
+  int* nitems_l2 = nullptr;
+  int64_t cap_l2 = 0;  // items of a sweep over any subset of rows
   int64_t *pad_off = nullptr, *iptr_off = nullptr;
   std::vector<int64_t> iptr_off_h;
   // graph and timing
@@ -230,12 +236,17 @@ Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_
   }
   // sweep splits: a lone dataset (or a small batch) splits its columns so
   // the launch has enough work items for every CU
-  const int cout = 48;  // collect-list slots per (row, split, half)
   const int qw = sweep_rows(gram_mode(opt.precision));
   int64_t rb_total = 0;
   for (auto& h : P.dsh) rb_total += cdiv(h.n, qw);
   for (auto& h : P.dsh) {
-    const int smax = std::max(1, 256 / cout);  // re-rank holds <= 512 keys per row
+    // A list (one lane: half of a split's column groups) collects the keys
+    // <= T, T = C-th smallest of the row's 128 group minima: about
+    // 64 * -ln(1 - C/128) keys. Capacity with 40 % headroom; the re-rank
+    // holds <= 512 keys per row, which bounds the split count.
+    const double expect = 64.0 * -std::log(1.0 - h.C / 128.0);
+    const int cneed = std::max(48, (int)((1.4 * expect + 8.0 + 7.0) / 8.0) * 8);
+    const int smax = std::max(1, 256 / cneed);
     if (opt.nsplit > 0) {
       h.nsplit = std::max(1, std::min<int>(opt.nsplit, smax));
     } else if (rb_total >= 2 * cus) {
@@ -243,10 +254,11 @@ Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_
     } else {
       h.nsplit = std::min(smax, choose_nsplit(h.n, qw, h.C, cus, rb_total - cdiv(h.n, qw)));
     }
-    // the sweep stores column offsets within a split as u16
-    h.nsplit = std::max<int>(h.nsplit, (int)cdiv(h.n, 65536));
     h.nlist = 2 * h.nsplit;
-    h.cout = cout;
+    // the lists share the re-rank's 512 key slots: fewer splits, longer
+    // lists (a list collects ~C/2 keys of its half-split on average, and
+    // overflows -- sending its row to the exact scan -- beyond cout)
+    h.cout = std::max(cneed, std::min(256, (512 / h.nlist) & ~7));
     P.sum_cand += h.n * h.nlist * h.cout;
   }
   P.xs = (int)std::max<int64_t>(1, std::min<int64_t>(8, nmax / 2048));
@@ -284,10 +296,13 @@ hm_status dalloc(hm_job* J, T** p, int64_t count) {
   if (count <= 0) count = 1;
   const size_t bytes = (size_t)count * sizeof(T);
   void* q = nullptr;
-  hipError_t e = hipMalloc(&q, bytes);
+  // stream-ordered: creating and destroying jobs never synchronises the
+  // device, so jobs on other streams keep running (the context's pool keeps
+  // freed memory for reuse)
+  hipError_t e = hipMallocAsync(&q, bytes, J->stream);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    return fail(J->ctx, HM_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    return fail(J->ctx, HM_ENOMEM, "hipMallocAsync of " + std::to_string(bytes) + " bytes failed");
   }
   J->allocs.push_back(q);
   J->bytes += bytes;
@@ -347,15 +362,15 @@ int64_t sweep_grid(const hm_ctx* ctx, int mode, int64_t items) {
 
 unsigned grid_for(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, cdiv(n, per)); }
 
-hm_status launch_sweep(hm_job* J) {
-  // grid and item order go together (see the item permutation in hm_job_create)
-  const unsigned g = (unsigned)sweep_grid(J->ctx, J->mode, J->n_items_l1);
+// Both passes of the candidate sweep in `mode` over `items` (a host count,
+// or a device count when nitems_dev is set; qlist maps item rows to global
+// row ids when set).
+hm_status launch_sweep(hm_job* J, int mode, const WorkItem* items, const int* nitems_dev,
+                       int nitems_host, const uint32_t* qlist, unsigned grid) {
 #define HM_GRAM(M, P)                                                                        \
-  hipLaunchKernelGGL((gram_sweep_kernel<M, P>), dim3(g), dim3(sweep_rows(M) / 32 * 64), 0,     \
-                     J->stream, J->d_desc,                                                    \
-                     (const WorkItem*)J->items_l1, (const int*)nullptr, J->n_items_l1,        \
-                     (const uint32_t*)nullptr)
-  switch (J->mode) {
+  hipLaunchKernelGGL((gram_sweep_kernel<M, P>), dim3(grid), dim3(sweep_rows(M) / 32 * 64), 0,  \
+                     J->stream, J->d_desc, items, nitems_dev, nitems_host, qlist)
+  switch (mode) {
     case 0: HM_GRAM(0, 0); HM_GRAM(0, 1); break;
     case 1: HM_GRAM(1, 0); HM_GRAM(1, 1); break;
     case 2: HM_GRAM(2, 0); HM_GRAM(2, 1); break;
@@ -389,17 +404,38 @@ hm_status enqueue_pipeline(hm_job* J) {
     mark(J, 1);
     // ---- candidate sweep + re-rank ----
     if (J->mode >= 0) {
-      HM_TRY(launch_sweep(J));
+      // grid and item order go together (see the item permutation in hm_job_create)
+      HM_TRY(launch_sweep(J, J->mode, J->items_l1, nullptr, J->n_items_l1, nullptr,
+                          (unsigned)sweep_grid(ctx, J->mode, J->n_items_l1)));
       mark(J, 2);
       hipLaunchKernelGGL(rerank_kernel, dim3(grid_for(R, 4)), dim3(256), 0, J->stream, J->d_desc,
-                         nds, R, J->mode, J->flag);
+                         nds, R, J->mode, J->flag, (const uint32_t*)nullptr, (const int*)nullptr);
       HM_LAUNCH(ctx);
+      mark(J, 3);
+      if (J->mode != 1) {
+        // escalation: rows the bf16 keys could not certify are swept again
+        // with FP32 keys (a far smaller error bound) and re-ranked; only
+        // rows still uncertified reach the exact scan below
+        HM_TRY(scan_excl<uint32_t>(J, J->flag, R, J->flagpos, (uint32_t*)nullptr, J->sc32));
+        hipLaunchKernelGGL(compact_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream,
+                           (const uint32_t*)J->flag, (const uint32_t*)J->flagpos, R, J->list,
+                           J->count);
+        hipLaunchKernelGGL(sweep_items_kernel, dim3(1), dim3(64), 0, J->stream, J->d_desc, nds,
+                           (const uint32_t*)J->list, (const int*)J->count, sweep_rows(1), cus,
+                           J->items_l2, J->nitems_l2, J->nitems_l2 + 1);
+        HM_LAUNCH(ctx);
+        HM_TRY(launch_sweep(J, 1, J->items_l2, J->nitems_l2, 0, J->list, (unsigned)cus));
+        hipLaunchKernelGGL(rerank_kernel, dim3(std::max<int64_t>(1, std::min<int64_t>(cdiv(R, 4), 8 * cus))),
+                           dim3(256), 0, J->stream, J->d_desc, nds, R, 1, J->flag,
+                           (const uint32_t*)J->list, (const int*)(J->nitems_l2 + 1));
+        HM_LAUNCH(ctx);
+      }
     } else {
       mark(J, 2);
       hipLaunchKernelGGL(fill_u32, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, J->flag, R, 1u);
       HM_LAUNCH(ctx);
+      mark(J, 3);
     }
-    mark(J, 3);
     // ---- exact scan of the uncertified rows ----
     HM_TRY(scan_excl<uint32_t>(J, J->flag, R, J->flagpos, (uint32_t*)nullptr, J->sc32));
     hipLaunchKernelGGL(compact_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream,
@@ -540,6 +576,13 @@ hm_status hm_ctx_create(hm_ctx** out, int device, uint64_t budget) {
     delete c;
     return HM_EDEVICE;
   }
+  // job buffers come from the device's default pool; keep freed blocks
+  // cached instead of returning them at every synchronisation
+  hipMemPool_t pool = nullptr;
+  if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+    uint64_t keep = ~0ull;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
   *out = c;
   return HM_OK;
 }
@@ -552,6 +595,10 @@ void hm_ctx_destroy(hm_ctx* c) {
 }
 
 const char* hm_last_error(const hm_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+uint64_t hm_ctx_budget(const hm_ctx* c) { return c ? c->budget : 0; }
+
+uint64_t hm_ctx_in_use(const hm_ctx* c) { return c ? c->in_use : 0; }
 
 hm_status hm_validate(const float* data, uint64_t len, uint32_t d, uint32_t k, uint64_t* bad) {
   if (d == 0 || len == 0 || len % d != 0) return HM_EBADSHAPE;
@@ -596,8 +643,6 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
         return fail(ctx, HM_EUNSUPPORTED, "n_neighbors > 64 is not supported on the GPU path");
     }
     if (n[i] >= (1ull << 31)) return fail(ctx, HM_EUNSUPPORTED, "n >= 2^31");
-    if (knn_stage && n[i] > 5ull * 65536ull)
-      return fail(ctx, HM_EUNSUPPORTED, "n > 327680 rows per dataset on the GPU path");
     if (metric && metric[i] != HM_EUCLIDEAN && metric[i] != HM_COSINE)
       return fail(ctx, HM_EINVAL, "metric must be \"euclidean\" or \"cosine\"");
   }
@@ -616,7 +661,11 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
 
   hm_job* J = new hm_job();
   J->ctx = ctx;
-  J->stream = ctx->stream;
+  // every job has its own stream: jobs of one context overlap on the device
+  if (hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete J;
+    return fail(ctx, HM_EDEVICE, "stream create failed");
+  }
   J->opt = o;
   J->stages = stages;
   J->nds = (int)nds;
@@ -666,6 +715,13 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     HM_A(J->list, R);
     HM_A(J->count, 1);
     HM_A(J->nitems_fb, 1);
+    if (J->mode == 0 || J->mode == 2) {
+      int64_t cap = 0;
+      for (auto& h : P.dsh) cap += cdiv(h.n, sweep_rows(1)) * h.nsplit;
+      J->cap_l2 = cap;
+      HM_A(J->items_l2, std::max<int64_t>(cap, 1));
+      HM_A(J->nitems_l2, 2);  // items, rows escalated
+    }
     HM_A(J->xs_fb, 1);
     HM_A(J->items_fb, std::max<int64_t>(P.fb_cap, 1));
     HM_A(J->xpart, R * P.xs * P.kmax);
@@ -752,7 +808,8 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
   pad_off_h[nds] = off_pad;
   hm_status st = HM_OK;
   auto up = [&](void* dst, const void* src, size_t bytes) {
-    if (st == HM_OK && hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+    if (st == HM_OK &&
+        hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, J->stream) != hipSuccess)
       st = fail(ctx, HM_EDEVICE, "descriptor upload failed");
   };
   up(J->d_desc, J->hdesc.data(), sizeof(DsDesc) * nds);
@@ -796,11 +853,14 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     if (s2 != HM_OK) return bail(s2);
     up(J->items_l1, J->items_l1_h.data(), sizeof(WorkItem) * it.size());
   }
+  if (st == HM_OK && hipStreamSynchronize(J->stream) != hipSuccess)  // host sources are locals
+    st = fail(ctx, HM_EDEVICE, "descriptor upload failed");
   if (st != HM_OK) return bail(st);
   for (auto& e : J->ev) {
     if (hipEventCreate(&e) != hipSuccess) return bail(fail(ctx, HM_EDEVICE, "event create"));
   }
   ctx->in_use += J->bytes;
+  J->counted = true;
   *out = J;
   return HM_OK;
 }
@@ -812,8 +872,12 @@ void hm_job_destroy(hm_job* J) {
   if (J->gexec) (void)hipGraphExecDestroy(J->gexec);
   for (auto e : J->ev)
     if (e) (void)hipEventDestroy(e);
-  for (void* p : J->allocs) (void)hipFree(p);
-  if (J->ctx->in_use >= J->bytes) J->ctx->in_use -= J->bytes;
+  for (void* p : J->allocs) (void)hipFreeAsync(p, J->stream);
+  if (J->stream) {
+    (void)hipStreamSynchronize(J->stream);
+    (void)hipStreamDestroy(J->stream);
+  }
+  if (J->counted) J->ctx->in_use -= J->bytes;
   delete J;
 }
 
@@ -920,6 +984,11 @@ hm_status hm_job_counters(hm_job* J, uint64_t* c) {
     c[HM_C_FALLBACK_ROWS] = (uint64_t)cnt;
     c[HM_C_CANDIDATES] = (uint64_t)J->dsh[0].nlist * J->dsh[0].cout;
     c[HM_C_NSPLIT] = (uint64_t)J->dsh[0].nsplit;
+    if (J->nitems_l2) {
+      int esc = 0;
+      HM_HIP(J->ctx, hipMemcpy(&esc, J->nitems_l2 + 1, sizeof(int), hipMemcpyDeviceToHost));
+      c[HM_C_ESCALATED] = (uint64_t)esc;
+    }
   }
   if (J->stages & HM_STAGE_FUZZY) {
     uint64_t nnz = 0;

@@ -1,0 +1,286 @@
+// Fit-request serving loop over device-resident jobs (host code).
+//
+// Requests (one dataset each: n x d rows, k, metric) wait in a FIFO queue.
+// hm_server_progress, called by the client's loop, retires finished jobs and
+// admits queued requests in arrival order into new jobs while job slots and
+// the context's HBM budget allow: a request that arrives while others run
+// joins the next job as soon as a slot and memory free up (continuous
+// batching), and the jobs run concurrently, each on its own stream. Results
+// stay in HBM until the client fetches them (hm_server_fetch); a job's
+// memory returns to the pool when its last request is fetched.
+//
+// The graph of a request does not depend on the job it lands in (every
+// output is per-row reference-order arithmetic), so a request's result is
+// the same alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/holomap_gpu.h"
+
+namespace {
+
+struct Request {
+  uint64_t id = 0;
+  uint64_t n = 0;
+  uint32_t d = 0, k = 0;
+  int32_t metric = 0;
+  std::vector<float> data;  // host copy, kept until the job has run
+  int state = 0;            // 0 queued, 1 running, 2 done, < 0 error
+  int job = -1;             // index into Server::jobs
+  uint32_t ds = 0;          // dataset slot within the job
+};
+
+struct Job {
+  hm_job* job = nullptr;
+  std::vector<uint64_t> reqs;
+  uint32_t unfetched = 0;
+  bool done = false;
+};
+
+}  // namespace
+
+struct hm_server {
+  hm_ctx* ctx = nullptr;
+  hm_server_config cfg;
+  uint64_t next_id = 1;
+  std::deque<uint64_t> queue;
+  std::unordered_map<uint64_t, Request> reqs;
+  std::vector<Job> jobs;  // slots; a slot with job == nullptr is free
+};
+
+extern "C" {
+
+void hm_server_default_config(hm_server_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->max_jobs = 4;
+  c->max_batch = 32;
+  c->max_batch_rows = 1u << 20;
+  hm_default_options(&c->opt);
+}
+
+hm_status hm_server_create(hm_ctx* ctx, const hm_server_config* cfg, hm_server** out) {
+  if (!ctx || !out) return HM_EINVAL;
+  hm_server* s = new hm_server();
+  s->ctx = ctx;
+  if (cfg) s->cfg = *cfg; else hm_server_default_config(&s->cfg);
+  if (s->cfg.max_jobs == 0 || s->cfg.max_batch == 0 || s->cfg.max_batch_rows == 0) {
+    delete s;
+    return HM_EINVAL;
+  }
+  s->cfg.opt.use_graph = 0;  // every job runs once
+  s->cfg.opt.timing = 0;
+  *out = s;
+  return HM_OK;
+}
+
+void hm_server_destroy(hm_server* s) {
+  if (!s) return;
+  for (auto& j : s->jobs)
+    if (j.job) hm_job_destroy(j.job);
+  delete s;
+}
+
+hm_status hm_server_submit(hm_server* s, const float* data, uint64_t n, uint32_t d, uint32_t k,
+                           int32_t metric, uint64_t* id) {
+  if (!s || !data || !id) return HM_EINVAL;
+  uint64_t bad = 0;
+  const hm_status v = hm_validate(data, n * d, d, k, &bad);
+  if (v != HM_OK) return v;
+  if (metric != HM_EUCLIDEAN && metric != HM_COSINE) return HM_EINVAL;
+  Request r;
+  r.id = s->next_id++;
+  r.n = n;
+  r.d = d;
+  r.k = k;
+  r.metric = metric;
+  r.data.assign(data, data + n * d);
+  *id = r.id;
+  s->queue.push_back(r.id);
+  s->reqs.emplace(r.id, std::move(r));
+  return HM_OK;
+}
+
+// A slot holds a job until its last request is fetched (running jobs are
+// limited separately, by cfg.max_jobs).
+static int free_slot(hm_server* s) {
+  for (size_t i = 0; i < s->jobs.size(); ++i)
+    if (!s->jobs[i].job) return (int)i;
+  s->jobs.emplace_back();
+  return (int)s->jobs.size() - 1;
+}
+
+static uint32_t running_jobs(const hm_server* s) {
+  uint32_t r = 0;
+  for (auto& j : s->jobs)
+    if (j.job && !j.done) ++r;
+  return r;
+}
+
+hm_status hm_server_progress(hm_server* s, uint32_t* completed) {
+  if (!s) return HM_EINVAL;
+  uint32_t fin = 0;
+  // retire finished jobs
+  for (auto& j : s->jobs) {
+    if (!j.job || j.done) continue;
+    const int q = hm_job_done(j.job);
+    if (q == 0) continue;
+    j.done = true;
+    for (uint64_t id : j.reqs) {
+      Request& r = s->reqs[id];
+      r.state = q == 1 ? 2 : HM_EDEVICE;
+      r.data.clear();
+      r.data.shrink_to_fit();
+      ++fin;
+    }
+  }
+  // admit queued requests, oldest first
+  const uint64_t budget = hm_ctx_budget(s->ctx);
+  while (!s->queue.empty() && running_jobs(s) < s->cfg.max_jobs) {
+    {  // a request larger than the whole budget can never run
+      const Request& h = s->reqs[s->queue.front()];
+      const uint64_t need1 = hm_job_estimate_bytes(1, &h.n, &h.d, &h.k, &s->cfg.opt);
+      if (budget && need1 > budget) {
+        Request& r = s->reqs[s->queue.front()];
+        r.state = HM_EBUDGET;
+        r.data.clear();
+        s->queue.pop_front();
+        ++fin;
+        continue;
+      }
+    }
+    // the longest queue prefix that fits the slot limits and the free budget
+    std::vector<uint64_t> ids, n;
+    std::vector<uint32_t> d, k;
+    std::vector<int32_t> m;
+    uint64_t rows = 0;
+    for (size_t qi = 0; qi < s->queue.size() && ids.size() < s->cfg.max_batch; ++qi) {
+      const Request& r = s->reqs[s->queue[qi]];
+      if (!ids.empty() && rows + r.n > s->cfg.max_batch_rows) break;
+      n.push_back(r.n);
+      d.push_back(r.d);
+      k.push_back(r.k);
+      m.push_back(r.metric);
+      const uint64_t need = hm_job_estimate_bytes((uint32_t)n.size(), n.data(), d.data(),
+                                                  k.data(), &s->cfg.opt);
+      if (budget && hm_ctx_in_use(s->ctx) + need > budget) {
+        n.pop_back();
+        d.pop_back();
+        k.pop_back();
+        m.pop_back();
+        break;
+      }
+      ids.push_back(r.id);
+      rows += r.n;
+    }
+    if (ids.empty()) break;  // the head waits for memory to free up
+    hm_job* J = nullptr;
+    hm_status st = hm_job_create(s->ctx, (uint32_t)ids.size(), n.data(), d.data(), k.data(),
+                                 m.data(), &s->cfg.opt, HM_STAGE_ALL, &J);
+    if (st == HM_EBUDGET || st == HM_ENOMEM) {
+      bool held = false;  // memory that will come back: live jobs
+      for (auto& j : s->jobs) held = held || j.job != nullptr;
+      if (held) break;  // retried when memory frees up
+      // nothing to wait for: the head cannot be allocated at all
+      Request& r = s->reqs[s->queue.front()];
+      r.state = st;
+      r.data.clear();
+      s->queue.pop_front();
+      ++fin;
+      continue;
+    }
+    for (size_t i = 0; i < ids.size(); ++i) s->queue.pop_front();
+    if (st == HM_OK) {
+      for (size_t i = 0; i < ids.size() && st == HM_OK; ++i)
+        st = hm_job_upload(J, (uint32_t)i, s->reqs[ids[i]].data.data());
+      if (st == HM_OK) st = hm_job_run(J);
+      if (st != HM_OK) hm_job_destroy(J);
+    }
+    if (st != HM_OK) {
+      for (uint64_t id : ids) {
+        s->reqs[id].state = st;
+        ++fin;
+      }
+      continue;
+    }
+    const int slot = free_slot(s);
+    Job& j = s->jobs[slot];
+    j.job = J;
+    j.reqs = ids;
+    j.unfetched = (uint32_t)ids.size();
+    j.done = false;
+    for (size_t i = 0; i < ids.size(); ++i) {
+      Request& r = s->reqs[ids[i]];
+      r.state = 1;
+      r.job = slot;
+      r.ds = (uint32_t)i;
+    }
+  }
+  if (completed) *completed = fin;
+  return HM_OK;
+}
+
+int hm_server_state(hm_server* s, uint64_t id) {
+  if (!s) return HM_EINVAL;
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return HM_EINVAL;
+  return it->second.state;
+}
+
+uint32_t hm_server_pending(const hm_server* s) {
+  if (!s) return 0;
+  uint32_t p = (uint32_t)s->queue.size();
+  for (auto& j : s->jobs)
+    if (j.job && !j.done) p += (uint32_t)j.reqs.size();
+  return p;
+}
+
+hm_status hm_server_result_info(hm_server* s, uint64_t id, uint64_t* n, uint32_t* k,
+                                uint64_t* nnz) {
+  if (!s) return HM_EINVAL;
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end() || it->second.state != 2) return HM_EINVAL;
+  const Request& r = it->second;
+  if (n) *n = r.n;
+  if (k) *k = r.k;
+  if (nnz) {
+    hm_graph_out o;
+    std::memset(&o, 0, sizeof(o));
+    const hm_status st = hm_job_download(s->jobs[r.job].job, r.ds, &o);  // sizes only
+    if (st != HM_OK) return st;
+    *nnz = o.nnz;
+  }
+  return HM_OK;
+}
+
+hm_status hm_server_fetch(hm_server* s, uint64_t id, hm_graph_out* out) {
+  if (!s) return HM_EINVAL;
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return HM_EINVAL;
+  Request& r = it->second;
+  if (r.state < 0) {
+    const hm_status st = r.state;
+    s->reqs.erase(it);
+    return st;
+  }
+  if (r.state != 2) return HM_EINVAL;
+  hm_status st = HM_OK;
+  if (out) st = hm_job_download(s->jobs[r.job].job, r.ds, out);
+  if (st != HM_OK) return st;
+  Job& j = s->jobs[r.job];
+  s->reqs.erase(it);
+  if (--j.unfetched == 0) {
+    hm_job_destroy(j.job);
+    j = Job();
+  }
+  return HM_OK;
+}
+
+}  // extern "C"

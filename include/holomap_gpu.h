@@ -91,11 +91,12 @@ typedef struct {
 #define HM_T_TOTAL 6
 
 /* Counters of the last run (hm_job_counters). */
-#define HM_NCOUNTERS 4
+#define HM_NCOUNTERS 5
 #define HM_C_FALLBACK_ROWS 0 /* rows recomputed by the exact scan */
 #define HM_C_NNZ 1           /* graph entries, all datasets */
-#define HM_C_CANDIDATES 2    /* candidate slots per row (splits x C) */
+#define HM_C_CANDIDATES 2    /* candidate key slots per row (lists x list capacity) */
 #define HM_C_NSPLIT 3        /* column splits used by the sweep */
+#define HM_C_ESCALATED 4     /* rows the BF16 keys left uncertified, swept again in FP32 */
 
 void hm_default_options(hm_options* opt);
 const char* hm_version(void);
@@ -106,6 +107,9 @@ hm_status hm_ctx_create(hm_ctx** out, int device, uint64_t hbm_budget_bytes);
 void hm_ctx_destroy(hm_ctx* ctx);
 const char* hm_last_error(const hm_ctx* ctx);
 int hm_device_count(void);
+/* HBM budget of the context (0 = none) and bytes held by its live jobs. */
+uint64_t hm_ctx_budget(const hm_ctx* ctx);
+uint64_t hm_ctx_in_use(const hm_ctx* ctx);
 
 /* Input validation of src/api.rs:113-144 restricted to what the graph
  * stages see (shape, n_neighbors >= 2, n > n_neighbors, finite data).
@@ -188,6 +192,42 @@ hm_status hm_job_counters(hm_job* job, uint64_t* c /* HM_NCOUNTERS */);
 hm_status hm_job_device_outputs(hm_job* job, uint32_t ds, const uint32_t** indices,
                                 const float** dists, const uint32_t** rows, const uint32_t** cols,
                                 const float** vals, const uint64_t** indptr);
+
+/* ---- serving loop: fit requests admitted into concurrent jobs ----
+ * No counterpart in the reference (it has no server for this path; its
+ * JSON-lines clusterer, crates/holomap-clusterer/src/main.rs:10-31, calls
+ * fit_transform once per request). Requests queue in arrival order;
+ * hm_server_progress retires finished jobs and admits the longest queue
+ * prefix that fits max_batch, max_batch_rows and the context's free HBM
+ * budget into a new job (continuous batching), up to max_jobs jobs running
+ * at once on their own streams. Results stay in device memory until
+ * fetched. A request larger than the whole budget ends in HM_EBUDGET.
+ * Single-threaded: one thread drives a server. */
+typedef struct hm_server hm_server;
+typedef struct {
+  uint32_t max_jobs;       /* jobs running at once; default 4 */
+  uint32_t max_batch;      /* requests per job; default 32 */
+  uint64_t max_batch_rows; /* rows per job (sum of n); default 2^20 */
+  hm_options opt;          /* per-job options (use_graph/timing forced off) */
+} hm_server_config;
+void hm_server_default_config(hm_server_config* cfg);
+hm_status hm_server_create(hm_ctx* ctx, const hm_server_config* cfg, hm_server** out);
+void hm_server_destroy(hm_server* srv);
+/* Queue one request (validated as hm_validate; the data is copied). */
+hm_status hm_server_submit(hm_server* srv, const float* data, uint64_t n, uint32_t n_features,
+                           uint32_t k, int32_t metric, uint64_t* id);
+/* Non-blocking step; *completed = requests finished (or failed) in this call. */
+hm_status hm_server_progress(hm_server* srv, uint32_t* completed);
+/* 0 queued, 1 running, 2 done, < 0: the request's error status. */
+int hm_server_state(hm_server* srv, uint64_t id);
+/* Requests queued or running. */
+uint32_t hm_server_pending(const hm_server* srv);
+/* Shape of a done request's result (for sizing hm_graph_out). */
+hm_status hm_server_result_info(hm_server* srv, uint64_t id, uint64_t* n, uint32_t* k,
+                                uint64_t* nnz);
+/* Copy a done request's results to the host (out may be NULL) and release
+ * the request; a failed request returns its error status and is released. */
+hm_status hm_server_fetch(hm_server* srv, uint64_t id, hm_graph_out* out);
 
 /* Parity helper: the device expf (the reference's libm method) over an
  * array, for testing the calibration arithmetic. */

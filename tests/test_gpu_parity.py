@@ -1,6 +1,7 @@
 """GPU parity: the HIP path against the oracle and the reference's golden
 fixtures. kNN indices/distances, rho, sigma (ref_f32 mode) and the fuzzy
 graph must be bit-identical; FP64 calibration within the stated tolerance."""
+import ctypes
 import json
 import os
 
@@ -280,3 +281,127 @@ def test_few_uncertified_rows_use_wide_fallback(ctx):
         job.close()
         assert_graph_equal(r, *ref)
         assert 1 <= fb <= 201, fb
+
+
+def test_c4_batch_matches_alone_and_oracle(ctx):
+    # 48 datasets of the C4 mix (N log-uniform in [200, 8000], d in
+    # {32, 64, 128, 256}, k in {10, 15, 30}, euclidean/cosine) in one job:
+    # every dataset's graph equals its single-dataset run byte for byte, and
+    # the smallest ones equal the oracle
+    sets = DS.c4(count=48)
+    job = H.Job(ctx, [s.n for s in sets], [s.d for s in sets], [s.k for s in sets],
+                [s.metric for s in sets])
+    for i, s in enumerate(sets):
+        job.upload(i, s.data)
+    job.run()
+    job.sync()
+    got = [job.download(i) for i in range(len(sets))]
+    job.close()
+    for i, s in enumerate(sets):
+        alone = H.knn_graph(s.data, s.d, s.k, s.metric, ctx=ctx)
+        r = got[i]
+        assert r.knn.indices.tobytes() == alone.knn.indices.tobytes(), i
+        assert r.knn.dists.tobytes() == alone.knn.dists.tobytes(), i
+        assert r.calib.sigmas.tobytes() == alone.calib.sigmas.tobytes(), i
+        assert r.graph.cols.tobytes() == alone.graph.cols.tobytes(), i
+        assert r.graph.vals.tobytes() == alone.graph.vals.tobytes(), i
+    for i in sorted(range(len(sets)), key=lambda j: sets[j].n * sets[j].d)[:6]:
+        s = sets[i]
+        assert_graph_equal(got[i], *O.knn_graph(s.data, s.d, s.k, s.metric))
+
+
+def _serve_all(srv, ids, ctx=None, budget=0):
+    """Drive the loop until every request finished; returns {id: result or exception}."""
+    out = {}
+    peak = 0
+    while len(out) < len(ids):
+        srv.progress()
+        if ctx is not None:
+            peak = max(peak, ctx.in_use)
+            if budget:
+                assert ctx.in_use <= budget
+        for rid in ids:
+            if rid in out:
+                continue
+            st = srv.state(rid)
+            if st == srv.DONE or st < 0:
+                try:
+                    out[rid] = srv.fetch(rid)
+                except Exception as e:  # noqa: BLE001 - the request's own error
+                    out[rid] = e
+    return out, peak
+
+
+def _same(a, b):
+    return (a.knn.indices.tobytes() == b.knn.indices.tobytes()
+            and a.knn.dists.tobytes() == b.knn.dists.tobytes()
+            and a.calib.sigmas.tobytes() == b.calib.sigmas.tobytes()
+            and a.calib.rhos.tobytes() == b.calib.rhos.tobytes()
+            and a.graph.rows.tobytes() == b.graph.rows.tobytes()
+            and a.graph.cols.tobytes() == b.graph.cols.tobytes()
+            and a.graph.vals.tobytes() == b.graph.vals.tobytes()
+            and a.indptr.tobytes() == b.indptr.tobytes())
+
+
+def test_server_results_same_alone_and_in_mix(ctx):
+    # 14 mixed requests (C5 shapes, smaller N) submitted together: they are
+    # batched into concurrent jobs; every result equals its run alone
+    reqs = [DS.c5_request(100 + i, n_min=300, n_max=4000) for i in range(14)]
+    srv = H.Server(ctx, max_jobs=3, max_batch=4)
+    ids = [srv.submit(r.data, r.d, r.k, r.metric) for r in reqs]
+    got, _ = _serve_all(srv, ids)
+    srv.close()
+    for rid, r in zip(ids, reqs):
+        alone = H.knn_graph(r.data, r.d, r.k, r.metric, ctx=ctx)
+        assert _same(got[rid], alone), rid
+
+
+def test_server_budget_admission():
+    # a context budget that holds about one request at a time: requests wait
+    # for memory, the budget is never exceeded, a request larger than the
+    # whole budget fails with MemoryError, the others are exact
+    from holomap_amd import _lib
+    reqs = [DS.c5_request(200 + i, n_min=500, n_max=3000) for i in range(6)]
+    big = DS.c5_request(7, n_min=20000, n_max=20001)
+    o = H._opts()
+    est = []
+    for r in reqs:
+        a = np.array([r.n], np.uint64)
+        b = np.array([r.d], np.uint32)
+        c = np.array([r.k], np.uint32)
+        est.append(int(_lib.load().hm_job_estimate_bytes(1, a.ctypes.data, b.ctypes.data,
+                                                          c.ctypes.data, ctypes.byref(o))))
+    budget = int(max(est) * 1.3)
+    bctx = H.Context(0, hbm_budget=budget)
+    srv = H.Server(bctx, max_jobs=4, max_batch=8)
+    ids = [srv.submit(r.data, r.d, r.k, r.metric) for r in reqs]
+    big_id = srv.submit(big.data, big.d, big.k, big.metric)
+    got, peak = _serve_all(srv, ids + [big_id], ctx=bctx, budget=budget)
+    srv.close()
+    assert isinstance(got[big_id], MemoryError)
+    assert 0 < peak <= budget
+    for rid, r in zip(ids, reqs):
+        assert _same(got[rid], H.knn_graph(r.data, r.d, r.k, r.metric))
+    bctx.close()
+
+
+def test_fp32_escalation_certifies_tight_cosine_clusters(ctx):
+    # 10 tight clusters (noise 0.5 around centres of norm ~23) under cosine:
+    # neighbour similarities differ by less than the bf16 key error bound,
+    # so rows are swept again with FP32 keys (enough of them that the
+    # re-sweep is cheaper than the exact scan), which certify them; the
+    # graph equals the oracle's
+    rng = np.random.default_rng(31)
+    n, d, k = 20000, 32, 30
+    cen = rng.normal(0.0, 4.0, size=(10, d))
+    x = (cen[rng.integers(0, 10, n)] + rng.normal(0.0, 0.5, size=(n, d))).astype(np.float32)
+    job = H.Job(ctx, [n], [d], [k], ["cosine"])
+    job.upload(0, x)
+    job.run()
+    job.sync()
+    r = job.download(0)
+    c = job.counters()
+    job.close()
+    assert_graph_equal(r, *O.knn_graph(x, d, k, "cosine"))
+    assert c["escalated_rows"] > 0
+    assert c["fallback_rows"] < c["escalated_rows"]
