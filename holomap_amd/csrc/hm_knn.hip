@@ -355,15 +355,21 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     return;
   }
 
-  // ---- gather and sort the candidate keys of all lists ----
-  const int m = L * CO;
-  int len = 64;
-  while (len < m) len <<= 1;
+  // ---- gather the stored keys of all lists (packed) and sort them ----
   const uint64_t* src = D.cand + (size_t)i * L * CO;
   const uint32_t* cnt = D.ccnt + (size_t)i * L;
+  int nvalid = 0;
+  for (int l = 0; l < L; ++l) nvalid += (int)cnt[l];  // each <= CO
+  int len = 64;
+  while (len < nvalid) len <<= 1;
   for (int e = lane; e < len; e += 64) {
-    const int l = e / CO;
-    a[e] = (e < m && (uint32_t)(e - l * CO) < cnt[l]) ? src[e] : ~0ull;
+    uint64_t v = ~0ull;
+    if (e < nvalid) {
+      int l = 0, off = 0;
+      while (e - off >= (int)cnt[l]) off += (int)cnt[l++];
+      v = src[(size_t)l * CO + (e - off)];
+    }
+    a[e] = v;
   }
   uint64_t taumin = ~0ull;
   for (int s = 0; s < L; ++s) {
@@ -372,10 +378,6 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   wave_sort_lds(a, len);
-  int nvalid = 0;
-  for (int e = lane; e < len; e += 64) nvalid += (a[e] != ~0ull) ? 1 : 0;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) nvalid += __shfl_xor(nvalid, o, 64);
   if (nvalid < k) {
     // cannot happen for n >= k (the lists keep min(C, columns) keys);
     // leave the row to the exact scan rather than decode an empty slot
@@ -536,16 +538,17 @@ __device__ __forceinline__ int list_lower_bound(const uint32_t* __restrict__ lis
 // row ids): per dataset, blocks of qw listed rows x its column splits.
 // Single thread; datasets are found by binary search in the list.
 //
-// The FP32 re-sweep pays a whole item (qw rows x a column split, both
-// passes) even for one listed row, while the exact scan pays per row, so a
-// handful of rows goes straight to the exact scan: the kernel compares the
-// two estimated times (FP32 MFMA sweep ~0.45 TFLOP/s per CU in
-// rounds of `cus` items, exact f64 scan ~2e13 flop/s over the device) and
-// emits no items (*nrows = 0: the re-rank of the escalated rows is skipped
-// too) when the scan is cheaper.
+// The re-sweep pays a whole item (qw rows x a column split, both passes)
+// even for one listed row, while the exact scan pays per row, so a handful of
+// rows goes straight to the exact scan: the kernel compares the two
+// estimated times (the re-sweep at `rate` Gram flop/s per CU -- measured
+// ~1.1e12 for bf16x3, ~0.45e12 for FP32 -- in rounds of `cus` items; the
+// exact f64 scan at ~2e13 flop/s over the device) and emits no items
+// (*nrows = 0: the re-rank of the escalated rows is skipped too) when the
+// scan is cheaper.
 __global__ void sweep_items_kernel(const DsDesc* __restrict__ ds_arr, int nds,
                                    const uint32_t* __restrict__ list, const int* __restrict__ count,
-                                   int qw, int cus, WorkItem* __restrict__ items,
+                                   int qw, int cus, double rate, WorkItem* __restrict__ items,
                                    int* __restrict__ nitems, int* __restrict__ nrows) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int cnt = *count;
@@ -558,7 +561,7 @@ __global__ void sweep_items_kernel(const DsDesc* __restrict__ ds_arr, int nds,
     if (q == p) continue;
     t_exact += (double)(q - p) * (double)D.n * D.kp * 3.0 / 2e13;
     const double span = (double)D.n / D.nsplit;
-    const double ti = 4.0 * span * D.kp * qw / 0.45e12;
+    const double ti = 4.0 * span * D.kp * qw / rate;
     t_item = ti > t_item ? ti : t_item;
     nit += (int64_t)((q - p + qw - 1) / qw) * D.nsplit;
   }

@@ -31,7 +31,7 @@ __global__ void exact_merge_kernel(const DsDesc*, int, const uint32_t*, const in
 __global__ void rerank_kernel(const DsDesc*, int, int64_t, int, uint32_t*, const uint32_t*,
                               const int*);
 __global__ void sweep_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int, int,
-                                   WorkItem*, int*, int*);
+                                   double, WorkItem*, int*, int*);
 __global__ void fallback_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int, int,
                                       int64_t, int64_t, int, WorkItem*, int*, int*);
 __global__ void smooth_knn_kernel(const DsDesc*, int, int64_t, const double*, int, float*, float*,
@@ -115,6 +115,7 @@ struct hm_job {
   std::vector<DsDesc> hdesc;
   int64_t total_rows = 0, total_pad = 0, total_edges = 0;
   int kmax = 0, mode = 0, xs = 1, fb_cap = 0, n_items_l1 = 0;
+  int esc_mode = -1;  // sweep mode of the escalation pass (-1: none)
   bool counted = false;  // bytes added to the context's in_use
   std::vector<WorkItem> items_l1_h;
   std::vector<void*> allocs;
@@ -173,6 +174,9 @@ hm_status fail(hm_ctx* ctx, hm_status st, const std::string& msg) {
       return fail((ctx), HM_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e_)); \
   } while (0)
 
+
+// Escalation sweep of uncertified rows: bf16 -> bf16x3, bf16x3 -> FP32.
+int esc_mode_of(int mode) { return mode == 0 ? 2 : (mode == 2 ? 1 : -1); }
 
 int gram_mode(int precision) {
   switch (precision) {
@@ -275,8 +279,7 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
   b += sum_nd * 4;                                   // x
   if (stages & HM_STAGE_KNN) {
     b += P.sum_xr * 4 + P.sum_cos * 4;               // xr, op32 (cosine)
-    if (mode == 0) b += P.sum_x16 * 2;               // op16
-    if (mode == 2) b += P.sum_x16 * 4;               // op16 + op16lo
+    if (mode == 0 || mode == 2) b += P.sum_x16 * 4;  // op16 + op16lo (bf16x3 or its escalation)
     b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + R * 12;
     b += P.sum_cand * 8 + R * 16 * 12 + R * 8 * 4;   // cand, ctau + ccnt, thr
     b += R * 12 + P.fb_cap * 16 + R * P.xs * P.kmax * 8;
@@ -412,21 +415,24 @@ hm_status enqueue_pipeline(hm_job* J) {
                          nds, R, J->mode, J->flag, (const uint32_t*)nullptr, (const int*)nullptr);
       HM_LAUNCH(ctx);
       mark(J, 3);
-      if (J->mode != 1) {
-        // escalation: rows the bf16 keys could not certify are swept again
-        // with FP32 keys (a far smaller error bound) and re-ranked; only
-        // rows still uncertified reach the exact scan below
+      if (J->esc_mode >= 0) {
+        // escalation: rows the primary keys could not certify are swept
+        // again with more precise keys (bf16 -> bf16x3, bf16x3 -> FP32; a far
+        // smaller error bound) and re-ranked; only rows still uncertified
+        // reach the exact scan below
+        const int em = J->esc_mode;
         HM_TRY(scan_excl<uint32_t>(J, J->flag, R, J->flagpos, (uint32_t*)nullptr, J->sc32));
         hipLaunchKernelGGL(compact_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream,
                            (const uint32_t*)J->flag, (const uint32_t*)J->flagpos, R, J->list,
                            J->count);
         hipLaunchKernelGGL(sweep_items_kernel, dim3(1), dim3(64), 0, J->stream, J->d_desc, nds,
-                           (const uint32_t*)J->list, (const int*)J->count, sweep_rows(1), cus,
-                           J->items_l2, J->nitems_l2, J->nitems_l2 + 1);
+                           (const uint32_t*)J->list, (const int*)J->count, sweep_rows(em), cus,
+                           em == 2 ? 1.1e12 : 0.45e12, J->items_l2, J->nitems_l2,
+                           J->nitems_l2 + 1);
         HM_LAUNCH(ctx);
-        HM_TRY(launch_sweep(J, 1, J->items_l2, J->nitems_l2, 0, J->list, (unsigned)cus));
+        HM_TRY(launch_sweep(J, em, J->items_l2, J->nitems_l2, 0, J->list, (unsigned)cus));
         hipLaunchKernelGGL(rerank_kernel, dim3(std::max<int64_t>(1, std::min<int64_t>(cdiv(R, 4), 8 * cus))),
-                           dim3(256), 0, J->stream, J->d_desc, nds, R, 1, J->flag,
+                           dim3(256), 0, J->stream, J->d_desc, nds, R, em, J->flag,
                            (const uint32_t*)J->list, (const int*)(J->nitems_l2 + 1));
         HM_LAUNCH(ctx);
       }
@@ -675,6 +681,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
   J->total_edges = P.total_edges;
   J->kmax = P.kmax;
   J->mode = gram_mode(o.precision);
+  J->esc_mode = esc_mode_of(J->mode);
   J->xs = P.xs;
   J->fb_cap = (int)P.fb_cap;
   auto bail = [&](hm_status s) {
@@ -696,7 +703,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     HM_A(J->xr, P.sum_xr);
     if (P.sum_cos > 0) HM_A(J->op32c, P.sum_cos);
     if (J->mode == 0 || J->mode == 2) HM_A(J->op16, P.sum_x16);
-    if (J->mode == 2) HM_A(J->op16lo, P.sum_x16);
+    if (J->mode == 2 || J->esc_mode == 2) HM_A(J->op16lo, P.sum_x16);
     HM_A(J->colbias, P.total_pad);
     HM_A(J->rstat, R * HM_NSTAT);
     HM_A(J->smax, (int64_t)nds * HM_NSTAT);
@@ -715,9 +722,9 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     HM_A(J->list, R);
     HM_A(J->count, 1);
     HM_A(J->nitems_fb, 1);
-    if (J->mode == 0 || J->mode == 2) {
+    if (J->esc_mode >= 0) {
       int64_t cap = 0;
-      for (auto& h : P.dsh) cap += cdiv(h.n, sweep_rows(1)) * h.nsplit;
+      for (auto& h : P.dsh) cap += cdiv(h.n, sweep_rows(J->esc_mode)) * h.nsplit;
       J->cap_l2 = cap;
       HM_A(J->items_l2, std::max<int64_t>(cap, 1));
       HM_A(J->nitems_l2, 2);  // items, rows escalated
