@@ -62,15 +62,20 @@ __device__ __forceinline__ void vm_wait() {
 // MODE 0: bf16 operand (op16); MODE 1: f32 operand (op32, fp32 MFMA);
 // MODE 2: bf16x3 -- hi (op16) and lo (op16lo) bf16 parts, three MFMAs per
 //         step (hi.hi + lo.hi + hi.lo).
-// A workgroup has NW = QW/32 waves: QW query rows share every candidate
-// slice (256 in modes 0 and 1; 128 in mode 2, whose stages are twice as big).
+// A workgroup has QW query rows that share every candidate slice (256 in
+// modes 0 and 1; 128 in mode 2, whose stages are twice as big) and one wave
+// per 32 of them (32 queries x 128 columns: four 32x32 tiles). With eight
+// waves, two share each SIMD, so one computes while the other waits. (A
+// 64-query wave tile halves the LDS reads per MFMA but leaves one wave per
+// SIMD; measured slower.)
 template <int MODE>
 struct SweepCfg {
   static constexpr int NT = MODE == 2 ? 2 : 1;   // parts per operand (hi, lo)
   static constexpr int ES = MODE == 1 ? 4 : 2;   // element bytes
   static constexpr int KC = 128 / ES;            // features per stage
   static constexpr int QW = MODE == 2 ? 128 : 256;  // query rows per item
-  static constexpr int NW = QW / 32;             // waves per workgroup
+  static constexpr int NU = 1;                   // 32-query tiles per wave
+  static constexpr int NW = QW / (32 * NU);      // waves per workgroup
   static constexpr int QG = QW / 8;              // 8-row copy groups of a query slice
   static constexpr int RG = NT * (QG + 16);      // copy groups per stage
   static constexpr int NI = RG / NW;             // operand copies per wave per stage
@@ -84,6 +89,7 @@ struct SweepCfg {
   static constexpr int VM = NI + 1;              // vmcnt entries per stage and wave
   static_assert(RG % NW == 0, "copy groups split evenly over the waves");
   static_assert(NSTAGE * STAGE <= 160 * 1024, "LDS");
+  static_assert(NW * 64 == sweep_threads(MODE), "the runtime launches sweep_threads(MODE)");
 };
 
 // Chunk slot of logical 16-byte chunk c in LDS row r: c ^ swz(r).
@@ -128,93 +134,107 @@ __device__ __forceinline__ void sweep_wait(int after) {
   else vm_wait<0>();
 }
 
-// Accumulate one stage: 32 queries x 128 columns x one 128-byte slice.
-// Operand reads go in two batches of 10 ahead of the MFMAs that use them
-// (sched_group_barrier keeps that order; the default schedule interleaves
-// reads and MFMAs one by one and exposes the LDS latency each time).
-// LOWREG: the second batch is read after the first batch's MFMAs (40
-// operand VGPRs instead of 80; pass 0 also holds 64 group minima).
+// Accumulate one stage: the wave's NU x 32 queries x 128 columns x one
+// 128-byte slice. Operand reads go in batches ahead of the MFMAs that use
+// them (sched_group_barrier keeps that order; the default schedule
+// interleaves reads and MFMAs one by one and exposes the LDS latency each
+// time). LOWREG reads the second batch after the first batch's MFMAs
+// (fewer live operand registers; pass 0 also holds the group minima).
 template <int MODE, bool LOWREG>
-__device__ __forceinline__ void sweep_mma(floatx16 (&acc)[4], const uint8_t* st,
-                                          const uint8_t* qrow_p, int q, int half, int swz) {
+__device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4],
+                                          const uint8_t* st, const uint8_t* qrow_p, int q,
+                                          int half, int swz) {
   using Cfg = SweepCfg<MODE>;
+  constexpr int NU = Cfg::NU;
   if constexpr (MODE == 1) {
     // chunk c = 2m + half holds features 8m + 4*half + j (j < 4); MFMA j
     // of step m sums features 8m + j and 8m + 4 + j over the two halves
-    floatx4 b4[4], a4[4][4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int cs = ((2 * m + half) ^ swz) * 16;
-      b4[m] = *(const floatx4*)(qrow_p + cs);
+    for (int mh = 0; mh < 2; ++mh) {
+      floatx4 b4[2][NU], a4[2][4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) a4[m][t] = *(const floatx4*)(st + Cfg::CT + (t * 32 + q) * 128 + cs);
+      for (int v = 0; v < 2; ++v) {
+        const int cs = ((2 * (2 * mh + v) + half) ^ swz) * 16;
+#pragma unroll
+        for (int u = 0; u < NU; ++u) b4[v][u] = *(const floatx4*)(qrow_p + u * 32 * 128 + cs);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) a4[v][t] = *(const floatx4*)(st + Cfg::CT + (t * 32 + q) * 128 + cs);
+      }
+#pragma unroll
+      for (int v = 0; v < 2; ++v)
+#pragma unroll
+        for (int u = 0; u < NU; ++u)
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+              acc[u][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[v][t][j], b4[v][u][j], acc[u][t],
+                                                               0, 0, 0);
+      if constexpr (LOWREG) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);  // this batch's reads
+        __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 16, 0);   // its MFMAs
+      }
     }
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-      for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[m][t][j], b4[m][j], acc[t], 0, 0, 0);
-    if constexpr (LOWREG) {
-      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);
-      __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
-      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);
-      __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
-    } else {
-      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);
-      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);
-      __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
-      __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
+    if constexpr (!LOWREG) {
+      __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 16, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 16, 0);
     }
   } else if constexpr (MODE == 0) {
-    bf16x8_t bh[4], ah[4][4];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const int cs = ((2 * ks + half) ^ swz) * 16;
-      bh[ks] = *(const bf16x8_t*)(qrow_p + cs);
+    for (int kh = 0; kh < 2; ++kh) {
+      bf16x8_t bh[2][NU], ah[2][4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) ah[ks][t] = *(const bf16x8_t*)(st + Cfg::CT + (t * 32 + q) * 128 + cs);
+      for (int v = 0; v < 2; ++v) {
+        const int cs = ((2 * (2 * kh + v) + half) ^ swz) * 16;
+#pragma unroll
+        for (int u = 0; u < NU; ++u) bh[v][u] = *(const bf16x8_t*)(qrow_p + u * 32 * 128 + cs);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) ah[v][t] = *(const bf16x8_t*)(st + Cfg::CT + (t * 32 + q) * 128 + cs);
+      }
+#pragma unroll
+      for (int v = 0; v < 2; ++v)
+#pragma unroll
+        for (int u = 0; u < NU; ++u)
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            acc[u][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[v][t], bh[v][u], acc[u][t], 0, 0, 0);
+      if constexpr (LOWREG) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);  // this batch's reads
+        __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 4, 0);    // its MFMAs
+      }
     }
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
-#pragma unroll
-      for (int t = 0; t < 4; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ks][t], bh[ks], acc[t], 0, 0, 0);
-    if constexpr (LOWREG) {
-      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);  // reads of steps 0-1
-      __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);   // MFMAs of steps 0-1
-      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);  // reads of steps 2-3
-      __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);   // MFMAs of steps 2-3
-    } else {
-      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);  // reads of steps 0-1
-      __builtin_amdgcn_sched_group_barrier(0x100, 10, 0);  // reads of steps 2-3
-      __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);   // MFMAs of steps 0-1
-      __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);   // MFMAs of steps 2-3
+    if constexpr (!LOWREG) {
+      __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);  // reads of steps 0-1
+      __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);  // reads of steps 2-3
+      __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 4, 0);    // MFMAs of steps 0-1
+      __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 4, 0);    // MFMAs of steps 2-3
     }
   } else {
+    static_assert(NU == 1, "bf16x3 waves hold 32 queries");
     // two halves of two k-steps: hi and lo parts of both operands
 #pragma unroll
     for (int kh = 0; kh < 2; ++kh) {
       bf16x8_t bh[2], bl[2], ah[2][4], al[2][4];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int cs = ((2 * (2 * kh + u) + half) ^ swz) * 16;
-        bh[u] = *(const bf16x8_t*)(qrow_p + cs);
-        bl[u] = *(const bf16x8_t*)(qrow_p + Cfg::QL + cs);
+      for (int v = 0; v < 2; ++v) {
+        const int cs = ((2 * (2 * kh + v) + half) ^ swz) * 16;
+        bh[v] = *(const bf16x8_t*)(qrow_p + cs);
+        bl[v] = *(const bf16x8_t*)(qrow_p + Cfg::QL + cs);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          ah[u][t] = *(const bf16x8_t*)(st + Cfg::CT + (t * 32 + q) * 128 + cs);
-          al[u][t] = *(const bf16x8_t*)(st + Cfg::CL + (t * 32 + q) * 128 + cs);
+          ah[v][t] = *(const bf16x8_t*)(st + Cfg::CT + (t * 32 + q) * 128 + cs);
+          al[v][t] = *(const bf16x8_t*)(st + Cfg::CL + (t * 32 + q) * 128 + cs);
         }
       }
 #pragma unroll
-      for (int u = 0; u < 2; ++u)
+      for (int v = 0; v < 2; ++v)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[u][t], bh[u], acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[u][t], bh[u], acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[u][t], bl[u], acc[t], 0, 0, 0);
+          acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[v][t], bh[v], acc[0][t], 0, 0, 0);
+          acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[v][t], bh[v], acc[0][t], 0, 0, 0);
+          acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[v][t], bl[v], acc[0][t], 0, 0, 0);
         }
       __builtin_amdgcn_sched_group_barrier(0x100, 20, 0);  // 20 LDS reads
       __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);  // 24 MFMAs
@@ -222,12 +242,13 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[4], const uint8_t* st,
   }
 }
 
-// Epilogue of one 128-column block. FULL: every column of the block is in
-// the split (no validity test). Pass 1 appends hits to dst[min(cnt, cout-1)]
-// and counts them all: cnt > cout afterwards means the list overflowed (its
-// contents are then unused). T is NaN for lanes without a query, so they
-// never hit. A position costs fma + compare + a skipped branch when no lane
-// of the wave hits.
+// Epilogue of one 128-column block for one 32-query tile (rem: columns of
+// the block inside the split, minus this lane half's offset; FULL: the whole
+// block is inside, no validity test). Pass 1 appends
+// hits to dst[min(cnt, cout-1)] and counts them all: cnt > cout afterwards
+// means the list overflowed (its contents are then unused). T is NaN for
+// lanes without a query, so they never hit. A position costs fma + compare
+// + a skipped branch when no lane of the wave hits.
 template <int PASS, bool FULL>
 __device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[4], float (&gmin)[4][16],
                                                const float* colb, int rem, uint32_t colbase,
@@ -319,25 +340,33 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
     }
     P.bias = D.colbias + cbeg + (wave & 1) * 64 + lane;
 
-    // this lane's own query
-    const int slot = wave * 32 + q;
-    const bool myvalid = slot < W.qcnt;
-    const int64_t myrow =
-        myvalid ? (qlist ? (int64_t)qlist[W.qbeg + slot] - D.row_off : (int64_t)(W.qbeg + slot))
-                : 0;
-
-    float gmin[4][16];
-    float T = __builtin_nanf("");  // no query: compares false
-    int cnt = 0;
+    // this lane's own queries: slot 32*(NU*wave + u) + q of the item, u < NU
+    constexpr int NU = Cfg::NU;
+    bool myvalid[NU];
+    int64_t myrow[NU];
+    float T[NU];
+    int cnt[NU];
+    uint64_t* dst[NU];
+    float gmin[NU][4][16];
     const int list = W.split * 2 + half;
-    uint64_t* dst = D.cand + ((size_t)myrow * D.nlist + list) * D.cout;
-    if constexpr (PASS == 0) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
+    for (int u = 0; u < NU; ++u) {
+      const int slot = 32 * (NU * wave + u) + q;
+      myvalid[u] = slot < W.qcnt;
+      myrow[u] = myvalid[u] ? (qlist ? (int64_t)qlist[W.qbeg + slot] - D.row_off
+                                     : (int64_t)(W.qbeg + slot))
+                            : 0;
+      T[u] = __builtin_nanf("");  // no query: compares false
+      cnt[u] = 0;
+      dst[u] = D.cand + ((size_t)myrow[u] * D.nlist + list) * D.cout;
+      if constexpr (PASS == 0) {
 #pragma unroll
-        for (int a = 0; a < 16; ++a) gmin[t][a] = INFINITY;
-    } else {
-      if (myvalid) T = D.thr[(size_t)myrow * D.nsplit + W.split];
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int a = 0; a < 16; ++a) gmin[u][t][a] = INFINITY;
+      } else {
+        if (myvalid[u]) T[u] = D.thr[(size_t)myrow[u] * D.nsplit + W.split];
+      }
     }
 
     HM_LDS_BARRIER();  // the previous item's last stage is no longer read
@@ -351,79 +380,84 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
       }
     }
 
-    floatx16 acc[4];
-    int blk = 0, ch = 0;  // stage g
-    for (int g = 0; g < total; ++g) {
-      HM_LDS_BARRIER();  // every wave is done with stage g-1 (its slot is reused now)
-      {
-        const int gi = g + NSTAGE - 1;
-        if (gi < total) {
-          sweep_issue<MODE>(smem + (gi % NSTAGE) * STAGE, P, bi * blkbytes, ci * 128,
-                            (int64_t)bi * 128, wave);
-          if (++ci == nchunk) { ci = 0; ++bi; }
-        }
-        const int after = total - 1 - g < NSTAGE - 1 ? total - 1 - g : NSTAGE - 1;
-        sweep_wait<MODE>(after);
-      }
-      HM_LDS_BARRIER();  // stage g is in LDS for every wave
-      const uint8_t* st = smem + (g % NSTAGE) * STAGE;
-      if (ch == 0) {
+    int g = 0;  // stage being computed
+    for (int blk = 0; blk < nblk; ++blk) {
+      floatx16 acc[NU][4];
+#pragma unroll
+      for (int u = 0; u < NU; ++u)
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
-          for (int e = 0; e < 16; ++e) acc[t][e] = 0.0f;
+          for (int e = 0; e < 16; ++e) acc[u][t][e] = 0.0f;
+      const uint8_t* st = smem;
+      for (int ch = 0; ch < nchunk; ++ch, ++g) {
+        HM_LDS_BARRIER();  // every wave is done with stage g-1 (its slot is reused now)
+        {
+          const int gi = g + NSTAGE - 1;
+          if (gi < total) {
+            sweep_issue<MODE>(smem + (gi % NSTAGE) * STAGE, P, bi * blkbytes, ci * 128,
+                              (int64_t)bi * 128, wave);
+            if (++ci == nchunk) { ci = 0; ++bi; }
+          }
+          const int after = total - 1 - g < NSTAGE - 1 ? total - 1 - g : NSTAGE - 1;
+          sweep_wait<MODE>(after);
+        }
+        HM_LDS_BARRIER();  // stage g is in LDS for every wave
+        st = smem + (g % NSTAGE) * STAGE;
+        // pass 0 also holds the group minima: it reads in smaller batches
+        sweep_mma<MODE, PASS == 0 && MODE != 2>(acc, st, st + Cfg::QT + (NU * wave * 32 + q) * 128, q, half,
+                                   swz);
       }
-      // two waves per SIMD in modes 0 and 1: pass 0 (64 group minima) reads
-      // in smaller batches to stay within 256 registers
-      sweep_mma<MODE, PASS == 0 && MODE != 2>(acc, st, st + Cfg::QT + (wave * 32 + q) * 128, q,
-                                              half, swz);
-      if (++ch != nchunk) continue;
-      ch = 0;
 
-      // ---- epilogue of block blk (its biases ride in this stage) ----
+      // ---- epilogue of block blk (its biases ride in its last stage) ----
       // 32-bit per-lane bases; position offsets are compile-time constants
       const int64_t cb = cbeg + (int64_t)blk * 128;
       const float* colb = (const float*)(st + Cfg::BIAS) + 4 * half;
       const uint32_t colbase = (uint32_t)cb + 4 * half;
+      const int rem = (int)(cend - cb < 128 ? cend - cb : 128) - 4 * half;
       if (cend - cb >= 128) {
-        sweep_epilogue<PASS, true>(acc, gmin, colb, 128, colbase, scale, T, D.cout, dst, cnt);
+#pragma unroll
+        for (int u = 0; u < NU; ++u)
+          sweep_epilogue<PASS, true>(acc[u], gmin[u], colb, rem, colbase, scale, T[u], D.cout,
+                                     dst[u], cnt[u]);
       } else {
-        const int rem = (int)(cend - cb) - 4 * half;
-        sweep_epilogue<PASS, false>(acc, gmin, colb, rem, colbase, scale, T, D.cout, dst, cnt);
+#pragma unroll
+        for (int u = 0; u < NU; ++u)
+          sweep_epilogue<PASS, false>(acc[u], gmin[u], colb, rem, colbase, scale, T[u], D.cout,
+                                      dst[u], cnt[u]);
       }
-      ++blk;
     }
 
-    if constexpr (PASS == 0) {
-      // T = C-th smallest of the row's 128 group minima (both lanes), found
-      // by bisection on the order-preserving u32 image of the key.
-      uint32_t o[4][16];
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
+    for (int u = 0; u < NU; ++u) {
+      if constexpr (PASS == 0) {
+        // T = C-th smallest of the row's 128 group minima (both lanes), found
+        // by bisection on the order-preserving u32 image of the key.
+        const int C = D.C;
+        uint32_t lo = 0u, hi = 0xffffffffu;
+        for (int itb = 0; itb < 32; ++itb) {
+          const uint32_t mid = lo + (hi - lo) / 2;
+          int c = 0;
 #pragma unroll
-        for (int a = 0; a < 16; ++a) o[t][a] = ordered_u32(gmin[t][a]);
-      const int C = D.C;
-      uint32_t lo = 0u, hi = 0xffffffffu;
-      for (int itb = 0; itb < 32; ++itb) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        int c = 0;
+          for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-          for (int a = 0; a < 16; ++a) c += o[t][a] <= mid ? 1 : 0;
-        c += __shfl_xor(c, 32, 64);
-        if (c >= C) hi = mid; else lo = mid + 1;
-      }
-      // lo == hi: the smallest image with >= C group minima at or below it
-      // (the image of +inf when the split has fewer than C columns)
-      if (myvalid && half == 0) D.thr[(size_t)myrow * D.nsplit + W.split] = unordered_f32(lo);
-    } else {
-      if (myvalid) {
-        // every column not listed has key > T; an overflowing list reports 0
-        // (no threshold) so the row is left to the exact scan
-        D.ctau[(size_t)myrow * D.nlist + list] =
-            cnt > D.cout ? 0ull : (((uint64_t)ordered_u32(T) << 32) | 0xffffffffull);
-        D.ccnt[(size_t)myrow * D.nlist + list] = (uint32_t)(cnt < D.cout ? cnt : D.cout);
+            for (int a = 0; a < 16; ++a) c += ordered_u32(gmin[u][t][a]) <= mid ? 1 : 0;
+          c += __shfl_xor(c, 32, 64);
+          if (c >= C) hi = mid; else lo = mid + 1;
+        }
+        // lo == hi: the smallest image with >= C group minima at or below it
+        // (the image of +inf when the split has fewer than C columns)
+        if (myvalid[u] && half == 0)
+          D.thr[(size_t)myrow[u] * D.nsplit + W.split] = unordered_f32(lo);
+      } else {
+        if (myvalid[u]) {
+          // every column not listed has key > T; an overflowing list reports
+          // 0 (no threshold) so the row is left to the exact scan
+          D.ctau[(size_t)myrow[u] * D.nlist + list] =
+              cnt[u] > D.cout ? 0ull : (((uint64_t)ordered_u32(T[u]) << 32) | 0xffffffffull);
+          D.ccnt[(size_t)myrow[u] * D.nlist + list] =
+              (uint32_t)(cnt[u] < D.cout ? cnt[u] : D.cout);
+        }
       }
     }
   }

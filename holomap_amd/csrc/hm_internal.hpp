@@ -50,6 +50,8 @@ struct DsDesc {
 // Query rows per work item of the candidate sweep (hm_gram.hip): 256 in the
 // bf16 and fp32 modes, 128 in bf16x3.
 __host__ __device__ constexpr int sweep_rows(int mode) { return mode == 2 ? 128 : 256; }
+// Threads per sweep workgroup: one wave per 32 query rows.
+__host__ __device__ constexpr int sweep_threads(int mode) { return sweep_rows(mode) / 32 * 64; }
 
 // Number of float statistics kept per row for the candidate error bound.
 #define HM_NSTAT 8

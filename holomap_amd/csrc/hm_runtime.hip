@@ -371,7 +371,7 @@ unsigned grid_for(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, cd
 hm_status launch_sweep(hm_job* J, int mode, const WorkItem* items, const int* nitems_dev,
                        int nitems_host, const uint32_t* qlist, unsigned grid) {
 #define HM_GRAM(M, P)                                                                        \
-  hipLaunchKernelGGL((gram_sweep_kernel<M, P>), dim3(grid), dim3(sweep_rows(M) / 32 * 64), 0,  \
+  hipLaunchKernelGGL((gram_sweep_kernel<M, P>), dim3(grid), dim3(sweep_threads(M)), 0,         \
                      J->stream, J->d_desc, items, nitems_dev, nitems_host, qlist)
   switch (mode) {
     case 0: HM_GRAM(0, 0); HM_GRAM(0, 1); break;
