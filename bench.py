@@ -140,23 +140,27 @@ def batch_bench(ctx, steps):
     from holomap_amd import datasets as DS
     sets = DS.c4()
     job = H.Job(ctx, [s.n for s in sets], [s.d for s in sets], [s.k for s in sets],
-                [s.metric for s in sets], use_graph=True)
+                [s.metric for s in sets], use_graph=True, autotune=True)
     for i, s in enumerate(sets):
         job.upload(i, s.data)
-    job.run()
+    t0 = time.perf_counter()
+    job.run()  # the first run autotunes the sweep configuration, then captures
     job.sync()
+    t_tune = time.perf_counter() - t0
     t0 = time.perf_counter()
     for _ in range(steps):
         job.run()
     job.sync()
     dt = (time.perf_counter() - t0) / steps
-    fb = job.counters()["fallback_rows"]
+    cnt = job.counters()
     job.close()
     rows = sum(s.n for s in sets)
     return {"workload": "c4: 256 datasets, N log-uniform [200,8000], d in {32,64,128,256}, "
                         "k in {10,15,30}, metric 50/50, seed 11",
             "datasets": len(sets), "rows": rows, "ms_per_step": dt * 1e3,
-            "graphs_per_s": len(sets) / dt, "pts_per_s": rows / dt, "fallback_rows": fb}
+            "graphs_per_s": len(sets) / dt, "pts_per_s": rows / dt,
+            "fallback_rows": cnt["fallback_rows"], "escalated_rows": cnt["escalated_rows"],
+            "autotune_s": t_tune, "tuned_nsplit": cnt["nsplit"]}
 
 
 def serve_bench(ctx, lambdas, n_req, pool=16, seed=0):

@@ -333,7 +333,7 @@ class Job:
 
     def __init__(self, ctx: Context, n: Sequence[int], d: Sequence[int], k: Sequence[int],
                  metric: Sequence = None, *, precision="bf16", calib="ref_f32", margin=-1,
-                 nsplit=0, use_graph=False, timing=False, stages=HM_STAGE_ALL):
+                 nsplit=0, use_graph=False, timing=False, autotune=False, stages=HM_STAGE_ALL):
         self.ctx = ctx
         nds = len(n)
         self.n = np.ascontiguousarray(n, np.uint64)
@@ -342,7 +342,7 @@ class Job:
         m = [0] * nds if metric is None else [_metric(v) for v in metric]
         self.metric = np.ascontiguousarray(m, np.int32)
         self.calib_mode = CALIBRATIONS[calib] if isinstance(calib, str) else int(calib)
-        o = _opts(precision, calib, margin, nsplit, use_graph, timing)
+        o = _opts(precision, calib, margin, nsplit, use_graph, timing, autotune)
         h = ctypes.c_void_p()
         st = _lib.load().hm_job_create(ctx.handle, nds, self.n.ctypes.data, self.d.ctypes.data,
                                        self.k.ctypes.data, self.metric.ctypes.data,

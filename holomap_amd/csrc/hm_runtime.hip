@@ -116,6 +116,9 @@ struct hm_job {
   int64_t total_rows = 0, total_pad = 0, total_edges = 0;
   int kmax = 0, mode = 0, xs = 1, fb_cap = 0, n_items_l1 = 0;
   int esc_mode = -1;  // sweep mode of the escalation pass (-1: none)
+  bool xcd_perm = true;  // permute sweep items for XCD-local L2 sharing
+  bool tuned = false;    // autotuning done (hm_options.autotune)
+  float tune_ms = 0.f;   // best pipeline time the autotuner measured
   bool counted = false;  // bytes added to the context's in_use
   std::vector<WorkItem> items_l1_h;
   std::vector<void*> allocs;
@@ -207,6 +210,41 @@ struct Plan {
   int kmax = 0, cmax = 0, xs = 1;
 };
 
+// Sweep configuration of every dataset: candidates C = k + margin (capped at
+// 64), column splits and list capacity. margin < 0: 2k; nsplit <= 0: chosen
+// so that the launch has enough work items for every CU. Buffers are sized
+// for any configuration (kSplitCap splits, kCandCap key slots per row), so
+// the autotuner can change it after allocation.
+constexpr int kSplitCap = 5;
+constexpr int kCandCap = 512;  // the re-rank sorts at most 512 keys per row
+void plan_sweep(std::vector<DsHost>& dsh, int margin_opt, int nsplit_opt, int mode, int cus,
+                int margin_mult = 2) {
+  const int qw = sweep_rows(mode);
+  int64_t rb_total = 0;
+  for (auto& h : dsh) rb_total += cdiv(h.n, qw);
+  for (auto& h : dsh) {
+    const int margin = margin_opt < 0 ? margin_mult * h.k : margin_opt;
+    h.C = std::max(h.k, std::min(h.k + margin, 64));
+    // A list (one lane: half of a split's column groups) collects the keys
+    // <= T, T = C-th smallest of the row's 128 group minima: about
+    // 64 * -ln(1 - C/128) keys. Capacity with 40 % headroom; the re-rank
+    // holds <= kCandCap keys per row, which bounds the split count.
+    const double expect = 64.0 * -std::log(1.0 - h.C / 128.0);
+    const int cneed = std::max(48, (int)((1.4 * expect + 8.0 + 7.0) / 8.0) * 8);
+    const int smax = std::max(1, std::min(kSplitCap, kCandCap / 2 / cneed));
+    if (nsplit_opt > 0) {
+      h.nsplit = std::max(1, std::min<int>(nsplit_opt, smax));
+    } else if (rb_total >= 2 * cus) {
+      h.nsplit = 1;
+    } else {
+      h.nsplit = std::min(smax, choose_nsplit(h.n, qw, h.C, cus, rb_total - cdiv(h.n, qw)));
+    }
+    h.nlist = 2 * h.nsplit;
+    // the lists share the re-rank's key slots: fewer splits, longer lists
+    h.cout = std::max(cneed, std::min(256, (kCandCap / h.nlist) & ~7));
+  }
+}
+
 Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_t* k,
                const int32_t* metric, const hm_options& opt, int cus) {
   Plan P;
@@ -221,11 +259,6 @@ Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_
     h.kp = (int)cdiv(h.d, 32) * 32;
     h.kp16 = (int)cdiv(h.d, 64) * 64;
     h.n_pad = cdiv(h.n, 128) * 128;
-    int margin = opt.margin < 0 ? 2 * h.k : opt.margin;
-    int C = h.k + margin;
-    C = std::min(C, 64);  // two collect lists of 48 keys hold ~C + 8 candidates
-    C = std::max(C, h.k);
-    h.C = C;
     h.row_off = P.total_rows;
     h.edge_off = P.total_edges;
     P.total_rows += h.n;
@@ -235,36 +268,11 @@ Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_
     P.sum_x16 += h.n_pad * h.kp16;
     if (h.metric == 1) P.sum_cos += h.n_pad * h.kp;
     P.kmax = std::max(P.kmax, h.k);
-    P.cmax = std::max(P.cmax, C);
     nmax = std::max(nmax, h.n);
   }
-  // sweep splits: a lone dataset (or a small batch) splits its columns so
-  // the launch has enough work items for every CU
-  const int qw = sweep_rows(gram_mode(opt.precision));
-  int64_t rb_total = 0;
-  for (auto& h : P.dsh) rb_total += cdiv(h.n, qw);
-  for (auto& h : P.dsh) {
-    // A list (one lane: half of a split's column groups) collects the keys
-    // <= T, T = C-th smallest of the row's 128 group minima: about
-    // 64 * -ln(1 - C/128) keys. Capacity with 40 % headroom; the re-rank
-    // holds <= 512 keys per row, which bounds the split count.
-    const double expect = 64.0 * -std::log(1.0 - h.C / 128.0);
-    const int cneed = std::max(48, (int)((1.4 * expect + 8.0 + 7.0) / 8.0) * 8);
-    const int smax = std::max(1, 256 / cneed);
-    if (opt.nsplit > 0) {
-      h.nsplit = std::max(1, std::min<int>(opt.nsplit, smax));
-    } else if (rb_total >= 2 * cus) {
-      h.nsplit = 1;
-    } else {
-      h.nsplit = std::min(smax, choose_nsplit(h.n, qw, h.C, cus, rb_total - cdiv(h.n, qw)));
-    }
-    h.nlist = 2 * h.nsplit;
-    // the lists share the re-rank's 512 key slots: fewer splits, longer
-    // lists (a list collects ~C/2 keys of its half-split on average, and
-    // overflows -- sending its row to the exact scan -- beyond cout)
-    h.cout = std::max(cneed, std::min(256, (512 / h.nlist) & ~7));
-    P.sum_cand += h.n * h.nlist * h.cout;
-  }
+  plan_sweep(P.dsh, opt.margin, opt.nsplit, gram_mode(opt.precision), cus);
+  for (auto& h : P.dsh) P.cmax = std::max(P.cmax, h.C);
+  P.sum_cand = P.total_rows * kCandCap;
   P.xs = (int)std::max<int64_t>(1, std::min<int64_t>(8, nmax / 2048));
   for (auto& h : P.dsh) P.fb_cap += cdiv(h.n, 32) * P.xs;
   return P;
@@ -281,7 +289,7 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
     b += P.sum_xr * 4 + P.sum_cos * 4;               // xr, op32 (cosine)
     if (mode == 0 || mode == 2) b += P.sum_x16 * 4;  // op16 + op16lo (bf16x3 or its escalation)
     b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + R * 12;
-    b += P.sum_cand * 8 + R * 16 * 12 + R * 8 * 4;   // cand, ctau + ccnt, thr
+    b += P.sum_cand * 8 + R * 2 * kSplitCap * 12 + R * kSplitCap * 4;  // cand, ctau+ccnt, thr
     b += R * 12 + P.fb_cap * 16 + R * P.xs * P.kmax * 8;
   }
   b += E * 8;                                        // knn idx + dist
@@ -630,6 +638,64 @@ uint64_t hm_job_estimate_bytes(uint32_t nds, const uint64_t* n, const uint32_t* 
   return plan_bytes(P, o, HM_STAGE_ALL);
 }
 
+// Writes the sweep configuration of J->dsh (C, splits, lists) into the
+// device descriptors and rebuilds the sweep's work items: (dataset, split,
+// row block), longest first, permuted for the XCDs. Blocks until uploaded.
+hm_status apply_sweep(hm_job* J) {
+  hm_ctx* ctx = J->ctx;
+  const int nds = J->nds;
+  for (int i = 0; i < nds; ++i) {
+    const DsHost& h = J->dsh[i];
+    DsDesc& D = J->hdesc[i];
+    D.C = h.C;
+    D.nsplit = h.nsplit;
+    D.nlist = h.nlist;
+    D.cout = h.cout;
+  }
+  if (hipMemcpyAsync(J->d_desc, J->hdesc.data(), sizeof(DsDesc) * nds, hipMemcpyHostToDevice,
+                     J->stream) != hipSuccess)
+    return fail(ctx, HM_EDEVICE, "descriptor upload failed");
+  if ((J->stages & HM_STAGE_KNN) && J->mode >= 0) {
+    std::vector<std::pair<int64_t, WorkItem>> it;
+    const int qw = sweep_rows(J->mode);
+    for (int i = 0; i < nds; ++i) {
+      const DsHost& h = J->dsh[i];
+      const int64_t cost = cdiv(h.n, h.nsplit) * h.kp;
+      for (int64_t rb = 0; rb < h.n; rb += qw)
+        for (int s = 0; s < h.nsplit; ++s)
+          it.push_back({cost, WorkItem{i, s, (int)rb, (int)std::min<int64_t>(qw, h.n - rb)}});
+    }
+    std::stable_sort(it.begin(), it.end(),
+                     [](const auto& a, const auto& b) { return a.first > b.first; });
+    // The sweep grid is G persistent workgroups; workgroup b takes items b,
+    // b + G, ... and runs on XCD b % X (round-robin dispatch). Item slots are
+    // permuted so that in every round each XCD receives a contiguous run of
+    // the (row block, split) order above: its workgroups then stream the same
+    // few row blocks and column splits at the same time and share them in
+    // that XCD's L2.
+    const int64_t ni = (int64_t)it.size();
+    const int64_t G = sweep_grid(ctx, J->mode, ni);
+    const int64_t X = (J->xcd_perm && ctx->num_cus % 8 == 0) ? 8 : 1;
+    J->items_l1_h.resize(it.size());
+    for (int64_t r0 = 0; r0 < ni; r0 += G) {
+      const int64_t rl = std::min(G, ni - r0);  // items in this round
+      const bool perm = X > 1 && rl % X == 0;
+      const int64_t per = rl / X;
+      for (int64_t l = 0; l < rl; ++l) {
+        const int64_t p = perm ? (l % per) * X + l / per : l;
+        J->items_l1_h[r0 + p] = it[r0 + l].second;
+      }
+    }
+    J->n_items_l1 = (int)ni;
+    if (hipMemcpyAsync(J->items_l1, J->items_l1_h.data(), sizeof(WorkItem) * ni,
+                       hipMemcpyHostToDevice, J->stream) != hipSuccess)
+      return fail(ctx, HM_EDEVICE, "work item upload failed");
+  }
+  if (hipStreamSynchronize(J->stream) != hipSuccess)
+    return fail(ctx, HM_EDEVICE, "descriptor upload failed");
+  return HM_OK;
+}
+
 hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint32_t* d,
                         const uint32_t* k, const int32_t* metric, const hm_options* opt,
                         uint32_t stages, hm_job** out) {
@@ -709,14 +775,11 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     HM_A(J->smax, (int64_t)nds * HM_NSTAT);
     HM_A(J->nz, R);
     HM_A(J->rnorm, R);
+    // sized for any sweep configuration (see plan_sweep)
     HM_A(J->cand, std::max<int64_t>(P.sum_cand, 1));
-    int64_t sum_tau = 0;
-    for (auto& h : P.dsh) sum_tau += h.n * h.nlist;
-    HM_A(J->ctau, sum_tau);
-    HM_A(J->ccnt, sum_tau);
-    int64_t sum_thr = 0;
-    for (auto& h : P.dsh) sum_thr += h.n * h.nsplit;
-    HM_A(J->thr, sum_thr);
+    HM_A(J->ctau, R * 2 * kSplitCap);
+    HM_A(J->ccnt, R * 2 * kSplitCap);
+    HM_A(J->thr, R * kSplitCap);
     HM_A(J->flag, R);
     HM_A(J->flagpos, R);
     HM_A(J->list, R);
@@ -724,7 +787,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     HM_A(J->nitems_fb, 1);
     if (J->esc_mode >= 0) {
       int64_t cap = 0;
-      for (auto& h : P.dsh) cap += cdiv(h.n, sweep_rows(J->esc_mode)) * h.nsplit;
+      for (auto& h : P.dsh) cap += cdiv(h.n, sweep_rows(J->esc_mode)) * kSplitCap;
       J->cap_l2 = cap;
       HM_A(J->items_l2, std::max<int64_t>(cap, 1));
       HM_A(J->nitems_l2, 2);  // items, rows escalated
@@ -762,7 +825,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
 
   // descriptors
   J->hdesc.resize(nds);
-  int64_t off_x16 = 0, off_x = 0, off_xr = 0, off_cos = 0, off_pad = 0, off_cand = 0, off_tau = 0, off_thr = 0;
+  int64_t off_x16 = 0, off_x = 0, off_xr = 0, off_cos = 0, off_pad = 0;
   std::vector<int64_t> pad_off_h(nds + 1, 0);
   std::vector<double> target_h(nds);
   J->iptr_off_h.resize(nds);
@@ -781,10 +844,10 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
       D.rnorm = J->rnorm + h.row_off;
       D.rstat = J->rstat + h.row_off * HM_NSTAT;
       D.smax = J->smax + (int64_t)i * HM_NSTAT;
-      D.cand = J->cand + off_cand;
-      D.ctau = J->ctau + off_tau;
-      D.ccnt = J->ccnt + off_tau;
-      D.thr = J->thr + off_thr;
+      D.cand = J->cand + h.row_off * kCandCap;
+      D.ctau = J->ctau + h.row_off * 2 * kSplitCap;
+      D.ccnt = J->ccnt + h.row_off * 2 * kSplitCap;
+      D.thr = J->thr + h.row_off * kSplitCap;
     }
     D.knn_idx = J->knn_idx + h.edge_off;
     D.knn_dist = J->knn_dist + h.edge_off;
@@ -806,9 +869,6 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     off_xr += h.n_pad * h.kp;
     off_x16 += h.n_pad * h.kp16;
     off_pad += h.n_pad;
-    off_cand += h.n * h.nlist * h.cout;
-    off_tau += h.n * h.nlist;
-    off_thr += h.n * h.nsplit;
     target_h[i] = std::log2((double)h.k);  // std f64::log2 (glibc), src/smooth_knn.rs:25
     J->iptr_off_h[i] = h.row_off + i;
   }
@@ -823,43 +883,13 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
   up(J->target, target_h.data(), sizeof(double) * nds);
   if (knn) up(J->pad_off, pad_off_h.data(), sizeof(int64_t) * (nds + 1));
   if (stages & HM_STAGE_FUZZY) up(J->iptr_off, J->iptr_off_h.data(), sizeof(int64_t) * nds);
-  // sweep work items: (dataset, split, 128-row block), longest first
   if (knn && J->mode >= 0) {
-    std::vector<std::pair<int64_t, WorkItem>> it;
-    const int qw = sweep_rows(J->mode);
-    for (uint32_t i = 0; i < nds; ++i) {
-      const DsHost& h = P.dsh[i];
-      const int64_t cost = cdiv(h.n, h.nsplit) * h.kp;
-      for (int64_t rb = 0; rb < h.n; rb += qw)
-        for (int s = 0; s < h.nsplit; ++s)
-          it.push_back({cost, WorkItem{(int)i, s, (int)rb, (int)std::min<int64_t>(qw, h.n - rb)}});
-    }
-    std::stable_sort(it.begin(), it.end(),
-                     [](const auto& a, const auto& b) { return a.first > b.first; });
-    // The sweep grid is G persistent workgroups; workgroup b takes items b,
-    // b + G, ... and runs on XCD b % X (round-robin dispatch). Item slots are
-    // permuted so that in every round each XCD receives a contiguous run of
-    // the (row block, split) order above: its workgroups then stream the same
-    // few row blocks and column splits at the same time and share them in
-    // that XCD's L2.
-    const int64_t ni = (int64_t)it.size();
-    const int64_t G = sweep_grid(ctx, J->mode, ni);
-    const int64_t X = (ctx->num_cus % 8 == 0) ? 8 : 1;
-    J->items_l1_h.resize(it.size());
-    for (int64_t r0 = 0; r0 < ni; r0 += G) {
-      const int64_t rl = std::min(G, ni - r0);  // items in this round
-      const bool perm = X > 1 && rl % X == 0;
-      const int64_t per = rl / X;
-      for (int64_t l = 0; l < rl; ++l) {
-        const int64_t p = perm ? (l % per) * X + l / per : l;
-        J->items_l1_h[r0 + p] = it[r0 + l].second;
-      }
-    }
-    J->n_items_l1 = (int)it.size();
-    hm_status s2 = dalloc(J, &J->items_l1, (int64_t)it.size());
+    int64_t cap = 0;
+    for (auto& h : P.dsh) cap += cdiv(h.n, sweep_rows(J->mode)) * kSplitCap;
+    hm_status s2 = dalloc(J, &J->items_l1, std::max<int64_t>(cap, 1));
     if (s2 != HM_OK) return bail(s2);
-    up(J->items_l1, J->items_l1_h.data(), sizeof(WorkItem) * it.size());
   }
+  if (st == HM_OK) st = apply_sweep(J);
   if (st == HM_OK && hipStreamSynchronize(J->stream) != hipSuccess)  // host sources are locals
     st = fail(ctx, HM_EDEVICE, "descriptor upload failed");
   if (st != HM_OK) return bail(st);
@@ -923,10 +953,78 @@ hm_status hm_job_set_calib(hm_job* J, uint32_t ds, const float* sig, const float
   return HM_OK;
 }
 
+// Autotuner (hm_options.autotune): times the whole pipeline for candidate
+// sweep configurations -- split count x XCD item permutation at the planned
+// margin, then margins k, 2k, 3k at the best of those -- and keeps the
+// fastest. The outputs do not depend on the configuration (it only decides
+// which columns are examined), so tuning changes time only.
+hm_status autotune(hm_job* J) {
+  hm_ctx* ctx = J->ctx;
+  const bool timed0 = J->timed;
+  J->timed = true;
+  auto time_cfg = [&](float* ms) -> hm_status {
+    HM_TRY(apply_sweep(J));
+    float best = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {  // the first run also warms caches
+      HM_TRY(enqueue_pipeline(J));
+      HM_HIP(ctx, hipStreamSynchronize(J->stream));
+      float t = 0.f;
+      HM_HIP(ctx, hipEventElapsedTime(&t, J->ev[0], J->ev[6]));
+      best = std::min(best, t);
+    }
+    *ms = best;
+    return HM_OK;
+  };
+  struct Cfg { int mult, nsplit; bool perm; std::vector<DsHost> dsh; float ms; };
+  std::vector<Cfg> tried;
+  auto same_plan = [](const std::vector<DsHost>& a, const std::vector<DsHost>& b) {
+    for (size_t i = 0; i < a.size(); ++i)
+      if (a[i].C != b[i].C || a[i].nsplit != b[i].nsplit) return false;
+    return true;
+  };
+  auto try_cfg = [&](int mult, int nsplit, bool perm) -> hm_status {
+    std::vector<DsHost> dsh = J->dsh;
+    plan_sweep(dsh, J->opt.margin, nsplit, J->mode, ctx->num_cus, mult);
+    for (auto& c : tried)
+      if (c.perm == perm && same_plan(c.dsh, dsh)) return HM_OK;
+    J->dsh = dsh;
+    J->xcd_perm = perm;
+    float ms = 0.f;
+    HM_TRY(time_cfg(&ms));
+    tried.push_back(Cfg{mult, nsplit, perm, dsh, ms});
+    return HM_OK;
+  };
+  const int nsplits[] = {0, 1, 2, 3, 4, 5};
+  for (int ns : nsplits) {
+    if (J->opt.nsplit > 0 && ns != J->opt.nsplit) continue;
+    for (bool perm : {true, false}) HM_TRY(try_cfg(2, ns, perm));
+  }
+  auto best_of = [&]() {
+    size_t b = 0;
+    for (size_t i = 1; i < tried.size(); ++i)
+      if (tried[i].ms < tried[b].ms) b = i;
+    return b;
+  };
+  if (J->opt.margin < 0) {
+    const Cfg b = tried[best_of()];
+    for (int mult : {1, 3}) HM_TRY(try_cfg(mult, b.nsplit, b.perm));
+  }
+  const Cfg& b = tried[best_of()];
+  J->dsh = b.dsh;
+  J->xcd_perm = b.perm;
+  J->tune_ms = b.ms;
+  HM_TRY(apply_sweep(J));
+  J->timed = timed0;
+  J->tuned = true;
+  return HM_OK;
+}
+
 hm_status hm_job_run(hm_job* J) {
   if (!J) return HM_EINVAL;
   hm_ctx* ctx = J->ctx;
   HM_HIP(ctx, hipSetDevice(ctx->device));
+  if (J->opt.autotune && !J->tuned && (J->stages & HM_STAGE_KNN) && J->mode >= 0)
+    HM_TRY(autotune(J));
   J->timed = J->opt.timing && !J->opt.use_graph;
   if (J->opt.use_graph) {
     if (!J->captured) {

@@ -405,3 +405,17 @@ def test_fp32_escalation_certifies_tight_cosine_clusters(ctx):
     assert_graph_equal(r, *O.knn_graph(x, d, k, "cosine"))
     assert c["escalated_rows"] > 0
     assert c["fallback_rows"] < c["escalated_rows"]
+
+
+def test_autotune_keeps_outputs(ctx):
+    # the autotuner changes the sweep configuration (splits, margin, item
+    # order), never the outputs
+    x = DS.c2(n=8000, d=64)
+    ref = O.knn_graph(x, 64, 15, "euclidean")
+    job = H.Job(ctx, [8000], [64], [15], ["euclidean"], autotune=True, use_graph=True)
+    job.upload(0, x)
+    for _ in range(2):
+        job.run()
+        job.sync()
+        assert_graph_equal(job.download(0), *ref)
+    job.close()
