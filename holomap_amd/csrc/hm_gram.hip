@@ -26,12 +26,19 @@
 // Selection key per (query i, column j):
 //   euclidean  key = ||x_j||^2 - 2 <x_i, x_j>   (row constant ||x_i||^2 dropped)
 //   cosine     key = -<w_i, w_j>, w = x/||x||; zero columns carry key -inf
+// The MFMAs accumulate key' = key / s (s = -2 euclidean, -1 cosine) directly:
+// each block's accumulators start at bias_j / s (exact: s is a power of two)
+// and add the dot products, so key = s * key' exactly and smaller keys are
+// larger key'. Columns outside the split start at -inf (key +inf). The
+// epilogue then needs no arithmetic on the key: a max (pass 0) or a compare
+// (pass 1) per position.
 //
-// Two passes over the same tiles, each with a two-instruction epilogue:
+// Two passes over the same tiles, each with a one-instruction epilogue:
 //  PASS 0 (threshold): lane (q, half) keeps the running minimum key of each
-//    of its 64 register positions -- 128 "groups" per row, each a fixed
-//    column subset. T = the C-th smallest group minimum. C distinct groups
-//    each contain a key <= T, so at least C keys of the split are <= T.
+//    of its 64 register positions (the maximum key') -- 128 "groups" per
+//    row, each a fixed column subset. T = the C-th smallest group minimum.
+//    C distinct groups each contain a key <= T, so at least C keys of the
+//    split are <= T.
 //  PASS 1 (collect): every column with key <= T is appended to its lane's
 //    list in global memory (the row's candidate slots; a few dozen appends
 //    per lane over a whole split). All other columns have key > T: that is
@@ -242,37 +249,29 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4]
   }
 }
 
-// Epilogue of one 128-column block for one 32-query tile (rem: columns of
-// the block inside the split, minus this lane half's offset; FULL: the whole
-// block is inside, no validity test). Pass 1 appends
-// hits to dst[min(cnt, cout-1)] and counts them all: cnt > cout afterwards
-// means the list overflowed (its contents are then unused). T is NaN for
-// lanes without a query, so they never hit. A position costs fma + compare
-// + a skipped branch when no lane of the wave hits.
-template <int PASS, bool FULL>
-__device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[4], float (&gmin)[4][16],
-                                               const float* colb, int rem, uint32_t colbase,
-                                               float scale, float T, int cout, uint64_t* dst,
-                                               int& cnt) {
+// Epilogue of one 128-column block for one 32-query tile. acc holds key'
+// (see the top of the file). Pass 0: gmax = max(gmax, key') per position
+// (v_med3 with +inf: a max without NaN canonicalisation). Pass 1: a hit is
+// key' >= Tp (Tp = T / s; NaN for lanes without a query: never); hits are
+// appended to dst[min(cnt, cout-1)] and all counted -- cnt > cout afterwards
+// means the list overflowed (its contents are then unused). A position
+// costs a compare and a skipped branch when no lane of the wave hits.
+template <int PASS>
+__device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[4], float (&gmax)[4][16],
+                                               uint32_t colbase, float scale, float Tp, int cout,
+                                               uint64_t* dst, int& cnt) {
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
 #pragma unroll
-    for (int a4 = 0; a4 < 4; ++a4) {
-      const int jc = t * 32 + 8 * a4;  // column of register a4*4 at half 0
-      const floatx4 cbv = *(const floatx4*)(colb + jc);
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int a = a4 * 4 + e;
-        const bool valid = FULL || jc + e < rem;
-        const float key = __builtin_fmaf(scale, acc[t][a], cbv[e]);
-        if constexpr (PASS == 0) {
-          gmin[t][a] = valid ? fminf(gmin[t][a], key) : gmin[t][a];
-        } else {
-          if (valid && key <= T) {
-            dst[cnt < cout ? cnt : cout - 1] =
-                ((uint64_t)ordered_u32(key) << 32) | (colbase + jc + e);
-            ++cnt;
-          }
+    for (int a = 0; a < 16; ++a) {
+      const int jl = t * 32 + 8 * (a >> 2) + (a & 3);  // column at lane half 0
+      if constexpr (PASS == 0) {
+        gmax[t][a] = __builtin_amdgcn_fmed3f(gmax[t][a], acc[t][a], INFINITY);
+      } else {
+        if (acc[t][a] >= Tp) {
+          dst[cnt < cout ? cnt : cout - 1] =
+              ((uint64_t)ordered_u32(scale * acc[t][a]) << 32) | (colbase + jl);
+          ++cnt;
         }
       }
     }
@@ -347,7 +346,7 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
     float T[NU];
     int cnt[NU];
     uint64_t* dst[NU];
-    float gmin[NU][4][16];
+    float gmax[NU][4][16];
     const int list = W.split * 2 + half;
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
@@ -356,16 +355,19 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
       myrow[u] = myvalid[u] ? (qlist ? (int64_t)qlist[W.qbeg + slot] - D.row_off
                                      : (int64_t)(W.qbeg + slot))
                             : 0;
-      T[u] = __builtin_nanf("");  // no query: compares false
+      T[u] = __builtin_nanf("");  // no query: compares false (T[u] holds T / s)
       cnt[u] = 0;
       dst[u] = D.cand + ((size_t)myrow[u] * D.nlist + list) * D.cout;
       if constexpr (PASS == 0) {
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
-          for (int a = 0; a < 16; ++a) gmin[u][t][a] = INFINITY;
+          for (int a = 0; a < 16; ++a) gmax[u][t][a] = -INFINITY;
       } else {
-        if (myvalid[u]) T[u] = D.thr[(size_t)myrow[u] * D.nsplit + W.split];
+        // T = +inf (a split with fewer than C columns) must not admit the
+        // columns outside the split, whose key' is -inf
+        if (myvalid[u])
+          T[u] = fmaxf(D.thr[(size_t)myrow[u] * D.nsplit + W.split] / scale, -0x1.fffffep127f);
       }
     }
 
@@ -381,58 +383,66 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
     }
 
     int g = 0;  // stage being computed
+    // advance the ring to stage g: reuse the slot of stage g-1 for stage
+    // g+NSTAGE-1, wait for stage g, return its slot
+    auto next_stage = [&]() -> const uint8_t* {
+      HM_LDS_BARRIER();  // every wave is done with stage g-1 (its slot is reused now)
+      const int gi = g + NSTAGE - 1;
+      if (gi < total) {
+        sweep_issue<MODE>(smem + (gi % NSTAGE) * STAGE, P, bi * blkbytes, ci * 128,
+                          (int64_t)bi * 128, wave);
+        if (++ci == nchunk) { ci = 0; ++bi; }
+      }
+      sweep_wait<MODE>(total - 1 - g < NSTAGE - 1 ? total - 1 - g : NSTAGE - 1);
+      HM_LDS_BARRIER();  // stage g is in LDS for every wave
+      return smem + (g % NSTAGE) * STAGE;
+    };
+    const float rs = 1.0f / scale;  // exact: scale is -1 or -2
     for (int blk = 0; blk < nblk; ++blk) {
+      const int64_t cb = cbeg + (int64_t)blk * 128;
+      const int remblk = (int)(cend - cb < 128 ? cend - cb : 128);
+      // first stage of the block: the accumulators start at bias_j / s
+      // (-inf for columns outside the split)
+      const uint8_t* st = next_stage();
       floatx16 acc[NU][4];
-#pragma unroll
-      for (int u = 0; u < NU; ++u)
+      {
+        const float* colb = (const float*)(st + Cfg::BIAS) + 4 * half;
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
-          for (int e = 0; e < 16; ++e) acc[u][t][e] = 0.0f;
-      const uint8_t* st = smem;
-      for (int ch = 0; ch < nchunk; ++ch, ++g) {
-        HM_LDS_BARRIER();  // every wave is done with stage g-1 (its slot is reused now)
-        {
-          const int gi = g + NSTAGE - 1;
-          if (gi < total) {
-            sweep_issue<MODE>(smem + (gi % NSTAGE) * STAGE, P, bi * blkbytes, ci * 128,
-                              (int64_t)bi * 128, wave);
-            if (++ci == nchunk) { ci = 0; ++bi; }
+          for (int a4 = 0; a4 < 4; ++a4) {
+            const int jc = t * 32 + 8 * a4;
+            const floatx4 cbv = *(const floatx4*)(colb + jc);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const float v = jc + 4 * half + e < remblk ? cbv[e] * rs : -INFINITY;
+#pragma unroll
+              for (int u = 0; u < NU; ++u) acc[u][t][a4 * 4 + e] = v;
+            }
           }
-          const int after = total - 1 - g < NSTAGE - 1 ? total - 1 - g : NSTAGE - 1;
-          sweep_wait<MODE>(after);
-        }
-        HM_LDS_BARRIER();  // stage g is in LDS for every wave
-        st = smem + (g % NSTAGE) * STAGE;
-        // pass 0 also holds the group minima: it reads in smaller batches
-        sweep_mma<MODE, PASS == 0 && MODE != 2>(acc, st, st + Cfg::QT + (NU * wave * 32 + q) * 128, q, half,
-                                   swz);
       }
-
-      // ---- epilogue of block blk (its biases ride in its last stage) ----
-      // 32-bit per-lane bases; position offsets are compile-time constants
-      const int64_t cb = cbeg + (int64_t)blk * 128;
-      const float* colb = (const float*)(st + Cfg::BIAS) + 4 * half;
+      // pass 0 also holds the group maxima: it reads in smaller batches
+      sweep_mma<MODE, PASS == 0 && MODE != 2>(acc, st, st + Cfg::QT + (NU * wave * 32 + q) * 128,
+                                              q, half, swz);
+      ++g;
+      for (int ch = 1; ch < nchunk; ++ch, ++g) {
+        st = next_stage();
+        sweep_mma<MODE, PASS == 0 && MODE != 2>(acc, st, st + Cfg::QT + (NU * wave * 32 + q) * 128,
+                                                q, half, swz);
+      }
+      // ---- epilogue of block blk ----
       const uint32_t colbase = (uint32_t)cb + 4 * half;
-      const int rem = (int)(cend - cb < 128 ? cend - cb : 128) - 4 * half;
-      if (cend - cb >= 128) {
 #pragma unroll
-        for (int u = 0; u < NU; ++u)
-          sweep_epilogue<PASS, true>(acc[u], gmin[u], colb, rem, colbase, scale, T[u], D.cout,
-                                     dst[u], cnt[u]);
-      } else {
-#pragma unroll
-        for (int u = 0; u < NU; ++u)
-          sweep_epilogue<PASS, false>(acc[u], gmin[u], colb, rem, colbase, scale, T[u], D.cout,
-                                      dst[u], cnt[u]);
-      }
+      for (int u = 0; u < NU; ++u)
+        sweep_epilogue<PASS>(acc[u], gmax[u], colbase, scale, T[u], D.cout, dst[u], cnt[u]);
     }
 
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
       if constexpr (PASS == 0) {
-        // T = C-th smallest of the row's 128 group minima (both lanes), found
-        // by bisection on the order-preserving u32 image of the key.
+        // T = C-th smallest of the row's 128 group minima (both lanes) =
+        // s * (C-th largest group maximum of key'), found by bisection on
+        // the order-preserving u32 image of -key'.
         const int C = D.C;
         uint32_t lo = 0u, hi = 0xffffffffu;
         for (int itb = 0; itb < 32; ++itb) {
@@ -441,20 +451,21 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
 #pragma unroll
           for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int a = 0; a < 16; ++a) c += ordered_u32(gmin[u][t][a]) <= mid ? 1 : 0;
+            for (int a = 0; a < 16; ++a) c += ordered_u32(-gmax[u][t][a]) <= mid ? 1 : 0;
           c += __shfl_xor(c, 32, 64);
           if (c >= C) hi = mid; else lo = mid + 1;
         }
-        // lo == hi: the smallest image with >= C group minima at or below it
-        // (the image of +inf when the split has fewer than C columns)
+        // lo == hi: the smallest image with >= C values at or below it (the
+        // image of +inf when the split has fewer than C columns); T = -s * it
         if (myvalid[u] && half == 0)
-          D.thr[(size_t)myrow[u] * D.nsplit + W.split] = unordered_f32(lo);
+          D.thr[(size_t)myrow[u] * D.nsplit + W.split] = -scale * unordered_f32(lo);
       } else {
         if (myvalid[u]) {
           // every column not listed has key > T; an overflowing list reports
           // 0 (no threshold) so the row is left to the exact scan
           D.ctau[(size_t)myrow[u] * D.nlist + list] =
-              cnt[u] > D.cout ? 0ull : (((uint64_t)ordered_u32(T[u]) << 32) | 0xffffffffull);
+              cnt[u] > D.cout ? 0ull
+                              : (((uint64_t)ordered_u32(scale * T[u]) << 32) | 0xffffffffull);
           D.ccnt[(size_t)myrow[u] * D.nlist + list] =
               (uint32_t)(cnt[u] < D.cout ? cnt[u] : D.cout);
         }

@@ -424,6 +424,10 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     nzi = (double)st[ST_NZ];
     E = 2.0 * E + (double)st[ST_NZ] * 0x1p-23 + (double)mx[ST_NZ] * 0x1p-23 +
         u * ((double)mx[ST_NZ] + 2.0 * dotmax);
+    // the sweep's accumulators start at -||x_j||^2 / 2: every accumulation
+    // step (at most one per product, three products per element for bf16x3)
+    // also rounds at that magnitude
+    E += (mode == 2 ? 3.0 : 1.0) * 2.0 * (kp + 1) * u * (double)mx[ST_NZ];
     E = E * (1.0 + 1e-3) + 1e-30;
   } else {
     E = E * (1.0 + 1e-3) + 1e-12;
