@@ -383,18 +383,18 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
     }
 
     int g = 0;  // stage being computed
-    // advance the ring to stage g: reuse the slot of stage g-1 for stage
-    // g+NSTAGE-1, wait for stage g, return its slot
+    // advance the ring to stage g: wait for this wave's copies of stage g;
+    // one barrier then publishes stage g and retires stage g-1 in every
+    // wave, whose slot is refilled with stage g+NSTAGE-1
     auto next_stage = [&]() -> const uint8_t* {
-      HM_LDS_BARRIER();  // every wave is done with stage g-1 (its slot is reused now)
+      sweep_wait<MODE>(total - 1 - g < NSTAGE - 2 ? total - 1 - g : NSTAGE - 2);
+      HM_LDS_BARRIER();
       const int gi = g + NSTAGE - 1;
       if (gi < total) {
         sweep_issue<MODE>(smem + (gi % NSTAGE) * STAGE, P, bi * blkbytes, ci * 128,
                           (int64_t)bi * 128, wave);
         if (++ci == nchunk) { ci = 0; ++bi; }
       }
-      sweep_wait<MODE>(total - 1 - g < NSTAGE - 1 ? total - 1 - g : NSTAGE - 1);
-      HM_LDS_BARRIER();  // stage g is in LDS for every wave
       return smem + (g % NSTAGE) * STAGE;
     };
     const float rs = 1.0f / scale;  // exact: scale is -1 or -2
