@@ -55,6 +55,10 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __attribute__((address_space(3))) void lds_void_t;
+// A global (not generic) pointer: its stores are global_store, counted by
+// vmcnt only. A flat store would also count in lgkmcnt, which every LDS
+// barrier waits for.
+typedef __attribute__((address_space(1))) uint64_t gptr_u64;
 
 // Workgroup barrier that orders LDS only. __syncthreads() also waits for
 // every outstanding global access (vmcnt 0), which would drain the copy
@@ -113,7 +117,12 @@ struct CopyPlan {
   const uint8_t* src[SweepCfg<MODE>::NI];
   bool cand[SweepCfg<MODE>::NI];
   const float* bias;
+  int64_t blim;  // the lane's bias column is in the split iff boff < blim
 };
+
+// Bias source of the columns past a split's end: key' = -inf (key +inf), so
+// the accumulators need no per-column check.
+__device__ const float kBiasOut = -INFINITY;
 
 template <int MODE>
 __device__ __forceinline__ void sweep_issue(uint8_t* stage, const CopyPlan<MODE>& P, int64_t boff_bytes,
@@ -126,8 +135,8 @@ __device__ __forceinline__ void sweep_issue(uint8_t* stage, const CopyPlan<MODE>
     __builtin_amdgcn_global_load_lds(src, (lds_void_t*)(stage + rb * 1024), 16, 0, 0);
   }
   __builtin_amdgcn_global_load_lds(
-      P.bias + boff, (lds_void_t*)(stage + Cfg::BIAS + (wave >> 1) * 512 + (wave & 1) * 256), 4, 0,
-      0);
+      boff < P.blim ? P.bias + boff : &kBiasOut,
+      (lds_void_t*)(stage + Cfg::BIAS + (wave >> 1) * 512 + (wave & 1) * 256), 4, 0, 0);
 }
 
 // Wait until this wave's copies of the stage computed next have landed;
@@ -145,9 +154,8 @@ __device__ __forceinline__ void sweep_wait(int after) {
 // 128-byte slice. Operand reads go in batches ahead of the MFMAs that use
 // them (sched_group_barrier keeps that order; the default schedule
 // interleaves reads and MFMAs one by one and exposes the LDS latency each
-// time). LOWREG reads the second batch after the first batch's MFMAs
-// (fewer live operand registers; pass 0 also holds the group minima).
-template <int MODE, bool LOWREG>
+// time); all of a stage's reads are issued before its first MFMA.
+template <int MODE>
 __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4],
                                           const uint8_t* st, const uint8_t* qrow_p, int q,
                                           int half, int swz) {
@@ -177,17 +185,11 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4]
             for (int j = 0; j < 4; ++j)
               acc[u][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[v][t][j], b4[v][u][j], acc[u][t],
                                                                0, 0, 0);
-      if constexpr (LOWREG) {
-        __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);  // this batch's reads
-        __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 16, 0);   // its MFMAs
-      }
     }
-    if constexpr (!LOWREG) {
-      __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);
-      __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);
-      __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 16, 0);
-      __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 16, 0);
-    }
+    __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 16, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 16, 0);
   } else if constexpr (MODE == 0) {
 #pragma unroll
     for (int kh = 0; kh < 2; ++kh) {
@@ -207,17 +209,11 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4]
 #pragma unroll
           for (int t = 0; t < 4; ++t)
             acc[u][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[v][t], bh[v][u], acc[u][t], 0, 0, 0);
-      if constexpr (LOWREG) {
-        __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);  // this batch's reads
-        __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 4, 0);    // its MFMAs
-      }
     }
-    if constexpr (!LOWREG) {
-      __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);  // reads of steps 0-1
-      __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);  // reads of steps 2-3
-      __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 4, 0);    // MFMAs of steps 0-1
-      __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 4, 0);    // MFMAs of steps 2-3
-    }
+    __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);  // reads of steps 0-1
+    __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);  // reads of steps 2-3
+    __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 4, 0);    // MFMAs of steps 0-1
+    __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 4, 0);    // MFMAs of steps 2-3
   } else {
     static_assert(NU == 1, "bf16x3 waves hold 32 queries");
     // two halves of two k-steps: hi and lo parts of both operands
@@ -249,28 +245,48 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4]
   }
 }
 
+// Accumulator start of a block: bias_j / s for the block's columns, as
+// copied with the stage (prep stores the bias already divided by s; columns
+// past the split's end were copied from kBiasOut).
+__device__ __forceinline__ void sweep_acc_start(floatx16 (&acc)[4], const uint8_t* bias, int half) {
+  const float* colb = (const float*)bias + 4 * half;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int a4 = 0; a4 < 4; ++a4) {
+      const floatx4 v = *(const floatx4*)(colb + t * 32 + 8 * a4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[t][a4 * 4 + e] = v[e];
+    }
+}
+
 // Epilogue of one 128-column block for one 32-query tile. acc holds key'
 // (see the top of the file). Pass 0: gmax = max(gmax, key') per position
 // (v_med3 with +inf: a max without NaN canonicalisation). Pass 1: a hit is
 // key' >= Tp (Tp = T / s; NaN for lanes without a query: never); hits are
-// appended to dst[min(cnt, cout-1)] and all counted -- cnt > cout afterwards
-// means the list overflowed (its contents are then unused). A position
-// costs a compare and a skipped branch when no lane of the wave hits.
+// appended as (key' bits, column) to dst[min(cnt, cout-1)] and all counted
+// -- cnt > cout afterwards means the list overflowed (its contents are then
+// unused). A position costs a compare and a skipped branch when no lane of
+// the wave hits.
 template <int PASS>
 __device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[4], float (&gmax)[4][16],
                                                uint32_t colbase, float scale, float Tp, int cout,
-                                               uint64_t* dst, int& cnt) {
+                                               gptr_u64* dst, int& cnt) {
+  // +inf in a register: with a literal the compiler turns the med3 into a
+  // NaN-canonicalising max (three instructions)
+  float pinf = INFINITY;
+  if constexpr (PASS == 0) asm volatile("" : "+v"(pinf));
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
 #pragma unroll
     for (int a = 0; a < 16; ++a) {
       const int jl = t * 32 + 8 * (a >> 2) + (a & 3);  // column at lane half 0
       if constexpr (PASS == 0) {
-        gmax[t][a] = __builtin_amdgcn_fmed3f(gmax[t][a], acc[t][a], INFINITY);
+        gmax[t][a] = __builtin_amdgcn_fmed3f(gmax[t][a], acc[t][a], pinf);
       } else {
         if (acc[t][a] >= Tp) {
           dst[cnt < cout ? cnt : cout - 1] =
-              ((uint64_t)ordered_u32(scale * acc[t][a]) << 32) | (colbase + jl);
+              ((uint64_t)__float_as_uint(acc[t][a]) << 32) | (colbase + jl);
           ++cnt;
         }
       }
@@ -338,6 +354,7 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
       P.cand[i] = cand;
     }
     P.bias = D.colbias + cbeg + (wave & 1) * 64 + lane;
+    P.blim = cend - cbeg - ((wave & 1) * 64 + lane);
 
     // this lane's own queries: slot 32*(NU*wave + u) + q of the item, u < NU
     constexpr int NU = Cfg::NU;
@@ -345,7 +362,7 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
     int64_t myrow[NU];
     float T[NU];
     int cnt[NU];
-    uint64_t* dst[NU];
+    gptr_u64* dst[NU];
     float gmax[NU][4][16];
     const int list = W.split * 2 + half;
 #pragma unroll
@@ -357,7 +374,7 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
                             : 0;
       T[u] = __builtin_nanf("");  // no query: compares false (T[u] holds T / s)
       cnt[u] = 0;
-      dst[u] = D.cand + ((size_t)myrow[u] * D.nlist + list) * D.cout;
+      dst[u] = (gptr_u64*)(D.cand + ((size_t)myrow[u] * D.nlist + list) * D.cout);
       if constexpr (PASS == 0) {
 #pragma unroll
         for (int t = 0; t < 4; ++t)
@@ -397,37 +414,26 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
       }
       return smem + (g % NSTAGE) * STAGE;
     };
-    const float rs = 1.0f / scale;  // exact: scale is -1 or -2
     for (int blk = 0; blk < nblk; ++blk) {
       const int64_t cb = cbeg + (int64_t)blk * 128;
-      const int remblk = (int)(cend - cb < 128 ? cend - cb : 128);
       // first stage of the block: the accumulators start at bias_j / s
       // (-inf for columns outside the split)
       const uint8_t* st = next_stage();
       floatx16 acc[NU][4];
       {
-        const float* colb = (const float*)(st + Cfg::BIAS) + 4 * half;
+        floatx16 a0[4];
+        sweep_acc_start(a0, st + Cfg::BIAS, half);
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int u = 0; u < NU; ++u)
 #pragma unroll
-          for (int a4 = 0; a4 < 4; ++a4) {
-            const int jc = t * 32 + 8 * a4;
-            const floatx4 cbv = *(const floatx4*)(colb + jc);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const float v = jc + 4 * half + e < remblk ? cbv[e] * rs : -INFINITY;
-#pragma unroll
-              for (int u = 0; u < NU; ++u) acc[u][t][a4 * 4 + e] = v;
-            }
-          }
+          for (int t = 0; t < 4; ++t) acc[u][t] = a0[t];
       }
-      // pass 0 also holds the group maxima: it reads in smaller batches
-      sweep_mma<MODE, PASS == 0 && MODE != 2>(acc, st, st + Cfg::QT + (NU * wave * 32 + q) * 128,
+      sweep_mma<MODE>(acc, st, st + Cfg::QT + (NU * wave * 32 + q) * 128,
                                               q, half, swz);
       ++g;
       for (int ch = 1; ch < nchunk; ++ch, ++g) {
         st = next_stage();
-        sweep_mma<MODE, PASS == 0 && MODE != 2>(acc, st, st + Cfg::QT + (NU * wave * 32 + q) * 128,
+        sweep_mma<MODE>(acc, st, st + Cfg::QT + (NU * wave * 32 + q) * 128,
                                                 q, half, swz);
       }
       // ---- epilogue of block blk ----

@@ -21,7 +21,7 @@ struct DsDesc {
                          //   (euclidean: == xr; cosine: rows / ||row||)
   const uint16_t* op16;  // same operand in bf16 (RNE), n_pad x kp16, or null
   const uint16_t* op16lo;  // bf16 of (operand - op16), n_pad x kp16, or null
-  const float* colbias;  // n_pad: euclid ||x_j||^2 (f32); cosine 0 / -inf
+  const float* colbias;  // n_pad: key bias / s: euclid -||x_j||^2/2 (f32); cosine 0 / +inf
   const double* rnorm;   // n: cosine sqrt(sum x^2) (f64, reference fold)
   const float* rstat;    // n x HM_NSTAT per-row bound statistics
   const float* smax;     // HM_NSTAT maxima over rows (per dataset)

@@ -362,12 +362,16 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   for (int l = 0; l < L; ++l) nvalid += (int)cnt[l];  // each <= CO
   int len = 64;
   while (len < nvalid) len <<= 1;
+  // the sweep stores (key' bits, column) with key' = key / s; sort keys are
+  // (order-preserving image of key = s * key', column)
+  const float sc = metric == 1 ? -1.0f : -2.0f;
   for (int e = lane; e < len; e += 64) {
     uint64_t v = ~0ull;
     if (e < nvalid) {
       int l = 0, off = 0;
       while (e - off >= (int)cnt[l]) off += (int)cnt[l++];
       v = src[(size_t)l * CO + (e - off)];
+      v = ((uint64_t)ordered_u32(sc * __uint_as_float((uint32_t)(v >> 32))) << 32) | (uint32_t)v;
     }
     a[e] = v;
   }

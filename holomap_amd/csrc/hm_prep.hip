@@ -141,12 +141,14 @@ __global__ __launch_bounds__(256) void prep_kernel(
   }
   if (lane == 0) {
     float* cb = const_cast<float*>(D.colbias);
+    // the sweep's accumulator start bias_j / s (s = -2 euclidean, -1 cosine;
+    // exact): key bias ||x_j||^2, or -inf for a zero row under cosine
     if (!real) {
       cb[i] = 0.0f;
     } else if (cosine) {
-      cb[i] = zero_row ? -INFINITY : 0.0f;
+      cb[i] = zero_row ? INFINITY : 0.0f;
     } else {
-      cb[i] = nz;
+      cb[i] = -0.5f * nz;
     }
     if (real) {
       // norms rounded up by a relative 2^-20 so the stored f32 bounds the
