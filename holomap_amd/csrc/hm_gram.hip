@@ -14,7 +14,7 @@
 //
 // Operand staging: a "stage" is one 128-byte slice of every operand row of a
 // block (64 bf16 or 32 f32 features) for the 128 query rows and the 128
-// candidate rows, plus the block's 128 column biases. Stages are copied
+// candidate rows, plus the next block's 128 column biases. Stages are copied
 // global -> LDS by the LDS-DMA path (global_load_lds, 16 bytes per lane, no
 // registers held while in flight) into a ring of NSTAGE slots, NSTAGE-1
 // stages ahead of the MFMAs; a wave waits for its own copies with
@@ -98,8 +98,9 @@ struct SweepCfg {
   static constexpr int STAGE = BIAS + NW / 2 * 512;
   static constexpr int NSTAGE = MODE == 2 ? 2 : 3;
   static constexpr int VM = NI + 1;              // vmcnt entries per stage and wave
+  static constexpr int BIAS0 = NSTAGE * STAGE;   // the item's first block's biases
   static_assert(RG % NW == 0, "copy groups split evenly over the waves");
-  static_assert(NSTAGE * STAGE <= 160 * 1024, "LDS");
+  static_assert(NSTAGE * STAGE + 512 <= 160 * 1024, "LDS");
   static_assert(NW * 64 == sweep_threads(MODE), "the runtime launches sweep_threads(MODE)");
 };
 
@@ -157,7 +158,7 @@ __device__ __forceinline__ void sweep_wait(int after) {
 // time); all of a stage's reads are issued before its first MFMA.
 template <int MODE>
 __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4],
-                                          const uint8_t* st, const uint8_t* qrow_p, int q,
+                                          const uint8_t* cand, const uint8_t* qrow_p, int q,
                                           int half, int swz) {
   using Cfg = SweepCfg<MODE>;
   constexpr int NU = Cfg::NU;
@@ -173,7 +174,7 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4]
 #pragma unroll
         for (int u = 0; u < NU; ++u) b4[v][u] = *(const floatx4*)(qrow_p + u * 32 * 128 + cs);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) a4[v][t] = *(const floatx4*)(st + Cfg::CT + (t * 32 + q) * 128 + cs);
+        for (int t = 0; t < 4; ++t) a4[v][t] = *(const floatx4*)(cand + (t * 32 + q) * 128 + cs);
       }
 #pragma unroll
       for (int v = 0; v < 2; ++v)
@@ -200,7 +201,7 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4]
 #pragma unroll
         for (int u = 0; u < NU; ++u) bh[v][u] = *(const bf16x8_t*)(qrow_p + u * 32 * 128 + cs);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) ah[v][t] = *(const bf16x8_t*)(st + Cfg::CT + (t * 32 + q) * 128 + cs);
+        for (int t = 0; t < 4; ++t) ah[v][t] = *(const bf16x8_t*)(cand + (t * 32 + q) * 128 + cs);
       }
 #pragma unroll
       for (int v = 0; v < 2; ++v)
@@ -227,8 +228,8 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4]
         bl[v] = *(const bf16x8_t*)(qrow_p + Cfg::QL + cs);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          ah[v][t] = *(const bf16x8_t*)(st + Cfg::CT + (t * 32 + q) * 128 + cs);
-          al[v][t] = *(const bf16x8_t*)(st + Cfg::CL + (t * 32 + q) * 128 + cs);
+          ah[v][t] = *(const bf16x8_t*)(cand + (t * 32 + q) * 128 + cs);
+          al[v][t] = *(const bf16x8_t*)(cand + (Cfg::CL - Cfg::CT) + (t * 32 + q) * 128 + cs);
         }
       }
 #pragma unroll
@@ -294,13 +295,49 @@ __device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[4], float (
   }
 }
 
+// End of an item for one lane. Pass 0: T = C-th smallest of the row's 128
+// group minima (both lanes) = s * (C-th largest group maximum of key'),
+// found by bisection on the order-preserving u32 image of -key'. Pass 1:
+// the list's threshold and count.
+template <int PASS>
+__device__ __forceinline__ void sweep_finish(const float (&gmax)[4][16], float T, int cnt,
+                                             bool myvalid, int64_t myrow, const DsDesc& D,
+                                             int split, int half, float scale) {
+  if constexpr (PASS == 0) {
+    const int C = D.C;
+    uint32_t lo = 0u, hi = 0xffffffffu;
+    for (int itb = 0; itb < 32; ++itb) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      int c = 0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int a = 0; a < 16; ++a) c += ordered_u32(-gmax[t][a]) <= mid ? 1 : 0;
+      c += __shfl_xor(c, 32, 64);
+      if (c >= C) hi = mid; else lo = mid + 1;
+    }
+    // lo == hi: the smallest image with >= C values at or below it (the
+    // image of +inf when the split has fewer than C columns); T = -s * it
+    if (myvalid && half == 0) D.thr[(size_t)myrow * D.nsplit + split] = -scale * unordered_f32(lo);
+  } else {
+    if (myvalid) {
+      // every column not listed has key > T; an overflowing list reports 0
+      // (no threshold) so the row is left to the exact scan
+      const int list = split * 2 + half;
+      D.ctau[(size_t)myrow * D.nlist + list] =
+          cnt > D.cout ? 0ull : (((uint64_t)ordered_u32(scale * T) << 32) | 0xffffffffull);
+      D.ccnt[(size_t)myrow * D.nlist + list] = (uint32_t)(cnt < D.cout ? cnt : D.cout);
+    }
+  }
+}
+
 template <int MODE, int PASS>
 __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
     const DsDesc* __restrict__ ds_arr, const WorkItem* __restrict__ items,
     const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist) {
   using Cfg = SweepCfg<MODE>;
   constexpr int NI = Cfg::NI, NSTAGE = Cfg::NSTAGE, STAGE = Cfg::STAGE, QG = Cfg::QG;
-  __shared__ __attribute__((aligned(1024))) uint8_t smem[NSTAGE * STAGE];
+  __shared__ __attribute__((aligned(1024))) uint8_t smem[NSTAGE * STAGE + 512];
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: LDS bases stay scalar
@@ -389,96 +426,238 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
     }
 
     HM_LDS_BARRIER();  // the previous item's last stage is no longer read
+    // the first block's biases (waves 0 and 1, 64 each), then stages
+    // 0 .. NSTAGE-2; every stage carries the biases of the block after its own
+    if (wave < 2)
+      __builtin_amdgcn_global_load_lds(0 < P.blim ? P.bias : &kBiasOut,
+                                       (lds_void_t*)(smem + Cfg::BIAS0 + wave * 256), 4, 0, 0);
     // (bi, ci): block and slice of the next stage to issue
     int bi = 0, ci = 0;
 #pragma unroll
-    for (int g = 0; g < NSTAGE - 1; ++g) {  // prologue: stages 0 .. NSTAGE-2
+    for (int g = 0; g < NSTAGE - 1; ++g) {
       if (g < total) {
-        sweep_issue<MODE>(smem + g * STAGE, P, bi * blkbytes, ci * 128, (int64_t)bi * 128, wave);
+        sweep_issue<MODE>(smem + g * STAGE, P, bi * blkbytes, ci * 128, (int64_t)(bi + 1) * 128,
+                          wave);
         if (++ci == nchunk) { ci = 0; ++bi; }
       }
     }
+    {
+      const int issued = total < NSTAGE - 1 ? total : NSTAGE - 1;
+      if (issued >= 2) vm_wait<2 * Cfg::VM>();
+      else if (issued == 1) vm_wait<Cfg::VM>();
+      else vm_wait<0>();
+    }
+    HM_LDS_BARRIER();  // the first block's biases are in LDS
+    floatx16 acc[NU][4];
+    {
+      floatx16 a0[4];
+      sweep_acc_start(a0, smem + Cfg::BIAS0, half);
+#pragma unroll
+      for (int u = 0; u < NU; ++u)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[u][t] = a0[t];
+    }
 
-    int g = 0;  // stage being computed
-    // advance the ring to stage g: wait for this wave's copies of stage g;
-    // one barrier then publishes stage g and retires stage g-1 in every
-    // wave, whose slot is refilled with stage g+NSTAGE-1
-    auto next_stage = [&]() -> const uint8_t* {
+    // stage g: wait for this wave's copies of stage g; one barrier then
+    // publishes stage g and retires stage g-1 in every wave, whose slot is
+    // refilled with stage g+NSTAGE-1
+    int ch = 0;                // slice of stage g within its block
+    uint32_t colbase = (uint32_t)cbeg + 4 * half;  // this lane's columns of the block
+    for (int g = 0; g < total; ++g) {
       sweep_wait<MODE>(total - 1 - g < NSTAGE - 2 ? total - 1 - g : NSTAGE - 2);
       HM_LDS_BARRIER();
       const int gi = g + NSTAGE - 1;
       if (gi < total) {
         sweep_issue<MODE>(smem + (gi % NSTAGE) * STAGE, P, bi * blkbytes, ci * 128,
-                          (int64_t)bi * 128, wave);
+                          (int64_t)(bi + 1) * 128, wave);
         if (++ci == nchunk) { ci = 0; ++bi; }
       }
-      return smem + (g % NSTAGE) * STAGE;
-    };
-    for (int blk = 0; blk < nblk; ++blk) {
-      const int64_t cb = cbeg + (int64_t)blk * 128;
-      // first stage of the block: the accumulators start at bias_j / s
-      // (-inf for columns outside the split)
-      const uint8_t* st = next_stage();
-      floatx16 acc[NU][4];
-      {
+      const uint8_t* st = smem + (g % NSTAGE) * STAGE;
+      sweep_mma<MODE>(acc, st + Cfg::CT, st + Cfg::QT + (NU * wave * 32 + q) * 128, q, half, swz);
+      if (++ch == nchunk) {
+        // ---- epilogue of the block; the next block starts at its biases ----
+#pragma unroll
+        for (int u = 0; u < NU; ++u)
+          sweep_epilogue<PASS>(acc[u], gmax[u], colbase, scale, T[u], D.cout, dst[u], cnt[u]);
         floatx16 a0[4];
         sweep_acc_start(a0, st + Cfg::BIAS, half);
 #pragma unroll
         for (int u = 0; u < NU; ++u)
 #pragma unroll
           for (int t = 0; t < 4; ++t) acc[u][t] = a0[t];
+        ch = 0;
+        colbase += 128;
       }
-      sweep_mma<MODE>(acc, st, st + Cfg::QT + (NU * wave * 32 + q) * 128,
-                                              q, half, swz);
-      ++g;
-      for (int ch = 1; ch < nchunk; ++ch, ++g) {
-        st = next_stage();
-        sweep_mma<MODE>(acc, st, st + Cfg::QT + (NU * wave * 32 + q) * 128,
-                                                q, half, swz);
-      }
-      // ---- epilogue of block blk ----
-      const uint32_t colbase = (uint32_t)cb + 4 * half;
-#pragma unroll
-      for (int u = 0; u < NU; ++u)
-        sweep_epilogue<PASS>(acc[u], gmax[u], colbase, scale, T[u], D.cout, dst[u], cnt[u]);
     }
 
 #pragma unroll
-    for (int u = 0; u < NU; ++u) {
-      if constexpr (PASS == 0) {
-        // T = C-th smallest of the row's 128 group minima (both lanes) =
-        // s * (C-th largest group maximum of key'), found by bisection on
-        // the order-preserving u32 image of -key'.
-        const int C = D.C;
-        uint32_t lo = 0u, hi = 0xffffffffu;
-        for (int itb = 0; itb < 32; ++itb) {
-          const uint32_t mid = lo + (hi - lo) / 2;
-          int c = 0;
-#pragma unroll
-          for (int t = 0; t < 4; ++t)
-#pragma unroll
-            for (int a = 0; a < 16; ++a) c += ordered_u32(-gmax[u][t][a]) <= mid ? 1 : 0;
-          c += __shfl_xor(c, 32, 64);
-          if (c >= C) hi = mid; else lo = mid + 1;
-        }
-        // lo == hi: the smallest image with >= C values at or below it (the
-        // image of +inf when the split has fewer than C columns); T = -s * it
-        if (myvalid[u] && half == 0)
-          D.thr[(size_t)myrow[u] * D.nsplit + W.split] = -scale * unordered_f32(lo);
-      } else {
-        if (myvalid[u]) {
-          // every column not listed has key > T; an overflowing list reports
-          // 0 (no threshold) so the row is left to the exact scan
-          D.ctau[(size_t)myrow[u] * D.nlist + list] =
-              cnt[u] > D.cout ? 0ull
-                              : (((uint64_t)ordered_u32(scale * T[u]) << 32) | 0xffffffffull);
-          D.ccnt[(size_t)myrow[u] * D.nlist + list] =
-              (uint32_t)(cnt[u] < D.cout ? cnt[u] : D.cout);
-        }
-      }
-    }
+    for (int u = 0; u < NU; ++u)
+      sweep_finish<PASS>(gmax[u], T[u], cnt[u], myvalid[u], myrow[u], D, W.split, half, scale);
   }
 }
+
+// ---------------------------------------------------------------------------
+// bf16 sweep with the item's query rows resident in LDS, for rows of at most
+// QresCfg::QMAXB bytes (192 bf16 features). The 256 query rows are copied
+// once per item; the ring stages then carry only the 128 candidate rows and
+// the biases (18 KB instead of 48 KB), a third of the copy traffic per MFMA.
+// Same tiles, MFMA sequence, epilogue and lists as gram_sweep_kernel<0, *>.
+// ---------------------------------------------------------------------------
+struct QresCfg {
+  static constexpr int QW = 256, NW = 8;
+  static constexpr int BIAS = 128 * 128;             // after the candidate rows
+  static constexpr int STAGE = BIAS + NW / 2 * 512;  // + NW/2 copies of the biases
+  static constexpr int NSTAGE = 3;
+  static constexpr int NI = 16 / NW;                 // candidate copy groups per wave
+  static constexpr int VM = NI + 1;                  // vmcnt entries per stage and wave
+  static constexpr int BIAS0 = NSTAGE * STAGE;       // the item's first block's biases
+  static constexpr int QOFF = BIAS0 + 512;           // slice c of query row r: QOFF + c*QW*128 + r*128
+  static constexpr int QMAXB = kQresRowBytes;        // row bytes held (three slices)
+  static constexpr int LDS = QOFF + QMAXB / 128 * QW * 128;
+  static_assert(LDS <= 160 * 1024, "LDS");
+  static_assert(QOFF % 128 == 0, "query rows are 128-byte aligned");
+  static_assert(NW * 64 == sweep_threads(0), "the runtime launches sweep_threads(0)");
+};
+
+template <int PASS>
+__global__ __launch_bounds__(QresCfg::NW * 64, 1) void gram_sweep_qres_kernel(
+    const DsDesc* __restrict__ ds_arr, const WorkItem* __restrict__ items,
+    const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist) {
+  using Cfg = QresCfg;
+  constexpr int NI = Cfg::NI, NSTAGE = Cfg::NSTAGE, STAGE = Cfg::STAGE, QW = Cfg::QW;
+  __shared__ __attribute__((aligned(1024))) uint8_t smem[Cfg::LDS];
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int q = lane & 31, half = lane >> 5;
+  const int nitems = nitems_dev ? *nitems_dev : nitems_host;
+  const int cb_even = ((lane & 7) ^ (lane >> 3)) * 16;  // copy side, even 8-row group
+  const int swz = row_swz(q);                           // read side
+
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem W = items[it];
+    const DsDesc& D = ds_arr[W.ds];
+    const int64_t n = D.n;
+    const int rowbytes = D.kp16 * 2;  // <= QMAXB (the runtime routes longer rows elsewhere)
+    const float scale = D.metric == 1 ? -1.0f : -2.0f;
+    const int64_t span = ((n + (int64_t)D.nsplit * 128 - 1) / ((int64_t)D.nsplit * 128)) * 128;
+    const int64_t cbeg = (int64_t)W.split * span;
+    const int64_t cend = cbeg + span < n ? cbeg + span : n;
+    const int nchunk = rowbytes / 128;
+    const int nblk = cend > cbeg ? (int)((cend - cbeg + 127) / 128) : 0;
+    const int total = nblk * nchunk;
+    const int64_t blkbytes = (int64_t)128 * rowbytes;
+    const uint8_t* op = (const uint8_t*)D.op16;
+
+    // candidate copies: wave w copies groups w*NI + i (8 rows each) of a block
+    const uint8_t* csrc[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int rb = wave * NI + i;  // uniform
+      csrc[i] = op + (cbeg + rb * 8 + (lane >> 3)) * rowbytes + ((rb & 1) ? (cb_even ^ 16) : cb_even);
+    }
+    const float* bsrc = D.colbias + cbeg + (wave & 1) * 64 + lane;
+    const int64_t blim = cend - cbeg - ((wave & 1) * 64 + lane);
+
+    const int slot = 32 * wave + q;
+    const bool myvalid = slot < W.qcnt;
+    const int64_t myrow =
+        myvalid ? (qlist ? (int64_t)qlist[W.qbeg + slot] - D.row_off : (int64_t)(W.qbeg + slot))
+                : 0;
+    const int list = W.split * 2 + half;
+    gptr_u64* dst = (gptr_u64*)(D.cand + ((size_t)myrow * D.nlist + list) * D.cout);
+    float T = __builtin_nanf("");  // no query: compares false (T holds T / s)
+    int cnt = 0;
+    float gmax[4][16];
+    if constexpr (PASS == 0) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int a = 0; a < 16; ++a) gmax[t][a] = -INFINITY;
+    } else {
+      // T = +inf (a split with fewer than C columns) must not admit the
+      // columns outside the split, whose key' is -inf
+      if (myvalid) T = fmaxf(D.thr[(size_t)myrow * D.nsplit + W.split] / scale, -0x1.fffffep127f);
+    }
+
+    HM_LDS_BARRIER();  // the previous item's stages and query rows are no longer read
+    // the item's query rows, slice by slice: wave w copies groups 4w + i
+    // (missing queries read row 0; their results are unused)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int qg = wave * 4 + i;  // uniform
+      const int r = qg * 8 + (lane >> 3);
+      int64_t row = 0;
+      if (r < W.qcnt) row = qlist ? (int64_t)qlist[W.qbeg + r] - D.row_off : (int64_t)(W.qbeg + r);
+      const uint8_t* src = op + row * rowbytes + ((qg & 1) ? (cb_even ^ 16) : cb_even);
+      for (int c = 0; c < nchunk; ++c)
+        __builtin_amdgcn_global_load_lds(src + c * 128,
+                                         (lds_void_t*)(smem + Cfg::QOFF + c * QW * 128 + qg * 1024),
+                                         16, 0, 0);
+    }
+    if (wave < 2)
+      __builtin_amdgcn_global_load_lds(0 < blim ? bsrc : &kBiasOut,
+                                       (lds_void_t*)(smem + Cfg::BIAS0 + wave * 256), 4, 0, 0);
+    // stage copies: the block's candidate slice and the next block's biases
+    auto issue = [&](int slotg, int bi, int ci) {
+      uint8_t* stage = smem + slotg * STAGE;
+#pragma unroll
+      for (int i = 0; i < NI; ++i)
+        __builtin_amdgcn_global_load_lds(csrc[i] + bi * blkbytes + ci * 128,
+                                         (lds_void_t*)(stage + (wave * NI + i) * 1024), 16, 0, 0);
+      const int64_t boff = (int64_t)(bi + 1) * 128;
+      __builtin_amdgcn_global_load_lds(
+          boff < blim ? bsrc + boff : &kBiasOut,
+          (lds_void_t*)(stage + Cfg::BIAS + (wave >> 1) * 512 + (wave & 1) * 256), 4, 0, 0);
+    };
+    int bi = 0, ci = 0;  // block and slice of the next stage to issue
+#pragma unroll
+    for (int g = 0; g < NSTAGE - 1; ++g) {
+      if (g < total) {
+        issue(g, bi, ci);
+        if (++ci == nchunk) { ci = 0; ++bi; }
+      }
+    }
+    {
+      const int issued = total < NSTAGE - 1 ? total : NSTAGE - 1;
+      if (issued >= 2) vm_wait<2 * Cfg::VM>();
+      else if (issued == 1) vm_wait<Cfg::VM>();
+      else vm_wait<0>();
+    }
+    HM_LDS_BARRIER();  // query rows and the first block's biases are in LDS
+    floatx16 acc[1][4];
+    sweep_acc_start(acc[0], smem + Cfg::BIAS0, half);
+
+    int ch = 0;
+    uint32_t colbase = (uint32_t)cbeg + 4 * half;
+    const uint8_t* qrow = smem + Cfg::QOFF + slot * 128;
+    for (int g = 0; g < total; ++g) {
+      // this wave's copies of stage g have landed; one barrier publishes
+      // stage g and retires stage g-1, whose slot gets stage g+NSTAGE-1
+      if (total - 1 - g >= 1) vm_wait<Cfg::VM>();
+      else vm_wait<0>();
+      HM_LDS_BARRIER();
+      if (g + NSTAGE - 1 < total) {
+        issue((g + NSTAGE - 1) % NSTAGE, bi, ci);
+        if (++ci == nchunk) { ci = 0; ++bi; }
+      }
+      const uint8_t* st = smem + (g % NSTAGE) * STAGE;
+      sweep_mma<0>(acc, st, qrow + ch * (QW * 128), q, half, swz);
+      if (++ch == nchunk) {
+        sweep_epilogue<PASS>(acc[0], gmax, colbase, scale, T, D.cout, dst, cnt);
+        sweep_acc_start(acc[0], st + Cfg::BIAS, half);
+        ch = 0;
+        colbase += 128;
+      }
+    }
+    sweep_finish<PASS>(gmax, T, cnt, myvalid, myrow, D, W.split, half, scale);
+  }
+}
+
+template __global__ void gram_sweep_qres_kernel<0>(const DsDesc*, const WorkItem*, const int*, int,
+                                                   const uint32_t*);
+template __global__ void gram_sweep_qres_kernel<1>(const DsDesc*, const WorkItem*, const int*, int,
+                                                   const uint32_t*);
 
 #define HM_INST(MODE, PASS)                                                       \
   template __global__ void gram_sweep_kernel<MODE, PASS>(const DsDesc*, const WorkItem*, \

@@ -52,6 +52,9 @@ struct DsDesc {
 __host__ __device__ constexpr int sweep_rows(int mode) { return mode == 2 ? 128 : 256; }
 // Threads per sweep workgroup: one wave per 32 query rows.
 __host__ __device__ constexpr int sweep_threads(int mode) { return sweep_rows(mode) / 32 * 64; }
+// bf16 operand rows (kp16 * 2 bytes) up to this length are swept with the
+// item's query rows resident in LDS (gram_sweep_qres_kernel).
+constexpr int kQresRowBytes = 384;
 
 // Number of float statistics kept per row for the candidate error bound.
 #define HM_NSTAT 8

@@ -22,6 +22,9 @@ __global__ void prep_kernel(const DsDesc*, int, int64_t, const int64_t*, const f
 __global__ void statmax_kernel(const DsDesc*);
 template <int MODE, int PASS>
 __global__ void gram_sweep_kernel(const DsDesc*, const WorkItem*, const int*, int, const uint32_t*);
+template <int PASS>
+__global__ void gram_sweep_qres_kernel(const DsDesc*, const WorkItem*, const int*, int,
+                                       const uint32_t*);
 __global__ void exact_scan_kernel(const DsDesc*, const WorkItem*, const int*, int,
                                   const uint32_t*, const int*, int, int, uint64_t*);
 __global__ void exact_group_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t,
@@ -115,6 +118,7 @@ struct hm_job {
   std::vector<DsDesc> hdesc;
   int64_t total_rows = 0, total_pad = 0, total_edges = 0;
   int kmax = 0, mode = 0, xs = 1, fb_cap = 0, n_items_l1 = 0;
+  int n_items_qres = 0;  // the first items of items_l1 go to the query-resident sweep
   int esc_mode = -1;  // sweep mode of the escalation pass (-1: none)
   bool xcd_perm = true;  // permute sweep items for XCD-local L2 sharing
   bool tuned = false;    // autotuning done (hm_options.autotune)
@@ -392,6 +396,26 @@ hm_status launch_sweep(hm_job* J, int mode, const WorkItem* items, const int* ni
   return HM_OK;
 }
 
+// Both passes of the primary sweep: the items of datasets with short bf16
+// rows (the first n_items_qres) keep their query rows in LDS, the rest
+// stream them with the candidates. Grids match the item permutation.
+hm_status launch_sweep_l1(hm_job* J) {
+  hm_ctx* ctx = J->ctx;
+  const int nq = J->n_items_qres, no = J->n_items_l1 - nq;
+  if (nq > 0) {
+    const unsigned grid = (unsigned)sweep_grid(ctx, 0, nq);
+    hipLaunchKernelGGL((gram_sweep_qres_kernel<0>), dim3(grid), dim3(sweep_threads(0)), 0,
+                       J->stream, J->d_desc, J->items_l1, nullptr, nq, nullptr);
+    hipLaunchKernelGGL((gram_sweep_qres_kernel<1>), dim3(grid), dim3(sweep_threads(0)), 0,
+                       J->stream, J->d_desc, J->items_l1, nullptr, nq, nullptr);
+    HM_LAUNCH(ctx);
+  }
+  if (no > 0)
+    HM_TRY(launch_sweep(J, J->mode, J->items_l1 + nq, nullptr, no, nullptr,
+                        (unsigned)sweep_grid(ctx, J->mode, no)));
+  return HM_OK;
+}
+
 void mark(hm_job* J, int i) {
   if (J->timed) (void)hipEventRecord(J->ev[i], J->stream);
 }
@@ -416,8 +440,7 @@ hm_status enqueue_pipeline(hm_job* J) {
     // ---- candidate sweep + re-rank ----
     if (J->mode >= 0) {
       // grid and item order go together (see the item permutation in hm_job_create)
-      HM_TRY(launch_sweep(J, J->mode, J->items_l1, nullptr, J->n_items_l1, nullptr,
-                          (unsigned)sweep_grid(ctx, J->mode, J->n_items_l1)));
+      HM_TRY(launch_sweep_l1(J));
       mark(J, 2);
       hipLaunchKernelGGL(rerank_kernel, dim3(grid_for(R, 4)), dim3(256), 0, J->stream, J->d_desc,
                          nds, R, J->mode, J->flag, (const uint32_t*)nullptr, (const int*)nullptr);
@@ -656,36 +679,46 @@ hm_status apply_sweep(hm_job* J) {
                      J->stream) != hipSuccess)
     return fail(ctx, HM_EDEVICE, "descriptor upload failed");
   if ((J->stages & HM_STAGE_KNN) && J->mode >= 0) {
-    std::vector<std::pair<int64_t, WorkItem>> it;
+    // two item lists, each longest first: datasets whose bf16 rows fit the
+    // query-resident sweep, then the rest
+    std::vector<std::pair<int64_t, WorkItem>> it[2];
     const int qw = sweep_rows(J->mode);
     for (int i = 0; i < nds; ++i) {
       const DsHost& h = J->dsh[i];
       const int64_t cost = cdiv(h.n, h.nsplit) * h.kp;
+      const int li = (J->mode == 0 && h.kp16 * 2 <= kQresRowBytes) ? 0 : 1;
       for (int64_t rb = 0; rb < h.n; rb += qw)
         for (int s = 0; s < h.nsplit; ++s)
-          it.push_back({cost, WorkItem{i, s, (int)rb, (int)std::min<int64_t>(qw, h.n - rb)}});
+          it[li].push_back({cost, WorkItem{i, s, (int)rb, (int)std::min<int64_t>(qw, h.n - rb)}});
     }
-    std::stable_sort(it.begin(), it.end(),
-                     [](const auto& a, const auto& b) { return a.first > b.first; });
     // The sweep grid is G persistent workgroups; workgroup b takes items b,
     // b + G, ... and runs on XCD b % X (round-robin dispatch). Item slots are
     // permuted so that in every round each XCD receives a contiguous run of
     // the (row block, split) order above: its workgroups then stream the same
     // few row blocks and column splits at the same time and share them in
-    // that XCD's L2.
-    const int64_t ni = (int64_t)it.size();
-    const int64_t G = sweep_grid(ctx, J->mode, ni);
+    // that XCD's L2. Each list is one launch with its own grid.
     const int64_t X = (J->xcd_perm && ctx->num_cus % 8 == 0) ? 8 : 1;
-    J->items_l1_h.resize(it.size());
-    for (int64_t r0 = 0; r0 < ni; r0 += G) {
-      const int64_t rl = std::min(G, ni - r0);  // items in this round
-      const bool perm = X > 1 && rl % X == 0;
-      const int64_t per = rl / X;
-      for (int64_t l = 0; l < rl; ++l) {
-        const int64_t p = perm ? (l % per) * X + l / per : l;
-        J->items_l1_h[r0 + p] = it[r0 + l].second;
+    const int64_t ni = (int64_t)(it[0].size() + it[1].size());
+    J->items_l1_h.resize(ni);
+    int64_t base = 0;
+    for (int li = 0; li < 2; ++li) {
+      auto& L = it[li];
+      std::stable_sort(L.begin(), L.end(),
+                       [](const auto& a, const auto& b) { return a.first > b.first; });
+      const int64_t nl = (int64_t)L.size();
+      const int64_t G = sweep_grid(ctx, J->mode, nl);
+      for (int64_t r0 = 0; r0 < nl; r0 += G) {
+        const int64_t rl = std::min(G, nl - r0);  // items in this round
+        const bool perm = X > 1 && rl % X == 0;
+        const int64_t per = rl / X;
+        for (int64_t l = 0; l < rl; ++l) {
+          const int64_t p = perm ? (l % per) * X + l / per : l;
+          J->items_l1_h[base + r0 + p] = L[r0 + l].second;
+        }
       }
+      base += nl;
     }
+    J->n_items_qres = (int)it[0].size();
     J->n_items_l1 = (int)ni;
     if (hipMemcpyAsync(J->items_l1, J->items_l1_h.data(), sizeof(WorkItem) * ni,
                        hipMemcpyHostToDevice, J->stream) != hipSuccess)
