@@ -180,6 +180,121 @@ __global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_
   }
 }
 
+// Same calibration with G lanes per row (k <= G <= 64): lane t holds the
+// row's distance t and evaluates term t of every bisection step (the exp
+// and its division), and every lane of the group folds the terms in the
+// reference's order t = 1 .. k-1 from the group's registers, so the sums,
+// the bracket updates and the outputs are those of smooth_knn_kernel bit
+// for bit. Loop bounds depend on the row only, so a group stays converged.
+template <int G>
+__global__ __launch_bounds__(256) void smooth_knn_group_kernel(
+    const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
+    const double* __restrict__ target_log2k, int mode, float* __restrict__ sigma,
+    float* __restrict__ rho, double* __restrict__ sigma64, uint32_t* __restrict__ zrow) {
+  static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0 && 256 % G == 0, "group size");
+  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int l = (int)(threadIdx.x % G);
+  if (g >= total_rows) return;  // whole groups leave together
+  const int di = calib_find_ds(ds, nds, g);
+  const DsDesc& D = ds[di];
+  const int64_t i = g - D.row_off;
+  const int k = D.k;
+  const float x = l < k ? D.knn_dist[i * k + l] : 0.0f;
+  const double target = target_log2k[di];
+
+  // rho: the first positive distance in row order
+  float r = 0.0f;
+  for (int t = 0; t < k; ++t) {
+    const float v = __shfl(x, t, G);
+    if (v > 0.0f) { r = v; break; }
+  }
+  const bool term = l >= 1 && l < k;  // lanes holding a term of the sum
+  if (mode == CALIB_REF_F32) {
+    const float fmax = 3.40282347e+38f;
+    float lo = 0.0f, hi = fmax, mid = 1.0f;
+    for (int it = 0; it < 64; ++it) {
+      float e = 0.0f;
+      if (term) {
+        const float dd = x - r;
+        e = dd > 0.0f ? hm_expf(-(dd / mid)) : 1.0f;
+      }
+      float psum = 0.0f;
+      for (int t = 1; t < k; ++t) psum += __shfl(e, t, G);
+      if (fabs((double)psum - target) < 1e-5) break;
+      if ((double)psum > target) {
+        hi = mid;
+        mid = (lo + hi) / 2.0f;
+      } else {
+        lo = mid;
+        if (hi >= fmax)
+          mid *= 2.0f;
+        else
+          mid = (lo + hi) / 2.0f;
+      }
+    }
+    float s = mid;
+    bool z = true;
+    if (r > 0.0f) {
+      double acc = 0.0;
+      for (int t = 0; t < k; ++t) acc = acc + (double)__shfl(x, t, G);
+      const float fl = 1e-3f * (float)(acc / (double)k);
+      if (s < fl) s = fl;
+      z = false;
+    }
+    if (l == 0) {
+      rho[g] = r;
+      zrow[g] = z ? 1u : 0u;  // z: the floor needs mean_all (calib_fixup_kernel)
+      sigma[g] = s;
+      if (sigma64) sigma64[g] = (double)s;
+    }
+  } else {
+    const double rr = (double)r;
+    double lo = 0.0, hi = INFINITY, mid = 1.0;
+    for (int it = 0; it < 2200; ++it) {
+      double e = 0.0;
+      if (term) {
+        const double dd = (double)x - rr;
+        e = dd > 0.0 ? exp(-(dd / mid)) : 1.0;
+      }
+      double psum = 0.0;
+      for (int t = 1; t < k; ++t) psum += __shfl(e, t, G);
+      if (psum == target) break;
+      double nmid;
+      if (psum > target) {
+        hi = mid;
+        nmid = 0.5 * (lo + hi);
+      } else {
+        lo = mid;
+        nmid = isinf(hi) ? mid * 2.0 : 0.5 * (lo + hi);
+      }
+      if (nmid == mid || isinf(nmid)) break;
+      mid = nmid;
+    }
+    double s = mid;
+    bool z = true;
+    if (r > 0.0f) {
+      double acc = 0.0;
+      for (int t = 0; t < k; ++t) acc += (double)__shfl(x, t, G);
+      const double fl = 1e-3 * (acc / (double)k);
+      if (s < fl) s = fl;
+      z = false;
+    }
+    if (l == 0) {
+      rho[g] = r;
+      zrow[g] = z ? 1u : 0u;
+      if (sigma64) sigma64[g] = s;
+      sigma[g] = (float)s;
+    }
+  }
+}
+
+template __global__ void smooth_knn_group_kernel<16>(const DsDesc*, int, int64_t, const double*,
+                                                     int, float*, float*, double*, uint32_t*);
+template __global__ void smooth_knn_group_kernel<32>(const DsDesc*, int, int64_t, const double*,
+                                                     int, float*, float*, double*, uint32_t*);
+template __global__ void smooth_knn_group_kernel<64>(const DsDesc*, int, int64_t, const double*,
+                                                     int, float*, float*, double*, uint32_t*);
+
 // Applies the global-mean floor to rows with rho == 0. One workgroup per
 // dataset; the mean is the reference's left-to-right f64 fold, computed by
 // one lane and only if the dataset has such a row.

@@ -40,6 +40,9 @@ __global__ void fallback_items_kernel(const DsDesc*, int, const uint32_t*, const
 __global__ void smooth_knn_kernel(const DsDesc*, int, int64_t, const double*, int, float*, float*,
                                   double*, uint32_t*);
 __global__ void calib_fixup_kernel(const DsDesc*, int, float*, double*, const uint32_t*);
+template <int G>
+__global__ void smooth_knn_group_kernel(const DsDesc*, int, int64_t, const double*, int, float*,
+                                        float*, double*, uint32_t*);
 __global__ void directed_kernel(const DsDesc*, int, int64_t, const float*, const float*, uint32_t,
                                 uint32_t*, uint32_t*, float*, uint32_t*);
 __global__ void expf_kernel(const float*, float*, int64_t);
@@ -503,9 +506,20 @@ hm_status enqueue_pipeline(hm_job* J) {
     for (int i = 1; i <= 4; ++i) mark(J, i);
   }
   if (J->stages & HM_STAGE_CALIB) {
-    hipLaunchKernelGGL(smooth_knn_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream,
-                       J->d_desc, nds, R, (const double*)J->target, J->opt.calib, J->sigma, J->rho,
-                       J->sigma64, J->zrow);
+    // G lanes per row when the largest k fits a group, else a thread per row
+    const int G = J->kmax <= 16 ? 16 : J->kmax <= 32 ? 32 : J->kmax <= 64 ? 64 : 0;
+#define HM_CALIB_G(GS)                                                                         \
+  hipLaunchKernelGGL((smooth_knn_group_kernel<GS>), dim3(grid_for(R * GS, 256)), dim3(256), 0, \
+                     J->stream, J->d_desc, nds, R, (const double*)J->target, J->opt.calib,     \
+                     J->sigma, J->rho, J->sigma64, J->zrow)
+    if (G == 16) HM_CALIB_G(16);
+    else if (G == 32) HM_CALIB_G(32);
+    else if (G == 64) HM_CALIB_G(64);
+    else
+      hipLaunchKernelGGL(smooth_knn_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream,
+                         J->d_desc, nds, R, (const double*)J->target, J->opt.calib, J->sigma,
+                         J->rho, J->sigma64, J->zrow);
+#undef HM_CALIB_G
     hipLaunchKernelGGL(calib_fixup_kernel, dim3(nds), dim3(256), 0, J->stream, J->d_desc,
                        J->opt.calib, J->sigma, J->sigma64, (const uint32_t*)J->zrow);
     HM_LAUNCH(ctx);
