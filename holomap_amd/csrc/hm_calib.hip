@@ -11,7 +11,8 @@
 // The floor for rows without a non-zero distance uses mean_all, a
 // sequential f64 mean over all n*k distances (src/smooth_knn.rs:26). That
 // fold is inherently serial, so it is evaluated lazily by one lane in
-// calib_fixup_kernel, and only when such a row exists.
+// calib_fixup_kernel, and only for datasets with such a row: every such row
+// sets its dataset's flag zany[ds] (a plain store of 1; all writers agree).
 #include "hm_internal.hpp"
 
 namespace hm {
@@ -93,7 +94,7 @@ __device__ __forceinline__ int calib_find_ds(const DsDesc* ds, int nds, int64_t 
 __global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
                                   const double* __restrict__ target_log2k, int mode,
                                   float* __restrict__ sigma, float* __restrict__ rho,
-                                  double* __restrict__ sigma64, uint32_t* __restrict__ zrow) {
+                                  double* __restrict__ sigma64, uint32_t* __restrict__ zany) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total_rows) return;
   const int di = calib_find_ds(ds, nds, g);
@@ -138,9 +139,8 @@ __global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_
       for (int t = 0; t < k; ++t) acc = acc + (double)row[t];
       const float fl = 1e-3f * (float)(acc / (double)k);
       if (s < fl) s = fl;
-      zrow[g] = 0;
     } else {
-      zrow[g] = 1;  // floor needs mean_all: applied by calib_fixup_kernel
+      zany[di] = 1u;  // floor needs mean_all: applied by calib_fixup_kernel
     }
     sigma[g] = s;
     if (sigma64) sigma64[g] = (double)s;
@@ -171,9 +171,8 @@ __global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_
       for (int t = 0; t < k; ++t) acc += (double)row[t];
       const double fl = 1e-3 * (acc / (double)k);
       if (s < fl) s = fl;
-      zrow[g] = 0;
     } else {
-      zrow[g] = 1;
+      zany[di] = 1u;
     }
     if (sigma64) sigma64[g] = s;
     sigma[g] = (float)s;
@@ -190,7 +189,7 @@ template <int G>
 __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
     const double* __restrict__ target_log2k, int mode, float* __restrict__ sigma,
-    float* __restrict__ rho, double* __restrict__ sigma64, uint32_t* __restrict__ zrow) {
+    float* __restrict__ rho, double* __restrict__ sigma64, uint32_t* __restrict__ zany) {
   static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0 && 256 % G == 0, "group size");
   const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const int l = (int)(threadIdx.x % G);
@@ -243,7 +242,7 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
     }
     if (l == 0) {
       rho[g] = r;
-      zrow[g] = z ? 1u : 0u;  // z: the floor needs mean_all (calib_fixup_kernel)
+      if (z) zany[di] = 1u;  // the floor needs mean_all (calib_fixup_kernel)
       sigma[g] = s;
       if (sigma64) sigma64[g] = (double)s;
     }
@@ -281,7 +280,7 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
     }
     if (l == 0) {
       rho[g] = r;
-      zrow[g] = z ? 1u : 0u;
+      if (z) zany[di] = 1u;
       if (sigma64) sigma64[g] = s;
       sigma[g] = (float)s;
     }
@@ -301,42 +300,38 @@ template __global__ void smooth_knn_group_kernel<64>(const DsDesc*, int, int64_t
 __global__ __launch_bounds__(256) void calib_fixup_kernel(const DsDesc* __restrict__ ds, int mode,
                                                           float* __restrict__ sigma,
                                                           double* __restrict__ sigma64,
-                                                          const uint32_t* __restrict__ zrow) {
+                                                          const float* __restrict__ rho,
+                                                          uint32_t* __restrict__ zany) {
+  if (!zany[blockIdx.x]) return;  // no row of this dataset needs the floor
   const DsDesc& D = ds[blockIdx.x];
-  __shared__ int any;
   __shared__ double mean_s;
-  if (threadIdx.x == 0) any = 0;
+  if (threadIdx.x == 0) {
+    const int64_t m = D.n * D.k;
+    double s = 0.0;
+    for (int64_t t = 0; t < m; ++t) s = s + (double)D.knn_dist[t];
+    mean_s = s / (double)m;
+  }
   __syncthreads();
-  int mine = 0;
-  for (int64_t i = threadIdx.x; i < D.n; i += 256) mine |= (int)zrow[D.row_off + i];
-  if (__syncthreads_or(mine)) {
-    if (threadIdx.x == 0) {
-      const int64_t m = D.n * D.k;
-      double s = 0.0;
-      for (int64_t t = 0; t < m; ++t) s = s + (double)D.knn_dist[t];
-      mean_s = s / (double)m;
-    }
-    __syncthreads();
-    const double mean_all = mean_s;
-    for (int64_t i = threadIdx.x; i < D.n; i += 256) {
-      const int64_t g = D.row_off + i;
-      if (!zrow[g]) continue;
-      if (mode == CALIB_REF_F32) {
-        const float fl = 1e-3f * (float)mean_all;
-        float s = sigma[g];
-        if (s < fl) s = fl;
-        sigma[g] = s;
-        if (sigma64) sigma64[g] = (double)s;
-      } else {
-        double s = sigma64 ? sigma64[g] : (double)sigma[g];
-        const double fl = 1e-3 * mean_all;
-        if (s < fl) s = fl;
-        if (sigma64) sigma64[g] = s;
-        sigma[g] = (float)s;
-      }
+  const double mean_all = mean_s;
+  for (int64_t i = threadIdx.x; i < D.n; i += 256) {
+    const int64_t g = D.row_off + i;
+    if (rho[g] > 0.0f) continue;  // rows with a non-zero distance have their floor
+    if (mode == CALIB_REF_F32) {
+      const float fl = 1e-3f * (float)mean_all;
+      float s = sigma[g];
+      if (s < fl) s = fl;
+      sigma[g] = s;
+      if (sigma64) sigma64[g] = (double)s;
+    } else {
+      double s = sigma64 ? sigma64[g] : (double)sigma[g];
+      const double fl = 1e-3 * mean_all;
+      if (s < fl) s = fl;
+      if (sigma64) sigma64[g] = s;
+      sigma[g] = (float)s;
     }
   }
-  (void)any;
+  __syncthreads();
+  if (threadIdx.x == 0) zany[blockIdx.x] = 0u;  // cleared for the next run
 }
 
 // Directed membership strengths (src/fuzzy.rs:31-48), one thread per edge.

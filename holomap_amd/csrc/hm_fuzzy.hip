@@ -3,10 +3,11 @@
 // The reference sorts the directed edges by (row, col), builds the sorted,
 // de-duplicated union of A and A^T coordinates and fuses each pair with the
 // probabilistic t-conorm w = (v + vt) - v*vt in f32. Here the same COO comes
-// from two stable radix sorts (hm_sort.hip):
+// from one stable radix sort (hm_sort.hip) and a per-row rank sort:
 //   sort 1 by global column -> every row c's in-edges, sources ascending;
-//   sort 2 of that sequence by source row -> every row's out-edges, columns
-//            ascending (list A_r);
+//   each row's kept out-edges ranked by column within the row (a lane group
+//            per row: a row has at most k of them) -> list A_r, columns
+//            ascending, at outptr[r] = exclusive scan of the kept counts;
 // then, load-balanced (one thread per element, no per-row loops, so rows of
 // very high in-degree cost nothing extra):
 //   * each out-edge r->c looks up its reverse c->r in A_c by binary search
@@ -24,14 +25,85 @@
 
 namespace hm {
 
-__global__ void gather_key2(const uint32_t* __restrict__ pay1s, const uint32_t* __restrict__ key1,
-                            const uint32_t* __restrict__ srcrow, int64_t n, uint32_t sentinel,
-                            uint32_t* __restrict__ key2) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t e = pay1s[p];
-  key2[p] = key1[e] != sentinel ? srcrow[e] : sentinel;
+// A lists, G lanes per row (G a power of two <= 64; lane t holds out-edge
+// slots t, t + G, ... of the row). PLACE 0: cnt[r] = kept out-edges of row
+// r. PLACE 1: each kept edge goes to outptr[r] + (number of kept edges of
+// the row with a smaller column): ocol/oval/p2s (column, weight, edge id)
+// and k2s (= r) at that slot.
+template <int G, int PLACE>
+__global__ __launch_bounds__(256) void union_rows_a(
+    const DsDesc* __restrict__ ds, int nds, int64_t total_rows, const uint32_t* __restrict__ key1,
+    const float* __restrict__ wval, uint32_t sentinel, uint32_t* __restrict__ cnt,
+    const uint32_t* __restrict__ outptr, uint32_t* __restrict__ ocol, float* __restrict__ oval,
+    uint32_t* __restrict__ p2s, uint32_t* __restrict__ k2s) {
+  static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "group size");
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int t = (int)(threadIdx.x % G);
+  if (r >= total_rows) return;  // whole groups leave together
+  int lo = 0, hi = nds - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ds[mid].row_off <= r) lo = mid; else hi = mid - 1;
+  }
+  const DsDesc& D = ds[lo];
+  const int k = D.k;
+  const int64_t e0 = D.edge_off + (r - D.row_off) * k;  // the row's first slot
+  if constexpr (PLACE == 0) {
+    uint32_t n_kept = 0;
+    for (int s = t; s < k; s += G) n_kept += key1[e0 + s] != sentinel ? 1u : 0u;
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) n_kept += __shfl_xor(n_kept, o, G);
+    if (t == 0) cnt[r] = n_kept;
+  } else {
+    auto place = [&](int s, uint32_t c, uint32_t rank) {
+      const uint32_t pos = outptr[r] + rank;
+      ocol[pos] = c;
+      oval[pos] = wval[e0 + s];
+      p2s[pos] = (uint32_t)(e0 + s);
+      k2s[pos] = (uint32_t)r;
+    };
+    if (k <= G) {
+      // one slot per lane; every lane of the group takes part in the shuffles
+      const uint32_t c = t < k ? key1[e0 + t] : sentinel;
+      uint32_t rank = 0;
+      for (int u = 0; u < k; ++u) {
+        const uint32_t cu = __shfl(c, u, G);
+        rank += (cu != sentinel && cu < c) ? 1u : 0u;
+      }
+      if (c != sentinel) place(t, c, rank);
+    } else {
+      for (int s = t; s < k; s += G) {
+        const uint32_t c = key1[e0 + s];
+        if (c == sentinel) continue;
+        uint32_t rank = 0;
+        for (int u = 0; u < k; ++u) {
+          const uint32_t cu = key1[e0 + u];
+          rank += (cu != sentinel && cu < c) ? 1u : 0u;
+        }
+        place(s, c, rank);
+      }
+    }
+  }
 }
+
+template __global__ void union_rows_a<16, 0>(const DsDesc*, int, int64_t, const uint32_t*,
+                                             const float*, uint32_t, uint32_t*, const uint32_t*,
+                                             uint32_t*, float*, uint32_t*, uint32_t*);
+template __global__ void union_rows_a<16, 1>(const DsDesc*, int, int64_t, const uint32_t*,
+                                             const float*, uint32_t, uint32_t*, const uint32_t*,
+                                             uint32_t*, float*, uint32_t*, uint32_t*);
+template __global__ void union_rows_a<32, 0>(const DsDesc*, int, int64_t, const uint32_t*,
+                                             const float*, uint32_t, uint32_t*, const uint32_t*,
+                                             uint32_t*, float*, uint32_t*, uint32_t*);
+template __global__ void union_rows_a<32, 1>(const DsDesc*, int, int64_t, const uint32_t*,
+                                             const float*, uint32_t, uint32_t*, const uint32_t*,
+                                             uint32_t*, float*, uint32_t*, uint32_t*);
+template __global__ void union_rows_a<64, 0>(const DsDesc*, int, int64_t, const uint32_t*,
+                                             const float*, uint32_t, uint32_t*, const uint32_t*,
+                                             uint32_t*, float*, uint32_t*, uint32_t*);
+template __global__ void union_rows_a<64, 1>(const DsDesc*, int, int64_t, const uint32_t*,
+                                             const float*, uint32_t, uint32_t*, const uint32_t*,
+                                             uint32_t*, float*, uint32_t*, uint32_t*);
 
 __device__ __forceinline__ uint32_t lb_u32(const uint32_t* __restrict__ a, uint32_t lo, uint32_t hi,
                                            uint32_t v) {
@@ -40,18 +112,6 @@ __device__ __forceinline__ uint32_t lb_u32(const uint32_t* __restrict__ a, uint3
     if (a[mid] < v) lo = mid + 1; else hi = mid;
   }
   return lo;
-}
-
-// A lists: column and weight of every sort-2 position (valid prefix only).
-__global__ void union_gather_a(const uint32_t* __restrict__ p2s, const uint32_t* __restrict__ key1,
-                               const float* __restrict__ wval, const uint32_t* __restrict__ outptr,
-                               int64_t total_rows, uint32_t* __restrict__ ocol,
-                               float* __restrict__ oval) {
-  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= (int64_t)outptr[total_rows]) return;
-  const uint32_t e = p2s[a];
-  ocol[a] = key1[e];
-  oval[a] = wval[e];
 }
 
 // Reverse lookup of every out-edge r->c in A_c; marks mutual edges by id.

@@ -39,7 +39,7 @@ __global__ void fallback_items_kernel(const DsDesc*, int, const uint32_t*, const
                                       int64_t, int64_t, int, WorkItem*, int*, int*);
 __global__ void smooth_knn_kernel(const DsDesc*, int, int64_t, const double*, int, float*, float*,
                                   double*, uint32_t*);
-__global__ void calib_fixup_kernel(const DsDesc*, int, float*, double*, const uint32_t*);
+__global__ void calib_fixup_kernel(const DsDesc*, int, float*, double*, const float*, uint32_t*);
 template <int G>
 __global__ void smooth_knn_group_kernel(const DsDesc*, int, int64_t, const double*, int, float*,
                                         float*, double*, uint32_t*);
@@ -48,6 +48,8 @@ __global__ void directed_kernel(const DsDesc*, int, int64_t, const float*, const
 __global__ void expf_kernel(const float*, float*, int64_t);
 template <typename T>
 __global__ void scan_tile_sums(const T*, int64_t, T*);
+template <typename T>
+__global__ void scan_small(const T*, int64_t, T*, T*, uint32_t*, int*);
 template <typename T>
 __global__ void scan_sums(T*, int64_t, T*);
 template <typename T>
@@ -58,10 +60,10 @@ __global__ void radix_scatter(const uint32_t*, const uint32_t*, int64_t, int, in
 __global__ void lower_bounds(const uint32_t*, int64_t, int64_t, uint32_t*);
 __global__ void compact_kernel(const uint32_t*, const uint32_t*, int64_t, uint32_t*, int*);
 __global__ void fill_u32(uint32_t*, int64_t, uint32_t);
-__global__ void gather_key2(const uint32_t*, const uint32_t*, const uint32_t*, int64_t, uint32_t,
-                            uint32_t*);
-__global__ void union_gather_a(const uint32_t*, const uint32_t*, const float*, const uint32_t*,
-                               int64_t, uint32_t*, float*);
+template <int G, int PLACE>
+__global__ void union_rows_a(const DsDesc*, int, int64_t, const uint32_t*, const float*, uint32_t,
+                             uint32_t*, const uint32_t*, uint32_t*, float*, uint32_t*, uint32_t*);
+
 __global__ void union_mutual(const uint32_t*, const uint32_t*, const uint32_t*, const float*,
                              const uint32_t*, int64_t, float*, uint32_t*);
 __global__ void union_nonmutual(const uint32_t*, const uint32_t*, const uint32_t*, int64_t, int64_t,
@@ -95,6 +97,7 @@ struct hm_ctx {
 namespace {
 
 constexpr int kScanTile = 4096;
+constexpr int64_t kScanSmall = 1024 * 64;  // arrays scanned by one workgroup
 constexpr int kRadixTile = 4096;
 
 struct DsHost {
@@ -142,8 +145,8 @@ struct hm_job {
            *indptr_g = nullptr, *total_nnz = nullptr, *indptr_l = nullptr, *sc64 = nullptr;
   uint32_t *ccnt = nullptr;
   uint32_t *knn_idx = nullptr, *flag = nullptr, *flagpos = nullptr, *list = nullptr,
-           *zrow = nullptr, *key1 = nullptr, *pay1 = nullptr, *srcrow = nullptr, *k1s = nullptr,
-           *p1s = nullptr, *key2 = nullptr, *k2s = nullptr, *p2s = nullptr, *tmpk = nullptr,
+           *zany = nullptr, *key1 = nullptr, *pay1 = nullptr, *srcrow = nullptr, *k1s = nullptr,
+           *p1s = nullptr, *cntu32 = nullptr, *k2s = nullptr, *p2s = nullptr, *tmpk = nullptr,
            *tmpv = nullptr, *inptr = nullptr, *outptr = nullptr, *rows = nullptr, *cols = nullptr,
            *table = nullptr, *toffs = nullptr, *sc32 = nullptr, *sc32t = nullptr,
            *ocol = nullptr, *mflag = nullptr, *nmf = nullptr, *bpos = nullptr, *bsrc = nullptr,
@@ -300,7 +303,7 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
     b += R * 12 + P.fb_cap * 16 + R * P.xs * P.kmax * 8;
   }
   b += E * 8;                                        // knn idx + dist
-  b += R * 24 + nds * 8;                             // sigma, rho, sigma64, zrow
+  b += R * 20 + nds * 12;                            // sigma, rho, sigma64; flags, targets
   if (stages & HM_STAGE_FUZZY) {
     b += E * 20 * 4 + (R + 1) * 12 + R * 16 + 16 + 2 * E * 12 + (R + nds) * 8;
     b += 2 * 256 * (cdiv(E, kRadixTile) + 1) * 4;
@@ -336,14 +339,36 @@ hm_status dalloc(hm_job* J, T** p, int64_t count) {
 
 // ---- device building blocks ----
 
+unsigned grid_for(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, cdiv(n, per)); }
+
 template <typename T>
 hm_status scan_excl(hm_job* J, const T* in, int64_t n, T* out, T* total, T* tmp) {
   if (n <= 0) return HM_OK;
+  if (n <= kScanSmall) {
+    hipLaunchKernelGGL(scan_small<T>, dim3(1), dim3(1024), 0, J->stream, in, n, out, total,
+                       (uint32_t*)nullptr, (int*)nullptr);
+    HM_LAUNCH(J->ctx);
+    return HM_OK;
+  }
   const int64_t nt = cdiv(n, kScanTile);
   hipLaunchKernelGGL(scan_tile_sums<T>, dim3((unsigned)nt), dim3(256), 0, J->stream, in, n, tmp);
   hipLaunchKernelGGL(scan_sums<T>, dim3(1), dim3(1024), 0, J->stream, tmp, nt, total);
   hipLaunchKernelGGL(scan_tiles<T>, dim3((unsigned)nt), dim3(256), 0, J->stream, in, n,
                      (const T*)tmp, out);
+  HM_LAUNCH(J->ctx);
+  return HM_OK;
+}
+
+// list[0 .. count) = indices i < n with flag[i] set, in order; count on the device.
+hm_status compact_flags(hm_job* J, const uint32_t* flag, int64_t n, uint32_t* list, int* count) {
+  if (n <= kScanSmall) {
+    hipLaunchKernelGGL(scan_small<uint32_t>, dim3(1), dim3(1024), 0, J->stream, flag, n,
+                       (uint32_t*)nullptr, (uint32_t*)nullptr, list, count);
+  } else {
+    HM_TRY(scan_excl<uint32_t>(J, flag, n, J->flagpos, (uint32_t*)nullptr, J->sc32));
+    hipLaunchKernelGGL(compact_kernel, dim3(grid_for(n, 256)), dim3(256), 0, J->stream, flag,
+                       (const uint32_t*)J->flagpos, n, list, count);
+  }
   HM_LAUNCH(J->ctx);
   return HM_OK;
 }
@@ -378,7 +403,6 @@ int64_t sweep_grid(const hm_ctx* ctx, int mode, int64_t items) {
   return std::max<int64_t>(1, std::min<int64_t>(items, ctx->num_cus));
 }
 
-unsigned grid_for(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, cdiv(n, per)); }
 
 // Both passes of the candidate sweep in `mode` over `items` (a host count,
 // or a device count when nitems_dev is set; qlist maps item rows to global
@@ -455,10 +479,7 @@ hm_status enqueue_pipeline(hm_job* J) {
         // smaller error bound) and re-ranked; only rows still uncertified
         // reach the exact scan below
         const int em = J->esc_mode;
-        HM_TRY(scan_excl<uint32_t>(J, J->flag, R, J->flagpos, (uint32_t*)nullptr, J->sc32));
-        hipLaunchKernelGGL(compact_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream,
-                           (const uint32_t*)J->flag, (const uint32_t*)J->flagpos, R, J->list,
-                           J->count);
+        HM_TRY(compact_flags(J, J->flag, R, J->list, J->count));
         hipLaunchKernelGGL(sweep_items_kernel, dim3(1), dim3(64), 0, J->stream, J->d_desc, nds,
                            (const uint32_t*)J->list, (const int*)J->count, sweep_rows(em), cus,
                            em == 2 ? 1.1e12 : 0.45e12, J->items_l2, J->nitems_l2,
@@ -477,9 +498,7 @@ hm_status enqueue_pipeline(hm_job* J) {
       mark(J, 3);
     }
     // ---- exact scan of the uncertified rows ----
-    HM_TRY(scan_excl<uint32_t>(J, J->flag, R, J->flagpos, (uint32_t*)nullptr, J->sc32));
-    hipLaunchKernelGGL(compact_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream,
-                       (const uint32_t*)J->flag, (const uint32_t*)J->flagpos, R, J->list, J->count);
+    HM_TRY(compact_flags(J, J->flag, R, J->list, J->count));
     const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(J->fb_cap, 2 * cus));
     // merge groups: G splits of k keys fit one 1024-key sort; two merge
     // levels cover up to G * G splits
@@ -511,17 +530,17 @@ hm_status enqueue_pipeline(hm_job* J) {
 #define HM_CALIB_G(GS)                                                                         \
   hipLaunchKernelGGL((smooth_knn_group_kernel<GS>), dim3(grid_for(R * GS, 256)), dim3(256), 0, \
                      J->stream, J->d_desc, nds, R, (const double*)J->target, J->opt.calib,     \
-                     J->sigma, J->rho, J->sigma64, J->zrow)
+                     J->sigma, J->rho, J->sigma64, J->zany)
     if (G == 16) HM_CALIB_G(16);
     else if (G == 32) HM_CALIB_G(32);
     else if (G == 64) HM_CALIB_G(64);
     else
       hipLaunchKernelGGL(smooth_knn_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream,
                          J->d_desc, nds, R, (const double*)J->target, J->opt.calib, J->sigma,
-                         J->rho, J->sigma64, J->zrow);
+                         J->rho, J->sigma64, J->zany);
 #undef HM_CALIB_G
     hipLaunchKernelGGL(calib_fixup_kernel, dim3(nds), dim3(256), 0, J->stream, J->d_desc,
-                       J->opt.calib, J->sigma, J->sigma64, (const uint32_t*)J->zrow);
+                       J->opt.calib, J->sigma, J->sigma64, (const float*)J->rho, J->zany);
     HM_LAUNCH(ctx);
   }
   mark(J, 5);
@@ -535,18 +554,20 @@ hm_status enqueue_pipeline(hm_job* J) {
     HM_TRY(radix_sort(J, J->key1, J->pay1, E, bits, J->k1s, J->p1s));
     hipLaunchKernelGGL(lower_bounds, dim3(grid_for(R + 1, 256)), dim3(256), 0, J->stream,
                        (const uint32_t*)J->k1s, E, R, J->inptr);
-    hipLaunchKernelGGL(gather_key2, dim3(grid_for(E, 256)), dim3(256), 0, J->stream,
-                       (const uint32_t*)J->p1s, (const uint32_t*)J->key1,
-                       (const uint32_t*)J->srcrow, E, sent, J->key2);
+    // A lists: every row's kept out-edges, columns ascending
+    const int GA = J->kmax <= 16 ? 16 : J->kmax <= 32 ? 32 : 64;
+#define HM_ROWS_A(GS, PL)                                                                      \
+  hipLaunchKernelGGL((union_rows_a<GS, PL>), dim3(grid_for(R * GS, 256)), dim3(256), 0,         \
+                     J->stream, J->d_desc, nds, R, (const uint32_t*)J->key1,                   \
+                     (const float*)J->wval, sent, J->cntu32, (const uint32_t*)J->outptr, J->ocol, \
+                     J->oval, J->p2s, J->k2s)
+    if (GA == 16) HM_ROWS_A(16, 0); else if (GA == 32) HM_ROWS_A(32, 0); else HM_ROWS_A(64, 0);
     HM_LAUNCH(ctx);
-    HM_TRY(radix_sort(J, J->key2, J->p1s, E, bits, J->k2s, J->p2s));
-    hipLaunchKernelGGL(lower_bounds, dim3(grid_for(R + 1, 256)), dim3(256), 0, J->stream,
-                       (const uint32_t*)J->k2s, E, R, J->outptr);
+    HM_TRY(scan_excl<uint32_t>(J, J->cntu32, R, J->outptr, J->outptr + R, J->sc32));
+    if (GA == 16) HM_ROWS_A(16, 1); else if (GA == 32) HM_ROWS_A(32, 1); else HM_ROWS_A(64, 1);
+#undef HM_ROWS_A
     // load-balanced union: reverse lookups, non-mutual in-edges, merge path
     const unsigned ge = grid_for(E, 256);
-    hipLaunchKernelGGL(union_gather_a, dim3(ge), dim3(256), 0, J->stream, (const uint32_t*)J->p2s,
-                       (const uint32_t*)J->key1, (const float*)J->wval,
-                       (const uint32_t*)J->outptr, R, J->ocol, J->oval);
     hipLaunchKernelGGL(union_mutual, dim3(ge), dim3(256), 0, J->stream, (const uint32_t*)J->k2s,
                        (const uint32_t*)J->p2s, (const uint32_t*)J->ocol, (const float*)J->oval,
                        (const uint32_t*)J->outptr, R, J->orev, J->mflag);
@@ -849,12 +870,14 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
   HM_A(J->sigma, R);
   HM_A(J->rho, R);
   HM_A(J->sigma64, R);
-  HM_A(J->zrow, R);
+  HM_A(J->zany, nds);  // per-dataset flags, kept clear between runs by calib_fixup_kernel
+  if (hipMemsetAsync(J->zany, 0, sizeof(uint32_t) * nds, J->stream) != hipSuccess)
+    return bail(fail(ctx, HM_EDEVICE, "flag clear failed"));
   HM_A(J->target, nds);
   const int64_t ntr = cdiv(E, kRadixTile) + 1;
   if (stages & HM_STAGE_FUZZY) {
     HM_A(J->key1, E); HM_A(J->pay1, E); HM_A(J->srcrow, E); HM_A(J->wval, E);
-    HM_A(J->k1s, E); HM_A(J->p1s, E); HM_A(J->key2, E); HM_A(J->k2s, E); HM_A(J->p2s, E);
+    HM_A(J->k1s, E); HM_A(J->p1s, E); HM_A(J->cntu32, R); HM_A(J->k2s, E); HM_A(J->p2s, E);
     HM_A(J->tmpk, E); HM_A(J->tmpv, E);
     HM_A(J->inptr, R + 1); HM_A(J->outptr, R + 1);
     HM_A(J->cntu, R); HM_A(J->indptr_g, R + 1); HM_A(J->total_nnz, 1);
