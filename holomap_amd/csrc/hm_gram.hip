@@ -39,9 +39,10 @@
 //    row, each a fixed column subset. T = the C-th smallest group minimum.
 //    C distinct groups each contain a key <= T, so at least C keys of the
 //    split are <= T.
-//  PASS 1 (collect): every column with key <= T is appended to its lane's
-//    list in global memory (the row's candidate slots; a few dozen appends
-//    per lane over a whole split). All other columns have key > T: that is
+//  PASS 1 (collect): with T the smallest of the row's split thresholds,
+//    every column with key <= T is appended to its lane's list in global
+//    memory (the row's candidate slots; a few dozen appends per lane over a
+//    whole split). All other columns have key > T: that is
 //    the threshold the certification bound uses. A list that overflows is
 //    reported (threshold 0) and its row is recomputed by the exact scan.
 //    The keys of both passes come from identical MFMA sequences;
@@ -295,6 +296,18 @@ __device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[4], float (
   }
 }
 
+// Pass-1 threshold of a row, as T / s: the smallest of the pass-0
+// thresholds of its column splits. Every split collects the keys <= T, and
+// the split whose own threshold is T holds at least C of them, so the row
+// gets >= C candidates in all (a single split's threshold would give each
+// split about C). T = +inf (every split has fewer than C columns) must not
+// admit the columns outside a split, whose key' is -inf: T / s >= -FLT_MAX.
+__device__ __forceinline__ float sweep_threshold(const DsDesc& D, int64_t row, float scale) {
+  float t = INFINITY;
+  for (int sp = 0; sp < D.nsplit; ++sp) t = fminf(t, D.thr[(size_t)row * D.nsplit + sp]);
+  return fmaxf(t / scale, -0x1.fffffep127f);
+}
+
 // End of an item for one lane. Pass 0: T = C-th smallest of the row's 128
 // group minima (both lanes) = s * (C-th largest group maximum of key'),
 // found by bisection on the order-preserving u32 image of -key'. Pass 1:
@@ -418,10 +431,7 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
 #pragma unroll
           for (int a = 0; a < 16; ++a) gmax[u][t][a] = -INFINITY;
       } else {
-        // T = +inf (a split with fewer than C columns) must not admit the
-        // columns outside the split, whose key' is -inf
-        if (myvalid[u])
-          T[u] = fmaxf(D.thr[(size_t)myrow[u] * D.nsplit + W.split] / scale, -0x1.fffffep127f);
+        if (myvalid[u]) T[u] = sweep_threshold(D, myrow[u], scale);
       }
     }
 
@@ -575,9 +585,7 @@ __global__ __launch_bounds__(QresCfg::NW * 64, 1) void gram_sweep_qres_kernel(
 #pragma unroll
         for (int a = 0; a < 16; ++a) gmax[t][a] = -INFINITY;
     } else {
-      // T = +inf (a split with fewer than C columns) must not admit the
-      // columns outside the split, whose key' is -inf
-      if (myvalid) T = fmaxf(D.thr[(size_t)myrow * D.nsplit + W.split] / scale, -0x1.fffffep127f);
+      if (myvalid) T = sweep_threshold(D, myrow, scale);
     }
 
     HM_LDS_BARRIER();  // the previous item's stages and query rows are no longer read
