@@ -329,7 +329,9 @@ __device__ __forceinline__ float f32_round_up(double v) {
   return f;
 }
 
-// One row (global id grow) by one wave; a, ex: the wave's 512-key LDS areas.
+// One row (global id grow) by one wave; a, ex: the wave's 512-key LDS areas
+// (they may be the same: a lane reads a[e] before it writes ex[e], and a is
+// not read after that).
 __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, int nds,
                                            int64_t grow, int mode, uint32_t* __restrict__ flag,
                                            uint64_t* a, uint64_t* ex) {
@@ -513,18 +515,19 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
 
 // All rows (rlist == nullptr: rows 0 .. total_rows-1, one per wave), or the
 // *rcount rows listed in rlist (global ids), grid-stride.
-__global__ __launch_bounds__(256) void rerank_kernel(const DsDesc* __restrict__ ds_arr, int nds,
-                                                     int64_t total_rows, int mode,
-                                                     uint32_t* __restrict__ flag,
-                                                     const uint32_t* __restrict__ rlist,
-                                                     const int* __restrict__ rcount) {
+// 4 KB of LDS and at most 72 VGPRs per wave: seven waves per SIMD keep
+// enough rows in flight to cover their dependent memory round trips.
+__global__ __launch_bounds__(256, 7) void rerank_kernel(const DsDesc* __restrict__ ds_arr, int nds,
+                                                        int64_t total_rows, int mode,
+                                                        uint32_t* __restrict__ flag,
+                                                        const uint32_t* __restrict__ rlist,
+                                                        const int* __restrict__ rcount) {
   __shared__ uint64_t cs[4][512];
-  __shared__ uint64_t es[4][512];
   const int wave = threadIdx.x >> 6;
   const int64_t cnt = rlist ? (int64_t)*rcount : total_rows;
   for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < cnt; p += (int64_t)gridDim.x * 4) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // previous row's LDS use is over
-    rerank_row(ds_arr, nds, rlist ? (int64_t)rlist[p] : p, mode, flag, cs[wave], es[wave]);
+    rerank_row(ds_arr, nds, rlist ? (int64_t)rlist[p] : p, mode, flag, cs[wave], cs[wave]);
   }
 }
 
