@@ -277,7 +277,10 @@ def main():
     t_launch = stage["sweep"] * 1e-3 / 2
     peak = PEAK.get(args.precision, PEAK["bf16"])
     ach = flops / t_launch if t_launch > 0 else 0.0
-    out["roofline"] = {"bound": "mfma", "kernel": "hm::gram_sweep_kernel (2 launches/step)",
+    # bf16 rows of up to 384 bytes (d <= 192) go to the query-resident sweep
+    qres = args.precision == "bf16" and 2 * (-(-d // 64) * 64) <= 384
+    kname = "hm::gram_sweep_qres_kernel" if qres else "hm::gram_sweep_kernel"
+    out["roofline"] = {"bound": "mfma", "kernel": f"{kname} (2 launches/step)",
                        "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                        "frac": ach / peak, "traffic": sweep_traffic(args.workload, args.precision),
                        "flops_per_launch": flops, "launch_ms": t_launch * 1e3}
