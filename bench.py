@@ -46,8 +46,9 @@ def workload(name):
 def run_path(ctx, x, d, k, metric, precision, steps, warmup):
     import holomap_amd as H
     n = x.shape[0]
-    # timed loop: the whole pipeline replayed from a captured HIP graph
-    job = H.Job(ctx, [n], [d], [k], [metric], precision=precision, use_graph=True)
+    # timed loop: the whole pipeline replayed from a captured HIP graph; the
+    # first (untimed) run autotunes the sweep configuration
+    job = H.Job(ctx, [n], [d], [k], [metric], precision=precision, use_graph=True, autotune=True)
     job.upload(0, x)
     for _ in range(max(warmup, 1)):
         job.run()
@@ -61,7 +62,7 @@ def run_path(ctx, x, d, k, metric, precision, steps, warmup):
     counters = job.counters()
     job.close()
     # per-stage device times (HIP events on the job's stream), eager launches
-    tj = H.Job(ctx, [n], [d], [k], [metric], precision=precision, timing=True)
+    tj = H.Job(ctx, [n], [d], [k], [metric], precision=precision, timing=True, autotune=True)
     tj.upload(0, x)
     tj.run()
     tj.sync()
@@ -264,7 +265,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {desc}", "N": n, "d": d, "k": k,
                    "metric": metric, "precision": args.precision, "candidates_per_row": cnt["candidates"],
-                   "nsplit": cnt["nsplit"], "parallelism": "single GPU"},
+                   "nsplit": cnt["nsplit"], "autotune": True, "parallelism": "single GPU"},
         "stage_ms": {kk: round(v, 4) for kk, v in stage.items()},
         "fallback_rows": cnt["fallback_rows"],
         "nnz": cnt["nnz"],
