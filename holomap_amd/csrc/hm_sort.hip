@@ -85,70 +85,64 @@ __global__ __launch_bounds__(256) void scan_tiles(const T* __restrict__ in, int6
   }
 }
 
-// Single-workgroup exclusive scan for short arrays (the runtime uses it up
-// to 64K elements). Wave w owns the contiguous segment [w*seg, w*seg + seg)
-// and reads it 64 consecutive elements at a time: one pass sums the
-// segments, the 16 totals give each segment its base, and a second pass
-// scans every 64-element step with wave shuffles, carrying the running sum.
-// With `list` set it instead compacts: list[pos] = i for every i with
-// in[i] != 0 (in is a 0/1 flag array), pos = the exclusive scan, and
+// Single-workgroup exclusive scan for short arrays: NT threads, at most
+// NT * 64 elements (the runtime's kScanSmall). Wave w owns the contiguous
+// segment [w*seg, w*seg + seg), seg a multiple of 64, and loads all of it
+// at once into registers (64 consecutive elements per step, at most 64
+// steps), so the scan costs one memory round trip. The segment sums give
+// each wave its base; each step is scanned with wave shuffles, carrying the
+// running sum. With `list` set it instead compacts: list[pos] = i for every
+// i with in[i] != 0 (in is a 0/1 flag array), pos = the exclusive scan, and
 // *count = the number of flags.
-template <typename T>
-__global__ __launch_bounds__(1024) void scan_small(const T* __restrict__ in, int64_t n,
-                                                   T* __restrict__ out, T* __restrict__ total,
-                                                   uint32_t* __restrict__ list,
-                                                   int* __restrict__ count) {
-  __shared__ T wsum[16];
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void scan_small(const T* __restrict__ in, int64_t n,
+                                                 T* __restrict__ out, T* __restrict__ total,
+                                                 uint32_t* __restrict__ list,
+                                                 int* __restrict__ count) {
+  constexpr int NW = NT / 64, MS = 64;  // waves; steps held per lane
+  __shared__ T wsum[NW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t seg = (n + 1023) / 1024 * 64;
+  const int64_t seg = (n + NT - 1) / NT * 64;  // <= MS * 64 for n <= NT * 64
   const int64_t b = wave * seg < n ? wave * seg : n;
   const int64_t e = b + seg < n ? b + seg : n;
-  // eight 64-element steps per trip: their loads are in flight together
-  constexpr int U = 8;
+  // steps of this lane holding an element: j < nv
+  const int nv = e - b > lane ? (int)((e - b - lane + 63) / 64) : 0;
+  const T* p = in + b + lane;
+  T v[MS];
+#pragma unroll
+  for (int j = 0; j < MS; ++j) v[j] = j < nv ? p[j * 64] : (T)0;
   T s = 0;
-  for (int64_t i0 = b; i0 < e; i0 += 64 * U) {
-    T v[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const int64_t i = i0 + j * 64 + lane;
-      v[j] = i < e ? in[i] : (T)0;
-    }
-#pragma unroll
-    for (int j = 0; j < U; ++j) s += v[j];
-  }
+  for (int j = 0; j < MS; ++j) s += v[j];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if (lane == 0) wsum[wave] = s;
   __syncthreads();
   T run = 0, tot = 0;
 #pragma unroll
-  for (int w = 0; w < 16; ++w) {
-    const T v = wsum[w];
-    run += w < wave ? v : (T)0;
-    tot += v;
+  for (int w = 0; w < NW; ++w) {
+    const T x = wsum[w];
+    run += w < wave ? x : (T)0;
+    tot += x;
   }
-  for (int64_t i00 = b; i00 < e; i00 += 64 * U) {
-    T vv[U];
+  const int nsteps = (int)(seg / 64);  // uniform
 #pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const int64_t i = i00 + j * 64 + lane;
-      vv[j] = i < e ? in[i] : (T)0;
-    }
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const int64_t i = i00 + j * 64 + lane;
-      const T v = vv[j];
-      T x = v;  // inclusive scan of the step
+  for (int j = 0; j < MS; ++j) {
+    if (j < nsteps) {
+      T x = v[j];  // inclusive scan of the step
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
         const T y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
       }
-      const T excl = run + x - v;
-      if (list) {
-        if (i < e && v) list[excl] = (uint32_t)i;
-      } else if (i < e) {
-        out[i] = excl;
+      const T excl = run + x - v[j];
+      if (j < nv) {
+        const int64_t i = b + lane + (int64_t)j * 64;
+        if (list) {
+          if (v[j]) list[excl] = (uint32_t)i;
+        } else {
+          out[i] = excl;
+        }
       }
       run += __shfl(x, 63, 64);
     }
@@ -159,10 +153,10 @@ __global__ __launch_bounds__(1024) void scan_small(const T* __restrict__ in, int
   }
 }
 
-template __global__ void scan_small<uint32_t>(const uint32_t*, int64_t, uint32_t*, uint32_t*,
-                                              uint32_t*, int*);
-template __global__ void scan_small<uint64_t>(const uint64_t*, int64_t, uint64_t*, uint64_t*,
-                                              uint32_t*, int*);
+template __global__ void scan_small<uint32_t, 1024>(const uint32_t*, int64_t, uint32_t*, uint32_t*,
+                                                    uint32_t*, int*);
+template __global__ void scan_small<uint64_t, 512>(const uint64_t*, int64_t, uint64_t*, uint64_t*,
+                                                   uint32_t*, int*);
 template __global__ void scan_tile_sums<uint32_t>(const uint32_t*, int64_t, uint32_t*);
 template __global__ void scan_tile_sums<uint64_t>(const uint64_t*, int64_t, uint64_t*);
 template __global__ void scan_sums<uint32_t>(uint32_t*, int64_t, uint32_t*);
