@@ -1101,9 +1101,7 @@ hm_status hm_job_run(hm_job* J) {
   HM_HIP(ctx, hipSetDevice(ctx->device));
   if (J->opt.autotune && !J->tuned && (J->stages & HM_STAGE_KNN) && J->mode >= 0)
     HM_TRY(autotune(J));
-  // with a graph, the stage events are recorded by event nodes of the graph:
-  // the times read after a replay are that replay's
-  J->timed = J->opt.timing != 0;
+  J->timed = J->opt.timing && !J->opt.use_graph;
   if (J->opt.use_graph) {
     if (!J->captured) {
       hipGraph_t g = nullptr;

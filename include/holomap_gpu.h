@@ -76,7 +76,7 @@ typedef struct {
   int32_t nsplit;     /* column splits of the candidate sweep; 0: auto */
   int32_t use_graph;  /* 1: capture the pipeline once, replay after */
   int32_t autotune;   /* 1: time the sweep configurations on first run */
-  int32_t timing;     /* 1: record per-stage device times (event nodes when graphed) */
+  int32_t timing;     /* 1: record per-stage device times (eager runs only) */
   int32_t reserved;
 } hm_options;
 
