@@ -55,6 +55,8 @@ __host__ __device__ constexpr int sweep_threads(int mode) { return sweep_rows(mo
 // bf16 operand rows (kp16 * 2 bytes) up to this length are swept with the
 // item's query rows resident in LDS (gram_sweep_qres_kernel).
 constexpr int kQresRowBytes = 384;
+// Workgroups per dataset of the per-dataset statistics maxima (hm_prep.hip).
+constexpr int kStatParts = 32;
 
 // Number of float statistics kept per row for the candidate error bound.
 #define HM_NSTAT 8

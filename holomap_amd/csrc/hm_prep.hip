@@ -35,21 +35,20 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
 
 // Thread per global row: sequential f64 fold of x_f^2 in feature order.
 __global__ void rownorm_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
-                               float* __restrict__ nz_out /* global rows */,
-                               double* __restrict__ rn_out /* global rows */) {
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                               float* __restrict__ nz_g) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total_rows) return;
-  int di = find_ds(ds, nds, g);
+  const int di = find_ds(ds, nds, g);
   const DsDesc& D = ds[di];
-  int64_t i = g - D.row_off;
+  const int64_t i = g - D.row_off;
   const float* x = D.x + i * D.d;
   double s = 0.0;
   for (int f = 0; f < D.d; ++f) {
-    double v = (double)x[f];
+    const double v = (double)x[f];
     s = s + v * v;
   }
-  nz_out[g] = (float)s;
-  rn_out[g] = sqrt(s);
+  nz_g[g] = (float)s;
+  const_cast<double*>(D.rnorm)[i] = sqrt(s);
 }
 
 // Wave per (dataset, padded row). Writes xr, op32 (cosine), op16 (+ op16lo),
@@ -57,7 +56,7 @@ __global__ void rownorm_kernel(const DsDesc* __restrict__ ds, int nds, int64_t t
 __global__ __launch_bounds__(256) void prep_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_pad_rows,
     const int64_t* __restrict__ pad_off /* nds+1 prefix of n_pad */,
-    const float* __restrict__ nz_g, const double* __restrict__ rn_g, int mode_bits) {
+    const float* __restrict__ nz_g, int mode_bits) {
   const int lane = threadIdx.x & 63;
   int64_t pr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (pr >= total_pad_rows) return;
@@ -83,7 +82,7 @@ __global__ __launch_bounds__(256) void prep_kernel(
   double rn = 0.0;
   float nz = 0.0f;
   if (real) {
-    rn = rn_g[D.row_off + i];
+    rn = D.rnorm[i];
     nz = nz_g[D.row_off + i];
   }
   const bool zero_row = real && rn == 0.0;
@@ -167,18 +166,12 @@ __global__ __launch_bounds__(256) void prep_kernel(
   }
 }
 
-// Per-dataset maxima of rstat. One block per dataset, fixed reduction tree.
-__global__ __launch_bounds__(256) void statmax_kernel(const DsDesc* __restrict__ ds) {
-  const DsDesc& D = ds[blockIdx.x];
+// Per-dataset maxima of rstat in two launches (max is exact and order-free):
+// statmax_part_kernel, grid (datasets, P): workgroup p reduces rows
+// p, p + P, ... in 256-row strides into part[ds][p]; statmax_kernel, one
+// workgroup per dataset, reduces the P partial maxima.
+__device__ __forceinline__ void statmax_block(float (&m)[HM_NSTAT], float* __restrict__ out) {
   __shared__ float red[256][HM_NSTAT];
-  float m[HM_NSTAT];
-#pragma unroll
-  for (int s = 0; s < HM_NSTAT; ++s) m[s] = 0.0f;
-  for (int64_t i = threadIdx.x; i < D.n; i += 256) {
-    const float* st = D.rstat + i * HM_NSTAT;
-#pragma unroll
-    for (int s = 0; s < HM_NSTAT; ++s) m[s] = fmaxf(m[s], st[s]);
-  }
 #pragma unroll
   for (int s = 0; s < HM_NSTAT; ++s) red[threadIdx.x][s] = m[s];
   __syncthreads();
@@ -190,7 +183,34 @@ __global__ __launch_bounds__(256) void statmax_kernel(const DsDesc* __restrict__
     }
     __syncthreads();
   }
-  if (threadIdx.x < HM_NSTAT) const_cast<float*>(D.smax)[threadIdx.x] = red[0][threadIdx.x];
+  if (threadIdx.x < HM_NSTAT) out[threadIdx.x] = red[0][threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void statmax_part_kernel(const DsDesc* __restrict__ ds,
+                                                           float* __restrict__ part) {
+  const DsDesc& D = ds[blockIdx.x];
+  constexpr int P = kStatParts;  // gridDim.y
+  const int p = (int)blockIdx.y;
+  float m[HM_NSTAT];
+#pragma unroll
+  for (int s = 0; s < HM_NSTAT; ++s) m[s] = 0.0f;
+  for (int64_t i = (int64_t)p * 256 + threadIdx.x; i < D.n; i += (int64_t)P * 256) {
+    const float* st = D.rstat + i * HM_NSTAT;
+#pragma unroll
+    for (int s = 0; s < HM_NSTAT; ++s) m[s] = fmaxf(m[s], st[s]);
+  }
+  statmax_block(m, part + ((size_t)blockIdx.x * P + p) * HM_NSTAT);
+}
+
+__global__ __launch_bounds__(256) void statmax_kernel(const DsDesc* __restrict__ ds,
+                                                      const float* __restrict__ part) {
+  const DsDesc& D = ds[blockIdx.x];
+  constexpr int P = kStatParts;
+  float m[HM_NSTAT];
+#pragma unroll
+  for (int s = 0; s < HM_NSTAT; ++s)
+    m[s] = threadIdx.x < P ? part[((size_t)blockIdx.x * P + threadIdx.x) * HM_NSTAT + s] : 0.0f;
+  statmax_block(m, const_cast<float*>(D.smax));
 }
 
 }  // namespace hm

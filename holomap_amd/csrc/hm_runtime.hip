@@ -16,10 +16,10 @@
 
 namespace hm {
 // kernels (defined in the other translation units)
-__global__ void rownorm_kernel(const DsDesc*, int, int64_t, float*, double*);
-__global__ void prep_kernel(const DsDesc*, int, int64_t, const int64_t*, const float*,
-                            const double*, int);
-__global__ void statmax_kernel(const DsDesc*);
+__global__ void rownorm_kernel(const DsDesc*, int, int64_t, float*);
+__global__ void prep_kernel(const DsDesc*, int, int64_t, const int64_t*, const float*, int);
+__global__ void statmax_part_kernel(const DsDesc*, float*);
+__global__ void statmax_kernel(const DsDesc*, const float*);
 template <int MODE, int PASS>
 __global__ void gram_sweep_kernel(const DsDesc*, const WorkItem*, const int*, int, const uint32_t*);
 template <int PASS>
@@ -142,7 +142,7 @@ struct hm_job {
   DsDesc* d_desc = nullptr;
   float *x = nullptr, *xr = nullptr, *op32c = nullptr, *colbias = nullptr, *rstat = nullptr,
         *thr = nullptr,
-        *smax = nullptr, *nz = nullptr, *knn_dist = nullptr, *sigma = nullptr, *rho = nullptr,
+        *smax = nullptr, *statpart = nullptr, *nz = nullptr, *knn_dist = nullptr, *sigma = nullptr, *rho = nullptr,
         *wval = nullptr, *vals = nullptr;
   uint16_t *op16 = nullptr, *op16lo = nullptr;
   double *rnorm = nullptr, *sigma64 = nullptr, *target = nullptr;
@@ -303,7 +303,7 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
   if (stages & HM_STAGE_KNN) {
     b += P.sum_xr * 4 + P.sum_cos * 4;               // xr, op32 (cosine)
     if (mode == 0 || mode == 2) b += P.sum_x16 * 4;  // op16 + op16lo (bf16x3 or its escalation)
-    b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + R * 12;
+    b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 * (1 + kStatParts) + R * 12;
     b += P.sum_cand * 8 + R * 2 * kSplitCap * 12 + R * kSplitCap * 4;  // cand, ctau+ccnt, thr
     b += R * 12 + P.fb_cap * 16 + R * P.xs * P.kmax * 8;
   }
@@ -462,12 +462,15 @@ hm_status enqueue_pipeline(hm_job* J) {
   if (J->stages & HM_STAGE_KNN) {
     // ---- prep ----
     hipLaunchKernelGGL(rownorm_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, J->d_desc,
-                       nds, R, J->nz, J->rnorm);
+                       nds, R, J->nz);
     const int mode_bits = (J->mode == 0 ? 1 : 0) | 2;
     hipLaunchKernelGGL(prep_kernel, dim3(grid_for(J->total_pad, 4)), dim3(256), 0, J->stream,
                        J->d_desc, nds, J->total_pad, (const int64_t*)J->pad_off,
-                       (const float*)J->nz, (const double*)J->rnorm, mode_bits);
-    hipLaunchKernelGGL(statmax_kernel, dim3(nds), dim3(256), 0, J->stream, J->d_desc);
+                       (const float*)J->nz, mode_bits);
+    hipLaunchKernelGGL(statmax_part_kernel, dim3(nds, kStatParts), dim3(256), 0, J->stream,
+                       J->d_desc, J->statpart);
+    hipLaunchKernelGGL(statmax_kernel, dim3(nds), dim3(256), 0, J->stream, J->d_desc,
+                       (const float*)J->statpart);
     HM_LAUNCH(ctx);
     mark(J, 1);
     // ---- candidate sweep + re-rank ----
@@ -847,6 +850,7 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     HM_A(J->colbias, P.total_pad);
     HM_A(J->rstat, R * HM_NSTAT);
     HM_A(J->smax, (int64_t)nds * HM_NSTAT);
+    HM_A(J->statpart, (int64_t)nds * kStatParts * HM_NSTAT);
     HM_A(J->nz, R);
     HM_A(J->rnorm, R);
     // sized for any sweep configuration (see plan_sweep)
