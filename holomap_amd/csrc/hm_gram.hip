@@ -272,7 +272,7 @@ __device__ __forceinline__ void sweep_acc_start(floatx16 (&acc)[4], const uint8_
 // the wave hits.
 template <int PASS>
 __device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[4], float (&gmax)[4][16],
-                                               uint32_t colbase, float scale, float Tp, int cout,
+                                               uint32_t colbase, float Tp, int cout,
                                                gptr_u64* dst, int& cnt) {
   // +inf in a register: with a literal the compiler turns the med3 into a
   // NaN-canonicalising max (three instructions)
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
         // ---- epilogue of the block; the next block starts at its biases ----
 #pragma unroll
         for (int u = 0; u < NU; ++u)
-          sweep_epilogue<PASS>(acc[u], gmax[u], colbase, scale, T[u], D.cout, dst[u], cnt[u]);
+          sweep_epilogue<PASS>(acc[u], gmax[u], colbase, T[u], D.cout, dst[u], cnt[u]);
         floatx16 a0[4];
         sweep_acc_start(a0, st + Cfg::BIAS, half);
 #pragma unroll
@@ -513,27 +513,25 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
 // the biases (18 KB instead of 48 KB), a third of the copy traffic per MFMA.
 // Same tiles, MFMA sequence, epilogue and lists as gram_sweep_kernel<0, *>.
 // ---------------------------------------------------------------------------
-struct QresCfg {
-  static constexpr int QW = 256, NW = 8;
-  static constexpr int BIAS = 128 * 128;             // after the candidate rows
-  static constexpr int STAGE = BIAS + NW / 2 * 512;  // + NW/2 copies of the biases
-  static constexpr int NSTAGE = 3;
-  static constexpr int NI = 16 / NW;                 // candidate copy groups per wave
-  static constexpr int VM = NI + 1;                  // vmcnt entries per stage and wave
-  static constexpr int BIAS0 = NSTAGE * STAGE;       // the item's first block's biases
-  static constexpr int QOFF = BIAS0 + 512;           // slice c of query row r: QOFF + c*QW*128 + r*128
-  static constexpr int QMAXB = kQresRowBytes;        // row bytes held (three slices)
-  static constexpr int LDS = QOFF + QMAXB / 128 * QW * 128;
-  static_assert(LDS <= 160 * 1024, "LDS");
-  static_assert(QOFF % 128 == 0, "query rows are 128-byte aligned");
-  static_assert(NW * 64 == sweep_threads(0), "the runtime launches sweep_threads(0)");
-};
+// Copies of one stage: wave w copies candidate groups w*NI + i (8 rows of
+// the block at byte offset `off` of their rows) and 64 of the biases at boff.
+template <class Cfg>
+__device__ __forceinline__ void qres_issue(uint8_t* stage, const uint8_t* const (&csrc)[Cfg::NI],
+                                           const float* bsrc, int64_t blim, int64_t off,
+                                           int64_t boff, int wave) {
+#pragma unroll
+  for (int i = 0; i < Cfg::NI; ++i)
+    __builtin_amdgcn_global_load_lds(csrc[i] + off,
+                                     (lds_void_t*)(stage + (wave * Cfg::NI + i) * 1024), 16, 0, 0);
+  __builtin_amdgcn_global_load_lds(
+      boff < blim ? bsrc + boff : &kBiasOut,
+      (lds_void_t*)(stage + Cfg::BIAS + (wave >> 1) * 512 + (wave & 1) * 256), 4, 0, 0);
+}
 
-template <int PASS>
-__global__ __launch_bounds__(QresCfg::NW * 64, 1) void gram_sweep_qres_kernel(
+template <class Cfg, int PASS>
+__global__ __launch_bounds__(Cfg::NW * 64, 2) void gram_sweep_qres_kernel(
     const DsDesc* __restrict__ ds_arr, const WorkItem* __restrict__ items,
     const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist) {
-  using Cfg = QresCfg;
   constexpr int NI = Cfg::NI, NSTAGE = Cfg::NSTAGE, STAGE = Cfg::STAGE, QW = Cfg::QW;
   __shared__ __attribute__((aligned(1024))) uint8_t smem[Cfg::LDS];
 
@@ -607,22 +605,12 @@ __global__ __launch_bounds__(QresCfg::NW * 64, 1) void gram_sweep_qres_kernel(
       __builtin_amdgcn_global_load_lds(0 < blim ? bsrc : &kBiasOut,
                                        (lds_void_t*)(smem + Cfg::BIAS0 + wave * 256), 4, 0, 0);
     // stage copies: the block's candidate slice and the next block's biases
-    auto issue = [&](int slotg, int bi, int ci) {
-      uint8_t* stage = smem + slotg * STAGE;
-#pragma unroll
-      for (int i = 0; i < NI; ++i)
-        __builtin_amdgcn_global_load_lds(csrc[i] + bi * blkbytes + ci * 128,
-                                         (lds_void_t*)(stage + (wave * NI + i) * 1024), 16, 0, 0);
-      const int64_t boff = (int64_t)(bi + 1) * 128;
-      __builtin_amdgcn_global_load_lds(
-          boff < blim ? bsrc + boff : &kBiasOut,
-          (lds_void_t*)(stage + Cfg::BIAS + (wave >> 1) * 512 + (wave & 1) * 256), 4, 0, 0);
-    };
     int bi = 0, ci = 0;  // block and slice of the next stage to issue
 #pragma unroll
     for (int g = 0; g < NSTAGE - 1; ++g) {
       if (g < total) {
-        issue(g, bi, ci);
+        qres_issue<Cfg>(smem + g * STAGE, csrc, bsrc, blim, bi * blkbytes + ci * 128,
+                        (int64_t)(bi + 1) * 128, wave);
         if (++ci == nchunk) { ci = 0; ++bi; }
       }
     }
@@ -642,17 +630,18 @@ __global__ __launch_bounds__(QresCfg::NW * 64, 1) void gram_sweep_qres_kernel(
     for (int g = 0; g < total; ++g) {
       // this wave's copies of stage g have landed; one barrier publishes
       // stage g and retires stage g-1, whose slot gets stage g+NSTAGE-1
-      if (total - 1 - g >= 1) vm_wait<Cfg::VM>();
+      if (NSTAGE > 2 && total - 1 - g >= 1) vm_wait<Cfg::VM>();
       else vm_wait<0>();
       HM_LDS_BARRIER();
       if (g + NSTAGE - 1 < total) {
-        issue((g + NSTAGE - 1) % NSTAGE, bi, ci);
+        qres_issue<Cfg>(smem + ((g + NSTAGE - 1) % NSTAGE) * STAGE, csrc, bsrc, blim,
+                        bi * blkbytes + ci * 128, (int64_t)(bi + 1) * 128, wave);
         if (++ci == nchunk) { ci = 0; ++bi; }
       }
       const uint8_t* st = smem + (g % NSTAGE) * STAGE;
       sweep_mma<0>(acc, st, qrow + ch * (QW * 128), q, half, swz);
       if (++ch == nchunk) {
-        sweep_epilogue<PASS>(acc[0], gmax, colbase, scale, T, D.cout, dst, cnt);
+        sweep_epilogue<PASS>(acc[0], gmax, colbase, T, D.cout, dst, cnt);
         sweep_acc_start(acc[0], st + Cfg::BIAS, half);
         ch = 0;
         colbase += 128;
@@ -662,10 +651,14 @@ __global__ __launch_bounds__(QresCfg::NW * 64, 1) void gram_sweep_qres_kernel(
   }
 }
 
-template __global__ void gram_sweep_qres_kernel<0>(const DsDesc*, const WorkItem*, const int*, int,
-                                                   const uint32_t*);
-template __global__ void gram_sweep_qres_kernel<1>(const DsDesc*, const WorkItem*, const int*, int,
-                                                   const uint32_t*);
+#define HM_QRES_INST(CFG, PASS)                                                                \
+  template __global__ void gram_sweep_qres_kernel<CFG, PASS>(const DsDesc*, const WorkItem*,      \
+                                                             const int*, int, const uint32_t*);
+HM_QRES_INST(QresWide, 0)
+HM_QRES_INST(QresWide, 1)
+HM_QRES_INST(QresPair, 0)
+HM_QRES_INST(QresPair, 1)
+#undef HM_QRES_INST
 
 #define HM_INST(MODE, PASS)                                                       \
   template __global__ void gram_sweep_kernel<MODE, PASS>(const DsDesc*, const WorkItem*, \

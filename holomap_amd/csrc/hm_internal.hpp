@@ -55,8 +55,38 @@ __host__ __device__ constexpr int sweep_threads(int mode) { return sweep_rows(mo
 // bf16 operand rows (kp16 * 2 bytes) up to this length are swept with the
 // item's query rows resident in LDS (gram_sweep_qres_kernel).
 constexpr int kQresRowBytes = 384;
+// ... and up to this length by two 128-query workgroups per CU instead.
+constexpr int kQresPairRowBytes = 256;
 // Workgroups per dataset of the per-dataset statistics maxima (hm_prep.hip).
 constexpr int kStatParts = 32;
+
+// NW waves (32 queries each), NSTAGE ring slots, rows of up to QMAXB bytes.
+template <int NW_, int NSTAGE_, int QMAXB_>
+struct QresCfgT {
+  static constexpr int NW = NW_, QW = 32 * NW_;
+  static constexpr int BIAS = 128 * 128;             // after the candidate rows
+  static constexpr int STAGE = BIAS + NW / 2 * 512;  // + NW/2 copies of the biases
+  static constexpr int NSTAGE = NSTAGE_;
+  static constexpr int NI = 16 / NW;                 // candidate copy groups per wave
+  static constexpr int VM = NI + 1;                  // vmcnt entries per stage and wave
+  static constexpr int BIAS0 = NSTAGE * STAGE;       // the item's first block's biases
+  static constexpr int QOFF = BIAS0 + 512;           // slice c of query row r: QOFF + c*QW*128 + r*128
+  static constexpr int QMAXB = QMAXB_;               // row bytes held
+  static constexpr int LDS = QOFF + QMAXB / 128 * QW * 128;
+  static_assert(QW / 8 == 4 * NW, "four query copy groups per wave and slice");
+  static_assert(QOFF % 128 == 0, "query rows are 128-byte aligned");
+  static_assert(NSTAGE >= 2 && NSTAGE <= 3, "waits cover at most one later stage");
+};
+// Query-resident bf16 sweep configurations (gram_sweep_qres_kernel).
+// One workgroup per CU: 256 queries, 3-slot ring, rows up to 384 bytes.
+using QresWide = QresCfgT<8, 3, kQresRowBytes>;
+// Two workgroups per CU, each with its own barrier, so one's copies, LDS
+// reads and epilogue overlap the other's MFMAs: 128 queries, 2-slot ring,
+// rows up to 256 bytes.
+using QresPair = QresCfgT<4, 2, kQresPairRowBytes>;
+static_assert(QresWide::LDS <= 160 * 1024 && 2 * QresPair::LDS <= 160 * 1024, "LDS");
+static_assert(QresWide::NW * 64 == sweep_threads(0), "the runtime launches sweep_threads(0)");
+
 
 // Number of float statistics kept per row for the candidate error bound.
 #define HM_NSTAT 8

@@ -22,7 +22,7 @@ __global__ void statmax_part_kernel(const DsDesc*, float*);
 __global__ void statmax_kernel(const DsDesc*, const float*);
 template <int MODE, int PASS>
 __global__ void gram_sweep_kernel(const DsDesc*, const WorkItem*, const int*, int, const uint32_t*);
-template <int PASS>
+template <class Cfg, int PASS>
 __global__ void gram_sweep_qres_kernel(const DsDesc*, const WorkItem*, const int*, int,
                                        const uint32_t*);
 __global__ void exact_scan_kernel(const DsDesc*, const WorkItem*, const int*, int,
@@ -129,7 +129,9 @@ struct hm_job {
   std::vector<DsDesc> hdesc;
   int64_t total_rows = 0, total_pad = 0, total_edges = 0;
   int kmax = 0, mode = 0, xs = 1, fb_cap = 0, n_items_l1 = 0;
-  int n_items_qres = 0;  // the first items of items_l1 go to the query-resident sweep
+  // items_l1 holds three consecutive lists: query-resident sweep with two
+  // workgroups per CU, query-resident with one, streaming (kSweepLists)
+  int n_items_list[3] = {0, 0, 0};
   int esc_mode = -1;  // sweep mode of the escalation pass (-1: none)
   bool xcd_perm = true;  // permute sweep items for XCD-local L2 sharing
   bool tuned = false;    // autotuning done (hm_options.autotune)
@@ -402,12 +404,23 @@ hm_status radix_sort(hm_job* J, const uint32_t* kin, const uint32_t* vin, int64_
   return HM_OK;
 }
 
-// Persistent grid of the candidate sweep: one workgroup per CU, at most one
-// per item.
-int64_t sweep_grid(const hm_ctx* ctx, int mode, int64_t items) {
+// Persistent grid of the candidate sweep: `per_cu` workgroups per CU, at
+// most one per item.
+int64_t sweep_grid(const hm_ctx* ctx, int mode, int64_t items, int per_cu = 1) {
   (void)mode;
-  return std::max<int64_t>(1, std::min<int64_t>(items, ctx->num_cus));
+  return std::max<int64_t>(1, std::min<int64_t>(items, (int64_t)ctx->num_cus * per_cu));
 }
+
+// The three lists of the primary sweep: bf16 rows of <= kQresPairRowBytes
+// (two 128-query workgroups per CU, queries in LDS), of <= kQresRowBytes
+// (one 256-query workgroup, queries in LDS), and the rest (streaming).
+int sweep_list_of(int mode, int kp16) {
+  if (mode == 0 && kp16 * 2 <= kQresPairRowBytes) return 0;
+  if (mode == 0 && kp16 * 2 <= kQresRowBytes) return 1;
+  return 2;
+}
+constexpr int kListRows[3] = {128, 256, 0};  // query rows per item (0: sweep_rows(mode))
+constexpr int kListPerCu[3] = {2, 1, 1};     // workgroups per CU
 
 
 // Both passes of the candidate sweep in `mode` over `items` (a host count,
@@ -429,23 +442,29 @@ hm_status launch_sweep(hm_job* J, int mode, const WorkItem* items, const int* ni
   return HM_OK;
 }
 
-// Both passes of the primary sweep: the items of datasets with short bf16
-// rows (the first n_items_qres) keep their query rows in LDS, the rest
-// stream them with the candidates. Grids match the item permutation.
+// Both passes of the primary sweep over its three item lists (datasets with
+// short bf16 rows keep their query rows in LDS, the rest stream them with
+// the candidates). Grids match the item permutation of each list.
 hm_status launch_sweep_l1(hm_job* J) {
   hm_ctx* ctx = J->ctx;
-  const int nq = J->n_items_qres, no = J->n_items_l1 - nq;
-  if (nq > 0) {
-    const unsigned grid = (unsigned)sweep_grid(ctx, 0, nq);
-    hipLaunchKernelGGL((gram_sweep_qres_kernel<0>), dim3(grid), dim3(sweep_threads(0)), 0,
-                       J->stream, J->d_desc, J->items_l1, nullptr, nq, nullptr);
-    hipLaunchKernelGGL((gram_sweep_qres_kernel<1>), dim3(grid), dim3(sweep_threads(0)), 0,
-                       J->stream, J->d_desc, J->items_l1, nullptr, nq, nullptr);
-    HM_LAUNCH(ctx);
-  }
-  if (no > 0)
-    HM_TRY(launch_sweep(J, J->mode, J->items_l1 + nq, nullptr, no, nullptr,
-                        (unsigned)sweep_grid(ctx, J->mode, no)));
+  const WorkItem* it = J->items_l1;
+  const int n0 = J->n_items_list[0], n1 = J->n_items_list[1], n2 = J->n_items_list[2];
+#define HM_QRES(CFG, N, PER)                                                                   \
+  do {                                                                                         \
+    const unsigned grid = (unsigned)sweep_grid(ctx, 0, N, PER);                                \
+    hipLaunchKernelGGL((gram_sweep_qres_kernel<CFG, 0>), dim3(grid), dim3(CFG::NW * 64), 0,      \
+                       J->stream, J->d_desc, it, nullptr, N, nullptr);                         \
+    hipLaunchKernelGGL((gram_sweep_qres_kernel<CFG, 1>), dim3(grid), dim3(CFG::NW * 64), 0,      \
+                       J->stream, J->d_desc, it, nullptr, N, nullptr);                         \
+    HM_LAUNCH(ctx);                                                                            \
+  } while (0)
+  if (n0 > 0) HM_QRES(QresPair, n0, kListPerCu[0]);
+  it += n0;
+  if (n1 > 0) HM_QRES(QresWide, n1, kListPerCu[1]);
+  it += n1;
+#undef HM_QRES
+  if (n2 > 0)
+    HM_TRY(launch_sweep(J, J->mode, it, nullptr, n2, nullptr, (unsigned)sweep_grid(ctx, J->mode, n2)));
   return HM_OK;
 }
 
@@ -723,14 +742,13 @@ hm_status apply_sweep(hm_job* J) {
                      J->stream) != hipSuccess)
     return fail(ctx, HM_EDEVICE, "descriptor upload failed");
   if ((J->stages & HM_STAGE_KNN) && J->mode >= 0) {
-    // two item lists, each longest first: datasets whose bf16 rows fit the
-    // query-resident sweep, then the rest
-    std::vector<std::pair<int64_t, WorkItem>> it[2];
-    const int qw = sweep_rows(J->mode);
+    // three item lists (sweep_list_of), each longest first
+    std::vector<std::pair<int64_t, WorkItem>> it[3];
     for (int i = 0; i < nds; ++i) {
       const DsHost& h = J->dsh[i];
       const int64_t cost = cdiv(h.n, h.nsplit) * h.kp;
-      const int li = (J->mode == 0 && h.kp16 * 2 <= kQresRowBytes) ? 0 : 1;
+      const int li = sweep_list_of(J->mode, h.kp16);
+      const int qw = kListRows[li] ? kListRows[li] : sweep_rows(J->mode);
       for (int64_t rb = 0; rb < h.n; rb += qw)
         for (int s = 0; s < h.nsplit; ++s)
           it[li].push_back({cost, WorkItem{i, s, (int)rb, (int)std::min<int64_t>(qw, h.n - rb)}});
@@ -742,15 +760,15 @@ hm_status apply_sweep(hm_job* J) {
     // few row blocks and column splits at the same time and share them in
     // that XCD's L2. Each list is one launch with its own grid.
     const int64_t X = (J->xcd_perm && ctx->num_cus % 8 == 0) ? 8 : 1;
-    const int64_t ni = (int64_t)(it[0].size() + it[1].size());
+    const int64_t ni = (int64_t)(it[0].size() + it[1].size() + it[2].size());
     J->items_l1_h.resize(ni);
     int64_t base = 0;
-    for (int li = 0; li < 2; ++li) {
+    for (int li = 0; li < 3; ++li) {
       auto& L = it[li];
       std::stable_sort(L.begin(), L.end(),
                        [](const auto& a, const auto& b) { return a.first > b.first; });
       const int64_t nl = (int64_t)L.size();
-      const int64_t G = sweep_grid(ctx, J->mode, nl);
+      const int64_t G = sweep_grid(ctx, J->mode, nl, kListPerCu[li]);
       for (int64_t r0 = 0; r0 < nl; r0 += G) {
         const int64_t rl = std::min(G, nl - r0);  // items in this round
         const bool perm = X > 1 && rl % X == 0;
@@ -762,7 +780,7 @@ hm_status apply_sweep(hm_job* J) {
       }
       base += nl;
     }
-    J->n_items_qres = (int)it[0].size();
+    for (int li = 0; li < 3; ++li) J->n_items_list[li] = (int)it[li].size();
     J->n_items_l1 = (int)ni;
     if (hipMemcpyAsync(J->items_l1, J->items_l1_h.data(), sizeof(WorkItem) * ni,
                        hipMemcpyHostToDevice, J->stream) != hipSuccess)
@@ -965,7 +983,10 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
   if (stages & HM_STAGE_FUZZY) up(J->iptr_off, J->iptr_off_h.data(), sizeof(int64_t) * nds);
   if (knn && J->mode >= 0) {
     int64_t cap = 0;
-    for (auto& h : P.dsh) cap += cdiv(h.n, sweep_rows(J->mode)) * kSplitCap;
+    for (auto& h : P.dsh) {
+      const int li = sweep_list_of(J->mode, h.kp16);
+      cap += cdiv(h.n, kListRows[li] ? kListRows[li] : sweep_rows(J->mode)) * kSplitCap;
+    }
     hm_status s2 = dalloc(J, &J->items_l1, std::max<int64_t>(cap, 1));
     if (s2 != HM_OK) return bail(s2);
   }

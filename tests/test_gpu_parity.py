@@ -102,8 +102,11 @@ def test_c1_full_graph(ctx, prec, metric):
     assert_graph_equal(res, *O.knn_graph(x, 50, 15, metric))
 
 
+# d <= 192 runs the query-resident bf16 sweep, longer rows the streaming one
+# (d = 200: four 128-byte slices with a zero-padded tail; d = 449: eight)
 @pytest.mark.parametrize("shape", [(257, 3, 5), (130, 33, 7), (129, 1, 2), (200, 50, 64),
-                                   (65, 130, 64), (1000, 17, 30)])
+                                   (65, 130, 64), (1000, 17, 30), (300, 200, 15),
+                                   (385, 449, 10)])
 @pytest.mark.parametrize("metric", ["euclidean", "cosine"])
 def test_ragged_shapes(ctx, shape, metric):
     n, d, k = shape
