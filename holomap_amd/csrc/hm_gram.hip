@@ -64,7 +64,15 @@ typedef __attribute__((address_space(1))) uint64_t gptr_u64;
 // Workgroup barrier that orders LDS only. __syncthreads() also waits for
 // every outstanding global access (vmcnt 0), which would drain the copy
 // pipeline at each barrier; the copies are waited for explicitly instead.
-#define HM_LDS_BARRIER() asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory")
+// The lgkm wait is the builtin (0xc07f: lgkmcnt 0, vmcnt and expcnt at
+// their maxima, i.e. not waited for), which the compiler's own wait
+// insertion sees: after it, no LDS read counts as outstanding, so the waits
+// it places before later MFMAs can be partial.
+#define HM_LDS_BARRIER()                 \
+  do {                                   \
+    __builtin_amdgcn_s_waitcnt(0xc07f);  \
+    asm volatile("s_barrier" ::: "memory"); \
+  } while (0)
 
 template <int N>
 __device__ __forceinline__ void vm_wait() {
@@ -193,29 +201,35 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4]
     __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 16, 0);
     __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 16, 0);
   } else if constexpr (MODE == 0) {
+    // four k-steps of 16 features. The reads of step s + 2 go out with the
+    // MFMAs of step s; before step s an explicit wait leaves only the reads
+    // of step s + 1 in flight (LDS reads complete in order). The compiler
+    // still strengthens some of these waits to 0: it cannot tell the reads
+    // apart from the LDS copies in flight.
+    constexpr int RS = NU + 4;  // LDS reads per step
+    bf16x8_t bh[4][NU], ah[4][4];
 #pragma unroll
-    for (int kh = 0; kh < 2; ++kh) {
-      bf16x8_t bh[2][NU], ah[2][4];
+    for (int s = 0; s < 4 + 2; ++s) {
+      if (s >= 2 && s < 5) __builtin_amdgcn_s_waitcnt(0xc07f | (RS << 8));  // lgkmcnt(RS)
+      if (s == 5) __builtin_amdgcn_s_waitcnt(0xc07f);                     // lgkmcnt(0)
+      __builtin_amdgcn_sched_barrier(0);
+      if (s < 4) {
+        const int cs = ((2 * s + half) ^ swz) * 16;
 #pragma unroll
-      for (int v = 0; v < 2; ++v) {
-        const int cs = ((2 * (2 * kh + v) + half) ^ swz) * 16;
+        for (int u = 0; u < NU; ++u) bh[s][u] = *(const bf16x8_t*)(qrow_p + u * 32 * 128 + cs);
 #pragma unroll
-        for (int u = 0; u < NU; ++u) bh[v][u] = *(const bf16x8_t*)(qrow_p + u * 32 * 128 + cs);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) ah[v][t] = *(const bf16x8_t*)(cand + (t * 32 + q) * 128 + cs);
+        for (int t = 0; t < 4; ++t) ah[s][t] = *(const bf16x8_t*)(cand + (t * 32 + q) * 128 + cs);
       }
-#pragma unroll
-      for (int v = 0; v < 2; ++v)
+      if (s >= 2) {
 #pragma unroll
         for (int u = 0; u < NU; ++u)
 #pragma unroll
           for (int t = 0; t < 4; ++t)
-            acc[u][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[v][t], bh[v][u], acc[u][t], 0, 0, 0);
+            acc[u][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[s - 2][t], bh[s - 2][u],
+                                                                acc[u][t], 0, 0, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
-    __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);  // reads of steps 0-1
-    __builtin_amdgcn_sched_group_barrier(0x100, 2 * (NU + 4), 0);  // reads of steps 2-3
-    __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 4, 0);    // MFMAs of steps 0-1
-    __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 4, 0);    // MFMAs of steps 2-3
   } else {
     static_assert(NU == 1, "bf16x3 waves hold 32 queries");
     // two halves of two k-steps: hi and lo parts of both operands

@@ -329,12 +329,22 @@ __device__ __forceinline__ float f32_round_up(double v) {
   return f;
 }
 
-// One row (global id grow) by one wave; a, ex: the wave's 512-key LDS areas
-// (they may be the same: a lane reads a[e] before it writes ex[e], and a is
-// not read after that).
+// One row (global id grow) by one wave; lds: the wave's 4 KB LDS area.
+//  1. the stored keys of all lists are packed into LDS (all loads at once)
+//     and read back as up to 8 keys per lane;
+//  2. the k-th smallest key by bisection on its order-preserving u32 image
+//     (ballot counts), instead of sorting all keys;
+//  3. the candidates whose lower bound can beat the k-th upper bound are
+//     compacted (ballot prefix) into a column list;
+//  4. their exact distances, one lane per candidate (LDS: the columns in
+//     the first 2 KB, the distance bits in the second);
+//  5. the k smallest exact keys (distance bits, column) by rank: each key
+//     counts the smaller ones; keys are distinct (distinct columns).
+// The set of candidates scored exactly, and hence the output, is the same
+// as sorting all keys and taking the plausible prefix.
 __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, int nds,
                                            int64_t grow, int mode, uint32_t* __restrict__ flag,
-                                           uint64_t* a, uint64_t* ex) {
+                                           uint64_t* lds) {
   const int lane = threadIdx.x & 63;
   int lo = 0, hi = nds - 1;
   while (lo < hi) {
@@ -357,38 +367,53 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     return;
   }
 
-  // ---- gather the stored keys of all lists (packed) and sort them ----
+  // ---- 1. the stored keys of all lists, packed ----
   const uint64_t* src = D.cand + (size_t)i * L * CO;
   const uint32_t* cnt = D.ccnt + (size_t)i * L;
-  int nvalid = 0;
-  for (int l = 0; l < L; ++l) nvalid += (int)cnt[l];  // each <= CO
-  int len = 64;
-  while (len < nvalid) len <<= 1;
-  // the sweep stores (key' bits, column) with key' = key / s; sort keys are
+  // the sweep stores (key' bits, column) with key' = key / s; kept keys are
   // (order-preserving image of key = s * key', column)
   const float sc = metric == 1 ? -1.0f : -2.0f;
-  for (int e = lane; e < len; e += 64) {
-    uint64_t v = ~0ull;
-    if (e < nvalid) {
-      int l = 0, off = 0;
-      while (e - off >= (int)cnt[l]) off += (int)cnt[l++];
-      v = src[(size_t)l * CO + (e - off)];
-      v = ((uint64_t)ordered_u32(sc * __uint_as_float((uint32_t)(v >> 32))) << 32) | (uint32_t)v;
+  int nvalid = 0;
+  for (int l = 0; l < L; ++l) {
+    const int c = (int)cnt[l];  // <= CO <= 256
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int t = lane + 64 * u;
+      if (t < c) {
+        const uint64_t v = src[(size_t)l * CO + t];
+        lds[nvalid + t] =
+            ((uint64_t)ordered_u32(sc * __uint_as_float((uint32_t)(v >> 32))) << 32) | (uint32_t)v;
+      }
     }
-    a[e] = v;
+    nvalid += c;
   }
   uint64_t taumin = ~0ull;
   for (int s = 0; s < L; ++s) {
     const uint64_t t = D.ctau[(size_t)i * L + s];
     taumin = t < taumin ? t : taumin;
   }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  wave_sort_lds(a, len);
   if (nvalid < k) {
     // cannot happen for n >= k (the lists keep min(C, columns) keys);
     // leave the row to the exact scan rather than decode an empty slot
     if (lane == 0) flag[grow] = 1;
     return;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  // the high halves (ordered keys) of slots lane + 64 m (nvalid <= 512)
+  const uint32_t* l32 = (const uint32_t*)lds;
+  uint32_t kh[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) kh[m] = lane + 64 * m < nvalid ? l32[2 * (lane + 64 * m) + 1] : ~0u;
+
+  // ---- 2. the k-th smallest key: smallest v with #{hi <= v} >= k ----
+  uint32_t blo = 0, bhi = 0xffffffffu;
+  while (blo < bhi) {
+    const uint32_t mid = blo + ((bhi - blo) >> 1);
+    int c = 0;
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      c += __popcll(__ballot(lane + 64 * m < nvalid && kh[m] <= mid));
+    if (c >= k) bhi = mid; else blo = mid + 1;
   }
 
   // ---- error bound of the candidate keys for this row ----
@@ -440,7 +465,7 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   }
 
   // ---- prune: candidates that cannot beat the k key-best ones ----
-  const double kk = (double)unordered_f32((uint32_t)(a[k - 1] >> 32));
+  const double kk = (double)unordered_f32(blo);
   double thr;  // a candidate with lower bound >= thr is strictly beyond the top-k
   if (metric == 0) {
     double ub = nzi + kk + E;
@@ -452,55 +477,82 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     ub = ub > 0.0 ? ub : 0.0;
     thr = (double)next_up(f32_round_up(ub));
   }
+  // ---- 3. the plausible candidates, compacted (column list in lds) ----
+  // cl[pos] (pos <= the slot's index e) overwrites packed slot pos / 2 <= e,
+  // which was read in this step or an earlier one, never in a later one
+  uint32_t* cl = (uint32_t*)lds;  // <= 512 columns
+  uint32_t* xd = cl + 512;        // their exact distances (f32 bits)
   int np = 0;
-  for (int e = lane; e < nvalid; e += 64) {
-    const double key = (double)unordered_f32((uint32_t)(a[e] >> 32));
-    double lb = metric == 0 ? (nzi + key - E) * (1.0 - eps_exact) : 1.0 + key - E - eps_exact;
-    np += (lb < thr) ? 1 : 0;  // monotone in e: a prefix
-  }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) np += __shfl_xor(np, o, 64);
-  if (np < k) np = k;
+  for (int m = 0; m < 8; ++m) {
+    const int e = lane + 64 * m;
+    bool pass = false;
+    uint32_t col = 0;
+    if (e < nvalid) {
+      col = cl[2 * e];  // the slot's low half
+      const double kv = (double)unordered_f32(kh[m]);
+      const double lb = metric == 0 ? (nzi + kv - E) * (1.0 - eps_exact) : 1.0 + kv - E - eps_exact;
+      pass = lb < thr;
+    }
+    const uint64_t bal = __ballot(pass);
+    const int pos = np + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // every lane's read first
+    if (pass) cl[pos] = col;
+    np += __popcll(bal);
+  }
+  if (np < k) {
+    // cannot happen (the k key-best candidates pass)
+    if (lane == 0) flag[grow] = 1;
+    return;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
-  // ---- exact distances of the plausible prefix ----
-  int elen = 64;
-  while (elen < np) elen <<= 1;
+  // ---- 4. exact distances, one lane per candidate ----
   const float* qa = D.xr + (size_t)i * kp;
   bool badid = false;
-  for (int e = lane; e < elen; e += 64) {
-    uint64_t key = ~0ull;
-    if (e < np) {
-      const uint32_t j = (uint32_t)a[e];
-      if ((int64_t)j >= D.n) {
-        badid = true;  // never produced by the sweep; guard the row read
-      } else {
-        float dist;
-        if ((int64_t)j == i) {
-          dist = 0.0f;
-        } else {
-          const double rnb = metric == 1 ? D.rnorm[j] : 0.0;
-          dist = ref_dist(qa, D.xr + (size_t)j * kp, d4, metric, rna, rnb);
-        }
-        key = dist_key(dist, j);
-      }
+  for (int e = lane; e < np; e += 64) {
+    const uint32_t j = cl[e];
+    float dist = 0.0f;
+    if ((int64_t)j >= D.n) {
+      badid = true;  // never produced by the sweep; guard the row read
+    } else if ((int64_t)j != i) {
+      const double rnb = metric == 1 ? D.rnorm[j] : 0.0;
+      dist = ref_dist(qa, D.xr + (size_t)j * kp, d4, metric, rna, rnb);
     }
-    ex[e] = key;
+    xd[e] = __float_as_uint(dist);
   }
   if (__ballot(badid) != 0ull) {
     if (lane == 0) flag[grow] = 1;  // recomputed by the exact scan
     return;
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  wave_sort_lds(ex, elen);
-  for (int t = lane; t < k; t += 64) {
-    oidx[t] = (uint32_t)ex[t];
-    odst[t] = __uint_as_float((uint32_t)(ex[t] >> 32));
+
+  // ---- 5. the k smallest exact keys by rank ----
+  uint64_t kth = 0;  // the k-th smallest exact key, found by the lane that holds it
+  for (int m = 0; 64 * m < np; ++m) {
+    const int e = lane + 64 * m;
+    const uint64_t mine = e < np ? dist_key(__uint_as_float(xd[e]), cl[e]) : ~0ull;
+    int rk = 0;
+    for (int f = 0; f < np; ++f)  // same address in every lane: broadcasts
+      rk += dist_key(__uint_as_float(xd[f]), cl[f]) < mine ? 1 : 0;
+    if (e < np && rk < k) {
+      oidx[rk] = (uint32_t)mine;
+      odst[rk] = __uint_as_float((uint32_t)(mine >> 32));
+      if (rk == k - 1) kth = mine;
+    }
+  }
+  // the one lane holding the k-th key: reduce by max (the others hold 0)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t t = shfl_xor_u64(kth, o);
+    kth = t > kth ? t : kth;
   }
 
   // ---- certification against every column outside the candidate set ----
   bool cert = taumin != 0ull;  // 0: a candidate list overflowed (incomplete)
   if (cert && taumin != ~0ull) {
-    const float dk = __uint_as_float((uint32_t)(ex[k - 1] >> 32));
+    const float dk = __uint_as_float((uint32_t)(kth >> 32));
     const double tf = (double)unordered_f32((uint32_t)(taumin >> 32));
     const double nu = (double)next_up(dk);
     if (metric == 0) {
@@ -515,9 +567,9 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
 
 // All rows (rlist == nullptr: rows 0 .. total_rows-1, one per wave), or the
 // *rcount rows listed in rlist (global ids), grid-stride.
-// 4 KB of LDS and at most 72 VGPRs per wave: seven waves per SIMD keep
+// 4 KB of LDS and at most 80 VGPRs per wave: six waves per SIMD keep
 // enough rows in flight to cover their dependent memory round trips.
-__global__ __launch_bounds__(256, 7) void rerank_kernel(const DsDesc* __restrict__ ds_arr, int nds,
+__global__ __launch_bounds__(256, 6) void rerank_kernel(const DsDesc* __restrict__ ds_arr, int nds,
                                                         int64_t total_rows, int mode,
                                                         uint32_t* __restrict__ flag,
                                                         const uint32_t* __restrict__ rlist,
@@ -527,7 +579,7 @@ __global__ __launch_bounds__(256, 7) void rerank_kernel(const DsDesc* __restrict
   const int64_t cnt = rlist ? (int64_t)*rcount : total_rows;
   for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < cnt; p += (int64_t)gridDim.x * 4) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // previous row's LDS use is over
-    rerank_row(ds_arr, nds, rlist ? (int64_t)rlist[p] : p, mode, flag, cs[wave], cs[wave]);
+    rerank_row(ds_arr, nds, rlist ? (int64_t)rlist[p] : p, mode, flag, cs[wave]);
   }
 }
 

@@ -33,6 +33,28 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
   return __uint_as_float((uint32_t)h << 16);
 }
 
+// f64 DPP move (two 32-bit halves); lanes without a source read +0.0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const uint64_t u = (uint64_t)__double_as_longlong(x);
+  const uint32_t lo =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROWS, 0xf, false);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+// Inclusive lane-order scan with DPP (row shifts 1, 2, 4, 8, then the row
+// broadcasts of lanes 15 and 31); lane 63 ends with the wave's sum.
+__device__ __forceinline__ double wave_sum63(double x) {
+  x += dpp_f64<0x111, 0xf>(x);
+  x += dpp_f64<0x112, 0xf>(x);
+  x += dpp_f64<0x114, 0xf>(x);
+  x += dpp_f64<0x118, 0xf>(x);
+  x += dpp_f64<0x142, 0xa>(x);
+  x += dpp_f64<0x143, 0xc>(x);
+  return x;
+}
+
 // Thread per global row: sequential f64 fold of x_f^2 in feature order.
 __global__ void rownorm_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
                                float* __restrict__ nz_g) {
@@ -41,9 +63,27 @@ __global__ void rownorm_kernel(const DsDesc* __restrict__ ds, int nds, int64_t t
   const int di = find_ds(ds, nds, g);
   const DsDesc& D = ds[di];
   const int64_t i = g - D.row_off;
-  const float* x = D.x + i * D.d;
+  const int d = D.d;
+  const float* x = D.x + i * d;
   double s = 0.0;
-  for (int f = 0; f < D.d; ++f) {
+  int f = 0;
+  if ((d & 3) == 0 && (((uintptr_t)D.x) & 15) == 0) {
+    // 16-byte rows: 8 float4 loads in flight per batch, folded in order
+    for (; f + 32 <= d; f += 32) {
+      float4 b4[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) b4[u] = *(const float4*)(x + f + 4 * u);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        double v;
+        v = (double)b4[u].x; s = s + v * v;
+        v = (double)b4[u].y; s = s + v * v;
+        v = (double)b4[u].z; s = s + v * v;
+        v = (double)b4[u].w; s = s + v * v;
+      }
+    }
+  }
+  for (; f < d; ++f) {
     const double v = (double)x[f];
     s = s + v * v;
   }
@@ -51,15 +91,29 @@ __global__ void rownorm_kernel(const DsDesc* __restrict__ ds, int nds, int64_t t
   const_cast<double*>(D.rnorm)[i] = sqrt(s);
 }
 
-// Wave per (dataset, padded row). Writes xr, op32 (cosine), op16 (+ op16lo),
-// colbias and rstat. mode_bits: 1 = produce op16 (bf16 sweep), 2 = bf16x3 stats.
+__device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int nds, int64_t pr,
+                                         const int64_t* __restrict__ pad_off,
+                                         const float* __restrict__ nz_g, int mode_bits,
+                                         int lane);
+
+// Wave per (dataset, padded row), grid-stride. Writes xr, op32 (cosine),
+// op16 (+ op16lo), colbias and rstat. mode_bits: 1 = produce op16 (bf16
+// sweep), 2 = bf16x3 stats.
 __global__ __launch_bounds__(256) void prep_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_pad_rows,
     const int64_t* __restrict__ pad_off /* nds+1 prefix of n_pad */,
     const float* __restrict__ nz_g, int mode_bits) {
   const int lane = threadIdx.x & 63;
-  int64_t pr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (pr >= total_pad_rows) return;
+  for (int64_t pr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); pr < total_pad_rows;
+       pr += (int64_t)gridDim.x * 4)
+    prep_row(ds, nds, pr, pad_off, nz_g, mode_bits, lane);
+}
+
+// One padded row by one wave (prep_kernel's body).
+__device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int nds, int64_t pr,
+                                         const int64_t* __restrict__ pad_off,
+                                         const float* __restrict__ nz_g, int mode_bits,
+                                         int lane) {
   // dataset by padded-row prefix
   int lo = 0, hi = nds - 1;
   while (lo < hi) {
@@ -128,17 +182,18 @@ __global__ __launch_bounds__(256) void prep_kernel(
       s_rn += r * r;
     }
   }
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) {
-    s_xn += __shfl_xor(s_xn, m, 64);
-    s_en += __shfl_xor(s_en, m, 64);
-    s_xo += __shfl_xor(s_xo, m, 64);
-    s_hn += __shfl_xor(s_hn, m, 64);
-    s_ln += __shfl_xor(s_ln, m, 64);
-    s_hln += __shfl_xor(s_hln, m, 64);
-    s_rn += __shfl_xor(s_rn, m, 64);
+  // wave sums in lane 63 (the statistics feed error bounds, which hold for
+  // any summation order)
+  s_xn = wave_sum63(s_xn);
+  s_en = wave_sum63(s_en);
+  s_xo = wave_sum63(s_xo);
+  if (mode_bits & 2) {  // the bf16x3 statistics (zero otherwise)
+    s_hn = wave_sum63(s_hn);
+    s_ln = wave_sum63(s_ln);
+    s_hln = wave_sum63(s_hln);
+    s_rn = wave_sum63(s_rn);
   }
-  if (lane == 0) {
+  if (lane == 63) {
     float* cb = const_cast<float*>(D.colbias);
     // the sweep's accumulator start bias_j / s (s = -2 euclidean, -1 cosine;
     // exact): key bias ||x_j||^2, or -inf for a zero row under cosine

@@ -85,15 +85,54 @@ __global__ __launch_bounds__(256) void scan_tiles(const T* __restrict__ in, int6
   }
 }
 
+// Lane-order inclusive scan of a wave with DPP: row shifts by 1, 2, 4, 8
+// scan each row of 16 lanes, then two row broadcasts (lane 15 into rows 1
+// and 3, lane 31 into rows 2 and 3) join the rows. Six dependent VALU adds,
+// no LDS round trips. Lanes whose DPP source is outside the row, or whose
+// row is masked off, add the `old` operand, 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t x) {
+  return ((uint64_t)dpp_u32<CTRL, ROWS>((uint32_t)(x >> 32)) << 32) |
+         dpp_u32<CTRL, ROWS>((uint32_t)x);
+}
+template <int CTRL, int ROWS, typename T>
+__device__ __forceinline__ T dpp_t(T x) {
+  if constexpr (sizeof(T) == 8) return dpp_u64<CTRL, ROWS>(x);
+  else return dpp_u32<CTRL, ROWS>(x);
+}
+template <typename T>
+__device__ __forceinline__ T wave_scan_incl(T x) {
+  x += dpp_t<0x111, 0xf>(x);  // row_shr:1
+  x += dpp_t<0x112, 0xf>(x);  // row_shr:2
+  x += dpp_t<0x114, 0xf>(x);  // row_shr:4
+  x += dpp_t<0x118, 0xf>(x);  // row_shr:8
+  x += dpp_t<0x142, 0xa>(x);  // row_bcast:15 into rows 1, 3
+  x += dpp_t<0x143, 0xc>(x);  // row_bcast:31 into rows 2, 3
+  return x;
+}
+template <typename T>
+__device__ __forceinline__ T wave_lane63(T x) {
+  if constexpr (sizeof(T) == 8)
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), 63) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, 63);
+  else
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
 // Single-workgroup exclusive scan for short arrays: NT threads, at most
 // NT * 64 elements (the runtime's kScanSmall). Wave w owns the contiguous
 // segment [w*seg, w*seg + seg), seg a multiple of 64, and loads all of it
 // at once into registers (64 consecutive elements per step, at most 64
-// steps), so the scan costs one memory round trip. The segment sums give
-// each wave its base; each step is scanned with wave shuffles, carrying the
-// running sum. With `list` set it instead compacts: list[pos] = i for every
-// i with in[i] != 0 (in is a 0/1 flag array), pos = the exclusive scan, and
-// *count = the number of flags.
+// steps), so the scan costs one memory round trip. Every step is scanned
+// on its own (the steps are independent, so their DPP scans overlap); the
+// step totals give each step's offset in the segment and the segment sums
+// each wave's base. With `list` set it instead compacts: list[pos] = i for
+// every i with in[i] != 0 (in is a 0/1 flag array), pos = the exclusive
+// scan, and *count = the number of flags.
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void scan_small(const T* __restrict__ in, int64_t n,
                                                  T* __restrict__ out, T* __restrict__ total,
@@ -107,44 +146,41 @@ __global__ __launch_bounds__(NT) void scan_small(const T* __restrict__ in, int64
   const int64_t e = b + seg < n ? b + seg : n;
   // steps of this lane holding an element: j < nv
   const int nv = e - b > lane ? (int)((e - b - lane + 63) / 64) : 0;
+  const int nsteps = (int)(seg / 64);  // uniform
   const T* p = in + b + lane;
   T v[MS];
 #pragma unroll
   for (int j = 0; j < MS; ++j) v[j] = j < nv ? p[j * 64] : (T)0;
-  T s = 0;
-#pragma unroll
-  for (int j = 0; j < MS; ++j) s += v[j];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (lane == 0) wsum[wave] = s;
-  __syncthreads();
-  T run = 0, tot = 0;
-#pragma unroll
-  for (int w = 0; w < NW; ++w) {
-    const T x = wsum[w];
-    run += w < wave ? x : (T)0;
-    tot += x;
-  }
-  const int nsteps = (int)(seg / 64);  // uniform
+  // v[j] <- the step's exclusive scan plus the step's offset in the segment
+  uint64_t flags = 0;
+  T run = 0;
 #pragma unroll
   for (int j = 0; j < MS; ++j) {
     if (j < nsteps) {
-      T x = v[j];  // inclusive scan of the step
+      const T x = wave_scan_incl(v[j]);
+      if (list) flags |= (uint64_t)(v[j] != 0) << j;
+      v[j] = run + x - v[j];
+      run += wave_lane63(x);
+    }
+  }
+  if (lane == 0) wsum[wave] = run;
+  __syncthreads();
+  T base = 0, tot = 0;
 #pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const T y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
+  for (int w = 0; w < NW; ++w) {
+    const T x = wsum[w];
+    base += w < wave ? x : (T)0;
+    tot += x;
+  }
+#pragma unroll
+  for (int j = 0; j < MS; ++j) {
+    if (j < nv) {
+      const int64_t i = b + lane + (int64_t)j * 64;
+      if (list) {
+        if ((flags >> j) & 1) list[base + v[j]] = (uint32_t)i;
+      } else {
+        out[i] = base + v[j];
       }
-      const T excl = run + x - v[j];
-      if (j < nv) {
-        const int64_t i = b + lane + (int64_t)j * 64;
-        if (list) {
-          if (v[j]) list[excl] = (uint32_t)i;
-        } else {
-          out[i] = excl;
-        }
-      }
-      run += __shfl(x, 63, 64);
     }
   }
   if (tid == 0) {
@@ -191,10 +227,16 @@ __global__ __launch_bounds__(256) void radix_hist(const uint32_t* __restrict__ k
   for (int t = threadIdx.x; t < 1024; t += 256) (&cnt[0][0])[t] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RX_TILE + wave * 1024;
+  uint32_t digs[16];  // all loads first: one memory round trip, not 16
+#pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int64_t idx = base + r * 64 + lane;
-    const bool valid = idx < n;
-    const uint32_t dig = valid ? (keys[idx] >> shift) & 255u : 0u;
+    digs[r] = idx < n ? (keys[idx] >> shift) & 255u : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const bool valid = base + r * 64 + lane < n;
+    const uint32_t dig = digs[r];
     const uint64_t peers = match_digit(dig, valid);
     const int leader = peers ? (int)__ffsll((long long)peers) - 1 : 64;
     if (valid && lane == leader) cnt[wave][dig] += (uint32_t)__popcll(peers);
@@ -220,11 +262,15 @@ __global__ __launch_bounds__(256) void radix_scatter(const uint32_t* __restrict_
   const int64_t base = (int64_t)blockIdx.x * RX_TILE + wave * 1024;
   uint32_t key[16], val[16], rank[16];
 #pragma unroll
+  for (int r = 0; r < 16; ++r) {  // all loads first: one memory round trip
+    const int64_t idx = base + r * 64 + lane;
+    key[r] = idx < n ? kin[idx] : 0u;
+    val[r] = idx < n ? vin[idx] : 0u;
+  }
+#pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int64_t idx = base + r * 64 + lane;
     const bool valid = idx < n;
-    key[r] = valid ? kin[idx] : 0u;
-    val[r] = valid ? vin[idx] : 0u;
     const uint32_t dig = (key[r] >> shift) & 255u;
     const uint64_t peers = match_digit(dig, valid);
     const uint32_t before = cnt[wave][dig];  // earlier rounds of this wave
