@@ -329,19 +329,70 @@ __device__ __forceinline__ float f32_round_up(double v) {
   return f;
 }
 
-// One row (global id grow) by one wave; lds: the wave's 4 KB LDS area.
-//  1. the stored keys of all lists are packed into LDS (all loads at once)
-//     and read back as up to 8 keys per lane;
-//  2. the k-th smallest key by bisection on its order-preserving u32 image
-//     (ballot counts), instead of sorting all keys;
+#define RR_LISTS 10   // lists per row at most (two per column split)
+#define RR_QROW 512   // query rows of at most this many floats are staged in LDS
+
+// ref_dist with the query row in LDS (read at the same address by every
+// lane) and the candidate row fetched 32 floats at a time, so each batch
+// costs one memory round trip; same operations in the same order.
+__device__ __forceinline__ float rr_dist(const float* qs, const float* __restrict__ b, int d4,
+                                         int metric, double rna, double rnb) {
+  double s = 0.0;
+  int f = 0;
+  for (; f + 32 <= d4; f += 32) {
+    float4 y[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) y[u] = *(const float4*)(b + f + 4 * u);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const float4 x = *(const float4*)(qs + f + 4 * u);
+      if (metric == 0) {
+        double t;
+        t = (double)x.x - (double)y[u].x; s = s + t * t;
+        t = (double)x.y - (double)y[u].y; s = s + t * t;
+        t = (double)x.z - (double)y[u].z; s = s + t * t;
+        t = (double)x.w - (double)y[u].w; s = s + t * t;
+      } else {
+        s = s + (double)x.x * (double)y[u].x;
+        s = s + (double)x.y * (double)y[u].y;
+        s = s + (double)x.z * (double)y[u].z;
+        s = s + (double)x.w * (double)y[u].w;
+      }
+    }
+  }
+  for (; f < d4; f += 4) {
+    const float4 x = *(const float4*)(qs + f);
+    const float4 y = *(const float4*)(b + f);
+    if (metric == 0) {
+      double t;
+      t = (double)x.x - (double)y.x; s = s + t * t;
+      t = (double)x.y - (double)y.y; s = s + t * t;
+      t = (double)x.z - (double)y.z; s = s + t * t;
+      t = (double)x.w - (double)y.w; s = s + t * t;
+    } else {
+      s = s + (double)x.x * (double)y.x;
+      s = s + (double)x.y * (double)y.y;
+      s = s + (double)x.z * (double)y.z;
+      s = s + (double)x.w * (double)y.w;
+    }
+  }
+  if (metric == 0) return (float)sqrt(s);
+  return (float)f64max0(1.0 - s / (rna * rnb));
+}
+
+// One row (global id grow) by one wave; lds: the wave's 6 KB LDS area.
+//  1. the stored keys of all lists, up to 8 per lane, loaded at once;
+//  2. an upper bound of the k-th smallest key by bisection on the keys'
+//     order-preserving u32 images (ballot counts), instead of sorting;
 //  3. the candidates whose lower bound can beat the k-th upper bound are
 //     compacted (ballot prefix) into a column list;
 //  4. their exact distances, one lane per candidate (LDS: the columns in
 //     the first 2 KB, the distance bits in the second);
 //  5. the k smallest exact keys (distance bits, column) by rank: each key
 //     counts the smaller ones; keys are distinct (distinct columns).
-// The set of candidates scored exactly, and hence the output, is the same
-// as sorting all keys and taking the plausible prefix.
+// Sorting all keys and scoring the plausible prefix would score a subset of
+// these candidates; the extra ones have lower bounds above that prefix's
+// threshold, so they cannot enter the top k: the output is the same.
 __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, int nds,
                                            int64_t grow, int mode, uint32_t* __restrict__ flag,
                                            uint64_t* lds) {
@@ -367,26 +418,15 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     return;
   }
 
-  // ---- 1. the stored keys of all lists, packed ----
+  // ---- 1. the stored keys of all lists: slot e = lane + 64 m of the packed
+  // order (list after list) holds list l = the last with off[l] <= e ----
   const uint64_t* src = D.cand + (size_t)i * L * CO;
   const uint32_t* cnt = D.ccnt + (size_t)i * L;
-  // the sweep stores (key' bits, column) with key' = key / s; kept keys are
-  // (order-preserving image of key = s * key', column)
-  const float sc = metric == 1 ? -1.0f : -2.0f;
-  int nvalid = 0;
-  for (int l = 0; l < L; ++l) {
-    const int c = (int)cnt[l];  // <= CO <= 256
+  int off[RR_LISTS + 1];  // uniform
+  off[0] = 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int t = lane + 64 * u;
-      if (t < c) {
-        const uint64_t v = src[(size_t)l * CO + t];
-        lds[nvalid + t] =
-            ((uint64_t)ordered_u32(sc * __uint_as_float((uint32_t)(v >> 32))) << 32) | (uint32_t)v;
-      }
-    }
-    nvalid += c;
-  }
+  for (int l = 0; l < RR_LISTS; ++l) off[l + 1] = off[l] + (l < L ? (int)cnt[l] : 0);
+  const int nvalid = off[RR_LISTS];  // <= L * CO <= 512
   uint64_t taumin = ~0ull;
   for (int s = 0; s < L; ++s) {
     const uint64_t t = D.ctau[(size_t)i * L + s];
@@ -398,21 +438,49 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     if (lane == 0) flag[grow] = 1;
     return;
   }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  // the high halves (ordered keys) of slots lane + 64 m (nvalid <= 512)
-  const uint32_t* l32 = (const uint32_t*)lds;
-  uint32_t kh[8];
+  // the sweep stores (key' bits, column) with key' = key / s; kh holds the
+  // order-preserving image of key = s * key', kl the column
+  const float sc = metric == 1 ? -1.0f : -2.0f;
+  const int mc = (nvalid + 63) >> 6;  // slot rounds in use (uniform)
+  uint32_t kh[8], kl[8];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) kh[m] = lane + 64 * m < nvalid ? l32[2 * (lane + 64 * m) + 1] : ~0u;
+  for (int m = 0; m < 8; ++m) {
+    kh[m] = ~0u;
+    kl[m] = 0;
+    const int e = lane + 64 * m;
+    if (m < mc && e < nvalid) {
+      int l = 0, base = 0;
+#pragma unroll
+      for (int q = 1; q < RR_LISTS; ++q)
+        if (e >= off[q]) { l = q; base = off[q]; }
+      const uint64_t v = src[(size_t)l * CO + (e - base)];
+      kh[m] = ordered_u32(sc * __uint_as_float((uint32_t)(v >> 32)));
+      kl[m] = (uint32_t)v;
+    }
+  }
 
-  // ---- 2. the k-th smallest key: smallest v with #{hi <= v} >= k ----
-  uint32_t blo = 0, bhi = 0xffffffffu;
-  while (blo < bhi) {
+  // ---- 2. an upper bound of the k-th smallest key: bisection keeps
+  // #{kh <= bhi} >= k; any such bhi gives a valid (at most slightly larger)
+  // pruning threshold, so it stops once the bracket is 2^10 ulps wide ----
+  uint32_t blo = 0xffffffffu, bhi = 0;
+#pragma unroll
+  for (int m = 0; m < 8; ++m)
+    if (m < mc && lane + 64 * m < nvalid) {
+      blo = kh[m] < blo ? kh[m] : blo;
+      bhi = kh[m] > bhi ? kh[m] : bhi;
+    }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t a = (uint32_t)__shfl_xor((int)blo, o, 64), b = (uint32_t)__shfl_xor((int)bhi, o, 64);
+    blo = a < blo ? a : blo;
+    bhi = b > bhi ? b : bhi;
+  }
+  while (bhi - blo > 1024u) {
     const uint32_t mid = blo + ((bhi - blo) >> 1);
     int c = 0;
 #pragma unroll
     for (int m = 0; m < 8; ++m)
-      c += __popcll(__ballot(lane + 64 * m < nvalid && kh[m] <= mid));
+      if (m < mc) c += __popcll(__ballot(lane + 64 * m < nvalid && kh[m] <= mid));
     if (c >= k) bhi = mid; else blo = mid + 1;
   }
 
@@ -465,7 +533,7 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   }
 
   // ---- prune: candidates that cannot beat the k key-best ones ----
-  const double kk = (double)unordered_f32(blo);
+  const double kk = (double)unordered_f32(bhi);  // >= the k-th smallest key
   double thr;  // a candidate with lower bound >= thr is strictly beyond the top-k
   if (metric == 0) {
     double ub = nzi + kk + E;
@@ -478,38 +546,38 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     thr = (double)next_up(f32_round_up(ub));
   }
   // ---- 3. the plausible candidates, compacted (column list in lds) ----
-  // cl[pos] (pos <= the slot's index e) overwrites packed slot pos / 2 <= e,
-  // which was read in this step or an earlier one, never in a later one
   uint32_t* cl = (uint32_t*)lds;  // <= 512 columns
   uint32_t* xd = cl + 512;        // their exact distances (f32 bits)
+  float* qs = (float*)(xd + 512); // the query row (d4 <= RR_QROW)
   int np = 0;
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
-    const int e = lane + 64 * m;
-    bool pass = false;
-    uint32_t col = 0;
-    if (e < nvalid) {
-      col = cl[2 * e];  // the slot's low half
-      const double kv = (double)unordered_f32(kh[m]);
-      const double lb = metric == 0 ? (nzi + kv - E) * (1.0 - eps_exact) : 1.0 + kv - E - eps_exact;
-      pass = lb < thr;
+    if (m < mc) {
+      bool pass = false;
+      if (lane + 64 * m < nvalid) {
+        const double kv = (double)unordered_f32(kh[m]);
+        const double lb = metric == 0 ? (nzi + kv - E) * (1.0 - eps_exact) : 1.0 + kv - E - eps_exact;
+        pass = lb < thr;
+      }
+      const uint64_t bal = __ballot(pass);
+      const int pos = np + (int)__builtin_amdgcn_mbcnt_hi(
+                               (uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+      if (pass) cl[pos] = kl[m];
+      np += __popcll(bal);
     }
-    const uint64_t bal = __ballot(pass);
-    const int pos = np + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // every lane's read first
-    if (pass) cl[pos] = col;
-    np += __popcll(bal);
   }
   if (np < k) {
     // cannot happen (the k key-best candidates pass)
     if (lane == 0) flag[grow] = 1;
     return;
   }
+  const float* qa = D.xr + (size_t)i * kp;
+  const bool qlds = d4 <= RR_QROW;
+  if (qlds)
+    for (int f = 4 * lane; f < d4; f += 256) *(float4*)(qs + f) = *(const float4*)(qa + f);
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
   // ---- 4. exact distances, one lane per candidate ----
-  const float* qa = D.xr + (size_t)i * kp;
   bool badid = false;
   for (int e = lane; e < np; e += 64) {
     const uint32_t j = cl[e];
@@ -518,7 +586,8 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
       badid = true;  // never produced by the sweep; guard the row read
     } else if ((int64_t)j != i) {
       const double rnb = metric == 1 ? D.rnorm[j] : 0.0;
-      dist = ref_dist(qa, D.xr + (size_t)j * kp, d4, metric, rna, rnb);
+      dist = qlds ? rr_dist(qs, D.xr + (size_t)j * kp, d4, metric, rna, rnb)
+                  : ref_dist(qa, D.xr + (size_t)j * kp, d4, metric, rna, rnb);
     }
     xd[e] = __float_as_uint(dist);
   }
@@ -567,14 +636,14 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
 
 // All rows (rlist == nullptr: rows 0 .. total_rows-1, one per wave), or the
 // *rcount rows listed in rlist (global ids), grid-stride.
-// 4 KB of LDS and at most 80 VGPRs per wave: six waves per SIMD keep
+// 6 KB of LDS and at most 80 VGPRs per wave: six waves per SIMD keep
 // enough rows in flight to cover their dependent memory round trips.
 __global__ __launch_bounds__(256, 6) void rerank_kernel(const DsDesc* __restrict__ ds_arr, int nds,
                                                         int64_t total_rows, int mode,
                                                         uint32_t* __restrict__ flag,
                                                         const uint32_t* __restrict__ rlist,
                                                         const int* __restrict__ rcount) {
-  __shared__ uint64_t cs[4][512];
+  __shared__ uint64_t cs[4][768];
   const int wave = threadIdx.x >> 6;
   const int64_t cnt = rlist ? (int64_t)*rcount : total_rows;
   for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < cnt; p += (int64_t)gridDim.x * 4) {
