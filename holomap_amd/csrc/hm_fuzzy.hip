@@ -2,20 +2,20 @@
 //
 // The reference sorts the directed edges by (row, col), builds the sorted,
 // de-duplicated union of A and A^T coordinates and fuses each pair with the
-// probabilistic t-conorm w = (v + vt) - v*vt in f32. Here the same COO comes
-// from one stable radix sort (hm_sort.hip) and a per-row rank sort:
-//   sort 1 by global column -> every row c's in-edges, sources ascending;
-//   each row's kept out-edges ranked by column within the row (a lane group
-//            per row: a row has at most k of them) -> list A_r, columns
-//            ascending, at outptr[r] = exclusive scan of the kept counts;
-// then, load-balanced (one thread per element, no per-row loops, so rows of
-// very high in-degree cost nothing extra):
-//   * each out-edge r->c looks up its reverse c->r in A_c by binary search
-//     (vt, or 0 if absent);
-//   * the in-edges i->c without a reverse form list B_c (sources ascending);
-//   * row r of the union is A_r merged with B_r (disjoint column sets), and
-//     every element finds its output slot as its rank in its own list plus
-//     the number of smaller columns in the other list ("merge path").
+// probabilistic t-conorm w = (v + vt) - v*vt in f32. Here one stable radix
+// sort of the kept edges by destination (hm_sort.hip) gives every row r its
+// in-edges i -> r, sources ascending, at positions [lo_r, hi_r) of the
+// sorted keys. Then a group of G lanes per row (G a power of two >= k; lane
+// t holds out-edge slot t):
+//   union_count_kernel: lo_r / hi_r by binary search, and the row's union
+//     size |A_r| + |I_r| - |A_r n I_r| (A_r: kept out-edge columns, I_r:
+//     in-edge sources; each out-edge column is compared with the in-edge
+//     sources, G of them at a time through lane shuffles);
+//   (exclusive scan of the sizes -> global row pointer)
+//   union_write_kernel: every element's output slot is its rank in the
+//     sorted union, #{a in A_r: a < x} + #{i in I_r: i < x}
+//     - #{m in A_r n I_r: m < x}, so no sort of the row is needed; it also
+//     writes the dataset-local row pointer.
 // For an entry (r, c): v = A[r][c], vt = A[c][r] (0 if absent), w evaluated
 // with the reference's operand order. With v, vt in [0, 1] and v + vt > 0,
 // w >= max(v, vt) - min(v, vt) and w == 0 would need v == vt == v*vt, so
@@ -25,220 +25,195 @@
 
 namespace hm {
 
-// A lists, G lanes per row (G a power of two <= 64; lane t holds out-edge
-// slots t, t + G, ... of the row). PLACE 0: cnt[r] = kept out-edges of row
-// r. PLACE 1: each kept edge goes to outptr[r] + (number of kept edges of
-// the row with a smaller column): ocol/oval/p2s (column, weight, edge id)
-// and k2s (= r) at that slot.
-template <int G, int PLACE>
-__global__ __launch_bounds__(256) void union_rows_a(
-    const DsDesc* __restrict__ ds, int nds, int64_t total_rows, const uint32_t* __restrict__ key1,
-    const float* __restrict__ wval, uint32_t sentinel, uint32_t* __restrict__ cnt,
-    const uint32_t* __restrict__ outptr, uint32_t* __restrict__ ocol, float* __restrict__ oval,
-    uint32_t* __restrict__ p2s, uint32_t* __restrict__ k2s) {
-  static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "group size");
-  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-  const int t = (int)(threadIdx.x % G);
-  if (r >= total_rows) return;  // whole groups leave together
+__device__ __forceinline__ int ds_of_row(const DsDesc* __restrict__ ds, int nds, int64_t r) {
   int lo = 0, hi = nds - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (ds[mid].row_off <= r) lo = mid; else hi = mid - 1;
   }
-  const DsDesc& D = ds[lo];
-  const int k = D.k;
-  const int64_t e0 = D.edge_off + (r - D.row_off) * k;  // the row's first slot
-  if constexpr (PLACE == 0) {
-    uint32_t n_kept = 0;
-    for (int s = t; s < k; s += G) n_kept += key1[e0 + s] != sentinel ? 1u : 0u;
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) n_kept += __shfl_xor(n_kept, o, G);
-    if (t == 0) cnt[r] = n_kept;
-  } else {
-    auto place = [&](int s, uint32_t c, uint32_t rank) {
-      const uint32_t pos = outptr[r] + rank;
-      ocol[pos] = c;
-      oval[pos] = wval[e0 + s];
-      p2s[pos] = (uint32_t)(e0 + s);
-      k2s[pos] = (uint32_t)r;
-    };
-    if (k <= G) {
-      // one slot per lane; every lane of the group takes part in the shuffles
-      const uint32_t c = t < k ? key1[e0 + t] : sentinel;
-      uint32_t rank = 0;
-      for (int u = 0; u < k; ++u) {
-        const uint32_t cu = __shfl(c, u, G);
-        rank += (cu != sentinel && cu < c) ? 1u : 0u;
-      }
-      if (c != sentinel) place(t, c, rank);
-    } else {
-      for (int s = t; s < k; s += G) {
-        const uint32_t c = key1[e0 + s];
-        if (c == sentinel) continue;
-        uint32_t rank = 0;
-        for (int u = 0; u < k; ++u) {
-          const uint32_t cu = key1[e0 + u];
-          rank += (cu != sentinel && cu < c) ? 1u : 0u;
-        }
-        place(s, c, rank);
-      }
-    }
-  }
+  return lo;
 }
 
-template __global__ void union_rows_a<16, 0>(const DsDesc*, int, int64_t, const uint32_t*,
-                                             const float*, uint32_t, uint32_t*, const uint32_t*,
-                                             uint32_t*, float*, uint32_t*, uint32_t*);
-template __global__ void union_rows_a<16, 1>(const DsDesc*, int, int64_t, const uint32_t*,
-                                             const float*, uint32_t, uint32_t*, const uint32_t*,
-                                             uint32_t*, float*, uint32_t*, uint32_t*);
-template __global__ void union_rows_a<32, 0>(const DsDesc*, int, int64_t, const uint32_t*,
-                                             const float*, uint32_t, uint32_t*, const uint32_t*,
-                                             uint32_t*, float*, uint32_t*, uint32_t*);
-template __global__ void union_rows_a<32, 1>(const DsDesc*, int, int64_t, const uint32_t*,
-                                             const float*, uint32_t, uint32_t*, const uint32_t*,
-                                             uint32_t*, float*, uint32_t*, uint32_t*);
-template __global__ void union_rows_a<64, 0>(const DsDesc*, int, int64_t, const uint32_t*,
-                                             const float*, uint32_t, uint32_t*, const uint32_t*,
-                                             uint32_t*, float*, uint32_t*, uint32_t*);
-template __global__ void union_rows_a<64, 1>(const DsDesc*, int, int64_t, const uint32_t*,
-                                             const float*, uint32_t, uint32_t*, const uint32_t*,
-                                             uint32_t*, float*, uint32_t*, uint32_t*);
-
-__device__ __forceinline__ uint32_t lb_u32(const uint32_t* __restrict__ a, uint32_t lo, uint32_t hi,
-                                           uint32_t v) {
+// first position p in [0, n) with a[p] >= v (a sorted)
+__device__ __forceinline__ int64_t lower_bound_u32(const uint32_t* __restrict__ a, int64_t n,
+                                                   uint32_t v) {
+  int64_t lo = 0, hi = n;
   while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
+    const int64_t mid = (lo + hi) >> 1;
     if (a[mid] < v) lo = mid + 1; else hi = mid;
   }
   return lo;
 }
 
-// Reverse lookup of every out-edge r->c in A_c; marks mutual edges by id.
-__global__ void union_mutual(const uint32_t* __restrict__ k2s, const uint32_t* __restrict__ p2s,
-                             const uint32_t* __restrict__ ocol, const float* __restrict__ oval,
-                             const uint32_t* __restrict__ outptr, int64_t total_rows,
-                             float* __restrict__ orev, uint32_t* __restrict__ mflag) {
-  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= (int64_t)outptr[total_rows]) return;
-  const uint32_t r = k2s[a], c = ocol[a];
-  const uint32_t lo = outptr[c], hi = outptr[c + 1];
-  const uint32_t f = lb_u32(ocol, lo, hi, r);
-  const bool found = f < hi && ocol[f] == r;
-  orev[a] = found ? oval[f] : 0.0f;
-  mflag[p2s[a]] = found ? 1u : 0u;
+// Value of lane U of the calling lane's G-lane group. A 16-lane group is
+// one DPP row: a row_newbcast move (VALU); wider groups use an LDS permute.
+template <int G, int U>
+__device__ __forceinline__ uint32_t grp_get(uint32_t x) {
+  if constexpr (G == 16)
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x150 + U, 0xf, 0xf, false);
+  else
+    return (uint32_t)__shfl((int)x, U, G);
 }
-
-// nm[p] = 1 for a valid in-edge (sort-1 position p) without a reverse edge.
-__global__ void union_nonmutual(const uint32_t* __restrict__ p1s, const uint32_t* __restrict__ mflag,
-                                const uint32_t* __restrict__ inptr, int64_t total_rows,
-                                int64_t total_edges, uint32_t* __restrict__ nm) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p > total_edges) return;
-  nm[p] = (p < (int64_t)inptr[total_rows] && !mflag[p1s[p]]) ? 1u : 0u;
-}
-
-// B lists (order kept) and their row pointers.
-__global__ void union_gather_b(const uint32_t* __restrict__ k1s, const uint32_t* __restrict__ p1s,
-                               const uint32_t* __restrict__ srcrow, const float* __restrict__ wval,
-                               const uint32_t* __restrict__ nm, const uint32_t* __restrict__ bpos,
-                               const uint32_t* __restrict__ inptr, int64_t total_rows,
-                               uint32_t* __restrict__ bsrc, uint32_t* __restrict__ brow,
-                               float* __restrict__ bval) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (int64_t)inptr[total_rows] || !nm[p]) return;
-  const uint32_t b = bpos[p], e = p1s[p];
-  bsrc[b] = srcrow[e];
-  brow[b] = k1s[p];
-  bval[b] = wval[e];
-}
-
-// bptr[c] = first B slot of row c; cnt[r] = |A_r| + |B_r|.
-__global__ void union_rowcount(const uint32_t* __restrict__ inptr, const uint32_t* __restrict__ outptr,
-                               const uint32_t* __restrict__ bpos, int64_t total_rows,
-                               uint32_t* __restrict__ bptr, uint64_t* __restrict__ cnt) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r > total_rows) return;
-  const uint32_t b0 = bpos[inptr[r]];
-  bptr[r] = b0;
-  if (r < total_rows) {
-    const uint32_t b1 = bpos[inptr[r + 1]];
-    cnt[r] = (uint64_t)(outptr[r + 1] - outptr[r]) + (uint64_t)(b1 - b0);
+// b(x_u, y_u) for u = 0 .. G-1 in order (unrolled, so U is a constant).
+template <int G, int U = 0, class B>
+__device__ __forceinline__ void grp_for(uint32_t x, uint32_t y, B& b) {
+  if constexpr (U < G) {
+    b(grp_get<G, U>(x), grp_get<G, U>(y));
+    grp_for<G, U + 1>(x, y, b);
   }
 }
 
-__device__ __forceinline__ int64_t row_offset_of(const DsDesc* __restrict__ ds, int nds, int64_t r) {
-  int lo = 0, hi = nds - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (ds[mid].row_off <= r) lo = mid; else hi = mid - 1;
+// Loop bodies over a group's broadcast values.
+struct FindSrc {  // is the out-edge column an in-edge source
+  uint32_t ca;
+  bool found;
+  __device__ void operator()(uint32_t su, uint32_t) { found |= su == ca; }
+};
+struct ScanIn {  // sources below the column; the reverse weight
+  uint32_t ca, less;
+  bool found;
+  float vt;
+  __device__ void operator()(uint32_t su, uint32_t wu) {
+    less += su < ca ? 1u : 0u;
+    if (su == ca) { found = true; vt = __uint_as_float(wu); }
   }
-  return ds[lo].row_off;
-}
-
-// A elements: entry (r, c) with v = A[r][c], vt = A[c][r].
-__global__ void union_scatter_a(const DsDesc* __restrict__ ds, int nds,
-                                const uint32_t* __restrict__ k2s, const uint32_t* __restrict__ ocol,
-                                const float* __restrict__ oval, const float* __restrict__ orev,
-                                const uint32_t* __restrict__ outptr, const uint32_t* __restrict__ bsrc,
-                                const uint32_t* __restrict__ bptr,
-                                const uint64_t* __restrict__ indptr_g, int64_t total_rows,
-                                uint32_t* __restrict__ rows, uint32_t* __restrict__ cols,
-                                float* __restrict__ vals) {
-  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= (int64_t)outptr[total_rows]) return;
-  const uint32_t r = k2s[a], c = ocol[a];
-  const uint32_t nb = lb_u32(bsrc, bptr[r], bptr[r + 1], c) - bptr[r];
-  const uint64_t pos = indptr_g[r] + (uint64_t)(a - outptr[r]) + nb;
-  const float v = oval[a], vt = orev[a];
-  const int64_t roff = row_offset_of(ds, nds, r);
-  rows[pos] = (uint32_t)(r - roff);
-  cols[pos] = (uint32_t)((int64_t)c - roff);
-  vals[pos] = (v + vt) - v * vt;  // f32, the reference's operand order
-}
-
-// B elements: entry (c, i) for an edge i->c without reverse: v = 0, vt = w.
-__global__ void union_scatter_b(const DsDesc* __restrict__ ds, int nds,
-                                const uint32_t* __restrict__ brow, const uint32_t* __restrict__ bsrc,
-                                const float* __restrict__ bval, const uint32_t* __restrict__ bptr,
-                                const uint32_t* __restrict__ ocol, const uint32_t* __restrict__ outptr,
-                                const uint64_t* __restrict__ indptr_g, int64_t total_rows,
-                                uint32_t* __restrict__ rows, uint32_t* __restrict__ cols,
-                                float* __restrict__ vals) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= (int64_t)bptr[total_rows]) return;
-  const uint32_t c = brow[b], i = bsrc[b];
-  const uint32_t na = lb_u32(ocol, outptr[c], outptr[c + 1], i) - outptr[c];
-  const uint64_t pos = indptr_g[c] + (uint64_t)(b - bptr[c]) + na;
-  const float v = 0.0f, vt = bval[b];
-  const int64_t roff = row_offset_of(ds, nds, c);
-  rows[pos] = (uint32_t)(c - roff);
-  cols[pos] = (uint32_t)((int64_t)i - roff);
-  vals[pos] = (v + vt) - v * vt;
-}
-
-// Per-dataset CSR row pointer (dataset-local) from the global offsets.
-__global__ void local_indptr_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
-                                    const uint64_t* __restrict__ indptr_g,
-                                    const uint64_t* __restrict__ total_nnz,
-                                    const int64_t* __restrict__ iptr_off,
-                                    uint64_t* __restrict__ indptr_l) {
-  // thread per global row slot g in [0, total_rows + nds): each dataset owns
-  // n+1 entries starting at iptr_off[ds] = row_off + ds
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total_rows + nds) return;
-  int lo = 0, hi = nds - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (iptr_off[mid] <= t) lo = mid; else hi = mid - 1;
+};
+struct ScanOut {  // out-edge columns (and mutual ones) below x; x among them
+  uint32_t x, less, mut;
+  bool in;
+  __device__ void operator()(uint32_t cu, uint32_t fu) {
+    less += cu < x ? 1u : 0u;
+    mut += (fu && cu < x) ? 1u : 0u;
+    in |= cu == x;
   }
-  const DsDesc& D = ds[lo];
-  const int64_t i = t - iptr_off[lo];  // 0..n
-  const int64_t g = D.row_off + i;
-  const uint64_t base = indptr_g[D.row_off];
-  const uint64_t at = g < total_rows ? indptr_g[g] : *total_nnz;
-  indptr_l[t] = at - base;
+};
+
+// inptr[r] = lo_r (and inptr[R] = hi of the last row), cnt[r] = union size.
+template <int G>
+__global__ __launch_bounds__(256) void union_count_kernel(
+    const DsDesc* __restrict__ ds, int nds, int64_t total_rows, int64_t total_edges,
+    const uint32_t* __restrict__ key1, const uint32_t* __restrict__ k1s,
+    const uint32_t* __restrict__ p1s, const uint32_t* __restrict__ srcrow, uint32_t sentinel,
+    uint32_t* __restrict__ inptr, uint64_t* __restrict__ cnt) {
+  static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "group size");
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int t = (int)(threadIdx.x % G);
+  if (r >= total_rows) return;  // whole groups leave together
+  const DsDesc& D = ds[ds_of_row(ds, nds, r)];
+  const int k = D.k;  // <= G
+  const int64_t e0 = D.edge_off + (r - D.row_off) * k;
+  const uint32_t ca = t < k ? key1[e0 + t] : sentinel;
+  // lane 0 finds lo, lane 1 hi; the group reads both
+  const int64_t b = t == 1 ? lower_bound_u32(k1s, total_edges, (uint32_t)(r + 1))
+                           : lower_bound_u32(k1s, total_edges, (uint32_t)r);
+  const int64_t lo = (uint32_t)__shfl((int)(uint32_t)b, 0, G);
+  const int64_t hi = (uint32_t)__shfl((int)(uint32_t)b, 1, G);
+  FindSrc f{ca, false};  // ca is also an in-edge source (a mutual pair)
+  for (int64_t p0 = lo; p0 < hi; p0 += G) {
+    const int64_t p = p0 + t;
+    const uint32_t src = p < hi ? srcrow[p1s[p]] : 0xffffffffu;
+    grp_for<G>(src, 0u, f);
+  }
+  const bool found = f.found;
+  uint32_t na = ca != sentinel ? 1u : 0u, nm = found ? 1u : 0u;
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) {
+    na += __shfl_xor(na, o, G);
+    nm += __shfl_xor(nm, o, G);
+  }
+  if (t == 0) {
+    inptr[r] = (uint32_t)lo;
+    if (r == total_rows - 1) inptr[total_rows] = (uint32_t)hi;
+    cnt[r] = (uint64_t)na + (uint64_t)(hi - lo) - nm;
+  }
 }
+
+// The union's entries of row r at indptr_g[r] + their rank; indptr_l.
+template <int G>
+__global__ __launch_bounds__(256) void union_write_kernel(
+    const DsDesc* __restrict__ ds, int nds, int64_t total_rows, const uint32_t* __restrict__ key1,
+    const float* __restrict__ wval, const uint32_t* __restrict__ p1s,
+    const uint32_t* __restrict__ srcrow, const uint32_t* __restrict__ inptr,
+    const uint64_t* __restrict__ indptr_g, const uint64_t* __restrict__ cnt,
+    const int64_t* __restrict__ iptr_off, uint32_t sentinel, uint32_t* __restrict__ rows,
+    uint32_t* __restrict__ cols, float* __restrict__ vals, uint64_t* __restrict__ indptr_l) {
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int t = (int)(threadIdx.x % G);
+  if (r >= total_rows) return;
+  const int di = ds_of_row(ds, nds, r);
+  const DsDesc& D = ds[di];
+  const int k = D.k;
+  const int64_t roff = D.row_off;
+  const int64_t e0 = D.edge_off + (r - roff) * k;
+  const uint32_t ca = t < k ? key1[e0 + t] : sentinel;
+  const float va = t < k ? wval[e0 + t] : 0.0f;
+  const int64_t lo = inptr[r], hi = inptr[r + 1];
+  // per out-edge: in-edge sources below its column, and its reverse weight
+  ScanIn si{ca, 0u, false, 0.0f};
+  for (int64_t p0 = lo; p0 < hi; p0 += G) {
+    const int64_t p = p0 + t;
+    uint32_t src = 0xffffffffu;
+    float w = 0.0f;
+    if (p < hi) {
+      const uint32_t e = p1s[p];
+      src = srcrow[e];
+      w = wval[e];
+    }
+    grp_for<G>(src, __float_as_uint(w), si);
+  }
+  // rank among the row's out-edges, and mutual columns below
+  ScanOut so{ca, 0u, 0u, false};
+  grp_for<G>(ca, si.found ? 1u : 0u, so);
+  const uint64_t base = indptr_g[r];
+  if (ca != sentinel) {
+    const uint64_t pos = base + so.less + si.less - so.mut;
+    const float vt = si.vt;
+    rows[pos] = (uint32_t)(r - roff);
+    cols[pos] = (uint32_t)((int64_t)ca - roff);
+    vals[pos] = (va + vt) - va * vt;  // f32, the reference's operand order
+  }
+  // in-edges i -> r without the out-edge r -> i: entry (r, i), v = 0, vt = w
+  for (int64_t p0 = lo; p0 < hi; p0 += G) {
+    const int64_t p = p0 + t;
+    uint32_t src = 0xffffffffu;
+    float w = 0.0f;
+    if (p < hi) {
+      const uint32_t e = p1s[p];
+      src = srcrow[e];
+      w = wval[e];
+    }
+    ScanOut sa{src, 0u, 0u, false};
+    grp_for<G>(ca, si.found ? 1u : 0u, sa);
+    if (p < hi && !sa.in) {
+      const uint64_t pos = base + sa.less + (uint64_t)(p - lo) - sa.mut;
+      const float v = 0.0f;
+      rows[pos] = (uint32_t)(r - roff);
+      cols[pos] = (uint32_t)((int64_t)src - roff);
+      vals[pos] = (v + w) - v * w;
+    }
+  }
+  if (t == 0) {
+    // dataset di owns indptr_l[iptr_off[di] .. + n]
+    const int64_t i = r - roff;
+    const uint64_t b0 = indptr_g[roff];
+    indptr_l[iptr_off[di] + i] = base - b0;
+    if (i == D.n - 1) indptr_l[iptr_off[di] + D.n] = base + cnt[r] - b0;
+  }
+}
+
+#define HM_UNION_INST(G)                                                                        \
+  template __global__ void union_count_kernel<G>(const DsDesc*, int, int64_t, int64_t,          \
+                                                 const uint32_t*, const uint32_t*,               \
+                                                 const uint32_t*, const uint32_t*, uint32_t,     \
+                                                 uint32_t*, uint64_t*);                          \
+  template __global__ void union_write_kernel<G>(                                               \
+      const DsDesc*, int, int64_t, const uint32_t*, const float*, const uint32_t*,              \
+      const uint32_t*, const uint32_t*, const uint64_t*, const uint64_t*, const int64_t*,       \
+      uint32_t, uint32_t*, uint32_t*, float*, uint64_t*);
+HM_UNION_INST(16)
+HM_UNION_INST(32)
+HM_UNION_INST(64)
+#undef HM_UNION_INST
 
 // CSR values widened to f64 (src/sparse.rs:32).
 __global__ void widen_kernel(const float* __restrict__ v, int64_t n, double* __restrict__ out) {

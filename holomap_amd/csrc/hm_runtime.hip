@@ -60,27 +60,15 @@ __global__ void radix_scatter(const uint32_t*, const uint32_t*, int64_t, int, in
 __global__ void lower_bounds(const uint32_t*, int64_t, int64_t, uint32_t*);
 __global__ void compact_kernel(const uint32_t*, const uint32_t*, int64_t, uint32_t*, int*);
 __global__ void fill_u32(uint32_t*, int64_t, uint32_t);
-template <int G, int PLACE>
-__global__ void union_rows_a(const DsDesc*, int, int64_t, const uint32_t*, const float*, uint32_t,
-                             uint32_t*, const uint32_t*, uint32_t*, float*, uint32_t*, uint32_t*);
-
-__global__ void union_mutual(const uint32_t*, const uint32_t*, const uint32_t*, const float*,
-                             const uint32_t*, int64_t, float*, uint32_t*);
-__global__ void union_nonmutual(const uint32_t*, const uint32_t*, const uint32_t*, int64_t, int64_t,
-                                uint32_t*);
-__global__ void union_gather_b(const uint32_t*, const uint32_t*, const uint32_t*, const float*,
-                               const uint32_t*, const uint32_t*, const uint32_t*, int64_t, uint32_t*,
-                               uint32_t*, float*);
-__global__ void union_rowcount(const uint32_t*, const uint32_t*, const uint32_t*, int64_t, uint32_t*,
-                               uint64_t*);
-__global__ void union_scatter_a(const DsDesc*, int, const uint32_t*, const uint32_t*, const float*,
-                                const float*, const uint32_t*, const uint32_t*, const uint32_t*,
-                                const uint64_t*, int64_t, uint32_t*, uint32_t*, float*);
-__global__ void union_scatter_b(const DsDesc*, int, const uint32_t*, const uint32_t*, const float*,
-                                const uint32_t*, const uint32_t*, const uint32_t*, const uint64_t*,
-                                int64_t, uint32_t*, uint32_t*, float*);
-__global__ void local_indptr_kernel(const DsDesc*, int, int64_t, const uint64_t*, const uint64_t*,
-                                    const int64_t*, uint64_t*);
+template <int G>
+__global__ void union_count_kernel(const DsDesc*, int, int64_t, int64_t, const uint32_t*,
+                                   const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
+                                   uint32_t*, uint64_t*);
+template <int G>
+__global__ void union_write_kernel(const DsDesc*, int, int64_t, const uint32_t*, const float*,
+                                   const uint32_t*, const uint32_t*, const uint32_t*,
+                                   const uint64_t*, const uint64_t*, const int64_t*, uint32_t,
+                                   uint32_t*, uint32_t*, float*, uint64_t*);
 }  // namespace hm
 
 using namespace hm;
@@ -153,12 +141,8 @@ struct hm_job {
   uint32_t *ccnt = nullptr;
   uint32_t *knn_idx = nullptr, *flag = nullptr, *flagpos = nullptr, *list = nullptr,
            *zany = nullptr, *key1 = nullptr, *pay1 = nullptr, *srcrow = nullptr, *k1s = nullptr,
-           *p1s = nullptr, *cntu32 = nullptr, *k2s = nullptr, *p2s = nullptr, *tmpk = nullptr,
-           *tmpv = nullptr, *inptr = nullptr, *outptr = nullptr, *rows = nullptr, *cols = nullptr,
-           *table = nullptr, *toffs = nullptr, *sc32 = nullptr, *sc32t = nullptr,
-           *ocol = nullptr, *mflag = nullptr, *nmf = nullptr, *bpos = nullptr, *bsrc = nullptr,
-           *brow = nullptr, *bptr = nullptr;
-  float *oval = nullptr, *orev = nullptr, *bval = nullptr;
+           *p1s = nullptr, *tmpk = nullptr, *tmpv = nullptr, *inptr = nullptr, *rows = nullptr,
+           *cols = nullptr, *table = nullptr, *toffs = nullptr, *sc32 = nullptr, *sc32t = nullptr;
   int *count = nullptr, *nitems_fb = nullptr, *xs_fb = nullptr;
   WorkItem *items_l1 = nullptr, *items_fb = nullptr, *items_l2 = nullptr;
   int* nitems_l2 = nullptr;
@@ -312,7 +296,9 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
   b += E * 8;                                        // knn idx + dist
   b += R * 20 + nds * 12;                            // sigma, rho, sigma64; flags, targets
   if (stages & HM_STAGE_FUZZY) {
-    b += E * 20 * 4 + (R + 1) * 12 + R * 16 + 16 + 2 * E * 12 + (R + nds) * 8;
+    // key1 pay1 srcrow wval k1s p1s tmpk tmpv; inptr + indptr_g; cntu;
+    // total; rows cols vals; indptr_l; iptr_off
+    b += E * 8 * 4 + (R + 1) * 12 + R * 8 + 16 + 2 * E * 12 + (R + nds) * 8 + nds * 8;
     b += 2 * 256 * (cdiv(E, kRadixTile) + 1) * 4;
   }
   b += 2 * (cdiv(std::max<int64_t>(256 * (cdiv(E, kRadixTile) + 1), R + 1), kScanTile) + 2) * 8;
@@ -582,54 +568,27 @@ hm_status enqueue_pipeline(hm_job* J) {
                        J->wval, J->srcrow);
     HM_LAUNCH(ctx);
     HM_TRY(radix_sort(J, J->key1, J->pay1, E, bits, J->k1s, J->p1s));
-    hipLaunchKernelGGL(lower_bounds, dim3(grid_for(R + 1, 256)), dim3(256), 0, J->stream,
-                       (const uint32_t*)J->k1s, E, R, J->inptr);
-    // A lists: every row's kept out-edges, columns ascending
-    const int GA = J->kmax <= 16 ? 16 : J->kmax <= 32 ? 32 : 64;
-#define HM_ROWS_A(GS, PL)                                                                      \
-  hipLaunchKernelGGL((union_rows_a<GS, PL>), dim3(grid_for(R * GS, 256)), dim3(256), 0,         \
-                     J->stream, J->d_desc, nds, R, (const uint32_t*)J->key1,                   \
-                     (const float*)J->wval, sent, J->cntu32, (const uint32_t*)J->outptr, J->ocol, \
-                     J->oval, J->p2s, J->k2s)
-    if (GA == 16) HM_ROWS_A(16, 0); else if (GA == 32) HM_ROWS_A(32, 0); else HM_ROWS_A(64, 0);
-    HM_LAUNCH(ctx);
-    HM_TRY(scan_excl<uint32_t>(J, J->cntu32, R, J->outptr, J->outptr + R, J->sc32));
-    if (GA == 16) HM_ROWS_A(16, 1); else if (GA == 32) HM_ROWS_A(32, 1); else HM_ROWS_A(64, 1);
-#undef HM_ROWS_A
-    // load-balanced union: reverse lookups, non-mutual in-edges, merge path
-    const unsigned ge = grid_for(E, 256);
-    hipLaunchKernelGGL(union_mutual, dim3(ge), dim3(256), 0, J->stream, (const uint32_t*)J->k2s,
-                       (const uint32_t*)J->p2s, (const uint32_t*)J->ocol, (const float*)J->oval,
-                       (const uint32_t*)J->outptr, R, J->orev, J->mflag);
-    hipLaunchKernelGGL(union_nonmutual, dim3(grid_for(E + 1, 256)), dim3(256), 0, J->stream,
-                       (const uint32_t*)J->p1s, (const uint32_t*)J->mflag,
-                       (const uint32_t*)J->inptr, R, E, J->nmf);
-    HM_LAUNCH(ctx);
-    HM_TRY(scan_excl<uint32_t>(J, J->nmf, E + 1, J->bpos, (uint32_t*)nullptr, J->sc32));
-    hipLaunchKernelGGL(union_gather_b, dim3(ge), dim3(256), 0, J->stream, (const uint32_t*)J->k1s,
-                       (const uint32_t*)J->p1s, (const uint32_t*)J->srcrow, (const float*)J->wval,
-                       (const uint32_t*)J->nmf, (const uint32_t*)J->bpos,
-                       (const uint32_t*)J->inptr, R, J->bsrc, J->brow, J->bval);
-    hipLaunchKernelGGL(union_rowcount, dim3(grid_for(R + 1, 256)), dim3(256), 0, J->stream,
-                       (const uint32_t*)J->inptr, (const uint32_t*)J->outptr,
-                       (const uint32_t*)J->bpos, R, J->bptr, J->cntu);
+    // a group of G >= k lanes per row: union sizes, their scan, the entries
+    const int GU = J->kmax <= 16 ? 16 : J->kmax <= 32 ? 32 : 64;
+#define HM_UNION_COUNT(GS)                                                                     \
+  hipLaunchKernelGGL((union_count_kernel<GS>), dim3(grid_for(R * GS, 256)), dim3(256), 0,       \
+                     J->stream, J->d_desc, nds, R, E, (const uint32_t*)J->key1,                 \
+                     (const uint32_t*)J->k1s, (const uint32_t*)J->p1s,                          \
+                     (const uint32_t*)J->srcrow, sent, J->inptr, J->cntu)
+#define HM_UNION_WRITE(GS)                                                                     \
+  hipLaunchKernelGGL((union_write_kernel<GS>), dim3(grid_for(R * GS, 256)), dim3(256), 0,       \
+                     J->stream, J->d_desc, nds, R, (const uint32_t*)J->key1,                    \
+                     (const float*)J->wval, (const uint32_t*)J->p1s,                            \
+                     (const uint32_t*)J->srcrow, (const uint32_t*)J->inptr,                     \
+                     (const uint64_t*)J->indptr_g, (const uint64_t*)J->cntu,                    \
+                     (const int64_t*)J->iptr_off, sent, J->rows, J->cols, J->vals, J->indptr_l)
+    if (GU == 16) HM_UNION_COUNT(16); else if (GU == 32) HM_UNION_COUNT(32); else HM_UNION_COUNT(64);
     HM_LAUNCH(ctx);
     HM_TRY(scan_excl<uint64_t>(J, J->cntu, R, J->indptr_g, J->total_nnz, J->sc64));
-    hipLaunchKernelGGL(union_scatter_a, dim3(ge), dim3(256), 0, J->stream, J->d_desc, nds,
-                       (const uint32_t*)J->k2s, (const uint32_t*)J->ocol, (const float*)J->oval,
-                       (const float*)J->orev, (const uint32_t*)J->outptr, (const uint32_t*)J->bsrc,
-                       (const uint32_t*)J->bptr, (const uint64_t*)J->indptr_g, R, J->rows, J->cols,
-                       J->vals);
-    hipLaunchKernelGGL(union_scatter_b, dim3(ge), dim3(256), 0, J->stream, J->d_desc, nds,
-                       (const uint32_t*)J->brow, (const uint32_t*)J->bsrc, (const float*)J->bval,
-                       (const uint32_t*)J->bptr, (const uint32_t*)J->ocol,
-                       (const uint32_t*)J->outptr, (const uint64_t*)J->indptr_g, R, J->rows,
-                       J->cols, J->vals);
+    if (GU == 16) HM_UNION_WRITE(16); else if (GU == 32) HM_UNION_WRITE(32); else HM_UNION_WRITE(64);
     HM_LAUNCH(ctx);
-    hipLaunchKernelGGL(local_indptr_kernel, dim3(grid_for(R + nds, 256)), dim3(256), 0, J->stream,
-                       J->d_desc, nds, R, (const uint64_t*)J->indptr_g,
-                       (const uint64_t*)J->total_nnz, (const int64_t*)J->iptr_off, J->indptr_l);
-    HM_LAUNCH(ctx);
+#undef HM_UNION_COUNT
+#undef HM_UNION_WRITE
   }
   mark(J, 6);
   return HM_OK;
@@ -907,16 +866,13 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
   const int64_t ntr = cdiv(E, kRadixTile) + 1;
   if (stages & HM_STAGE_FUZZY) {
     HM_A(J->key1, E); HM_A(J->pay1, E); HM_A(J->srcrow, E); HM_A(J->wval, E);
-    HM_A(J->k1s, E); HM_A(J->p1s, E); HM_A(J->cntu32, R); HM_A(J->k2s, E); HM_A(J->p2s, E);
+    HM_A(J->k1s, E); HM_A(J->p1s, E);
     HM_A(J->tmpk, E); HM_A(J->tmpv, E);
-    HM_A(J->inptr, R + 1); HM_A(J->outptr, R + 1);
+    HM_A(J->inptr, R + 1);
     HM_A(J->cntu, R); HM_A(J->indptr_g, R + 1); HM_A(J->total_nnz, 1);
     HM_A(J->rows, 2 * E); HM_A(J->cols, 2 * E); HM_A(J->vals, 2 * E);
     HM_A(J->indptr_l, R + nds); HM_A(J->iptr_off, nds);
     HM_A(J->table, 256 * ntr); HM_A(J->toffs, 256 * ntr);
-    HM_A(J->ocol, E); HM_A(J->oval, E); HM_A(J->orev, E); HM_A(J->mflag, E);
-    HM_A(J->nmf, E + 1); HM_A(J->bpos, E + 1); HM_A(J->bsrc, E); HM_A(J->brow, E);
-    HM_A(J->bval, E); HM_A(J->bptr, R + 1);
   }
   const int64_t scan_max = std::max<int64_t>(std::max<int64_t>(256 * ntr, R + 1), E + 1);
   HM_A(J->sc32, cdiv(scan_max, kScanTile) + 2);
