@@ -5,7 +5,7 @@
 //     sum_f x_f^2 (src/metric.rs:61-66 keeps na/nb as such folds), giving the
 //     cosine denominator factor sqrt(na) bit-exactly and the Euclidean key
 //     bias ||x||^2 rounded to f32;
-//   * prep_kernel     (wave per row): a zero-padded f32 copy (n_pad x kp),
+//   * prep_kernel     (16 lanes per row): a zero-padded f32 copy (n_pad x kp),
 //     the candidate-selection operand (euclid: the copy itself; cosine: the
 //     row scaled to unit length), its bf16 (RNE) image, and the statistics
 //     the certification bound needs (representation error norms);
@@ -43,15 +43,13 @@ __device__ __forceinline__ double dpp_f64(double x) {
       (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xf, false);
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
-// Inclusive lane-order scan with DPP (row shifts 1, 2, 4, 8, then the row
-// broadcasts of lanes 15 and 31); lane 63 ends with the wave's sum.
-__device__ __forceinline__ double wave_sum63(double x) {
+// Inclusive scan of each 16-lane DPP row (row shifts 1, 2, 4, 8); lane 15
+// of the row ends with the row's sum.
+__device__ __forceinline__ double row_sum15(double x) {
   x += dpp_f64<0x111, 0xf>(x);
   x += dpp_f64<0x112, 0xf>(x);
   x += dpp_f64<0x114, 0xf>(x);
   x += dpp_f64<0x118, 0xf>(x);
-  x += dpp_f64<0x142, 0xa>(x);
-  x += dpp_f64<0x143, 0xc>(x);
   return x;
 }
 
@@ -96,17 +94,17 @@ __device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int nds,
                                          const float* __restrict__ nz_g, int mode_bits,
                                          int lane);
 
-// Wave per (dataset, padded row), grid-stride. Writes xr, op32 (cosine),
-// op16 (+ op16lo), colbias and rstat. mode_bits: 1 = produce op16 (bf16
-// sweep), 2 = bf16x3 stats.
+// 16 lanes (one DPP row) per (dataset, padded row), four rows per wave, so
+// a wave overlaps the memory round trips of four rows. Writes xr, op32
+// (cosine), op16 (+ op16lo), colbias and rstat. mode_bits: 1 = produce op16
+// (bf16 sweep), 2 = bf16x3 stats.
 __global__ __launch_bounds__(256) void prep_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_pad_rows,
     const int64_t* __restrict__ pad_off /* nds+1 prefix of n_pad */,
     const float* __restrict__ nz_g, int mode_bits) {
-  const int lane = threadIdx.x & 63;
-  for (int64_t pr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); pr < total_pad_rows;
-       pr += (int64_t)gridDim.x * 4)
-    prep_row(ds, nds, pr, pad_off, nz_g, mode_bits, lane);
+  const int64_t pr = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  if (pr >= total_pad_rows) return;  // whole rows leave together
+  prep_row(ds, nds, pr, pad_off, nz_g, mode_bits, (int)(threadIdx.x & 15));
 }
 
 // One padded row by one wave (prep_kernel's body).
@@ -127,7 +125,7 @@ __device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int nds,
   float* op32 = const_cast<float*>(D.op32) + i * kp;
   uint16_t* op16 = D.op16 ? const_cast<uint16_t*>(D.op16) + i * D.kp16 : nullptr;
   uint16_t* op16lo = D.op16lo ? const_cast<uint16_t*>(D.op16lo) + i * D.kp16 : nullptr;
-  for (int f = kp + lane; f < D.kp16; f += 64) {  // bf16 rows are padded further
+  for (int f = kp + lane; f < D.kp16; f += 16) {  // bf16 rows are padded further
     if (op16) op16[f] = 0;
     if (op16lo) op16lo[f] = 0;
   }
@@ -142,7 +140,7 @@ __device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int nds,
   const bool zero_row = real && rn == 0.0;
   // statistics accumulated per lane, reduced by a fixed butterfly
   double s_xn = 0, s_en = 0, s_xo = 0, s_hn = 0, s_ln = 0, s_hln = 0, s_rn = 0;
-  for (int f = lane; f < kp; f += 64) {
+  for (int f = lane; f < kp; f += 16) {
     float v = (real && f < d) ? D.x[i * d + f] : 0.0f;
     xr[f] = v;
     float w = v;
@@ -182,18 +180,18 @@ __device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int nds,
       s_rn += r * r;
     }
   }
-  // wave sums in lane 63 (the statistics feed error bounds, which hold for
-  // any summation order)
-  s_xn = wave_sum63(s_xn);
-  s_en = wave_sum63(s_en);
-  s_xo = wave_sum63(s_xo);
+  // the row's sums in its lane 15 (the statistics feed error bounds, which
+  // hold for any summation order)
+  s_xn = row_sum15(s_xn);
+  s_en = row_sum15(s_en);
+  s_xo = row_sum15(s_xo);
   if (mode_bits & 2) {  // the bf16x3 statistics (zero otherwise)
-    s_hn = wave_sum63(s_hn);
-    s_ln = wave_sum63(s_ln);
-    s_hln = wave_sum63(s_hln);
-    s_rn = wave_sum63(s_rn);
+    s_hn = row_sum15(s_hn);
+    s_ln = row_sum15(s_ln);
+    s_hln = row_sum15(s_hln);
+    s_rn = row_sum15(s_rn);
   }
-  if (lane == 63) {
+  if (lane == 15) {
     float* cb = const_cast<float*>(D.colbias);
     // the sweep's accumulator start bias_j / s (s = -2 euclidean, -1 cosine;
     // exact): key bias ||x_j||^2, or -inf for a zero row under cosine

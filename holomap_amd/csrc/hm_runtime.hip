@@ -469,8 +469,7 @@ hm_status enqueue_pipeline(hm_job* J) {
     hipLaunchKernelGGL(rownorm_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, J->d_desc,
                        nds, R, J->nz);
     const int mode_bits = (J->mode == 0 ? 1 : 0) | 2;
-    hipLaunchKernelGGL(prep_kernel, dim3(std::min<int64_t>(grid_for(J->total_pad, 4), 8 * (int64_t)cus)),
-                       dim3(256), 0, J->stream,
+    hipLaunchKernelGGL(prep_kernel, dim3(grid_for(J->total_pad, 16)), dim3(256), 0, J->stream,
                        J->d_desc, nds, J->total_pad, (const int64_t*)J->pad_off,
                        (const float*)J->nz, mode_bits);
     hipLaunchKernelGGL(statmax_part_kernel, dim3(nds, kStatParts), dim3(256), 0, J->stream,
