@@ -48,7 +48,7 @@ __global__ void directed_kernel(const DsDesc*, int, int64_t, const float*, const
 __global__ void expf_kernel(const float*, float*, int64_t);
 template <typename T>
 __global__ void scan_tile_sums(const T*, int64_t, T*);
-template <typename T, int NT>
+template <typename T, int NT, int MS>
 __global__ void scan_small(const T*, int64_t, T*, T*, uint32_t*, int*);
 template <typename T>
 __global__ void scan_sums(T*, int64_t, T*);
@@ -85,12 +85,14 @@ struct hm_ctx {
 namespace {
 
 constexpr int kScanTile = 4096;
-// Arrays scanned by one workgroup (scan_small: NT threads x 64 elements
-// held per lane): 1024 threads for 32-bit values, 512 for 64-bit ones.
+// Arrays scanned by one workgroup (scan_small: 512 threads x kScanSmallQuads
+// quads of 4 elements held per lane): 32 quads for 32-bit values, 16 for
+// 64-bit ones.
+constexpr int kScanSmallThreads = 512;
 template <typename T>
-constexpr int kScanSmallThreads = sizeof(T) == 4 ? 1024 : 512;
+constexpr int kScanSmallQuads = sizeof(T) == 4 ? 32 : 16;
 template <typename T>
-constexpr int64_t kScanSmall = (int64_t)kScanSmallThreads<T> * 64;
+constexpr int64_t kScanSmall = (int64_t)kScanSmallThreads * 4 * kScanSmallQuads<T>;
 constexpr int kRadixTile = 4096;
 
 struct DsHost {
@@ -338,8 +340,8 @@ template <typename T>
 hm_status scan_excl(hm_job* J, const T* in, int64_t n, T* out, T* total, T* tmp) {
   if (n <= 0) return HM_OK;
   if (n <= kScanSmall<T>) {
-    constexpr int NT = kScanSmallThreads<T>;
-    hipLaunchKernelGGL((scan_small<T, NT>), dim3(1), dim3(NT), 0, J->stream, in, n, out, total,
+    constexpr int NT = kScanSmallThreads, MS = kScanSmallQuads<T>;
+    hipLaunchKernelGGL((scan_small<T, NT, MS>), dim3(1), dim3(NT), 0, J->stream, in, n, out, total,
                        (uint32_t*)nullptr, (int*)nullptr);
     HM_LAUNCH(J->ctx);
     return HM_OK;
@@ -356,7 +358,8 @@ hm_status scan_excl(hm_job* J, const T* in, int64_t n, T* out, T* total, T* tmp)
 // list[0 .. count) = indices i < n with flag[i] set, in order; count on the device.
 hm_status compact_flags(hm_job* J, const uint32_t* flag, int64_t n, uint32_t* list, int* count) {
   if (n <= kScanSmall<uint32_t>) {
-    hipLaunchKernelGGL((scan_small<uint32_t, 1024>), dim3(1), dim3(1024), 0, J->stream, flag, n,
+    hipLaunchKernelGGL((scan_small<uint32_t, kScanSmallThreads, kScanSmallQuads<uint32_t>>), dim3(1),
+                       dim3(kScanSmallThreads), 0, J->stream, flag, n,
                        (uint32_t*)nullptr, (uint32_t*)nullptr, list, count);
   } else {
     HM_TRY(scan_excl<uint32_t>(J, flag, n, J->flagpos, (uint32_t*)nullptr, J->sc32));

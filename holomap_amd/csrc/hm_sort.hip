@@ -124,42 +124,74 @@ __device__ __forceinline__ T wave_lane63(T x) {
 }
 
 // Single-workgroup exclusive scan for short arrays: NT threads, at most
-// NT * 64 elements (the runtime's kScanSmall). Wave w owns the contiguous
-// segment [w*seg, w*seg + seg), seg a multiple of 64, and loads all of it
-// at once into registers (64 consecutive elements per step, at most 64
-// steps), so the scan costs one memory round trip. Every step is scanned
-// on its own (the steps are independent, so their DPP scans overlap); the
-// step totals give each step's offset in the segment and the segment sums
-// each wave's base. With `list` set it instead compacts: list[pos] = i for
-// every i with in[i] != 0 (in is a 0/1 flag array), pos = the exclusive
-// scan, and *count = the number of flags.
-template <typename T, int NT>
+// NT * 4 * MS elements (the runtime's kScanSmall). Wave w owns the contiguous
+// segment [w*seg, w*seg + seg), seg a multiple of 256, and loads all of it
+// at once into registers: step j of a lane is the quad of 4 consecutive
+// elements at 4 * (64 j + lane) (at most MS steps), so the scan costs one
+// memory round trip. A quad is scanned serially in its lane, the quad sums
+// of a step by one DPP wave scan (the steps are independent, so their scans
+// overlap); the step totals give each step's offset in the segment and the
+// segment sums each wave's base. With `list` set it instead compacts:
+// list[pos] = i for every i with in[i] != 0 (in is a 0/1 flag array), pos
+// = the exclusive scan, and *count = the number of flags.
+template <typename T>
+struct Quad {
+  T x[4];
+};
+template <typename T>
+__device__ __forceinline__ Quad<T> load_quad(const T* __restrict__ p, int64_t i, int64_t e,
+                                             bool vec) {
+  Quad<T> q;
+  if (vec && i + 3 < e) {
+    if constexpr (sizeof(T) == 4) {
+      const uint4 u = *(const uint4*)(p + i);
+      q.x[0] = u.x; q.x[1] = u.y; q.x[2] = u.z; q.x[3] = u.w;
+    } else {
+      const ulonglong2 a = *(const ulonglong2*)(p + i), b = *(const ulonglong2*)(p + i + 2);
+      q.x[0] = a.x; q.x[1] = a.y; q.x[2] = b.x; q.x[3] = b.y;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) q.x[c] = i + c < e ? p[i + c] : (T)0;
+  }
+  return q;
+}
+
+template <typename T, int NT, int MS>
 __global__ __launch_bounds__(NT) void scan_small(const T* __restrict__ in, int64_t n,
                                                  T* __restrict__ out, T* __restrict__ total,
                                                  uint32_t* __restrict__ list,
                                                  int* __restrict__ count) {
-  constexpr int NW = NT / 64, MS = 64;  // waves; steps held per lane
+  constexpr int NW = NT / 64;  // waves
   __shared__ T wsum[NW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t seg = (n + NT - 1) / NT * 64;  // <= MS * 64 for n <= NT * 64
+  const int64_t seg = (n + NW * 256 - 1) / (NW * 256) * 256;  // <= MS * 256 for n <= NT * 4 * MS
   const int64_t b = wave * seg < n ? wave * seg : n;
   const int64_t e = b + seg < n ? b + seg : n;
-  // steps of this lane holding an element: j < nv
-  const int nv = e - b > lane ? (int)((e - b - lane + 63) / 64) : 0;
-  const int nsteps = (int)(seg / 64);  // uniform
-  const T* p = in + b + lane;
-  T v[MS];
+  const int nsteps = (int)(seg / 256);  // uniform
+  const bool vec = (((uintptr_t)in | (list ? (uintptr_t)0 : (uintptr_t)out)) & 15) == 0;
+  Quad<T> v[MS];
 #pragma unroll
-  for (int j = 0; j < MS; ++j) v[j] = j < nv ? p[j * 64] : (T)0;
-  // v[j] <- the step's exclusive scan plus the step's offset in the segment
-  uint64_t flags = 0;
+  for (int j = 0; j < MS; ++j)
+    if (j < nsteps) v[j] = load_quad(in, b + 4 * (64 * j + lane), e, vec);
+  // v[j] <- each element's exclusive scan plus the step's offset in the segment
+  uint64_t flags[(MS + 15) / 16] = {};
   T run = 0;
 #pragma unroll
   for (int j = 0; j < MS; ++j) {
     if (j < nsteps) {
-      const T x = wave_scan_incl(v[j]);
-      if (list) flags |= (uint64_t)(v[j] != 0) << j;
-      v[j] = run + x - v[j];
+      T pre[4];
+      T t = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        pre[c] = t;
+        t += v[j].x[c];
+        if (list) flags[(4 * j + c) >> 6] |= (uint64_t)(v[j].x[c] != 0) << ((4 * j + c) & 63);
+      }
+      const T x = wave_scan_incl(t);
+      const T q0 = run + x - t;  // the quad's exclusive offset
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[j].x[c] = q0 + pre[c];
       run += wave_lane63(x);
     }
   }
@@ -174,12 +206,25 @@ __global__ __launch_bounds__(NT) void scan_small(const T* __restrict__ in, int64
   }
 #pragma unroll
   for (int j = 0; j < MS; ++j) {
-    if (j < nv) {
-      const int64_t i = b + lane + (int64_t)j * 64;
+    if (j < nsteps) {
+      const int64_t i = b + 4 * (64 * j + lane);
       if (list) {
-        if ((flags >> j) & 1) list[base + v[j]] = (uint32_t)i;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (i + c < e && ((flags[(4 * j + c) >> 6] >> ((4 * j + c) & 63)) & 1))
+            list[base + v[j].x[c]] = (uint32_t)(i + c);
+      } else if (vec && i + 3 < e) {
+        if constexpr (sizeof(T) == 4) {
+          *(uint4*)(out + i) =
+              make_uint4(base + v[j].x[0], base + v[j].x[1], base + v[j].x[2], base + v[j].x[3]);
+        } else {
+          *(ulonglong2*)(out + i) = make_ulonglong2(base + v[j].x[0], base + v[j].x[1]);
+          *(ulonglong2*)(out + i + 2) = make_ulonglong2(base + v[j].x[2], base + v[j].x[3]);
+        }
       } else {
-        out[i] = base + v[j];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (i + c < e) out[i + c] = base + v[j].x[c];
       }
     }
   }
@@ -189,9 +234,9 @@ __global__ __launch_bounds__(NT) void scan_small(const T* __restrict__ in, int64
   }
 }
 
-template __global__ void scan_small<uint32_t, 1024>(const uint32_t*, int64_t, uint32_t*, uint32_t*,
-                                                    uint32_t*, int*);
-template __global__ void scan_small<uint64_t, 512>(const uint64_t*, int64_t, uint64_t*, uint64_t*,
+template __global__ void scan_small<uint32_t, 512, 32>(const uint32_t*, int64_t, uint32_t*,
+                                                       uint32_t*, uint32_t*, int*);
+template __global__ void scan_small<uint64_t, 512, 16>(const uint64_t*, int64_t, uint64_t*, uint64_t*,
                                                    uint32_t*, int*);
 template __global__ void scan_tile_sums<uint32_t>(const uint32_t*, int64_t, uint32_t*);
 template __global__ void scan_tile_sums<uint64_t>(const uint64_t*, int64_t, uint64_t*);
