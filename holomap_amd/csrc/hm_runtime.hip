@@ -56,7 +56,7 @@ template <typename T>
 __global__ void scan_tiles(const T*, int64_t, const T*, T*);
 __global__ void radix_hist(const uint32_t*, int64_t, int, int64_t, uint32_t*);
 __global__ void radix_scatter(const uint32_t*, const uint32_t*, int64_t, int, int64_t,
-                              const uint32_t*, uint32_t*, uint32_t*);
+                              const uint32_t*, const uint32_t*, uint32_t*, uint32_t*);
 __global__ void lower_bounds(const uint32_t*, int64_t, int64_t, uint32_t*);
 __global__ void compact_kernel(const uint32_t*, const uint32_t*, int64_t, uint32_t*, int*);
 __global__ void fill_u32(uint32_t*, int64_t, uint32_t);
@@ -94,6 +94,9 @@ constexpr int kScanSmallQuads = sizeof(T) == 4 ? 32 : 16;
 template <typename T>
 constexpr int64_t kScanSmall = (int64_t)kScanSmallThreads * 4 * kScanSmallQuads<T>;
 constexpr int kRadixTile = 4096;
+// Radix passes of at most this many tiles skip the table scan (each scatter
+// workgroup reads its digit row of the table: tiles x 256 loads).
+constexpr int64_t kRadixOwnOffsets = 128;
 
 struct DsHost {
   int64_t n, n_pad, row_off, edge_off;
@@ -382,9 +385,13 @@ hm_status radix_sort(hm_job* J, const uint32_t* kin, const uint32_t* vin, int64_
   for (int p = 0; p < passes; ++p) {
     hipLaunchKernelGGL(radix_hist, dim3((unsigned)nt), dim3(256), 0, J->stream, sk, n, 8 * p, nt,
                        J->table);
-    HM_TRY(scan_excl<uint32_t>(J, J->table, 256 * nt, J->toffs, (uint32_t*)nullptr, J->sc32));
+    // short tables: every scatter workgroup sums the table itself
+    const bool own = nt <= kRadixOwnOffsets;
+    if (!own)
+      HM_TRY(scan_excl<uint32_t>(J, J->table, 256 * nt, J->toffs, (uint32_t*)nullptr, J->sc32));
     hipLaunchKernelGGL(radix_scatter, dim3((unsigned)nt), dim3(256), 0, J->stream, sk, sv, n,
-                       8 * p, nt, (const uint32_t*)J->toffs, dk, dv);
+                       8 * p, nt, own ? (const uint32_t*)nullptr : (const uint32_t*)J->toffs,
+                       (const uint32_t*)J->table, dk, dv);
     HM_LAUNCH(J->ctx);
     sk = dk;
     sv = dv;

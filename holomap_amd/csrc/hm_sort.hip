@@ -293,14 +293,20 @@ __global__ __launch_bounds__(256) void radix_hist(const uint32_t* __restrict__ k
 }
 
 // offs = exclusive scan of table (digit-major)
+// offs = exclusive scan of the digit-major table, or nullptr: then each
+// workgroup derives its own offsets from the table (its earlier tiles of a
+// digit plus the totals of the smaller digits), for short tables, saving
+// the scan launch.
 __global__ __launch_bounds__(256) void radix_scatter(const uint32_t* __restrict__ kin,
                                                      const uint32_t* __restrict__ vin, int64_t n,
                                                      int shift, int64_t ntiles,
                                                      const uint32_t* __restrict__ offs,
+                                                     const uint32_t* __restrict__ table,
                                                      uint32_t* __restrict__ kout,
                                                      uint32_t* __restrict__ vout) {
   __shared__ uint32_t cnt[4][256];
   __shared__ uint32_t wbase[4][256];
+  __shared__ uint32_t dsum[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int t = threadIdx.x; t < 1024; t += 256) (&cnt[0][0])[t] = 0;
   __syncthreads();
@@ -328,7 +334,25 @@ __global__ __launch_bounds__(256) void radix_scatter(const uint32_t* __restrict_
   __syncthreads();
   {
     const int dg = threadIdx.x;
-    uint32_t run = offs[(int64_t)dg * ntiles + blockIdx.x];
+    uint32_t run;
+    if (offs) {
+      run = offs[(int64_t)dg * ntiles + blockIdx.x];
+    } else {
+      const uint32_t* row = table + (int64_t)dg * ntiles;
+      uint32_t before = 0, tot = 0;
+#pragma unroll 8
+      for (int64_t t = 0; t < ntiles; ++t) {
+        const uint32_t c = row[t];
+        before += t < (int64_t)blockIdx.x ? c : 0u;
+        tot += c;
+      }
+      const uint32_t incl = wave_scan_incl(tot);  // digits of this wave
+      if (lane == 63) dsum[wave] = incl;
+      __syncthreads();
+      uint32_t wb = 0;
+      for (int w = 0; w < wave; ++w) wb += dsum[w];
+      run = wb + incl - tot + before;
+    }
     for (int w = 0; w < 4; ++w) {
       wbase[w][dg] = run;
       run += cnt[w][dg];
