@@ -5,17 +5,17 @@
 // probabilistic t-conorm w = (v + vt) - v*vt in f32. Here one stable radix
 // sort of the kept edges by destination (hm_sort.hip) gives every row r its
 // in-edges i -> r, sources ascending, at positions [lo_r, hi_r) of the
-// sorted keys. Then a group of G lanes per row (G a power of two >= k; lane
-// t holds out-edge slot t):
-//   union_count_kernel: lo_r / hi_r by binary search, and the row's union
-//     size |A_r| + |I_r| - |A_r n I_r| (A_r: kept out-edge columns, I_r:
-//     in-edge sources; each out-edge column is compared with the in-edge
-//     sources, G of them at a time through lane shuffles);
+// sorted keys. A_r: the row's kept out-edge columns (at most k), I_r: its
+// in-edge sources.
+//   union_count_kernel (a group of G >= k lanes per row, lane t = out-edge
+//     slot t): lo_r / hi_r by binary search; for each out-edge its count of
+//     smaller in-edge sources and its reverse weight; the union size
+//     |A_r| + |I_r| - |A_r n I_r|;
 //   (exclusive scan of the sizes -> global row pointer)
-//   union_write_kernel: every element's output slot is its rank in the
-//     sorted union, #{a in A_r: a < x} + #{i in I_r: i < x}
-//     - #{m in A_r n I_r: m < x}, so no sort of the row is needed; it also
-//     writes the dataset-local row pointer.
+//   union_write_kernel: every entry goes to its rank in the sorted union,
+//     #{a in A_r: a < x} + #{i in I_r: i < x} - #{m in A_r n I_r: m < x}, so
+//     no row is sorted; out-edges by their row's group, in-edges one thread
+//     each (so no row's in-degree serialises), plus the local row pointer.
 // For an entry (r, c): v = A[r][c], vt = A[c][r] (0 if absent), w evaluated
 // with the reference's operand order. With v, vt in [0, 1] and v + vt > 0,
 // w >= max(v, vt) - min(v, vt) and w == 0 would need v == vt == v*vt, so
@@ -64,11 +64,6 @@ __device__ __forceinline__ void grp_for(uint32_t x, uint32_t y, B& b) {
 }
 
 // Loop bodies over a group's broadcast values.
-struct FindSrc {  // is the out-edge column an in-edge source
-  uint32_t ca;
-  bool found;
-  __device__ void operator()(uint32_t su, uint32_t) { found |= su == ca; }
-};
 struct ScanIn {  // sources below the column; the reverse weight
   uint32_t ca, less;
   bool found;
@@ -88,13 +83,20 @@ struct ScanOut {  // out-edge columns (and mutual ones) below x; x among them
   }
 };
 
-// inptr[r] = lo_r (and inptr[R] = hi of the last row), cnt[r] = union size.
+// Row group (G lanes, lane t = out-edge slot t): in-edge range [lo, hi) by
+// binary search (inptr[r] = lo; inptr[R] = hi of the last row); per kept
+// out-edge r -> c: ai[e] = #{in-edge sources < c} | (c is one: bit 31) and
+// av[e] = the weight of c -> r (0 if absent); cnt[r] = union size. Sources
+// are compared G at a time through group broadcasts when the row has at most
+// 4G in-edges, else each lane binary-searches them, so a row of very high
+// in-degree costs O(k log in-degree).
 template <int G>
 __global__ __launch_bounds__(256) void union_count_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_rows, int64_t total_edges,
-    const uint32_t* __restrict__ key1, const uint32_t* __restrict__ k1s,
-    const uint32_t* __restrict__ p1s, const uint32_t* __restrict__ srcrow, uint32_t sentinel,
-    uint32_t* __restrict__ inptr, uint64_t* __restrict__ cnt) {
+    const uint32_t* __restrict__ key1, const float* __restrict__ wval,
+    const uint32_t* __restrict__ k1s, const uint32_t* __restrict__ p1s,
+    const uint32_t* __restrict__ srcrow, uint32_t sentinel, uint32_t* __restrict__ inptr,
+    uint64_t* __restrict__ cnt, uint32_t* __restrict__ ai, float* __restrict__ av) {
   static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "group size");
   const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const int t = (int)(threadIdx.x % G);
@@ -108,14 +110,36 @@ __global__ __launch_bounds__(256) void union_count_kernel(
                            : lower_bound_u32(k1s, total_edges, (uint32_t)r);
   const int64_t lo = (uint32_t)__shfl((int)(uint32_t)b, 0, G);
   const int64_t hi = (uint32_t)__shfl((int)(uint32_t)b, 1, G);
-  FindSrc f{ca, false};  // ca is also an in-edge source (a mutual pair)
-  for (int64_t p0 = lo; p0 < hi; p0 += G) {
-    const int64_t p = p0 + t;
-    const uint32_t src = p < hi ? srcrow[p1s[p]] : 0xffffffffu;
-    grp_for<G>(src, 0u, f);
+  ScanIn si{ca, 0u, false, 0.0f};
+  if (hi - lo <= 4 * G) {
+    for (int64_t p0 = lo; p0 < hi; p0 += G) {
+      const int64_t p = p0 + t;
+      uint32_t src = 0xffffffffu;
+      float w = 0.0f;
+      if (p < hi) {
+        const uint32_t e = p1s[p];
+        src = srcrow[e];
+        w = wval[e];
+      }
+      grp_for<G>(src, __float_as_uint(w), si);
+    }
+  } else if (ca != sentinel) {
+    int64_t a = lo, z = hi;  // first source >= ca
+    while (a < z) {
+      const int64_t mid = (a + z) >> 1;
+      if (srcrow[p1s[mid]] < ca) a = mid + 1; else z = mid;
+    }
+    si.less = (uint32_t)(a - lo);
+    if (a < hi) {
+      const uint32_t e = p1s[a];
+      if (srcrow[e] == ca) { si.found = true; si.vt = wval[e]; }
+    }
   }
-  const bool found = f.found;
-  uint32_t na = ca != sentinel ? 1u : 0u, nm = found ? 1u : 0u;
+  if (t < k) {
+    ai[e0 + t] = si.less | (si.found ? 0x80000000u : 0u);
+    av[e0 + t] = si.vt;
+  }
+  uint32_t na = ca != sentinel ? 1u : 0u, nm = si.found ? 1u : 0u;
 #pragma unroll
   for (int o = G / 2; o > 0; o >>= 1) {
     na += __shfl_xor(na, o, G);
@@ -128,88 +152,93 @@ __global__ __launch_bounds__(256) void union_count_kernel(
   }
 }
 
-// The union's entries of row r at indptr_g[r] + their rank; indptr_l.
+// The union's entries at indptr_g[r] + their rank in row r's sorted union,
+// #{a in A_r: a < x} + #{i in I_r: i < x} - #{mutual m: m < x}:
+//  * threads [0, R*G): row groups place the row's kept out-edges (and write
+//    the dataset-local row pointer);
+//  * threads [R*G, R*G + E): one per destination-sorted in-edge i -> r
+//    without the out-edge r -> i (entry (r, i), v = 0, vt = w), counting
+//    against the row's <= k out-edges, so rows of any in-degree spread over
+//    as many threads as they have in-edges.
 template <int G>
 __global__ __launch_bounds__(256) void union_write_kernel(
-    const DsDesc* __restrict__ ds, int nds, int64_t total_rows, const uint32_t* __restrict__ key1,
-    const float* __restrict__ wval, const uint32_t* __restrict__ p1s,
+    const DsDesc* __restrict__ ds, int nds, int64_t total_rows, int64_t total_edges,
+    const uint32_t* __restrict__ key1, const float* __restrict__ wval,
+    const uint32_t* __restrict__ k1s, const uint32_t* __restrict__ p1s,
     const uint32_t* __restrict__ srcrow, const uint32_t* __restrict__ inptr,
+    const uint32_t* __restrict__ ai, const float* __restrict__ av,
     const uint64_t* __restrict__ indptr_g, const uint64_t* __restrict__ cnt,
     const int64_t* __restrict__ iptr_off, uint32_t sentinel, uint32_t* __restrict__ rows,
     uint32_t* __restrict__ cols, float* __restrict__ vals, uint64_t* __restrict__ indptr_l) {
-  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-  const int t = (int)(threadIdx.x % G);
-  if (r >= total_rows) return;
-  const int di = ds_of_row(ds, nds, r);
-  const DsDesc& D = ds[di];
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gt < total_rows * G) {
+    const int64_t r = gt / G;
+    const int t = (int)(gt % G);
+    const int di = ds_of_row(ds, nds, r);
+    const DsDesc& D = ds[di];
+    const int k = D.k;
+    const int64_t roff = D.row_off;
+    const int64_t e0 = D.edge_off + (r - roff) * k;
+    const uint32_t ca = t < k ? key1[e0 + t] : sentinel;
+    const float va = t < k ? wval[e0 + t] : 0.0f;
+    const uint32_t a = t < k ? ai[e0 + t] : 0u;
+    const float vt = t < k ? av[e0 + t] : 0.0f;
+    // rank among the row's out-edges, and mutual columns below
+    ScanOut so{ca, 0u, 0u, false};
+    grp_for<G>(ca, a >> 31, so);
+    const uint64_t base = indptr_g[r];
+    if (ca != sentinel) {
+      const uint64_t pos = base + so.less + (a & 0x7fffffffu) - so.mut;
+      rows[pos] = (uint32_t)(r - roff);
+      cols[pos] = (uint32_t)((int64_t)ca - roff);
+      vals[pos] = (va + vt) - va * vt;  // f32, the reference's operand order
+    }
+    if (t == 0) {
+      // dataset di owns indptr_l[iptr_off[di] .. + n]
+      const int64_t i = r - roff;
+      const uint64_t b0 = indptr_g[roff];
+      indptr_l[iptr_off[di] + i] = base - b0;
+      if (i == D.n - 1) indptr_l[iptr_off[di] + D.n] = base + cnt[r] - b0;
+    }
+    return;
+  }
+  const int64_t p = gt - total_rows * G;
+  if (p >= total_edges) return;
+  const uint32_t r = k1s[p];
+  if (r == sentinel) return;  // dropped edges sort last
+  const uint32_t e = p1s[p];
+  const uint32_t src = srcrow[e];
+  const float w = wval[e];
+  const DsDesc& D = ds[ds_of_row(ds, nds, r)];
   const int k = D.k;
   const int64_t roff = D.row_off;
-  const int64_t e0 = D.edge_off + (r - roff) * k;
-  const uint32_t ca = t < k ? key1[e0 + t] : sentinel;
-  const float va = t < k ? wval[e0 + t] : 0.0f;
-  const int64_t lo = inptr[r], hi = inptr[r + 1];
-  // per out-edge: in-edge sources below its column, and its reverse weight
-  ScanIn si{ca, 0u, false, 0.0f};
-  for (int64_t p0 = lo; p0 < hi; p0 += G) {
-    const int64_t p = p0 + t;
-    uint32_t src = 0xffffffffu;
-    float w = 0.0f;
-    if (p < hi) {
-      const uint32_t e = p1s[p];
-      src = srcrow[e];
-      w = wval[e];
-    }
-    grp_for<G>(src, __float_as_uint(w), si);
+  const int64_t e0 = D.edge_off + ((int64_t)r - roff) * k;
+  bool in_a = false;
+  uint32_t less_a = 0, mut = 0;
+  for (int t = 0; t < k; ++t) {
+    const uint32_t c = key1[e0 + t];  // sentinel (> every row) when dropped
+    const bool m = (ai[e0 + t] >> 31) != 0u;
+    in_a |= c == src;
+    less_a += c < src ? 1u : 0u;
+    mut += (m && c < src) ? 1u : 0u;
   }
-  // rank among the row's out-edges, and mutual columns below
-  ScanOut so{ca, 0u, 0u, false};
-  grp_for<G>(ca, si.found ? 1u : 0u, so);
-  const uint64_t base = indptr_g[r];
-  if (ca != sentinel) {
-    const uint64_t pos = base + so.less + si.less - so.mut;
-    const float vt = si.vt;
-    rows[pos] = (uint32_t)(r - roff);
-    cols[pos] = (uint32_t)((int64_t)ca - roff);
-    vals[pos] = (va + vt) - va * vt;  // f32, the reference's operand order
-  }
-  // in-edges i -> r without the out-edge r -> i: entry (r, i), v = 0, vt = w
-  for (int64_t p0 = lo; p0 < hi; p0 += G) {
-    const int64_t p = p0 + t;
-    uint32_t src = 0xffffffffu;
-    float w = 0.0f;
-    if (p < hi) {
-      const uint32_t e = p1s[p];
-      src = srcrow[e];
-      w = wval[e];
-    }
-    ScanOut sa{src, 0u, 0u, false};
-    grp_for<G>(ca, si.found ? 1u : 0u, sa);
-    if (p < hi && !sa.in) {
-      const uint64_t pos = base + sa.less + (uint64_t)(p - lo) - sa.mut;
-      const float v = 0.0f;
-      rows[pos] = (uint32_t)(r - roff);
-      cols[pos] = (uint32_t)((int64_t)src - roff);
-      vals[pos] = (v + w) - v * w;
-    }
-  }
-  if (t == 0) {
-    // dataset di owns indptr_l[iptr_off[di] .. + n]
-    const int64_t i = r - roff;
-    const uint64_t b0 = indptr_g[roff];
-    indptr_l[iptr_off[di] + i] = base - b0;
-    if (i == D.n - 1) indptr_l[iptr_off[di] + D.n] = base + cnt[r] - b0;
-  }
+  if (in_a) return;  // placed with the out-edge r -> src
+  const uint64_t pos = indptr_g[r] + less_a + (uint64_t)(p - inptr[r]) - mut;
+  const float v = 0.0f;
+  rows[pos] = (uint32_t)((int64_t)r - roff);
+  cols[pos] = (uint32_t)((int64_t)src - roff);
+  vals[pos] = (v + w) - v * w;
 }
 
 #define HM_UNION_INST(G)                                                                        \
-  template __global__ void union_count_kernel<G>(const DsDesc*, int, int64_t, int64_t,          \
-                                                 const uint32_t*, const uint32_t*,               \
-                                                 const uint32_t*, const uint32_t*, uint32_t,     \
-                                                 uint32_t*, uint64_t*);                          \
+  template __global__ void union_count_kernel<G>(                                               \
+      const DsDesc*, int, int64_t, int64_t, const uint32_t*, const float*, const uint32_t*,     \
+      const uint32_t*, const uint32_t*, uint32_t, uint32_t*, uint64_t*, uint32_t*, float*);     \
   template __global__ void union_write_kernel<G>(                                               \
-      const DsDesc*, int, int64_t, const uint32_t*, const float*, const uint32_t*,              \
-      const uint32_t*, const uint32_t*, const uint64_t*, const uint64_t*, const int64_t*,       \
-      uint32_t, uint32_t*, uint32_t*, float*, uint64_t*);
+      const DsDesc*, int, int64_t, int64_t, const uint32_t*, const float*, const uint32_t*,     \
+      const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*, const float*,         \
+      const uint64_t*, const uint64_t*, const int64_t*, uint32_t, uint32_t*, uint32_t*, float*, \
+      uint64_t*);
 HM_UNION_INST(16)
 HM_UNION_INST(32)
 HM_UNION_INST(64)

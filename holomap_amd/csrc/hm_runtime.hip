@@ -62,13 +62,16 @@ __global__ void compact_kernel(const uint32_t*, const uint32_t*, int64_t, uint32
 __global__ void fill_u32(uint32_t*, int64_t, uint32_t);
 template <int G>
 __global__ void union_count_kernel(const DsDesc*, int, int64_t, int64_t, const uint32_t*,
-                                   const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
-                                   uint32_t*, uint64_t*);
+                                   const float*, const uint32_t*, const uint32_t*,
+                                   const uint32_t*, uint32_t, uint32_t*, uint64_t*, uint32_t*,
+                                   float*);
 template <int G>
-__global__ void union_write_kernel(const DsDesc*, int, int64_t, const uint32_t*, const float*,
+__global__ void union_write_kernel(const DsDesc*, int, int64_t, int64_t, const uint32_t*,
+                                   const float*, const uint32_t*, const uint32_t*,
                                    const uint32_t*, const uint32_t*, const uint32_t*,
-                                   const uint64_t*, const uint64_t*, const int64_t*, uint32_t,
-                                   uint32_t*, uint32_t*, float*, uint64_t*);
+                                   const float*, const uint64_t*, const uint64_t*,
+                                   const int64_t*, uint32_t, uint32_t*, uint32_t*, float*,
+                                   uint64_t*);
 }  // namespace hm
 
 using namespace hm;
@@ -579,18 +582,23 @@ hm_status enqueue_pipeline(hm_job* J) {
     HM_TRY(radix_sort(J, J->key1, J->pay1, E, bits, J->k1s, J->p1s));
     // a group of G >= k lanes per row: union sizes, their scan, the entries
     const int GU = J->kmax <= 16 ? 16 : J->kmax <= 32 ? 32 : 64;
+    // (the radix sort's ping-pong buffers are free again: per out-edge
+    // in-edge counts and reverse weights)
+    uint32_t* ai = J->tmpk;
+    float* av = (float*)J->tmpv;
 #define HM_UNION_COUNT(GS)                                                                     \
   hipLaunchKernelGGL((union_count_kernel<GS>), dim3(grid_for(R * GS, 256)), dim3(256), 0,       \
                      J->stream, J->d_desc, nds, R, E, (const uint32_t*)J->key1,                 \
-                     (const uint32_t*)J->k1s, (const uint32_t*)J->p1s,                          \
-                     (const uint32_t*)J->srcrow, sent, J->inptr, J->cntu)
+                     (const float*)J->wval, (const uint32_t*)J->k1s, (const uint32_t*)J->p1s,   \
+                     (const uint32_t*)J->srcrow, sent, J->inptr, J->cntu, ai, av)
 #define HM_UNION_WRITE(GS)                                                                     \
-  hipLaunchKernelGGL((union_write_kernel<GS>), dim3(grid_for(R * GS, 256)), dim3(256), 0,       \
-                     J->stream, J->d_desc, nds, R, (const uint32_t*)J->key1,                    \
-                     (const float*)J->wval, (const uint32_t*)J->p1s,                            \
+  hipLaunchKernelGGL((union_write_kernel<GS>), dim3(grid_for(R * GS + E, 256)), dim3(256), 0,   \
+                     J->stream, J->d_desc, nds, R, E, (const uint32_t*)J->key1,                 \
+                     (const float*)J->wval, (const uint32_t*)J->k1s, (const uint32_t*)J->p1s,   \
                      (const uint32_t*)J->srcrow, (const uint32_t*)J->inptr,                     \
-                     (const uint64_t*)J->indptr_g, (const uint64_t*)J->cntu,                    \
-                     (const int64_t*)J->iptr_off, sent, J->rows, J->cols, J->vals, J->indptr_l)
+                     (const uint32_t*)ai, (const float*)av, (const uint64_t*)J->indptr_g,       \
+                     (const uint64_t*)J->cntu, (const int64_t*)J->iptr_off, sent, J->rows,      \
+                     J->cols, J->vals, J->indptr_l)
     if (GU == 16) HM_UNION_COUNT(16); else if (GU == 32) HM_UNION_COUNT(32); else HM_UNION_COUNT(64);
     HM_LAUNCH(ctx);
     HM_TRY(scan_excl<uint64_t>(J, J->cntu, R, J->indptr_g, J->total_nnz, J->sc64));
