@@ -11,7 +11,7 @@
 //  * rerank_kernel      -- after the MFMA candidate sweep: recomputes the
 //    plausible candidates exactly, takes the top-k and certifies that no
 //    column outside the candidate set can enter it.
-#include "hm_internal.hpp"
+#include "hm_scan.hpp"
 
 namespace hm {
 
@@ -678,11 +678,10 @@ __device__ __forceinline__ int list_lower_bound(const uint32_t* __restrict__ lis
 // exact f64 scan at ~2e13 flop/s over the device) and emits no items
 // (*nrows = 0: the re-rank of the escalated rows is skipped too) when the
 // scan is cheaper.
-__global__ void sweep_items_kernel(const DsDesc* __restrict__ ds_arr, int nds,
-                                   const uint32_t* __restrict__ list, const int* __restrict__ count,
-                                   int qw, int cus, double rate, WorkItem* __restrict__ items,
-                                   int* __restrict__ nitems, int* __restrict__ nrows) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void sweep_items_body(const DsDesc* __restrict__ ds_arr, int nds,
+                                 const uint32_t* __restrict__ list, const int* __restrict__ count,
+                                 int qw, int cus, double rate, WorkItem* __restrict__ items,
+                                 int* __restrict__ nitems, int* __restrict__ nrows) {
   const int cnt = *count;
   double t_exact = 0.0, t_item = 0.0;
   int64_t nit = 0;
@@ -715,17 +714,23 @@ __global__ void sweep_items_kernel(const DsDesc* __restrict__ ds_arr, int nds,
   }
   *nitems = w;
 }
+__global__ void sweep_items_kernel(const DsDesc* __restrict__ ds_arr, int nds,
+                                   const uint32_t* __restrict__ list, const int* __restrict__ count,
+                                   int qw, int cus, double rate, WorkItem* __restrict__ items,
+                                   int* __restrict__ nitems, int* __restrict__ nrows) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    sweep_items_body(ds_arr, nds, list, count, qw, cus, rate, items, nitems, nrows);
+}
 
 // The column split count starts at the planned xs and grows when few rows are
 // flagged (a handful of rows would otherwise run on a handful of CUs), up to
 // xs_cap (which keeps the merge at two levels) and within the item and
 // partial-list capacities, which hold item_cap items and row_cap * xs lists.
-__global__ void fallback_items_kernel(const DsDesc* __restrict__ ds_arr, int nds,
-                                      const uint32_t* __restrict__ list, const int* __restrict__ count,
-                                      int xs, int xs_cap, int64_t item_cap, int64_t row_cap,
-                                      int target, WorkItem* __restrict__ items,
-                                      int* __restrict__ nitems, int* __restrict__ xs_out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void fallback_items_body(const DsDesc* __restrict__ ds_arr, int nds,
+                                    const uint32_t* __restrict__ list, const int* __restrict__ count,
+                                    int xs, int xs_cap, int64_t item_cap, int64_t row_cap,
+                                    int target, WorkItem* __restrict__ items,
+                                    int* __restrict__ nitems, int* __restrict__ xs_out) {
   const int cnt = *count;
   int64_t nqb = 0;
   for (int di = 0; di < nds; ++di) {
@@ -752,6 +757,40 @@ __global__ void fallback_items_kernel(const DsDesc* __restrict__ ds_arr, int nds
   }
   *nitems = w;
   *xs_out = (int)xd;
+}
+__global__ void fallback_items_kernel(const DsDesc* __restrict__ ds_arr, int nds,
+                                      const uint32_t* __restrict__ list, const int* __restrict__ count,
+                                      int xs, int xs_cap, int64_t item_cap, int64_t row_cap,
+                                      int target, WorkItem* __restrict__ items,
+                                      int* __restrict__ nitems, int* __restrict__ xs_out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    fallback_items_body(ds_arr, nds, list, count, xs, xs_cap, item_cap, row_cap, target, items,
+                        nitems, xs_out);
+}
+
+// Flag compaction fused with the work list it feeds (flag arrays of at most
+// kScanSmall entries: one workgroup of 512 threads compacts, then its
+// first thread builds the items from the list, which the barrier has made
+// visible to it).
+__global__ __launch_bounds__(512) void flag_sweep_items_kernel(
+    const uint32_t* __restrict__ flag, int64_t n, uint32_t* __restrict__ list,
+    int* __restrict__ count, const DsDesc* __restrict__ ds_arr, int nds, int qw, int cus,
+    double rate, WorkItem* __restrict__ items, int* __restrict__ nitems, int* __restrict__ nrows) {
+  scan_small_body<uint32_t, 512, 32>(flag, n, nullptr, nullptr, list, count);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    sweep_items_body(ds_arr, nds, list, count, qw, cus, rate, items, nitems, nrows);
+}
+__global__ __launch_bounds__(512) void flag_fallback_items_kernel(
+    const uint32_t* __restrict__ flag, int64_t n, uint32_t* __restrict__ list,
+    int* __restrict__ count, const DsDesc* __restrict__ ds_arr, int nds, int xs, int xs_cap,
+    int64_t item_cap, int64_t row_cap, int target, WorkItem* __restrict__ items,
+    int* __restrict__ nitems, int* __restrict__ xs_out) {
+  scan_small_body<uint32_t, 512, 32>(flag, n, nullptr, nullptr, list, count);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    fallback_items_body(ds_arr, nds, list, count, xs, xs_cap, item_cap, row_cap, target, items,
+                        nitems, xs_out);
 }
 
 }  // namespace hm

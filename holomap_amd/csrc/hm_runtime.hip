@@ -37,6 +37,11 @@ __global__ void sweep_items_kernel(const DsDesc*, int, const uint32_t*, const in
                                    double, WorkItem*, int*, int*);
 __global__ void fallback_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int, int,
                                       int64_t, int64_t, int, WorkItem*, int*, int*);
+__global__ void flag_sweep_items_kernel(const uint32_t*, int64_t, uint32_t*, int*, const DsDesc*,
+                                        int, int, int, double, WorkItem*, int*, int*);
+__global__ void flag_fallback_items_kernel(const uint32_t*, int64_t, uint32_t*, int*,
+                                           const DsDesc*, int, int, int, int64_t, int64_t, int,
+                                           WorkItem*, int*, int*);
 __global__ void smooth_knn_kernel(const DsDesc*, int, int64_t, const double*, int, float*, float*,
                                   double*, uint32_t*);
 __global__ void calib_fixup_kernel(const DsDesc*, int, float*, double*, const float*, uint32_t*);
@@ -96,6 +101,8 @@ template <typename T>
 constexpr int kScanSmallQuads = sizeof(T) == 4 ? 32 : 16;
 template <typename T>
 constexpr int64_t kScanSmall = (int64_t)kScanSmallThreads * 4 * kScanSmallQuads<T>;
+static_assert(kScanSmallThreads == 512 && kScanSmallQuads<uint32_t> == 32,
+              "flag_*_items_kernel compact with scan_small_body<uint32_t, 512, 32>");
 constexpr int kRadixTile = 4096;
 // Radix passes of at most this many tiles skip the table scan (each scatter
 // workgroup reads its digit row of the table: tiles x 256 loads).
@@ -507,11 +514,18 @@ hm_status enqueue_pipeline(hm_job* J) {
         // smaller error bound) and re-ranked; only rows still uncertified
         // reach the exact scan below
         const int em = J->esc_mode;
-        HM_TRY(compact_flags(J, J->flag, R, J->list, J->count));
-        hipLaunchKernelGGL(sweep_items_kernel, dim3(1), dim3(64), 0, J->stream, J->d_desc, nds,
-                           (const uint32_t*)J->list, (const int*)J->count, sweep_rows(em), cus,
-                           em == 2 ? 1.1e12 : 0.45e12, J->items_l2, J->nitems_l2,
-                           J->nitems_l2 + 1);
+        const double rate = em == 2 ? 1.1e12 : 0.45e12;
+        if (R <= kScanSmall<uint32_t>) {  // compaction and items in one launch
+          hipLaunchKernelGGL(flag_sweep_items_kernel, dim3(1), dim3(kScanSmallThreads), 0,
+                             J->stream, (const uint32_t*)J->flag, R, J->list, J->count, J->d_desc,
+                             nds, sweep_rows(em), cus, rate, J->items_l2, J->nitems_l2,
+                             J->nitems_l2 + 1);
+        } else {
+          HM_TRY(compact_flags(J, J->flag, R, J->list, J->count));
+          hipLaunchKernelGGL(sweep_items_kernel, dim3(1), dim3(64), 0, J->stream, J->d_desc, nds,
+                             (const uint32_t*)J->list, (const int*)J->count, sweep_rows(em), cus,
+                             rate, J->items_l2, J->nitems_l2, J->nitems_l2 + 1);
+        }
         HM_LAUNCH(ctx);
         HM_TRY(launch_sweep(J, em, J->items_l2, J->nitems_l2, 0, J->list, (unsigned)cus));
         hipLaunchKernelGGL(rerank_kernel, dim3(std::max<int64_t>(1, std::min<int64_t>(cdiv(R, 4), 8 * cus))),
@@ -526,7 +540,6 @@ hm_status enqueue_pipeline(hm_job* J) {
       mark(J, 3);
     }
     // ---- exact scan of the uncertified rows ----
-    HM_TRY(compact_flags(J, J->flag, R, J->list, J->count));
     const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(J->fb_cap, 2 * cus));
     // merge groups: G splits of k keys fit one 1024-key sort; two merge
     // levels cover up to G * G splits
@@ -534,9 +547,17 @@ hm_status enqueue_pipeline(hm_job* J) {
     while (kpow < J->kmax) kpow <<= 1;
     const int G = std::max(1, 1024 / kpow);
     const int xs_cap = std::max(J->xs, std::min(512, G * G));
-    hipLaunchKernelGGL(fallback_items_kernel, dim3(1), dim3(64), 0, J->stream, J->d_desc, nds,
-                       (const uint32_t*)J->list, (const int*)J->count, J->xs, xs_cap,
-                       (int64_t)J->fb_cap, R, (int)gx, J->items_fb, J->nitems_fb, J->xs_fb);
+    if (R <= kScanSmall<uint32_t>) {  // compaction and items in one launch
+      hipLaunchKernelGGL(flag_fallback_items_kernel, dim3(1), dim3(kScanSmallThreads), 0,
+                         J->stream, (const uint32_t*)J->flag, R, J->list, J->count, J->d_desc, nds,
+                         J->xs, xs_cap, (int64_t)J->fb_cap, R, (int)gx, J->items_fb, J->nitems_fb,
+                         J->xs_fb);
+    } else {
+      HM_TRY(compact_flags(J, J->flag, R, J->list, J->count));
+      hipLaunchKernelGGL(fallback_items_kernel, dim3(1), dim3(64), 0, J->stream, J->d_desc, nds,
+                         (const uint32_t*)J->list, (const int*)J->count, J->xs, xs_cap,
+                         (int64_t)J->fb_cap, R, (int)gx, J->items_fb, J->nitems_fb, J->xs_fb);
+    }
     hipLaunchKernelGGL(exact_scan_kernel, dim3(gx), dim3(256), 0, J->stream, J->d_desc,
                        (const WorkItem*)J->items_fb, (const int*)J->nitems_fb, 0,
                        (const uint32_t*)J->list, (const int*)J->xs_fb, J->xs, J->kmax, J->xpart);
