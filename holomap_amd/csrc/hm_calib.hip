@@ -185,6 +185,45 @@ __global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_
 // reference's order t = 1 .. k-1 from the group's registers, so the sums,
 // the bracket updates and the outputs are those of smooth_knn_kernel bit
 // for bit. Loop bounds depend on the row only, so a group stays converged.
+// The value of lane U of the calling lane's G-lane group: a DPP row
+// broadcast (one VALU move) for 16-lane groups, an LDS permute otherwise.
+// (The kernel folds with these for 16-lane groups only; wider groups keep a
+// loop of permutes, the unrolled form being too long.)
+template <int G, int U>
+__device__ __forceinline__ float grp_lane_f32(float x) {
+  if constexpr (G == 16)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x150 + U, 0xf, 0xf, false));
+  else
+    return __shfl(x, U, G);
+}
+template <int G, int U>
+__device__ __forceinline__ double grp_lane_f64(double x) {
+  if constexpr (G == 16) {
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x150 + U, 0xf, 0xf, false);
+    const uint32_t hi =
+        (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x150 + U, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+  } else {
+    return __shfl(x, U, G);
+  }
+}
+// psum + e_1 + e_2 + ... + e_{k-1}, left to right (e_t: lane t's value); the
+// slots t >= k add +0.0, which leaves the non-negative partial sums as they
+// are, so the result is the reference's fold bit for bit.
+template <int G, int U = 1, typename T>
+__device__ __forceinline__ T grp_fold(T e, int k, T psum) {
+  if constexpr (U < G) {
+    T v;
+    if constexpr (sizeof(T) == 4) v = grp_lane_f32<G, U>(e);
+    else v = grp_lane_f64<G, U>(e);
+    psum += U < k ? v : (T)0;
+    return grp_fold<G, U + 1>(e, k, psum);
+  } else {
+    return psum;
+  }
+}
+
 template <int G>
 __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
@@ -218,7 +257,11 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
         e = dd > 0.0f ? hm_expf(-(dd / mid)) : 1.0f;
       }
       float psum = 0.0f;
-      for (int t = 1; t < k; ++t) psum += __shfl(e, t, G);
+      if constexpr (G == 16) {
+        psum = grp_fold<G>(e, k, 0.0f);
+      } else {
+        for (int t = 1; t < k; ++t) psum += __shfl(e, t, G);
+      }
       if (fabs((double)psum - target) < 1e-5) break;
       if ((double)psum > target) {
         hi = mid;
@@ -256,7 +299,11 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
         e = dd > 0.0 ? exp(-(dd / mid)) : 1.0;
       }
       double psum = 0.0;
-      for (int t = 1; t < k; ++t) psum += __shfl(e, t, G);
+      if constexpr (G == 16) {
+        psum = grp_fold<G>(e, k, 0.0);
+      } else {
+        for (int t = 1; t < k; ++t) psum += __shfl(e, t, G);
+      }
       if (psum == target) break;
       double nmid;
       if (psum > target) {
