@@ -244,10 +244,4 @@ HM_UNION_INST(32)
 HM_UNION_INST(64)
 #undef HM_UNION_INST
 
-// CSR values widened to f64 (src/sparse.rs:32).
-__global__ void widen_kernel(const float* __restrict__ v, int64_t n, double* __restrict__ out) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) out[t] = (double)v[t];
-}
-
 }  // namespace hm
