@@ -282,29 +282,49 @@ __device__ __forceinline__ void sweep_acc_start(floatx16 (&acc)[4], const uint8_
 // key' >= Tp (Tp = T / s; NaN for lanes without a query: never); hits are
 // appended as (key' bits, column) to dst[min(cnt, cout-1)] and all counted
 // -- cnt > cout afterwards means the list overflowed (its contents are then
-// unused). A position costs a compare and a skipped branch when no lane of
-// the wave hits.
+// unused).
 template <int PASS>
 __device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[4], float (&gmax)[4][16],
                                                uint32_t colbase, float Tp, int cout,
                                                gptr_u64* dst, int& cnt) {
-  // +inf in a register: with a literal the compiler turns the med3 into a
-  // NaN-canonicalising max (three instructions)
-  float pinf = INFINITY;
-  if constexpr (PASS == 0) asm volatile("" : "+v"(pinf));
+  if constexpr (PASS == 0) {
+    // +inf in a register: with a literal the compiler turns the med3 into a
+    // NaN-canonicalising max (three instructions)
+    float pinf = INFINITY;
+    asm volatile("" : "+v"(pinf));
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int a = 0; a < 16; ++a) {
-      const int jl = t * 32 + 8 * (a >> 2) + (a & 3);  // column at lane half 0
-      if constexpr (PASS == 0) {
-        gmax[t][a] = __builtin_amdgcn_fmed3f(gmax[t][a], acc[t][a], pinf);
-      } else {
-        if (acc[t][a] >= Tp) {
-          dst[cnt < cout ? cnt : cout - 1] =
-              ((uint64_t)__float_as_uint(acc[t][a]) << 32) | (colbase + jl);
-          ++cnt;
-        }
+      for (int a = 0; a < 16; ++a) gmax[t][a] = __builtin_amdgcn_fmed3f(gmax[t][a], acc[t][a], pinf);
+  } else {
+    // A branch per position (compare, exec save, branch, exec restore in
+    // sequence) costs ~75 cycles each, although hits are rare (about one
+    // per lane and block). Instead: a tile's 16 hit bits first (plain VALU),
+    // then one pass per hit of the lane (positions ascending, the order the
+    // lists keep), the value picked by a bitwise select tree on the position
+    // (a tree of conditional moves on its bits; no register indexing).
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      uint32_t m = 0;
+#pragma unroll
+      for (int a = 0; a < 16; ++a) m |= acc[t][a] >= Tp ? 1u << a : 0u;
+      while (m) {
+        const uint32_t a = (uint32_t)__builtin_ctz(m);
+        m &= m - 1;
+        const uint32_t m0 = 0u - (a & 1u), m1 = 0u - ((a >> 1) & 1u);
+        const uint32_t m2 = 0u - ((a >> 2) & 1u), m3 = 0u - ((a >> 3) & 1u);
+        uint32_t l1[8], l2[4], l3[2];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          l1[q] = (__float_as_uint(acc[t][2 * q + 1]) & m0) | (__float_as_uint(acc[t][2 * q]) & ~m0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) l2[q] = (l1[2 * q + 1] & m1) | (l1[2 * q] & ~m1);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) l3[q] = (l2[2 * q + 1] & m2) | (l2[2 * q] & ~m2);
+        const uint32_t bits = (l3[1] & m3) | (l3[0] & ~m3);
+        const uint32_t jl = t * 32 + 8 * (a >> 2) + (a & 3);  // column at lane half 0
+        dst[cnt < cout ? cnt : cout - 1] = ((uint64_t)bits << 32) | (colbase + jl);
+        ++cnt;
       }
     }
   }
