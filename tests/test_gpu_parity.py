@@ -191,6 +191,30 @@ def test_batch_equals_alone(ctx):
     job.close()
 
 
+def test_floor_and_maxima_over_replays_and_batches(ctx):
+    # The global-mean floor's per-dataset flags are set during a run and
+    # cleared for the next one: a batch where only some datasets need the
+    # floor, replayed, and re-uploaded with the roles swapped.
+    rng = np.random.default_rng(11)
+    specs = [(600, 12, 15, "euclidean"), (900, 40, 10, "cosine"), (500, 12, 15, "euclidean")]
+    data = [rng.normal(size=(n, d)).astype(np.float32) for n, d, _, _ in specs]
+    data[0][100:130] = data[0][5]  # 31 identical rows: rho = 0, floor applies
+    data[1][:40] = data[1][7]      # 40 parallel copies: cosine rho = 0 as well
+    job = H.Job(ctx, [s[0] for s in specs], [s[1] for s in specs], [s[2] for s in specs],
+                [s[3] for s in specs], use_graph=True)
+    for phase in range(2):
+        for i, x in enumerate(data):
+            job.upload(i, x)
+        refs = [O.knn_graph(x, d, k, m) for x, (_, d, k, m) in zip(data, specs)]
+        for _ in range(3):
+            job.run()
+            job.sync()
+            for i in range(len(specs)):
+                assert_graph_equal(job.download(i), *refs[i])
+        data[0], data[2] = rng.normal(size=(600, 12)).astype(np.float32), data[0][:500].copy()
+    job.close()
+
+
 def test_graph_replay(ctx):
     x = DS.c2(n=3000, d=64)
     ref = O.knn_graph(x, 64, 15, "euclidean")
