@@ -187,8 +187,8 @@ __global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_
 // for bit. Loop bounds depend on the row only, so a group stays converged.
 // The value of lane U of the calling lane's G-lane group: a DPP row
 // broadcast (one VALU move) for 16-lane groups, an LDS permute otherwise.
-// (The kernel folds with these for 16-lane groups only; wider groups keep a
-// loop of permutes, the unrolled form being too long.)
+// (The kernel folds with these for 16- and 32-lane groups; 64-lane groups
+// keep a loop of permutes, the unrolled form being too long.)
 template <int G, int U>
 __device__ __forceinline__ float grp_lane_f32(float x) {
   if constexpr (G == 16)
@@ -222,6 +222,51 @@ __device__ __forceinline__ T grp_fold(T e, int k, T psum) {
   } else {
     return psum;
   }
+}
+// Both 16-lane rows of every 32-lane group hold row 0's values (lo) and
+// row 1's values (hi): v_permlane16_swap exchanges the odd rows of its first
+// operand with the even rows of its second, within each pair of rows.
+__device__ __forceinline__ void rows_of_pair(uint32_t x, uint32_t& lo, uint32_t& hi) {
+  const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+  lo = r[0];
+  hi = r[1];
+}
+template <typename T>
+__device__ __forceinline__ void rows_of_pair_t(T x, T& lo, T& hi) {
+  if constexpr (sizeof(T) == 4) {
+    uint32_t a, b;
+    rows_of_pair(__float_as_uint(x), a, b);
+    lo = __uint_as_float(a);
+    hi = __uint_as_float(b);
+  } else {
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    uint32_t la, lb, ha, hb;
+    rows_of_pair((uint32_t)u, la, lb);
+    rows_of_pair((uint32_t)(u >> 32), ha, hb);
+    lo = __longlong_as_double((long long)(((uint64_t)ha << 32) | la));
+    hi = __longlong_as_double((long long)(((uint64_t)hb << 32) | lb));
+  }
+}
+// The same fold for 32-lane groups: terms 1..15 from row 0's values, then
+// 16..31 from row 1's, each by a DPP row broadcast.
+template <int U, typename T>
+__device__ __forceinline__ T row_fold(T v, int k, int base, T psum) {
+  if constexpr (U < 16) {
+    T w;
+    if constexpr (sizeof(T) == 4) w = grp_lane_f32<16, U>(v);
+    else w = grp_lane_f64<16, U>(v);
+    psum += base + U < k ? w : (T)0;
+    return row_fold<U + 1>(v, k, base, psum);
+  } else {
+    return psum;
+  }
+}
+template <typename T>
+__device__ __forceinline__ T grp_fold32(T e, int k, T psum) {
+  T lo, hi;
+  rows_of_pair_t(e, lo, hi);
+  psum = row_fold<1>(lo, k, 0, psum);
+  return row_fold<0>(hi, k, 16, psum);
 }
 
 template <int G>
@@ -259,6 +304,8 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
       float psum = 0.0f;
       if constexpr (G == 16) {
         psum = grp_fold<G>(e, k, 0.0f);
+      } else if constexpr (G == 32) {
+        psum = grp_fold32(e, k, 0.0f);
       } else {
         for (int t = 1; t < k; ++t) psum += __shfl(e, t, G);
       }
@@ -301,6 +348,8 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
       double psum = 0.0;
       if constexpr (G == 16) {
         psum = grp_fold<G>(e, k, 0.0);
+      } else if constexpr (G == 32) {
+        psum = grp_fold32(e, k, 0.0);
       } else {
         for (int t = 1; t < k; ++t) psum += __shfl(e, t, G);
       }

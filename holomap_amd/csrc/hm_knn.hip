@@ -652,9 +652,6 @@ __global__ __launch_bounds__(256, 6) void rerank_kernel(const DsDesc* __restrict
   }
 }
 
-// Fallback work list: for each dataset, blocks of <= XS_QB flagged rows
-// (from the sorted global list) x xs column splits. Single thread; the
-// dataset count is small.
 // First position in the sorted list [0, cnt) holding a row id >= v.
 __device__ __forceinline__ int list_lower_bound(const uint32_t* __restrict__ list, int cnt,
                                                 int64_t v) {
@@ -666,9 +663,50 @@ __device__ __forceinline__ int list_lower_bound(const uint32_t* __restrict__ lis
   return lo;
 }
 
+// Block-wide helpers of the work-list builders (every thread of the
+// workgroup calls them; NT is a multiple of 64). sh: NT / 64 words.
+template <int NT>
+__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t& total,
+                                                        uint64_t* sh) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t inc = wave_scan_incl(v);
+  if (lane == 63) sh[wave] = inc;
+  __syncthreads();
+  uint64_t base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) {
+    const uint64_t x = sh[w];
+    base += w < wave ? x : 0ull;
+    tot += x;
+  }
+  __syncthreads();  // sh is rewritten by the next call
+  total = tot;
+  return base + inc - v;
+}
+// Sum (MAX = false) or maximum of a double over the workgroup, in a fixed
+// order (the same result on every run).
+template <int NT, bool MAX>
+__device__ __forceinline__ double block_reduce_f64(double v, double* sh) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_xor(v, o, 64);
+    v = MAX ? (t > v ? t : v) : v + t;
+  }
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double r = sh[0];
+#pragma unroll
+  for (int w = 1; w < NT / 64; ++w) r = MAX ? (sh[w] > r ? sh[w] : r) : r + sh[w];
+  __syncthreads();
+  return r;
+}
+
 // Work items of a candidate sweep over the listed rows only (sorted global
-// row ids): per dataset, blocks of qw listed rows x its column splits.
-// Single thread; datasets are found by binary search in the list.
+// row ids): per dataset, blocks of qw listed rows x its column splits, in
+// dataset order. One workgroup of NT threads, a thread per dataset (its
+// range of the list by binary search); the item offsets are a scan over the
+// datasets.
 //
 // The re-sweep pays a whole item (qw rows x a column split, both passes)
 // even for one listed row, while the exact scan pays per row, so a handful of
@@ -678,66 +716,93 @@ __device__ __forceinline__ int list_lower_bound(const uint32_t* __restrict__ lis
 // exact f64 scan at ~2e13 flop/s over the device) and emits no items
 // (*nrows = 0: the re-rank of the escalated rows is skipped too) when the
 // scan is cheaper.
-__device__ void sweep_items_body(const DsDesc* __restrict__ ds_arr, int nds,
-                                 const uint32_t* __restrict__ list, const int* __restrict__ count,
-                                 int qw, int cus, double rate, WorkItem* __restrict__ items,
-                                 int* __restrict__ nitems, int* __restrict__ nrows) {
+template <int NT>
+__device__ void sweep_items_block(const DsDesc* __restrict__ ds_arr, int nds,
+                                  const uint32_t* __restrict__ list, const int* __restrict__ count,
+                                  int qw, int cus, double rate, WorkItem* __restrict__ items,
+                                  int* __restrict__ nitems, int* __restrict__ nrows) {
+  __shared__ uint64_t sh[NT / 64];
+  __shared__ double shd[NT / 64];
   const int cnt = *count;
-  double t_exact = 0.0, t_item = 0.0;
-  int64_t nit = 0;
-  for (int di = 0; di < nds; ++di) {
+  double te = 0.0, ti = 0.0;
+  uint64_t nq = 0;
+  for (int di = threadIdx.x; di < nds; di += NT) {
     const DsDesc& D = ds_arr[di];
     const int p = list_lower_bound(list, cnt, D.row_off);
     const int q = list_lower_bound(list, cnt, D.row_off + D.n);
     if (q == p) continue;
-    t_exact += (double)(q - p) * (double)D.n * D.kp * 3.0 / 2e13;
+    te += (double)(q - p) * (double)D.n * D.kp * 3.0 / 2e13;
     const double span = (double)D.n / D.nsplit;
-    const double ti = 4.0 * span * D.kp * qw / rate;
-    t_item = ti > t_item ? ti : t_item;
-    nit += (int64_t)((q - p + qw - 1) / qw) * D.nsplit;
+    const double t = 4.0 * span * D.kp * qw / rate;
+    ti = t > ti ? t : ti;
+    nq += (uint64_t)((q - p + qw - 1) / qw) * D.nsplit;
   }
+  const double t_exact = block_reduce_f64<NT, false>(te, shd);
+  const double t_item = block_reduce_f64<NT, true>(ti, shd);
+  uint64_t nit;
+  (void)block_excl_scan_u64<NT>(nq, nit, sh);
   const double t_sweep = (double)((nit + cus - 1) / cus) * t_item;
-  if (cnt == 0 || t_exact <= t_sweep) {
-    *nitems = 0;
-    *nrows = 0;
+  if (cnt == 0 || t_exact <= t_sweep) {  // uniform
+    if (threadIdx.x == 0) {
+      *nitems = 0;
+      *nrows = 0;
+    }
     return;
   }
-  *nrows = cnt;
-  int w = 0;
-  for (int di = 0; di < nds; ++di) {
-    const int p = list_lower_bound(list, cnt, ds_arr[di].row_off);
-    const int q = list_lower_bound(list, cnt, ds_arr[di].row_off + ds_arr[di].n);
+  uint64_t base = 0;
+  for (int c0 = 0; c0 < nds; c0 += NT) {  // uniform
+    const int di = c0 + (int)threadIdx.x;
+    int p = 0, q = 0, ns = 0;
+    if (di < nds) {
+      p = list_lower_bound(list, cnt, ds_arr[di].row_off);
+      q = list_lower_bound(list, cnt, ds_arr[di].row_off + ds_arr[di].n);
+      ns = ds_arr[di].nsplit;
+    }
+    uint64_t tot;
+    const uint64_t off = block_excl_scan_u64<NT>((uint64_t)((q - p + qw - 1) / qw) * ns, tot, sh);
+    int w = (int)(base + off);
     for (int b = p; b < q; b += qw) {
       const int qc = q - b < qw ? q - b : qw;
-      for (int s = 0; s < ds_arr[di].nsplit; ++s) items[w++] = WorkItem{di, s, b, qc};
+      for (int s = 0; s < ns; ++s) items[w++] = WorkItem{di, s, b, qc};
     }
+    base += tot;
   }
-  *nitems = w;
+  if (threadIdx.x == 0) {
+    *nrows = cnt;
+    *nitems = (int)base;
+  }
 }
-__global__ void sweep_items_kernel(const DsDesc* __restrict__ ds_arr, int nds,
-                                   const uint32_t* __restrict__ list, const int* __restrict__ count,
-                                   int qw, int cus, double rate, WorkItem* __restrict__ items,
-                                   int* __restrict__ nitems, int* __restrict__ nrows) {
-  if (threadIdx.x == 0 && blockIdx.x == 0)
-    sweep_items_body(ds_arr, nds, list, count, qw, cus, rate, items, nitems, nrows);
+__global__ __launch_bounds__(256) void sweep_items_kernel(
+    const DsDesc* __restrict__ ds_arr, int nds, const uint32_t* __restrict__ list,
+    const int* __restrict__ count, int qw, int cus, double rate, WorkItem* __restrict__ items,
+    int* __restrict__ nitems, int* __restrict__ nrows) {
+  sweep_items_block<256>(ds_arr, nds, list, count, qw, cus, rate, items, nitems, nrows);
 }
 
 // The column split count starts at the planned xs and grows when few rows are
 // flagged (a handful of rows would otherwise run on a handful of CUs), up to
 // xs_cap (which keeps the merge at two levels) and within the item and
 // partial-list capacities, which hold item_cap items and row_cap * xs lists.
-__device__ void fallback_items_body(const DsDesc* __restrict__ ds_arr, int nds,
-                                    const uint32_t* __restrict__ list, const int* __restrict__ count,
-                                    int xs, int xs_cap, int64_t item_cap, int64_t row_cap,
-                                    int target, WorkItem* __restrict__ items,
-                                    int* __restrict__ nitems, int* __restrict__ xs_out) {
+// Items: per dataset, blocks of <= XS_QB listed rows x the splits, in dataset
+// order; one workgroup, a thread per dataset, as above.
+template <int NT>
+__device__ void fallback_items_block(const DsDesc* __restrict__ ds_arr, int nds,
+                                     const uint32_t* __restrict__ list,
+                                     const int* __restrict__ count, int xs, int xs_cap,
+                                     int64_t item_cap, int64_t row_cap, int target,
+                                     WorkItem* __restrict__ items, int* __restrict__ nitems,
+                                     int* __restrict__ xs_out) {
+  __shared__ uint64_t sh[NT / 64];
   const int cnt = *count;
-  int64_t nqb = 0;
-  for (int di = 0; di < nds; ++di) {
+  uint64_t part = 0;
+  for (int di = threadIdx.x; di < nds; di += NT) {
     const int p = list_lower_bound(list, cnt, ds_arr[di].row_off);
     const int q = list_lower_bound(list, cnt, ds_arr[di].row_off + ds_arr[di].n);
-    nqb += (q - p + XS_QB - 1) / XS_QB;
+    part += (uint64_t)((q - p + XS_QB - 1) / XS_QB);
   }
+  uint64_t nqb_u;
+  (void)block_excl_scan_u64<NT>(part, nqb_u, sh);
+  const int64_t nqb = (int64_t)nqb_u;
   int64_t xd = xs;
   if (nqb > 0 && nqb * xs < target) {
     int64_t g = target / nqb;
@@ -746,40 +811,47 @@ __device__ void fallback_items_body(const DsDesc* __restrict__ ds_arr, int nds,
     g = g < row_cap * xs / cnt ? g : row_cap * xs / cnt;
     xd = g > xd ? g : xd;
   }
-  int w = 0;
-  for (int di = 0; di < nds; ++di) {
-    const int p = list_lower_bound(list, cnt, ds_arr[di].row_off);
-    const int q = list_lower_bound(list, cnt, ds_arr[di].row_off + ds_arr[di].n);
+  uint64_t base = 0;
+  for (int c0 = 0; c0 < nds; c0 += NT) {  // uniform
+    const int di = c0 + (int)threadIdx.x;
+    int p = 0, q = 0;
+    if (di < nds) {
+      p = list_lower_bound(list, cnt, ds_arr[di].row_off);
+      q = list_lower_bound(list, cnt, ds_arr[di].row_off + ds_arr[di].n);
+    }
+    uint64_t tot;
+    const uint64_t off =
+        block_excl_scan_u64<NT>((uint64_t)((q - p + XS_QB - 1) / XS_QB) * (uint64_t)xd, tot, sh);
+    int w = (int)(base + off);
     for (int b = p; b < q; b += XS_QB) {
       const int qc = q - b < XS_QB ? q - b : XS_QB;
       for (int s = 0; s < (int)xd; ++s) items[w++] = WorkItem{di, s, b, qc};
     }
+    base += tot;
   }
-  *nitems = w;
-  *xs_out = (int)xd;
+  if (threadIdx.x == 0) {
+    *nitems = (int)base;
+    *xs_out = (int)xd;
+  }
 }
-__global__ void fallback_items_kernel(const DsDesc* __restrict__ ds_arr, int nds,
-                                      const uint32_t* __restrict__ list, const int* __restrict__ count,
-                                      int xs, int xs_cap, int64_t item_cap, int64_t row_cap,
-                                      int target, WorkItem* __restrict__ items,
-                                      int* __restrict__ nitems, int* __restrict__ xs_out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0)
-    fallback_items_body(ds_arr, nds, list, count, xs, xs_cap, item_cap, row_cap, target, items,
-                        nitems, xs_out);
+__global__ __launch_bounds__(256) void fallback_items_kernel(
+    const DsDesc* __restrict__ ds_arr, int nds, const uint32_t* __restrict__ list,
+    const int* __restrict__ count, int xs, int xs_cap, int64_t item_cap, int64_t row_cap,
+    int target, WorkItem* __restrict__ items, int* __restrict__ nitems, int* __restrict__ xs_out) {
+  fallback_items_block<256>(ds_arr, nds, list, count, xs, xs_cap, item_cap, row_cap, target,
+                            items, nitems, xs_out);
 }
 
 // Flag compaction fused with the work list it feeds (flag arrays of at most
-// kScanSmall entries: one workgroup of 512 threads compacts, then its
-// first thread builds the items from the list, which the barrier has made
-// visible to it).
+// kScanSmall entries: one workgroup of 512 threads compacts, then builds the
+// items from the list, which the barrier has made visible to it).
 __global__ __launch_bounds__(512) void flag_sweep_items_kernel(
     const uint32_t* __restrict__ flag, int64_t n, uint32_t* __restrict__ list,
     int* __restrict__ count, const DsDesc* __restrict__ ds_arr, int nds, int qw, int cus,
     double rate, WorkItem* __restrict__ items, int* __restrict__ nitems, int* __restrict__ nrows) {
   scan_small_body<uint32_t, 512, 32>(flag, n, nullptr, nullptr, list, count);
   __syncthreads();
-  if (threadIdx.x == 0)
-    sweep_items_body(ds_arr, nds, list, count, qw, cus, rate, items, nitems, nrows);
+  sweep_items_block<512>(ds_arr, nds, list, count, qw, cus, rate, items, nitems, nrows);
 }
 __global__ __launch_bounds__(512) void flag_fallback_items_kernel(
     const uint32_t* __restrict__ flag, int64_t n, uint32_t* __restrict__ list,
@@ -788,9 +860,8 @@ __global__ __launch_bounds__(512) void flag_fallback_items_kernel(
     int* __restrict__ nitems, int* __restrict__ xs_out) {
   scan_small_body<uint32_t, 512, 32>(flag, n, nullptr, nullptr, list, count);
   __syncthreads();
-  if (threadIdx.x == 0)
-    fallback_items_body(ds_arr, nds, list, count, xs, xs_cap, item_cap, row_cap, target, items,
-                        nitems, xs_out);
+  fallback_items_block<512>(ds_arr, nds, list, count, xs, xs_cap, item_cap, row_cap, target,
+                            items, nitems, xs_out);
 }
 
 }  // namespace hm

@@ -522,7 +522,7 @@ hm_status enqueue_pipeline(hm_job* J) {
                              J->nitems_l2 + 1);
         } else {
           HM_TRY(compact_flags(J, J->flag, R, J->list, J->count));
-          hipLaunchKernelGGL(sweep_items_kernel, dim3(1), dim3(64), 0, J->stream, J->d_desc, nds,
+          hipLaunchKernelGGL(sweep_items_kernel, dim3(1), dim3(256), 0, J->stream, J->d_desc, nds,
                              (const uint32_t*)J->list, (const int*)J->count, sweep_rows(em), cus,
                              rate, J->items_l2, J->nitems_l2, J->nitems_l2 + 1);
         }
@@ -554,7 +554,7 @@ hm_status enqueue_pipeline(hm_job* J) {
                          J->xs_fb);
     } else {
       HM_TRY(compact_flags(J, J->flag, R, J->list, J->count));
-      hipLaunchKernelGGL(fallback_items_kernel, dim3(1), dim3(64), 0, J->stream, J->d_desc, nds,
+      hipLaunchKernelGGL(fallback_items_kernel, dim3(1), dim3(256), 0, J->stream, J->d_desc, nds,
                          (const uint32_t*)J->list, (const int*)J->count, J->xs, xs_cap,
                          (int64_t)J->fb_cap, R, (int)gx, J->items_fb, J->nitems_fb, J->xs_fb);
     }

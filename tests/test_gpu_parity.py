@@ -232,18 +232,20 @@ def test_graph_replay(ctx):
     job.close()
 
 
-def test_f64_calibration_vs_fp64_oracle(ctx):
+# k = 15, 30, 50: lane groups of 16 and 32 (DPP folds) and 64 (permutes)
+@pytest.mark.parametrize("k", [15, 30, 50])
+def test_f64_calibration_vs_fp64_oracle(ctx, k):
     x = DS.c1()
-    job = H.Job(ctx, [1000], [50], [15], ["euclidean"], calib="f64")
+    job = H.Job(ctx, [1000], [50], [k], ["euclidean"], calib="f64")
     job.upload(0, x)
     job.run()
     job.sync()
     r = job.download(0)
     job.close()
-    c64 = O.smooth_knn_f64(r.knn.dists, 15)
+    c64 = O.smooth_knn_f64(r.knn.dists, k)
     rel = np.abs(r.calib.sigmas64 - c64.sigmas) / c64.sigmas
     assert rel.max() < 1e-12
-    ref32 = O.smooth_knn(r.knn.dists, 15)
+    ref32 = O.smooth_knn(r.knn.dists, k)
     # distance of the reference's f32 search from the FP64 root
     assert (np.abs(ref32.sigmas - c64.sigmas) / c64.sigmas).max() < 1e-3
 
