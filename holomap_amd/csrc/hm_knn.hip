@@ -864,4 +864,51 @@ __global__ __launch_bounds__(512) void flag_fallback_items_kernel(
                             items, nitems, xs_out);
 }
 
+// The exact fallback's list when an escalation ran: the rows still flagged
+// are among the escalated rows (the list its flag compaction left), so the
+// list is compacted in place -- chunks of NT * PER entries, each read whole
+// before any of its kept entries moves down -- and the work list built from
+// it. Cheaper than compacting the whole flag array when few rows escalated.
+template <int NT, int PER>
+__device__ void list_compact_block(const uint32_t* __restrict__ flag, uint32_t* __restrict__ list,
+                                   int* __restrict__ count) {
+  __shared__ uint64_t sh[NT / 64];
+  const int cnt = *count;
+  uint64_t base = 0;
+  for (int c0 = 0; c0 < cnt; c0 += NT * PER) {  // uniform
+    uint32_t v[PER];
+    bool keep[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = c0 + (int)threadIdx.x * PER + u;
+      v[u] = i < cnt ? list[i] : 0u;
+    }
+    uint64_t m = 0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = c0 + (int)threadIdx.x * PER + u;
+      keep[u] = i < cnt && flag[v[u]] != 0u;
+      m += keep[u] ? 1u : 0u;
+    }
+    uint64_t tot;
+    uint64_t pos = base + block_excl_scan_u64<NT>(m, tot, sh);  // (its barriers: chunk read)
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+      if (keep[u]) list[pos++] = v[u];
+    base += tot;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *count = (int)base;
+}
+__global__ __launch_bounds__(512) void list_fallback_items_kernel(
+    const uint32_t* __restrict__ flag, uint32_t* __restrict__ list, int* __restrict__ count,
+    const DsDesc* __restrict__ ds_arr, int nds, int xs, int xs_cap, int64_t item_cap,
+    int64_t row_cap, int target, WorkItem* __restrict__ items, int* __restrict__ nitems,
+    int* __restrict__ xs_out) {
+  list_compact_block<512, 8>(flag, list, count);
+  __syncthreads();
+  fallback_items_block<512>(ds_arr, nds, list, count, xs, xs_cap, item_cap, row_cap, target,
+                            items, nitems, xs_out);
+}
+
 }  // namespace hm

@@ -42,6 +42,8 @@ __global__ void flag_sweep_items_kernel(const uint32_t*, int64_t, uint32_t*, int
 __global__ void flag_fallback_items_kernel(const uint32_t*, int64_t, uint32_t*, int*,
                                            const DsDesc*, int, int, int, int64_t, int64_t, int,
                                            WorkItem*, int*, int*);
+__global__ void list_fallback_items_kernel(const uint32_t*, uint32_t*, int*, const DsDesc*, int,
+                                           int, int, int64_t, int64_t, int, WorkItem*, int*, int*);
 __global__ void smooth_knn_kernel(const DsDesc*, int, int64_t, const double*, int, float*, float*,
                                   double*, uint32_t*);
 __global__ void calib_fixup_kernel(const DsDesc*, int, float*, double*, const float*, uint32_t*);
@@ -547,7 +549,13 @@ hm_status enqueue_pipeline(hm_job* J) {
     while (kpow < J->kmax) kpow <<= 1;
     const int G = std::max(1, 1024 / kpow);
     const int xs_cap = std::max(J->xs, std::min(512, G * G));
-    if (R <= kScanSmall<uint32_t>) {  // compaction and items in one launch
+    if (J->mode >= 0 && J->esc_mode >= 0) {
+      // the rows still flagged are among the escalated ones: compact that list
+      hipLaunchKernelGGL(list_fallback_items_kernel, dim3(1), dim3(512), 0, J->stream,
+                         (const uint32_t*)J->flag, J->list, J->count, J->d_desc, nds, J->xs,
+                         xs_cap, (int64_t)J->fb_cap, R, (int)gx, J->items_fb, J->nitems_fb,
+                         J->xs_fb);
+    } else if (R <= kScanSmall<uint32_t>) {  // compaction and items in one launch
       hipLaunchKernelGGL(flag_fallback_items_kernel, dim3(1), dim3(kScanSmallThreads), 0,
                          J->stream, (const uint32_t*)J->flag, R, J->list, J->count, J->d_desc, nds,
                          J->xs, xs_cap, (int64_t)J->fb_cap, R, (int)gx, J->items_fb, J->nitems_fb,
