@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "kNN-graph pts/s BF16+rerank vs FP32 (bit-exact idx); p50/p99 fit latency"
 PEAK = {"bf16": 2.5e15, "bf16x3": 2.5e15, "fp32": 157.3e12}
+HBM_PEAK = 8.0e12  # bytes/s
 
 
 def workload(name):
@@ -281,6 +282,24 @@ def main():
         "fallback_rows": cnt["fallback_rows"],
         "nnz": cnt["nnz"],
     }
+    # achieved bandwidth of the other stages against their algorithmic bytes
+    # (HIP-event stage times of the eager runs; these stages are bound by
+    # dependent memory round trips, not by HBM bandwidth):
+    #  top-k (re-rank): the listed keys (C per row, 8 bytes), the k winners'
+    #    rows and the query row (d floats each), the k outputs (idx + dist);
+    #  calibration: k distances in, sigma, rho and the f64 sigma out;
+    #  symmetrisation: kNN idx + dist and sigma/rho in, the COO (row, col,
+    #    value) and the row pointer out.
+    C = cnt["candidates"]
+    sb = {"topk": (n * (C * 8 + (k + 1) * d * 4 + k * 8), "rerank"),
+          "calibration": (n * (k * 4 + 16), "calib"),
+          "symmetrisation": (n * k * 8 + n * 8 + cnt["nnz"] * 12 + (n + 1) * 8, "fuzzy")}
+    out["stage_bandwidth"] = {}
+    for name, (nbytes, st) in sb.items():
+        t = stage[st] * 1e-3
+        gbs = nbytes / t / 1e9 if t > 0 else 0.0
+        out["stage_bandwidth"][name] = {"bytes": int(nbytes), "ms": round(stage[st], 4),
+                                        "GB/s": round(gbs, 1), "frac_of_hbm": gbs * 1e9 / HBM_PEAK}
     # roofline of the dominant kernel, the candidate sweep (MFMA-bound). It is
     # launched twice per step (threshold pass, collect pass); each launch does
     # the algorithmic 2*N*N*d flops of one Gram matrix. The sweep stage time is
