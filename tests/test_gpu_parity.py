@@ -312,6 +312,48 @@ def test_few_uncertified_rows_use_wide_fallback(ctx):
         assert 1 <= fb <= 201, fb
 
 
+def test_escalation_then_exact_scan_for_the_rest(ctx):
+    # 20 copies of the pattern above in a larger cloud: ~4,000 rows fail the
+    # bf16 bound, too many for the exact scan to be cheaper, so they are
+    # re-swept in bf16x3 and re-ranked; the shell rows are then certified,
+    # while the 20 centres (200 neighbours at distance 1 up to rounding) are
+    # not and go on to the exact scan, which takes its rows from the
+    # escalated-row list compacted in place.
+    rng = np.random.default_rng(23)
+    n, d, k = 30000, 32, 15
+    x = (rng.normal(size=(n, d)) * 3.0 + 10.0).astype(np.float32)
+    centres = []
+    for ci in range(20):
+        b = ci * 201
+        c = (rng.normal(size=d) * 3.0 + 10.0).astype(np.float32)
+        u = rng.normal(size=(200, d))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        x[b] = c
+        x[b + 1:b + 201] = (c + u).astype(np.float32)
+        centres.append(b)
+    out = {}
+    for prec in ["bf16", "exact"]:
+        job = H.Job(ctx, [n], [d], [k], ["euclidean"], precision=prec)
+        job.upload(0, x)
+        job.run()
+        job.sync()
+        out[prec] = (job.download(0), job.counters())
+        job.close()
+    (r, cnt), (e, _) = out["bf16"], out["exact"]
+    assert cnt["escalated_rows"] >= 1000, cnt
+    assert 1 <= cnt["fallback_rows"] < cnt["escalated_rows"], cnt
+    assert r.knn.indices.tobytes() == e.knn.indices.tobytes()
+    assert r.knn.dists.tobytes() == e.knn.dists.tobytes()
+    assert r.calib.sigmas.tobytes() == e.calib.sigmas.tobytes()
+    assert r.graph.cols.tobytes() == e.graph.cols.tobytes()
+    assert r.graph.vals.tobytes() == e.graph.vals.tobytes()
+    rows = np.unique(np.concatenate([np.array(centres), centres[3] + np.arange(1, 40),
+                                     rng.choice(n, 150, replace=False)])).astype(np.uint32)
+    sub = O.exact_knn_rows(x, d, k, "euclidean", rows, nthreads=8)
+    assert np.array_equal(r.knn.indices[rows], sub.indices)
+    assert np.array_equal(r.knn.dists[rows].view(np.uint32), sub.dists.view(np.uint32))
+
+
 def test_c4_batch_matches_alone_and_oracle(ctx):
     # 48 datasets of the C4 mix (N log-uniform in [200, 8000], d in
     # {32, 64, 128, 256}, k in {10, 15, 30}, euclidean/cosine) in one job:
