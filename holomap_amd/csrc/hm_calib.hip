@@ -91,10 +91,28 @@ __device__ __forceinline__ int calib_find_ds(const DsDesc* ds, int nds, int64_t 
 
 // target_log2k[ds] is log2(k) computed on the host (glibc), as the
 // reference's std f64::log2 does.
-__global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
-                                  const double* __restrict__ target_log2k, int mode,
-                                  float* __restrict__ sigma, float* __restrict__ rho,
-                                  double* __restrict__ sigma64, uint32_t* __restrict__ zany) {
+//
+// Thread per row. KM > 0: the row's k <= KM distances are held in registers
+// (loops unrolled to KM with a t < k guard), for jobs with enough rows that
+// the VALU work, not latency, bounds the calibration: a lane evaluates all
+// k - 1 terms of its row, where the lane-group kernel spends a lane per term
+// and repeats the bracket updates in every lane. KM = 0: rows of any k, read
+// from memory.
+#define HM_ROW_LOOP(T0, BODY)                                  \
+  do {                                                         \
+    if constexpr (KM > 0) {                                    \
+      _Pragma("unroll") for (int t = (T0); t < KM; ++t) {      \
+        if (t < k) { BODY }                                    \
+      }                                                        \
+    } else {                                                   \
+      for (int t = (T0); t < k; ++t) { BODY }                  \
+    }                                                          \
+  } while (0)
+template <int KM>
+__global__ __launch_bounds__(256) void smooth_knn_kernel(
+    const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
+    const double* __restrict__ target_log2k, int mode, float* __restrict__ sigma,
+    float* __restrict__ rho, double* __restrict__ sigma64, uint32_t* __restrict__ zany) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total_rows) return;
   const int di = calib_find_ds(ds, nds, g);
@@ -103,10 +121,23 @@ __global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_
   const int k = D.k;
   const float* row = D.knn_dist + i * k;
   const double target = target_log2k[di];
+  float xr[KM > 0 ? KM : 1];
+  if constexpr (KM > 0) {
+#pragma unroll
+    for (int t = 0; t < KM; ++t) xr[t] = t < k ? row[t] : 0.0f;
+  }
+#define X(t) (KM > 0 ? xr[(KM > 0 ? (t) : 0)] : row[t])
 
+  // rho: the first positive distance in row order
   float r = 0.0f;
-  for (int t = 0; t < k; ++t) {
-    if (row[t] > 0.0f) { r = row[t]; break; }
+  if constexpr (KM > 0) {
+#pragma unroll
+    for (int t = KM - 1; t >= 0; --t)
+      if (t < k && xr[t] > 0.0f) r = xr[t];
+  } else {
+    for (int t = 0; t < k; ++t) {
+      if (row[t] > 0.0f) { r = row[t]; break; }
+    }
   }
   rho[g] = r;
   if (mode == CALIB_REF_F32) {
@@ -114,13 +145,13 @@ __global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_
     float lo = 0.0f, hi = fmax, mid = 1.0f;
     for (int it = 0; it < 64; ++it) {
       float psum = 0.0f;
-      for (int t = 1; t < k; ++t) {
-        const float dd = row[t] - r;
+      HM_ROW_LOOP(1, {
+        const float dd = X(t) - r;
         if (dd > 0.0f)
           psum += hm_expf(-(dd / mid));
         else
           psum += 1.0f;
-      }
+      });
       if (fabs((double)psum - target) < 1e-5) break;
       if ((double)psum > target) {
         hi = mid;
@@ -136,7 +167,7 @@ __global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_
     float s = mid;
     if (r > 0.0f) {
       double acc = 0.0;
-      for (int t = 0; t < k; ++t) acc = acc + (double)row[t];
+      HM_ROW_LOOP(0, { acc = acc + (double)X(t); });
       const float fl = 1e-3f * (float)(acc / (double)k);
       if (s < fl) s = fl;
     } else {
@@ -149,10 +180,10 @@ __global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_
     double lo = 0.0, hi = INFINITY, mid = 1.0;
     for (int it = 0; it < 2200; ++it) {
       double psum = 0.0;
-      for (int t = 1; t < k; ++t) {
-        const double dd = (double)row[t] - rr;
+      HM_ROW_LOOP(1, {
+        const double dd = (double)X(t) - rr;
         psum += dd > 0.0 ? exp(-(dd / mid)) : 1.0;
-      }
+      });
       if (psum == target) break;
       double nmid;
       if (psum > target) {
@@ -168,7 +199,7 @@ __global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_
     double s = mid;
     if (r > 0.0f) {
       double acc = 0.0;
-      for (int t = 0; t < k; ++t) acc += (double)row[t];
+      HM_ROW_LOOP(0, { acc += (double)X(t); });
       const double fl = 1e-3 * (acc / (double)k);
       if (s < fl) s = fl;
     } else {
@@ -177,7 +208,17 @@ __global__ void smooth_knn_kernel(const DsDesc* __restrict__ ds, int nds, int64_
     if (sigma64) sigma64[g] = s;
     sigma[g] = (float)s;
   }
+#undef X
 }
+#undef HM_ROW_LOOP
+template __global__ void smooth_knn_kernel<0>(const DsDesc*, int, int64_t, const double*, int,
+                                              float*, float*, double*, uint32_t*);
+template __global__ void smooth_knn_kernel<16>(const DsDesc*, int, int64_t, const double*, int,
+                                               float*, float*, double*, uint32_t*);
+template __global__ void smooth_knn_kernel<32>(const DsDesc*, int, int64_t, const double*, int,
+                                               float*, float*, double*, uint32_t*);
+template __global__ void smooth_knn_kernel<64>(const DsDesc*, int, int64_t, const double*, int,
+                                               float*, float*, double*, uint32_t*);
 
 // Same calibration with G lanes per row (k <= G <= 64): lane t holds the
 // row's distance t and evaluates term t of every bisection step (the exp

@@ -59,6 +59,9 @@ constexpr int kQresRowBytes = 384;
 constexpr int kQresPairRowBytes = 256;
 // Workgroups per dataset of the per-dataset statistics maxima (hm_prep.hip).
 constexpr int kStatParts = 32;
+// Calibration: a thread per row (rather than a lane group per row) for jobs
+// of at least this many rows per CU (four waves per SIMD).
+constexpr int kCalibThreadRowsPerCu = 4 * 4 * 64;
 
 // NW waves (32 queries each), NSTAGE ring slots, rows of up to QMAXB bytes.
 template <int NW_, int NSTAGE_, int QMAXB_>

@@ -44,6 +44,7 @@ __global__ void flag_fallback_items_kernel(const uint32_t*, int64_t, uint32_t*, 
                                            WorkItem*, int*, int*);
 __global__ void list_fallback_items_kernel(const uint32_t*, uint32_t*, int*, const DsDesc*, int,
                                            int, int, int64_t, int64_t, int, WorkItem*, int*, int*);
+template <int KM>
 __global__ void smooth_knn_kernel(const DsDesc*, int, int64_t, const double*, int, float*, float*,
                                   double*, uint32_t*);
 __global__ void calib_fixup_kernel(const DsDesc*, int, float*, double*, const float*, uint32_t*);
@@ -582,20 +583,35 @@ hm_status enqueue_pipeline(hm_job* J) {
     for (int i = 1; i <= 4; ++i) mark(J, i);
   }
   if (J->stages & HM_STAGE_CALIB) {
-    // G lanes per row when the largest k fits a group, else a thread per row
+    // G lanes per row when the largest k fits a group, else a thread per row.
+    // Jobs with enough rows for four waves per SIMD at a thread per row are
+    // bound by VALU work rather than latency: a thread per row then, its
+    // distances in registers (a lane evaluates every term of its row instead
+    // of one).
     const int G = J->kmax <= 16 ? 16 : J->kmax <= 32 ? 32 : J->kmax <= 64 ? 64 : 0;
+    const bool by_thread = G == 0 || R >= kCalibThreadRowsPerCu * (int64_t)cus;
 #define HM_CALIB_G(GS)                                                                         \
   hipLaunchKernelGGL((smooth_knn_group_kernel<GS>), dim3(grid_for(R * GS, 256)), dim3(256), 0, \
                      J->stream, J->d_desc, nds, R, (const double*)J->target, J->opt.calib,     \
                      J->sigma, J->rho, J->sigma64, J->zany)
-    if (G == 16) HM_CALIB_G(16);
-    else if (G == 32) HM_CALIB_G(32);
-    else if (G == 64) HM_CALIB_G(64);
-    else
-      hipLaunchKernelGGL(smooth_knn_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream,
-                         J->d_desc, nds, R, (const double*)J->target, J->opt.calib, J->sigma,
-                         J->rho, J->sigma64, J->zany);
+#define HM_CALIB_T(KM)                                                                         \
+  hipLaunchKernelGGL((smooth_knn_kernel<KM>), dim3(grid_for(R, 256)), dim3(256), 0, J->stream, \
+                     J->d_desc, nds, R, (const double*)J->target, J->opt.calib, J->sigma,       \
+                     J->rho, J->sigma64, J->zany)
+    if (by_thread) {
+      if (G == 16) HM_CALIB_T(16);
+      else if (G == 32) HM_CALIB_T(32);
+      else if (G == 64) HM_CALIB_T(64);
+      else HM_CALIB_T(0);
+    } else if (G == 16) {
+      HM_CALIB_G(16);
+    } else if (G == 32) {
+      HM_CALIB_G(32);
+    } else {
+      HM_CALIB_G(64);
+    }
 #undef HM_CALIB_G
+#undef HM_CALIB_T
     hipLaunchKernelGGL(calib_fixup_kernel, dim3(nds), dim3(256), 0, J->stream, J->d_desc,
                        J->opt.calib, J->sigma, J->sigma64, (const float*)J->rho, J->zany);
     HM_LAUNCH(ctx);

@@ -354,6 +354,46 @@ def test_escalation_then_exact_scan_for_the_rest(ctx):
     assert np.array_equal(r.knn.dists[rows].view(np.uint32), sub.dists.view(np.uint32))
 
 
+@pytest.mark.parametrize("calib", ["ref_f32", "f64"])
+def test_calibration_by_thread_matches_lane_groups(ctx, calib):
+    # A job of >= 1024 rows per CU calibrates with a thread per row (its
+    # distances in registers); each dataset alone is small enough for the
+    # lane-group kernel. Both fold the same terms in the same order.
+    rng = np.random.default_rng(31)
+    ks = [10, 15, 30, 15]
+    specs = [(7000, 8, ks[i % 4], "euclidean" if i % 3 else "cosine") for i in range(40)]
+    assert sum(s[0] for s in specs) >= 1024 * 256
+    data = []
+    for n, d, _, _ in specs:
+        c = rng.normal(size=(12, d)) * 4.0
+        data.append((c[rng.integers(0, 12, n)] + rng.normal(size=(n, d))).astype(np.float32))
+    data[5][:40] = data[5][3]  # rho = 0 rows: the floor
+    job = H.Job(ctx, [s[0] for s in specs], [s[1] for s in specs], [s[2] for s in specs],
+                [s[3] for s in specs], calib=calib)
+    for i, x in enumerate(data):
+        job.upload(i, x)
+    job.run()
+    job.sync()
+    got = [job.download(i, with_sigma64=True) for i in range(len(specs))]
+    job.close()
+    for i, (n, d, k, m) in enumerate(specs):
+        a = H.Job(ctx, [n], [d], [k], [m], calib=calib)
+        a.upload(0, data[i])
+        a.run()
+        a.sync()
+        alone = a.download(0, with_sigma64=True)
+        a.close()
+        r = got[i]
+        assert r.knn.indices.tobytes() == alone.knn.indices.tobytes(), i
+        assert r.calib.rhos.tobytes() == alone.calib.rhos.tobytes(), i
+        assert r.calib.sigmas.tobytes() == alone.calib.sigmas.tobytes(), i
+        assert r.calib.sigmas64.tobytes() == alone.calib.sigmas64.tobytes(), i
+        assert r.graph.vals.tobytes() == alone.graph.vals.tobytes(), i
+    if calib == "ref_f32":
+        n, d, k, m = specs[5]
+        assert_graph_equal(got[5], *O.knn_graph(data[5], d, k, m))
+
+
 def test_c4_batch_matches_alone_and_oracle(ctx):
     # 48 datasets of the C4 mix (N log-uniform in [200, 8000], d in
     # {32, 64, 128, 256}, k in {10, 15, 30}, euclidean/cosine) in one job:
