@@ -530,3 +530,110 @@ def test_autotune_keeps_outputs(ctx):
         job.sync()
         assert_graph_equal(job.download(0), *ref)
     job.close()
+
+
+def oracle_graph_mt(x, d, k, metric, nthreads=8):
+    """The oracle's graph with its kNN rows computed on several threads
+    (rows are independent: the same bits as one thread)."""
+    n = x.size // d
+    knn = O.exact_knn_rows(x, d, k, metric, np.arange(n, dtype=np.uint32), nthreads=nthreads)
+    cal = O.smooth_knn(knn.dists, k)
+    return knn, cal, O.fuzzy_simplicial_set(knn, cal)
+
+
+# rows longer than 512 floats: the re-rank reads the query row from global
+# memory instead of staging it in LDS
+@pytest.mark.parametrize("shape", [(1500, 513, 15), (2000, 640, 30), (2500, 784, 15)])
+@pytest.mark.parametrize("metric", ["euclidean", "cosine"])
+def test_long_rows_rerank(ctx, shape, metric):
+    n, d, k = shape
+    rng = np.random.default_rng(d)
+    x = DS.gaussian_mixture(n, d, 12, 1.5, rng)
+    x[7] = x[3]  # an exact duplicate: the index tie-break
+    ref = oracle_graph_mt(x, d, k, metric)
+    for prec in ["bf16", "fp32", "bf16x3"]:
+        res = H.knn_graph(x, d, k, metric, precision=prec, ctx=ctx)
+        assert_graph_equal(res, *ref)
+
+
+# finite inputs whose squares leave the f32 range: |x| ~ 1e19 overflows the
+# f32 keys and bias (||x||^2 ~ 1e40), |x| ~ 1e-20 underflows the products;
+# the reference computes both in f64 (its distances stay finite)
+@pytest.mark.parametrize("scale", [1e19, 1e-20, "mixed"])
+@pytest.mark.parametrize("metric", ["euclidean", "cosine"])
+def test_extreme_magnitudes(ctx, scale, metric):
+    rng = np.random.default_rng(17)
+    n, d, k = 1200, 24, 15
+    x = DS.gaussian_mixture(n, d, 8, 3.0, rng).astype(np.float64)
+    if scale == "mixed":
+        x[:60] *= 1e19
+        x[60:120] *= 1e-20
+    else:
+        x *= scale
+    x = x.astype(np.float32)
+    assert np.isfinite(x).all()
+    ref = oracle_graph_mt(x, d, k, metric)
+    for prec in ["bf16", "fp32"]:
+        res = H.knn_graph(x, d, k, metric, precision=prec, ctx=ctx)
+        assert_graph_equal(res, *ref)
+
+
+def _knn_graph_like_rust_shim(ctx, x, d, k, metric):
+    """hm_knn_graph called as the Rust shim calls it: host buffers sized as
+    the reference's Vecs, cap = 2nk, no row pointer, no FP64 sigmas,
+    default options."""
+    from holomap_amd import _lib
+    L = _lib.load()
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    n = x.size // d
+    idx = np.empty(n * k, np.uint32)
+    dist = np.empty(n * k, np.float32)
+    sig = np.empty(n, np.float32)
+    rho = np.empty(n, np.float32)
+    cap = 2 * n * k
+    rows = np.empty(cap, np.uint32)
+    cols = np.empty(cap, np.uint32)
+    vals = np.empty(cap, np.float32)
+    opt = _lib.Options()
+    L.hm_default_options(ctypes.byref(opt))
+    o = _lib.GraphOut()
+    o.indices, o.dists = idx.ctypes.data, dist.ctypes.data
+    o.sigmas, o.rhos = sig.ctypes.data, rho.ctypes.data
+    o.rows, o.cols, o.vals = rows.ctypes.data, cols.ctypes.data, vals.ctypes.data
+    o.indptr = None
+    o.sigmas64 = None
+    o.cap = cap
+    m = _lib.HM_EUCLIDEAN if metric == "euclidean" else _lib.HM_COSINE
+    st = L.hm_knn_graph(ctx.handle, x.ctypes.data, n, d, k, m, ctypes.byref(opt), ctypes.byref(o))
+    nnz = int(o.nnz)
+    return st, (idx.reshape(n, k), dist.reshape(n, k), sig, rho, rows[:nnz], cols[:nnz],
+                vals[:nnz])
+
+
+def test_hm_knn_graph_c_abi_as_the_rust_shim(ctx):
+    cases = []
+    for name in FIXTURES:
+        fx, data = load_fixture(name)
+        cases.append((data, fx["n_features"], fx["n_neighbors"], fx["metric"]))
+    cases.append((DS.c1(), 50, 15, "euclidean"))
+    cases.append((DS.c1(), 50, 15, "cosine"))
+    for data, d, k, metric in cases:
+        st, (idx, dist, sig, rho, rows, cols, vals) = _knn_graph_like_rust_shim(ctx, data, d, k,
+                                                                                metric)
+        assert st == 0, ctx.last_error()
+        knn, cal, g = O.knn_graph(data, d, k, metric)
+        assert np.array_equal(idx, knn.indices)
+        assert np.array_equal(dist.view(np.uint32), knn.dists.view(np.uint32))
+        assert np.array_equal(sig.view(np.uint32), cal.sigmas.view(np.uint32))
+        assert np.array_equal(rho.view(np.uint32), cal.rhos.view(np.uint32))
+        assert np.array_equal(rows, g.rows) and np.array_equal(cols, g.cols)
+        assert np.array_equal(vals.view(np.uint32), g.vals.view(np.uint32))
+    # the failure path the shim reports: a status and hm_last_error's message
+    from holomap_amd import _lib
+    x = np.ones((10, 3), np.float32)
+    st, _ = _knn_graph_like_rust_shim(ctx, x, 3, 10, "euclidean")
+    assert st == _lib.HM_ETOOFEW
+    assert ctx.last_error()
+    x[4, 1] = np.inf
+    st, _ = _knn_graph_like_rust_shim(ctx, x, 3, 5, "euclidean")
+    assert st == _lib.HM_ENONFINITE
