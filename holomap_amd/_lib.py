@@ -20,6 +20,7 @@ HM_EDEVICE = -5
 HM_ENOMEM = -6
 HM_EBUDGET = -7
 HM_EUNSUPPORTED = -8
+HM_MAX_NEIGHBORS = 1024
 
 HM_EUCLIDEAN = 0
 HM_COSINE = 1
@@ -102,7 +103,7 @@ def load():
         "hm_job_create": (i32, [vp, u32, vp, vp, vp, vp, P(Options), u32, P(vp)]),
         "hm_job_destroy": (None, [vp]),
         "hm_job_bytes": (u64, [vp]),
-        "hm_job_estimate_bytes": (u64, [u32, vp, vp, vp, P(Options)]),
+        "hm_job_estimate_bytes": (u64, [u32, vp, vp, vp, vp, P(Options)]),
         "hm_job_upload": (i32, [vp, u32, vp]),
         "hm_job_upload_device": (i32, [vp, u32, vp]),
         "hm_job_set_knn": (i32, [vp, u32, vp, vp]),

@@ -7,8 +7,9 @@
 // in-edges i -> r, sources ascending, at positions [lo_r, hi_r) of the
 // sorted keys. A_r: the row's kept out-edge columns (at most k), I_r: its
 // in-edge sources.
-//   union_count_kernel (a group of G >= k lanes per row, lane t = out-edge
-//     slot t): lo_r / hi_r by binary search; for each out-edge its count of
+//   union_count_kernel (a group of G lanes per row, lane t = out-edge slot t;
+//     k > 64 rows: lane t takes slots t, t + 64, ...): lo_r / hi_r by binary
+//     search; for each out-edge its count of
 //     smaller in-edge sources and its reverse weight; the union size
 //     |A_r| + |I_r| - |A_r n I_r|;
 //   (exclusive scan of the sizes -> global row pointer)
@@ -102,14 +103,50 @@ __global__ __launch_bounds__(256) void union_count_kernel(
   const int t = (int)(threadIdx.x % G);
   if (r >= total_rows) return;  // whole groups leave together
   const DsDesc& D = ds[ds_of_row(ds, nds, r)];
-  const int k = D.k;  // <= G
+  const int k = D.k;
   const int64_t e0 = D.edge_off + (r - D.row_off) * k;
-  const uint32_t ca = t < k ? key1[e0 + t] : sentinel;
   // lane 0 finds lo, lane 1 hi; the group reads both
   const int64_t b = t == 1 ? lower_bound_u32(k1s, total_edges, (uint32_t)(r + 1))
                            : lower_bound_u32(k1s, total_edges, (uint32_t)r);
   const int64_t lo = (uint32_t)__shfl((int)(uint32_t)b, 0, G);
   const int64_t hi = (uint32_t)__shfl((int)(uint32_t)b, 1, G);
+  if (k > G) {
+    // more out-edges than lanes (k > 64, uniform over the group): lane t
+    // takes slots t, t + G, ..., each by binary search of the in-edge sources
+    uint32_t na = 0, nm = 0;
+    for (int s = t; s < k; s += G) {
+      const uint32_t c = key1[e0 + s];
+      uint32_t less = 0, fbit = 0;
+      float vt = 0.0f;
+      if (c != sentinel) {
+        int64_t a = lo, z = hi;  // first source >= c
+        while (a < z) {
+          const int64_t mid = (a + z) >> 1;
+          if (srcrow[p1s[mid]] < c) a = mid + 1; else z = mid;
+        }
+        less = (uint32_t)(a - lo);
+        if (a < hi) {
+          const uint32_t e = p1s[a];
+          if (srcrow[e] == c) { fbit = 0x80000000u; vt = wval[e]; ++nm; }
+        }
+        ++na;
+      }
+      ai[e0 + s] = less | fbit;
+      av[e0 + s] = vt;
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+      na += __shfl_xor(na, o, G);
+      nm += __shfl_xor(nm, o, G);
+    }
+    if (t == 0) {
+      inptr[r] = (uint32_t)lo;
+      if (r == total_rows - 1) inptr[total_rows] = (uint32_t)hi;
+      cnt[r] = (uint64_t)na + (uint64_t)(hi - lo) - nm;
+    }
+    return;
+  }
+  const uint32_t ca = t < k ? key1[e0 + t] : sentinel;
   ScanIn si{ca, 0u, false, 0.0f};
   if (hi - lo <= 4 * G) {
     for (int64_t p0 = lo; p0 < hi; p0 += G) {
@@ -179,19 +216,41 @@ __global__ __launch_bounds__(256) void union_write_kernel(
     const int k = D.k;
     const int64_t roff = D.row_off;
     const int64_t e0 = D.edge_off + (r - roff) * k;
-    const uint32_t ca = t < k ? key1[e0 + t] : sentinel;
-    const float va = t < k ? wval[e0 + t] : 0.0f;
-    const uint32_t a = t < k ? ai[e0 + t] : 0u;
-    const float vt = t < k ? av[e0 + t] : 0.0f;
-    // rank among the row's out-edges, and mutual columns below
-    ScanOut so{ca, 0u, 0u, false};
-    grp_for<G>(ca, a >> 31, so);
     const uint64_t base = indptr_g[r];
-    if (ca != sentinel) {
-      const uint64_t pos = base + so.less + (a & 0x7fffffffu) - so.mut;
-      rows[pos] = (uint32_t)(r - roff);
-      cols[pos] = (uint32_t)((int64_t)ca - roff);
-      vals[pos] = (va + vt) - va * vt;  // f32, the reference's operand order
+    if (k > G) {
+      // more out-edges than lanes: lane t places slots t, t + G, ..., each
+      // ranked against the row's k columns read from memory
+      for (int s = t; s < k; s += G) {
+        const uint32_t ca = key1[e0 + s];
+        if (ca == sentinel) continue;
+        const float va = wval[e0 + s];
+        const uint32_t a = ai[e0 + s];
+        const float vt = av[e0 + s];
+        uint32_t less = 0, mut = 0;
+        for (int u = 0; u < k; ++u) {
+          const uint32_t cu = key1[e0 + u];
+          less += cu < ca ? 1u : 0u;
+          mut += ((ai[e0 + u] >> 31) != 0u && cu < ca) ? 1u : 0u;
+        }
+        const uint64_t pos = base + less + (a & 0x7fffffffu) - mut;
+        rows[pos] = (uint32_t)(r - roff);
+        cols[pos] = (uint32_t)((int64_t)ca - roff);
+        vals[pos] = (va + vt) - va * vt;  // f32, the reference's operand order
+      }
+    } else {
+      const uint32_t ca = t < k ? key1[e0 + t] : sentinel;
+      const float va = t < k ? wval[e0 + t] : 0.0f;
+      const uint32_t a = t < k ? ai[e0 + t] : 0u;
+      const float vt = t < k ? av[e0 + t] : 0.0f;
+      // rank among the row's out-edges, and mutual columns below
+      ScanOut so{ca, 0u, 0u, false};
+      grp_for<G>(ca, a >> 31, so);
+      if (ca != sentinel) {
+        const uint64_t pos = base + so.less + (a & 0x7fffffffu) - so.mut;
+        rows[pos] = (uint32_t)(r - roff);
+        cols[pos] = (uint32_t)((int64_t)ca - roff);
+        vals[pos] = (va + vt) - va * vt;  // f32, the reference's operand order
+      }
     }
     if (t == 0) {
       // dataset di owns indptr_l[iptr_off[di] .. + n]

@@ -57,6 +57,14 @@ __host__ __device__ constexpr int sweep_threads(int mode) { return sweep_rows(mo
 constexpr int kQresRowBytes = 384;
 // ... and up to this length by two 128-query workgroups per CU instead.
 constexpr int kQresPairRowBytes = 256;
+// Neighbour counts: the MFMA candidate sweep serves k <= kSweepMaxK (more
+// neighbours than its 128 selection groups per row can certify are left to
+// the exact scan); the exact scan and its merges serve k <= kMaxK.
+constexpr int kSweepMaxK = 64;
+constexpr int kMaxK = 1024;
+// Queries per wave of the exact-scan shape for a job whose largest k is k
+// (exact_scan_kernel<QPW, QCAP>, QCAP >= k + 64; items hold 4 * QPW queries).
+__host__ __device__ constexpr int xs_qpw_for_k(int k) { return k <= 64 ? 8 : (k <= 448 ? 4 : 1); }
 // Workgroups per dataset of the per-dataset statistics maxima (hm_prep.hip).
 constexpr int kStatParts = 32;
 // Calibration: a thread per row (rather than a lane group per row) for jobs

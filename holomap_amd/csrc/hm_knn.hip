@@ -78,26 +78,32 @@ __device__ __forceinline__ void sort128(uint64_t& v0, uint64_t& v1) {
 }
 
 // ---------------------------------------------------------------------------
-// exact_scan_kernel
-// Work item: (ds, split, qbeg, qcnt<=32). Queries are qlist[qbeg + r] (global
-// row ids) or, without a list, local rows qbeg + r. Output: for query slot p
-// (= list position, or local row in identity mode) the split's k smallest
-// keys, sorted, at xpart[(p*xs + split)*kmax ...].
+// exact_scan_kernel<QPW, QCAP>
+// Work item: (ds, split, qbeg, qcnt <= 4*QPW). Queries are qlist[qbeg + r]
+// (global row ids) or, without a list, local rows qbeg + r. Output: for
+// query slot p (= list position, or local row in identity mode) the split's
+// k smallest keys, sorted, at xpart[(p*xs + split)*kmax ...].
+// Each of the four waves owns QPW queries and keeps, per query, a buffer of
+// QCAP keys below the running k-th smallest (tau): a step appends at most
+// 64 keys, and a full buffer is sorted down to its k smallest, so
+// QCAP >= k + 64. Three shapes cover k <= 64 (registers sort), k <= 448 and
+// k <= 1024 (LDS sorts).
 // ---------------------------------------------------------------------------
-#define XS_QB 32   // queries per work item
 #define XS_CB 64   // candidate columns per step
 #define XS_KF 64   // features per LDS chunk
-#define XS_QCAP 128
 
+template <int QPW, int QCAP>
 __global__ __launch_bounds__(256) void exact_scan_kernel(
     const DsDesc* __restrict__ ds_arr, const WorkItem* __restrict__ items,
     const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist,
     const int* __restrict__ xs_dev, int xs_host, int kmax, uint64_t* __restrict__ xpart) {
-  __shared__ double qf[XS_QB][XS_KF];
+  constexpr int QB = 4 * QPW;
+  static_assert(QCAP >= 128 && (QCAP & (QCAP - 1)) == 0, "buffer");
+  __shared__ double qf[QB][XS_KF];
   __shared__ float cf[XS_CB][XS_KF + 1];
-  __shared__ uint64_t buf[XS_QB][XS_QCAP];
-  __shared__ int64_t qrow_s[XS_QB];
-  __shared__ double qrn_s[XS_QB];
+  __shared__ uint64_t buf[QB][QCAP];
+  __shared__ int64_t qrow_s[QB];
+  __shared__ double qrn_s[QB];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nitems = nitems_dev ? *nitems_dev : nitems_host;
@@ -107,13 +113,13 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(
     const DsDesc& D = ds_arr[W.ds];
     const int64_t n = D.n;
     const int kp = D.kp, d = D.d, k = D.k, metric = D.metric;
-    // waves whose 8 query slots are all empty skip the arithmetic
-    const bool wact = wave * 8 < W.qcnt;
+    // waves whose query slots are all empty skip the arithmetic
+    const bool wact = wave * QPW < W.qcnt;
     const int64_t span = ((n + (int64_t)xs * XS_CB - 1) / ((int64_t)xs * XS_CB)) * XS_CB;
     const int64_t cbeg = (int64_t)W.split * span;
     const int64_t cend = cbeg + span < n ? cbeg + span : n;
     __syncthreads();  // previous item fully done with the shared arrays
-    if (tid < XS_QB) {
+    if (tid < QB) {
       int64_t row = -1;
       if (tid < W.qcnt) {
         const int p = W.qbeg + tid;
@@ -123,21 +129,21 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(
       qrn_s[tid] = (row >= 0 && metric == 1) ? D.rnorm[row] : 0.0;
     }
     __syncthreads();
-    uint64_t tau[8];
-    int cnt[8];
-    int64_t myq[8];
-    double myrn[8];
+    uint64_t tau[QPW];
+    int cnt[QPW];
+    int64_t myq[QPW];
+    double myrn[QPW];
 #pragma unroll
-    for (int qq = 0; qq < 8; ++qq) {
+    for (int qq = 0; qq < QPW; ++qq) {
       tau[qq] = ~0ull;
       cnt[qq] = 0;
-      myq[qq] = qrow_s[wave * 8 + qq];
-      myrn[qq] = qrn_s[wave * 8 + qq];
+      myq[qq] = qrow_s[wave * QPW + qq];
+      myrn[qq] = qrn_s[wave * QPW + qq];
     }
     for (int64_t cb = cbeg; cb < cend; cb += XS_CB) {
-      double acc[8];
+      double acc[QPW];
 #pragma unroll
-      for (int qq = 0; qq < 8; ++qq) acc[qq] = 0.0;
+      for (int qq = 0; qq < QPW; ++qq) acc[qq] = 0.0;
       for (int f0 = 0; f0 < d; f0 += XS_KF) {
         __syncthreads();
         {  // candidates: 64 rows x 64 features, 16 per thread
@@ -153,18 +159,16 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(
             cf[r][c + e + 3] = v.w;
           }
         }
-        {  // queries: 32 rows x 64 features as f64, 8 per thread
-          const int r = tid >> 3, c = (tid & 7) * 8;
+        // queries: QB rows x 64 features as f64, a float4 per thread and step
+        for (int e = tid; e < QB * (XS_KF / 4); e += 256) {
+          const int r = e / (XS_KF / 4), c = (e % (XS_KF / 4)) * 4;
           const int64_t row = qrow_s[r];
-#pragma unroll
-          for (int e = 0; e < 8; e += 4) {
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (row >= 0 && f0 + c + e < kp) v = *(const float4*)(D.xr + (size_t)row * kp + f0 + c + e);
-            qf[r][c + e] = (double)v.x;
-            qf[r][c + e + 1] = (double)v.y;
-            qf[r][c + e + 2] = (double)v.z;
-            qf[r][c + e + 3] = (double)v.w;
-          }
+          float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (row >= 0 && f0 + c < kp) v = *(const float4*)(D.xr + (size_t)row * kp + f0 + c);
+          qf[r][c] = (double)v.x;
+          qf[r][c + 1] = (double)v.y;
+          qf[r][c + 2] = (double)v.z;
+          qf[r][c + 3] = (double)v.w;
         }
         __syncthreads();
         const int nf = d - f0 < XS_KF ? d - f0 : XS_KF;
@@ -173,8 +177,8 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(
           for (int f = 0; f < nf; ++f) {
             const double b = (double)cf[lane][f];
 #pragma unroll
-            for (int qq = 0; qq < 8; ++qq) {
-              const double t = qf[wave * 8 + qq][f] - b;
+            for (int qq = 0; qq < QPW; ++qq) {
+              const double t = qf[wave * QPW + qq][f] - b;
               acc[qq] = acc[qq] + t * t;
             }
           }
@@ -182,15 +186,15 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(
           for (int f = 0; f < nf; ++f) {
             const double b = (double)cf[lane][f];
 #pragma unroll
-            for (int qq = 0; qq < 8; ++qq) acc[qq] = acc[qq] + qf[wave * 8 + qq][f] * b;
+            for (int qq = 0; qq < QPW; ++qq) acc[qq] = acc[qq] + qf[wave * QPW + qq][f] * b;
           }
         }
       }
       const int64_t j = cb + lane;
       const double rnc = (metric == 1 && j < n) ? D.rnorm[j] : 0.0;
 #pragma unroll
-      for (int qq = 0; qq < 8; ++qq) {
-        const int slot = wave * 8 + qq;
+      for (int qq = 0; qq < QPW; ++qq) {
+        const int slot = wave * QPW + qq;
         if (myq[qq] < 0) continue;  // uniform
         float dist;
         if (metric == 0) dist = (float)sqrt(acc[qq]);
@@ -200,13 +204,21 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(
         bool hit = (j < cend) && key < tau[qq];
         uint64_t mask = __ballot(hit);
         if (mask == 0ull) continue;
-        if (cnt[qq] + __popcll(mask) > XS_QCAP) {
-          uint64_t v0 = lane < cnt[qq] ? buf[slot][lane] : ~0ull;
-          uint64_t v1 = lane + 64 < cnt[qq] ? buf[slot][lane + 64] : ~0ull;
-          sort128(v0, v1);
-          if (lane < k) buf[slot][lane] = v0;
-          if (lane + 64 < k) buf[slot][lane + 64] = v1;
-          tau[qq] = (k - 1) < 64 ? readlane_u64(v0, k - 1) : readlane_u64(v1, k - 65);
+        if (cnt[qq] + __popcll(mask) > QCAP) {
+          // full: keep the k smallest, tau = the k-th
+          if constexpr (QCAP == 128) {
+            uint64_t v0 = lane < cnt[qq] ? buf[slot][lane] : ~0ull;
+            uint64_t v1 = lane + 64 < cnt[qq] ? buf[slot][lane + 64] : ~0ull;
+            sort128(v0, v1);
+            if (lane < k) buf[slot][lane] = v0;
+            if (lane + 64 < k) buf[slot][lane + 64] = v1;
+            tau[qq] = (k - 1) < 64 ? readlane_u64(v0, k - 1) : readlane_u64(v1, k - 65);
+          } else {
+            for (int e = cnt[qq] + lane; e < QCAP; e += 64) buf[slot][e] = ~0ull;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            wave_sort_lds(buf[slot], QCAP);
+            tau[qq] = buf[slot][k - 1];
+          }
           cnt[qq] = k;
           hit = (j < cend) && key < tau[qq];
           mask = __ballot(hit);
@@ -218,19 +230,36 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(
     }
     // final: sort each row's buffer, emit its k smallest
 #pragma unroll
-    for (int qq = 0; qq < 8; ++qq) {
-      const int slot = wave * 8 + qq;
+    for (int qq = 0; qq < QPW; ++qq) {
+      const int slot = wave * QPW + qq;
       if (myq[qq] < 0) continue;
-      uint64_t v0 = lane < cnt[qq] ? buf[slot][lane] : ~0ull;
-      uint64_t v1 = lane + 64 < cnt[qq] ? buf[slot][lane + 64] : ~0ull;
-      sort128(v0, v1);
       const int p = W.qbeg + slot;
       uint64_t* dst = xpart + ((size_t)p * xs + W.split) * kmax;
-      if (lane < k) dst[lane] = v0;
-      if (lane + 64 < k) dst[lane + 64] = v1;
+      if constexpr (QCAP == 128) {
+        uint64_t v0 = lane < cnt[qq] ? buf[slot][lane] : ~0ull;
+        uint64_t v1 = lane + 64 < cnt[qq] ? buf[slot][lane + 64] : ~0ull;
+        sort128(v0, v1);
+        if (lane < k) dst[lane] = v0;
+        if (lane + 64 < k) dst[lane + 64] = v1;
+      } else {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int e = cnt[qq] + lane; e < QCAP; e += 64) buf[slot][e] = ~0ull;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        wave_sort_lds(buf[slot], QCAP);
+        for (int t = lane; t < k; t += 64) dst[t] = buf[slot][t];
+      }
     }
   }
 }
+
+#define HM_XS_INST(QPW, QCAP)                                                                    \
+  template __global__ void exact_scan_kernel<QPW, QCAP>(const DsDesc*, const WorkItem*,          \
+                                                        const int*, int, const uint32_t*,       \
+                                                        const int*, int, int, uint64_t*);
+HM_XS_INST(8, 128)
+HM_XS_INST(4, 512)
+HM_XS_INST(1, 2048)
+#undef HM_XS_INST
 
 // Merging the xs sorted per-split lists of every listed query (or local
 // row in identity mode) into its knn row, in two levels so a wave never
@@ -732,6 +761,7 @@ __device__ void sweep_items_block(const DsDesc* __restrict__ ds_arr, int nds,
     const int q = list_lower_bound(list, cnt, D.row_off + D.n);
     if (q == p) continue;
     te += (double)(q - p) * (double)D.n * D.kp * 3.0 / 2e13;
+    if (D.nsplit == 0) continue;  // k beyond the sweep: its rows go to the exact scan
     const double span = (double)D.n / D.nsplit;
     const double t = 4.0 * span * D.kp * qw / rate;
     ti = t > ti ? t : ti;
@@ -742,7 +772,7 @@ __device__ void sweep_items_block(const DsDesc* __restrict__ ds_arr, int nds,
   uint64_t nit;
   (void)block_excl_scan_u64<NT>(nq, nit, sh);
   const double t_sweep = (double)((nit + cus - 1) / cus) * t_item;
-  if (cnt == 0 || t_exact <= t_sweep) {  // uniform
+  if (cnt == 0 || nit == 0 || t_exact <= t_sweep) {  // uniform
     if (threadIdx.x == 0) {
       *nitems = 0;
       *nrows = 0;
@@ -783,12 +813,13 @@ __global__ __launch_bounds__(256) void sweep_items_kernel(
 // flagged (a handful of rows would otherwise run on a handful of CUs), up to
 // xs_cap (which keeps the merge at two levels) and within the item and
 // partial-list capacities, which hold item_cap items and row_cap * xs lists.
-// Items: per dataset, blocks of <= XS_QB listed rows x the splits, in dataset
-// order; one workgroup, a thread per dataset, as above.
+// Items: per dataset, blocks of <= qb listed rows (the exact scan's queries
+// per item) x the splits, in dataset order; one workgroup, a thread per
+// dataset, as above.
 template <int NT>
 __device__ void fallback_items_block(const DsDesc* __restrict__ ds_arr, int nds,
                                      const uint32_t* __restrict__ list,
-                                     const int* __restrict__ count, int xs, int xs_cap,
+                                     const int* __restrict__ count, int qb, int xs, int xs_cap,
                                      int64_t item_cap, int64_t row_cap, int target,
                                      WorkItem* __restrict__ items, int* __restrict__ nitems,
                                      int* __restrict__ xs_out) {
@@ -798,7 +829,7 @@ __device__ void fallback_items_block(const DsDesc* __restrict__ ds_arr, int nds,
   for (int di = threadIdx.x; di < nds; di += NT) {
     const int p = list_lower_bound(list, cnt, ds_arr[di].row_off);
     const int q = list_lower_bound(list, cnt, ds_arr[di].row_off + ds_arr[di].n);
-    part += (uint64_t)((q - p + XS_QB - 1) / XS_QB);
+    part += (uint64_t)((q - p + qb - 1) / qb);
   }
   uint64_t nqb_u;
   (void)block_excl_scan_u64<NT>(part, nqb_u, sh);
@@ -821,10 +852,10 @@ __device__ void fallback_items_block(const DsDesc* __restrict__ ds_arr, int nds,
     }
     uint64_t tot;
     const uint64_t off =
-        block_excl_scan_u64<NT>((uint64_t)((q - p + XS_QB - 1) / XS_QB) * (uint64_t)xd, tot, sh);
+        block_excl_scan_u64<NT>((uint64_t)((q - p + qb - 1) / qb) * (uint64_t)xd, tot, sh);
     int w = (int)(base + off);
-    for (int b = p; b < q; b += XS_QB) {
-      const int qc = q - b < XS_QB ? q - b : XS_QB;
+    for (int b = p; b < q; b += qb) {
+      const int qc = q - b < qb ? q - b : qb;
       for (int s = 0; s < (int)xd; ++s) items[w++] = WorkItem{di, s, b, qc};
     }
     base += tot;
@@ -836,9 +867,9 @@ __device__ void fallback_items_block(const DsDesc* __restrict__ ds_arr, int nds,
 }
 __global__ __launch_bounds__(256) void fallback_items_kernel(
     const DsDesc* __restrict__ ds_arr, int nds, const uint32_t* __restrict__ list,
-    const int* __restrict__ count, int xs, int xs_cap, int64_t item_cap, int64_t row_cap,
+    const int* __restrict__ count, int qb, int xs, int xs_cap, int64_t item_cap, int64_t row_cap,
     int target, WorkItem* __restrict__ items, int* __restrict__ nitems, int* __restrict__ xs_out) {
-  fallback_items_block<256>(ds_arr, nds, list, count, xs, xs_cap, item_cap, row_cap, target,
+  fallback_items_block<256>(ds_arr, nds, list, count, qb, xs, xs_cap, item_cap, row_cap, target,
                             items, nitems, xs_out);
 }
 
@@ -855,12 +886,12 @@ __global__ __launch_bounds__(512) void flag_sweep_items_kernel(
 }
 __global__ __launch_bounds__(512) void flag_fallback_items_kernel(
     const uint32_t* __restrict__ flag, int64_t n, uint32_t* __restrict__ list,
-    int* __restrict__ count, const DsDesc* __restrict__ ds_arr, int nds, int xs, int xs_cap,
-    int64_t item_cap, int64_t row_cap, int target, WorkItem* __restrict__ items,
+    int* __restrict__ count, const DsDesc* __restrict__ ds_arr, int nds, int qb, int xs,
+    int xs_cap, int64_t item_cap, int64_t row_cap, int target, WorkItem* __restrict__ items,
     int* __restrict__ nitems, int* __restrict__ xs_out) {
   scan_small_body<uint32_t, 512, 32>(flag, n, nullptr, nullptr, list, count);
   __syncthreads();
-  fallback_items_block<512>(ds_arr, nds, list, count, xs, xs_cap, item_cap, row_cap, target,
+  fallback_items_block<512>(ds_arr, nds, list, count, qb, xs, xs_cap, item_cap, row_cap, target,
                             items, nitems, xs_out);
 }
 
@@ -902,12 +933,12 @@ __device__ void list_compact_block(const uint32_t* __restrict__ flag, uint32_t* 
 }
 __global__ __launch_bounds__(512) void list_fallback_items_kernel(
     const uint32_t* __restrict__ flag, uint32_t* __restrict__ list, int* __restrict__ count,
-    const DsDesc* __restrict__ ds_arr, int nds, int xs, int xs_cap, int64_t item_cap,
+    const DsDesc* __restrict__ ds_arr, int nds, int qb, int xs, int xs_cap, int64_t item_cap,
     int64_t row_cap, int target, WorkItem* __restrict__ items, int* __restrict__ nitems,
     int* __restrict__ xs_out) {
   list_compact_block<512, 8>(flag, list, count);
   __syncthreads();
-  fallback_items_block<512>(ds_arr, nds, list, count, xs, xs_cap, item_cap, row_cap, target,
+  fallback_items_block<512>(ds_arr, nds, list, count, qb, xs, xs_cap, item_cap, row_cap, target,
                             items, nitems, xs_out);
 }
 

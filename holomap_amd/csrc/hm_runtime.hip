@@ -25,6 +25,7 @@ __global__ void gram_sweep_kernel(const DsDesc*, const WorkItem*, const int*, in
 template <class Cfg, int PASS>
 __global__ void gram_sweep_qres_kernel(const DsDesc*, const WorkItem*, const int*, int,
                                        const uint32_t*);
+template <int QPW, int QCAP>
 __global__ void exact_scan_kernel(const DsDesc*, const WorkItem*, const int*, int,
                                   const uint32_t*, const int*, int, int, uint64_t*);
 __global__ void exact_group_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t,
@@ -36,14 +37,15 @@ __global__ void rerank_kernel(const DsDesc*, int, int64_t, int, uint32_t*, const
 __global__ void sweep_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int, int,
                                    double, WorkItem*, int*, int*);
 __global__ void fallback_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int, int,
-                                      int64_t, int64_t, int, WorkItem*, int*, int*);
+                                      int, int64_t, int64_t, int, WorkItem*, int*, int*);
 __global__ void flag_sweep_items_kernel(const uint32_t*, int64_t, uint32_t*, int*, const DsDesc*,
                                         int, int, int, double, WorkItem*, int*, int*);
 __global__ void flag_fallback_items_kernel(const uint32_t*, int64_t, uint32_t*, int*,
-                                           const DsDesc*, int, int, int, int64_t, int64_t, int,
-                                           WorkItem*, int*, int*);
+                                           const DsDesc*, int, int, int, int, int64_t, int64_t,
+                                           int, WorkItem*, int*, int*);
 __global__ void list_fallback_items_kernel(const uint32_t*, uint32_t*, int*, const DsDesc*, int,
-                                           int, int, int64_t, int64_t, int, WorkItem*, int*, int*);
+                                           int, int, int, int64_t, int64_t, int, WorkItem*, int*,
+                                           int*);
 template <int KM>
 __global__ void smooth_knn_kernel(const DsDesc*, int, int64_t, const double*, int, float*, float*,
                                   double*, uint32_t*);
@@ -83,6 +85,7 @@ __global__ void union_write_kernel(const DsDesc*, int, int64_t, int64_t, const u
 }  // namespace hm
 
 using namespace hm;
+static_assert(kMaxK == HM_MAX_NEIGHBORS, "the public limit is the exact scan's");
 
 struct hm_ctx {
   int device = 0;
@@ -222,6 +225,13 @@ int choose_nsplit(int64_t n, int qw, int C, int cus, int64_t other_items) {
   return best;
 }
 
+// Splits merged per 1024-key sort of the exact scan's partial lists.
+int merge_group(int kmax) {
+  int kpow = 1;
+  while (kpow < kmax) kpow <<= 1;
+  return std::max(1, 1024 / kpow);
+}
+
 struct Plan {
   std::vector<DsHost> dsh;
   int64_t total_rows = 0, total_pad = 0, total_edges = 0, sum_xr = 0, sum_x16 = 0, sum_cos = 0,
@@ -240,8 +250,18 @@ void plan_sweep(std::vector<DsHost>& dsh, int margin_opt, int nsplit_opt, int mo
                 int margin_mult = 2) {
   const int qw = sweep_rows(mode);
   int64_t rb_total = 0;
-  for (auto& h : dsh) rb_total += cdiv(h.n, qw);
+  for (auto& h : dsh)
+    if (h.k <= kSweepMaxK) rb_total += cdiv(h.n, qw);
   for (auto& h : dsh) {
+    if (h.k > kSweepMaxK) {
+      // no sweep items: the re-rank finds no candidates and flags every
+      // row, which the exact scan then computes
+      h.C = h.k;
+      h.nsplit = 0;
+      h.nlist = 0;
+      h.cout = 0;
+      continue;
+    }
     const int margin = margin_opt < 0 ? margin_mult * h.k : margin_opt;
     h.C = std::max(h.k, std::min(h.k + margin, 64));
     // A list (one lane: half of a split's column groups) collects the keys
@@ -292,8 +312,12 @@ Plan make_plan(uint32_t nds, const uint64_t* n, const uint32_t* d, const uint32_
   plan_sweep(P.dsh, opt.margin, opt.nsplit, gram_mode(opt.precision), cus);
   for (auto& h : P.dsh) P.cmax = std::max(P.cmax, h.C);
   P.sum_cand = P.total_rows * kCandCap;
+  // exact-scan column splits: the two merge levels hold G * G splits of k
+  // keys (G splits per 1024-key sort)
   P.xs = (int)std::max<int64_t>(1, std::min<int64_t>(8, nmax / 2048));
-  for (auto& h : P.dsh) P.fb_cap += cdiv(h.n, 32) * P.xs;
+  P.xs = std::min(P.xs, merge_group(P.kmax) * merge_group(P.kmax));
+  const int qb = 4 * xs_qpw_for_k(P.kmax);
+  for (auto& h : P.dsh) P.fb_cap += cdiv(h.n, qb) * P.xs;
   return P;
 }
 
@@ -546,30 +570,34 @@ hm_status enqueue_pipeline(hm_job* J) {
     const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(J->fb_cap, 2 * cus));
     // merge groups: G splits of k keys fit one 1024-key sort; two merge
     // levels cover up to G * G splits
-    int kpow = 1;
-    while (kpow < J->kmax) kpow <<= 1;
-    const int G = std::max(1, 1024 / kpow);
+    const int G = merge_group(J->kmax);
     const int xs_cap = std::max(J->xs, std::min(512, G * G));
+    const int qpw = xs_qpw_for_k(J->kmax), qb = 4 * qpw;
     if (J->mode >= 0 && J->esc_mode >= 0) {
       // the rows still flagged are among the escalated ones: compact that list
       hipLaunchKernelGGL(list_fallback_items_kernel, dim3(1), dim3(512), 0, J->stream,
-                         (const uint32_t*)J->flag, J->list, J->count, J->d_desc, nds, J->xs,
+                         (const uint32_t*)J->flag, J->list, J->count, J->d_desc, nds, qb, J->xs,
                          xs_cap, (int64_t)J->fb_cap, R, (int)gx, J->items_fb, J->nitems_fb,
                          J->xs_fb);
     } else if (R <= kScanSmall<uint32_t>) {  // compaction and items in one launch
       hipLaunchKernelGGL(flag_fallback_items_kernel, dim3(1), dim3(kScanSmallThreads), 0,
                          J->stream, (const uint32_t*)J->flag, R, J->list, J->count, J->d_desc, nds,
-                         J->xs, xs_cap, (int64_t)J->fb_cap, R, (int)gx, J->items_fb, J->nitems_fb,
-                         J->xs_fb);
+                         qb, J->xs, xs_cap, (int64_t)J->fb_cap, R, (int)gx, J->items_fb,
+                         J->nitems_fb, J->xs_fb);
     } else {
       HM_TRY(compact_flags(J, J->flag, R, J->list, J->count));
       hipLaunchKernelGGL(fallback_items_kernel, dim3(1), dim3(256), 0, J->stream, J->d_desc, nds,
-                         (const uint32_t*)J->list, (const int*)J->count, J->xs, xs_cap,
+                         (const uint32_t*)J->list, (const int*)J->count, qb, J->xs, xs_cap,
                          (int64_t)J->fb_cap, R, (int)gx, J->items_fb, J->nitems_fb, J->xs_fb);
     }
-    hipLaunchKernelGGL(exact_scan_kernel, dim3(gx), dim3(256), 0, J->stream, J->d_desc,
-                       (const WorkItem*)J->items_fb, (const int*)J->nitems_fb, 0,
-                       (const uint32_t*)J->list, (const int*)J->xs_fb, J->xs, J->kmax, J->xpart);
+#define HM_XSCAN(QPW, QCAP)                                                                    \
+  hipLaunchKernelGGL((exact_scan_kernel<QPW, QCAP>), dim3(gx), dim3(256), 0, J->stream,         \
+                     J->d_desc, (const WorkItem*)J->items_fb, (const int*)J->nitems_fb, 0,     \
+                     (const uint32_t*)J->list, (const int*)J->xs_fb, J->xs, J->kmax, J->xpart)
+    if (qpw == 8) HM_XSCAN(8, 128);
+    else if (qpw == 4) HM_XSCAN(4, 512);
+    else HM_XSCAN(1, 2048);
+#undef HM_XSCAN
     const unsigned gm = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(R, 4), 8 * cus));
     hipLaunchKernelGGL(exact_group_merge_kernel, dim3(gm), dim3(256), 0, J->stream, J->d_desc,
                        nds, (const uint32_t*)J->list, (const int*)J->count, (int64_t)0,
@@ -740,10 +768,15 @@ hm_status hm_validate(const float* data, uint64_t len, uint32_t d, uint32_t k, u
 }
 
 uint64_t hm_job_estimate_bytes(uint32_t nds, const uint64_t* n, const uint32_t* d,
-                               const uint32_t* k, const hm_options* opt) {
+                               const uint32_t* k, const int32_t* metric, const hm_options* opt) {
   hm_options o;
   if (opt) o = *opt; else hm_default_options(&o);
-  Plan P = make_plan(nds, n, d, k, nullptr, o, 256);
+  std::vector<int32_t> m;
+  if (!metric) {
+    m.assign(nds, HM_COSINE);  // the cosine operand makes it the larger estimate
+    metric = m.data();
+  }
+  Plan P = make_plan(nds, n, d, k, metric, o, 256);
   return plan_bytes(P, o, HM_STAGE_ALL);
 }
 
@@ -769,6 +802,7 @@ hm_status apply_sweep(hm_job* J) {
     std::vector<std::pair<int64_t, WorkItem>> it[3];
     for (int i = 0; i < nds; ++i) {
       const DsHost& h = J->dsh[i];
+      if (h.nsplit == 0) continue;  // k beyond the sweep: exact scan only
       const int64_t cost = cdiv(h.n, h.nsplit) * h.kp;
       const int li = sweep_list_of(J->mode, h.kp16);
       const int qw = kListRows[li] ? kListRows[li] : sweep_rows(J->mode);
@@ -829,8 +863,9 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
       // stage-level kNN (src/knn.rs:22) needs k <= n; fit_transform's
       // stricter n > k >= 2 is checked by hm_validate in hm_knn_graph
       if (n[i] < k[i]) return fail(ctx, HM_ETOOFEW, "need at least n_neighbors points");
-      if (k[i] > 64)
-        return fail(ctx, HM_EUNSUPPORTED, "n_neighbors > 64 is not supported on the GPU path");
+      if (k[i] > (uint32_t)kMaxK)
+        return fail(ctx, HM_EUNSUPPORTED,
+                    "n_neighbors > " + std::to_string(kMaxK) + " is not supported on the GPU path");
     }
     if (n[i] >= (1ull << 31)) return fail(ctx, HM_EUNSUPPORTED, "n >= 2^31");
     if (metric && metric[i] != HM_EUCLIDEAN && metric[i] != HM_COSINE)
@@ -840,6 +875,8 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
     return fail(ctx, HM_EINVAL, "unknown precision");
   if (o.calib != HM_CALIB_REF_F32 && o.calib != HM_CALIB_F64)
     return fail(ctx, HM_EINVAL, "unknown calibration mode");
+  if (o.nsplit < 0 || o.nsplit > kSplitCap)
+    return fail(ctx, HM_EINVAL, "nsplit must be in 0.." + std::to_string(kSplitCap));
   HM_HIP(ctx, hipSetDevice(ctx->device));
   Plan P = make_plan(nds, n, d, k, metric, o, ctx->num_cus);
   if (P.total_rows >= 0xffffffffll || P.total_edges >= 0xffffffffll)
@@ -1126,6 +1163,7 @@ hm_status autotune(hm_job* J) {
       if (tried[i].ms < tried[b].ms) b = i;
     return b;
   };
+  if (tried.empty()) return fail(ctx, HM_EINVAL, "autotune: no sweep configuration to time");
   if (J->opt.margin < 0) {
     const Cfg b = tried[best_of()];
     for (int mult : {1, 3}) HM_TRY(try_cfg(mult, b.nsplit, b.perm));

@@ -95,6 +95,9 @@ hm_status hm_server_submit(hm_server* s, const float* data, uint64_t n, uint32_t
   const hm_status v = hm_validate(data, n * d, d, k, &bad);
   if (v != HM_OK) return v;
   if (metric != HM_EUCLIDEAN && metric != HM_COSINE) return HM_EINVAL;
+  // the limits hm_job_create applies: a request it would refuse never
+  // enters the queue (where it would fail the requests batched with it)
+  if (k > HM_MAX_NEIGHBORS || n >= (1ull << 31)) return HM_EUNSUPPORTED;
   Request r;
   r.id = s->next_id++;
   r.n = n;
@@ -143,16 +146,20 @@ hm_status hm_server_progress(hm_server* s, uint32_t* completed) {
   }
   // admit queued requests, oldest first
   const uint64_t budget = hm_ctx_budget(s->ctx);
+  auto fail_head = [&](hm_status st) {
+    Request& r = s->reqs[s->queue.front()];
+    r.state = st;
+    r.data.clear();
+    r.data.shrink_to_fit();
+    s->queue.pop_front();
+    ++fin;
+  };
   while (!s->queue.empty() && running_jobs(s) < s->cfg.max_jobs) {
     {  // a request larger than the whole budget can never run
       const Request& h = s->reqs[s->queue.front()];
-      const uint64_t need1 = hm_job_estimate_bytes(1, &h.n, &h.d, &h.k, &s->cfg.opt);
+      const uint64_t need1 = hm_job_estimate_bytes(1, &h.n, &h.d, &h.k, &h.metric, &s->cfg.opt);
       if (budget && need1 > budget) {
-        Request& r = s->reqs[s->queue.front()];
-        r.state = HM_EBUDGET;
-        r.data.clear();
-        s->queue.pop_front();
-        ++fin;
+        fail_head(HM_EBUDGET);
         continue;
       }
     }
@@ -169,7 +176,7 @@ hm_status hm_server_progress(hm_server* s, uint32_t* completed) {
       k.push_back(r.k);
       m.push_back(r.metric);
       const uint64_t need = hm_job_estimate_bytes((uint32_t)n.size(), n.data(), d.data(),
-                                                  k.data(), &s->cfg.opt);
+                                                  k.data(), m.data(), &s->cfg.opt);
       if (budget && hm_ctx_in_use(s->ctx) + need > budget) {
         n.pop_back();
         d.pop_back();
@@ -181,21 +188,28 @@ hm_status hm_server_progress(hm_server* s, uint32_t* completed) {
       rows += r.n;
     }
     if (ids.empty()) break;  // the head waits for memory to free up
+    bool held = false;  // memory that will come back: live jobs
+    for (auto& j : s->jobs) held = held || j.job != nullptr;
     hm_job* J = nullptr;
-    hm_status st = hm_job_create(s->ctx, (uint32_t)ids.size(), n.data(), d.data(), k.data(),
-                                 m.data(), &s->cfg.opt, HM_STAGE_ALL, &J);
-    if (st == HM_EBUDGET || st == HM_ENOMEM) {
-      bool held = false;  // memory that will come back: live jobs
-      for (auto& j : s->jobs) held = held || j.job != nullptr;
-      if (held) break;  // retried when memory frees up
-      // nothing to wait for: the head cannot be allocated at all
-      Request& r = s->reqs[s->queue.front()];
-      r.state = st;
-      r.data.clear();
-      s->queue.pop_front();
-      ++fin;
+    hm_status st = HM_OK;
+    size_t take = ids.size();
+    for (;;) {
+      st = hm_job_create(s->ctx, (uint32_t)take, n.data(), d.data(), k.data(), m.data(),
+                         &s->cfg.opt, HM_STAGE_ALL, &J);
+      if (st == HM_OK) break;
+      const bool mem = st == HM_EBUDGET || st == HM_ENOMEM;
+      if (mem && held) break;  // retried when memory frees up
+      if (take == 1) break;
+      // a shorter prefix (memory), or the head alone (any other refusal, so
+      // that one request's failure does not fail the others)
+      take = mem ? take / 2 : 1;
+    }
+    if (st != HM_OK) {
+      if ((st == HM_EBUDGET || st == HM_ENOMEM) && held) break;
+      fail_head(st);  // the head alone cannot run
       continue;
     }
+    ids.resize(take);
     for (size_t i = 0; i < ids.size(); ++i) s->queue.pop_front();
     if (st == HM_OK) {
       for (size_t i = 0; i < ids.size() && st == HM_OK; ++i)

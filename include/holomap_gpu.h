@@ -42,6 +42,10 @@ typedef int32_t hm_status;
 #define HM_EBUDGET -7    /* request exceeds the context's HBM budget */
 #define HM_EUNSUPPORTED -8
 
+/* Largest n_neighbors the GPU path serves (the reference has no limit
+ * below n; larger values return HM_EUNSUPPORTED). */
+#define HM_MAX_NEIGHBORS 1024
+
 /* Metric (src/metric.rs:8-14). */
 #define HM_EUCLIDEAN 0
 #define HM_COSINE 1
@@ -169,9 +173,10 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
 void hm_job_destroy(hm_job* job);
 /* Device bytes held by the job. */
 uint64_t hm_job_bytes(const hm_job* job);
-/* Estimate without allocating (admission control). */
+/* Estimate without allocating (admission control); metric may be NULL
+ * (every dataset counted as cosine, the larger of the two). */
 uint64_t hm_job_estimate_bytes(uint32_t nds, const uint64_t* n, const uint32_t* d,
-                               const uint32_t* k, const hm_options* opt);
+                               const uint32_t* k, const int32_t* metric, const hm_options* opt);
 /* Host -> device copy of dataset `ds` (n*d floats). */
 hm_status hm_job_upload(hm_job* job, uint32_t ds, const float* data);
 /* Device -> device copy from a caller device pointer (n*d floats). */

@@ -93,6 +93,17 @@ def test_estimate_bytes_is_host_only():
     d = np.array([784], np.uint32)
     k = np.array([15], np.uint32)
     o = _lib.default_options()
-    b = L.hm_job_estimate_bytes(1, n.ctypes.data, d.ctypes.data, k.ctypes.data, ctypes.byref(o))
+    eu = np.array([_lib.HM_EUCLIDEAN], np.int32)
+    co = np.array([_lib.HM_COSINE], np.int32)
+    b = L.hm_job_estimate_bytes(1, n.ctypes.data, d.ctypes.data, k.ctypes.data, eu.ctypes.data,
+                                ctypes.byref(o))
     # at least the input, its padded copy and the bf16 operand
     assert b > 50000 * 784 * 10
+    # cosine adds the normalised f32 operand (n_pad x kp floats); a NULL
+    # metric array is counted as cosine
+    bc = L.hm_job_estimate_bytes(1, n.ctypes.data, d.ctypes.data, k.ctypes.data, co.ctypes.data,
+                                 ctypes.byref(o))
+    assert bc - b == 50048 * 800 * 4
+    bn = L.hm_job_estimate_bytes(1, n.ctypes.data, d.ctypes.data, k.ctypes.data, None,
+                                 ctypes.byref(o))
+    assert bn == bc

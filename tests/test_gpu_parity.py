@@ -480,8 +480,10 @@ def test_server_budget_admission():
         a = np.array([r.n], np.uint64)
         b = np.array([r.d], np.uint32)
         c = np.array([r.k], np.uint32)
+        m = np.array([0 if r.metric == "euclidean" else 1], np.int32)
         est.append(int(_lib.load().hm_job_estimate_bytes(1, a.ctypes.data, b.ctypes.data,
-                                                          c.ctypes.data, ctypes.byref(o))))
+                                                          c.ctypes.data, m.ctypes.data,
+                                                          ctypes.byref(o))))
     budget = int(max(est) * 1.3)
     bctx = H.Context(0, hbm_budget=budget)
     srv = H.Server(bctx, max_jobs=4, max_batch=8)
@@ -637,3 +639,100 @@ def test_hm_knn_graph_c_abi_as_the_rust_shim(ctx):
     x[4, 1] = np.inf
     st, _ = _knn_graph_like_rust_shim(ctx, x, 3, 5, "euclidean")
     assert st == _lib.HM_ENONFINITE
+
+
+# n_neighbors beyond the sweep's 64: every row of such a dataset is computed
+# by the exact scan (buffers of 512 and 2048 keys per query for k > 64 and
+# k > 448), the calibration runs a thread per row, and the symmetrisation's
+# row groups take several out-edges per lane
+@pytest.mark.parametrize("k", [65, 100, 200, 449, 1000])
+def test_large_k_graph(ctx, k):
+    rng = np.random.default_rng(k)
+    n, d = 1300, 20
+    x = DS.gaussian_mixture(n, d, 6, 3.0, rng)
+    x[11] = x[10]
+    metric = "euclidean" if k % 2 else "cosine"
+    ref = oracle_graph_mt(x, d, k, metric)
+    for prec in ["bf16", "exact"]:
+        res = H.knn_graph(x, d, k, metric, precision=prec, ctx=ctx)
+        assert_graph_equal(res, *ref)
+
+
+def test_large_k_mixed_batch(ctx):
+    # a batch mixing k <= 64 datasets (swept) with k > 64 ones (exact scan)
+    rng = np.random.default_rng(41)
+    specs = [(900, 16, 15, "euclidean"), (1100, 24, 80, "cosine"), (700, 40, 30, "cosine"),
+             (1500, 8, 150, "euclidean")]
+    data = [DS.gaussian_mixture(n, d, 5, 3.0, rng) for n, d, _, _ in specs]
+    job = H.Job(ctx, [s[0] for s in specs], [s[1] for s in specs], [s[2] for s in specs],
+                [s[3] for s in specs])
+    for i, x in enumerate(data):
+        job.upload(i, x)
+    job.run()
+    job.sync()
+    for i, (n, d, k, m) in enumerate(specs):
+        assert_graph_equal(job.download(i), *oracle_graph_mt(data[i], d, k, m))
+    job.close()
+
+
+def test_fuzzy_stage_with_large_k(ctx):
+    # the stage-level symmetrisation with k = 80 (no kNN stage in the job)
+    rng = np.random.default_rng(43)
+    n, d, k = 900, 10, 80
+    x = DS.gaussian_mixture(n, d, 4, 2.0, rng)
+    knn = O.exact_knn_rows(x, d, k, "euclidean", np.arange(n, dtype=np.uint32), nthreads=8)
+    cal = O.smooth_knn(knn.dists, k)
+    g = O.fuzzy_simplicial_set(knn, cal)
+    got = H.fuzzy_simplicial_set(H.Knn(k, knn.indices, knn.dists),
+                                 H.SmoothKnn(cal.sigmas, cal.rhos), ctx=ctx)
+    assert np.array_equal(got.rows, g.rows) and np.array_equal(got.cols, g.cols)
+    assert np.array_equal(got.vals.view(np.uint32), g.vals.view(np.uint32))
+    sc = H.smooth_knn(knn.dists, k, ctx=ctx)
+    assert np.array_equal(sc.sigmas.view(np.uint32), cal.sigmas.view(np.uint32))
+
+
+def test_k_beyond_the_limit_is_refused(ctx):
+    from holomap_amd import _lib
+    x = np.random.default_rng(2).normal(size=(_lib.HM_MAX_NEIGHBORS + 10, 4)).astype(np.float32)
+    with pytest.raises(H.DeviceError, match="not supported"):
+        H.knn_graph(x, 4, _lib.HM_MAX_NEIGHBORS + 1, ctx=ctx)
+    # the serving loop refuses it at submit, so it never fails a batch
+    srv = H.Server(ctx)
+    with pytest.raises(H.DeviceError):
+        srv.submit(x, 4, _lib.HM_MAX_NEIGHBORS + 1)
+    ok = srv.submit(x[:300], 4, 15)
+    while srv.state(ok) not in (srv.DONE,) and srv.state(ok) >= 0:
+        srv.progress()
+    r = srv.fetch(ok)
+    assert_graph_equal(r, *O.knn_graph(x[:300], 4, 15, "euclidean"))
+    srv.close()
+
+
+def test_server_tight_cosine_budget():
+    # a budget that holds one cosine request (by its metric-aware estimate):
+    # every request is admitted in turn and none fails for memory
+    from holomap_amd import _lib
+    reqs = [DS.c5_request(300 + i, n_min=800, n_max=2500) for i in range(5)]
+    reqs = [DS.Dataset(r.data / np.maximum(np.linalg.norm(r.data, axis=1, keepdims=True), 1e-30),
+                       r.n, r.d, r.k, "cosine") for r in reqs]
+    o = H._opts()
+    est = []
+    for r in reqs:
+        a = np.array([r.n], np.uint64)
+        b = np.array([r.d], np.uint32)
+        c = np.array([r.k], np.uint32)
+        m = np.array([_lib.HM_COSINE], np.int32)
+        est.append(int(_lib.load().hm_job_estimate_bytes(1, a.ctypes.data, b.ctypes.data,
+                                                          c.ctypes.data, m.ctypes.data,
+                                                          ctypes.byref(o))))
+    budget = max(est) + 4096
+    bctx = H.Context(0, hbm_budget=budget)
+    srv = H.Server(bctx, max_jobs=2, max_batch=4)
+    ids = [srv.submit(r.data.astype(np.float32), r.d, r.k, r.metric) for r in reqs]
+    got, peak = _serve_all(srv, ids, ctx=bctx, budget=budget)
+    srv.close()
+    assert 0 < peak <= budget
+    for rid, r in zip(ids, reqs):
+        assert not isinstance(got[rid], Exception), got[rid]
+        assert _same(got[rid], H.knn_graph(r.data.astype(np.float32), r.d, r.k, r.metric))
+    bctx.close()
