@@ -22,6 +22,13 @@ namespace hm {
 // exp(r) = 1 + (r*c/(2-c) - lo + hi), c = r - r^2 (P1 + r^2 P2), then an exact
 // power-of-two scaling. Every operation is a single f32 rounding (this file
 // is compiled without contraction).
+// The method and its constants are those of FreeBSD msun's e_expf.c, which
+// carries this notice:
+//   Conversion to float by Ian Lance Taylor, Cygnus Support, ian@cygnus.com.
+//   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//   Developed at SunPro, a Sun Microsystems, Inc. business.
+//   Permission to use, copy, modify, and distribute this software is freely
+//   granted, provided that this notice is preserved.
 __device__ __forceinline__ float hm_scalbnf(float x, int n) {
   float y = x;
   if (n > 127) {

@@ -116,7 +116,14 @@ void hmo_exact_knn_rows(const float* data, size_t n, size_t d, size_t k, int met
  * FreeBSD msun e_expf.c method the crate follows: reduction x = k*ln2 + r
  * with ln2 split hi/lo, the degree-2 Remez rational approximation
  * c = r - r^2 (P1 + r^2 P2), exp(r) = 1 + (r c/(2-c) - lo + hi), and an exact
- * power-of-two scaling (scalbnf). */
+ * power-of-two scaling (scalbnf).
+ * The method and its constants are those of FreeBSD msun's e_expf.c, which
+ * carries this notice:
+ *   Conversion to float by Ian Lance Taylor, Cygnus Support, ian@cygnus.com.
+ *   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+ *   Developed at SunPro, a Sun Microsystems, Inc. business.
+ *   Permission to use, copy, modify, and distribute this software is freely
+ *   granted, provided that this notice is preserved. */
 static float scalbnf_(float x, int n) {
     float y = x;
     if (n > 127) {
