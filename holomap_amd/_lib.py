@@ -21,6 +21,7 @@ HM_ENOMEM = -6
 HM_EBUDGET = -7
 HM_EUNSUPPORTED = -8
 HM_MAX_NEIGHBORS = 1024
+HM_NOALLOC_SCRATCH_BYTES = 256 << 20
 
 HM_EUCLIDEAN = 0
 HM_COSINE = 1
@@ -49,7 +50,7 @@ EXPORTS = [
     "hm_job_device_outputs", "hm_expf_array", "hm_ctx_budget", "hm_ctx_in_use",
     "hm_server_default_config", "hm_server_create", "hm_server_destroy", "hm_server_submit",
     "hm_server_progress", "hm_server_state", "hm_server_pending", "hm_server_result_info",
-    "hm_server_fetch",
+    "hm_server_fetch", "hm_knn_graph_noalloc",
 ]
 
 
@@ -100,6 +101,7 @@ def load():
         "hm_fuzzy_simplicial_set": (i32, [vp, vp, vp, vp, vp, u64, u32, vp, vp, vp, u64, P(u64)]),
         "hm_csr_from_fuzzy": (i32, [vp, vp, u64, u64, vp]),
         "hm_knn_graph": (i32, [vp, vp, u64, u32, u32, i32, P(Options), P(GraphOut)]),
+        "hm_knn_graph_noalloc": (i32, [vp, vp, u64, u32, u32, i32, P(Options), P(GraphOut)]),
         "hm_job_create": (i32, [vp, u32, vp, vp, vp, vp, P(Options), u32, P(vp)]),
         "hm_job_destroy": (None, [vp]),
         "hm_job_bytes": (u64, [vp]),

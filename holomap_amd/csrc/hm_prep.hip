@@ -267,3 +267,51 @@ __global__ __launch_bounds__(256) void statmax_kernel(const DsDesc* __restrict__
 }
 
 }  // namespace hm
+
+namespace hm {
+
+// Index of the first non-finite value of x[0 .. len), or ~0 (input
+// validation of src/api.rs:142-144 on device data), in two launches without
+// atomics: nonfinite_part_kernel writes each workgroup's smallest such
+// index, nonfinite_min_kernel reduces the partial results. Thread t visits
+// t, t + stride, ... in increasing order, so its first hit is its smallest.
+__device__ __forceinline__ uint64_t block_min_u64(uint64_t v) {
+  __shared__ uint64_t red[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t t = shfl_xor_u64(v, o);
+    v = t < v ? t : v;
+  }
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  uint64_t r = red[0];
+#pragma unroll
+  for (int w = 1; w < 4; ++w) r = red[w] < r ? red[w] : r;
+  return r;
+}
+
+__global__ __launch_bounds__(256) void nonfinite_part_kernel(const float* __restrict__ x,
+                                                             int64_t len,
+                                                             uint64_t* __restrict__ part) {
+  uint64_t best = ~0ull;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < len; i += (int64_t)gridDim.x * 256) {
+    if ((__float_as_uint(x[i]) & 0x7f800000u) == 0x7f800000u) {
+      best = (uint64_t)i;
+      break;
+    }
+  }
+  const uint64_t m = block_min_u64(best);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(256) void nonfinite_min_kernel(const uint64_t* __restrict__ part,
+                                                            int nparts,
+                                                            uint64_t* __restrict__ out) {
+  uint64_t best = ~0ull;
+  for (int i = threadIdx.x; i < nparts; i += 256) best = part[i] < best ? part[i] : best;
+  const uint64_t m = block_min_u64(best);
+  if (threadIdx.x == 0) *out = m;
+}
+
+}  // namespace hm

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -70,6 +71,8 @@ __global__ void radix_scatter(const uint32_t*, const uint32_t*, int64_t, int, in
 __global__ void lower_bounds(const uint32_t*, int64_t, int64_t, uint32_t*);
 __global__ void compact_kernel(const uint32_t*, const uint32_t*, int64_t, uint32_t*, int*);
 __global__ void fill_u32(uint32_t*, int64_t, uint32_t);
+__global__ void nonfinite_part_kernel(const float*, int64_t, uint64_t*);
+__global__ void nonfinite_min_kernel(const uint64_t*, int, uint64_t*);
 template <int G>
 __global__ void union_count_kernel(const DsDesc*, int, int64_t, int64_t, const uint32_t*,
                                    const float*, const uint32_t*, const uint32_t*,
@@ -93,8 +96,42 @@ struct hm_ctx {
   hipStream_t stream = nullptr;
   uint64_t budget = 0;
   uint64_t in_use = 0;
+  uint8_t* scratch = nullptr;  // the device's copy of `scratch` (hm_knn_graph_noalloc)
   std::string err;
 };
+
+// Device memory of the allocation-free entry point: every intermediate
+// buffer of hm_knn_graph_noalloc is carved from this array. Each GPU holds
+// its own copy (at its own address) from the library's load until the
+// process ends. Host code never uses the array's name as an address: the
+// device address comes from hipGetSymbolAddress, once per device
+// (scratch_address), and only that address reaches kernels and copies.
+static __device__ __align__(256) uint8_t scratch[HM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+// The last 16 KiB hold the input validation's partial results; the rest is
+// the job's.
+constexpr uint64_t kScratchTail = 16384;
+std::mutex g_scratch_mu[kMaxDevices];  // serialises the calls that share a device's array
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_init_mu;
+
+// The scratch array's address on `device` (the current device), looked up
+// once per device; nullptr with *err set if the lookup fails.
+uint8_t* scratch_address(int device, hipError_t* err) {
+  std::lock_guard<std::mutex> lock(g_scratch_init_mu);
+  *err = hipSuccess;
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    *err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (*err != hipSuccess) return nullptr;
+    g_scratch_ptr[device] = (uint8_t*)p;
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
 
 namespace {
 
@@ -146,6 +183,11 @@ struct hm_job {
   bool tuned = false;    // autotuning done (hm_options.autotune)
   float tune_ms = 0.f;   // best pipeline time the autotuner measured
   bool counted = false;  // bytes added to the context's in_use
+  // allocation-free jobs (hm_knn_graph_noalloc): buffers are carved from the
+  // device's scratch array and the stream is the context's
+  uint8_t* arena = nullptr;
+  uint64_t arena_cap = 0;
+  bool own_stream = true;
   std::vector<WorkItem> items_l1_h;
   std::vector<void*> allocs;
   uint64_t bytes = 0;
@@ -351,6 +393,15 @@ template <typename T>
 hm_status dalloc(hm_job* J, T** p, int64_t count) {
   if (count <= 0) count = 1;
   const size_t bytes = (size_t)count * sizeof(T);
+  if (J->arena) {
+    const uint64_t off = (J->bytes + 255) & ~(uint64_t)255;
+    if (off + bytes > J->arena_cap)
+      return fail(J->ctx, HM_EBUDGET, "the job needs more than the " +
+                                          std::to_string(J->arena_cap) + "-byte scratch array");
+    J->bytes = off + bytes;
+    *p = (T*)(J->arena + off);
+    return HM_OK;
+  }
   void* q = nullptr;
   // stream-ordered: creating and destroying jobs never synchronises the
   // device, so jobs on other streams keep running (the context's pool keeps
@@ -727,6 +778,16 @@ hm_status hm_ctx_create(hm_ctx** out, int device, uint64_t budget) {
     delete c;
     return HM_EDEVICE;
   }
+  {  // the allocation-free entry point's array on this device
+    hipError_t e = hipSuccess;
+    c->scratch = scratch_address(device, &e);
+    if (!c->scratch) {
+      (void)hipGetLastError();
+      (void)hipStreamDestroy(c->stream);
+      delete c;
+      return HM_EDEVICE;
+    }
+  }
   // job buffers come from the device's default pool; keep freed blocks
   // cached instead of returning them at every synchronisation
   hipMemPool_t pool = nullptr;
@@ -848,9 +909,14 @@ hm_status apply_sweep(hm_job* J) {
   return HM_OK;
 }
 
-hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint32_t* d,
-                        const uint32_t* k, const int32_t* metric, const hm_options* opt,
-                        uint32_t stages, hm_job** out) {
+// Job planning and allocation. arena == nullptr: buffers from the context's
+// memory pool on a stream of the job's own (hm_job_create); otherwise every
+// buffer is carved from `arena` (arena_cap bytes of device memory) and the
+// job runs on the context's stream, with no device allocation, stream or
+// event created (hm_knn_graph_noalloc).
+static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint32_t* d,
+                            const uint32_t* k, const int32_t* metric, const hm_options* opt,
+                            uint32_t stages, uint8_t* arena, uint64_t arena_cap, hm_job** out) {
   if (!ctx || !out || nds == 0 || !n || !d || !k) return fail(ctx, HM_EINVAL, "bad job arguments");
   *out = nullptr;
   hm_options o;
@@ -882,14 +948,24 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
   if (P.total_rows >= 0xffffffffll || P.total_edges >= 0xffffffffll)
     return fail(ctx, HM_EUNSUPPORTED, "batch too large for 32-bit row/edge ids");
   const uint64_t need = plan_bytes(P, o, stages);
-  if (ctx->budget && ctx->in_use + need > ctx->budget)
+  if (arena) {
+    if (need > arena_cap)
+      return fail(ctx, HM_EBUDGET, "job needs " + std::to_string(need) + " bytes; the scratch "
+                                   "array holds " + std::to_string(arena_cap));
+  } else if (ctx->budget && ctx->in_use + need > ctx->budget) {
     return fail(ctx, HM_EBUDGET, "job needs " + std::to_string(need) + " bytes, budget left " +
                                      std::to_string(ctx->budget - ctx->in_use));
+  }
 
   hm_job* J = new hm_job();
   J->ctx = ctx;
-  // every job has its own stream: jobs of one context overlap on the device
-  if (hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (arena) {
+    J->arena = arena;
+    J->arena_cap = arena_cap;
+    J->own_stream = false;
+    J->stream = ctx->stream;
+  } else if (hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking) != hipSuccess) {
+    // every job has its own stream: jobs of one context overlap on the device
     delete J;
     return fail(ctx, HM_EDEVICE, "stream create failed");
   }
@@ -959,8 +1035,10 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
   HM_A(J->rho, R);
   HM_A(J->sigma64, R);
   HM_A(J->zany, nds);  // per-dataset flags, kept clear between runs by calib_fixup_kernel
-  if (hipMemsetAsync(J->zany, 0, sizeof(uint32_t) * nds, J->stream) != hipSuccess)
-    return bail(fail(ctx, HM_EDEVICE, "flag clear failed"));
+  // (a fill kernel: the scratch array keeps the last call's contents)
+  hipLaunchKernelGGL(fill_u32, dim3(grid_for(nds, 256)), dim3(256), 0, J->stream, J->zany,
+                     (int64_t)nds, 0u);
+  if (hipGetLastError() != hipSuccess) return bail(fail(ctx, HM_EDEVICE, "flag clear failed"));
   HM_A(J->target, nds);
   const int64_t ntr = cdiv(E, kRadixTile) + 1;
   if (stages & HM_STAGE_FUZZY) {
@@ -1051,13 +1129,21 @@ hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint
   if (st == HM_OK && hipStreamSynchronize(J->stream) != hipSuccess)  // host sources are locals
     st = fail(ctx, HM_EDEVICE, "descriptor upload failed");
   if (st != HM_OK) return bail(st);
-  for (auto& e : J->ev) {
-    if (hipEventCreate(&e) != hipSuccess) return bail(fail(ctx, HM_EDEVICE, "event create"));
+  if (!arena) {
+    for (auto& e : J->ev) {
+      if (hipEventCreate(&e) != hipSuccess) return bail(fail(ctx, HM_EDEVICE, "event create"));
+    }
+    ctx->in_use += J->bytes;
+    J->counted = true;
   }
-  ctx->in_use += J->bytes;
-  J->counted = true;
   *out = J;
   return HM_OK;
+}
+
+hm_status hm_job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const uint32_t* d,
+                        const uint32_t* k, const int32_t* metric, const hm_options* opt,
+                        uint32_t stages, hm_job** out) {
+  return job_create(ctx, nds, n, d, k, metric, opt, stages, nullptr, 0, out);
 }
 
 void hm_job_destroy(hm_job* J) {
@@ -1068,7 +1154,7 @@ void hm_job_destroy(hm_job* J) {
   for (auto e : J->ev)
     if (e) (void)hipEventDestroy(e);
   for (void* p : J->allocs) (void)hipFreeAsync(p, J->stream);
-  if (J->stream) {
+  if (J->stream && J->own_stream) {
     (void)hipStreamSynchronize(J->stream);
     (void)hipStreamDestroy(J->stream);
   }
@@ -1439,6 +1525,71 @@ hm_status hm_knn_graph(hm_ctx* ctx, const float* data, uint64_t n, uint32_t d, u
   if (s == HM_OK) s = hm_job_run(J);
   if (s == HM_OK) s = hm_job_sync(J);
   if (s == HM_OK) s = hm_job_download(J, 0, out);
+  hm_job_destroy(J);
+  return s;
+}
+
+hm_status hm_knn_graph_noalloc(hm_ctx* ctx, const float* data, uint64_t n, uint32_t d,
+                               uint32_t k, int32_t metric, const hm_options* opt,
+                               hm_graph_out* out) {
+  if (!ctx || !data || !out) return fail(ctx, HM_EINVAL, "null argument");
+  if (!ctx->scratch) return fail(ctx, HM_EDEVICE, "no scratch array on this device");
+  const hm_status v = hm_validate(nullptr, n * d, d, k, nullptr);  // shape and k
+  if (v != HM_OK) return fail(ctx, v, "input validation failed");
+  // calls on one device share its array: one at a time
+  std::lock_guard<std::mutex> lock(g_scratch_mu[ctx->device]);
+  HM_HIP(ctx, hipSetDevice(ctx->device));
+  hm_options o;
+  if (opt) o = *opt; else hm_default_options(&o);
+  o.use_graph = 0;  // one eager run; no events, no autotuning runs
+  o.autotune = 0;
+  o.timing = 0;
+  hipStream_t st = ctx->stream;
+  // finite input (src/api.rs:142-144), checked on the device
+  uint64_t* part = (uint64_t*)(ctx->scratch + (HM_NOALLOC_SCRATCH_BYTES - kScratchTail));
+  uint64_t* first = part + 1024;
+  const int64_t len = (int64_t)(n * d);
+  const unsigned g = (unsigned)std::min<int64_t>(1024, cdiv(len, 256));
+  hipLaunchKernelGGL(nonfinite_part_kernel, dim3(g), dim3(256), 0, st, data, len, part);
+  hipLaunchKernelGGL(nonfinite_min_kernel, dim3(1), dim3(256), 0, st, (const uint64_t*)part,
+                     (int)g, first);
+  HM_LAUNCH(ctx);
+  uint64_t bad = 0;
+  HM_HIP(ctx, hipMemcpyAsync(&bad, first, 8, hipMemcpyDeviceToHost, st));
+  HM_HIP(ctx, hipStreamSynchronize(st));
+  if (bad != ~0ull)
+    return fail(ctx, HM_ENONFINITE, "non-finite input at index " + std::to_string(bad));
+  hm_job* J = nullptr;
+  HM_TRY(job_create(ctx, 1, &n, &d, &k, &metric, &o, HM_STAGE_ALL, ctx->scratch,
+                    HM_NOALLOC_SCRATCH_BYTES - kScratchTail, &J));
+  hm_status s = hm_job_upload_device(J, 0, data);
+  if (s == HM_OK) s = hm_job_run(J);
+  uint64_t nnz = 0;
+  if (s == HM_OK && (hipMemcpyAsync(&nnz, J->total_nnz, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                     hipStreamSynchronize(st) != hipSuccess))
+    s = fail(ctx, HM_EDEVICE, "graph build failed");
+  if (s == HM_OK) {
+    out->nnz = nnz;
+    if ((out->rows || out->cols || out->vals) && out->cap < nnz)
+      s = fail(ctx, HM_EINVAL, "COO capacity " + std::to_string(out->cap) + " < nnz " +
+                                   std::to_string(nnz));
+  }
+  auto copy = [&](void* dst, const void* src, uint64_t bytes) {
+    if (s == HM_OK && dst && bytes &&
+        hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+      s = fail(ctx, HM_EDEVICE, "output copy failed");
+  };
+  copy(out->indices, J->knn_idx, n * k * 4);
+  copy(out->dists, J->knn_dist, n * k * 4);
+  copy(out->sigmas, J->sigma, n * 4);
+  copy(out->rhos, J->rho, n * 4);
+  copy(out->sigmas64, J->sigma64, n * 8);
+  copy(out->rows, J->rows, nnz * 4);
+  copy(out->cols, J->cols, nnz * 4);
+  copy(out->vals, J->vals, nnz * 4);
+  copy(out->indptr, J->indptr_l, (n + 1) * 8);
+  if (s == HM_OK && hipStreamSynchronize(st) != hipSuccess)
+    s = fail(ctx, HM_EDEVICE, "output copy failed");
   hm_job_destroy(J);
   return s;
 }

@@ -162,6 +162,25 @@ typedef struct {
 hm_status hm_knn_graph(hm_ctx* ctx, const float* data, uint64_t n, uint32_t n_features,
                        uint32_t k, int32_t metric, const hm_options* opt, hm_graph_out* out);
 
+/* ---- allocation-free graph build ----
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under stream capture, serving loops and harnesses that
+ * forbid hipMalloc): hm_knn_graph's computation with the input `data`
+ * (n * n_features floats) and every non-NULL field of `out` in the
+ * caller's DEVICE memory, and every intermediate buffer carved from a fixed
+ * scratch array of HM_NOALLOC_SCRATCH_BYTES that each GPU the library runs
+ * on holds until the process ends. It calls no hipMalloc or hipFree.
+ * Inputs whose graph build needs more than the array (about N = 30,000
+ * rows at d = 128, k = 15) fail with HM_EBUDGET. Calls on one GPU share its
+ * array: overlapping calls would race on it, so the library runs them one
+ * at a time (a lock per device), each returning when its device work and
+ * output copies are done; out->nnz is written on the host. Runs on the
+ * context's stream; options use_graph, autotune and timing are ignored. */
+#define HM_NOALLOC_SCRATCH_BYTES (256ull << 20)
+hm_status hm_knn_graph_noalloc(hm_ctx* ctx, const float* data, uint64_t n, uint32_t n_features,
+                               uint32_t k, int32_t metric, const hm_options* opt,
+                               hm_graph_out* out);
+
 /* ---- device-resident jobs: one or many datasets per launch ---- */
 
 /* Plans a batch of `nds` datasets (shapes n[i] x d[i], k[i], metric[i]).
