@@ -389,7 +389,8 @@ class Job:
         a = np.zeros(_lib.HM_NCOUNTERS, np.uint64)
         self.ctx._check(_lib.load().hm_job_counters(self.handle, a.ctypes.data))
         return {"fallback_rows": int(a[0]), "nnz": int(a[1]), "candidates": int(a[2]),
-                "nsplit": int(a[3]), "escalated_rows": int(a[4])}
+                "nsplit": int(a[3]), "escalated_rows": int(a[4]), "listed": int(a[5]),
+                "target": int(a[6]), "sample_stride": int(a[7])}
 
     def download(self, ds: int, with_sigma64: bool = False) -> GraphResult:
         n, k = int(self.n[ds]), int(self.k[ds])

@@ -36,7 +36,7 @@ HM_STAGE_CALIB = 2
 HM_STAGE_FUZZY = 4
 HM_STAGE_ALL = 7
 HM_NTIMES = 8
-HM_NCOUNTERS = 5
+HM_NCOUNTERS = 8
 TIME_NAMES = ["prep", "sweep", "rerank", "fallback", "calib", "fuzzy", "total"]
 
 # every symbol include/holomap_gpu.h declares

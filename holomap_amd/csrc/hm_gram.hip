@@ -349,23 +349,57 @@ __device__ __forceinline__ float sweep_threshold(const DsDesc& D, int64_t row, f
 template <int PASS>
 __device__ __forceinline__ void sweep_finish(const float (&gmax)[4][16], float T, int cnt,
                                              bool myvalid, int64_t myrow, const DsDesc& D,
-                                             int split, int half, float scale) {
+                                             int split, int half, float scale, int C) {
   if constexpr (PASS == 0) {
-    const int C = D.C;
-    uint32_t lo = 0u, hi = 0xffffffffu;
-    for (int itb = 0; itb < 32; ++itb) {
-      const uint32_t mid = lo + (hi - lo) / 2;
-      int c = 0;
+    if (C <= 8) {
+      // few to select (a sampled threshold pass): the C-th largest group
+      // maximum by extracting the largest value of the lane pair C times
+      // at most (with multiplicity); -inf when fewer than C values exist
+      float g[4][16];
 #pragma unroll
       for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int a = 0; a < 16; ++a) c += ordered_u32(-gmax[t][a]) <= mid ? 1 : 0;
-      c += __shfl_xor(c, 32, 64);
-      if (c >= C) hi = mid; else lo = mid + 1;
+        for (int a = 0; a < 16; ++a) g[t][a] = gmax[t][a];
+      float m = -INFINITY;
+      int need = C;
+      for (int it = 0; it < C; ++it) {  // uniform over the wave
+        m = -INFINITY;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int a = 0; a < 16; ++a) m = fmaxf(m, g[t][a]);
+        m = fmaxf(m, __shfl_xor(m, 32, 64));
+        int c = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int a = 0; a < 16; ++a) c += g[t][a] == m ? 1 : 0;
+        c += __shfl_xor(c, 32, 64);
+        if (c >= need || !(m > -INFINITY)) break;
+        need -= c;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int a = 0; a < 16; ++a) g[t][a] = g[t][a] == m ? -INFINITY : g[t][a];
+      }
+      if (myvalid && half == 0) D.thr[(size_t)myrow * D.nsplit + split] = scale * m;
+    } else {
+      uint32_t lo = 0u, hi = 0xffffffffu;
+      for (int itb = 0; itb < 32; ++itb) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        int c = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int a = 0; a < 16; ++a) c += ordered_u32(-gmax[t][a]) <= mid ? 1 : 0;
+        c += __shfl_xor(c, 32, 64);
+        if (c >= C) hi = mid; else lo = mid + 1;
+      }
+      // lo == hi: the smallest image with >= C values at or below it (the
+      // image of +inf when the split has fewer than C columns); T = -s * it
+      if (myvalid && half == 0)
+        D.thr[(size_t)myrow * D.nsplit + split] = -scale * unordered_f32(lo);
     }
-    // lo == hi: the smallest image with >= C values at or below it (the
-    // image of +inf when the split has fewer than C columns); T = -s * it
-    if (myvalid && half == 0) D.thr[(size_t)myrow * D.nsplit + split] = -scale * unordered_f32(lo);
   } else {
     if (myvalid) {
       // every column not listed has key > T; an overflowing list reports 0
@@ -381,7 +415,8 @@ __device__ __forceinline__ void sweep_finish(const float (&gmax)[4][16], float T
 template <int MODE, int PASS>
 __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
     const DsDesc* __restrict__ ds_arr, const WorkItem* __restrict__ items,
-    const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist) {
+    const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist,
+    int sample) {
   using Cfg = SweepCfg<MODE>;
   constexpr int NI = Cfg::NI, NSTAGE = Cfg::NSTAGE, STAGE = Cfg::STAGE, QG = Cfg::QG;
   __shared__ __attribute__((aligned(1024))) uint8_t smem[NSTAGE * STAGE + 512];
@@ -409,9 +444,14 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
     const int64_t cbeg = (int64_t)W.split * span;
     const int64_t cend = cbeg + span < n ? cbeg + span : n;
     const int nchunk = rowbytes / 128;
-    const int nblk = cend > cbeg ? (int)((cend - cbeg + 127) / 128) : 0;
+    // the threshold pass of a sampled sweep visits every bstride-th block
+    const int bstride = (PASS == 0 && sample) ? D.pstride : 1;
+    const int csel = (PASS == 0 && sample) ? D.C0 : D.C;
+    const int nblk_all = cend > cbeg ? (int)((cend - cbeg + 127) / 128) : 0;
+    const int nblk = (nblk_all + bstride - 1) / bstride;
     const int total = nblk * nchunk;
-    const int64_t blkbytes = (int64_t)128 * rowbytes;
+    const int64_t blkbytes = (int64_t)128 * rowbytes * bstride;  // between swept blocks
+    const int64_t bcols = (int64_t)128 * bstride;
 
     const uint8_t* ophi = (MODE == 1) ? (const uint8_t*)D.op32 : (const uint8_t*)D.op16;
     const uint8_t* oplo = (MODE == 2) ? (const uint8_t*)D.op16lo : ophi;
@@ -480,7 +520,7 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
 #pragma unroll
     for (int g = 0; g < NSTAGE - 1; ++g) {
       if (g < total) {
-        sweep_issue<MODE>(smem + g * STAGE, P, bi * blkbytes, ci * 128, (int64_t)(bi + 1) * 128,
+        sweep_issue<MODE>(smem + g * STAGE, P, bi * blkbytes, ci * 128, (int64_t)(bi + 1) * bcols,
                           wave);
         if (++ci == nchunk) { ci = 0; ++bi; }
       }
@@ -513,7 +553,7 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
       const int gi = g + NSTAGE - 1;
       if (gi < total) {
         sweep_issue<MODE>(smem + (gi % NSTAGE) * STAGE, P, bi * blkbytes, ci * 128,
-                          (int64_t)(bi + 1) * 128, wave);
+                          (int64_t)(bi + 1) * bcols, wave);
         if (++ci == nchunk) { ci = 0; ++bi; }
       }
       const uint8_t* st = smem + (g % NSTAGE) * STAGE;
@@ -530,13 +570,14 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
 #pragma unroll
           for (int t = 0; t < 4; ++t) acc[u][t] = a0[t];
         ch = 0;
-        colbase += 128;
+        colbase += (uint32_t)bcols;
       }
     }
 
 #pragma unroll
     for (int u = 0; u < NU; ++u)
-      sweep_finish<PASS>(gmax[u], T[u], cnt[u], myvalid[u], myrow[u], D, W.split, half, scale);
+      sweep_finish<PASS>(gmax[u], T[u], cnt[u], myvalid[u], myrow[u], D, W.split, half, scale,
+                         csel);
   }
 }
 
@@ -565,7 +606,8 @@ __device__ __forceinline__ void qres_issue(uint8_t* stage, const uint8_t* const 
 template <class Cfg, int PASS>
 __global__ __launch_bounds__(Cfg::NW * 64, 2) void gram_sweep_qres_kernel(
     const DsDesc* __restrict__ ds_arr, const WorkItem* __restrict__ items,
-    const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist) {
+    const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist,
+    int sample) {
   constexpr int NI = Cfg::NI, NSTAGE = Cfg::NSTAGE, STAGE = Cfg::STAGE, QW = Cfg::QW;
   __shared__ __attribute__((aligned(1024))) uint8_t smem[Cfg::LDS];
 
@@ -586,9 +628,14 @@ __global__ __launch_bounds__(Cfg::NW * 64, 2) void gram_sweep_qres_kernel(
     const int64_t cbeg = (int64_t)W.split * span;
     const int64_t cend = cbeg + span < n ? cbeg + span : n;
     const int nchunk = rowbytes / 128;
-    const int nblk = cend > cbeg ? (int)((cend - cbeg + 127) / 128) : 0;
+    // the threshold pass of a sampled sweep visits every bstride-th block
+    const int bstride = (PASS == 0 && sample) ? D.pstride : 1;
+    const int csel = (PASS == 0 && sample) ? D.C0 : D.C;
+    const int nblk_all = cend > cbeg ? (int)((cend - cbeg + 127) / 128) : 0;
+    const int nblk = (nblk_all + bstride - 1) / bstride;
     const int total = nblk * nchunk;
-    const int64_t blkbytes = (int64_t)128 * rowbytes;
+    const int64_t blkbytes = (int64_t)128 * rowbytes * bstride;  // between swept blocks
+    const int64_t bcols = (int64_t)128 * bstride;
     const uint8_t* op = (const uint8_t*)D.op16;
 
     // candidate copies: wave w copies groups w*NI + i (8 rows each) of a block
@@ -644,7 +691,7 @@ __global__ __launch_bounds__(Cfg::NW * 64, 2) void gram_sweep_qres_kernel(
     for (int g = 0; g < NSTAGE - 1; ++g) {
       if (g < total) {
         qres_issue<Cfg>(smem + g * STAGE, csrc, bsrc, blim, bi * blkbytes + ci * 128,
-                        (int64_t)(bi + 1) * 128, wave);
+                        (int64_t)(bi + 1) * bcols, wave);
         if (++ci == nchunk) { ci = 0; ++bi; }
       }
     }
@@ -669,7 +716,7 @@ __global__ __launch_bounds__(Cfg::NW * 64, 2) void gram_sweep_qres_kernel(
       HM_LDS_BARRIER();
       if (g + NSTAGE - 1 < total) {
         qres_issue<Cfg>(smem + ((g + NSTAGE - 1) % NSTAGE) * STAGE, csrc, bsrc, blim,
-                        bi * blkbytes + ci * 128, (int64_t)(bi + 1) * 128, wave);
+                        bi * blkbytes + ci * 128, (int64_t)(bi + 1) * bcols, wave);
         if (++ci == nchunk) { ci = 0; ++bi; }
       }
       const uint8_t* st = smem + (g % NSTAGE) * STAGE;
@@ -678,16 +725,17 @@ __global__ __launch_bounds__(Cfg::NW * 64, 2) void gram_sweep_qres_kernel(
         sweep_epilogue<PASS>(acc[0], gmax, colbase, T, D.cout, dst, cnt);
         sweep_acc_start(acc[0], st + Cfg::BIAS, half);
         ch = 0;
-        colbase += 128;
+        colbase += (uint32_t)bcols;
       }
     }
-    sweep_finish<PASS>(gmax, T, cnt, myvalid, myrow, D, W.split, half, scale);
+    sweep_finish<PASS>(gmax, T, cnt, myvalid, myrow, D, W.split, half, scale, csel);
   }
 }
 
 #define HM_QRES_INST(CFG, PASS)                                                                \
   template __global__ void gram_sweep_qres_kernel<CFG, PASS>(const DsDesc*, const WorkItem*,      \
-                                                             const int*, int, const uint32_t*);
+                                                             const int*, int, const uint32_t*,   \
+                                                             int);
 HM_QRES_INST(QresWide, 0)
 HM_QRES_INST(QresWide, 1)
 HM_QRES_INST(QresPair, 0)
@@ -696,7 +744,7 @@ HM_QRES_INST(QresPair, 1)
 
 #define HM_INST(MODE, PASS)                                                       \
   template __global__ void gram_sweep_kernel<MODE, PASS>(const DsDesc*, const WorkItem*, \
-                                                         const int*, int, const uint32_t*);
+                                                         const int*, int, const uint32_t*, int);
 HM_INST(0, 0)
 HM_INST(0, 1)
 HM_INST(1, 0)

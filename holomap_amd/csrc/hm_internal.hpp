@@ -45,6 +45,9 @@ struct DsDesc {
   int32_t nlist;         // candidate lists per row = 2 * nsplit (one per
                          //   half of the 64-lane wave, see hm_gram.hip)
   int32_t cout;          // key slots per list
+  int32_t C0;            // threshold pass: group maxima selected per (row, split)
+  int32_t pstride;       // threshold pass: every pstride-th 128-column block
+                         //   of a split is swept (a sample of its columns)
 };
 
 // Query rows per work item of the candidate sweep (hm_gram.hip): 256 in the

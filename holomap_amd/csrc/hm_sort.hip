@@ -257,4 +257,31 @@ __global__ void fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Sum of a u32 array in two launches (a fixed order: the same total on every
+// run, no atomics): per-workgroup partial sums, then one workgroup adds them.
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v) {
+  __shared__ uint64_t red[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += shfl_xor_u64(v, o);
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+__global__ __launch_bounds__(256) void sum_u32_part_kernel(const uint32_t* __restrict__ a,
+                                                           int64_t n, uint64_t* __restrict__ part) {
+  uint64_t s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    s += a[i];
+  const uint64_t t = block_sum_u64(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+__global__ __launch_bounds__(256) void sum_u64_kernel(const uint64_t* __restrict__ part, int np,
+                                                      uint64_t* __restrict__ out) {
+  uint64_t s = 0;
+  for (int i = threadIdx.x; i < np; i += 256) s += part[i];
+  const uint64_t t = block_sum_u64(s);
+  if (threadIdx.x == 0) *out = t;
+}
+
 }  // namespace hm

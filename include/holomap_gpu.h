@@ -95,12 +95,15 @@ typedef struct {
 #define HM_T_TOTAL 6
 
 /* Counters of the last run (hm_job_counters). */
-#define HM_NCOUNTERS 5
+#define HM_NCOUNTERS 8
 #define HM_C_FALLBACK_ROWS 0 /* rows recomputed by the exact scan */
 #define HM_C_NNZ 1           /* graph entries, all datasets */
 #define HM_C_CANDIDATES 2    /* candidate key slots per row (lists x list capacity) */
-#define HM_C_NSPLIT 3        /* column splits used by the sweep */
-#define HM_C_ESCALATED 4     /* rows the BF16 keys left uncertified, swept again in FP32 */
+#define HM_C_NSPLIT 3        /* column splits used by the sweep (dataset 0) */
+#define HM_C_ESCALATED 4     /* rows the primary keys left uncertified, swept again */
+#define HM_C_LISTED 5        /* candidate keys listed for dataset 0's rows */
+#define HM_C_TARGET 6        /* candidates targeted per row, k + margin (dataset 0) */
+#define HM_C_SAMPLE 7        /* block stride of the threshold pass's sample (dataset 0) */
 
 void hm_default_options(hm_options* opt);
 const char* hm_version(void);

@@ -309,7 +309,9 @@ def test_few_uncertified_rows_use_wide_fallback(ctx):
         fb = job.counters()["fallback_rows"]
         job.close()
         assert_graph_equal(r, *ref)
-        assert 1 <= fb <= 201, fb
+        # those rows, and the occasional row whose sampled threshold listed
+        # too few candidates to certify
+        assert 1 <= fb <= 260, fb
 
 
 def test_escalation_then_exact_scan_for_the_rest(ctx):
