@@ -96,6 +96,34 @@ __device__ __forceinline__ int calib_find_ds(const DsDesc* ds, int nds, int64_t 
   return lo;
 }
 
+// Directed membership strength of edge t of row i (global row g) of
+// dataset D (src/fuzzy.rs:31-48): w = 1 if d - rho <= 0 or sigma == 0, else
+// exp(-((d - rho) / sigma)); self edges and w == 0 are dropped (key1 =
+// sentinel). Written for the symmetrisation's sort: key1 = global column,
+// pay1 = edge id, wval = w, srcrow = g.
+__device__ __forceinline__ void directed_edge(const DsDesc& D, int64_t i, int64_t g, int t,
+                                              float d, float rho, float sigma, uint32_t sentinel,
+                                              uint32_t* __restrict__ key1,
+                                              uint32_t* __restrict__ pay1,
+                                              float* __restrict__ wval,
+                                              uint32_t* __restrict__ srcrow) {
+  const int64_t le = i * D.k + t;
+  const int64_t e = D.edge_off + le;
+  const uint32_t col = D.knn_idx[le];
+  const float dd = d - rho;
+  const float w = (dd <= 0.0f || sigma == 0.0f) ? 1.0f : hm_expf(-(dd / sigma));
+  const bool keep = (int64_t)col != i && w != 0.0f;
+  key1[e] = keep ? (uint32_t)(D.row_off + col) : sentinel;
+  pay1[e] = (uint32_t)e;
+  wval[e] = w;
+  srcrow[e] = (uint32_t)g;
+}
+
+// The calibration kernels also write the row's directed edges when key1 is
+// set (the symmetrisation follows): a row's sigma is final here except for
+// the global-mean floor of rows with rho == 0, whose distances are all 0, so
+// their weights are 1 whatever sigma is.
+//
 // target_log2k[ds] is log2(k) computed on the host (glibc), as the
 // reference's std f64::log2 does.
 //
@@ -119,7 +147,9 @@ template <int KM>
 __global__ __launch_bounds__(256) void smooth_knn_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
     const double* __restrict__ target_log2k, int mode, float* __restrict__ sigma,
-    float* __restrict__ rho, double* __restrict__ sigma64, uint32_t* __restrict__ zany) {
+    float* __restrict__ rho, double* __restrict__ sigma64, uint32_t* __restrict__ zany,
+    uint32_t sentinel, uint32_t* __restrict__ key1, uint32_t* __restrict__ pay1,
+    float* __restrict__ wval, uint32_t* __restrict__ srcrow) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total_rows) return;
   const int di = calib_find_ds(ds, nds, g);
@@ -182,6 +212,7 @@ __global__ __launch_bounds__(256) void smooth_knn_kernel(
     }
     sigma[g] = s;
     if (sigma64) sigma64[g] = (double)s;
+    if (key1) HM_ROW_LOOP(0, { directed_edge(D, i, g, t, X(t), r, s, sentinel, key1, pay1, wval, srcrow); });
   } else {
     const double rr = (double)r;
     double lo = 0.0, hi = INFINITY, mid = 1.0;
@@ -214,18 +245,21 @@ __global__ __launch_bounds__(256) void smooth_knn_kernel(
     }
     if (sigma64) sigma64[g] = s;
     sigma[g] = (float)s;
+    const float sf = (float)s;
+    if (key1) HM_ROW_LOOP(0, { directed_edge(D, i, g, t, X(t), r, sf, sentinel, key1, pay1, wval, srcrow); });
   }
 #undef X
 }
 #undef HM_ROW_LOOP
-template __global__ void smooth_knn_kernel<0>(const DsDesc*, int, int64_t, const double*, int,
-                                              float*, float*, double*, uint32_t*);
-template __global__ void smooth_knn_kernel<16>(const DsDesc*, int, int64_t, const double*, int,
-                                               float*, float*, double*, uint32_t*);
-template __global__ void smooth_knn_kernel<32>(const DsDesc*, int, int64_t, const double*, int,
-                                               float*, float*, double*, uint32_t*);
-template __global__ void smooth_knn_kernel<64>(const DsDesc*, int, int64_t, const double*, int,
-                                               float*, float*, double*, uint32_t*);
+#define HM_CALIB_T_INST(KM)                                                                    \
+  template __global__ void smooth_knn_kernel<KM>(const DsDesc*, int, int64_t, const double*, int, \
+                                                 float*, float*, double*, uint32_t*, uint32_t,   \
+                                                 uint32_t*, uint32_t*, float*, uint32_t*);
+HM_CALIB_T_INST(0)
+HM_CALIB_T_INST(16)
+HM_CALIB_T_INST(32)
+HM_CALIB_T_INST(64)
+#undef HM_CALIB_T_INST
 
 // Same calibration with G lanes per row (k <= G <= 64): lane t holds the
 // row's distance t and evaluates term t of every bisection step (the exp
@@ -321,7 +355,9 @@ template <int G>
 __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
     const double* __restrict__ target_log2k, int mode, float* __restrict__ sigma,
-    float* __restrict__ rho, double* __restrict__ sigma64, uint32_t* __restrict__ zany) {
+    float* __restrict__ rho, double* __restrict__ sigma64, uint32_t* __restrict__ zany,
+    uint32_t sentinel, uint32_t* __restrict__ key1, uint32_t* __restrict__ pay1,
+    float* __restrict__ wval, uint32_t* __restrict__ srcrow) {
   static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0 && 256 % G == 0, "group size");
   const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const int l = (int)(threadIdx.x % G);
@@ -384,6 +420,7 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
       sigma[g] = s;
       if (sigma64) sigma64[g] = (double)s;
     }
+    if (key1 && l < k) directed_edge(D, i, g, l, x, r, s, sentinel, key1, pay1, wval, srcrow);
   } else {
     const double rr = (double)r;
     double lo = 0.0, hi = INFINITY, mid = 1.0;
@@ -428,15 +465,19 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
       if (sigma64) sigma64[g] = s;
       sigma[g] = (float)s;
     }
+    if (key1 && l < k) directed_edge(D, i, g, l, x, r, (float)s, sentinel, key1, pay1, wval, srcrow);
   }
 }
 
-template __global__ void smooth_knn_group_kernel<16>(const DsDesc*, int, int64_t, const double*,
-                                                     int, float*, float*, double*, uint32_t*);
-template __global__ void smooth_knn_group_kernel<32>(const DsDesc*, int, int64_t, const double*,
-                                                     int, float*, float*, double*, uint32_t*);
-template __global__ void smooth_knn_group_kernel<64>(const DsDesc*, int, int64_t, const double*,
-                                                     int, float*, float*, double*, uint32_t*);
+#define HM_CALIB_G_INST(G)                                                                    \
+  template __global__ void smooth_knn_group_kernel<G>(const DsDesc*, int, int64_t,              \
+                                                      const double*, int, float*, float*,       \
+                                                      double*, uint32_t*, uint32_t, uint32_t*,  \
+                                                      uint32_t*, float*, uint32_t*);
+HM_CALIB_G_INST(16)
+HM_CALIB_G_INST(32)
+HM_CALIB_G_INST(64)
+#undef HM_CALIB_G_INST
 
 // Applies the global-mean floor to rows with rho == 0. One workgroup per
 // dataset; the mean is the reference's left-to-right f64 fold, computed by
@@ -478,8 +519,8 @@ __global__ __launch_bounds__(256) void calib_fixup_kernel(const DsDesc* __restri
   if (threadIdx.x == 0) zany[blockIdx.x] = 0u;  // cleared for the next run
 }
 
-// Directed membership strengths (src/fuzzy.rs:31-48), one thread per edge.
-// key1[e] = global column id, or `sentinel` for dropped edges (self, w == 0).
+// Directed membership strengths (src/fuzzy.rs:31-48), one thread per edge,
+// for jobs whose sigma / rho are given rather than calibrated.
 __global__ void directed_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_edges,
                                 const float* __restrict__ sigma, const float* __restrict__ rho,
                                 uint32_t sentinel, uint32_t* __restrict__ key1,
@@ -496,15 +537,8 @@ __global__ void directed_kernel(const DsDesc* __restrict__ ds, int nds, int64_t 
   const int64_t le = e - D.edge_off;
   const int64_t i = le / D.k;
   const int64_t g = D.row_off + i;
-  const uint32_t col = D.knn_idx[le];
-  const float dd = D.knn_dist[le] - rho[g];
-  const float s = sigma[g];
-  const float w = (dd <= 0.0f || s == 0.0f) ? 1.0f : hm_expf(-(dd / s));
-  const bool keep = (int64_t)col != i && w != 0.0f;
-  key1[e] = keep ? (uint32_t)(D.row_off + col) : sentinel;
-  pay1[e] = (uint32_t)e;
-  wval[e] = w;
-  srcrow[e] = (uint32_t)g;
+  directed_edge(D, i, g, (int)(le - i * D.k), D.knn_dist[le], rho[g], sigma[g], sentinel, key1,
+                pay1, wval, srcrow);
 }
 
 // Standalone expf over an array (parity tests of the device expf).

@@ -68,8 +68,6 @@ constexpr int kMaxK = 1024;
 // Queries per wave of the exact-scan shape for a job whose largest k is k
 // (exact_scan_kernel<QPW, QCAP>, QCAP >= k + 64; items hold 4 * QPW queries).
 __host__ __device__ constexpr int xs_qpw_for_k(int k) { return k <= 64 ? 8 : (k <= 448 ? 4 : 1); }
-// Workgroups per dataset of the per-dataset statistics maxima (hm_prep.hip).
-constexpr int kStatParts = 32;
 // Calibration: a thread per row (rather than a lane group per row) for jobs
 // of at least this many rows per CU (four waves per SIMD).
 constexpr int kCalibThreadRowsPerCu = 4 * 4 * 64;

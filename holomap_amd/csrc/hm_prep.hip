@@ -10,7 +10,8 @@
 //     row scaled to unit length), its bf16 (RNE) image, and the statistics
 //     the certification bound needs (representation error norms);
 //   * statmax_kernel: per-dataset maxima of the statistics (max is exact and
-//     order-free, computed by a fixed two-level tree).
+//     order-free): prep_kernel's workgroups write the maxima of their rows,
+//     one workgroup per dataset reduces them.
 #include "hm_internal.hpp"
 
 namespace hm {
@@ -92,26 +93,44 @@ __global__ void rownorm_kernel(const DsDesc* __restrict__ ds, int nds, int64_t t
 __device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int nds, int64_t pr,
                                          const int64_t* __restrict__ pad_off,
                                          const float* __restrict__ nz_g, int mode_bits,
-                                         int lane);
+                                         int lane, float (&stout)[HM_NSTAT]);
 
 // 16 lanes (one DPP row) per (dataset, padded row), four rows per wave, so
 // a wave overlaps the memory round trips of four rows. Writes xr, op32
 // (cosine), op16 (+ op16lo), colbias and rstat. mode_bits: 1 = produce op16
-// (bf16 sweep), 2 = bf16x3 stats.
+// (bf16 sweep), 2 = bf16x3 stats. A workgroup's 16 rows belong to one
+// dataset (padded row counts are multiples of 128); it also writes the
+// maxima of their statistics to part[blockIdx.x] (statmax_kernel reduces
+// them per dataset).
 __global__ __launch_bounds__(256) void prep_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_pad_rows,
     const int64_t* __restrict__ pad_off /* nds+1 prefix of n_pad */,
-    const float* __restrict__ nz_g, int mode_bits) {
+    const float* __restrict__ nz_g, int mode_bits, float* __restrict__ part) {
+  __shared__ float red[16][HM_NSTAT];
   const int64_t pr = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
-  if (pr >= total_pad_rows) return;  // whole rows leave together
-  prep_row(ds, nds, pr, pad_off, nz_g, mode_bits, (int)(threadIdx.x & 15));
+  float st[HM_NSTAT];
+#pragma unroll
+  for (int s = 0; s < HM_NSTAT; ++s) st[s] = 0.0f;
+  if (pr < total_pad_rows)  // (always: the grid covers whole 16-row groups)
+    prep_row(ds, nds, pr, pad_off, nz_g, mode_bits, (int)(threadIdx.x & 15), st);
+  if ((threadIdx.x & 15) == 15) {
+#pragma unroll
+    for (int s = 0; s < HM_NSTAT; ++s) red[threadIdx.x >> 4][s] = st[s];
+  }
+  __syncthreads();
+  if (threadIdx.x < HM_NSTAT) {
+    float m = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) m = fmaxf(m, red[r][threadIdx.x]);
+    part[(size_t)blockIdx.x * HM_NSTAT + threadIdx.x] = m;
+  }
 }
 
 // One padded row by one wave (prep_kernel's body).
 __device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int nds, int64_t pr,
                                          const int64_t* __restrict__ pad_off,
                                          const float* __restrict__ nz_g, int mode_bits,
-                                         int lane) {
+                                         int lane, float (&stout)[HM_NSTAT]) {
   // dataset by padded-row prefix
   int lo = 0, hi = nds - 1;
   while (lo < hi) {
@@ -215,16 +234,28 @@ __device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int nds,
       st[ST_LN] = (float)(sqrt(s_ln) * up);
       st[ST_HLN] = (float)(sqrt(s_hln) * up);
       st[ST_RN] = (float)(sqrt(s_rn) * up);
+#pragma unroll
+      for (int k = 0; k < HM_NSTAT; ++k) stout[k] = st[k];
     }
   }
 }
 
-// Per-dataset maxima of rstat in two launches (max is exact and order-free):
-// statmax_part_kernel, grid (datasets, P): workgroup p reduces rows
-// p, p + P, ... in 256-row strides into part[ds][p]; statmax_kernel, one
-// workgroup per dataset, reduces the P partial maxima.
-__device__ __forceinline__ void statmax_block(float (&m)[HM_NSTAT], float* __restrict__ out) {
+// Per-dataset maxima of the row statistics (max is exact and order-free):
+// one workgroup per dataset reduces the partial maxima prep_kernel wrote for
+// the dataset's 16-row groups, pad_off[ds] / 16 .. pad_off[ds + 1] / 16.
+__global__ __launch_bounds__(256) void statmax_kernel(const DsDesc* __restrict__ ds,
+                                                      const float* __restrict__ part,
+                                                      const int64_t* __restrict__ pad_off) {
   __shared__ float red[256][HM_NSTAT];
+  const DsDesc& D = ds[blockIdx.x];
+  const int64_t g0 = pad_off[blockIdx.x] / 16, g1 = pad_off[blockIdx.x + 1] / 16;
+  float m[HM_NSTAT];
+#pragma unroll
+  for (int s = 0; s < HM_NSTAT; ++s) m[s] = 0.0f;
+  for (int64_t g = g0 + threadIdx.x; g < g1; g += 256) {
+#pragma unroll
+    for (int s = 0; s < HM_NSTAT; ++s) m[s] = fmaxf(m[s], part[g * HM_NSTAT + s]);
+  }
 #pragma unroll
   for (int s = 0; s < HM_NSTAT; ++s) red[threadIdx.x][s] = m[s];
   __syncthreads();
@@ -236,34 +267,7 @@ __device__ __forceinline__ void statmax_block(float (&m)[HM_NSTAT], float* __res
     }
     __syncthreads();
   }
-  if (threadIdx.x < HM_NSTAT) out[threadIdx.x] = red[0][threadIdx.x];
-}
-
-__global__ __launch_bounds__(256) void statmax_part_kernel(const DsDesc* __restrict__ ds,
-                                                           float* __restrict__ part) {
-  const DsDesc& D = ds[blockIdx.x];
-  constexpr int P = kStatParts;  // gridDim.y
-  const int p = (int)blockIdx.y;
-  float m[HM_NSTAT];
-#pragma unroll
-  for (int s = 0; s < HM_NSTAT; ++s) m[s] = 0.0f;
-  for (int64_t i = (int64_t)p * 256 + threadIdx.x; i < D.n; i += (int64_t)P * 256) {
-    const float* st = D.rstat + i * HM_NSTAT;
-#pragma unroll
-    for (int s = 0; s < HM_NSTAT; ++s) m[s] = fmaxf(m[s], st[s]);
-  }
-  statmax_block(m, part + ((size_t)blockIdx.x * P + p) * HM_NSTAT);
-}
-
-__global__ __launch_bounds__(256) void statmax_kernel(const DsDesc* __restrict__ ds,
-                                                      const float* __restrict__ part) {
-  const DsDesc& D = ds[blockIdx.x];
-  constexpr int P = kStatParts;
-  float m[HM_NSTAT];
-#pragma unroll
-  for (int s = 0; s < HM_NSTAT; ++s)
-    m[s] = threadIdx.x < P ? part[((size_t)blockIdx.x * P + threadIdx.x) * HM_NSTAT + s] : 0.0f;
-  statmax_block(m, const_cast<float*>(D.smax));
+  if (threadIdx.x < HM_NSTAT) const_cast<float*>(D.smax)[threadIdx.x] = red[0][threadIdx.x];
 }
 
 }  // namespace hm

@@ -18,9 +18,8 @@
 namespace hm {
 // kernels (defined in the other translation units)
 __global__ void rownorm_kernel(const DsDesc*, int, int64_t, float*);
-__global__ void prep_kernel(const DsDesc*, int, int64_t, const int64_t*, const float*, int);
-__global__ void statmax_part_kernel(const DsDesc*, float*);
-__global__ void statmax_kernel(const DsDesc*, const float*);
+__global__ void prep_kernel(const DsDesc*, int, int64_t, const int64_t*, const float*, int, float*);
+__global__ void statmax_kernel(const DsDesc*, const float*, const int64_t*);
 template <int MODE, int PASS>
 __global__ void gram_sweep_kernel(const DsDesc*, const WorkItem*, const int*, int, const uint32_t*,
                                   int);
@@ -50,11 +49,13 @@ __global__ void list_fallback_items_kernel(const uint32_t*, uint32_t*, int*, con
                                            int*);
 template <int KM>
 __global__ void smooth_knn_kernel(const DsDesc*, int, int64_t, const double*, int, float*, float*,
-                                  double*, uint32_t*);
+                                  double*, uint32_t*, uint32_t, uint32_t*, uint32_t*, float*,
+                                  uint32_t*);
 __global__ void calib_fixup_kernel(const DsDesc*, int, float*, double*, const float*, uint32_t*);
 template <int G>
 __global__ void smooth_knn_group_kernel(const DsDesc*, int, int64_t, const double*, int, float*,
-                                        float*, double*, uint32_t*);
+                                        float*, double*, uint32_t*, uint32_t, uint32_t*, uint32_t*,
+                                        float*, uint32_t*);
 __global__ void directed_kernel(const DsDesc*, int, int64_t, const float*, const float*, uint32_t,
                                 uint32_t*, uint32_t*, float*, uint32_t*);
 __global__ void expf_kernel(const float*, float*, int64_t);
@@ -397,7 +398,8 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
   if (stages & HM_STAGE_KNN) {
     b += P.sum_xr * 4 + P.sum_cos * 4;               // xr, op32 (cosine)
     if (mode == 0 || mode == 2) b += P.sum_x16 * 4;  // op16 + op16lo (bf16x3 or its escalation)
-    b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 * (1 + kStatParts) + R * 12;
+    b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + P.total_pad / 16 * HM_NSTAT * 4 +
+         R * 12;
     b += P.sum_cand * 8 + R * 2 * kSplitCap * 12 + R * kSplitCap * 4;  // cand, ctau+ccnt, thr
     b += 520 * 8;                                                          // counter sums
     b += R * 12 + P.fb_cap * 16 + R * P.xs * P.kmax * 8;
@@ -597,11 +599,9 @@ hm_status enqueue_pipeline(hm_job* J) {
     const int mode_bits = (J->mode == 0 ? 1 : 0) | 2;
     hipLaunchKernelGGL(prep_kernel, dim3(grid_for(J->total_pad, 16)), dim3(256), 0, J->stream,
                        J->d_desc, nds, J->total_pad, (const int64_t*)J->pad_off,
-                       (const float*)J->nz, mode_bits);
-    hipLaunchKernelGGL(statmax_part_kernel, dim3(nds, kStatParts), dim3(256), 0, J->stream,
-                       J->d_desc, J->statpart);
+                       (const float*)J->nz, mode_bits, J->statpart);
     hipLaunchKernelGGL(statmax_kernel, dim3(nds), dim3(256), 0, J->stream, J->d_desc,
-                       (const float*)J->statpart);
+                       (const float*)J->statpart, (const int64_t*)J->pad_off);
     HM_LAUNCH(ctx);
     mark(J, 1);
     // ---- candidate sweep + re-rank ----
@@ -697,14 +697,20 @@ hm_status enqueue_pipeline(hm_job* J) {
     // of one).
     const int G = J->kmax <= 16 ? 16 : J->kmax <= 32 ? 32 : J->kmax <= 64 ? 64 : 0;
     const bool by_thread = G == 0 || R >= kCalibThreadRowsPerCu * (int64_t)cus;
+    // with the symmetrisation to follow, the calibration also writes the
+    // directed edges
+    const bool edges = (J->stages & HM_STAGE_FUZZY) != 0;
+    const uint32_t sent = (uint32_t)R;
+    uint32_t* k1 = edges ? J->key1 : nullptr;
 #define HM_CALIB_G(GS)                                                                         \
   hipLaunchKernelGGL((smooth_knn_group_kernel<GS>), dim3(grid_for(R * GS, 256)), dim3(256), 0, \
                      J->stream, J->d_desc, nds, R, (const double*)J->target, J->opt.calib,     \
-                     J->sigma, J->rho, J->sigma64, J->zany)
+                     J->sigma, J->rho, J->sigma64, J->zany, sent, k1, J->pay1, J->wval,        \
+                     J->srcrow)
 #define HM_CALIB_T(KM)                                                                         \
   hipLaunchKernelGGL((smooth_knn_kernel<KM>), dim3(grid_for(R, 256)), dim3(256), 0, J->stream, \
                      J->d_desc, nds, R, (const double*)J->target, J->opt.calib, J->sigma,       \
-                     J->rho, J->sigma64, J->zany)
+                     J->rho, J->sigma64, J->zany, sent, k1, J->pay1, J->wval, J->srcrow)
     if (by_thread) {
       if (G == 16) HM_CALIB_T(16);
       else if (G == 32) HM_CALIB_T(32);
@@ -727,10 +733,12 @@ hm_status enqueue_pipeline(hm_job* J) {
   if (J->stages & HM_STAGE_FUZZY) {
     const uint32_t sent = (uint32_t)R;
     const int bits = bitlen((uint64_t)R);
-    hipLaunchKernelGGL(directed_kernel, dim3(grid_for(E, 256)), dim3(256), 0, J->stream, J->d_desc,
-                       nds, E, (const float*)J->sigma, (const float*)J->rho, sent, J->key1, J->pay1,
-                       J->wval, J->srcrow);
-    HM_LAUNCH(ctx);
+    if (!(J->stages & HM_STAGE_CALIB)) {  // else written by the calibration
+      hipLaunchKernelGGL(directed_kernel, dim3(grid_for(E, 256)), dim3(256), 0, J->stream,
+                         J->d_desc, nds, E, (const float*)J->sigma, (const float*)J->rho, sent,
+                         J->key1, J->pay1, J->wval, J->srcrow);
+      HM_LAUNCH(ctx);
+    }
     HM_TRY(radix_sort(J, J->key1, J->pay1, E, bits, J->k1s, J->p1s));
     // a group of G >= k lanes per row: union sizes, their scan, the entries
     const int GU = J->kmax <= 16 ? 16 : J->kmax <= 32 ? 32 : 64;
@@ -1034,7 +1042,7 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
     HM_A(J->colbias, P.total_pad);
     HM_A(J->rstat, R * HM_NSTAT);
     HM_A(J->smax, (int64_t)nds * HM_NSTAT);
-    HM_A(J->statpart, (int64_t)nds * kStatParts * HM_NSTAT);
+    HM_A(J->statpart, P.total_pad / 16 * HM_NSTAT);
     HM_A(J->nz, R);
     HM_A(J->rnorm, R);
     // sized for any sweep configuration (see plan_sweep)
