@@ -8,6 +8,8 @@ argument meaning, output layout and error behaviour follow the Rust crate:
   smooth_knn -> SmoothKnn     src/smooth_knn.rs:16-80
   fuzzy_simplicial_set        src/fuzzy.rs:17-85
   Csr.from_fuzzy / row        src/sparse.rs:12-41
+  connected_components        src/components.rs:7-39
+  Laplacian.new / matvec      src/sparse.rs:46-93
   HolomapError variants       src/api.rs:17-44
   Holomap.builder(seed)       src/api.rs:94-107, setters :195-235, and
   Holomap.graph(data, n_features) = the validation of src/api.rs:113-144
@@ -34,6 +36,7 @@ __all__ = [
     "Metric", "Knn", "SmoothKnn", "FuzzyGraph", "Csr", "HolomapError", "BadShape",
     "TooFewPoints", "InvalidParameter", "NonFiniteInput", "DeviceError", "Context", "Job",
     "exact_knn", "smooth_knn", "fuzzy_simplicial_set", "knn_graph", "Holomap",
+    "Components", "connected_components", "Laplacian",
     "PRECISIONS", "CALIBRATIONS",
 ]
 
@@ -151,6 +154,71 @@ class Csr:
     def row(self, i: int):
         lo, hi = int(self.indptr[i]), int(self.indptr[i + 1])
         return self.indices[lo:hi], self.vals[lo:hi]
+
+
+@dataclass
+class Components:
+    """src/components.rs:7-11."""
+    n_components: int
+    labels: np.ndarray  # uint32 (n,)
+
+
+def _csr_arrays(csr: Csr):
+    ip = np.ascontiguousarray(csr.indptr, np.uint64)
+    ix = np.ascontiguousarray(csr.indices, np.uint32)
+    return ip, ix
+
+
+def connected_components(csr: Csr, ctx: "Context" = None) -> Components:
+    """src/components.rs:13-39 on the device: labels in first-seen order
+    (for the symmetric graphs fuzzy_simplicial_set returns)."""
+    c = ctx or Context.default()
+    ip, ix = _csr_arrays(csr)
+    lab = np.empty(csr.n, np.uint32)
+    nc = ctypes.c_uint64()
+    c._check(_lib.load().hm_connected_components(c.handle, ip.ctypes.data, ix.ctypes.data, csr.n,
+                                                 lab.ctypes.data, ctypes.byref(nc)))
+    return Components(int(nc.value), lab)
+
+
+class Laplacian:
+    """src/sparse.rs:46-99: the symmetric-normalised Laplacian, stored as
+    the normalised adjacency S = D^-1/2 W D^-1/2 (a Csr of f64 values) and
+    the degrees; built and applied on the device."""
+
+    def __init__(self, s: Csr, degrees: np.ndarray):
+        self.s = s
+        self.degrees = degrees
+
+    @staticmethod
+    def new(g: FuzzyGraph, ctx: "Context" = None) -> "Laplacian":
+        """src/sparse.rs:55-78."""
+        c = ctx or Context.default()
+        csr = Csr.from_fuzzy(g, ctx=c)
+        ip, ix = _csr_arrays(csr)
+        w = np.ascontiguousarray(g.vals, np.float32)
+        sv = np.empty(max(w.size, 1), np.float64)
+        deg = np.empty(g.n, np.float64)
+        c._check(_lib.load().hm_laplacian_new(c.handle, ip.ctypes.data, ix.ctypes.data,
+                                              w.ctypes.data, g.n, sv.ctypes.data,
+                                              deg.ctypes.data))
+        return Laplacian(Csr(g.n, csr.indptr, csr.indices, sv[:w.size]), deg)
+
+    def n(self) -> int:
+        return self.s.n
+
+    def matvec(self, x, ctx: "Context" = None) -> np.ndarray:
+        """src/sparse.rs:87-93: y = L x = x - S x."""
+        c = ctx or Context.default()
+        ip, ix = _csr_arrays(self.s)
+        sv = np.ascontiguousarray(self.s.vals, np.float64)
+        xx = np.ascontiguousarray(x, np.float64)
+        assert xx.size == self.s.n
+        y = np.empty(self.s.n, np.float64)
+        c._check(_lib.load().hm_laplacian_matvec(c.handle, ip.ctypes.data, ix.ctypes.data,
+                                                 sv.ctypes.data, self.s.n, xx.ctypes.data,
+                                                 y.ctypes.data))
+        return y
 
 
 # ---------------------------------------------------------------- context
@@ -416,6 +484,21 @@ class Job:
         m = int(o.nnz)
         g = FuzzyGraph(n, rows[:m].copy(), cols[:m].copy(), vals[:m].copy())
         return GraphResult(Knn(k, idx, dist), SmoothKnn(sig, rho, s64), g, indptr)
+
+    def connected_components(self, ds: int) -> Components:
+        """src/components.rs:13-39 on dataset ds's resident graph."""
+        lab = np.empty(int(self.n[ds]), np.uint32)
+        nc = ctypes.c_uint64()
+        self.ctx._check(_lib.load().hm_job_connected_components(self.handle, ds, lab.ctypes.data,
+                                                                ctypes.byref(nc)))
+        return Components(int(nc.value), lab)
+
+    def laplacian_matvec(self, ds: int, x_dev: int, y_dev: int) -> None:
+        """y = L x (src/sparse.rs:87-93) on dataset ds's resident graph; x and
+        y are device pointers to n doubles."""
+        self.ctx._check(_lib.load().hm_job_laplacian_matvec(self.handle, ds,
+                                                            ctypes.c_void_p(x_dev),
+                                                            ctypes.c_void_p(y_dev)))
 
     def close(self):
         if getattr(self, "handle", None):

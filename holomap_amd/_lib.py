@@ -50,7 +50,8 @@ EXPORTS = [
     "hm_job_device_outputs", "hm_expf_array", "hm_ctx_budget", "hm_ctx_in_use",
     "hm_server_default_config", "hm_server_create", "hm_server_destroy", "hm_server_submit",
     "hm_server_progress", "hm_server_state", "hm_server_pending", "hm_server_result_info",
-    "hm_server_fetch", "hm_knn_graph_noalloc",
+    "hm_server_fetch", "hm_knn_graph_noalloc", "hm_connected_components", "hm_laplacian_new",
+    "hm_laplacian_matvec", "hm_job_connected_components", "hm_job_laplacian_matvec",
 ]
 
 
@@ -102,6 +103,11 @@ def load():
         "hm_csr_from_fuzzy": (i32, [vp, vp, u64, u64, vp]),
         "hm_knn_graph": (i32, [vp, vp, u64, u32, u32, i32, P(Options), P(GraphOut)]),
         "hm_knn_graph_noalloc": (i32, [vp, vp, u64, u32, u32, i32, P(Options), P(GraphOut)]),
+        "hm_connected_components": (i32, [vp, vp, vp, u64, vp, P(u64)]),
+        "hm_laplacian_new": (i32, [vp, vp, vp, vp, u64, vp, vp]),
+        "hm_laplacian_matvec": (i32, [vp, vp, vp, vp, u64, vp, vp]),
+        "hm_job_connected_components": (i32, [vp, u32, vp, P(u64)]),
+        "hm_job_laplacian_matvec": (i32, [vp, u32, vp, vp]),
         "hm_job_create": (i32, [vp, u32, vp, vp, vp, vp, P(Options), u32, P(vp)]),
         "hm_job_destroy": (None, [vp]),
         "hm_job_bytes": (u64, [vp]),

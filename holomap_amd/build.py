@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libholomap_gpu.so")
 SOURCES = ["hm_prep.hip", "hm_gram.hip", "hm_knn.hip", "hm_calib.hip", "hm_sort.hip",
-           "hm_fuzzy.hip", "hm_runtime.hip", "hm_serve.hip"]
+           "hm_fuzzy.hip", "hm_graph.hip", "hm_runtime.hip", "hm_serve.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HM_OFFLOAD_ARCH", "gfx950")
 CFLAGS = [

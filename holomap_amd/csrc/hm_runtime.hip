@@ -76,6 +76,16 @@ __global__ void fill_u32(uint32_t*, int64_t, uint32_t);
 __global__ void sum_u32_part_kernel(const uint32_t*, int64_t, uint64_t*);
 __global__ void sum_u64_kernel(const uint64_t*, int, uint64_t*);
 __global__ void nonfinite_part_kernel(const float*, int64_t, uint64_t*);
+__global__ void cc_init_kernel(uint32_t*, int64_t);
+__global__ void cc_step_kernel(const uint64_t*, const uint32_t*, int64_t, uint32_t*, uint32_t*);
+__global__ void cc_jump_kernel(uint32_t*, int64_t);
+__global__ void cc_root_kernel(const uint32_t*, int64_t, uint32_t*);
+__global__ void cc_label_kernel(const uint32_t*, const uint32_t*, int64_t, uint32_t*);
+__global__ void lap_degree_kernel(const uint64_t*, const float*, int64_t, double*, double*);
+__global__ void lap_scale_kernel(const uint64_t*, const uint32_t*, const float*, const double*,
+                                 int64_t, double*);
+__global__ void lap_matvec_kernel(const uint64_t*, const uint32_t*, const double*, const float*,
+                                  const double*, int64_t, const double*, double*);
 __global__ void nonfinite_min_kernel(const uint64_t*, int, uint64_t*);
 template <int G>
 __global__ void union_count_kernel(const DsDesc*, int, int64_t, int64_t, const uint32_t*,
@@ -217,6 +227,10 @@ struct hm_job {
   int64_t cap_l2 = 0;  // items of a sweep over any subset of rows
   int64_t *pad_off = nullptr, *iptr_off = nullptr;
   std::vector<int64_t> iptr_off_h;
+  // Laplacian of the resident graphs (hm_job_laplacian_matvec): 1/sqrt of
+  // the degrees, all datasets, recomputed after each run
+  double* lap_inv = nullptr;
+  bool lap_valid = false;
   // graph and timing
   hipGraphExec_t gexec = nullptr;
   bool captured = false;
@@ -1309,6 +1323,7 @@ hm_status hm_job_run(hm_job* J) {
   if (!J) return HM_EINVAL;
   hm_ctx* ctx = J->ctx;
   HM_HIP(ctx, hipSetDevice(ctx->device));
+  J->lap_valid = false;
   if (J->opt.autotune && !J->tuned && (J->stages & HM_STAGE_KNN) && J->mode >= 0)
     HM_TRY(autotune(J));
   J->timed = J->opt.timing && !J->opt.use_graph;
@@ -1650,6 +1665,260 @@ hm_status hm_knn_graph_noalloc(hm_ctx* ctx, const float* data, uint64_t n, uint3
     s = fail(ctx, HM_EDEVICE, "output copy failed");
   hm_job_destroy(J);
   return s;
+}
+
+// ---- consumers of the graph's CSR: components and Laplacian ----
+
+}  // extern "C"
+
+namespace {
+
+// Exclusive scan of n u32 values on `st` (tmp: cdiv(n, kScanTile) + 2 values).
+hm_status scan_u32_excl(hm_ctx* ctx, hipStream_t st, const uint32_t* in, int64_t n,
+                        uint32_t* out, uint32_t* total, uint32_t* tmp) {
+  if (n <= kScanSmall<uint32_t>) {
+    constexpr int NT = kScanSmallThreads, MS = kScanSmallQuads<uint32_t>;
+    hipLaunchKernelGGL((scan_small<uint32_t, NT, MS>), dim3(1), dim3(NT), 0, st, in, n, out, total,
+                       (uint32_t*)nullptr, (int*)nullptr);
+  } else {
+    const int64_t nt = cdiv(n, kScanTile);
+    hipLaunchKernelGGL(scan_tile_sums<uint32_t>, dim3((unsigned)nt), dim3(256), 0, st, in, n, tmp);
+    hipLaunchKernelGGL(scan_sums<uint32_t>, dim3(1), dim3(1024), 0, st, tmp, nt, total);
+    hipLaunchKernelGGL(scan_tiles<uint32_t>, dim3((unsigned)nt), dim3(256), 0, st, in, n,
+                       (const uint32_t*)tmp, out);
+  }
+  HM_LAUNCH(ctx);
+  return HM_OK;
+}
+
+// Stream-ordered device buffers released when the owner goes out of scope.
+struct DevBufs {
+  hipStream_t st;
+  std::vector<void*> p;
+  explicit DevBufs(hipStream_t s) : st(s) {}
+  ~DevBufs() {
+    for (void* q : p) (void)hipFreeAsync(q, st);
+  }
+  template <typename T>
+  T* get(hm_ctx* ctx, int64_t count, hm_status* status) {
+    void* q = nullptr;
+    if (hipMallocAsync(&q, (size_t)std::max<int64_t>(count, 1) * sizeof(T), st) != hipSuccess) {
+      (void)hipGetLastError();
+      *status = fail(ctx, HM_ENOMEM, "device allocation failed");
+      return nullptr;
+    }
+    p.push_back(q);
+    return (T*)q;
+  }
+};
+
+// Component labels (first-seen order) of the n-vertex CSR graph in device
+// memory into out (device); the count into *ncomp. Synchronises `st`.
+hm_status components_device(hm_ctx* ctx, hipStream_t st, const uint64_t* indptr,
+                            const uint32_t* indices, int64_t n, uint32_t* out, uint64_t* ncomp) {
+  hm_status s = HM_OK;
+  DevBufs B(st);
+  uint32_t* lab = B.get<uint32_t>(ctx, n, &s);
+  uint32_t* root = lab ? B.get<uint32_t>(ctx, n, &s) : nullptr;
+  uint32_t* rank = root ? B.get<uint32_t>(ctx, n, &s) : nullptr;
+  uint32_t* flag = rank ? B.get<uint32_t>(ctx, 2, &s) : nullptr;
+  uint32_t* tmp = flag ? B.get<uint32_t>(ctx, cdiv(n, kScanTile) + 2, &s) : nullptr;
+  if (!tmp) return s;
+  const unsigned g = grid_for(n, 256);
+  hipLaunchKernelGGL(cc_init_kernel, dim3(g), dim3(256), 0, st, lab, n);
+  HM_LAUNCH(ctx);
+  // every step that changes a label lowers it; a step that changes none
+  // leaves a fixed point (within diameter + 1 steps)
+  for (int64_t it = 0; it <= n; ++it) {
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, st, flag, (int64_t)1, 0u);
+    hipLaunchKernelGGL(cc_step_kernel, dim3(g), dim3(256), 0, st, indptr, indices, n, lab, flag);
+    hipLaunchKernelGGL(cc_jump_kernel, dim3(g), dim3(256), 0, st, lab, n);
+    hipLaunchKernelGGL(cc_jump_kernel, dim3(g), dim3(256), 0, st, lab, n);
+    HM_LAUNCH(ctx);
+    uint32_t changed = 0;
+    HM_HIP(ctx, hipMemcpyAsync(&changed, flag, 4, hipMemcpyDeviceToHost, st));
+    HM_HIP(ctx, hipStreamSynchronize(st));
+    if (!changed) break;
+  }
+  hipLaunchKernelGGL(cc_root_kernel, dim3(g), dim3(256), 0, st, (const uint32_t*)lab, n, root);
+  HM_TRY(scan_u32_excl(ctx, st, root, n, rank, flag + 1, tmp));
+  hipLaunchKernelGGL(cc_label_kernel, dim3(g), dim3(256), 0, st, (const uint32_t*)lab,
+                     (const uint32_t*)rank, n, out);
+  HM_LAUNCH(ctx);
+  uint32_t nc = 0;
+  HM_HIP(ctx, hipMemcpyAsync(&nc, flag + 1, 4, hipMemcpyDeviceToHost, st));
+  HM_HIP(ctx, hipStreamSynchronize(st));
+  *ncomp = nc;
+  return HM_OK;
+}
+
+// A caller's CSR must index within its n vertices: indptr non-decreasing
+// from 0, indices < n (the kernels follow them).
+hm_status check_csr(hm_ctx* ctx, const uint64_t* indptr, const uint32_t* indices, uint64_t n) {
+  if (indptr[0] != 0) return fail(ctx, HM_EINVAL, "indptr[0] must be 0");
+  for (uint64_t i = 0; i < n; ++i)
+    if (indptr[i + 1] < indptr[i]) return fail(ctx, HM_EINVAL, "indptr must be non-decreasing");
+  for (uint64_t p = 0; p < indptr[n]; ++p)
+    if (indices[p] >= n) return fail(ctx, HM_EINVAL, "column index out of range");
+  return HM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hm_status hm_connected_components(hm_ctx* ctx, const uint64_t* indptr, const uint32_t* indices,
+                                  uint64_t n, uint32_t* labels, uint64_t* n_components) {
+  if (!ctx || !indptr || !labels || !n_components || (!indices && n && indptr[n]))
+    return fail(ctx, HM_EINVAL, "null argument");
+  if (n == 0) {
+    *n_components = 0;
+    return HM_OK;
+  }
+  if (n >= (1ull << 32)) return fail(ctx, HM_EUNSUPPORTED, "n >= 2^32");
+  HM_TRY(check_csr(ctx, indptr, indices, n));
+  HM_HIP(ctx, hipSetDevice(ctx->device));
+  const uint64_t nnz = indptr[n];
+  hipStream_t st = ctx->stream;
+  hm_status s = HM_OK;
+  DevBufs B(st);
+  uint64_t* dptr = B.get<uint64_t>(ctx, n + 1, &s);
+  uint32_t* didx = dptr ? B.get<uint32_t>(ctx, nnz, &s) : nullptr;
+  uint32_t* dout = didx ? B.get<uint32_t>(ctx, n, &s) : nullptr;
+  if (!dout) return s;
+  HM_HIP(ctx, hipMemcpyAsync(dptr, indptr, (n + 1) * 8, hipMemcpyHostToDevice, st));
+  if (nnz) HM_HIP(ctx, hipMemcpyAsync(didx, indices, nnz * 4, hipMemcpyHostToDevice, st));
+  HM_TRY(components_device(ctx, st, dptr, didx, (int64_t)n, dout, n_components));
+  HM_HIP(ctx, hipMemcpyAsync(labels, dout, n * 4, hipMemcpyDeviceToHost, st));
+  HM_HIP(ctx, hipStreamSynchronize(st));
+  return HM_OK;
+}
+
+hm_status hm_laplacian_new(hm_ctx* ctx, const uint64_t* indptr, const uint32_t* indices,
+                           const float* vals, uint64_t n, double* svals, double* degrees) {
+  if (!ctx || !indptr || (n && indptr[n] && (!indices || !vals)))
+    return fail(ctx, HM_EINVAL, "null argument");
+  if (n == 0) return HM_OK;
+  HM_TRY(check_csr(ctx, indptr, indices, n));
+  HM_HIP(ctx, hipSetDevice(ctx->device));
+  const uint64_t nnz = indptr[n];
+  hipStream_t st = ctx->stream;
+  hm_status s = HM_OK;
+  DevBufs B(st);
+  uint64_t* dptr = B.get<uint64_t>(ctx, n + 1, &s);
+  uint32_t* didx = dptr ? B.get<uint32_t>(ctx, nnz, &s) : nullptr;
+  float* dval = didx ? B.get<float>(ctx, nnz, &s) : nullptr;
+  double* ddeg = dval ? B.get<double>(ctx, n, &s) : nullptr;
+  double* dinv = ddeg ? B.get<double>(ctx, n, &s) : nullptr;
+  double* dsv = dinv ? B.get<double>(ctx, nnz, &s) : nullptr;
+  if (!dsv) return s;
+  HM_HIP(ctx, hipMemcpyAsync(dptr, indptr, (n + 1) * 8, hipMemcpyHostToDevice, st));
+  if (nnz) {
+    HM_HIP(ctx, hipMemcpyAsync(didx, indices, nnz * 4, hipMemcpyHostToDevice, st));
+    HM_HIP(ctx, hipMemcpyAsync(dval, vals, nnz * 4, hipMemcpyHostToDevice, st));
+  }
+  const unsigned g = grid_for((int64_t)n, 256);
+  hipLaunchKernelGGL(lap_degree_kernel, dim3(g), dim3(256), 0, st, (const uint64_t*)dptr,
+                     (const float*)dval, (int64_t)n, ddeg, dinv);
+  hipLaunchKernelGGL(lap_scale_kernel, dim3(g), dim3(256), 0, st, (const uint64_t*)dptr,
+                     (const uint32_t*)didx, (const float*)dval, (const double*)dinv, (int64_t)n, dsv);
+  HM_LAUNCH(ctx);
+  if (degrees) HM_HIP(ctx, hipMemcpyAsync(degrees, ddeg, n * 8, hipMemcpyDeviceToHost, st));
+  if (svals && nnz) HM_HIP(ctx, hipMemcpyAsync(svals, dsv, nnz * 8, hipMemcpyDeviceToHost, st));
+  HM_HIP(ctx, hipStreamSynchronize(st));
+  return HM_OK;
+}
+
+hm_status hm_laplacian_matvec(hm_ctx* ctx, const uint64_t* indptr, const uint32_t* indices,
+                              const double* svals, uint64_t n, const double* x, double* y) {
+  if (!ctx || !indptr || !x || !y || (n && indptr[n] && (!indices || !svals)))
+    return fail(ctx, HM_EINVAL, "null argument");
+  if (n == 0) return HM_OK;
+  HM_TRY(check_csr(ctx, indptr, indices, n));
+  HM_HIP(ctx, hipSetDevice(ctx->device));
+  const uint64_t nnz = indptr[n];
+  hipStream_t st = ctx->stream;
+  hm_status s = HM_OK;
+  DevBufs B(st);
+  uint64_t* dptr = B.get<uint64_t>(ctx, n + 1, &s);
+  uint32_t* didx = dptr ? B.get<uint32_t>(ctx, nnz, &s) : nullptr;
+  double* dsv = didx ? B.get<double>(ctx, nnz, &s) : nullptr;
+  double* dx = dsv ? B.get<double>(ctx, n, &s) : nullptr;
+  double* dy = dx ? B.get<double>(ctx, n, &s) : nullptr;
+  if (!dy) return s;
+  HM_HIP(ctx, hipMemcpyAsync(dptr, indptr, (n + 1) * 8, hipMemcpyHostToDevice, st));
+  if (nnz) {
+    HM_HIP(ctx, hipMemcpyAsync(didx, indices, nnz * 4, hipMemcpyHostToDevice, st));
+    HM_HIP(ctx, hipMemcpyAsync(dsv, svals, nnz * 8, hipMemcpyHostToDevice, st));
+  }
+  HM_HIP(ctx, hipMemcpyAsync(dx, x, n * 8, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(lap_matvec_kernel, dim3(grid_for((int64_t)n, 256)), dim3(256), 0, st,
+                     (const uint64_t*)dptr, (const uint32_t*)didx, (const double*)dsv,
+                     (const float*)nullptr, (const double*)nullptr, (int64_t)n, (const double*)dx,
+                     dy);
+  HM_LAUNCH(ctx);
+  HM_HIP(ctx, hipMemcpyAsync(y, dy, n * 8, hipMemcpyDeviceToHost, st));
+  HM_HIP(ctx, hipStreamSynchronize(st));
+  return HM_OK;
+}
+
+// Row pointer (local, n+1) and the start of dataset ds's COO in the job's
+// graph arrays.
+static hm_status job_csr(hm_job* J, uint32_t ds, const uint64_t** indptr, uint64_t* s0) {
+  if (!(J->stages & HM_STAGE_FUZZY)) return fail(J->ctx, HM_EINVAL, "the job builds no graph");
+  HM_HIP(J->ctx, hipStreamSynchronize(J->stream));
+  HM_HIP(J->ctx, hipMemcpy(s0, J->indptr_g + J->dsh[ds].row_off, 8, hipMemcpyDeviceToHost));
+  *indptr = J->indptr_l + J->iptr_off_h[ds];
+  return HM_OK;
+}
+
+hm_status hm_job_connected_components(hm_job* J, uint32_t ds, uint32_t* labels,
+                                      uint64_t* n_components) {
+  if (!J || ds >= (uint32_t)J->nds || !labels || !n_components) return HM_EINVAL;
+  hm_ctx* ctx = J->ctx;
+  HM_HIP(ctx, hipSetDevice(ctx->device));
+  const uint64_t* iptr = nullptr;
+  uint64_t s0 = 0;
+  HM_TRY(job_csr(J, ds, &iptr, &s0));
+  const int64_t n = J->dsh[ds].n;
+  hm_status s = HM_OK;
+  DevBufs B(J->stream);
+  uint32_t* dout = B.get<uint32_t>(ctx, n, &s);
+  if (!dout) return s;
+  HM_TRY(components_device(ctx, J->stream, iptr, J->cols + s0, n, dout, n_components));
+  HM_HIP(ctx, hipMemcpyAsync(labels, dout, n * 4, hipMemcpyDeviceToHost, J->stream));
+  HM_HIP(ctx, hipStreamSynchronize(J->stream));
+  return HM_OK;
+}
+
+hm_status hm_job_laplacian_matvec(hm_job* J, uint32_t ds, const double* x, double* y) {
+  if (!J || ds >= (uint32_t)J->nds || !x || !y) return HM_EINVAL;
+  hm_ctx* ctx = J->ctx;
+  HM_HIP(ctx, hipSetDevice(ctx->device));
+  const uint64_t* iptr = nullptr;
+  uint64_t s0 = 0;
+  HM_TRY(job_csr(J, ds, &iptr, &s0));
+  const DsHost& h = J->dsh[ds];
+  if (!J->lap_inv) HM_TRY(dalloc(J, &J->lap_inv, J->total_rows));
+  if (!J->lap_valid) {  // degrees of every dataset's graph of the last run
+    for (int i = 0; i < J->nds; ++i) {
+      const uint64_t* ip = nullptr;
+      uint64_t b0 = 0;
+      HM_TRY(job_csr(J, (uint32_t)i, &ip, &b0));
+      hipLaunchKernelGGL(lap_degree_kernel, dim3(grid_for(J->dsh[i].n, 256)), dim3(256), 0,
+                         J->stream, ip, (const float*)(J->vals + b0), J->dsh[i].n,
+                         (double*)nullptr, J->lap_inv + J->dsh[i].row_off);
+    }
+    HM_LAUNCH(ctx);
+    J->lap_valid = true;
+  }
+  hipLaunchKernelGGL(lap_matvec_kernel, dim3(grid_for(h.n, 256)), dim3(256), 0, J->stream, iptr,
+                     (const uint32_t*)(J->cols + s0), (const double*)nullptr,
+                     (const float*)(J->vals + s0), (const double*)(J->lap_inv + h.row_off), h.n, x,
+                     y);
+  HM_LAUNCH(ctx);
+  HM_HIP(ctx, hipStreamSynchronize(J->stream));
+  return HM_OK;
 }
 
 hm_status hm_expf_array(hm_ctx* ctx, const float* x, float* y, uint64_t n) {

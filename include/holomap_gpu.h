@@ -165,6 +165,30 @@ typedef struct {
 hm_status hm_knn_graph(hm_ctx* ctx, const float* data, uint64_t n, uint32_t n_features,
                        uint32_t k, int32_t metric, const hm_options* opt, hm_graph_out* out);
 
+/* ---- consumers of the graph's CSR: src/components.rs, src/sparse.rs ----
+ * For graphs fuzzy_simplicial_set returns (symmetric, columns ascending in
+ * each row), given as CSR: indptr (n+1, from 0), indices (indptr[n]).
+ * Every output is the reference's bit for bit (its f64 folds in its order). */
+
+/* src/components.rs:13-39: labels (n) in first-seen order (the component
+ * of vertex 0 is 0, ...), *n_components the count. */
+hm_status hm_connected_components(hm_ctx* ctx, const uint64_t* indptr, const uint32_t* indices,
+                                  uint64_t n, uint32_t* labels, uint64_t* n_components);
+/* src/sparse.rs:55-78 (Laplacian::new): from the graph's f32 weights, the
+ * normalised adjacency S = D^-1/2 W D^-1/2 on the same pattern (svals,
+ * indptr[n] values) and the degrees (n); either output may be NULL. */
+hm_status hm_laplacian_new(hm_ctx* ctx, const uint64_t* indptr, const uint32_t* indices,
+                           const float* vals, uint64_t n, double* svals, double* degrees);
+/* src/sparse.rs:87-93 (Laplacian::matvec): y = x - S x (n values each). */
+hm_status hm_laplacian_matvec(hm_ctx* ctx, const uint64_t* indptr, const uint32_t* indices,
+                              const double* svals, uint64_t n, const double* x, double* y);
+/* The same on a job's resident graph of dataset ds (no host copy of the
+ * graph): labels to host memory; the Laplacian product with x and y in
+ * DEVICE memory (n doubles each), for iterative eigensolvers. */
+hm_status hm_job_connected_components(hm_job* job, uint32_t ds, uint32_t* labels,
+                                      uint64_t* n_components);
+hm_status hm_job_laplacian_matvec(hm_job* job, uint32_t ds, const double* x, double* y);
+
 /* ---- allocation-free graph build ----
  * No counterpart in the reference. For callers that cannot allocate device
  * memory (code under stream capture, serving loops and harnesses that

@@ -62,6 +62,12 @@ def lib():
         L.hmo_fuzzy.restype = _sz
         L.hmo_csr_indptr.argtypes = [_u32p, _sz, _sz, _u64p]
         L.hmo_csr_indptr.restype = None
+        L.hmo_connected_components.argtypes = [_u64p, _u32p, _sz, _u32p]
+        L.hmo_connected_components.restype = _sz
+        L.hmo_laplacian_new.argtypes = [_u64p, _u32p, _f32p, _sz, _f64p, _f64p]
+        L.hmo_laplacian_new.restype = None
+        L.hmo_laplacian_matvec.argtypes = [_u64p, _u32p, _f64p, _sz, _f64p, _f64p]
+        L.hmo_laplacian_matvec.restype = None
         _lib = L
     return _lib
 
@@ -180,3 +186,42 @@ def knn_graph(data, n_features: int, k: int, metric="euclidean"):
     calib = smooth_knn(knn.dists, k)
     graph = fuzzy_simplicial_set(knn, calib)
     return knn, calib, graph
+
+
+@dataclass
+class Components:
+    n_components: int
+    labels: np.ndarray
+
+
+def connected_components(indptr, indices, n: int) -> Components:
+    """src/components.rs:13-39 on a CSR (indptr n+1, indices)."""
+    ip = np.ascontiguousarray(indptr, np.uint64)
+    ix = np.ascontiguousarray(indices, np.uint32)
+    lab = np.empty(n, np.uint32)
+    nc = lib().hmo_connected_components(ip, ix if ix.size else np.zeros(1, np.uint32), n, lab)
+    return Components(int(nc), lab)
+
+
+def laplacian_new(indptr, indices, vals, n: int):
+    """src/sparse.rs:55-78: (S values, degrees)."""
+    ip = np.ascontiguousarray(indptr, np.uint64)
+    ix = np.ascontiguousarray(indices, np.uint32)
+    v = np.ascontiguousarray(vals, np.float32)
+    s = np.empty(max(v.size, 1), np.float64)
+    d = np.empty(n, np.float64)
+    lib().hmo_laplacian_new(ip, ix if ix.size else np.zeros(1, np.uint32),
+                            v if v.size else np.zeros(1, np.float32), n, s, d)
+    return s[:v.size], d
+
+
+def laplacian_matvec(indptr, indices, svals, n: int, x) -> np.ndarray:
+    """src/sparse.rs:87-93: y = x - S x."""
+    ip = np.ascontiguousarray(indptr, np.uint64)
+    ix = np.ascontiguousarray(indices, np.uint32)
+    sv = np.ascontiguousarray(svals, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty(n, np.float64)
+    lib().hmo_laplacian_matvec(ip, ix if ix.size else np.zeros(1, np.uint32),
+                               sv if sv.size else np.zeros(1, np.float64), n, x, y)
+    return y

@@ -362,3 +362,59 @@ void hmo_csr_indptr(const uint32_t* rows, size_t nnz, size_t n, uint64_t* indptr
     for (size_t t = 0; t < nnz; ++t) indptr[rows[t] + 1] += 1;
     for (size_t i = 0; i < n; ++i) indptr[i + 1] += indptr[i];
 }
+
+/* src/components.rs:13-39. A search from every unlabelled vertex in
+ * ascending order; the Vec the reference calls `queue` is used as a stack
+ * (push, pop from the end). Labels are assigned when a vertex is pushed. */
+size_t hmo_connected_components(const uint64_t* indptr, const uint32_t* indices, size_t n,
+                                uint32_t* labels) {
+    const uint32_t unlabeled = 0xffffffffu;
+    for (size_t v = 0; v < n; ++v) labels[v] = unlabeled;
+    size_t* stack = (size_t*)malloc((n + 1) * sizeof(size_t));
+    uint32_t ncomp = 0;
+    for (size_t start = 0; start < n; ++start) {
+        if (labels[start] != unlabeled) continue;
+        size_t top = 0;
+        labels[start] = ncomp;
+        stack[top++] = start;
+        while (top > 0) {
+            const size_t v = stack[--top];
+            for (uint64_t p = indptr[v]; p < indptr[v + 1]; ++p) {
+                const uint32_t u = indices[p];
+                if (labels[u] == unlabeled) {
+                    labels[u] = ncomp;
+                    stack[top++] = u;
+                }
+            }
+        }
+        ncomp += 1;
+    }
+    free(stack);
+    return ncomp;
+}
+
+/* src/sparse.rs:55-78. Rust's f64 Sum folds left to right from -0.0. */
+void hmo_laplacian_new(const uint64_t* indptr, const uint32_t* indices, const float* vals,
+                       size_t n, double* svals, double* degrees) {
+    double* inv = (double*)malloc((n + 1) * sizeof(double));
+    for (size_t i = 0; i < n; ++i) {
+        double d = -0.0;
+        for (uint64_t p = indptr[i]; p < indptr[i + 1]; ++p) d = d + (double)vals[p];
+        degrees[i] = d;
+        inv[i] = 1.0 / sqrt(d);
+    }
+    for (size_t i = 0; i < n; ++i)
+        for (uint64_t p = indptr[i]; p < indptr[i + 1]; ++p)
+            svals[p] = (double)vals[p] * (inv[i] * inv[indices[p]]);
+    free(inv);
+}
+
+/* src/sparse.rs:87-93. */
+void hmo_laplacian_matvec(const uint64_t* indptr, const uint32_t* indices, const double* svals,
+                          size_t n, const double* x, double* y) {
+    for (size_t i = 0; i < n; ++i) {
+        double sx = -0.0;
+        for (uint64_t p = indptr[i]; p < indptr[i + 1]; ++p) sx = sx + svals[p] * x[indices[p]];
+        y[i] = x[i] - sx;
+    }
+}

@@ -19,6 +19,9 @@
  *   hmo_smooth_knn       src/smooth_knn.rs:12-84
  *   hmo_fuzzy            src/fuzzy.rs:25-85
  *   hmo_csr_indptr       src/sparse.rs:21-35
+ *   hmo_connected_components  src/components.rs:13-39
+ *   hmo_laplacian_new    src/sparse.rs:55-78
+ *   hmo_laplacian_matvec src/sparse.rs:87-93
  * hmo_smooth_knn_f64 is an FP64 bisection run to convergence: the FP64
  * oracle for the GPU's FP64 calibration mode (no reference counterpart).
  */
@@ -52,6 +55,15 @@ size_t hmo_fuzzy(const uint32_t* indices, const float* dists, const float* sigma
                  const float* rhos, size_t n, size_t k, uint32_t* rows,
                  uint32_t* cols, float* vals);
 void hmo_csr_indptr(const uint32_t* rows, size_t nnz, size_t n, uint64_t* indptr);
+/* src/components.rs:13-39: returns the component count. */
+size_t hmo_connected_components(const uint64_t* indptr, const uint32_t* indices, size_t n,
+                                uint32_t* labels);
+/* src/sparse.rs:55-78: S values (nnz) and degrees (n) from f32 weights. */
+void hmo_laplacian_new(const uint64_t* indptr, const uint32_t* indices, const float* vals,
+                       size_t n, double* svals, double* degrees);
+/* src/sparse.rs:87-93: y = x - S x. */
+void hmo_laplacian_matvec(const uint64_t* indptr, const uint32_t* indices, const double* svals,
+                          size_t n, const double* x, double* y);
 
 #ifdef __cplusplus
 }

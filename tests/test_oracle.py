@@ -235,3 +235,107 @@ def test_fuzzy_invariants():
         assert m[(c, r)] == v                       # bitwise symmetric
     ip = O.csr_indptr(g.rows, 500)
     assert ip[-1] == g.rows.size and np.all(np.diff(ip.astype(np.int64)) >= 0)
+
+
+# ---------------------------------------------------------------- components, Laplacian
+def sym_csr(n, edges, w=None):
+    """Symmetric, (row, col)-sorted CSR of an undirected edge list (the
+    reference's test helper shape, src/components.rs:47-58)."""
+    coo = sorted({(a, b) for a, b in edges} | {(b, a) for a, b in edges})
+    rows = np.array([r for r, _ in coo], np.uint32)
+    cols = np.array([c for _, c in coo], np.uint32)
+    vals = np.ones(len(coo), np.float32) if w is None else np.array(
+        [w[(min(r, c), max(r, c))] for r, c in coo], np.float32)
+    return O.csr_indptr(rows, n), cols, vals
+
+
+def test_components_known_answers():
+    # src/components.rs:60-83
+    c = O.connected_components(*sym_csr(3, [(0, 1), (1, 2)])[:2], 3)
+    assert c.n_components == 1 and c.labels.tolist() == [0, 0, 0]
+    c = O.connected_components(*sym_csr(4, [(0, 3), (1, 2)])[:2], 4)
+    assert c.n_components == 2 and c.labels.tolist() == [0, 1, 1, 0]
+    c = O.connected_components(*sym_csr(3, [(0, 1)])[:2], 3)
+    assert c.n_components == 2 and c.labels.tolist() == [0, 0, 1]
+
+
+def test_components_against_scipy_and_min_vertex_rank():
+    # first-seen labels = rank of each component's smallest vertex; the
+    # partition itself from scipy's connected_components
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components as cc
+    rng = np.random.default_rng(3)
+    for n, m in [(50, 30), (400, 300), (1000, 2000)]:
+        e = rng.integers(0, n, size=(m, 2))
+        e = [(int(a), int(b)) for a, b in e if a != b]
+        ip, ix, _ = sym_csr(n, e)
+        got = O.connected_components(ip, ix, n)
+        a = sp.csr_matrix((np.ones(ix.size), ix.astype(np.int64), ip.astype(np.int64)), shape=(n, n))
+        k, lab = cc(a, directed=False)
+        assert got.n_components == k
+        mins = {}
+        for v in range(n):
+            mins.setdefault(lab[v], v)
+        order = {c: r for r, c in enumerate(sorted(mins, key=lambda c: mins[c]))}
+        assert got.labels.tolist() == [order[lab[v]] for v in range(n)]
+
+
+def p3():
+    # src/sparse.rs:104-111: path 0-1-2, unit weights
+    return sym_csr(3, [(0, 1), (1, 2)])
+
+
+def test_laplacian_known_answers():
+    # src/sparse.rs:113-146
+    ip, ix, w = p3()
+    assert ip.tolist() == [0, 1, 3, 4]
+    s, d = O.laplacian_new(ip, ix, w, 3)
+    assert d.tolist() == [1.0, 2.0, 1.0]
+    y = O.laplacian_matvec(ip, ix, s, 3, np.array([1.0, np.sqrt(2.0), 1.0]))
+    assert np.all(np.abs(y) < 1e-12)  # the trivial eigenvector D^1/2 1
+    v1 = np.array([1.0, 0.0, -1.0])  # eigenvalue 1
+    assert np.all(np.abs(O.laplacian_matvec(ip, ix, s, 3, v1) - v1) < 1e-12)
+    # dense I - S against the product on basis vectors (src/sparse.rs:148-162)
+    dense = np.eye(3)
+    for i in range(3):
+        for p in range(ip[i], ip[i + 1]):
+            dense[i, ix[p]] -= s[p]
+    for j in range(3):
+        e = np.zeros(3)
+        e[j] = 1.0
+        assert np.all(np.abs(dense[:, j] - O.laplacian_matvec(ip, ix, s, 3, e)) < 1e-15)
+
+
+def test_laplacian_against_scipy_and_spectrum():
+    # S = D^-1/2 W D^-1/2 on a weighted symmetric graph: the product within
+    # 1e-13 of scipy's, L's spectrum in [0, 2] with the trivial eigenvalue 0
+    import scipy.sparse as sp
+    rng = np.random.default_rng(5)
+    n = 300
+    e = [(int(a), int(b)) for a, b in rng.integers(0, n, size=(1500, 2)) if a != b]
+    e += [(i, i + 1) for i in range(n - 1)]  # connected
+    w = {(min(a, b), max(a, b)): np.float32(rng.uniform(0.05, 1.0)) for a, b in e}
+    ip, ix, vals = sym_csr(n, e, w)
+    s, d = O.laplacian_new(ip, ix, vals, n)
+    W = sp.csr_matrix((vals.astype(np.float64), ix.astype(np.int64), ip.astype(np.int64)),
+                      shape=(n, n))
+    deg = np.asarray(W.sum(1)).ravel()
+    assert np.allclose(d, deg, rtol=1e-13, atol=0)
+    Dm = sp.diags(1.0 / np.sqrt(deg))
+    S = (Dm @ W @ Dm).tocsr()
+    S.sort_indices()
+    assert np.allclose(s, S.data, rtol=1e-13, atol=0)
+    x = rng.normal(size=n)
+    assert np.allclose(O.laplacian_matvec(ip, ix, s, n, x), x - S @ x, rtol=1e-12, atol=1e-12)
+    ev = np.linalg.eigvalsh(np.eye(n) - S.toarray())
+    assert ev.min() > -1e-10 and ev.max() < 2.0 + 1e-10 and abs(ev[0]) < 1e-10
+
+
+def test_laplacian_isolated_vertex_degree_is_negative_zero():
+    # Rust's f64 Sum starts at -0.0: an empty row has degree -0.0
+    ip, ix, w = sym_csr(3, [(0, 1)])
+    s, d = O.laplacian_new(ip, ix, w, 3)
+    assert d[2] == 0.0 and np.signbit(d[2])
+    x = np.array([1.0, 2.0, -0.0])
+    y = O.laplacian_matvec(ip, ix, s, 3, x)
+    assert y[2] == 0.0 and not np.signbit(y[2])  # -0.0 - (-0.0) = +0.0

@@ -5,7 +5,6 @@ on two calls in a row, the second with a smaller input (the array keeps the
 first call's contents), plus its refusals: an input too large for the array,
 a non-finite value, a short COO capacity."""
 import ctypes
-import os
 
 import numpy as np
 import pytest
@@ -13,49 +12,9 @@ import pytest
 import holomap_amd as H
 from holomap_amd import _lib
 from holomap_amd import datasets as DS
+from hip_util import Hip
 
 pytestmark = pytest.mark.gpu
-
-
-class Hip:
-    """hipMalloc / hipMemcpy / hipFree of the HIP runtime (the test owns the
-    device buffers it hands to the entry point)."""
-
-    def __init__(self):
-        path = "libamdhip64.so"
-        for cand in ("/opt/rocm/lib/libamdhip64.so",):
-            if os.path.exists(cand):
-                path = cand
-        self.L = ctypes.CDLL(path)
-        self.L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
-        self.L.hipFree.argtypes = [ctypes.c_void_p]
-        self.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
-                                     ctypes.c_int]
-        self.L.hipMemGetInfo.argtypes = [ctypes.POINTER(ctypes.c_size_t),
-                                         ctypes.POINTER(ctypes.c_size_t)]
-        self.bufs = []
-
-    def alloc(self, nbytes):
-        p = ctypes.c_void_p()
-        assert self.L.hipMalloc(ctypes.byref(p), max(int(nbytes), 1)) == 0
-        self.bufs.append(p.value)
-        return p.value
-
-    def h2d(self, dst, a):
-        assert self.L.hipMemcpy(dst, a.ctypes.data, a.nbytes, 1) == 0
-
-    def d2h(self, a, src):
-        assert self.L.hipMemcpy(a.ctypes.data, src, a.nbytes, 2) == 0
-
-    def free_bytes(self):
-        f, t = ctypes.c_size_t(), ctypes.c_size_t()
-        assert self.L.hipMemGetInfo(ctypes.byref(f), ctypes.byref(t)) == 0
-        return f.value
-
-    def close(self):
-        for p in self.bufs:
-            self.L.hipFree(p)
-        self.bufs = []
 
 
 def noalloc_graph(ctx, hip, x, d, k, metric, cap=None, watch=None):
