@@ -10,6 +10,7 @@ argument meaning, output layout and error behaviour follow the Rust crate:
   Csr.from_fuzzy / row        src/sparse.rs:12-41
   connected_components        src/components.rs:7-39
   Laplacian.new / matvec      src/sparse.rs:46-93
+  EdgeSchedule, schedule_edges, default_n_epochs   src/sgd.rs:23-63
   HolomapError variants       src/api.rs:17-44
   Holomap.builder(seed)       src/api.rs:94-107, setters :195-235, and
   Holomap.graph(data, n_features) = the validation of src/api.rs:113-144
@@ -36,7 +37,8 @@ __all__ = [
     "Metric", "Knn", "SmoothKnn", "FuzzyGraph", "Csr", "HolomapError", "BadShape",
     "TooFewPoints", "InvalidParameter", "NonFiniteInput", "DeviceError", "Context", "Job",
     "exact_knn", "smooth_knn", "fuzzy_simplicial_set", "knn_graph", "Holomap",
-    "Components", "connected_components", "Laplacian",
+    "Components", "connected_components", "Laplacian", "EdgeSchedule", "schedule_edges",
+    "default_n_epochs",
     "PRECISIONS", "CALIBRATIONS",
 ]
 
@@ -219,6 +221,38 @@ class Laplacian:
                                                  sv.ctypes.data, self.s.n, xx.ctypes.data,
                                                  y.ctypes.data))
         return y
+
+
+def default_n_epochs(n: int) -> int:
+    """src/sgd.rs:23-25."""
+    return 500 if n <= 10_000 else 200
+
+
+@dataclass
+class EdgeSchedule:
+    """src/sgd.rs:30-34."""
+    head: np.ndarray  # uint32
+    tail: np.ndarray  # uint32
+    epochs_per_sample: np.ndarray  # float64
+
+
+def schedule_edges(graph: FuzzyGraph, n_epochs: int, ctx: "Context" = None) -> EdgeSchedule:
+    """src/sgd.rs:36-63 on the device: edges below max(w) / prune_epochs
+    dropped, epochs_per_sample = max(w) / w, in COO order."""
+    c = ctx or Context.default()
+    r = np.ascontiguousarray(graph.rows, np.uint32)
+    cl = np.ascontiguousarray(graph.cols, np.uint32)
+    w = np.ascontiguousarray(graph.vals, np.float32)
+    m = r.size
+    h = np.empty(max(m, 1), np.uint32)
+    t = np.empty(max(m, 1), np.uint32)
+    e = np.empty(max(m, 1), np.float64)
+    cnt = ctypes.c_uint64()
+    c._check(_lib.load().hm_schedule_edges(c.handle, r.ctypes.data, cl.ctypes.data, w.ctypes.data,
+                                           m, graph.n, int(n_epochs), h.ctypes.data,
+                                           t.ctypes.data, e.ctypes.data, ctypes.byref(cnt)))
+    k = int(cnt.value)
+    return EdgeSchedule(h[:k].copy(), t[:k].copy(), e[:k].copy())
 
 
 # ---------------------------------------------------------------- context
@@ -492,6 +526,25 @@ class Job:
         self.ctx._check(_lib.load().hm_job_connected_components(self.handle, ds, lab.ctypes.data,
                                                                 ctypes.byref(nc)))
         return Components(int(nc.value), lab)
+
+    def schedule_edges(self, ds: int, n_epochs: int) -> EdgeSchedule:
+        """src/sgd.rs:36-63 on dataset ds's resident graph."""
+        nnz = self.counters_nnz(ds)
+        h = np.empty(max(nnz, 1), np.uint32)
+        t = np.empty(max(nnz, 1), np.uint32)
+        e = np.empty(max(nnz, 1), np.float64)
+        cnt = ctypes.c_uint64()
+        self.ctx._check(_lib.load().hm_job_schedule_edges(self.handle, ds, int(n_epochs),
+                                                          h.ctypes.data, t.ctypes.data,
+                                                          e.ctypes.data, ctypes.byref(cnt)))
+        k = int(cnt.value)
+        return EdgeSchedule(h[:k].copy(), t[:k].copy(), e[:k].copy())
+
+    def counters_nnz(self, ds: int) -> int:
+        """Graph entries of dataset ds."""
+        o = _lib.GraphOut()
+        self.ctx._check(_lib.load().hm_job_download(self.handle, ds, ctypes.byref(o)))
+        return int(o.nnz)
 
     def laplacian_matvec(self, ds: int, x_dev: int, y_dev: int) -> None:
         """y = L x (src/sparse.rs:87-93) on dataset ds's resident graph; x and

@@ -116,4 +116,60 @@ __global__ void lap_matvec_kernel(const uint64_t* __restrict__ indptr,
   y[i] = x[i] - sx;
 }
 
+// ---- edge schedule of the SGD layout (src/sgd.rs:36-63) ----
+// max_w = fold(0.0, f32::max) over the weights (a max: order-free, two
+// launches); cutoff = max_w / prune_epochs in f32; an edge is kept unless
+// w < cutoff, in COO order (flags, exclusive scan, scatter), with
+// epochs_per_sample = f64(max_w) / f64(w).
+__device__ __forceinline__ float block_max_f32(float v) {
+  __shared__ float red[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  return fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+__global__ __launch_bounds__(256) void sched_max_part_kernel(const float* __restrict__ w,
+                                                             int64_t m, float* __restrict__ part) {
+  float v = 0.0f;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < m; e += (int64_t)gridDim.x * 256)
+    v = fmaxf(v, w[e]);
+  const float r = block_max_f32(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(256) void sched_max_kernel(const float* __restrict__ part, int np,
+                                                        float* __restrict__ out) {
+  float v = 0.0f;
+  for (int i = threadIdx.x; i < np; i += 256) v = fmaxf(v, part[i]);
+  const float r = block_max_f32(v);
+  if (threadIdx.x == 0) *out = r;
+}
+
+__global__ void sched_flag_kernel(const float* __restrict__ w, int64_t m,
+                                  const float* __restrict__ maxw, uint32_t prune_epochs,
+                                  uint32_t* __restrict__ keep) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  const float cutoff = *maxw / (float)prune_epochs;
+  keep[e] = w[e] < cutoff ? 0u : 1u;
+}
+
+__global__ void sched_scatter_kernel(const uint32_t* __restrict__ rows,
+                                     const uint32_t* __restrict__ cols,
+                                     const float* __restrict__ w, int64_t m,
+                                     const uint32_t* __restrict__ keep,
+                                     const uint32_t* __restrict__ pos,
+                                     const float* __restrict__ maxw, uint32_t* __restrict__ head,
+                                     uint32_t* __restrict__ tail, double* __restrict__ eps) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m || !keep[e]) return;
+  const uint32_t p = pos[e];
+  head[p] = rows[e];
+  tail[p] = cols[e];
+  eps[p] = (double)*maxw / (double)w[e];
+}
+
 }  // namespace hm

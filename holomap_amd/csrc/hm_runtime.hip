@@ -86,6 +86,12 @@ __global__ void lap_scale_kernel(const uint64_t*, const uint32_t*, const float*,
                                  int64_t, double*);
 __global__ void lap_matvec_kernel(const uint64_t*, const uint32_t*, const double*, const float*,
                                   const double*, int64_t, const double*, double*);
+__global__ void sched_max_part_kernel(const float*, int64_t, float*);
+__global__ void sched_max_kernel(const float*, int, float*);
+__global__ void sched_flag_kernel(const float*, int64_t, const float*, uint32_t, uint32_t*);
+__global__ void sched_scatter_kernel(const uint32_t*, const uint32_t*, const float*, int64_t,
+                                     const uint32_t*, const uint32_t*, const float*, uint32_t*,
+                                     uint32_t*, double*);
 __global__ void nonfinite_min_kernel(const uint64_t*, int, uint64_t*);
 template <int G>
 __global__ void union_count_kernel(const DsDesc*, int, int64_t, int64_t, const uint32_t*,
@@ -1752,6 +1758,44 @@ hm_status components_device(hm_ctx* ctx, hipStream_t st, const uint64_t* indptr,
   return HM_OK;
 }
 
+// src/sgd.rs:23-25: umap-learn's n_epochs default.
+uint32_t default_n_epochs(uint64_t n) { return n <= 10000 ? 500u : 200u; }
+
+// The edge schedule of the m-entry COO (rows, cols, w: device) into head,
+// tail, eps (device, capacity m); the kept count into *count. Synchronises.
+hm_status schedule_device(hm_ctx* ctx, hipStream_t st, const uint32_t* rows, const uint32_t* cols,
+                          const float* w, int64_t m, uint32_t prune_epochs, uint32_t* head,
+                          uint32_t* tail, double* eps, uint64_t* count) {
+  if (m == 0) {
+    *count = 0;
+    return HM_OK;
+  }
+  hm_status s = HM_OK;
+  DevBufs B(st);
+  const unsigned gp = (unsigned)std::min<int64_t>(512, cdiv(m, 256));
+  float* part = B.get<float>(ctx, gp + 1, &s);
+  uint32_t* keep = part ? B.get<uint32_t>(ctx, m, &s) : nullptr;
+  uint32_t* pos = keep ? B.get<uint32_t>(ctx, m, &s) : nullptr;
+  uint32_t* tot = pos ? B.get<uint32_t>(ctx, cdiv(m, kScanTile) + 3, &s) : nullptr;
+  if (!tot) return s;
+  hipLaunchKernelGGL(sched_max_part_kernel, dim3(gp), dim3(256), 0, st, w, m, part);
+  hipLaunchKernelGGL(sched_max_kernel, dim3(1), dim3(256), 0, st, (const float*)part, (int)gp,
+                     part + gp);
+  hipLaunchKernelGGL(sched_flag_kernel, dim3(grid_for(m, 256)), dim3(256), 0, st, w, m,
+                     (const float*)(part + gp), prune_epochs, keep);
+  HM_LAUNCH(ctx);
+  HM_TRY(scan_u32_excl(ctx, st, keep, m, pos, tot, tot + 1));
+  hipLaunchKernelGGL(sched_scatter_kernel, dim3(grid_for(m, 256)), dim3(256), 0, st, rows, cols, w,
+                     m, (const uint32_t*)keep, (const uint32_t*)pos, (const float*)(part + gp),
+                     head, tail, eps);
+  HM_LAUNCH(ctx);
+  uint32_t c = 0;
+  HM_HIP(ctx, hipMemcpyAsync(&c, tot, 4, hipMemcpyDeviceToHost, st));
+  HM_HIP(ctx, hipStreamSynchronize(st));
+  *count = c;
+  return HM_OK;
+}
+
 // A caller's CSR must index within its n vertices: indptr non-decreasing
 // from 0, indices < n (the kernels follow them).
 hm_status check_csr(hm_ctx* ctx, const uint64_t* indptr, const uint32_t* indices, uint64_t n) {
@@ -1862,6 +1906,43 @@ hm_status hm_laplacian_matvec(hm_ctx* ctx, const uint64_t* indptr, const uint32_
   return HM_OK;
 }
 
+hm_status hm_schedule_edges(hm_ctx* ctx, const uint32_t* rows, const uint32_t* cols,
+                            const float* vals, uint64_t nnz, uint64_t n, uint32_t n_epochs,
+                            uint32_t* head, uint32_t* tail, double* epochs_per_sample,
+                            uint64_t* count) {
+  if (!ctx || !count || (nnz && (!rows || !cols || !vals || !head || !tail || !epochs_per_sample)))
+    return fail(ctx, HM_EINVAL, "null argument");
+  if (nnz >= (1ull << 32)) return fail(ctx, HM_EUNSUPPORTED, "nnz >= 2^32");
+  const uint32_t prune = n_epochs > 10 ? n_epochs : default_n_epochs(n);
+  if (nnz == 0) {
+    *count = 0;
+    return HM_OK;
+  }
+  HM_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  hm_status s = HM_OK;
+  DevBufs B(st);
+  uint32_t* dr = B.get<uint32_t>(ctx, nnz, &s);
+  uint32_t* dc = dr ? B.get<uint32_t>(ctx, nnz, &s) : nullptr;
+  float* dw = dc ? B.get<float>(ctx, nnz, &s) : nullptr;
+  uint32_t* dh = dw ? B.get<uint32_t>(ctx, nnz, &s) : nullptr;
+  uint32_t* dt = dh ? B.get<uint32_t>(ctx, nnz, &s) : nullptr;
+  double* de = dt ? B.get<double>(ctx, nnz, &s) : nullptr;
+  if (!de) return s;
+  HM_HIP(ctx, hipMemcpyAsync(dr, rows, nnz * 4, hipMemcpyHostToDevice, st));
+  HM_HIP(ctx, hipMemcpyAsync(dc, cols, nnz * 4, hipMemcpyHostToDevice, st));
+  HM_HIP(ctx, hipMemcpyAsync(dw, vals, nnz * 4, hipMemcpyHostToDevice, st));
+  HM_TRY(schedule_device(ctx, st, dr, dc, dw, (int64_t)nnz, prune, dh, dt, de, count));
+  const uint64_t c = *count;
+  if (c) {
+    HM_HIP(ctx, hipMemcpyAsync(head, dh, c * 4, hipMemcpyDeviceToHost, st));
+    HM_HIP(ctx, hipMemcpyAsync(tail, dt, c * 4, hipMemcpyDeviceToHost, st));
+    HM_HIP(ctx, hipMemcpyAsync(epochs_per_sample, de, c * 8, hipMemcpyDeviceToHost, st));
+  }
+  HM_HIP(ctx, hipStreamSynchronize(st));
+  return HM_OK;
+}
+
 // Row pointer (local, n+1) and the start of dataset ds's COO in the job's
 // graph arrays.
 static hm_status job_csr(hm_job* J, uint32_t ds, const uint64_t** indptr, uint64_t* s0) {
@@ -1887,6 +1968,36 @@ hm_status hm_job_connected_components(hm_job* J, uint32_t ds, uint32_t* labels,
   if (!dout) return s;
   HM_TRY(components_device(ctx, J->stream, iptr, J->cols + s0, n, dout, n_components));
   HM_HIP(ctx, hipMemcpyAsync(labels, dout, n * 4, hipMemcpyDeviceToHost, J->stream));
+  HM_HIP(ctx, hipStreamSynchronize(J->stream));
+  return HM_OK;
+}
+
+hm_status hm_job_schedule_edges(hm_job* J, uint32_t ds, uint32_t n_epochs, uint32_t* head,
+                                uint32_t* tail, double* epochs_per_sample, uint64_t* count) {
+  if (!J || ds >= (uint32_t)J->nds || !head || !tail || !epochs_per_sample || !count)
+    return HM_EINVAL;
+  hm_ctx* ctx = J->ctx;
+  HM_HIP(ctx, hipSetDevice(ctx->device));
+  const uint64_t* iptr = nullptr;
+  uint64_t s0 = 0, s1 = 0;
+  HM_TRY(job_csr(J, ds, &iptr, &s0));
+  HM_HIP(ctx, hipMemcpy(&s1, iptr + J->dsh[ds].n, 8, hipMemcpyDeviceToHost));
+  const int64_t m = (int64_t)s1;  // the dataset's entries (local row pointer's end)
+  const uint32_t prune = n_epochs > 10 ? n_epochs : default_n_epochs((uint64_t)J->dsh[ds].n);
+  hm_status s = HM_OK;
+  DevBufs B(J->stream);
+  uint32_t* dh = B.get<uint32_t>(ctx, m, &s);
+  uint32_t* dt = dh ? B.get<uint32_t>(ctx, m, &s) : nullptr;
+  double* de = dt ? B.get<double>(ctx, m, &s) : nullptr;
+  if (!de) return s;
+  HM_TRY(schedule_device(ctx, J->stream, J->rows + s0, J->cols + s0, J->vals + s0, m, prune, dh,
+                         dt, de, count));
+  const uint64_t c = *count;
+  if (c) {
+    HM_HIP(ctx, hipMemcpyAsync(head, dh, c * 4, hipMemcpyDeviceToHost, J->stream));
+    HM_HIP(ctx, hipMemcpyAsync(tail, dt, c * 4, hipMemcpyDeviceToHost, J->stream));
+    HM_HIP(ctx, hipMemcpyAsync(epochs_per_sample, de, c * 8, hipMemcpyDeviceToHost, J->stream));
+  }
   HM_HIP(ctx, hipStreamSynchronize(J->stream));
   return HM_OK;
 }

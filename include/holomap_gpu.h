@@ -189,6 +189,20 @@ hm_status hm_job_connected_components(hm_job* job, uint32_t ds, uint32_t* labels
                                       uint64_t* n_components);
 hm_status hm_job_laplacian_matvec(hm_job* job, uint32_t ds, const double* x, double* y);
 
+/* src/sgd.rs:36-63 (schedule_edges): the SGD layout's edge list from the
+ * graph's COO (rows, cols, vals: nnz; n vertices): edges with weight below
+ * max(vals) / prune_epochs dropped (prune_epochs = n_epochs if > 10, else
+ * 500 for n <= 10,000 and 200 above), in COO order; head, tail,
+ * epochs_per_sample = max / w (capacity nnz each); *count kept. */
+hm_status hm_schedule_edges(hm_ctx* ctx, const uint32_t* rows, const uint32_t* cols,
+                            const float* vals, uint64_t nnz, uint64_t n, uint32_t n_epochs,
+                            uint32_t* head, uint32_t* tail, double* epochs_per_sample,
+                            uint64_t* count);
+/* The same on a job's resident graph of dataset ds (host outputs, capacity
+ * the dataset's nnz). */
+hm_status hm_job_schedule_edges(hm_job* job, uint32_t ds, uint32_t n_epochs, uint32_t* head,
+                                uint32_t* tail, double* epochs_per_sample, uint64_t* count);
+
 /* ---- allocation-free graph build ----
  * No counterpart in the reference. For callers that cannot allocate device
  * memory (code under stream capture, serving loops and harnesses that

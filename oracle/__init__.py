@@ -68,6 +68,8 @@ def lib():
         L.hmo_laplacian_new.restype = None
         L.hmo_laplacian_matvec.argtypes = [_u64p, _u32p, _f64p, _sz, _f64p, _f64p]
         L.hmo_laplacian_matvec.restype = None
+        L.hmo_schedule_edges.argtypes = [_u32p, _u32p, _f32p, _sz, _sz, _sz, _u32p, _u32p, _f64p]
+        L.hmo_schedule_edges.restype = _sz
         _lib = L
     return _lib
 
@@ -225,3 +227,24 @@ def laplacian_matvec(indptr, indices, svals, n: int, x) -> np.ndarray:
     lib().hmo_laplacian_matvec(ip, ix if ix.size else np.zeros(1, np.uint32),
                                sv if sv.size else np.zeros(1, np.float64), n, x, y)
     return y
+
+
+@dataclass
+class EdgeSchedule:
+    head: np.ndarray
+    tail: np.ndarray
+    epochs_per_sample: np.ndarray
+
+
+def schedule_edges(graph: FuzzyGraph, n_epochs: int) -> EdgeSchedule:
+    """src/sgd.rs:36-63."""
+    m = graph.rows.size
+    h = np.empty(max(m, 1), np.uint32)
+    t = np.empty(max(m, 1), np.uint32)
+    e = np.empty(max(m, 1), np.float64)
+    z32 = np.zeros(1, np.uint32)
+    c = lib().hmo_schedule_edges(np.ascontiguousarray(graph.rows, np.uint32) if m else z32,
+                                 np.ascontiguousarray(graph.cols, np.uint32) if m else z32,
+                                 np.ascontiguousarray(graph.vals, np.float32) if m
+                                 else np.zeros(1, np.float32), m, graph.n, n_epochs, h, t, e)
+    return EdgeSchedule(h[:c].copy(), t[:c].copy(), e[:c].copy())

@@ -418,3 +418,24 @@ void hmo_laplacian_matvec(const uint64_t* indptr, const uint32_t* indices, const
         y[i] = x[i] - sx;
     }
 }
+
+/* src/sgd.rs:23-63: max by fold(0.0, f32::max) (the non-NaN operand wins),
+ * cutoff in f32, edges kept unless w < cutoff, eps = f64(max) / f64(w). */
+size_t hmo_schedule_edges(const uint32_t* rows, const uint32_t* cols, const float* vals,
+                          size_t nnz, size_t n, size_t n_epochs, uint32_t* head, uint32_t* tail,
+                          double* eps) {
+    float max_w = 0.0f;
+    for (size_t e = 0; e < nnz; ++e) max_w = fmaxf(max_w, vals[e]);
+    const size_t prune = n_epochs > 10 ? n_epochs : (n <= 10000 ? 500 : 200);
+    const float cutoff = max_w / (float)prune;
+    size_t c = 0;
+    for (size_t e = 0; e < nnz; ++e) {
+        const float w = vals[e];
+        if (w < cutoff) continue;
+        head[c] = rows[e];
+        tail[c] = cols[e];
+        eps[c] = (double)max_w / (double)w;
+        ++c;
+    }
+    return c;
+}

@@ -112,3 +112,27 @@ def test_csr_input_checks(ctx):
                 np.ones(3))
     with pytest.raises(H.InvalidParameter):
         H.connected_components(bad, ctx=ctx)
+
+
+def test_schedule_edges_vs_oracle(ctx):
+    gs = graphs(ctx)
+    for name, g in gs:
+        for ne in [5, 200, 500, 2000]:
+            s = H.schedule_edges(g, ne, ctx=ctx)
+            r = O.schedule_edges(O.FuzzyGraph(g.n, g.rows, g.cols, g.vals), ne)
+            assert s.head.tobytes() == r.head.tobytes(), (name, ne)
+            assert s.tail.tobytes() == r.tail.tobytes(), (name, ne)
+            assert s.epochs_per_sample.tobytes() == r.epochs_per_sample.tobytes(), (name, ne)
+    # the resident graph of a job: the same schedule as from its download
+    x = DS.c2(n=4000, d=32)
+    job = H.Job(ctx, [4000], [32], [15], ["euclidean"])
+    job.upload(0, x)
+    job.run()
+    job.sync()
+    g = job.download(0).graph
+    for ne in [7, 300]:
+        s = job.schedule_edges(0, ne)
+        r = O.schedule_edges(O.FuzzyGraph(g.n, g.rows, g.cols, g.vals), ne)
+        assert s.head.tobytes() == r.head.tobytes() and s.tail.tobytes() == r.tail.tobytes()
+        assert s.epochs_per_sample.tobytes() == r.epochs_per_sample.tobytes()
+    job.close()

@@ -339,3 +339,36 @@ def test_laplacian_isolated_vertex_degree_is_negative_zero():
     x = np.array([1.0, 2.0, -0.0])
     y = O.laplacian_matvec(ip, ix, s, 3, x)
     assert y[2] == 0.0 and not np.signbit(y[2])  # -0.0 - (-0.0) = +0.0
+
+
+def test_schedule_edges_known_answers():
+    # src/sgd.rs:168-189, 230-242
+    g = O.FuzzyGraph(4, np.array([0, 1, 2], np.uint32), np.array([1, 2, 3], np.uint32),
+                     np.array([1.0, 0.5, 0.001], np.float32))
+    s = O.schedule_edges(g, 500)
+    assert s.head.tolist() == [0, 1] and s.tail.tolist() == [1, 2]
+    assert s.epochs_per_sample.tolist() == [1.0, 2.0]
+    g = O.FuzzyGraph(3, np.array([0, 1], np.uint32), np.array([1, 2], np.uint32),
+                     np.array([1.0, 0.01], np.float32))
+    assert O.schedule_edges(g, 10).head.size == 2
+    # n_epochs <= 10 prunes by the default schedule: 500 up to 10,000 points,
+    # 200 above (src/sgd.rs:23-25): 0.004 survives 1/500, not 1/200
+    for n, kept in [(100, 2), (10000, 2), (10001, 1)]:
+        g = O.FuzzyGraph(n, np.array([0, 1], np.uint32), np.array([1, 2], np.uint32),
+                         np.array([1.0, 0.004], np.float32))
+        assert O.schedule_edges(g, 5).head.size == kept, n
+
+
+def test_schedule_edges_against_numpy():
+    rng = np.random.default_rng(9)
+    m = 5000
+    g = O.FuzzyGraph(3000, rng.integers(0, 3000, m).astype(np.uint32),
+                     rng.integers(0, 3000, m).astype(np.uint32),
+                     (rng.random(m) ** 6).astype(np.float32))
+    for ne in [3, 50, 500]:
+        s = O.schedule_edges(g, ne)
+        mx = np.float32(g.vals.max())
+        prune = ne if ne > 10 else 500
+        keep = ~(g.vals < np.float32(mx / np.float32(prune)))
+        assert np.array_equal(s.head, g.rows[keep]) and np.array_equal(s.tail, g.cols[keep])
+        assert np.array_equal(s.epochs_per_sample, np.float64(mx) / g.vals[keep].astype(np.float64))
