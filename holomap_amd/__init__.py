@@ -649,6 +649,18 @@ class Server:
             pass
 
 
+def l2_normalise(vectors, ctx: Context = None) -> np.ndarray:
+    """crates/holomap-clusterer/src/pipeline.rs:151-162 on the device: each
+    row divided by its f32 norm, zero rows as they are."""
+    c = ctx or Context.default()
+    a = np.ascontiguousarray(vectors, np.float32)
+    assert a.ndim == 2
+    out = np.empty_like(a)
+    c._check(_lib.load().hm_l2_normalise(c.handle, a.ctypes.data, a.shape[0], a.shape[1],
+                                         out.ctypes.data))
+    return out
+
+
 def expf_device(x, ctx: Context = None) -> np.ndarray:
     """The device's f32 exp (the reference's libm method) over an array."""
     c = ctx or Context.default()

@@ -52,7 +52,7 @@ EXPORTS = [
     "hm_server_progress", "hm_server_state", "hm_server_pending", "hm_server_result_info",
     "hm_server_fetch", "hm_knn_graph_noalloc", "hm_connected_components", "hm_laplacian_new",
     "hm_laplacian_matvec", "hm_job_connected_components", "hm_job_laplacian_matvec",
-    "hm_schedule_edges", "hm_job_schedule_edges",
+    "hm_schedule_edges", "hm_job_schedule_edges", "hm_l2_normalise",
 ]
 
 
@@ -111,6 +111,7 @@ def load():
         "hm_job_laplacian_matvec": (i32, [vp, u32, vp, vp]),
         "hm_schedule_edges": (i32, [vp, vp, vp, vp, u64, u64, u32, vp, vp, vp, P(u64)]),
         "hm_job_schedule_edges": (i32, [vp, u32, u32, vp, vp, vp, P(u64)]),
+        "hm_l2_normalise": (i32, [vp, vp, u64, u32, vp]),
         "hm_job_create": (i32, [vp, u32, vp, vp, vp, vp, P(Options), u32, P(vp)]),
         "hm_job_destroy": (None, [vp]),
         "hm_job_bytes": (u64, [vp]),

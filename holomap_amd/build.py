@@ -2,7 +2,8 @@
 
 Each HIP source is compiled with hipcc to an object in holomap_amd/_build/
 (in parallel) and linked into one shared library that exports the C ABI of
-include/holomap_gpu.h.
+include/holomap_gpu.h; the JSON-lines front door (csrc/hm_jsonl.cpp) is
+linked against it as holomap_amd/holomap_gpu_jsonl.
 
 Numerics flags: -ffp-contract=off keeps every a*b+c as two roundings (the
 reference's f64 distance folds and f32 calibration depend on it; the MFMA
@@ -20,6 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libholomap_gpu.so")
+JSONL = os.path.join(HERE, "holomap_gpu_jsonl")  # the JSON-lines front door
 SOURCES = ["hm_prep.hip", "hm_gram.hip", "hm_knn.hip", "hm_calib.hip", "hm_sort.hip",
            "hm_fuzzy.hip", "hm_graph.hip", "hm_runtime.hip", "hm_serve.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -41,6 +43,18 @@ def _compile(src: str) -> str:
     return obj
 
 
+def build_jsonl() -> str:
+    """The JSON-lines front door (host C++ over the library's C ABI)."""
+    tmp = JSONL + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "hm_jsonl.cpp"),
+           f"-L{HERE}", "-lholomap_gpu", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hm_jsonl.cpp failed:\n{r.stderr}")
+    os.replace(tmp, JSONL)
+    return JSONL
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -51,8 +65,9 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_jsonl()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {JSONL}")
     return LIB
 
 

@@ -309,6 +309,26 @@ __global__ __launch_bounds__(256) void nonfinite_part_kernel(const float* __rest
   if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
+// f32 L2 normalisation of each row (crates/holomap-clusterer/src/
+// pipeline.rs:151-162): norm = sqrt of the squares folded left to right from
+// -0.0 (Rust's f32 Sum), a zero row left as it is, else every value divided
+// by the norm; a thread per row keeps the fold's order.
+__global__ void l2_normalise_kernel(const float* __restrict__ x, int64_t n, int d,
+                                    float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* r = x + i * d;
+  float s = -0.0f;
+  for (int f = 0; f < d; ++f) s = s + r[f] * r[f];
+  const float norm = sqrtf(s);
+  float* o = out + i * d;
+  if (norm == 0.0f) {
+    for (int f = 0; f < d; ++f) o[f] = r[f];
+  } else {
+    for (int f = 0; f < d; ++f) o[f] = r[f] / norm;
+  }
+}
+
 __global__ __launch_bounds__(256) void nonfinite_min_kernel(const uint64_t* __restrict__ part,
                                                             int nparts,
                                                             uint64_t* __restrict__ out) {

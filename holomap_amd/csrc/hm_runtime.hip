@@ -93,6 +93,7 @@ __global__ void sched_scatter_kernel(const uint32_t*, const uint32_t*, const flo
                                      const uint32_t*, const uint32_t*, const float*, uint32_t*,
                                      uint32_t*, double*);
 __global__ void nonfinite_min_kernel(const uint64_t*, int, uint64_t*);
+__global__ void l2_normalise_kernel(const float*, int64_t, int, float*);
 template <int G>
 __global__ void union_count_kernel(const DsDesc*, int, int64_t, int64_t, const uint32_t*,
                                    const float*, const uint32_t*, const uint32_t*,
@@ -2030,6 +2031,34 @@ hm_status hm_job_laplacian_matvec(hm_job* J, uint32_t ds, const double* x, doubl
   HM_LAUNCH(ctx);
   HM_HIP(ctx, hipStreamSynchronize(J->stream));
   return HM_OK;
+}
+
+hm_status hm_l2_normalise(hm_ctx* ctx, const float* data, uint64_t n, uint32_t d, float* out) {
+  if (!ctx || !data || !out) return fail(ctx, HM_EINVAL, "null argument");
+  if (n == 0 || d == 0) return HM_OK;
+  HM_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const uint64_t m = n * d;
+  float *dx = nullptr, *dy = nullptr;
+  HM_HIP(ctx, hipMallocAsync((void**)&dx, m * 4, st));
+  if (hipMallocAsync((void**)&dy, m * 4, st) != hipSuccess) {
+    (void)hipFreeAsync(dx, st);
+    return fail(ctx, HM_ENOMEM, "device allocation failed");
+  }
+  hm_status s = HM_OK;
+  if (hipMemcpyAsync(dx, data, m * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+    s = fail(ctx, HM_EDEVICE, "upload failed");
+  if (s == HM_OK) {
+    hipLaunchKernelGGL(l2_normalise_kernel, dim3(grid_for((int64_t)n, 256)), dim3(256), 0, st,
+                       (const float*)dx, (int64_t)n, (int)d, dy);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(out, dy, m * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      s = fail(ctx, HM_EDEVICE, "l2_normalise kernel failed");
+  }
+  (void)hipFreeAsync(dx, st);
+  (void)hipFreeAsync(dy, st);
+  return s;
 }
 
 hm_status hm_expf_array(hm_ctx* ctx, const float* x, float* y, uint64_t n) {

@@ -294,6 +294,10 @@ hm_status hm_server_result_info(hm_server* srv, uint64_t id, uint64_t* n, uint32
  * the request; a failed request returns its error status and is released. */
 hm_status hm_server_fetch(hm_server* srv, uint64_t id, hm_graph_out* out);
 
+/* crates/holomap-clusterer/src/pipeline.rs:151-162 (l2_normalise): every
+ * row of n x d divided by its f32 norm (zero rows copied); host buffers. */
+hm_status hm_l2_normalise(hm_ctx* ctx, const float* data, uint64_t n, uint32_t d, float* out);
+
 /* Parity helper: the device expf (the reference's libm method) over an
  * array, for testing the calibration arithmetic. */
 hm_status hm_expf_array(hm_ctx* ctx, const float* x, float* y, uint64_t n);

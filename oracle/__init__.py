@@ -70,6 +70,8 @@ def lib():
         L.hmo_laplacian_matvec.restype = None
         L.hmo_schedule_edges.argtypes = [_u32p, _u32p, _f32p, _sz, _sz, _sz, _u32p, _u32p, _f64p]
         L.hmo_schedule_edges.restype = _sz
+        L.hmo_l2_normalise.argtypes = [_f32p, _sz, _sz, _f32p]
+        L.hmo_l2_normalise.restype = None
         _lib = L
     return _lib
 
@@ -248,3 +250,11 @@ def schedule_edges(graph: FuzzyGraph, n_epochs: int) -> EdgeSchedule:
                                  np.ascontiguousarray(graph.vals, np.float32) if m
                                  else np.zeros(1, np.float32), m, graph.n, n_epochs, h, t, e)
     return EdgeSchedule(h[:c].copy(), t[:c].copy(), e[:c].copy())
+
+
+def l2_normalise(x) -> np.ndarray:
+    """crates/holomap-clusterer/src/pipeline.rs:151-162."""
+    a = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(a)
+    lib().hmo_l2_normalise(a.reshape(-1), a.shape[0], a.shape[1], out.reshape(-1))
+    return out

@@ -439,3 +439,16 @@ size_t hmo_schedule_edges(const uint32_t* rows, const uint32_t* cols, const floa
     }
     return c;
 }
+
+/* crates/holomap-clusterer/src/pipeline.rs:151-162: norm = sqrt(sum of
+ * x*x) with Rust's f32 Sum (a left fold from -0.0), zero rows copied, else
+ * x / norm, all in f32. */
+void hmo_l2_normalise(const float* x, size_t n, size_t d, float* out) {
+    for (size_t i = 0; i < n; ++i) {
+        const float* r = x + i * d;
+        float s = -0.0f;
+        for (size_t f = 0; f < d; ++f) s = s + r[f] * r[f];
+        const float norm = sqrtf(s);
+        for (size_t f = 0; f < d; ++f) out[i * d + f] = norm == 0.0f ? r[f] : r[f] / norm;
+    }
+}

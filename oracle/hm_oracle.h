@@ -23,6 +23,7 @@
  *   hmo_laplacian_new    src/sparse.rs:55-78
  *   hmo_laplacian_matvec src/sparse.rs:87-93
  *   hmo_schedule_edges   src/sgd.rs:23-63
+ *   hmo_l2_normalise     crates/holomap-clusterer/src/pipeline.rs:151-162
  * hmo_smooth_knn_f64 is an FP64 bisection run to convergence: the FP64
  * oracle for the GPU's FP64 calibration mode (no reference counterpart).
  */
@@ -66,6 +67,8 @@ void hmo_laplacian_new(const uint64_t* indptr, const uint32_t* indices, const fl
 size_t hmo_schedule_edges(const uint32_t* rows, const uint32_t* cols, const float* vals,
                           size_t nnz, size_t n, size_t n_epochs, uint32_t* head, uint32_t* tail,
                           double* eps);
+/* crates/holomap-clusterer/src/pipeline.rs:151-162. */
+void hmo_l2_normalise(const float* x, size_t n, size_t d, float* out);
 /* src/sparse.rs:87-93: y = x - S x. */
 void hmo_laplacian_matvec(const uint64_t* indptr, const uint32_t* indices, const double* svals,
                           size_t n, const double* x, double* y);

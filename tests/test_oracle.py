@@ -372,3 +372,24 @@ def test_schedule_edges_against_numpy():
         keep = ~(g.vals < np.float32(mx / np.float32(prune)))
         assert np.array_equal(s.head, g.rows[keep]) and np.array_equal(s.tail, g.cols[keep])
         assert np.array_equal(s.epochs_per_sample, np.float64(mx) / g.vals[keep].astype(np.float64))
+
+
+def test_l2_normalise_against_numpy_sequential():
+    # crates/holomap-clusterer/src/pipeline.rs:151-162: an f32 left fold of
+    # the squares (numpy's cumsum is sequential), sqrt, f32 division; zero
+    # rows unchanged
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(200, 37)).astype(np.float32) * np.float32(3.0)
+    x[5] = 0.0
+    x[6, :] = 0.0
+    x[6, 3] = -2.5
+    got = O.l2_normalise(x)
+    for i in range(x.shape[0]):
+        s = np.cumsum(x[i] * x[i], dtype=np.float32)[-1]
+        norm = np.sqrt(s, dtype=np.float32)
+        want = x[i] if norm == 0 else x[i] / norm
+        assert got[i].tobytes() == want.astype(np.float32).tobytes(), i
+    assert got[5].tobytes() == x[5].tobytes()
+    assert got[6, 3] == -1.0
+    assert O.l2_normalise(np.array([[3.0, 4.0]], np.float32)).tolist() == [[0.6000000238418579,
+                                                                            0.800000011920929]]
