@@ -49,6 +49,7 @@ pub const HM_T_FALLBACK: usize = 3;
 pub const HM_T_CALIB: usize = 4;
 pub const HM_T_FUZZY: usize = 5;
 pub const HM_T_TOTAL: usize = 6;
+pub const HM_T_THRESHOLD: usize = 7;
 
 pub const HM_NCOUNTERS: usize = 8;
 pub const HM_C_FALLBACK_ROWS: usize = 0;

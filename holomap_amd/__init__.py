@@ -413,20 +413,40 @@ class GraphResult:
 
 def knn_graph(data, n_features: int, k: int, metric="euclidean", *, precision="bf16",
               calib="ref_f32", margin: int = -1, ctx: Context = None) -> GraphResult:
-    """The fused seam of src/api.rs:147-149 plus the CSR row pointer."""
+    """The fused seam of src/api.rs:147-149 plus the CSR row pointer: one
+    hm_knn_graph call (host buffers in and out; the C entry plans, runs and
+    frees a job on the context's device)."""
     x = np.ascontiguousarray(data, np.float32).reshape(-1)
     _validate(x, n_features, k)
     c = ctx or Context.default()
     n = x.size // n_features
-    job = Job(c, [n], [n_features], [k], [_metric(metric)], precision=precision, calib=calib,
-              margin=margin)
-    try:
-        job.upload(0, x)
-        job.run()
-        job.sync()
-        return job.download(0)
-    finally:
-        job.close()
+    idx = np.empty((n, k), np.uint32)
+    dist = np.empty((n, k), np.float32)
+    sig = np.empty(n, np.float32)
+    rho = np.empty(n, np.float32)
+    cm = CALIBRATIONS[calib] if isinstance(calib, str) else int(calib)
+    s64 = np.empty(n, np.float64) if cm == HM_CALIB_F64 else None
+    cap = 2 * n * k
+    rows = np.empty(cap, np.uint32)
+    cols = np.empty(cap, np.uint32)
+    vals = np.empty(cap, np.float32)
+    indptr = np.empty(n + 1, np.uint64)
+    o = _lib.GraphOut()
+    o.indices, o.dists = idx.ctypes.data, dist.ctypes.data
+    o.sigmas, o.rhos = sig.ctypes.data, rho.ctypes.data
+    o.sigmas64 = s64.ctypes.data if s64 is not None else None
+    o.rows, o.cols, o.vals = rows.ctypes.data, cols.ctypes.data, vals.ctypes.data
+    o.indptr = indptr.ctypes.data
+    o.cap = cap
+    opt = _opts(precision, calib, margin)
+    st = _lib.load().hm_knn_graph(c.handle, x.ctypes.data, n, n_features, k, _metric(metric),
+                                  ctypes.byref(opt), ctypes.byref(o))
+    if st == _lib.HM_EBUDGET:
+        raise MemoryError(c.last_error())
+    c._check(st, n=n, k=k)
+    m = int(o.nnz)
+    g = FuzzyGraph(n, rows[:m].copy(), cols[:m].copy(), vals[:m].copy())
+    return GraphResult(Knn(k, idx, dist), SmoothKnn(sig, rho, s64), g, indptr)
 
 
 # ---------------------------------------------------------------- jobs

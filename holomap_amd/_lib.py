@@ -37,7 +37,7 @@ HM_STAGE_FUZZY = 4
 HM_STAGE_ALL = 7
 HM_NTIMES = 8
 HM_NCOUNTERS = 8
-TIME_NAMES = ["prep", "sweep", "rerank", "fallback", "calib", "fuzzy", "total"]
+TIME_NAMES = ["prep", "sweep", "rerank", "fallback", "calib", "fuzzy", "total", "threshold"]
 
 # every symbol include/holomap_gpu.h declares
 EXPORTS = [

@@ -555,51 +555,69 @@ constexpr int kListRows[3] = {128, 256, 0};  // query rows per item (0: sweep_ro
 constexpr int kListPerCu[3] = {2, 1, 1};     // workgroups per CU
 
 
-// Both passes of the candidate sweep in `mode` over `items` (a host count,
-// or a device count when nitems_dev is set; qlist maps item rows to global
-// row ids when set). sample: the threshold pass sweeps each dataset's
-// sample of blocks (the primary sweep) rather than every block (the
-// escalation's re-sweep, whose rows need the tightest threshold).
+// Passes of the candidate sweep in `mode` over `items` (a host count, or a
+// device count when nitems_dev is set; qlist maps item rows to global row
+// ids when set): passes 0 (threshold only), 1 (collect only) or 2 (both).
+// sample: the threshold pass sweeps each dataset's sample of blocks (the
+// primary sweep) rather than every block (the escalation's re-sweep, whose
+// rows need the tightest threshold).
 hm_status launch_sweep(hm_job* J, int mode, const WorkItem* items, const int* nitems_dev,
-                       int nitems_host, const uint32_t* qlist, unsigned grid, int sample) {
+                       int nitems_host, const uint32_t* qlist, unsigned grid, int sample,
+                       int passes) {
 #define HM_GRAM(M, P)                                                                        \
   hipLaunchKernelGGL((gram_sweep_kernel<M, P>), dim3(grid), dim3(sweep_threads(M)), 0,         \
                      J->stream, J->d_desc, items, nitems_dev, nitems_host, qlist, sample)
+#define HM_GRAM2(M)                      \
+  do {                                   \
+    if (passes != 1) HM_GRAM(M, 0);      \
+    if (passes != 0) HM_GRAM(M, 1);      \
+  } while (0)
   switch (mode) {
-    case 0: HM_GRAM(0, 0); HM_GRAM(0, 1); break;
-    case 1: HM_GRAM(1, 0); HM_GRAM(1, 1); break;
-    case 2: HM_GRAM(2, 0); HM_GRAM(2, 1); break;
+    case 0: HM_GRAM2(0); break;
+    case 1: HM_GRAM2(1); break;
+    case 2: HM_GRAM2(2); break;
     default: break;
   }
+#undef HM_GRAM2
 #undef HM_GRAM
   HM_LAUNCH(J->ctx);
   return HM_OK;
 }
 
-// Both passes of the primary sweep over its three item lists (datasets with
-// short bf16 rows keep their query rows in LDS, the rest stream them with
-// the candidates). Grids match the item permutation of each list.
+void mark(hm_job* J, int i);
+
+// The primary sweep over its three item lists (datasets with short bf16
+// rows keep their query rows in LDS, the rest stream them with the
+// candidates): the threshold passes of every list, then the collect passes
+// (a list's collect pass reads only its own thresholds). Grids match the
+// item permutation of each list.
 hm_status launch_sweep_l1(hm_job* J) {
   hm_ctx* ctx = J->ctx;
-  const WorkItem* it = J->items_l1;
+  const WorkItem* it0 = J->items_l1;
   const int n0 = J->n_items_list[0], n1 = J->n_items_list[1], n2 = J->n_items_list[2];
+  for (int pass = 0; pass < 2; ++pass) {
+    const WorkItem* it = it0;
 #define HM_QRES(CFG, N, PER)                                                                   \
   do {                                                                                         \
     const unsigned grid = (unsigned)sweep_grid(ctx, 0, N, PER);                                \
-    hipLaunchKernelGGL((gram_sweep_qres_kernel<CFG, 0>), dim3(grid), dim3(CFG::NW * 64), 0,      \
-                       J->stream, J->d_desc, it, nullptr, N, nullptr, 1);                      \
-    hipLaunchKernelGGL((gram_sweep_qres_kernel<CFG, 1>), dim3(grid), dim3(CFG::NW * 64), 0,      \
-                       J->stream, J->d_desc, it, nullptr, N, nullptr, 1);                      \
+    if (pass == 0)                                                                             \
+      hipLaunchKernelGGL((gram_sweep_qres_kernel<CFG, 0>), dim3(grid), dim3(CFG::NW * 64), 0,    \
+                         J->stream, J->d_desc, it, nullptr, N, nullptr, 1);                    \
+    else                                                                                       \
+      hipLaunchKernelGGL((gram_sweep_qres_kernel<CFG, 1>), dim3(grid), dim3(CFG::NW * 64), 0,    \
+                         J->stream, J->d_desc, it, nullptr, N, nullptr, 1);                    \
     HM_LAUNCH(ctx);                                                                            \
   } while (0)
-  if (n0 > 0) HM_QRES(QresPair, n0, kListPerCu[0]);
-  it += n0;
-  if (n1 > 0) HM_QRES(QresWide, n1, kListPerCu[1]);
-  it += n1;
+    if (n0 > 0) HM_QRES(QresPair, n0, kListPerCu[0]);
+    it += n0;
+    if (n1 > 0) HM_QRES(QresWide, n1, kListPerCu[1]);
+    it += n1;
 #undef HM_QRES
-  if (n2 > 0)
-    HM_TRY(launch_sweep(J, J->mode, it, nullptr, n2, nullptr,
-                        (unsigned)sweep_grid(ctx, J->mode, n2), 1));
+    if (n2 > 0)
+      HM_TRY(launch_sweep(J, J->mode, it, nullptr, n2, nullptr,
+                          (unsigned)sweep_grid(ctx, J->mode, n2), 1, pass));
+    if (pass == 0) mark(J, 7);
+  }
   return HM_OK;
 }
 
@@ -654,13 +672,14 @@ hm_status enqueue_pipeline(hm_job* J) {
                              rate, J->items_l2, J->nitems_l2, J->nitems_l2 + 1);
         }
         HM_LAUNCH(ctx);
-        HM_TRY(launch_sweep(J, em, J->items_l2, J->nitems_l2, 0, J->list, (unsigned)cus, 0));
+        HM_TRY(launch_sweep(J, em, J->items_l2, J->nitems_l2, 0, J->list, (unsigned)cus, 0, 2));
         hipLaunchKernelGGL(rerank_kernel, dim3(std::max<int64_t>(1, std::min<int64_t>(cdiv(R, 4), 8 * cus))),
                            dim3(256), 0, J->stream, J->d_desc, nds, R, em, J->flag,
                            (const uint32_t*)J->list, (const int*)(J->nitems_l2 + 1));
         HM_LAUNCH(ctx);
       }
     } else {
+      mark(J, 7);
       mark(J, 2);
       hipLaunchKernelGGL(fill_u32, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, J->flag, R, 1u);
       HM_LAUNCH(ctx);
@@ -709,6 +728,7 @@ hm_status enqueue_pipeline(hm_job* J) {
     mark(J, 4);
   } else {
     for (int i = 1; i <= 4; ++i) mark(J, i);
+    mark(J, 7);
   }
   if (J->stages & HM_STAGE_CALIB) {
     // G lanes per row when the largest k fits a group, else a thread per row.
@@ -1360,12 +1380,14 @@ hm_status hm_job_sync(hm_job* J) {
   if (!J) return HM_EINVAL;
   HM_HIP(J->ctx, hipStreamSynchronize(J->stream));
   if (J->timed) {
-    // stage boundaries: 0 start,1 prep,2 sweep,3 rerank,4 fallback,5 calib,6 fuzzy
-    const int map[6][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {5, 6}};
-    for (int s = 0; s < 6; ++s) {
+    // stage boundaries: 0 start,1 prep,2 sweep,3 rerank,4 fallback,5 calib,
+    // 6 fuzzy; 7 the sweep's threshold passes (inside the sweep stage)
+    const int map[7][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {5, 6}, {1, 7}};
+    for (int s = 0; s < 7; ++s) {
+      const int slot = s < 6 ? s : HM_T_THRESHOLD;
       float ms = 0.f;
       (void)hipEventElapsedTime(&ms, J->ev[map[s][0]], J->ev[map[s][1]]);
-      J->times[s] = ms;
+      J->times[slot] = ms;
     }
     float tot = 0.f;
     (void)hipEventElapsedTime(&tot, J->ev[0], J->ev[6]);

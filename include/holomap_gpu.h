@@ -93,6 +93,7 @@ typedef struct {
 #define HM_T_CALIB 4
 #define HM_T_FUZZY 5
 #define HM_T_TOTAL 6
+#define HM_T_THRESHOLD 7 /* the sweep's threshold passes, part of HM_T_SWEEP */
 
 /* Counters of the last run (hm_job_counters). */
 #define HM_NCOUNTERS 8
