@@ -316,7 +316,12 @@ def target_check(ctx, steps, with_cpu=True):
            "roofline": sweep_roofline(n, d, "bf16", stage, "c3"),
            "fp32_roofline": sweep_roofline(n, d, "fp32", st32, "c3"),
            "stage_bandwidth": stage_bandwidth(n, d, k, cnt, stage),
-           "fp64_error": fp64_error(x, k, res)}
+           # the library's own statement (largest relative error of the f32
+           # distances, sigmas and directed strengths against its f64 values)
+           # and an independent FP64 chain computed here in numpy
+           "fp64_error": {"library_statement": dict(zip(["distance", "sigma", "strength"],
+                                                        res.err64 or (None,) * 3)),
+                          **fp64_error(x, k, res)}}
     if with_cpu:
         log("c3: cpu baseline")
         cb = cpu_baseline(x, d, k, metric, res)

@@ -73,6 +73,8 @@ pub(crate) fn knn_graph(data: &[f32], n_features: usize, k: usize, metric: Metri
             sigmas64: std::ptr::null_mut(),
             cap: cap as u64,
             nnz: 0,
+            dists64: std::ptr::null_mut(),
+            err64: [0.0; 3],
         };
         let m = match metric {
             Metric::Euclidean => sys::HM_EUCLIDEAN,

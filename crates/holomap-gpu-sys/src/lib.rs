@@ -103,6 +103,8 @@ pub struct hm_graph_out {
     pub sigmas64: *mut f64,
     pub cap: u64,
     pub nnz: u64,
+    pub dists64: *mut f64,
+    pub err64: [f64; 3],
 }
 
 #[repr(C)]

@@ -116,6 +116,7 @@ class Knn:
     k: int
     indices: np.ndarray  # uint32 (n, k)
     dists: np.ndarray  # float32 (n, k)
+    dists64: Optional[np.ndarray] = None  # float64 (n, k): FP64 calibration with the kNN here
 
 
 @dataclass
@@ -409,6 +410,15 @@ class GraphResult:
     calib: SmoothKnn
     graph: FuzzyGraph
     indptr: np.ndarray
+    # FP64 calibration with the kNN on the device: the largest relative error
+    # of the f32 distances, sigmas and directed membership strengths against
+    # their f64 values (the FP64 chain); None otherwise
+    err64: Optional[tuple] = None
+
+
+def _err64(o) -> Optional[tuple]:
+    e = tuple(float(v) for v in o.err64)
+    return None if e[0] < 0 else e
 
 
 def knn_graph(data, n_features: int, k: int, metric="euclidean", *, precision="bf16",
@@ -426,6 +436,7 @@ def knn_graph(data, n_features: int, k: int, metric="euclidean", *, precision="b
     rho = np.empty(n, np.float32)
     cm = CALIBRATIONS[calib] if isinstance(calib, str) else int(calib)
     s64 = np.empty(n, np.float64) if cm == HM_CALIB_F64 else None
+    d64 = np.empty((n, k), np.float64) if cm == HM_CALIB_F64 else None
     cap = 2 * n * k
     rows = np.empty(cap, np.uint32)
     cols = np.empty(cap, np.uint32)
@@ -435,6 +446,7 @@ def knn_graph(data, n_features: int, k: int, metric="euclidean", *, precision="b
     o.indices, o.dists = idx.ctypes.data, dist.ctypes.data
     o.sigmas, o.rhos = sig.ctypes.data, rho.ctypes.data
     o.sigmas64 = s64.ctypes.data if s64 is not None else None
+    o.dists64 = d64.ctypes.data if d64 is not None else None
     o.rows, o.cols, o.vals = rows.ctypes.data, cols.ctypes.data, vals.ctypes.data
     o.indptr = indptr.ctypes.data
     o.cap = cap
@@ -446,7 +458,7 @@ def knn_graph(data, n_features: int, k: int, metric="euclidean", *, precision="b
     c._check(st, n=n, k=k)
     m = int(o.nnz)
     g = FuzzyGraph(n, rows[:m].copy(), cols[:m].copy(), vals[:m].copy())
-    return GraphResult(Knn(k, idx, dist), SmoothKnn(sig, rho, s64), g, indptr)
+    return GraphResult(Knn(k, idx, dist, d64), SmoothKnn(sig, rho, s64), g, indptr, _err64(o))
 
 
 # ---------------------------------------------------------------- jobs
@@ -520,7 +532,9 @@ class Job:
         dist = np.empty((n, k), np.float32)
         sig = np.empty(n, np.float32)
         rho = np.empty(n, np.float32)
-        s64 = np.empty(n, np.float64) if (with_sigma64 or self.calib_mode == HM_CALIB_F64) else None
+        f64 = self.calib_mode == HM_CALIB_F64
+        s64 = np.empty(n, np.float64) if (with_sigma64 or f64) else None
+        d64 = np.empty((n, k), np.float64) if (f64 and self.stages & HM_STAGE_KNN) else None
         cap = 2 * n * k
         rows = np.empty(cap, np.uint32)
         cols = np.empty(cap, np.uint32)
@@ -530,6 +544,7 @@ class Job:
         o.indices, o.dists = idx.ctypes.data, dist.ctypes.data
         o.sigmas, o.rhos = sig.ctypes.data, rho.ctypes.data
         o.sigmas64 = s64.ctypes.data if s64 is not None else None
+        o.dists64 = d64.ctypes.data if d64 is not None else None
         if self.stages & HM_STAGE_FUZZY:
             o.rows, o.cols, o.vals = rows.ctypes.data, cols.ctypes.data, vals.ctypes.data
             o.indptr = indptr.ctypes.data
@@ -537,7 +552,8 @@ class Job:
         self.ctx._check(_lib.load().hm_job_download(self.handle, ds, ctypes.byref(o)))
         m = int(o.nnz)
         g = FuzzyGraph(n, rows[:m].copy(), cols[:m].copy(), vals[:m].copy())
-        return GraphResult(Knn(k, idx, dist), SmoothKnn(sig, rho, s64), g, indptr)
+        return GraphResult(Knn(k, idx, dist, d64), SmoothKnn(sig, rho, s64), g, indptr,
+                           _err64(o))
 
     def connected_components(self, ds: int) -> Components:
         """src/components.rs:13-39 on dataset ds's resident graph."""

@@ -69,7 +69,8 @@ class GraphOut(ctypes.Structure):
                 ("rows", ctypes.c_void_p), ("cols", ctypes.c_void_p),
                 ("vals", ctypes.c_void_p), ("indptr", ctypes.c_void_p),
                 ("sigmas64", ctypes.c_void_p), ("cap", ctypes.c_uint64),
-                ("nnz", ctypes.c_uint64)]
+                ("nnz", ctypes.c_uint64), ("dists64", ctypes.c_void_p),
+                ("err64", ctypes.c_double * 3)]
 
 
 class ServerConfig(ctypes.Structure):

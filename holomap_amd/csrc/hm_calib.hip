@@ -6,7 +6,13 @@
 // hi seeded at f32::MAX with doubling), with expf evaluated by the same
 // algorithm the reference's libm crate uses (see hm_expf). The result is
 // bit-identical to the reference arithmetic. Mode CALIB_F64 runs the same
-// bracket search in f64 with f64 exp until the bracket stops moving.
+// bracket search in f64 with f64 exp until the bracket stops moving; when
+// the kNN ran on the device it keeps the f64 distances (knn_dist64), and the
+// whole chain -- rho, the search, the floors and the directed weights
+// exp(-(d - rho) / sigma) -- is evaluated in f64 and rounded to f32 once at
+// the end, with each row's largest relative rounding error (distances,
+// sigma, weights) stated in err3. Without them it searches on the f32
+// distances widened.
 //
 // The floor for rows without a non-zero distance uses mean_all, a
 // sequential f64 mean over all n*k distances (src/smooth_knn.rs:26). That
@@ -119,6 +125,36 @@ __device__ __forceinline__ void directed_edge(const DsDesc& D, int64_t i, int64_
   srcrow[e] = (uint32_t)g;
 }
 
+// The same edge from the f64 chain: w = exp(-((d - rho) / sigma)) in f64,
+// rounded to f32 once (dropped when that is 0). Returns the relative error
+// of the f32 weight (0 for dropped or unit weights).
+__device__ __forceinline__ double directed_edge64(const DsDesc& D, int64_t i, int64_t g, int t,
+                                                  double d, double rho, double sigma,
+                                                  uint32_t sentinel, uint32_t* __restrict__ key1,
+                                                  uint32_t* __restrict__ pay1,
+                                                  float* __restrict__ wval,
+                                                  uint32_t* __restrict__ srcrow) {
+  const int64_t le = i * D.k + t;
+  const int64_t e = D.edge_off + le;
+  const uint32_t col = D.knn_idx[le];
+  const double dd = d - rho;
+  const double w = (dd <= 0.0 || sigma == 0.0) ? 1.0 : exp(-(dd / sigma));
+  const float wf = (float)w;
+  const bool keep = (int64_t)col != i && wf != 0.0f;
+  if (key1) {
+    key1[e] = keep ? (uint32_t)(D.row_off + col) : sentinel;
+    pay1[e] = (uint32_t)e;
+    wval[e] = wf;
+    srcrow[e] = (uint32_t)g;
+  }
+  return keep ? fabs((double)wf - w) / w : 0.0;
+}
+
+// Relative error of an f32 value against the f64 value it rounds (0 for 0).
+__device__ __forceinline__ double rel_err(float f, double v) {
+  return v != 0.0 ? fabs((double)f - v) / fabs(v) : 0.0;
+}
+
 // The calibration kernels also write the row's directed edges when key1 is
 // set (the symmetrisation follows): a row's sigma is final here except for
 // the global-mean floor of rows with rho == 0, whose distances are all 0, so
@@ -214,12 +250,23 @@ __global__ __launch_bounds__(256) void smooth_knn_kernel(
     if (sigma64) sigma64[g] = (double)s;
     if (key1) HM_ROW_LOOP(0, { directed_edge(D, i, g, t, X(t), r, s, sentinel, key1, pay1, wval, srcrow); });
   } else {
-    const double rr = (double)r;
+    // the f64 chain from the f64 distances when they were kept, else from
+    // the f32 distances widened
+    const double* row64 = D.knn_dist64 ? D.knn_dist64 + i * k : nullptr;
+#define X64(t) (row64 ? row64[t] : (double)X(t))
+    double rr = (double)r;
+    if (row64) {
+      rr = 0.0;
+      for (int t = 0; t < k; ++t) {
+        if (row64[t] > 0.0) { rr = row64[t]; break; }
+      }
+      rho[g] = (float)rr;
+    }
     double lo = 0.0, hi = INFINITY, mid = 1.0;
     for (int it = 0; it < 2200; ++it) {
       double psum = 0.0;
       HM_ROW_LOOP(1, {
-        const double dd = (double)X(t) - rr;
+        const double dd = X64(t) - rr;
         psum += dd > 0.0 ? exp(-(dd / mid)) : 1.0;
       });
       if (psum == target) break;
@@ -235,9 +282,9 @@ __global__ __launch_bounds__(256) void smooth_knn_kernel(
       mid = nmid;
     }
     double s = mid;
-    if (r > 0.0f) {
+    if (rr > 0.0) {
       double acc = 0.0;
-      HM_ROW_LOOP(0, { acc += (double)X(t); });
+      HM_ROW_LOOP(0, { acc += X64(t); });
       const double fl = 1e-3 * (acc / (double)k);
       if (s < fl) s = fl;
     } else {
@@ -245,8 +292,21 @@ __global__ __launch_bounds__(256) void smooth_knn_kernel(
     }
     if (sigma64) sigma64[g] = s;
     sigma[g] = (float)s;
-    const float sf = (float)s;
-    if (key1) HM_ROW_LOOP(0, { directed_edge(D, i, g, t, X(t), r, sf, sentinel, key1, pay1, wval, srcrow); });
+    if (row64) {
+      double ed = 0.0, ew = 0.0;
+      HM_ROW_LOOP(0, {
+        ed = fmax(ed, rel_err(X(t), row64[t]));
+        ew = fmax(ew, directed_edge64(D, i, g, t, row64[t], rr, s, sentinel, key1, pay1, wval,
+                                      srcrow));
+      });
+      D.err3[i * 3 + 0] = (float)ed;
+      D.err3[i * 3 + 1] = (float)rel_err((float)s, s);
+      D.err3[i * 3 + 2] = (float)ew;
+    } else {
+      const float sf = (float)s;
+      if (key1) HM_ROW_LOOP(0, { directed_edge(D, i, g, t, X(t), r, sf, sentinel, key1, pay1, wval, srcrow); });
+    }
+#undef X64
   }
 #undef X
 }
@@ -422,12 +482,23 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
     }
     if (key1 && l < k) directed_edge(D, i, g, l, x, r, s, sentinel, key1, pay1, wval, srcrow);
   } else {
-    const double rr = (double)r;
+    // the f64 chain from the f64 distances when they were kept (lane t
+    // holds distance t in f64), else from the f32 distances widened
+    const bool chain = D.knn_dist64 != nullptr;
+    const double x64 = chain ? (l < k ? D.knn_dist64[i * k + l] : 0.0) : (double)x;
+    double rr = (double)r;
+    if (chain) {
+      rr = 0.0;
+      for (int t = 0; t < k; ++t) {
+        const double v = __shfl(x64, t, G);
+        if (v > 0.0) { rr = v; break; }
+      }
+    }
     double lo = 0.0, hi = INFINITY, mid = 1.0;
     for (int it = 0; it < 2200; ++it) {
       double e = 0.0;
       if (term) {
-        const double dd = (double)x - rr;
+        const double dd = x64 - rr;
         e = dd > 0.0 ? exp(-(dd / mid)) : 1.0;
       }
       double psum = 0.0;
@@ -452,20 +523,37 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
     }
     double s = mid;
     bool z = true;
-    if (r > 0.0f) {
+    if (rr > 0.0) {
       double acc = 0.0;
-      for (int t = 0; t < k; ++t) acc += (double)__shfl(x, t, G);
+      for (int t = 0; t < k; ++t) acc += __shfl(x64, t, G);
       const double fl = 1e-3 * (acc / (double)k);
       if (s < fl) s = fl;
       z = false;
     }
     if (l == 0) {
-      rho[g] = r;
+      rho[g] = chain ? (float)rr : r;
       if (z) zany[di] = 1u;
       if (sigma64) sigma64[g] = s;
       sigma[g] = (float)s;
     }
-    if (key1 && l < k) directed_edge(D, i, g, l, x, r, (float)s, sentinel, key1, pay1, wval, srcrow);
+    if (chain) {
+      double ed = 0.0, ew = 0.0;
+      if (l < k) {
+        ed = rel_err(x, x64);
+        ew = directed_edge64(D, i, g, l, x64, rr, s, sentinel, key1, pay1, wval, srcrow);
+      }
+      for (int o = G / 2; o > 0; o >>= 1) {
+        ed = fmax(ed, __shfl_xor(ed, o, G));
+        ew = fmax(ew, __shfl_xor(ew, o, G));
+      }
+      if (l == 0) {
+        D.err3[i * 3 + 0] = (float)ed;
+        D.err3[i * 3 + 1] = (float)rel_err((float)s, s);
+        D.err3[i * 3 + 2] = (float)ew;
+      }
+    } else if (key1 && l < k) {
+      directed_edge(D, i, g, l, x, r, (float)s, sentinel, key1, pay1, wval, srcrow);
+    }
   }
 }
 
@@ -493,7 +581,10 @@ __global__ __launch_bounds__(256) void calib_fixup_kernel(const DsDesc* __restri
   if (threadIdx.x == 0) {
     const int64_t m = D.n * D.k;
     double s = 0.0;
-    for (int64_t t = 0; t < m; ++t) s = s + (double)D.knn_dist[t];
+    if (D.knn_dist64)
+      for (int64_t t = 0; t < m; ++t) s = s + D.knn_dist64[t];
+    else
+      for (int64_t t = 0; t < m; ++t) s = s + (double)D.knn_dist[t];
     mean_s = s / (double)m;
   }
   __syncthreads();
@@ -513,6 +604,7 @@ __global__ __launch_bounds__(256) void calib_fixup_kernel(const DsDesc* __restri
       if (s < fl) s = fl;
       if (sigma64) sigma64[g] = s;
       sigma[g] = (float)s;
+      if (D.err3) D.err3[i * 3 + 1] = (float)rel_err((float)s, s);
     }
   }
   __syncthreads();

@@ -31,6 +31,12 @@ struct DsDesc {
   float* thr;            // sweep pass-0 thresholds: n x nsplit
   uint32_t* knn_idx;     // n x k (output)
   float* knn_dist;       // n x k (output)
+  double* knn_dist64;    // n x k f64 distances before the f32 rounding, or null:
+                         //   kept (and the calibration runs on them) in the
+                         //   FP64 calibration mode when the kNN runs on the device
+  float* err3;           // n x 3 per-row relative error of the f32 outputs against
+                         //   their f64 values (distances, sigma, directed
+                         //   weights: the largest of the row), with knn_dist64
   int64_t n;             // rows
   int64_t n_pad;         // rows padded to the column-block size
   int64_t row_off;       // first global row id

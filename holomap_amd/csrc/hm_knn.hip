@@ -21,9 +21,11 @@ __device__ __forceinline__ double f64max0(double v) {
 }
 
 // Reference distance of query row a (uniform) to row b, over the zero-padded
-// copies (padding adds exact zeros: (0-0)^2 and 0*0 change no partial sum).
-__device__ __forceinline__ float ref_dist(const float* __restrict__ a, const float* __restrict__ b,
-                                          int d4, int metric, double rna, double rnb) {
+// copies (padding adds exact zeros: (0-0)^2 and 0*0 change no partial sum),
+// in f64 before the reference's f32 rounding.
+__device__ __forceinline__ double ref_dist64(const float* __restrict__ a,
+                                             const float* __restrict__ b, int d4, int metric,
+                                             double rna, double rnb) {
   double s = 0.0;
   if (metric == 0) {
     for (int f = 0; f < d4; f += 4) {
@@ -35,7 +37,7 @@ __device__ __forceinline__ float ref_dist(const float* __restrict__ a, const flo
       t = (double)x.z - (double)y.z; s = s + t * t;
       t = (double)x.w - (double)y.w; s = s + t * t;
     }
-    return (float)sqrt(s);
+    return sqrt(s);
   } else {
     for (int f = 0; f < d4; f += 4) {
       const float4 x = *(const float4*)(a + f);
@@ -45,8 +47,22 @@ __device__ __forceinline__ float ref_dist(const float* __restrict__ a, const flo
       s = s + (double)x.z * (double)y.z;
       s = s + (double)x.w * (double)y.w;
     }
-    return (float)f64max0(1.0 - s / (rna * rnb));
+    return f64max0(1.0 - s / (rna * rnb));
   }
+}
+__device__ __forceinline__ float ref_dist(const float* __restrict__ a, const float* __restrict__ b,
+                                          int d4, int metric, double rna, double rnb) {
+  return (float)ref_dist64(a, b, d4, metric, rna, rnb);
+}
+
+// The f64 distance of output pair (row i, column j) of dataset D, for the
+// FP64 calibration: the fold whose f32 rounding is the output distance (0
+// for the row itself, as the exact kNN sets it).
+__device__ __forceinline__ double pair_dist64(const DsDesc& D, int64_t i, uint32_t j) {
+  if ((int64_t)j == i) return 0.0;
+  const int d4 = (D.d + 3) & ~3;
+  return ref_dist64(D.xr + (size_t)i * D.kp, D.xr + (size_t)j * D.kp, d4, D.metric,
+                    D.metric == 1 ? D.rnorm[i] : 0.0, D.metric == 1 ? D.rnorm[j] : 0.0);
 }
 
 __device__ __forceinline__ uint64_t dist_key(float dist, uint32_t j) {
@@ -336,6 +352,7 @@ __global__ __launch_bounds__(256) void exact_merge_kernel(
       const uint64_t key = a[t];
       D.knn_idx[row * k + t] = (uint32_t)key;
       D.knn_dist[row * k + t] = __uint_as_float((uint32_t)(key >> 32));
+      if (D.knn_dist64) D.knn_dist64[row * k + t] = pair_dist64(D, row, (uint32_t)key);
     }
   }
 }
@@ -442,7 +459,11 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   if (metric == 1 && rna == 0.0) {
     // zero query: every cosine distance is max(NaN, 0) = 0, so the list is
     // the k lowest indices (self included) at distance 0
-    for (int t = lane; t < k; t += 64) { oidx[t] = (uint32_t)t; odst[t] = 0.0f; }
+    for (int t = lane; t < k; t += 64) {
+      oidx[t] = (uint32_t)t;
+      odst[t] = 0.0f;
+      if (D.knn_dist64) D.knn_dist64[i * k + t] = 0.0;
+    }
     if (lane == 0) flag[grow] = 0;
     return;
   }
@@ -637,6 +658,7 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     if (e < np && rk < k) {
       oidx[rk] = (uint32_t)mine;
       odst[rk] = __uint_as_float((uint32_t)(mine >> 32));
+      if (D.knn_dist64) D.knn_dist64[i * k + rk] = pair_dist64(D, i, (uint32_t)mine);
       if (rk == k - 1) kth = mine;
     }
   }

@@ -219,7 +219,8 @@ struct hm_job {
         *smax = nullptr, *statpart = nullptr, *nz = nullptr, *knn_dist = nullptr, *sigma = nullptr, *rho = nullptr,
         *wval = nullptr, *vals = nullptr;
   uint16_t *op16 = nullptr, *op16lo = nullptr;
-  double *rnorm = nullptr, *sigma64 = nullptr, *target = nullptr;
+  double *rnorm = nullptr, *sigma64 = nullptr, *target = nullptr, *dist64 = nullptr;
+  float* err3 = nullptr;  // FP64 chain: per-row rounding errors (n x 3)
   uint64_t *stat64 = nullptr;  // partial sums of hm_job_counters
   uint64_t *cand = nullptr, *ctau = nullptr, *xpart = nullptr, *cntu = nullptr,
            *indptr_g = nullptr, *total_nnz = nullptr, *indptr_l = nullptr, *sc64 = nullptr;
@@ -427,6 +428,7 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
   }
   b += E * 8;                                        // knn idx + dist
   b += R * 20 + nds * 12;                            // sigma, rho, sigma64; flags, targets
+  if ((stages & HM_STAGE_KNN) && opt.calib == HM_CALIB_F64) b += E * 8 + R * 12;  // dist64, err3
   if (stages & HM_STAGE_FUZZY) {
     // key1 pay1 srcrow wval k1s p1s tmpk tmpv; inptr + indptr_g; cntu;
     // total; rows cols vals; indptr_l; iptr_off
@@ -1114,6 +1116,11 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
   HM_A(J->sigma, R);
   HM_A(J->rho, R);
   HM_A(J->sigma64, R);
+  // the FP64 calibration runs on the kNN's f64 distances when the kNN runs here
+  if (knn && o.calib == HM_CALIB_F64) {
+    HM_A(J->dist64, E);
+    HM_A(J->err3, R * 3);
+  }
   HM_A(J->zany, nds);  // per-dataset flags, kept clear between runs by calib_fixup_kernel
   // (a fill kernel: the scratch array keeps the last call's contents)
   hipLaunchKernelGGL(fill_u32, dim3(grid_for(nds, 256)), dim3(256), 0, J->stream, J->zany,
@@ -1164,6 +1171,8 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
     }
     D.knn_idx = J->knn_idx + h.edge_off;
     D.knn_dist = J->knn_dist + h.edge_off;
+    D.knn_dist64 = J->dist64 ? J->dist64 + h.edge_off : nullptr;
+    D.err3 = J->err3 ? J->err3 + h.row_off * 3 : nullptr;
     D.n = h.n;
     D.n_pad = h.n_pad;
     D.row_off = h.row_off;
@@ -1450,6 +1459,24 @@ hm_status hm_job_counters(hm_job* J, uint64_t* c) {
   return HM_OK;
 }
 
+}  // extern "C"
+
+// The largest relative rounding error of dataset ds's f32 distances, sigmas
+// and directed membership strengths against their f64 values (the FP64
+// chain), or -1 each without that chain.
+static hm_status fp64_error(hm_job* J, uint32_t ds, double* err) {
+  err[0] = err[1] = err[2] = -1.0;
+  if (!J->err3) return HM_OK;
+  const DsHost& h = J->dsh[ds];
+  std::vector<float> e((size_t)h.n * 3);
+  HM_HIP(J->ctx, hipMemcpy(e.data(), J->err3 + h.row_off * 3, e.size() * 4, hipMemcpyDeviceToHost));
+  err[0] = err[1] = err[2] = 0.0;
+  for (size_t i = 0; i < e.size(); ++i) err[i % 3] = std::max(err[i % 3], (double)e[i]);
+  return HM_OK;
+}
+
+extern "C" {
+
 hm_status hm_job_download(hm_job* J, uint32_t ds, hm_graph_out* o) {
   if (!J || !o || ds >= (uint32_t)J->nds) return HM_EINVAL;
   hm_ctx* ctx = J->ctx;
@@ -1461,6 +1488,13 @@ hm_status hm_job_download(hm_job* J, uint32_t ds, hm_graph_out* o) {
   if (o->sigmas) HM_HIP(ctx, hipMemcpy(o->sigmas, J->sigma + h.row_off, h.n * 4, hipMemcpyDeviceToHost));
   if (o->rhos) HM_HIP(ctx, hipMemcpy(o->rhos, J->rho + h.row_off, h.n * 4, hipMemcpyDeviceToHost));
   if (o->sigmas64) HM_HIP(ctx, hipMemcpy(o->sigmas64, J->sigma64 + h.row_off, h.n * 8, hipMemcpyDeviceToHost));
+  if (o->dists64) {
+    if (!J->dist64)
+      return fail(ctx, HM_EINVAL, "no f64 distances: they are kept in the FP64 calibration mode "
+                                  "with the kNN stage");
+    HM_HIP(ctx, hipMemcpy(o->dists64, J->dist64 + h.edge_off, m * 8, hipMemcpyDeviceToHost));
+  }
+  HM_TRY(fp64_error(J, ds, o->err64));
   o->nnz = 0;
   if (J->stages & HM_STAGE_FUZZY) {
     uint64_t se[2];
@@ -1686,6 +1720,11 @@ hm_status hm_knn_graph_noalloc(hm_ctx* ctx, const float* data, uint64_t n, uint3
   copy(out->sigmas, J->sigma, n * 4);
   copy(out->rhos, J->rho, n * 4);
   copy(out->sigmas64, J->sigma64, n * 8);
+  if (s == HM_OK && out->dists64 && !J->dist64)
+    s = fail(ctx, HM_EINVAL, "no f64 distances: they are kept in the FP64 calibration mode");
+  copy(out->dists64, J->dist64, n * k * 8);
+  // (the error statement needs a host reduction: not made here)
+  out->err64[0] = out->err64[1] = out->err64[2] = -1.0;
   copy(out->rows, J->rows, nnz * 4);
   copy(out->cols, J->cols, nnz * 4);
   copy(out->vals, J->vals, nnz * 4);

@@ -160,6 +160,13 @@ typedef struct {
   double* sigmas64;  /* n or NULL: FP64 calibration values (HM_CALIB_F64) */
   uint64_t cap;      /* capacity of rows/cols/vals */
   uint64_t nnz;      /* out */
+  double* dists64;   /* n*k or NULL: the f64 distances the f32 ones round
+                        (HM_CALIB_F64 with the kNN on the device) */
+  double err64[3];   /* out: with HM_CALIB_F64 and the kNN on the device,
+                        the largest relative error of the returned f32
+                        distances, sigmas and directed membership strengths
+                        against their f64 values (the FP64 chain); -1 each
+                        otherwise (and from hm_knn_graph_noalloc) */
 } hm_graph_out;
 
 /* src/api.rs:147-149 fused: exact_knn -> smooth_knn -> fuzzy_simplicial_set. */

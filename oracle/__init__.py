@@ -58,6 +58,12 @@ def lib():
         L.hmo_smooth_knn.restype = None
         L.hmo_smooth_knn_f64.argtypes = [_f32p, _sz, _sz, _f64p, _f64p]
         L.hmo_smooth_knn_f64.restype = None
+        L.hmo_smooth_knn_f64d.argtypes = [_f64p, _sz, _sz, _f64p, _f64p]
+        L.hmo_smooth_knn_f64d.restype = None
+        L.hmo_pair_dist64.argtypes = [_f32p, _sz, _sz, _sz, ctypes.c_int, _u32p, _f64p]
+        L.hmo_pair_dist64.restype = None
+        L.hmo_fuzzy_f64.argtypes = [_u32p, _f64p, _f64p, _f64p, _sz, _sz, _u32p, _u32p, _f32p]
+        L.hmo_fuzzy_f64.restype = _sz
         L.hmo_fuzzy.argtypes = [_u32p, _f32p, _f32p, _f32p, _sz, _sz, _u32p, _u32p, _f32p]
         L.hmo_fuzzy.restype = _sz
         L.hmo_csr_indptr.argtypes = [_u32p, _sz, _sz, _u64p]
@@ -144,13 +150,44 @@ def smooth_knn(knn_dists, k: int) -> SmoothKnn:
 
 
 def smooth_knn_f64(knn_dists, k: int) -> SmoothKnn:
-    """FP64 bisection run to convergence (oracle for the FP64 GPU mode)."""
-    d = np.ascontiguousarray(knn_dists, np.float32).reshape(-1)
-    n = d.size // k
+    """FP64 bisection run to convergence (oracle for the FP64 GPU mode), on
+    f64 distances when given float64, else on the f32 distances widened."""
+    n = np.asarray(knn_dists).size // k
     s = np.empty(n, np.float64)
     r = np.empty(n, np.float64)
-    lib().hmo_smooth_knn_f64(d, n, k, s, r)
+    if np.asarray(knn_dists).dtype == np.float64:
+        lib().hmo_smooth_knn_f64d(np.ascontiguousarray(knn_dists, np.float64).reshape(-1), n, k,
+                                  s, r)
+    else:
+        lib().hmo_smooth_knn_f64(np.ascontiguousarray(knn_dists, np.float32).reshape(-1), n, k,
+                                 s, r)
     return SmoothKnn(s, r)
+
+
+def pair_dist64(data, n_features: int, indices, metric="euclidean") -> np.ndarray:
+    """The f64 distances (src/metric.rs before the f32 rounding) of row i to
+    indices[i, t]; the row itself at 0."""
+    x = np.ascontiguousarray(data, np.float32).reshape(-1)
+    idx = np.ascontiguousarray(indices, np.uint32)
+    n, k = idx.shape
+    out = np.empty((n, k), np.float64)
+    lib().hmo_pair_dist64(x, n, n_features, k, _metric_code(metric), idx.reshape(-1),
+                          out.reshape(-1))
+    return out
+
+
+def fuzzy_simplicial_set_f64(indices, dists64, sigmas64, rhos64) -> FuzzyGraph:
+    """The FP64 chain's graph: f64 strengths rounded once, the f32 union."""
+    idx = np.ascontiguousarray(indices, np.uint32)
+    n, k = idx.shape
+    cap = 2 * n * k
+    rows = np.empty(cap, np.uint32)
+    cols = np.empty(cap, np.uint32)
+    vals = np.empty(cap, np.float32)
+    nnz = lib().hmo_fuzzy_f64(idx.reshape(-1), np.ascontiguousarray(dists64, np.float64).reshape(-1),
+                              np.ascontiguousarray(sigmas64, np.float64),
+                              np.ascontiguousarray(rhos64, np.float64), n, k, rows, cols, vals)
+    return FuzzyGraph(n, rows[:nnz].copy(), cols[:nnz].copy(), vals[:nnz].copy())
 
 
 @dataclass

@@ -19,19 +19,22 @@ static inline float u2f(uint32_t u) { float x; memcpy(&x, &u, 4); return x; }
 
 /* src/metric.rs:42-53: f64 difference, squared, summed left to right from
  * the first feature; sqrt in f64 (correctly rounded); cast to f32 (RNE). */
-float hmo_euclidean(const float* a, const float* b, size_t d) {
+static double euclidean64(const float* a, const float* b, size_t d) {
     double sq = -0.0; /* Rust's f64 Sum starts at -0.0; terms are >= +0 */
     for (size_t f = 0; f < d; ++f) {
         double t = (double)a[f] - (double)b[f];
         sq = sq + t * t;
     }
-    return (float)sqrt(sq);
+    return sqrt(sq);
+}
+float hmo_euclidean(const float* a, const float* b, size_t d) {
+    return (float)euclidean64(a, b, d);
 }
 
 /* src/metric.rs:58-70: three independent sequential f64 folds, then
  * (1 - dot/(sqrt(na)*sqrt(nb))).max(0.0) -- f64::max returns the non-NaN
  * operand, so a zero vector (0/0 = NaN) yields distance 0. */
-float hmo_cosine(const float* a, const float* b, size_t d) {
+static double cosine64(const float* a, const float* b, size_t d) {
     double dot = 0.0, na = 0.0, nb = 0.0;
     for (size_t f = 0; f < d; ++f) {
         double x = (double)a[f], y = (double)b[f];
@@ -40,8 +43,24 @@ float hmo_cosine(const float* a, const float* b, size_t d) {
         nb = nb + y * y;
     }
     double v = 1.0 - dot / (sqrt(na) * sqrt(nb));
-    double r = (v != v) ? 0.0 : (v > 0.0 ? v : 0.0); /* f64::max(v, 0.0) */
-    return (float)r;
+    return (v != v) ? 0.0 : (v > 0.0 ? v : 0.0); /* f64::max(v, 0.0) */
+}
+float hmo_cosine(const float* a, const float* b, size_t d) {
+    return (float)cosine64(a, b, d);
+}
+
+/* The f64 values the kNN distances round (src/metric.rs before the `as
+ * f32`), for given pairs: out[i*k+t] for row i and column idx[i*k+t]; the
+ * row itself at 0, as knn.rs:38-40 sets it. */
+void hmo_pair_dist64(const float* data, size_t n, size_t d, size_t k, int metric,
+                     const uint32_t* idx, double* out) {
+    for (size_t i = 0; i < n; ++i)
+        for (size_t t = 0; t < k; ++t) {
+            const size_t j = idx[i * k + t];
+            out[i * k + t] = j == i ? 0.0
+                           : metric == 0 ? euclidean64(data + i * d, data + j * d, d)
+                                         : cosine64(data + i * d, data + j * d, d);
+        }
 }
 
 /* f32::total_cmp key: order-preserving map of the bit pattern to int32. */
@@ -239,24 +258,25 @@ void hmo_smooth_knn(const float* knn_dists, size_t n, size_t k, float* sigmas,
 /* FP64 oracle: the same bracket search with double state and double exp,
  * iterated until the bracket stops moving (convergence), so sigma is the
  * root of sum_j exp(-(d_j - rho)/sigma) = log2(k) to double precision.
- * Floors as src/smooth_knn.rs:70-77, evaluated in double. */
-void hmo_smooth_knn_f64(const float* knn_dists, size_t n, size_t k, double* sigmas,
-                        double* rhos) {
+ * Floors as src/smooth_knn.rs:70-77, evaluated in double. On f64
+ * distances (hmo_smooth_knn_f64d) or the f32 ones widened. */
+void hmo_smooth_knn_f64d(const double* knn_dists, size_t n, size_t k, double* sigmas,
+                         double* rhos) {
     const double target = log2((double)k);
     double mean_all = 0.0;
-    for (size_t t = 0; t < n * k; ++t) mean_all += (double)knn_dists[t];
+    for (size_t t = 0; t < n * k; ++t) mean_all += knn_dists[t];
     mean_all /= (double)(n * k);
     for (size_t i = 0; i < n; ++i) {
-        const float* row = knn_dists + i * k;
+        const double* row = knn_dists + i * k;
         double rho = 0.0;
         for (size_t t = 0; t < k; ++t)
-            if (row[t] > 0.0f) { rho = (double)row[t]; break; }
+            if (row[t] > 0.0) { rho = row[t]; break; }
         rhos[i] = rho;
         double lo = 0.0, hi = INFINITY, mid = 1.0;
         for (int it = 0; it < 2200; ++it) {
             double psum = 0.0;
             for (size_t t = 1; t < k; ++t) {
-                double dd = (double)row[t] - rho;
+                double dd = row[t] - rho;
                 psum += dd > 0.0 ? exp(-(dd / mid)) : 1.0;
             }
             if (psum == target) break;
@@ -272,11 +292,19 @@ void hmo_smooth_knn_f64(const float* knn_dists, size_t n, size_t k, double* sigm
             mid = nmid;
         }
         double rmean = 0.0;
-        for (size_t t = 0; t < k; ++t) rmean += (double)row[t];
+        for (size_t t = 0; t < k; ++t) rmean += row[t];
         rmean /= (double)k;
         double floor_ = rho > 0.0 ? 1e-3 * rmean : 1e-3 * mean_all;
         sigmas[i] = mid < floor_ ? floor_ : mid;
     }
+}
+
+void hmo_smooth_knn_f64(const float* knn_dists, size_t n, size_t k, double* sigmas,
+                        double* rhos) {
+    double* w = (double*)malloc(n * k * sizeof(double) + 1);
+    for (size_t t = 0; t < n * k; ++t) w[t] = (double)knn_dists[t];
+    hmo_smooth_knn_f64d(w, n, k, sigmas, rhos);
+    free(w);
 }
 
 typedef struct { uint32_t r, c; float v; } trip_t;
@@ -308,27 +336,9 @@ static float lookup(const trip_t* dir, size_t m, uint32_t r, uint32_t c) {
     return 0.0f;
 }
 
-/* src/fuzzy.rs:25-85 */
-size_t hmo_fuzzy(const uint32_t* indices, const float* dists, const float* sigmas,
-                 const float* rhos, size_t n, size_t k, uint32_t* rows,
-                 uint32_t* cols, float* vals) {
-    trip_t* dir = (trip_t*)malloc(n * k * sizeof(trip_t) + 1);
-    size_t m = 0;
-    for (size_t i = 0; i < n; ++i) {
-        for (size_t j = 0; j < k; ++j) {
-            uint32_t col = indices[i * k + j];
-            if ((size_t)col == i) continue;
-            float dd = dists[i * k + j] - rhos[i];
-            float val = (dd <= 0.0f || sigmas[i] == 0.0f) ? 1.0f
-                                                         : hmo_expf(-(dd / sigmas[i]));
-            if (val != 0.0f) {
-                dir[m].r = (uint32_t)i;
-                dir[m].c = col;
-                dir[m].v = val;
-                ++m;
-            }
-        }
-    }
+/* src/fuzzy.rs:50-84: the sorted directed triples' union with their
+ * transpose, w = (v + vt) - v * vt in f32. */
+static size_t fuzzy_union(trip_t* dir, size_t m, uint32_t* rows, uint32_t* cols, float* vals) {
     qsort(dir, m, sizeof(trip_t), trip_cmp);
     coord_t* co = (coord_t*)malloc(2 * m * sizeof(coord_t) + 1);
     for (size_t t = 0; t < m; ++t) {
@@ -352,6 +362,59 @@ size_t hmo_fuzzy(const uint32_t* indices, const float* dists, const float* sigma
         }
     }
     free(co);
+    return nnz;
+}
+
+/* src/fuzzy.rs:25-85 */
+size_t hmo_fuzzy(const uint32_t* indices, const float* dists, const float* sigmas,
+                 const float* rhos, size_t n, size_t k, uint32_t* rows,
+                 uint32_t* cols, float* vals) {
+    trip_t* dir = (trip_t*)malloc(n * k * sizeof(trip_t) + 1);
+    size_t m = 0;
+    for (size_t i = 0; i < n; ++i) {
+        for (size_t j = 0; j < k; ++j) {
+            uint32_t col = indices[i * k + j];
+            if ((size_t)col == i) continue;
+            float dd = dists[i * k + j] - rhos[i];
+            float val = (dd <= 0.0f || sigmas[i] == 0.0f) ? 1.0f
+                                                         : hmo_expf(-(dd / sigmas[i]));
+            if (val != 0.0f) {
+                dir[m].r = (uint32_t)i;
+                dir[m].c = col;
+                dir[m].v = val;
+                ++m;
+            }
+        }
+    }
+    const size_t nnz = fuzzy_union(dir, m, rows, cols, vals);
+    free(dir);
+    return nnz;
+}
+
+/* The FP64 chain's graph: directed strengths exp(-((d - rho) / sigma)) in
+ * f64 from f64 distances, sigmas and rhos (1 where d - rho <= 0 or sigma ==
+ * 0), rounded to f32 once (dropped when that is 0), then the same f32
+ * union as src/fuzzy.rs:50-84. */
+size_t hmo_fuzzy_f64(const uint32_t* indices, const double* dists, const double* sigmas,
+                     const double* rhos, size_t n, size_t k, uint32_t* rows,
+                     uint32_t* cols, float* vals) {
+    trip_t* dir = (trip_t*)malloc(n * k * sizeof(trip_t) + 1);
+    size_t m = 0;
+    for (size_t i = 0; i < n; ++i) {
+        for (size_t j = 0; j < k; ++j) {
+            uint32_t col = indices[i * k + j];
+            if ((size_t)col == i) continue;
+            double dd = dists[i * k + j] - rhos[i];
+            float val = (float)((dd <= 0.0 || sigmas[i] == 0.0) ? 1.0 : exp(-(dd / sigmas[i])));
+            if (val != 0.0f) {
+                dir[m].r = (uint32_t)i;
+                dir[m].c = col;
+                dir[m].v = val;
+                ++m;
+            }
+        }
+    }
+    const size_t nnz = fuzzy_union(dir, m, rows, cols, vals);
     free(dir);
     return nnz;
 }

@@ -24,8 +24,11 @@
  *   hmo_laplacian_matvec src/sparse.rs:87-93
  *   hmo_schedule_edges   src/sgd.rs:23-63
  *   hmo_l2_normalise     crates/holomap-clusterer/src/pipeline.rs:151-162
- * hmo_smooth_knn_f64 is an FP64 bisection run to convergence: the FP64
- * oracle for the GPU's FP64 calibration mode (no reference counterpart).
+ * hmo_smooth_knn_f64 / _f64d is an FP64 bisection run to convergence, with
+ * hmo_pair_dist64 (the f64 values src/metric.rs rounds to f32) and
+ * hmo_fuzzy_f64 (strengths in f64, rounded once, then the f32 union): the
+ * FP64 chain, the oracle for the GPU's FP64 calibration mode (no reference
+ * counterpart).
  */
 #ifndef HM_ORACLE_H
 #define HM_ORACLE_H
@@ -52,10 +55,19 @@ void hmo_smooth_knn(const float* knn_dists, size_t n, size_t k, float* sigmas,
                     float* rhos);
 void hmo_smooth_knn_f64(const float* knn_dists, size_t n, size_t k, double* sigmas,
                         double* rhos);
+void hmo_smooth_knn_f64d(const double* knn_dists, size_t n, size_t k, double* sigmas,
+                         double* rhos);
+/* f64 distances of pairs (row i, column idx[i*k+t]); the row itself at 0. */
+void hmo_pair_dist64(const float* data, size_t n, size_t d, size_t k, int metric,
+                     const uint32_t* idx, double* out);
 /* rows/cols/vals capacity must be >= 2*n*k. Returns nnz. */
 size_t hmo_fuzzy(const uint32_t* indices, const float* dists, const float* sigmas,
                  const float* rhos, size_t n, size_t k, uint32_t* rows,
                  uint32_t* cols, float* vals);
+/* The FP64 chain's graph (capacity >= 2*n*k). Returns nnz. */
+size_t hmo_fuzzy_f64(const uint32_t* indices, const double* dists, const double* sigmas,
+                     const double* rhos, size_t n, size_t k, uint32_t* rows,
+                     uint32_t* cols, float* vals);
 void hmo_csr_indptr(const uint32_t* rows, size_t nnz, size_t n, uint64_t* indptr);
 /* src/components.rs:13-39: returns the component count. */
 size_t hmo_connected_components(const uint64_t* indptr, const uint32_t* indices, size_t n,

@@ -11,6 +11,7 @@ import pytest
 import holomap_amd as H
 import oracle as O
 from holomap_amd import datasets as DS
+from fp64_chain import check_fp64_chain
 
 pytestmark = pytest.mark.gpu
 
@@ -234,20 +235,38 @@ def test_graph_replay(ctx):
 
 # k = 15, 30, 50: lane groups of 16 and 32 (DPP folds) and 64 (permutes)
 @pytest.mark.parametrize("k", [15, 30, 50])
-def test_f64_calibration_vs_fp64_oracle(ctx, k):
+@pytest.mark.parametrize("metric", ["euclidean", "cosine"])
+def test_f64_calibration_vs_fp64_oracle(ctx, k, metric):
     x = DS.c1()
-    job = H.Job(ctx, [1000], [50], [k], ["euclidean"], calib="f64")
+    job = H.Job(ctx, [1000], [50], [k], [metric], calib="f64")
     job.upload(0, x)
     job.run()
     job.sync()
     r = job.download(0)
     job.close()
-    c64 = O.smooth_knn_f64(r.knn.dists, k)
-    rel = np.abs(r.calib.sigmas64 - c64.sigmas) / c64.sigmas
-    assert rel.max() < 1e-12
+    check_fp64_chain(r, x, 50, k, metric)
+    # the same through the fused host entry
+    f = H.knn_graph(x, 50, k, metric, calib="f64", ctx=ctx)
+    assert f.knn.dists64.tobytes() == r.knn.dists64.tobytes()
+    assert f.graph.vals.tobytes() == r.graph.vals.tobytes() and f.err64 == r.err64
+    # the reference's f32 search stays within 1e-3 of the FP64 root
     ref32 = O.smooth_knn(r.knn.dists, k)
-    # distance of the reference's f32 search from the FP64 root
-    assert (np.abs(ref32.sigmas - c64.sigmas) / c64.sigmas).max() < 1e-3
+    assert (np.abs(ref32.sigmas - r.calib.sigmas64) / r.calib.sigmas64).max() < 1e-3
+    # without the FP64 mode there is no statement and no f64 distances
+    assert H.knn_graph(x, 50, k, metric, ctx=ctx).err64 is None
+
+
+def test_f64_chain_on_escalated_and_fallback_rows(ctx):
+    # rows the bf16 keys cannot certify take the bf16x3 re-sweep and the
+    # exact scan: their f64 distances come from the re-rank of the re-sweep
+    # and from the exact merge
+    rng = np.random.default_rng(44)
+    n, d, k = 2500, 40, 15
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    x[:300] = x[0] + rng.normal(size=(300, d)).astype(np.float32) * 1e-4  # a tight cluster
+    for prec in ["bf16", "exact"]:
+        r = H.knn_graph(x, d, k, "euclidean", precision=prec, calib="f64", ctx=ctx)
+        check_fp64_chain(r, x, d, k, "euclidean")
 
 
 @pytest.mark.parametrize("metric", ["euclidean", "cosine"])

@@ -20,6 +20,7 @@ import pytest
 import holomap_amd as H
 import oracle as O
 from holomap_amd import datasets as DS
+from fp64_chain import check_fp64_chain
 
 pytestmark = pytest.mark.gpu
 
@@ -85,16 +86,14 @@ def test_c3_bf16_vs_oracle_and_modes(ctx, c3_data):
 
 
 def test_c3_f64_calibration_vs_fp64_oracle(ctx, c3_data):
+    # the configuration: margin 2k, FP64 smooth-kNN calibration
     x = c3_data
     k = 15
-    r, _ = run_job(ctx, x, 784, k, "euclidean", calib="f64")
-    c64 = O.smooth_knn_f64(r.knn.dists, k)
-    assert np.array_equal(r.calib.rhos, c64.rhos.astype(np.float32))
-    rel = np.abs(r.calib.sigmas64 - c64.sigmas) / c64.sigmas
-    assert rel.max() < 1e-12, rel.max()
+    r, _ = run_job(ctx, x, 784, k, "euclidean", calib="f64", margin=2 * k)
+    check_fp64_chain(r, x, 784, k, "euclidean")
     # the FP64 root's distance from the reference's f32 search
     ref32 = O.smooth_knn(r.knn.dists, k)
-    assert (np.abs(ref32.sigmas - c64.sigmas) / c64.sigmas).max() < 1e-3
+    assert (np.abs(ref32.sigmas - r.calib.sigmas64) / r.calib.sigmas64).max() < 1e-3
 
 
 def test_c4_full_batch(ctx):

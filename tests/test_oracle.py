@@ -393,3 +393,97 @@ def test_l2_normalise_against_numpy_sequential():
     assert got[6, 3] == -1.0
     assert O.l2_normalise(np.array([[3.0, 4.0]], np.float32)).tolist() == [[0.6000000238418579,
                                                                             0.800000011920929]]
+
+
+# ---- the FP64 chain (oracle of the GPU's FP64 calibration mode) ----
+def np_euclid64(a, b):
+    s = 0.0
+    for x, y in zip(a.astype(np.float64), b.astype(np.float64)):
+        t = x - y
+        s = s + t * t
+    return np.sqrt(s)
+
+
+def np_cosine64(a, b):
+    dot = na = nb = 0.0
+    for x, y in zip(a.astype(np.float64), b.astype(np.float64)):
+        dot += x * y
+        na += x * x
+        nb += y * y
+    with np.errstate(invalid="ignore", divide="ignore"):
+        v = 1.0 - dot / (np.sqrt(na) * np.sqrt(nb))
+    return 0.0 if np.isnan(v) else max(v, 0.0)
+
+
+@pytest.mark.parametrize("metric", ["euclidean", "cosine"])
+def test_pair_dist64_is_the_unrounded_fold(metric):
+    rng = np.random.default_rng(15)
+    n, d, k = 40, 9, 7
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    x[5] = x[2]
+    x[9] = 0.0
+    knn = O.exact_knn(x, d, k, metric)
+    d64 = O.pair_dist64(x, d, knn.indices, metric)
+    # rounds to the kNN's f32 distances bit for bit
+    assert np.array_equal(d64.astype(np.float32).view(np.uint32), knn.dists.view(np.uint32))
+    # and is the sequential f64 fold itself (the row itself at 0)
+    f = np_euclid64 if metric == "euclidean" else np_cosine64
+    for i in range(n):
+        for t in range(k):
+            j = int(knn.indices[i, t])
+            assert d64[i, t] == (0.0 if j == i else f(x[i], x[j]))
+
+
+def test_smooth_knn_f64d_on_f64_distances():
+    rng = np.random.default_rng(16)
+    x = rng.normal(size=(300, 5)).astype(np.float32)
+    x[40:45] = x[39]  # rows with zero distances
+    k = 12
+    knn = O.exact_knn(x, 5, k, "euclidean")
+    d64 = O.pair_dist64(x, 5, knn.indices)
+    c = O.smooth_knn_f64(d64, k)
+    # on f32-representable inputs it is the widening variant bit for bit
+    w = O.smooth_knn_f64(knn.dists.astype(np.float64), k)
+    assert np.array_equal(w.sigmas, O.smooth_knn_f64(knn.dists, k).sigmas)
+    # rho: the first positive f64 distance; sigma solves the equation in f64
+    for i in range(300):
+        pos = d64[i][d64[i] > 0]
+        assert c.rhos[i] == (pos[0] if pos.size else 0.0)
+        dd = d64[i, 1:] - c.rhos[i]
+        with np.errstate(over="ignore"):
+            ps = np.where(dd > 0, np.exp(-dd / c.sigmas[i]), 1.0).sum()
+        floor = 1e-3 * d64[i].mean()
+        assert abs(ps - np.log2(k)) < 1e-9 or c.sigmas[i] == pytest.approx(floor, rel=1e-12)
+    # a mistake planted in the distances moves sigma
+    bad = d64.copy()
+    bad[:, 2] *= 1.001
+    assert not np.array_equal(O.smooth_knn_f64(bad, k).sigmas, c.sigmas)
+
+
+def test_fuzzy_f64_strengths_rounded_once():
+    rng = np.random.default_rng(17)
+    x = rng.normal(size=(200, 6)).astype(np.float32)
+    k = 8
+    knn = O.exact_knn(x, 6, k, "euclidean")
+    d64 = O.pair_dist64(x, 6, knn.indices)
+    c = O.smooth_knn_f64(d64, k)
+    g = O.fuzzy_simplicial_set_f64(knn.indices, d64, c.sigmas, c.rhos)
+    # numpy restatement: f64 strengths, rounded to f32, f32 t-conorm
+    dd = d64 - c.rhos[:, None]
+    w = np.where((dd <= 0) | (c.sigmas[:, None] == 0), 1.0, np.exp(-(dd / c.sigmas[:, None])))
+    w32 = w.astype(np.float32)
+    A = {}
+    for i in range(200):
+        for t in range(k):
+            j = int(knn.indices[i, t])
+            if j != i and w32[i, t] != 0:
+                A[(i, j)] = w32[i, t]
+    keys = sorted(set(A) | {(c_, r) for (r, c_) in A})
+    exp_rows = np.array([r for r, _ in keys], np.uint32)
+    exp_cols = np.array([c_ for _, c_ in keys], np.uint32)
+    a = np.array([A.get(p, np.float32(0)) for p in keys], np.float32)
+    b = np.array([A.get((p[1], p[0]), np.float32(0)) for p in keys], np.float32)
+    exp_vals = (a + b) - a * b
+    assert np.array_equal(g.rows, exp_rows) and np.array_equal(g.cols, exp_cols)
+    # numpy's exp and libm's may differ by an ulp of f64: at most one f32 step apart
+    assert np.max(np.abs(g.vals - exp_vals) / exp_vals) < 2.4e-7

@@ -84,6 +84,10 @@ def struct_fields(text, name, rust_side):
         if not line:
             continue
         t, f = line.rsplit(" ", 1)
+        arr = re.fullmatch(r"(\w+)\[(\d+)\]", f)
+        if arr:  # a fixed-size array field
+            out.append((arr.group(1), f"[{c_type_to_rust(t)}; {arr.group(2)}]"))
+            continue
         out.append((f.lstrip("*"), c_type_to_rust(t + "*" * f.count("*"))))
     return out
 
