@@ -51,12 +51,18 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+# a fixed sweep configuration instead of the autotuner's (--nsplit), for
+# profiles whose every dispatch has the same configuration
+FIXED = {"nsplit": 0}
+
+
 def run_path(ctx, x, d, k, metric, precision, steps, warmup, margin=-1, calib="ref_f32"):
     """margin >= 0 pins the candidate count C = k + margin (the autotuner then
     only picks column splits and item order); -1 lets it also try k, 2k, 3k."""
     import holomap_amd as H
     n = x.shape[0]
-    kw = dict(precision=precision, margin=margin, calib=calib, autotune=True)
+    kw = dict(precision=precision, margin=margin, calib=calib, autotune=FIXED["nsplit"] == 0,
+              nsplit=FIXED["nsplit"])
     # per-stage device times: HIP events on the job's stream around each stage
     # of eager runs of an identically configured (autotuned) job; run first,
     # so the timed job's kernels are the last ones of the path
@@ -421,6 +427,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--precision", default="bf16")
+    ap.add_argument("--nsplit", type=int, default=0,
+                    help="column splits of the sweep, fixed (no autotuning); 0: autotuned")
+    ap.add_argument("--calib", default="ref_f32", choices=["ref_f32", "f64"],
+                    help="calibration mode of the timed path")
+    ap.add_argument("--margin", type=int, default=-1,
+                    help="candidates beyond k per row (C = k + margin); -1: 2k, autotuned")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-target", action="store_true",
@@ -435,6 +447,7 @@ def main():
                     help="only run a plain elementwise copy of MIB MiB (reads and writes that "
                          "many bytes once) to calibrate the HBM byte counters, then exit")
     args = ap.parse_args()
+    FIXED["nsplit"] = args.nsplit
     if args.copy_probe:
         import torch
         a = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
@@ -465,7 +478,8 @@ def main():
         H.knn_graph(x, d, k, metric, precision=args.precision, ctx=ctx)
     t_host = (time.perf_counter() - t0) / reps
     log(f"{args.workload}: timed path")
-    dt, stage, cnt, res = run_path(ctx, x, d, k, metric, args.precision, args.steps, args.warmup)
+    dt, stage, cnt, res = run_path(ctx, x, d, k, metric, args.precision, args.steps, args.warmup,
+                                   margin=args.margin, calib=args.calib)
     out = {
         "metric": METRIC,
         "value": n / dt,
@@ -480,7 +494,8 @@ def main():
         "dtype": args.precision,
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {desc}", "N": n, "d": d, "k": k,
-                   "metric": metric, "precision": args.precision, "autotune": True,
+                   "metric": metric, "precision": args.precision, "calib": args.calib,
+                   "autotune": args.nsplit == 0,
                    "parallelism": "single GPU", **sweep_config(cnt, n, k)},
         "stage_ms": {kk: round(v, 4) for kk, v in stage.items()},
         "fallback_rows": cnt["fallback_rows"],

@@ -1290,7 +1290,7 @@ hm_status hm_job_set_calib(hm_job* J, uint32_t ds, const float* sig, const float
 
 // Autotuner (hm_options.autotune): times the whole pipeline for candidate
 // sweep configurations -- split count x XCD item permutation at the planned
-// margin, then margins k, 2k, 3k at the best of those -- and keeps the
+// margin, then margins 0, k, 3k at the best of those -- and keeps the
 // fastest. The outputs do not depend on the configuration (it only decides
 // which columns are examined), so tuning changes time only.
 hm_status autotune(hm_job* J) {
@@ -1343,7 +1343,7 @@ hm_status autotune(hm_job* J) {
   if (tried.empty()) return fail(ctx, HM_EINVAL, "autotune: no sweep configuration to time");
   if (J->opt.margin < 0) {
     const Cfg b = tried[best_of()];
-    for (int mult : {1, 3}) HM_TRY(try_cfg(mult, b.nsplit, b.perm));
+    for (int mult : {0, 1, 3}) HM_TRY(try_cfg(mult, b.nsplit, b.perm));
   }
   const Cfg& b = tried[best_of()];
   J->dsh = b.dsh;
