@@ -378,19 +378,21 @@ __device__ __forceinline__ float f32_round_up(double v) {
 #define RR_LISTS 10   // lists per row at most (two per column split)
 #define RR_QROW 512   // query rows of at most this many floats are staged in LDS
 
-// ref_dist with the query row in LDS (read at the same address by every
-// lane) and the candidate row fetched 32 floats at a time, so each batch
-// costs one memory round trip; same operations in the same order.
-__device__ __forceinline__ float rr_dist(const float* qs, const float* __restrict__ b, int d4,
+// ref_dist64 with the query row in LDS (read at the same address by every
+// lane) and the candidate row fetched 16 floats at a time, so each batch
+// costs one memory round trip; same operations in the same order. (32
+// floats per batch need more registers than six waves per SIMD leave and
+// spill: measured slower at N = 20,000 x d = 128 and N = 50,000 x d = 784.)
+__device__ __forceinline__ double rr_dist64(const float* qs, const float* __restrict__ b, int d4,
                                          int metric, double rna, double rnb) {
   double s = 0.0;
   int f = 0;
-  for (; f + 32 <= d4; f += 32) {
-    float4 y[8];
+  for (; f + 16 <= d4; f += 16) {
+    float4 y[4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) y[u] = *(const float4*)(b + f + 4 * u);
+    for (int u = 0; u < 4; ++u) y[u] = *(const float4*)(b + f + 4 * u);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < 4; ++u) {
       const float4 x = *(const float4*)(qs + f + 4 * u);
       if (metric == 0) {
         double t;
@@ -422,8 +424,8 @@ __device__ __forceinline__ float rr_dist(const float* qs, const float* __restric
       s = s + (double)x.w * (double)y.w;
     }
   }
-  if (metric == 0) return (float)sqrt(s);
-  return (float)f64max0(1.0 - s / (rna * rnb));
+  if (metric == 0) return sqrt(s);
+  return f64max0(1.0 - s / (rna * rnb));
 }
 
 // One row (global id grow) by one wave; lds: the wave's 6 KB LDS area.
@@ -627,19 +629,22 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     for (int f = 4 * lane; f < d4; f += 256) *(float4*)(qs + f) = *(const float4*)(qa + f);
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
-  // ---- 4. exact distances, one lane per candidate ----
+  // ---- 4. exact distances, one lane per candidate (the f64 value of the
+  // lane's first candidate is kept for the FP64 calibration) ----
   bool badid = false;
+  double first64 = 0.0;
   for (int e = lane; e < np; e += 64) {
     const uint32_t j = cl[e];
-    float dist = 0.0f;
+    double dist = 0.0;
     if ((int64_t)j >= D.n) {
       badid = true;  // never produced by the sweep; guard the row read
     } else if ((int64_t)j != i) {
       const double rnb = metric == 1 ? D.rnorm[j] : 0.0;
-      dist = qlds ? rr_dist(qs, D.xr + (size_t)j * kp, d4, metric, rna, rnb)
-                  : ref_dist(qa, D.xr + (size_t)j * kp, d4, metric, rna, rnb);
+      dist = qlds ? rr_dist64(qs, D.xr + (size_t)j * kp, d4, metric, rna, rnb)
+                  : ref_dist64(qa, D.xr + (size_t)j * kp, d4, metric, rna, rnb);
     }
-    xd[e] = __float_as_uint(dist);
+    if (e == lane) first64 = dist;
+    xd[e] = __float_as_uint((float)dist);
   }
   if (__ballot(badid) != 0ull) {
     if (lane == 0) flag[grow] = 1;  // recomputed by the exact scan
@@ -658,7 +663,8 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     if (e < np && rk < k) {
       oidx[rk] = (uint32_t)mine;
       odst[rk] = __uint_as_float((uint32_t)(mine >> 32));
-      if (D.knn_dist64) D.knn_dist64[i * k + rk] = pair_dist64(D, i, (uint32_t)mine);
+      if (D.knn_dist64)
+        D.knn_dist64[i * k + rk] = m == 0 ? first64 : pair_dist64(D, i, (uint32_t)mine);
       if (rk == k - 1) kth = mine;
     }
   }
