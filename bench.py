@@ -285,14 +285,16 @@ def fp64_error(x, k, res):
 
 
 def sweep_traffic(workload_name, precision):
-    """HBM bytes per sweep launch from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", f"{workload_name}_{precision}_gram_sweep_traffic.md")
+    """HBM bytes per collect-pass launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload_name}_gram_sweep_traffic.md")
     if not os.path.exists(path):
         return None
+    key = f"traffic_bytes_per_launch[{precision}]:"
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key):
                 return float(line.split(":", 1)[1])
     return None
 
