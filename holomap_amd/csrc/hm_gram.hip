@@ -84,11 +84,14 @@ __device__ __forceinline__ void vm_wait() {
 //         step (hi.hi + lo.hi + hi.lo).
 // A workgroup has QW query rows that share every candidate slice (256 in
 // modes 0 and 1; 128 in mode 2, whose stages are twice as big) and one wave
-// per 32 of them (32 queries x 128 columns: four 32x32 tiles). With eight
-// waves, two share each SIMD, so one computes while the other waits. (A
-// 64-query wave tile halves the LDS reads per MFMA but leaves one wave per
-// SIMD; measured slower.)
-template <int MODE>
+// per 32 of them. A wave computes its 32 queries x CW columns (CW / 32 tiles
+// of 32x32). With eight waves, two share each SIMD, so one computes while
+// the other waits. (A 64-query wave tile halves the LDS reads per MFMA but
+// leaves one wave per SIMD; measured slower.)
+// CW = 256 (the bf16 collect pass): twice the MFMAs per stage and barrier,
+// and the query slices are re-read once per 256 columns instead of 128; its
+// 128 accumulators leave no room for the threshold pass's group maxima.
+template <int MODE, int CW = 128>
 struct SweepCfg {
   static constexpr int NT = MODE == 2 ? 2 : 1;   // parts per operand (hi, lo)
   static constexpr int ES = MODE == 1 ? 4 : 2;   // element bytes
@@ -96,20 +99,25 @@ struct SweepCfg {
   static constexpr int QW = MODE == 2 ? 128 : 256;  // query rows per item
   static constexpr int NU = 1;                   // 32-query tiles per wave
   static constexpr int NW = QW / (32 * NU);      // waves per workgroup
+  static constexpr int NCT = CW / 32;            // 32-column tiles per wave
   static constexpr int QG = QW / 8;              // 8-row copy groups of a query slice
-  static constexpr int RG = NT * (QG + 16);      // copy groups per stage
+  static constexpr int CG = CW / 8;              // ... and of a candidate slice
+  static constexpr int RG = NT * (QG + CG);      // copy groups per stage
   static constexpr int NI = RG / NW;             // operand copies per wave per stage
   static constexpr int QT = 0;                   // stage offsets: query hi,
   static constexpr int CT = QW * 128;            //   candidate hi,
-  static constexpr int QL = (QW + 128) * 128;    //   query lo, candidate lo
-  static constexpr int CL = (2 * QW + 128) * 128;
-  static constexpr int BIAS = NT * (QW + 128) * 128;  // NW/2 copies of the 128 biases
-  static constexpr int STAGE = BIAS + NW / 2 * 512;
-  static constexpr int NSTAGE = MODE == 2 ? 2 : 3;
+  static constexpr int QL = (QW + CW) * 128;     //   query lo, candidate lo
+  static constexpr int CL = (2 * QW + CW) * 128;
+  static constexpr int NBC = CW / 64;            // waves per copy of a block's CW biases
+  static constexpr int BIAS = NT * (QW + CW) * 128;  // NW / NBC copies of the biases
+  static constexpr int STAGE = BIAS + NW * 256;
+  static constexpr int NSTAGE = (MODE == 2 || CW == 256) ? 2 : 3;
   static constexpr int VM = NI + 1;              // vmcnt entries per stage and wave
   static constexpr int BIAS0 = NSTAGE * STAGE;   // the item's first block's biases
+  static constexpr int LDS = BIAS0 + CW * 4;
   static_assert(RG % NW == 0, "copy groups split evenly over the waves");
-  static_assert(NSTAGE * STAGE + 512 <= 160 * 1024, "LDS");
+  static_assert(NW % NBC == 0, "whole copies of the biases");
+  static_assert(LDS <= 160 * 1024, "LDS");
   static_assert(NW * 64 == sweep_threads(MODE), "the runtime launches sweep_threads(MODE)");
 };
 
@@ -122,10 +130,10 @@ __device__ __forceinline__ int row_swz(int r) { return (r & 7) ^ ((r >> 3) & 1);
 // -> row rb*8 + l/8 of the stage, chunk slot l%8) and 64 biases. src[i] is
 // the lane's source for feature slice 0 (and, for candidate rows, block 0 of
 // the split); cand[i] marks candidate groups, which also advance by block.
-template <int MODE>
+template <int MODE, int CW = 128>
 struct CopyPlan {
-  const uint8_t* src[SweepCfg<MODE>::NI];
-  bool cand[SweepCfg<MODE>::NI];
+  const uint8_t* src[SweepCfg<MODE, CW>::NI];
+  bool cand[SweepCfg<MODE, CW>::NI];
   const float* bias;
   int64_t blim;  // the lane's bias column is in the split iff boff < blim
 };
@@ -134,10 +142,10 @@ struct CopyPlan {
 // the accumulators need no per-column check.
 __device__ const float kBiasOut = -INFINITY;
 
-template <int MODE>
-__device__ __forceinline__ void sweep_issue(uint8_t* stage, const CopyPlan<MODE>& P, int64_t boff_bytes,
-                                            int koff, int64_t boff, int wave) {
-  using Cfg = SweepCfg<MODE>;
+template <int MODE, int CW>
+__device__ __forceinline__ void sweep_issue(uint8_t* stage, const CopyPlan<MODE, CW>& P,
+                                            int64_t boff_bytes, int koff, int64_t boff, int wave) {
+  using Cfg = SweepCfg<MODE, CW>;
 #pragma unroll
   for (int i = 0; i < Cfg::NI; ++i) {
     const int rb = wave * Cfg::NI + i;
@@ -146,15 +154,17 @@ __device__ __forceinline__ void sweep_issue(uint8_t* stage, const CopyPlan<MODE>
   }
   __builtin_amdgcn_global_load_lds(
       boff < P.blim ? P.bias + boff : &kBiasOut,
-      (lds_void_t*)(stage + Cfg::BIAS + (wave >> 1) * 512 + (wave & 1) * 256), 4, 0, 0);
+      (lds_void_t*)(stage + Cfg::BIAS + (wave / Cfg::NBC) * (Cfg::NBC * 256) +
+                    (wave % Cfg::NBC) * 256),
+      4, 0, 0);
 }
 
 // Wait until this wave's copies of the stage computed next have landed;
 // `after` stages were issued after it.
-template <int MODE>
+template <int MODE, int CW>
 __device__ __forceinline__ void sweep_wait(int after) {
-  constexpr int V = SweepCfg<MODE>::VM;
-  static_assert(SweepCfg<MODE>::NSTAGE <= 3, "waits for up to two later stages");
+  constexpr int V = SweepCfg<MODE, CW>::VM;
+  static_assert(SweepCfg<MODE, CW>::NSTAGE <= 3, "waits for up to two later stages");
   if (after >= 2) vm_wait<2 * V>();
   else if (after == 1) vm_wait<V>();
   else vm_wait<0>();
@@ -165,12 +175,13 @@ __device__ __forceinline__ void sweep_wait(int after) {
 // them (sched_group_barrier keeps that order; the default schedule
 // interleaves reads and MFMAs one by one and exposes the LDS latency each
 // time); all of a stage's reads are issued before its first MFMA.
-template <int MODE>
-__device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4],
+template <int MODE, int NCT = 4>
+__device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][NCT],
                                           const uint8_t* cand, const uint8_t* qrow_p, int q,
                                           int half, int swz) {
   using Cfg = SweepCfg<MODE>;
   constexpr int NU = Cfg::NU;
+  static_assert(NCT == 4 || MODE == 0, "wide tiles in the bf16 mode only");
   if constexpr (MODE == 1) {
     // chunk c = 2m + half holds features 8m + 4*half + j (j < 4); MFMA j
     // of step m sums features 8m + j and 8m + 4 + j over the two halves
@@ -201,31 +212,39 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4]
     __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 16, 0);
     __builtin_amdgcn_sched_group_barrier(0x008, 2 * NU * 16, 0);
   } else if constexpr (MODE == 0) {
-    // four k-steps of 16 features. The reads of step s + 2 go out with the
+    // four k-steps of 16 features. The reads of step s + LA go out with the
     // MFMAs of step s; before step s an explicit wait leaves only the reads
-    // of step s + 1 in flight (LDS reads complete in order). The compiler
-    // still strengthens some of these waits to 0: it cannot tell the reads
-    // apart from the LDS copies in flight.
-    constexpr int RS = NU + 4;  // LDS reads per step
-    bf16x8_t bh[4][NU], ah[4][4];
+    // of the later steps in flight (LDS reads complete in order). The
+    // compiler still strengthens some of these waits to 0: it cannot tell
+    // the reads apart from the LDS copies in flight. Wide tiles (8 column
+    // tiles: 8 MFMAs per step) look one step ahead, which keeps their
+    // operands beside 128 accumulators.
+    constexpr int RS = NU + NCT;  // LDS reads per step
+    constexpr int LA = NCT > 4 ? 1 : 2;
+    static_assert(RS < 16, "lgkmcnt");
+    bf16x8_t bh[4][NU], ah[4][NCT];
 #pragma unroll
-    for (int s = 0; s < 4 + 2; ++s) {
-      if (s >= 2 && s < 5) __builtin_amdgcn_s_waitcnt(0xc07f | (RS << 8));  // lgkmcnt(RS)
-      if (s == 5) __builtin_amdgcn_s_waitcnt(0xc07f);                     // lgkmcnt(0)
+    for (int s = 0; s < 4 + LA; ++s) {
+      if (s >= LA) {
+        // reads of steps s - LA + 1 .. min(s, 4) - 1 may stay in flight
+        const int keep = (s < 4 ? s : 4) - (s - LA) - 1;
+        if (keep >= 1) __builtin_amdgcn_s_waitcnt(0xc07f | (RS << 8));  // lgkmcnt(RS)
+        else __builtin_amdgcn_s_waitcnt(0xc07f);                        // lgkmcnt(0)
+      }
       __builtin_amdgcn_sched_barrier(0);
       if (s < 4) {
         const int cs = ((2 * s + half) ^ swz) * 16;
 #pragma unroll
         for (int u = 0; u < NU; ++u) bh[s][u] = *(const bf16x8_t*)(qrow_p + u * 32 * 128 + cs);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) ah[s][t] = *(const bf16x8_t*)(cand + (t * 32 + q) * 128 + cs);
+        for (int t = 0; t < NCT; ++t) ah[s][t] = *(const bf16x8_t*)(cand + (t * 32 + q) * 128 + cs);
       }
-      if (s >= 2) {
+      if (s >= LA) {
 #pragma unroll
         for (int u = 0; u < NU; ++u)
 #pragma unroll
-          for (int t = 0; t < 4; ++t)
-            acc[u][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[s - 2][t], bh[s - 2][u],
+          for (int t = 0; t < NCT; ++t)
+            acc[u][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[s - LA][t], bh[s - LA][u],
                                                                 acc[u][t], 0, 0, 0);
       }
       __builtin_amdgcn_sched_barrier(0);
@@ -264,10 +283,12 @@ __device__ __forceinline__ void sweep_mma(floatx16 (&acc)[SweepCfg<MODE>::NU][4]
 // Accumulator start of a block: bias_j / s for the block's columns, as
 // copied with the stage (prep stores the bias already divided by s; columns
 // past the split's end were copied from kBiasOut).
-__device__ __forceinline__ void sweep_acc_start(floatx16 (&acc)[4], const uint8_t* bias, int half) {
+template <int NCT>
+__device__ __forceinline__ void sweep_acc_start(floatx16 (&acc)[NCT], const uint8_t* bias,
+                                                int half) {
   const float* colb = (const float*)bias + 4 * half;
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
+  for (int t = 0; t < NCT; ++t)
 #pragma unroll
     for (int a4 = 0; a4 < 4; ++a4) {
       const floatx4 v = *(const floatx4*)(colb + t * 32 + 8 * a4);
@@ -276,17 +297,18 @@ __device__ __forceinline__ void sweep_acc_start(floatx16 (&acc)[4], const uint8_
     }
 }
 
-// Epilogue of one 128-column block for one 32-query tile. acc holds key'
-// (see the top of the file). Pass 0: gmax = max(gmax, key') per position
+// Epilogue of one block (NCT x 32 columns) for one 32-query tile. acc holds
+// key' (see the top of the file). Pass 0: gmax = max(gmax, key') per position
 // (v_med3 with +inf: a max without NaN canonicalisation). Pass 1: a hit is
 // key' >= Tp (Tp = T / s; NaN for lanes without a query: never); hits are
 // appended as (key' bits, column) to dst[min(cnt, cout-1)] and all counted
 // -- cnt > cout afterwards means the list overflowed (its contents are then
 // unused).
-template <int PASS>
-__device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[4], float (&gmax)[4][16],
+template <int PASS, int NCT>
+__device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[NCT], float (&gmax)[4][16],
                                                uint32_t colbase, float Tp, int cout,
                                                gptr_u64* dst, int& cnt) {
+  static_assert(PASS == 1 || NCT == 4, "group maxima for 128-column blocks");
   if constexpr (PASS == 0) {
     // +inf in a register: with a literal the compiler turns the med3 into a
     // NaN-canonicalising max (three instructions)
@@ -304,7 +326,7 @@ __device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[4], float (
     // lists keep), the value picked by a bitwise select tree on the position
     // (a tree of conditional moves on its bits; no register indexing).
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < NCT; ++t) {
       uint32_t m = 0;
 #pragma unroll
       for (int a = 0; a < 16; ++a) m |= acc[t][a] >= Tp ? 1u << a : 0u;
@@ -412,14 +434,16 @@ __device__ __forceinline__ void sweep_finish(const float (&gmax)[4][16], float T
   }
 }
 
-template <int MODE, int PASS>
-__global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
+template <int MODE, int PASS, int CW>
+__global__ __launch_bounds__((SweepCfg<MODE, CW>::NW * 64), 1) void gram_sweep_kernel(
     const DsDesc* __restrict__ ds_arr, const WorkItem* __restrict__ items,
     const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist,
     int sample) {
-  using Cfg = SweepCfg<MODE>;
+  using Cfg = SweepCfg<MODE, CW>;
   constexpr int NI = Cfg::NI, NSTAGE = Cfg::NSTAGE, STAGE = Cfg::STAGE, QG = Cfg::QG;
-  __shared__ __attribute__((aligned(1024))) uint8_t smem[NSTAGE * STAGE + 512];
+  constexpr int NCT = Cfg::NCT, NBC = Cfg::NBC;
+  static_assert(CW == 128 || PASS == 1, "the threshold pass keeps 128-column group maxima");
+  __shared__ __attribute__((aligned(1024))) uint8_t smem[Cfg::LDS];
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: LDS bases stay scalar
@@ -447,20 +471,20 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
     // the threshold pass of a sampled sweep visits every bstride-th block
     const int bstride = (PASS == 0 && sample) ? D.pstride : 1;
     const int csel = (PASS == 0 && sample) ? D.C0 : D.C;
-    const int nblk_all = cend > cbeg ? (int)((cend - cbeg + 127) / 128) : 0;
+    const int nblk_all = cend > cbeg ? (int)((cend - cbeg + CW - 1) / CW) : 0;
     const int nblk = (nblk_all + bstride - 1) / bstride;
     const int total = nblk * nchunk;
-    const int64_t blkbytes = (int64_t)128 * rowbytes * bstride;  // between swept blocks
-    const int64_t bcols = (int64_t)128 * bstride;
+    const int64_t blkbytes = (int64_t)CW * rowbytes * bstride;  // between swept blocks
+    const int64_t bcols = (int64_t)CW * bstride;
 
     const uint8_t* ophi = (MODE == 1) ? (const uint8_t*)D.op32 : (const uint8_t*)D.op16;
     const uint8_t* oplo = (MODE == 2) ? (const uint8_t*)D.op16lo : ophi;
-    CopyPlan<MODE> P;
+    CopyPlan<MODE, CW> P;
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
       const int rb = wave * NI + i;  // uniform
-      const bool lo = rb >= QG + 16;
-      const int rl = lo ? rb - (QG + 16) : rb;  // group within the hi or lo part
+      const bool lo = rb >= QG + Cfg::CG;
+      const int rl = lo ? rb - (QG + Cfg::CG) : rb;  // group within the hi or lo part
       const bool cand = rl >= QG;
       const int r = ((cand ? rl - QG : rl) << 3) + (lane >> 3);  // row within its tile
       int64_t row;
@@ -477,8 +501,8 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
       P.src[i] = (lo ? oplo : ophi) + row * rowbytes + cbyte;
       P.cand[i] = cand;
     }
-    P.bias = D.colbias + cbeg + (wave & 1) * 64 + lane;
-    P.blim = cend - cbeg - ((wave & 1) * 64 + lane);
+    P.bias = D.colbias + cbeg + (wave % NBC) * 64 + lane;
+    P.blim = cend - cbeg - ((wave % NBC) * 64 + lane);
 
     // this lane's own queries: slot 32*(NU*wave + u) + q of the item, u < NU
     constexpr int NU = Cfg::NU;
@@ -510,9 +534,9 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
     }
 
     HM_LDS_BARRIER();  // the previous item's last stage is no longer read
-    // the first block's biases (waves 0 and 1, 64 each), then stages
+    // the first block's biases (waves 0 .. NBC-1, 64 each), then stages
     // 0 .. NSTAGE-2; every stage carries the biases of the block after its own
-    if (wave < 2)
+    if (wave < NBC)
       __builtin_amdgcn_global_load_lds(0 < P.blim ? P.bias : &kBiasOut,
                                        (lds_void_t*)(smem + Cfg::BIAS0 + wave * 256), 4, 0, 0);
     // (bi, ci): block and slice of the next stage to issue
@@ -520,8 +544,8 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
 #pragma unroll
     for (int g = 0; g < NSTAGE - 1; ++g) {
       if (g < total) {
-        sweep_issue<MODE>(smem + g * STAGE, P, bi * blkbytes, ci * 128, (int64_t)(bi + 1) * bcols,
-                          wave);
+        sweep_issue<MODE, CW>(smem + g * STAGE, P, bi * blkbytes, ci * 128,
+                              (int64_t)(bi + 1) * bcols, wave);
         if (++ci == nchunk) { ci = 0; ++bi; }
       }
     }
@@ -532,14 +556,14 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
       else vm_wait<0>();
     }
     HM_LDS_BARRIER();  // the first block's biases are in LDS
-    floatx16 acc[NU][4];
+    floatx16 acc[NU][NCT];
     {
-      floatx16 a0[4];
+      floatx16 a0[NCT];
       sweep_acc_start(a0, smem + Cfg::BIAS0, half);
 #pragma unroll
       for (int u = 0; u < NU; ++u)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[u][t] = a0[t];
+        for (int t = 0; t < NCT; ++t) acc[u][t] = a0[t];
     }
 
     // stage g: wait for this wave's copies of stage g; one barrier then
@@ -548,27 +572,28 @@ __global__ __launch_bounds__(SweepCfg<MODE>::NW * 64, 1) void gram_sweep_kernel(
     int ch = 0;                // slice of stage g within its block
     uint32_t colbase = (uint32_t)cbeg + 4 * half;  // this lane's columns of the block
     for (int g = 0; g < total; ++g) {
-      sweep_wait<MODE>(total - 1 - g < NSTAGE - 2 ? total - 1 - g : NSTAGE - 2);
+      sweep_wait<MODE, CW>(total - 1 - g < NSTAGE - 2 ? total - 1 - g : NSTAGE - 2);
       HM_LDS_BARRIER();
       const int gi = g + NSTAGE - 1;
       if (gi < total) {
-        sweep_issue<MODE>(smem + (gi % NSTAGE) * STAGE, P, bi * blkbytes, ci * 128,
-                          (int64_t)(bi + 1) * bcols, wave);
+        sweep_issue<MODE, CW>(smem + (gi % NSTAGE) * STAGE, P, bi * blkbytes, ci * 128,
+                              (int64_t)(bi + 1) * bcols, wave);
         if (++ci == nchunk) { ci = 0; ++bi; }
       }
       const uint8_t* st = smem + (g % NSTAGE) * STAGE;
-      sweep_mma<MODE>(acc, st + Cfg::CT, st + Cfg::QT + (NU * wave * 32 + q) * 128, q, half, swz);
+      sweep_mma<MODE, NCT>(acc, st + Cfg::CT, st + Cfg::QT + (NU * wave * 32 + q) * 128, q, half,
+                           swz);
       if (++ch == nchunk) {
         // ---- epilogue of the block; the next block starts at its biases ----
 #pragma unroll
         for (int u = 0; u < NU; ++u)
           sweep_epilogue<PASS>(acc[u], gmax[u], colbase, T[u], D.cout, dst[u], cnt[u]);
-        floatx16 a0[4];
+        floatx16 a0[NCT];
         sweep_acc_start(a0, st + Cfg::BIAS, half);
 #pragma unroll
         for (int u = 0; u < NU; ++u)
 #pragma unroll
-          for (int t = 0; t < 4; ++t) acc[u][t] = a0[t];
+          for (int t = 0; t < NCT; ++t) acc[u][t] = a0[t];
         ch = 0;
         colbase += (uint32_t)bcols;
       }
@@ -742,14 +767,15 @@ HM_QRES_INST(QresPair, 0)
 HM_QRES_INST(QresPair, 1)
 #undef HM_QRES_INST
 
-#define HM_INST(MODE, PASS)                                                       \
-  template __global__ void gram_sweep_kernel<MODE, PASS>(const DsDesc*, const WorkItem*, \
-                                                         const int*, int, const uint32_t*, int);
-HM_INST(0, 0)
-HM_INST(0, 1)
-HM_INST(1, 0)
-HM_INST(1, 1)
-HM_INST(2, 0)
-HM_INST(2, 1)
+#define HM_INST(MODE, PASS, CW)                                                       \
+  template __global__ void gram_sweep_kernel<MODE, PASS, CW>(const DsDesc*, const WorkItem*, \
+                                                             const int*, int, const uint32_t*, int);
+HM_INST(0, 0, 128)
+HM_INST(0, 1, 128)
+HM_INST(0, 1, 256)
+HM_INST(1, 0, 128)
+HM_INST(1, 1, 128)
+HM_INST(2, 0, 128)
+HM_INST(2, 1, 128)
 
 }  // namespace hm

@@ -20,7 +20,7 @@ namespace hm {
 __global__ void rownorm_kernel(const DsDesc*, int, int64_t, float*);
 __global__ void prep_kernel(const DsDesc*, int, int64_t, const int64_t*, const float*, int, float*);
 __global__ void statmax_kernel(const DsDesc*, const float*, const int64_t*);
-template <int MODE, int PASS>
+template <int MODE, int PASS, int CW>
 __global__ void gram_sweep_kernel(const DsDesc*, const WorkItem*, const int*, int, const uint32_t*,
                                   int);
 template <class Cfg, int PASS>
@@ -313,6 +313,14 @@ struct Plan {
 // so that the launch has enough work items for every CU. Buffers are sized
 // for any configuration (kSplitCap splits, kCandCap key slots per row), so
 // the autotuner can change it after allocation.
+// The primary bf16 collect pass over rows longer than the query-resident
+// limit sweeps 256-column blocks (twice the MFMAs per stage and barrier,
+// half the query-slice re-reads); its last block may read up to 128
+// operand rows past a split's end (masked by their -inf biases), which the
+// operand's allocation covers.
+constexpr bool kWideCollect = true;
+constexpr int kWideSlackRows = 128;
+
 constexpr int kSplitCap = 5;
 constexpr int kCandCap = 512;  // the re-rank sorts at most 512 keys per row
 constexpr int kSampleStride = 4;  // threshold pass: at most every 4th block
@@ -420,6 +428,9 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
   if (stages & HM_STAGE_KNN) {
     b += P.sum_xr * 4 + P.sum_cos * 4;               // xr, op32 (cosine)
     if (mode == 0 || mode == 2) b += P.sum_x16 * 4;  // op16 + op16lo (bf16x3 or its escalation)
+    int64_t kp16_max = 0;
+    for (auto& h : P.dsh) kp16_max = std::max<int64_t>(kp16_max, h.kp16);
+    b += kWideSlackRows * kp16_max * 2;  // rows the wide collect pass may read past the end
     b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + P.total_pad / 16 * HM_NSTAT * 4 +
          R * 12;
     b += P.sum_cand * 8 + R * 2 * kSplitCap * 12 + R * kSplitCap * 4;  // cand, ctau+ccnt, thr
@@ -566,13 +577,19 @@ constexpr int kListPerCu[3] = {2, 1, 1};     // workgroups per CU
 hm_status launch_sweep(hm_job* J, int mode, const WorkItem* items, const int* nitems_dev,
                        int nitems_host, const uint32_t* qlist, unsigned grid, int sample,
                        int passes) {
-#define HM_GRAM(M, P)                                                                        \
-  hipLaunchKernelGGL((gram_sweep_kernel<M, P>), dim3(grid), dim3(sweep_threads(M)), 0,         \
+#define HM_GRAM(M, P, CW)                                                                    \
+  hipLaunchKernelGGL((gram_sweep_kernel<M, P, CW>), dim3(grid), dim3(sweep_threads(M)), 0,     \
                      J->stream, J->d_desc, items, nitems_dev, nitems_host, qlist, sample)
-#define HM_GRAM2(M)                      \
-  do {                                   \
-    if (passes != 1) HM_GRAM(M, 0);      \
-    if (passes != 0) HM_GRAM(M, 1);      \
+  // the primary bf16 collect pass sweeps 256-column blocks (kWideCollect)
+#define HM_GRAM2(M)                                               \
+  do {                                                            \
+    if (passes != 0 && M == 0 && sample && kWideCollect) {        \
+      if (passes != 1) HM_GRAM(M, 0, 128);                        \
+      HM_GRAM(M, 1, 256);                                         \
+    } else {                                                      \
+      if (passes != 1) HM_GRAM(M, 0, 128);                        \
+      if (passes != 0) HM_GRAM(M, 1, 128);                        \
+    }                                                             \
   } while (0)
   switch (mode) {
     case 0: HM_GRAM2(0); break;
@@ -1072,15 +1089,18 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
     if (s_ != HM_OK) return bail(s_);           \
   } while (0)
   const int64_t R = P.total_rows, E = P.total_edges;
-  int64_t sum_nd = 0;
-  for (auto& h : P.dsh) sum_nd += h.n * h.d;
+  int64_t sum_nd = 0, kp16_max = 0;
+  for (auto& h : P.dsh) {
+    sum_nd += h.n * h.d;
+    kp16_max = std::max<int64_t>(kp16_max, h.kp16);
+  }
   HM_A(J->d_desc, nds);
   HM_A(J->x, sum_nd);
   const bool knn = stages & HM_STAGE_KNN;
   if (knn) {
     HM_A(J->xr, P.sum_xr);
     if (P.sum_cos > 0) HM_A(J->op32c, P.sum_cos);
-    if (J->mode == 0 || J->mode == 2) HM_A(J->op16, P.sum_x16);
+    if (J->mode == 0 || J->mode == 2) HM_A(J->op16, P.sum_x16 + kWideSlackRows * kp16_max);
     if (J->mode == 2 || J->esc_mode == 2) HM_A(J->op16lo, P.sum_x16);
     HM_A(J->colbias, P.total_pad);
     HM_A(J->rstat, R * HM_NSTAT);
@@ -1123,6 +1143,11 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
   }
   HM_A(J->zany, nds);  // per-dataset flags, kept clear between runs by calib_fixup_kernel
   // (a fill kernel: the scratch array keeps the last call's contents)
+  if (J->op16) {  // zero bf16 operand rows past the last dataset (read, never used)
+    const int64_t words = kWideSlackRows * kp16_max / 2;
+    hipLaunchKernelGGL(fill_u32, dim3(grid_for(words, 256)), dim3(256), 0, J->stream,
+                       (uint32_t*)(J->op16 + P.sum_x16), words, 0u);
+  }
   hipLaunchKernelGGL(fill_u32, dim3(grid_for(nds, 256)), dim3(256), 0, J->stream, J->zany,
                      (int64_t)nds, 0u);
   if (hipGetLastError() != hipSuccess) return bail(fail(ctx, HM_EDEVICE, "flag clear failed"));
