@@ -326,8 +326,10 @@ def _opts(precision="bf16", calib="ref_f32", margin=-1, nsplit=0, use_graph=Fals
                                 autotune=int(autotune))
 
 
-def _validate(x: np.ndarray, n_features: int, k: int):
-    """src/api.rs:113-144 (the checks the graph stages depend on)."""
+def _validate(x: np.ndarray, n_features: int, k: int, finite: bool = True):
+    """src/api.rs:113-144 (the checks the graph stages depend on). finite=False
+    leaves the scan for non-finite values to the C entry, which refuses them
+    (HM_ENONFINITE; _raise_nonfinite then names the index)."""
     if n_features == 0 or x.size == 0 or x.size % n_features != 0:
         raise BadShape(x.size, n_features)
     if k < 2:
@@ -335,6 +337,11 @@ def _validate(x: np.ndarray, n_features: int, k: int):
     n = x.size // n_features
     if n <= k:
         raise TooFewPoints(n, k)
+    if finite:
+        _raise_nonfinite(x)
+
+
+def _raise_nonfinite(x: np.ndarray):
     bad = np.flatnonzero(~np.isfinite(x))
     if bad.size:
         raise NonFiniteInput(int(bad[0]))
@@ -427,7 +434,7 @@ def knn_graph(data, n_features: int, k: int, metric="euclidean", *, precision="b
     hm_knn_graph call (host buffers in and out; the C entry plans, runs and
     frees a job on the context's device)."""
     x = np.ascontiguousarray(data, np.float32).reshape(-1)
-    _validate(x, n_features, k)
+    _validate(x, n_features, k, finite=False)  # hm_knn_graph scans for non-finite values
     c = ctx or Context.default()
     n = x.size // n_features
     idx = np.empty((n, k), np.uint32)
@@ -455,6 +462,8 @@ def knn_graph(data, n_features: int, k: int, metric="euclidean", *, precision="b
                                   ctypes.byref(opt), ctypes.byref(o))
     if st == _lib.HM_EBUDGET:
         raise MemoryError(c.last_error())
+    if st == _lib.HM_ENONFINITE:
+        _raise_nonfinite(x)
     c._check(st, n=n, k=k)
     m = int(o.nnz)
     g = FuzzyGraph(n, rows[:m].copy(), cols[:m].copy(), vals[:m].copy())
@@ -626,11 +635,13 @@ class Server:
 
     def submit(self, data, n_features: int, k: int, metric="euclidean") -> int:
         x = np.ascontiguousarray(data, np.float32).reshape(-1)
-        _validate(x, n_features, k)
+        _validate(x, n_features, k, finite=False)  # hm_server_submit scans for them
         rid = ctypes.c_uint64()
-        self.ctx._check(_lib.load().hm_server_submit(self.handle, x.ctypes.data,
-                                                     x.size // n_features, n_features, k,
-                                                     _metric(metric), ctypes.byref(rid)))
+        st = _lib.load().hm_server_submit(self.handle, x.ctypes.data, x.size // n_features,
+                                          n_features, k, _metric(metric), ctypes.byref(rid))
+        if st == _lib.HM_ENONFINITE:
+            _raise_nonfinite(x)
+        self.ctx._check(st)
         self._shape[rid.value] = (x.size // n_features, k)
         return rid.value
 
@@ -659,6 +670,7 @@ class Server:
         dist = np.empty((n, k), np.float32)
         sig = np.empty(n, np.float32)
         rho = np.empty(n, np.float32)
+        # exactly nnz entries (known before the copy): no truncating copies
         rows = np.empty(max(m, 1), np.uint32)
         cols = np.empty(max(m, 1), np.uint32)
         vals = np.empty(max(m, 1), np.float32)
@@ -670,7 +682,7 @@ class Server:
         o.indptr = indptr.ctypes.data
         o.cap = max(m, 1)
         self.ctx._check(L.hm_server_fetch(self.handle, rid, ctypes.byref(o)))
-        g = FuzzyGraph(n, rows[:m].copy(), cols[:m].copy(), vals[:m].copy())
+        g = FuzzyGraph(n, rows[:m], cols[:m], vals[:m])
         return GraphResult(Knn(k, idx, dist), SmoothKnn(sig, rho, None), g, indptr)
 
     def close(self):

@@ -118,6 +118,8 @@ struct hm_ctx {
   uint64_t budget = 0;
   uint64_t in_use = 0;
   uint8_t* scratch = nullptr;  // the device's copy of `scratch` (hm_knn_graph_noalloc)
+  uint8_t* stage = nullptr;    // pinned host staging of downloads (stage_cap bytes)
+  uint64_t stage_cap = 0;
   std::string err;
 };
 
@@ -899,6 +901,7 @@ void hm_ctx_destroy(hm_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stage) (void)hipHostFree(c->stage);
   delete c;
 }
 
@@ -1508,17 +1511,20 @@ hm_status hm_job_download(hm_job* J, uint32_t ds, hm_graph_out* o) {
   HM_HIP(ctx, hipStreamSynchronize(J->stream));
   const DsHost& h = J->dsh[ds];
   const size_t m = (size_t)h.n * h.k;
-  if (o->indices) HM_HIP(ctx, hipMemcpy(o->indices, J->knn_idx + h.edge_off, m * 4, hipMemcpyDeviceToHost));
-  if (o->dists) HM_HIP(ctx, hipMemcpy(o->dists, J->knn_dist + h.edge_off, m * 4, hipMemcpyDeviceToHost));
-  if (o->sigmas) HM_HIP(ctx, hipMemcpy(o->sigmas, J->sigma + h.row_off, h.n * 4, hipMemcpyDeviceToHost));
-  if (o->rhos) HM_HIP(ctx, hipMemcpy(o->rhos, J->rho + h.row_off, h.n * 4, hipMemcpyDeviceToHost));
-  if (o->sigmas64) HM_HIP(ctx, hipMemcpy(o->sigmas64, J->sigma64 + h.row_off, h.n * 8, hipMemcpyDeviceToHost));
-  if (o->dists64) {
-    if (!J->dist64)
-      return fail(ctx, HM_EINVAL, "no f64 distances: they are kept in the FP64 calibration mode "
-                                  "with the kNN stage");
-    HM_HIP(ctx, hipMemcpy(o->dists64, J->dist64 + h.edge_off, m * 8, hipMemcpyDeviceToHost));
-  }
+  if (o->dists64 && !J->dist64)
+    return fail(ctx, HM_EINVAL, "no f64 distances: they are kept in the FP64 calibration mode "
+                                "with the kNN stage");
+  struct Piece { void* dst; const void* src; size_t bytes; };
+  std::vector<Piece> pc;
+  auto want = [&](void* dst, const void* src, size_t bytes) {
+    if (dst && bytes) pc.push_back(Piece{dst, src, bytes});
+  };
+  want(o->indices, J->knn_idx + h.edge_off, m * 4);
+  want(o->dists, J->knn_dist + h.edge_off, m * 4);
+  want(o->sigmas, J->sigma + h.row_off, h.n * 4);
+  want(o->rhos, J->rho + h.row_off, h.n * 4);
+  want(o->sigmas64, J->sigma64 + h.row_off, h.n * 8);
+  if (J->dist64) want(o->dists64, J->dist64 + h.edge_off, m * 8);
   HM_TRY(fp64_error(J, ds, o->err64));
   o->nnz = 0;
   if (J->stages & HM_STAGE_FUZZY) {
@@ -1532,11 +1538,44 @@ hm_status hm_job_download(hm_job* J, uint32_t ds, hm_graph_out* o) {
     o->nnz = nnz;
     if ((o->rows || o->cols || o->vals) && o->cap < nnz)
       return fail(ctx, HM_EINVAL, "COO capacity " + std::to_string(o->cap) + " < nnz " + std::to_string(nnz));
-    if (o->rows) HM_HIP(ctx, hipMemcpy(o->rows, J->rows + se[0], nnz * 4, hipMemcpyDeviceToHost));
-    if (o->cols) HM_HIP(ctx, hipMemcpy(o->cols, J->cols + se[0], nnz * 4, hipMemcpyDeviceToHost));
-    if (o->vals) HM_HIP(ctx, hipMemcpy(o->vals, J->vals + se[0], nnz * 4, hipMemcpyDeviceToHost));
-    if (o->indptr)
-      HM_HIP(ctx, hipMemcpy(o->indptr, J->indptr_l + J->iptr_off_h[ds], (h.n + 1) * 8, hipMemcpyDeviceToHost));
+    want(o->rows, J->rows + se[0], nnz * 4);
+    want(o->cols, J->cols + se[0], nnz * 4);
+    want(o->vals, J->vals + se[0], nnz * 4);
+    want(o->indptr, J->indptr_l + J->iptr_off_h[ds], (h.n + 1) * 8);
+  }
+  // The outputs go through the context's pinned staging buffer: asynchronous
+  // DMA copies, one synchronisation, then host copies (a copy into pageable
+  // memory is staged and synchronised piece by piece otherwise). If pinning
+  // fails, the copies go directly.
+  uint64_t total = 0;
+  for (auto& p : pc) total += (p.bytes + 255) & ~(uint64_t)255;
+  if (total > ctx->stage_cap) {
+    uint64_t cap = 1ull << 20;
+    while (cap < total) cap <<= 1;
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
+    ctx->stage = nullptr;
+    ctx->stage_cap = 0;
+    if (hipHostMalloc((void**)&ctx->stage, cap, hipHostMallocDefault) == hipSuccess) {
+      ctx->stage_cap = cap;
+    } else {
+      (void)hipGetLastError();
+      ctx->stage = nullptr;
+    }
+  }
+  if (!ctx->stage) {
+    for (auto& p : pc) HM_HIP(ctx, hipMemcpy(p.dst, p.src, p.bytes, hipMemcpyDeviceToHost));
+    return HM_OK;
+  }
+  uint64_t off = 0;
+  for (auto& p : pc) {
+    HM_HIP(ctx, hipMemcpyAsync(ctx->stage + off, p.src, p.bytes, hipMemcpyDeviceToHost, J->stream));
+    off += (p.bytes + 255) & ~(uint64_t)255;
+  }
+  HM_HIP(ctx, hipStreamSynchronize(J->stream));
+  off = 0;
+  for (auto& p : pc) {
+    std::memcpy(p.dst, ctx->stage + off, p.bytes);
+    off += (p.bytes + 255) & ~(uint64_t)255;
   }
   return HM_OK;
 }

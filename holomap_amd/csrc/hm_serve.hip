@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -26,12 +27,60 @@
 
 namespace {
 
+// A request's rows are copied at submission into pinned host memory, so
+// that their upload is an asynchronous DMA (a copy from pageable memory
+// blocks the client's thread for its whole length). Blocks come in
+// power-of-two size classes (>= 1 MiB) and are reused across requests, so
+// pinning is paid once per block; past kPinnedCap bytes pinned in all, or if
+// pinning fails, requests fall back to pageable memory.
+constexpr uint64_t kPinnedCap = 8ull << 30;
+struct PinnedPool {
+  std::unordered_map<int, std::vector<void*>> free;  // size class -> blocks
+  uint64_t pinned = 0;
+  static int size_class(uint64_t bytes) {
+    int c = 20;
+    while ((1ull << c) < bytes) ++c;
+    return c;
+  }
+  // A block of >= bytes; *cls = its class, or -1 for pageable memory.
+  void* get(uint64_t bytes, int* cls) {
+    const int c = size_class(bytes);
+    auto& v = free[c];
+    if (!v.empty()) {
+      void* p = v.back();
+      v.pop_back();
+      *cls = c;
+      return p;
+    }
+    void* p = nullptr;
+    if (pinned + (1ull << c) <= kPinnedCap &&
+        hipHostMalloc(&p, (size_t)1 << c, hipHostMallocDefault) == hipSuccess) {
+      pinned += 1ull << c;
+      *cls = c;
+      return p;
+    }
+    (void)hipGetLastError();  // a failed pinning is not an error: pageable memory instead
+    *cls = -1;
+    return std::malloc(bytes ? bytes : 1);
+  }
+  void put(void* p, int cls) {
+    if (!p) return;
+    if (cls >= 0) free[cls].push_back(p);
+    else std::free(p);
+  }
+  ~PinnedPool() {
+    for (auto& kv : free)
+      for (void* p : kv.second) (void)hipHostFree(p);
+  }
+};
+
 struct Request {
   uint64_t id = 0;
   uint64_t n = 0;
   uint32_t d = 0, k = 0;
   int32_t metric = 0;
-  std::vector<float> data;  // host copy, kept until the job has run
+  float* data = nullptr;    // host copy (pinned when possible), kept until the job has run
+  int data_cls = -1;        // its pool size class (-1: pageable)
   int state = 0;            // 0 queued, 1 running, 2 done, < 0 error
   int job = -1;             // index into Server::jobs
   uint32_t ds = 0;          // dataset slot within the job
@@ -47,6 +96,7 @@ struct Job {
 }  // namespace
 
 struct hm_server {
+  PinnedPool pool;  // destroyed last: the requests' blocks are returned before
   hm_ctx* ctx = nullptr;
   hm_server_config cfg;
   uint64_t next_id = 1;
@@ -81,10 +131,16 @@ hm_status hm_server_create(hm_ctx* ctx, const hm_server_config* cfg, hm_server**
   return HM_OK;
 }
 
+static void release_data(hm_server* s, Request& r) {
+  s->pool.put(r.data, r.data_cls);
+  r.data = nullptr;
+}
+
 void hm_server_destroy(hm_server* s) {
   if (!s) return;
   for (auto& j : s->jobs)
-    if (j.job) hm_job_destroy(j.job);
+    if (j.job) hm_job_destroy(j.job);  // synchronises with its stream: uploads are over
+  for (auto& kv : s->reqs) release_data(s, kv.second);
   delete s;
 }
 
@@ -104,7 +160,8 @@ hm_status hm_server_submit(hm_server* s, const float* data, uint64_t n, uint32_t
   r.d = d;
   r.k = k;
   r.metric = metric;
-  r.data.assign(data, data + n * d);
+  r.data = (float*)s->pool.get(n * d * sizeof(float), &r.data_cls);
+  std::memcpy(r.data, data, n * d * sizeof(float));
   *id = r.id;
   s->queue.push_back(r.id);
   s->reqs.emplace(r.id, std::move(r));
@@ -139,8 +196,7 @@ hm_status hm_server_progress(hm_server* s, uint32_t* completed) {
     for (uint64_t id : j.reqs) {
       Request& r = s->reqs[id];
       r.state = q == 1 ? 2 : HM_EDEVICE;
-      r.data.clear();
-      r.data.shrink_to_fit();
+      release_data(s, r);
       ++fin;
     }
   }
@@ -149,8 +205,7 @@ hm_status hm_server_progress(hm_server* s, uint32_t* completed) {
   auto fail_head = [&](hm_status st) {
     Request& r = s->reqs[s->queue.front()];
     r.state = st;
-    r.data.clear();
-    r.data.shrink_to_fit();
+    release_data(s, r);
     s->queue.pop_front();
     ++fin;
   };
@@ -213,13 +268,14 @@ hm_status hm_server_progress(hm_server* s, uint32_t* completed) {
     for (size_t i = 0; i < ids.size(); ++i) s->queue.pop_front();
     if (st == HM_OK) {
       for (size_t i = 0; i < ids.size() && st == HM_OK; ++i)
-        st = hm_job_upload(J, (uint32_t)i, s->reqs[ids[i]].data.data());
+        st = hm_job_upload(J, (uint32_t)i, s->reqs[ids[i]].data);
       if (st == HM_OK) st = hm_job_run(J);
       if (st != HM_OK) hm_job_destroy(J);
     }
-    if (st != HM_OK) {
+    if (st != HM_OK) {  // (the job, destroyed, has finished its uploads)
       for (uint64_t id : ids) {
         s->reqs[id].state = st;
+        release_data(s, s->reqs[id]);
         ++fin;
       }
       continue;
@@ -281,6 +337,7 @@ hm_status hm_server_fetch(hm_server* s, uint64_t id, hm_graph_out* out) {
   Request& r = it->second;
   if (r.state < 0) {
     const hm_status st = r.state;
+    release_data(s, r);
     s->reqs.erase(it);
     return st;
   }
