@@ -917,10 +917,22 @@ hm_status hm_validate(const float* data, uint64_t len, uint32_t d, uint32_t k, u
   const uint64_t n = len / d;
   if (n <= k) return HM_ETOOFEW;
   if (data) {
-    for (uint64_t i = 0; i < len; ++i) {
-      if (!std::isfinite(data[i])) {
-        if (bad) *bad = i;
-        return HM_ENONFINITE;
+    // blocks of 4096 values tested with a branch-free OR of the exponent
+    // test (vectorised); only a block holding a non-finite value is searched
+    for (uint64_t b = 0; b < len; b += 4096) {
+      const uint64_t e = std::min<uint64_t>(len, b + 4096);
+      uint32_t any = 0;
+      for (uint64_t i = b; i < e; ++i) {
+        uint32_t u;
+        std::memcpy(&u, data + i, 4);
+        any |= (u & 0x7f800000u) == 0x7f800000u ? 1u : 0u;
+      }
+      if (!any) continue;
+      for (uint64_t i = b; i < e; ++i) {
+        if (!std::isfinite(data[i])) {
+          if (bad) *bad = i;
+          return HM_ENONFINITE;
+        }
       }
     }
   }
