@@ -120,6 +120,7 @@ struct hm_ctx {
   uint8_t* scratch = nullptr;  // the device's copy of `scratch` (hm_knn_graph_noalloc)
   uint8_t* stage = nullptr;    // pinned host staging of downloads (stage_cap bytes)
   uint64_t stage_cap = 0;
+  std::vector<hipStream_t> spare_streams;  // streams of destroyed jobs, for reuse
   std::string err;
 };
 
@@ -902,6 +903,7 @@ void hm_ctx_destroy(hm_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->stage) (void)hipHostFree(c->stage);
+  for (hipStream_t st : c->spare_streams) (void)hipStreamDestroy(st);
   delete c;
 }
 
@@ -1077,8 +1079,12 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
     J->arena_cap = arena_cap;
     J->own_stream = false;
     J->stream = ctx->stream;
+  } else if (!ctx->spare_streams.empty()) {
+    // every job has its own stream (a destroyed job's, when one is spare):
+    // jobs of one context overlap on the device
+    J->stream = ctx->spare_streams.back();
+    ctx->spare_streams.pop_back();
   } else if (hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking) != hipSuccess) {
-    // every job has its own stream: jobs of one context overlap on the device
     delete J;
     return fail(ctx, HM_EDEVICE, "stream create failed");
   }
@@ -1260,10 +1266,7 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
   if (st == HM_OK && hipStreamSynchronize(J->stream) != hipSuccess)  // host sources are locals
     st = fail(ctx, HM_EDEVICE, "descriptor upload failed");
   if (st != HM_OK) return bail(st);
-  if (!arena) {
-    for (auto& e : J->ev) {
-      if (hipEventCreate(&e) != hipSuccess) return bail(fail(ctx, HM_EDEVICE, "event create"));
-    }
+  if (!arena) {  // (the timing events are created by the first timed run)
     ctx->in_use += J->bytes;
     J->counted = true;
   }
@@ -1286,8 +1289,11 @@ void hm_job_destroy(hm_job* J) {
     if (e) (void)hipEventDestroy(e);
   for (void* p : J->allocs) (void)hipFreeAsync(p, J->stream);
   if (J->stream && J->own_stream) {
-    (void)hipStreamSynchronize(J->stream);
-    (void)hipStreamDestroy(J->stream);
+    // the stream goes back to the context (up to 16 kept) once its frees ran
+    if (hipStreamSynchronize(J->stream) == hipSuccess && J->ctx->spare_streams.size() < 16)
+      J->ctx->spare_streams.push_back(J->stream);
+    else
+      (void)hipStreamDestroy(J->stream);
   }
   if (J->counted) J->ctx->in_use -= J->bytes;
   delete J;
@@ -1400,6 +1406,10 @@ hm_status hm_job_run(hm_job* J) {
   hm_ctx* ctx = J->ctx;
   HM_HIP(ctx, hipSetDevice(ctx->device));
   J->lap_valid = false;
+  if ((J->opt.timing || J->opt.autotune) && !J->ev[0]) {  // timing events, on first need
+    for (auto& e : J->ev)
+      if (!e && hipEventCreate(&e) != hipSuccess) return fail(ctx, HM_EDEVICE, "event create");
+  }
   if (J->opt.autotune && !J->tuned && (J->stages & HM_STAGE_KNN) && J->mode >= 0)
     HM_TRY(autotune(J));
   J->timed = J->opt.timing && !J->opt.use_graph;
