@@ -105,14 +105,16 @@ __device__ __forceinline__ int calib_find_ds(const DsDesc* ds, int nds, int64_t 
 // Directed membership strength of edge t of row i (global row g) of
 // dataset D (src/fuzzy.rs:31-48): w = 1 if d - rho <= 0 or sigma == 0, else
 // exp(-((d - rho) / sigma)); self edges and w == 0 are dropped (key1 =
-// sentinel). Written for the symmetrisation's sort: key1 = global column,
-// pay1 = edge id, wval = w, srcrow = g.
+// sentinel). Written for the symmetrisation's transpose: key1 = global
+// column, wval = w, and each kept edge takes the next slot of its column's
+// in-edge count, erank = the count before (an integer atomic: which edge
+// gets which slot depends on scheduling, the counts do not).
 __device__ __forceinline__ void directed_edge(const DsDesc& D, int64_t i, int64_t g, int t,
                                               float d, float rho, float sigma, uint32_t sentinel,
                                               uint32_t* __restrict__ key1,
-                                              uint32_t* __restrict__ pay1,
+                                              uint32_t* __restrict__ indeg,
                                               float* __restrict__ wval,
-                                              uint32_t* __restrict__ srcrow) {
+                                              uint32_t* __restrict__ erank) {
   const int64_t le = i * D.k + t;
   const int64_t e = D.edge_off + le;
   const uint32_t col = D.knn_idx[le];
@@ -120,9 +122,8 @@ __device__ __forceinline__ void directed_edge(const DsDesc& D, int64_t i, int64_
   const float w = (dd <= 0.0f || sigma == 0.0f) ? 1.0f : hm_expf(-(dd / sigma));
   const bool keep = (int64_t)col != i && w != 0.0f;
   key1[e] = keep ? (uint32_t)(D.row_off + col) : sentinel;
-  pay1[e] = (uint32_t)e;
   wval[e] = w;
-  srcrow[e] = (uint32_t)g;
+  erank[e] = keep ? atomicAdd(indeg + D.row_off + col, 1u) : 0u;
 }
 
 // The same edge from the f64 chain: w = exp(-((d - rho) / sigma)) in f64,
@@ -131,9 +132,9 @@ __device__ __forceinline__ void directed_edge(const DsDesc& D, int64_t i, int64_
 __device__ __forceinline__ double directed_edge64(const DsDesc& D, int64_t i, int64_t g, int t,
                                                   double d, double rho, double sigma,
                                                   uint32_t sentinel, uint32_t* __restrict__ key1,
-                                                  uint32_t* __restrict__ pay1,
+                                                  uint32_t* __restrict__ indeg,
                                                   float* __restrict__ wval,
-                                                  uint32_t* __restrict__ srcrow) {
+                                                  uint32_t* __restrict__ erank) {
   const int64_t le = i * D.k + t;
   const int64_t e = D.edge_off + le;
   const uint32_t col = D.knn_idx[le];
@@ -143,9 +144,8 @@ __device__ __forceinline__ double directed_edge64(const DsDesc& D, int64_t i, in
   const bool keep = (int64_t)col != i && wf != 0.0f;
   if (key1) {
     key1[e] = keep ? (uint32_t)(D.row_off + col) : sentinel;
-    pay1[e] = (uint32_t)e;
     wval[e] = wf;
-    srcrow[e] = (uint32_t)g;
+    erank[e] = keep ? atomicAdd(indeg + D.row_off + col, 1u) : 0u;
   }
   return keep ? fabs((double)wf - w) / w : 0.0;
 }
@@ -184,8 +184,8 @@ __global__ __launch_bounds__(256) void smooth_knn_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
     const double* __restrict__ target_log2k, int mode, float* __restrict__ sigma,
     float* __restrict__ rho, double* __restrict__ sigma64, uint32_t* __restrict__ zany,
-    uint32_t sentinel, uint32_t* __restrict__ key1, uint32_t* __restrict__ pay1,
-    float* __restrict__ wval, uint32_t* __restrict__ srcrow) {
+    uint32_t sentinel, uint32_t* __restrict__ key1, uint32_t* __restrict__ indeg,
+    float* __restrict__ wval, uint32_t* __restrict__ erank) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total_rows) return;
   const int di = calib_find_ds(ds, nds, g);
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(256) void smooth_knn_kernel(
     }
     sigma[g] = s;
     if (sigma64) sigma64[g] = (double)s;
-    if (key1) HM_ROW_LOOP(0, { directed_edge(D, i, g, t, X(t), r, s, sentinel, key1, pay1, wval, srcrow); });
+    if (key1) HM_ROW_LOOP(0, { directed_edge(D, i, g, t, X(t), r, s, sentinel, key1, indeg, wval, erank); });
   } else {
     // the f64 chain from the f64 distances when they were kept, else from
     // the f32 distances widened
@@ -296,15 +296,15 @@ __global__ __launch_bounds__(256) void smooth_knn_kernel(
       double ed = 0.0, ew = 0.0;
       HM_ROW_LOOP(0, {
         ed = fmax(ed, rel_err(X(t), row64[t]));
-        ew = fmax(ew, directed_edge64(D, i, g, t, row64[t], rr, s, sentinel, key1, pay1, wval,
-                                      srcrow));
+        ew = fmax(ew, directed_edge64(D, i, g, t, row64[t], rr, s, sentinel, key1, indeg, wval,
+                                      erank));
       });
       D.err3[i * 3 + 0] = (float)ed;
       D.err3[i * 3 + 1] = (float)rel_err((float)s, s);
       D.err3[i * 3 + 2] = (float)ew;
     } else {
       const float sf = (float)s;
-      if (key1) HM_ROW_LOOP(0, { directed_edge(D, i, g, t, X(t), r, sf, sentinel, key1, pay1, wval, srcrow); });
+      if (key1) HM_ROW_LOOP(0, { directed_edge(D, i, g, t, X(t), r, sf, sentinel, key1, indeg, wval, erank); });
     }
 #undef X64
   }
@@ -416,8 +416,8 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
     const double* __restrict__ target_log2k, int mode, float* __restrict__ sigma,
     float* __restrict__ rho, double* __restrict__ sigma64, uint32_t* __restrict__ zany,
-    uint32_t sentinel, uint32_t* __restrict__ key1, uint32_t* __restrict__ pay1,
-    float* __restrict__ wval, uint32_t* __restrict__ srcrow) {
+    uint32_t sentinel, uint32_t* __restrict__ key1, uint32_t* __restrict__ indeg,
+    float* __restrict__ wval, uint32_t* __restrict__ erank) {
   static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0 && 256 % G == 0, "group size");
   const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const int l = (int)(threadIdx.x % G);
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
       sigma[g] = s;
       if (sigma64) sigma64[g] = (double)s;
     }
-    if (key1 && l < k) directed_edge(D, i, g, l, x, r, s, sentinel, key1, pay1, wval, srcrow);
+    if (key1 && l < k) directed_edge(D, i, g, l, x, r, s, sentinel, key1, indeg, wval, erank);
   } else {
     // the f64 chain from the f64 distances when they were kept (lane t
     // holds distance t in f64), else from the f32 distances widened
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
       double ed = 0.0, ew = 0.0;
       if (l < k) {
         ed = rel_err(x, x64);
-        ew = directed_edge64(D, i, g, l, x64, rr, s, sentinel, key1, pay1, wval, srcrow);
+        ew = directed_edge64(D, i, g, l, x64, rr, s, sentinel, key1, indeg, wval, erank);
       }
       for (int o = G / 2; o > 0; o >>= 1) {
         ed = fmax(ed, __shfl_xor(ed, o, G));
@@ -552,7 +552,7 @@ __global__ __launch_bounds__(256) void smooth_knn_group_kernel(
         D.err3[i * 3 + 2] = (float)ew;
       }
     } else if (key1 && l < k) {
-      directed_edge(D, i, g, l, x, r, (float)s, sentinel, key1, pay1, wval, srcrow);
+      directed_edge(D, i, g, l, x, r, (float)s, sentinel, key1, indeg, wval, erank);
     }
   }
 }
@@ -616,8 +616,8 @@ __global__ __launch_bounds__(256) void calib_fixup_kernel(const DsDesc* __restri
 __global__ void directed_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_edges,
                                 const float* __restrict__ sigma, const float* __restrict__ rho,
                                 uint32_t sentinel, uint32_t* __restrict__ key1,
-                                uint32_t* __restrict__ pay1, float* __restrict__ wval,
-                                uint32_t* __restrict__ srcrow) {
+                                uint32_t* __restrict__ indeg, float* __restrict__ wval,
+                                uint32_t* __restrict__ erank) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= total_edges) return;
   int lo = 0, hi = nds - 1;
@@ -630,7 +630,7 @@ __global__ void directed_kernel(const DsDesc* __restrict__ ds, int nds, int64_t 
   const int64_t i = le / D.k;
   const int64_t g = D.row_off + i;
   directed_edge(D, i, g, (int)(le - i * D.k), D.knn_dist[le], rho[g], sigma[g], sentinel, key1,
-                pay1, wval, srcrow);
+                indeg, wval, erank);
 }
 
 // Standalone expf over an array (parity tests of the device expf).

@@ -55,10 +55,12 @@ __device__ __forceinline__ double row_sum15(double x) {
 }
 
 // Thread per global row: sequential f64 fold of x_f^2 in feature order.
+// zero_g (optional): a per-row count cleared for a later kernel of the run.
 __global__ void rownorm_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
-                               float* __restrict__ nz_g) {
+                               float* __restrict__ nz_g, uint32_t* __restrict__ zero_g) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total_rows) return;
+  if (zero_g) zero_g[g] = 0u;
   const int di = find_ds(ds, nds, g);
   const DsDesc& D = ds[di];
   const int64_t i = g - D.row_off;

@@ -17,7 +17,7 @@
 
 namespace hm {
 // kernels (defined in the other translation units)
-__global__ void rownorm_kernel(const DsDesc*, int, int64_t, float*);
+__global__ void rownorm_kernel(const DsDesc*, int, int64_t, float*, uint32_t*);
 __global__ void prep_kernel(const DsDesc*, int, int64_t, const int64_t*, const float*, int, float*);
 __global__ void statmax_kernel(const DsDesc*, const float*, const int64_t*);
 template <int MODE, int PASS, int CW>
@@ -67,9 +67,6 @@ template <typename T>
 __global__ void scan_sums(T*, int64_t, T*);
 template <typename T>
 __global__ void scan_tiles(const T*, int64_t, const T*, T*);
-__global__ void radix_hist(const uint32_t*, int64_t, int, int64_t, uint32_t*);
-__global__ void radix_scatter(const uint32_t*, const uint32_t*, int64_t, int, int64_t,
-                              const uint32_t*, const uint32_t*, uint32_t*, uint32_t*);
 __global__ void lower_bounds(const uint32_t*, int64_t, int64_t, uint32_t*);
 __global__ void compact_kernel(const uint32_t*, const uint32_t*, int64_t, uint32_t*, int*);
 __global__ void fill_u32(uint32_t*, int64_t, uint32_t);
@@ -94,18 +91,19 @@ __global__ void sched_scatter_kernel(const uint32_t*, const uint32_t*, const flo
                                      uint32_t*, double*);
 __global__ void nonfinite_min_kernel(const uint64_t*, int, uint64_t*);
 __global__ void l2_normalise_kernel(const float*, int64_t, int, float*);
+__global__ void transpose_scatter_kernel(const DsDesc*, int, int64_t, const uint32_t*,
+                                         const uint32_t*, const float*, uint32_t, const uint32_t*,
+                                         uint32_t*, uint32_t*, float*);
 template <int G>
-__global__ void union_count_kernel(const DsDesc*, int, int64_t, int64_t, const uint32_t*,
-                                   const float*, const uint32_t*, const uint32_t*,
-                                   const uint32_t*, uint32_t, uint32_t*, uint64_t*, uint32_t*,
+__global__ void union_count_kernel(const DsDesc*, int, int64_t, const uint32_t*, const uint32_t*,
+                                   const uint32_t*, const float*, uint32_t, uint64_t*, uint32_t*,
                                    float*);
 template <int G>
 __global__ void union_write_kernel(const DsDesc*, int, int64_t, int64_t, const uint32_t*,
                                    const float*, const uint32_t*, const uint32_t*,
-                                   const uint32_t*, const uint32_t*, const uint32_t*,
-                                   const float*, const uint64_t*, const uint64_t*,
-                                   const int64_t*, uint32_t, uint32_t*, uint32_t*, float*,
-                                   uint64_t*);
+                                   const uint32_t*, const float*, const uint32_t*, const float*,
+                                   const uint64_t*, const int64_t*, uint32_t, uint64_t, uint32_t*,
+                                   uint32_t*, float*, uint64_t*);
 }  // namespace hm
 
 using namespace hm;
@@ -170,10 +168,6 @@ template <typename T>
 constexpr int64_t kScanSmall = (int64_t)kScanSmallThreads * 4 * kScanSmallQuads<T>;
 static_assert(kScanSmallThreads == 512 && kScanSmallQuads<uint32_t> == 32,
               "flag_*_items_kernel compact with scan_small_body<uint32_t, 512, 32>");
-constexpr int kRadixTile = 4096;
-// Radix passes of at most this many tiles skip the table scan (each scatter
-// workgroup reads its digit row of the table: tiles x 256 loads).
-constexpr int64_t kRadixOwnOffsets = 128;
 
 struct DsHost {
   int64_t n, n_pad, row_off, edge_off;
@@ -220,7 +214,7 @@ struct hm_job {
   float *x = nullptr, *xr = nullptr, *op32c = nullptr, *colbias = nullptr, *rstat = nullptr,
         *thr = nullptr,
         *smax = nullptr, *statpart = nullptr, *nz = nullptr, *knn_dist = nullptr, *sigma = nullptr, *rho = nullptr,
-        *wval = nullptr, *vals = nullptr;
+        *wval = nullptr, *vals = nullptr, *in_w = nullptr;
   uint16_t *op16 = nullptr, *op16lo = nullptr;
   double *rnorm = nullptr, *sigma64 = nullptr, *target = nullptr, *dist64 = nullptr;
   float* err3 = nullptr;  // FP64 chain: per-row rounding errors (n x 3)
@@ -229,9 +223,9 @@ struct hm_job {
            *indptr_g = nullptr, *total_nnz = nullptr, *indptr_l = nullptr, *sc64 = nullptr;
   uint32_t *ccnt = nullptr;
   uint32_t *knn_idx = nullptr, *flag = nullptr, *flagpos = nullptr, *list = nullptr,
-           *zany = nullptr, *key1 = nullptr, *pay1 = nullptr, *srcrow = nullptr, *k1s = nullptr,
-           *p1s = nullptr, *tmpk = nullptr, *tmpv = nullptr, *inptr = nullptr, *rows = nullptr,
-           *cols = nullptr, *table = nullptr, *toffs = nullptr, *sc32 = nullptr, *sc32t = nullptr;
+           *zany = nullptr, *key1 = nullptr, *erank = nullptr, *k1s = nullptr, *p1s = nullptr,
+           *tmpk = nullptr, *tmpv = nullptr, *inptr = nullptr, *rows = nullptr, *cols = nullptr,
+           *sc32 = nullptr;
   int *count = nullptr, *nitems_fb = nullptr, *xs_fb = nullptr;
   WorkItem *items_l1 = nullptr, *items_fb = nullptr, *items_l2 = nullptr;
   int* nitems_l2 = nullptr;
@@ -444,12 +438,11 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
   b += R * 20 + nds * 12;                            // sigma, rho, sigma64; flags, targets
   if ((stages & HM_STAGE_KNN) && opt.calib == HM_CALIB_F64) b += E * 8 + R * 12;  // dist64, err3
   if (stages & HM_STAGE_FUZZY) {
-    // key1 pay1 srcrow wval k1s p1s tmpk tmpv; inptr + indptr_g; cntu;
-    // total; rows cols vals; indptr_l; iptr_off
-    b += E * 8 * 4 + (R + 1) * 12 + R * 8 + 16 + 2 * E * 12 + (R + nds) * 8 + nds * 8;
-    b += 2 * 256 * (cdiv(E, kRadixTile) + 1) * 4;
+    // key1 erank wval in_dst in_src in_w ai av; inptr + indptr_g (its last
+    // entry the total); union sizes; rows cols vals; indptr_l; iptr_off
+    b += E * 4 * 8 + (R + 1) * 12 + R * 8 + 2 * E * 12 + (R + nds) * 8 + nds * 8;
   }
-  b += 2 * (cdiv(std::max<int64_t>(256 * (cdiv(E, kRadixTile) + 1), R + 1), kScanTile) + 2) * 8;
+  b += 2 * (cdiv(std::max<int64_t>(R + 1, E + 1), kScanTile) + 2) * 8;
   return b + (1u << 20);
 }
 
@@ -522,33 +515,6 @@ hm_status compact_flags(hm_job* J, const uint32_t* flag, int64_t n, uint32_t* li
                        (const uint32_t*)J->flagpos, n, list, count);
   }
   HM_LAUNCH(J->ctx);
-  return HM_OK;
-}
-
-hm_status radix_sort(hm_job* J, const uint32_t* kin, const uint32_t* vin, int64_t n, int bits,
-                     uint32_t* kout, uint32_t* vout) {
-  if (n <= 0) return HM_OK;
-  const int passes = std::max(1, (bits + 7) / 8);
-  const int64_t nt = cdiv(n, kRadixTile);
-  const uint32_t* sk = kin;
-  const uint32_t* sv = vin;
-  uint32_t* dk = (passes & 1) ? kout : J->tmpk;
-  uint32_t* dv = (passes & 1) ? vout : J->tmpv;
-  for (int p = 0; p < passes; ++p) {
-    hipLaunchKernelGGL(radix_hist, dim3((unsigned)nt), dim3(256), 0, J->stream, sk, n, 8 * p, nt,
-                       J->table);
-    // short tables: every scatter workgroup sums the table itself
-    const bool own = nt <= kRadixOwnOffsets;
-    if (!own)
-      HM_TRY(scan_excl<uint32_t>(J, J->table, 256 * nt, J->toffs, (uint32_t*)nullptr, J->sc32));
-    hipLaunchKernelGGL(radix_scatter, dim3((unsigned)nt), dim3(256), 0, J->stream, sk, sv, n,
-                       8 * p, nt, own ? (const uint32_t*)nullptr : (const uint32_t*)J->toffs,
-                       (const uint32_t*)J->table, dk, dv);
-    HM_LAUNCH(J->ctx);
-    sk = dk;
-    sv = dv;
-    if (dk == kout) { dk = J->tmpk; dv = J->tmpv; } else { dk = kout; dv = vout; }
-  }
   return HM_OK;
 }
 
@@ -655,8 +621,9 @@ hm_status enqueue_pipeline(hm_job* J) {
   mark(J, 0);
   if (J->stages & HM_STAGE_KNN) {
     // ---- prep ----
+    // (also clears the in-degree counts of the symmetrisation to follow)
     hipLaunchKernelGGL(rownorm_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, J->d_desc,
-                       nds, R, J->nz);
+                       nds, R, J->nz, (J->stages & HM_STAGE_FUZZY) ? J->inptr : nullptr);
     const int mode_bits = (J->mode == 0 ? 1 : 0) | 2;
     hipLaunchKernelGGL(prep_kernel, dim3(grid_for(J->total_pad, 16)), dim3(256), 0, J->stream,
                        J->d_desc, nds, J->total_pad, (const int64_t*)J->pad_off,
@@ -765,15 +732,19 @@ hm_status enqueue_pipeline(hm_job* J) {
     const bool edges = (J->stages & HM_STAGE_FUZZY) != 0;
     const uint32_t sent = (uint32_t)R;
     uint32_t* k1 = edges ? J->key1 : nullptr;
+    if (edges && !(J->stages & HM_STAGE_KNN)) {  // else cleared by rownorm_kernel
+      hipLaunchKernelGGL(fill_u32, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, J->inptr, R, 0u);
+      HM_LAUNCH(ctx);
+    }
 #define HM_CALIB_G(GS)                                                                         \
   hipLaunchKernelGGL((smooth_knn_group_kernel<GS>), dim3(grid_for(R * GS, 256)), dim3(256), 0, \
                      J->stream, J->d_desc, nds, R, (const double*)J->target, J->opt.calib,     \
-                     J->sigma, J->rho, J->sigma64, J->zany, sent, k1, J->pay1, J->wval,        \
-                     J->srcrow)
+                     J->sigma, J->rho, J->sigma64, J->zany, sent, k1, J->inptr, J->wval,       \
+                     J->erank)
 #define HM_CALIB_T(KM)                                                                         \
   hipLaunchKernelGGL((smooth_knn_kernel<KM>), dim3(grid_for(R, 256)), dim3(256), 0, J->stream, \
                      J->d_desc, nds, R, (const double*)J->target, J->opt.calib, J->sigma,       \
-                     J->rho, J->sigma64, J->zany, sent, k1, J->pay1, J->wval, J->srcrow)
+                     J->rho, J->sigma64, J->zany, sent, k1, J->inptr, J->wval, J->erank)
     if (by_thread) {
       if (G == 16) HM_CALIB_T(16);
       else if (G == 32) HM_CALIB_T(32);
@@ -795,36 +766,49 @@ hm_status enqueue_pipeline(hm_job* J) {
   mark(J, 5);
   if (J->stages & HM_STAGE_FUZZY) {
     const uint32_t sent = (uint32_t)R;
-    const int bits = bitlen((uint64_t)R);
     if (!(J->stages & HM_STAGE_CALIB)) {  // else written by the calibration
+      if (!(J->stages & HM_STAGE_KNN))  // else cleared by rownorm_kernel
+        hipLaunchKernelGGL(fill_u32, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, J->inptr, R,
+                           0u);
       hipLaunchKernelGGL(directed_kernel, dim3(grid_for(E, 256)), dim3(256), 0, J->stream,
                          J->d_desc, nds, E, (const float*)J->sigma, (const float*)J->rho, sent,
-                         J->key1, J->pay1, J->wval, J->srcrow);
+                         J->key1, J->inptr, J->wval, J->erank);
       HM_LAUNCH(ctx);
     }
-    HM_TRY(radix_sort(J, J->key1, J->pay1, E, bits, J->k1s, J->p1s));
-    // a group of G >= k lanes per row: union sizes, their scan, the entries
+    // transpose: in-degree counts -> bucket offsets (in place; inptr[R] =
+    // the kept edges), then every kept edge to its slot of its destination's
+    // bucket
+    HM_TRY(scan_excl<uint32_t>(J, J->inptr, R, J->inptr, J->inptr + R, J->sc32));
+    uint32_t* in_dst = J->k1s;
+    uint32_t* in_src = J->p1s;
+    float* in_w = J->in_w;
+    hipLaunchKernelGGL(transpose_scatter_kernel, dim3(grid_for(E, 256)), dim3(256), 0, J->stream,
+                       J->d_desc, nds, E, (const uint32_t*)J->key1, (const uint32_t*)J->erank,
+                       (const float*)J->wval, sent, (const uint32_t*)J->inptr, in_dst, in_src,
+                       in_w);
+    HM_LAUNCH(ctx);
+    // a group of G >= k lanes per row: per out-edge in-edge counts and
+    // reverse weights, the union sizes and their scan, then the entries
     const int GU = J->kmax <= 16 ? 16 : J->kmax <= 32 ? 32 : 64;
-    // (the radix sort's ping-pong buffers are free again: per out-edge
-    // in-edge counts and reverse weights)
     uint32_t* ai = J->tmpk;
     float* av = (float*)J->tmpv;
 #define HM_UNION_COUNT(GS)                                                                     \
   hipLaunchKernelGGL((union_count_kernel<GS>), dim3(grid_for(R * GS, 256)), dim3(256), 0,       \
-                     J->stream, J->d_desc, nds, R, E, (const uint32_t*)J->key1,                 \
-                     (const float*)J->wval, (const uint32_t*)J->k1s, (const uint32_t*)J->p1s,   \
-                     (const uint32_t*)J->srcrow, sent, J->inptr, J->cntu, ai, av)
+                     J->stream, J->d_desc, nds, R, (const uint32_t*)J->key1,                    \
+                     (const uint32_t*)J->inptr, (const uint32_t*)in_src, (const float*)in_w,    \
+                     sent, J->cntu, ai, av)
 #define HM_UNION_WRITE(GS)                                                                     \
   hipLaunchKernelGGL((union_write_kernel<GS>), dim3(grid_for(R * GS + E, 256)), dim3(256), 0,   \
                      J->stream, J->d_desc, nds, R, E, (const uint32_t*)J->key1,                 \
-                     (const float*)J->wval, (const uint32_t*)J->k1s, (const uint32_t*)J->p1s,   \
-                     (const uint32_t*)J->srcrow, (const uint32_t*)J->inptr,                     \
-                     (const uint32_t*)ai, (const float*)av, (const uint64_t*)J->indptr_g,       \
-                     (const uint64_t*)J->cntu, (const int64_t*)J->iptr_off, sent, J->rows,      \
-                     J->cols, J->vals, J->indptr_l)
+                     (const float*)J->wval, (const uint32_t*)J->inptr, (const uint32_t*)in_dst, \
+                     (const uint32_t*)in_src, (const float*)in_w, (const uint32_t*)ai,          \
+                     (const float*)av, (const uint64_t*)J->indptr_g,                            \
+                     (const int64_t*)J->iptr_off, sent, (uint64_t)(2 * E), J->rows, J->cols,    \
+                     J->vals, J->indptr_l)
     if (GU == 16) HM_UNION_COUNT(16); else if (GU == 32) HM_UNION_COUNT(32); else HM_UNION_COUNT(64);
     HM_LAUNCH(ctx);
-    HM_TRY(scan_excl<uint64_t>(J, J->cntu, R, J->indptr_g, J->total_nnz, J->sc64));
+    // (indptr_g[R] is the total: J->total_nnz points there)
+    HM_TRY(scan_excl<uint64_t>(J, J->cntu, R, J->indptr_g, J->indptr_g + R, J->sc64));
     if (GU == 16) HM_UNION_WRITE(16); else if (GU == 32) HM_UNION_WRITE(32); else HM_UNION_WRITE(64);
     HM_LAUNCH(ctx);
 #undef HM_UNION_COUNT
@@ -1173,18 +1157,17 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
                      (int64_t)nds, 0u);
   if (hipGetLastError() != hipSuccess) return bail(fail(ctx, HM_EDEVICE, "flag clear failed"));
   HM_A(J->target, nds);
-  const int64_t ntr = cdiv(E, kRadixTile) + 1;
   if (stages & HM_STAGE_FUZZY) {
-    HM_A(J->key1, E); HM_A(J->pay1, E); HM_A(J->srcrow, E); HM_A(J->wval, E);
+    HM_A(J->key1, E); HM_A(J->in_w, E); HM_A(J->erank, E); HM_A(J->wval, E);
     HM_A(J->k1s, E); HM_A(J->p1s, E);
     HM_A(J->tmpk, E); HM_A(J->tmpv, E);
     HM_A(J->inptr, R + 1);
-    HM_A(J->cntu, R); HM_A(J->indptr_g, R + 1); HM_A(J->total_nnz, 1);
+    HM_A(J->cntu, R); HM_A(J->indptr_g, R + 1);
+    J->total_nnz = J->indptr_g + R;
     HM_A(J->rows, 2 * E); HM_A(J->cols, 2 * E); HM_A(J->vals, 2 * E);
     HM_A(J->indptr_l, R + nds); HM_A(J->iptr_off, nds);
-    HM_A(J->table, 256 * ntr); HM_A(J->toffs, 256 * ntr);
   }
-  const int64_t scan_max = std::max<int64_t>(std::max<int64_t>(256 * ntr, R + 1), E + 1);
+  const int64_t scan_max = std::max<int64_t>(R + 1, E + 1);
   HM_A(J->sc32, cdiv(scan_max, kScanTile) + 2);
   HM_A(J->sc64, cdiv(scan_max, kScanTile) + 2);
 #undef HM_A

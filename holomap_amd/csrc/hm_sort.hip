@@ -1,10 +1,6 @@
-// Deterministic, atomic-free building blocks for the symmetrisation:
-//   * exclusive prefix scan (three-phase: tile sums, single-workgroup scan of
-//     the sums, tile scan + offset) for u32 and u64;
-//   * stable LSD radix sort of (u32 key, u32 payload) with 8-bit digits.
-//     Tile digit counts and in-tile ranks come from wave ballots (a lane's
-//     peers are the lanes with the same digit), never from atomics, so the
-//     output permutation is a fixed function of the input.
+// Deterministic, atomic-free building blocks: exclusive prefix scan
+// (three-phase: tile sums, single-workgroup scan of the sums, tile scan +
+// offset) for u32 and u64, flag compaction, fills and sums.
 #include "hm_scan.hpp"
 
 namespace hm {
@@ -103,132 +99,6 @@ template __global__ void scan_sums<uint32_t>(uint32_t*, int64_t, uint32_t*);
 template __global__ void scan_sums<uint64_t>(uint64_t*, int64_t, uint64_t*);
 template __global__ void scan_tiles<uint32_t>(const uint32_t*, int64_t, const uint32_t*, uint32_t*);
 template __global__ void scan_tiles<uint64_t>(const uint64_t*, int64_t, const uint64_t*, uint64_t*);
-
-// ---------------------------------------------------------------------------
-// Radix sort. Tile = 4096 elements = 4 waves x 16 rounds x 64 lanes; element
-// order inside a tile is (wave, round, lane), i.e. index order, so ranks
-// computed in that order keep the sort stable.
-// ---------------------------------------------------------------------------
-#define RX_TILE 4096
-
-__device__ __forceinline__ uint64_t match_digit(uint32_t dig, bool valid) {
-  uint64_t peers = __ballot(valid);
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const bool bit = (dig >> b) & 1u;
-    const uint64_t bb = __ballot(bit);
-    peers &= bit ? bb : ~bb;
-  }
-  return valid ? peers : 0ull;
-}
-
-// table[digit * ntiles + tile] = count of `digit` in the tile
-__global__ __launch_bounds__(256) void radix_hist(const uint32_t* __restrict__ keys, int64_t n,
-                                                  int shift, int64_t ntiles,
-                                                  uint32_t* __restrict__ table) {
-  __shared__ uint32_t cnt[4][256];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int t = threadIdx.x; t < 1024; t += 256) (&cnt[0][0])[t] = 0;
-  __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * RX_TILE + wave * 1024;
-  uint32_t digs[16];  // all loads first: one memory round trip, not 16
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int64_t idx = base + r * 64 + lane;
-    digs[r] = idx < n ? (keys[idx] >> shift) & 255u : 0u;
-  }
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const bool valid = base + r * 64 + lane < n;
-    const uint32_t dig = digs[r];
-    const uint64_t peers = match_digit(dig, valid);
-    const int leader = peers ? (int)__ffsll((long long)peers) - 1 : 64;
-    if (valid && lane == leader) cnt[wave][dig] += (uint32_t)__popcll(peers);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  }
-  __syncthreads();
-  const int dg = threadIdx.x;
-  table[(int64_t)dg * ntiles + blockIdx.x] = cnt[0][dg] + cnt[1][dg] + cnt[2][dg] + cnt[3][dg];
-}
-
-// offs = exclusive scan of table (digit-major)
-// offs = exclusive scan of the digit-major table, or nullptr: then each
-// workgroup derives its own offsets from the table (its earlier tiles of a
-// digit plus the totals of the smaller digits), for short tables, saving
-// the scan launch.
-__global__ __launch_bounds__(256) void radix_scatter(const uint32_t* __restrict__ kin,
-                                                     const uint32_t* __restrict__ vin, int64_t n,
-                                                     int shift, int64_t ntiles,
-                                                     const uint32_t* __restrict__ offs,
-                                                     const uint32_t* __restrict__ table,
-                                                     uint32_t* __restrict__ kout,
-                                                     uint32_t* __restrict__ vout) {
-  __shared__ uint32_t cnt[4][256];
-  __shared__ uint32_t wbase[4][256];
-  __shared__ uint32_t dsum[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int t = threadIdx.x; t < 1024; t += 256) (&cnt[0][0])[t] = 0;
-  __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * RX_TILE + wave * 1024;
-  uint32_t key[16], val[16], rank[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {  // all loads first: one memory round trip
-    const int64_t idx = base + r * 64 + lane;
-    key[r] = idx < n ? kin[idx] : 0u;
-    val[r] = idx < n ? vin[idx] : 0u;
-  }
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int64_t idx = base + r * 64 + lane;
-    const bool valid = idx < n;
-    const uint32_t dig = (key[r] >> shift) & 255u;
-    const uint64_t peers = match_digit(dig, valid);
-    const uint32_t before = cnt[wave][dig];  // earlier rounds of this wave
-    rank[r] = before + (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const int leader = peers ? (int)__ffsll((long long)peers) - 1 : 64;
-    if (valid && lane == leader) cnt[wave][dig] = before + (uint32_t)__popcll(peers);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  }
-  __syncthreads();
-  {
-    const int dg = threadIdx.x;
-    uint32_t run;
-    if (offs) {
-      run = offs[(int64_t)dg * ntiles + blockIdx.x];
-    } else {
-      const uint32_t* row = table + (int64_t)dg * ntiles;
-      uint32_t before = 0, tot = 0;
-#pragma unroll 8
-      for (int64_t t = 0; t < ntiles; ++t) {
-        const uint32_t c = row[t];
-        before += t < (int64_t)blockIdx.x ? c : 0u;
-        tot += c;
-      }
-      const uint32_t incl = wave_scan_incl(tot);  // digits of this wave
-      if (lane == 63) dsum[wave] = incl;
-      __syncthreads();
-      uint32_t wb = 0;
-      for (int w = 0; w < wave; ++w) wb += dsum[w];
-      run = wb + incl - tot + before;
-    }
-    for (int w = 0; w < 4; ++w) {
-      wbase[w][dg] = run;
-      run += cnt[w][dg];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int64_t idx = base + r * 64 + lane;
-    if (idx < n) {
-      const uint32_t dig = (key[r] >> shift) & 255u;
-      const uint32_t pos = wbase[wave][dig] + rank[r];
-      kout[pos] = key[r];
-      vout[pos] = val[r];
-    }
-  }
-}
 
 // lower_bound of every value v in [0, nv) in a sorted key array
 __global__ void lower_bounds(const uint32_t* __restrict__ sorted, int64_t n, int64_t nv,
