@@ -117,9 +117,15 @@ def sweep_roofline(n, d, precision, stage, workload_name):
     t_sweep = stage["sweep"] * 1e-3
     peak = PEAK.get(precision, PEAK["bf16"])
     ach = flops / t_collect if t_collect > 0 else 0.0
-    qres = precision == "bf16" and 2 * (-(-d // 64) * 64) <= 384
-    kname = "hm::gram_sweep_qres_kernel<*, 1>" if qres else \
-        f"hm::gram_sweep_kernel<{ {'bf16': 0, 'fp32': 1, 'bf16x3': 2}.get(precision, 0)}, 1>"
+    rowbytes = 2 * (-(-d // 64) * 64)  # bf16 operand row
+    if precision == "bf16" and rowbytes <= 256:
+        kname = "hm::gram_sweep_qreg_kernel<1>"
+    elif precision == "bf16" and rowbytes <= 384:
+        kname = "hm::gram_sweep_qres_kernel<hm::QresCfgT<8, 3, 384>, 1>"
+    elif precision == "bf16":
+        kname = "hm::gram_sweep_kernel<0, 1, 256>"
+    else:
+        kname = f"hm::gram_sweep_kernel<{ {'fp32': 1, 'bf16x3': 2}.get(precision, 1)}, 1, 128>"
     return {"bound": "mfma", "kernel": f"{kname} (collect pass, 1 launch/step)",
             "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
             "frac": ach / peak, "traffic": sweep_traffic(workload_name, precision),

@@ -321,10 +321,12 @@ __device__ __forceinline__ void sweep_epilogue(const floatx16 (&acc)[NCT], float
   } else {
     // A branch per position (compare, exec save, branch, exec restore in
     // sequence) costs ~75 cycles each, although hits are rare (about one
-    // per lane and block). Instead: a tile's 16 hit bits first (plain VALU),
-    // then one pass per hit of the lane (positions ascending, the order the
-    // lists keep), the value picked by a bitwise select tree on the position
-    // (a tree of conditional moves on its bits; no register indexing).
+    // per lane and block); so does a scalar branch per position on the
+    // wave's hit mask (measured slower). Instead: a tile's 16 hit bits first
+    // (plain VALU), then one pass per hit of the lane (positions ascending,
+    // the order the lists keep), the value picked by a bitwise select tree
+    // on the position (a tree of conditional moves on its bits; no register
+    // indexing).
 #pragma unroll
     for (int t = 0; t < NCT; ++t) {
       uint32_t m = 0;
@@ -406,21 +408,43 @@ __device__ __forceinline__ void sweep_finish(const float (&gmax)[4][16], float T
       }
       if (myvalid && half == 0) D.thr[(size_t)myrow * D.nsplit + split] = scale * m;
     } else {
-      uint32_t lo = 0u, hi = 0xffffffffu;
-      for (int itb = 0; itb < 32; ++itb) {
+      // bisection on the order-preserving images of -key' (ascending in the
+      // key), bracketed by the pair's smallest and largest image: hi keeps
+      // >= C values at or below it, the C-th smallest stays >= lo. It stops
+      // once the bracket is 2^10 images wide; T from hi is then at most
+      // that many ulps above the C-th smallest group minimum (a few more
+      // keys listed) and still has >= C groups at or below it.
+      uint32_t img[4][16];
+      uint32_t lo = 0xffffffffu, hi = 0u;
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int a = 0; a < 16; ++a) {
+          img[t][a] = ordered_u32(-gmax[t][a]);
+          lo = img[t][a] < lo ? img[t][a] : lo;
+          hi = img[t][a] > hi ? img[t][a] : hi;
+        }
+      {
+        const uint32_t plo = (uint32_t)__shfl_xor((int)lo, 32, 64);
+        const uint32_t phi = (uint32_t)__shfl_xor((int)hi, 32, 64);
+        lo = plo < lo ? plo : lo;
+        hi = phi > hi ? phi : hi;
+      }
+      // both lanes of a pair hold the same bracket: they iterate together
+      while (hi - lo > 1024u) {
         const uint32_t mid = lo + (hi - lo) / 2;
         int c = 0;
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
-          for (int a = 0; a < 16; ++a) c += ordered_u32(-gmax[t][a]) <= mid ? 1 : 0;
+          for (int a = 0; a < 16; ++a) c += img[t][a] <= mid ? 1 : 0;
         c += __shfl_xor(c, 32, 64);
         if (c >= C) hi = mid; else lo = mid + 1;
       }
-      // lo == hi: the smallest image with >= C values at or below it (the
-      // image of +inf when the split has fewer than C columns); T = -s * it
+      // hi: the image of +inf when the split has fewer than C columns;
+      // T = -s * the value
       if (myvalid && half == 0)
-        D.thr[(size_t)myrow * D.nsplit + split] = -scale * unordered_f32(lo);
+        D.thr[(size_t)myrow * D.nsplit + split] = -scale * unordered_f32(hi);
     }
   } else {
     if (myvalid) {
@@ -757,14 +781,209 @@ __global__ __launch_bounds__(Cfg::NW * 64, 2) void gram_sweep_qres_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// bf16 sweep with the query operand in registers, for rows of at most
+// QregCfg::QMAXB bytes (128 bf16 features). A wave's B operand (its 32
+// queries, chunk 2s + half of slice c for k-step s) is read from LDS once per
+// item into eight registers of 16 bytes, so a stage's LDS reads are the
+// candidate slices only, and the LDS the query rows would occupy holds two
+// more ring slots: three stages are in flight while one is computed (a
+// 2-slot ring leaves each copy one stage's MFMAs to land, less than its
+// round trip). Same tiles, MFMA order, epilogue and lists as
+// gram_sweep_qres_kernel.
+// ---------------------------------------------------------------------------
+// One 128-byte slice of a block: the wave's 32 queries (registers) x 128
+// candidate columns (LDS), four k-steps of 16 features. The candidate reads
+// of step s + 2 go out with the MFMAs of step s (as in sweep_mma<0>).
+__device__ __forceinline__ void qreg_mma(floatx16 (&acc)[4], const uint8_t* cand,
+                                         const bf16x8_t (&qr)[4], int q, int half, int swz) {
+  constexpr int LA = 2;
+  bf16x8_t ah[4][4];
+#pragma unroll
+  for (int s = 0; s < 4 + LA; ++s) {
+    if (s >= LA) {
+      // reads of steps s - LA + 1 .. min(s, 4) - 1 may stay in flight
+      const int keep = (s < 4 ? s : 4) - (s - LA) - 1;
+      if (keep >= 1) __builtin_amdgcn_s_waitcnt(0xc07f | (4 << 8));  // lgkmcnt(4)
+      else __builtin_amdgcn_s_waitcnt(0xc07f);                       // lgkmcnt(0)
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (s < 4) {
+      const int cs = ((2 * s + half) ^ swz) * 16;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) ah[s][t] = *(const bf16x8_t*)(cand + (t * 32 + q) * 128 + cs);
+    }
+    if (s >= LA) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[s - LA][t], qr[s - LA], acc[t], 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int PASS>
+__global__ __launch_bounds__(QregCfg::NW * 64, 2) void gram_sweep_qreg_kernel(
+    const DsDesc* __restrict__ ds_arr, const WorkItem* __restrict__ items,
+    const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist,
+    int sample) {
+  using Cfg = QregCfg;
+  constexpr int NI = Cfg::NI, STAGE = Cfg::STAGE, QW = Cfg::QW, VM = Cfg::VM;
+  static_assert(Cfg::NSTAGE == 4, "ring slot (g + 2) % 4, three stages ahead");
+  __shared__ __attribute__((aligned(1024))) uint8_t smem[Cfg::LDS];
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int q = lane & 31, half = lane >> 5;
+  const int nitems = nitems_dev ? *nitems_dev : nitems_host;
+  const int cb_even = ((lane & 7) ^ (lane >> 3)) * 16;  // copy side, even 8-row group
+  const int swz = row_swz(q);                           // read side
+  // stage g lives in ring slot (g + 2) % 4: stages 0 and 1 are copied into
+  // slots 2 and 3 while the query rows occupy slots 0 and 1
+  auto slot = [](int g) { return (g + 2) & 3; };
+
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem W = items[it];
+    const DsDesc& D = ds_arr[W.ds];
+    const int64_t n = D.n;
+    const int rowbytes = D.kp16 * 2;  // <= QMAXB (the runtime routes longer rows elsewhere)
+    const float scale = D.metric == 1 ? -1.0f : -2.0f;
+    const int64_t span = ((n + (int64_t)D.nsplit * 128 - 1) / ((int64_t)D.nsplit * 128)) * 128;
+    const int64_t cbeg = (int64_t)W.split * span;
+    const int64_t cend = cbeg + span < n ? cbeg + span : n;
+    const int nchunk = rowbytes / 128;  // 1 or 2
+    // the threshold pass of a sampled sweep visits every bstride-th block
+    const int bstride = (PASS == 0 && sample) ? D.pstride : 1;
+    const int csel = (PASS == 0 && sample) ? D.C0 : D.C;
+    const int nblk_all = cend > cbeg ? (int)((cend - cbeg + 127) / 128) : 0;
+    const int nblk = (nblk_all + bstride - 1) / bstride;
+    const int total = nblk * nchunk;
+    const int64_t blkbytes = (int64_t)128 * rowbytes * bstride;  // between swept blocks
+    const int64_t bcols = (int64_t)128 * bstride;
+    const uint8_t* op = (const uint8_t*)D.op16;
+
+    // candidate copies: wave w copies groups w*NI + i (8 rows each) of a block
+    const uint8_t* csrc[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int rb = wave * NI + i;  // uniform
+      csrc[i] = op + (cbeg + rb * 8 + (lane >> 3)) * rowbytes + ((rb & 1) ? (cb_even ^ 16) : cb_even);
+    }
+    const float* bsrc = D.colbias + cbeg + (wave & 1) * 64 + lane;
+    const int64_t blim = cend - cbeg - ((wave & 1) * 64 + lane);
+
+    const int qslot = 32 * wave + q;
+    const bool myvalid = qslot < W.qcnt;
+    const int64_t myrow =
+        myvalid ? (qlist ? (int64_t)qlist[W.qbeg + qslot] - D.row_off : (int64_t)(W.qbeg + qslot))
+                : 0;
+    const int list = W.split * 2 + half;
+    gptr_u64* dst = (gptr_u64*)(D.cand + ((size_t)myrow * D.nlist + list) * D.cout);
+    float T = __builtin_nanf("");  // no query: compares false (T holds T / s)
+    int cnt = 0;
+    float gmax[4][16];
+    if constexpr (PASS == 0) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int a = 0; a < 16; ++a) gmax[t][a] = -INFINITY;
+    } else {
+      if (myvalid) T = sweep_threshold(D, myrow, scale);
+    }
+
+    HM_LDS_BARRIER();  // the previous item's ring slots are no longer read
+    // the item's query rows into slots 0 and 1 (slice c of query row r at
+    // c*QW*128 + r*128; missing queries read row 0, their results unused):
+    // wave w copies groups 4w + i
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int qg = wave * 4 + i;  // uniform
+      const int r = qg * 8 + (lane >> 3);
+      int64_t row = 0;
+      if (r < W.qcnt) row = qlist ? (int64_t)qlist[W.qbeg + r] - D.row_off : (int64_t)(W.qbeg + r);
+      const uint8_t* src = op + row * rowbytes + ((qg & 1) ? (cb_even ^ 16) : cb_even);
+      for (int c = 0; c < nchunk; ++c)
+        __builtin_amdgcn_global_load_lds(src + c * 128, (lds_void_t*)(smem + c * QW * 128 + qg * 1024),
+                                         16, 0, 0);
+    }
+    if (wave < 2)
+      __builtin_amdgcn_global_load_lds(0 < blim ? bsrc : &kBiasOut,
+                                       (lds_void_t*)(smem + Cfg::BIAS0 + wave * 256), 4, 0, 0);
+    // stages 0 and 1 (the block's candidate slice and the next block's biases)
+    int bi = 0, ci = 0;  // block and slice of the next stage to issue
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      if (g < total) {
+        qres_issue<Cfg>(smem + slot(g) * STAGE, csrc, bsrc, blim, bi * blkbytes + ci * 128,
+                        (int64_t)(bi + 1) * bcols, wave);
+        if (++ci == nchunk) { ci = 0; ++bi; }
+      }
+    }
+    // the query rows and first biases were issued before the stages
+    if (total >= 2) vm_wait<2 * VM>();
+    else if (total == 1) vm_wait<VM>();
+    else vm_wait<0>();
+    HM_LDS_BARRIER();  // query rows and the first block's biases are in LDS
+    bf16x8_t qreg[2][4];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int cs = ((2 * s + half) ^ swz) * 16;
+        qreg[c][s] = c < nchunk ? *(const bf16x8_t*)(smem + c * QW * 128 + qslot * 128 + cs)
+                                : bf16x8_t{};
+      }
+    floatx16 acc[4];
+    sweep_acc_start(acc, smem + Cfg::BIAS0, half);
+    HM_LDS_BARRIER();  // every wave holds its queries: slots 0 and 1 are free
+    if (2 < total) {
+      qres_issue<Cfg>(smem + slot(2) * STAGE, csrc, bsrc, blim, bi * blkbytes + ci * 128,
+                      (int64_t)(bi + 1) * bcols, wave);
+      if (++ci == nchunk) { ci = 0; ++bi; }
+    }
+
+    int g = 0;
+    uint32_t colbase = (uint32_t)cbeg + 4 * half;
+    for (int b = 0; b < nblk; ++b) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        if (c < nchunk) {
+          // this wave's copies of stage g have landed (stages g+1 and g+2
+          // were issued after it); one barrier publishes stage g and retires
+          // stage g-1, whose slot gets stage g+3
+          const int after = total - 1 - g < 2 ? total - 1 - g : 2;
+          if (after >= 2) vm_wait<2 * VM>();
+          else if (after == 1) vm_wait<VM>();
+          else vm_wait<0>();
+          HM_LDS_BARRIER();
+          if (g + 3 < total) {
+            qres_issue<Cfg>(smem + slot(g + 3) * STAGE, csrc, bsrc, blim, bi * blkbytes + ci * 128,
+                            (int64_t)(bi + 1) * bcols, wave);
+            if (++ci == nchunk) { ci = 0; ++bi; }
+          }
+          qreg_mma(acc, smem + slot(g) * STAGE, qreg[c], q, half, swz);
+          ++g;
+        }
+      }
+      sweep_epilogue<PASS>(acc, gmax, colbase, T, D.cout, dst, cnt);
+      sweep_acc_start(acc, smem + slot(g - 1) * STAGE + Cfg::BIAS, half);
+      colbase += (uint32_t)bcols;
+    }
+    sweep_finish<PASS>(gmax, T, cnt, myvalid, myrow, D, W.split, half, scale, csel);
+  }
+}
+
+template __global__ void gram_sweep_qreg_kernel<0>(const DsDesc*, const WorkItem*, const int*, int,
+                                                   const uint32_t*, int);
+template __global__ void gram_sweep_qreg_kernel<1>(const DsDesc*, const WorkItem*, const int*, int,
+                                                   const uint32_t*, int);
+
 #define HM_QRES_INST(CFG, PASS)                                                                \
   template __global__ void gram_sweep_qres_kernel<CFG, PASS>(const DsDesc*, const WorkItem*,      \
                                                              const int*, int, const uint32_t*,   \
                                                              int);
 HM_QRES_INST(QresWide, 0)
 HM_QRES_INST(QresWide, 1)
-HM_QRES_INST(QresPair, 0)
-HM_QRES_INST(QresPair, 1)
 #undef HM_QRES_INST
 
 #define HM_INST(MODE, PASS, CW)                                                       \

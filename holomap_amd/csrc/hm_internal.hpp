@@ -64,7 +64,8 @@ __host__ __device__ constexpr int sweep_threads(int mode) { return sweep_rows(mo
 // bf16 operand rows (kp16 * 2 bytes) up to this length are swept with the
 // item's query rows resident in LDS (gram_sweep_qres_kernel).
 constexpr int kQresRowBytes = 384;
-// ... and up to this length by two 128-query workgroups per CU instead.
+// ... and up to this length with the queries in registers instead
+// (gram_sweep_qreg_kernel, two 128-query workgroups per CU).
 constexpr int kQresPairRowBytes = 256;
 // Neighbour counts: the MFMA candidate sweep serves k <= kSweepMaxK (more
 // neighbours than its 128 selection groups per row can certify are left to
@@ -98,11 +99,27 @@ struct QresCfgT {
 // Query-resident bf16 sweep configurations (gram_sweep_qres_kernel).
 // One workgroup per CU: 256 queries, 3-slot ring, rows up to 384 bytes.
 using QresWide = QresCfgT<8, 3, kQresRowBytes>;
-// Two workgroups per CU, each with its own barrier, so one's copies, LDS
-// reads and epilogue overlap the other's MFMAs: 128 queries, 2-slot ring,
-// rows up to 256 bytes.
-using QresPair = QresCfgT<4, 2, kQresPairRowBytes>;
-static_assert(QresWide::LDS <= 160 * 1024 && 2 * QresPair::LDS <= 160 * 1024, "LDS");
+static_assert(QresWide::LDS <= 160 * 1024, "LDS");
+
+// bf16 sweep with the query operand in registers (gram_sweep_qreg_kernel),
+// for rows of at most 256 bytes: two 128-query workgroups per CU, a 4-slot
+// ring of candidate stages (three in flight while one is computed); the
+// item's query rows pass through ring slots 0 and 1 on their way to the
+// registers.
+struct QregCfg {
+  static constexpr int NW = 4, QW = 32 * NW;
+  static constexpr int BIAS = 128 * 128;             // after the candidate rows
+  static constexpr int STAGE = BIAS + NW / 2 * 512;  // + NW/2 copies of the biases
+  static constexpr int NSTAGE = 4;
+  static constexpr int NI = 16 / NW;                 // candidate copy groups per wave
+  static constexpr int VM = NI + 1;                  // vmcnt entries per stage and wave
+  static constexpr int BIAS0 = NSTAGE * STAGE;       // the item's first block's biases
+  static constexpr int LDS = BIAS0 + 512;
+  static constexpr int QMAXB = kQresPairRowBytes;    // row bytes held
+  static_assert(QMAXB / 128 * QW * 128 <= 2 * STAGE, "query rows fit ring slots 0 and 1");
+  static_assert(QW / 8 == 4 * NW, "four query copy groups per wave and slice");
+};
+static_assert(2 * QregCfg::LDS <= 160 * 1024, "two workgroups per CU");
 static_assert(QresWide::NW * 64 == sweep_threads(0), "the runtime launches sweep_threads(0)");
 
 

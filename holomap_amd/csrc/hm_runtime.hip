@@ -26,6 +26,9 @@ __global__ void gram_sweep_kernel(const DsDesc*, const WorkItem*, const int*, in
 template <class Cfg, int PASS>
 __global__ void gram_sweep_qres_kernel(const DsDesc*, const WorkItem*, const int*, int,
                                        const uint32_t*, int);
+template <int PASS>
+__global__ void gram_sweep_qreg_kernel(const DsDesc*, const WorkItem*, const int*, int,
+                                       const uint32_t*, int);
 template <int QPW, int QCAP>
 __global__ void exact_scan_kernel(const DsDesc*, const WorkItem*, const int*, int,
                                   const uint32_t*, const int*, int, int, uint64_t*);
@@ -193,7 +196,7 @@ struct hm_job {
   std::vector<DsDesc> hdesc;
   int64_t total_rows = 0, total_pad = 0, total_edges = 0;
   int kmax = 0, mode = 0, xs = 1, fb_cap = 0, n_items_l1 = 0;
-  // items_l1 holds three consecutive lists: query-resident sweep with two
+  // items_l1 holds three consecutive lists: queries in registers with two
   // workgroups per CU, query-resident with one, streaming (kSweepLists)
   int n_items_list[3] = {0, 0, 0};
   int esc_mode = -1;  // sweep mode of the escalation pass (-1: none)
@@ -526,8 +529,9 @@ int64_t sweep_grid(const hm_ctx* ctx, int mode, int64_t items, int per_cu = 1) {
 }
 
 // The three lists of the primary sweep: bf16 rows of <= kQresPairRowBytes
-// (two 128-query workgroups per CU, queries in LDS), of <= kQresRowBytes
-// (one 256-query workgroup, queries in LDS), and the rest (streaming).
+// (two 128-query workgroups per CU, queries in registers), of <=
+// kQresRowBytes (one 256-query workgroup, queries in LDS), and the rest
+// (streaming).
 int sweep_list_of(int mode, int kp16) {
   if (mode == 0 && kp16 * 2 <= kQresPairRowBytes) return 0;
   if (mode == 0 && kp16 * 2 <= kQresRowBytes) return 1;
@@ -596,7 +600,16 @@ hm_status launch_sweep_l1(hm_job* J) {
                          J->stream, J->d_desc, it, nullptr, N, nullptr, 1);                    \
     HM_LAUNCH(ctx);                                                                            \
   } while (0)
-    if (n0 > 0) HM_QRES(QresPair, n0, kListPerCu[0]);
+    if (n0 > 0) {
+      const unsigned grid = (unsigned)sweep_grid(ctx, 0, n0, kListPerCu[0]);
+      if (pass == 0)
+        hipLaunchKernelGGL(gram_sweep_qreg_kernel<0>, dim3(grid), dim3(QregCfg::NW * 64), 0,
+                           J->stream, J->d_desc, it, nullptr, n0, nullptr, 1);
+      else
+        hipLaunchKernelGGL(gram_sweep_qreg_kernel<1>, dim3(grid), dim3(QregCfg::NW * 64), 0,
+                           J->stream, J->d_desc, it, nullptr, n0, nullptr, 1);
+      HM_LAUNCH(ctx);
+    }
     it += n0;
     if (n1 > 0) HM_QRES(QresWide, n1, kListPerCu[1]);
     it += n1;
