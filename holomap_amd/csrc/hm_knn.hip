@@ -378,52 +378,78 @@ __device__ __forceinline__ float f32_round_up(double v) {
 #define RR_LISTS 10   // lists per row at most (two per column split)
 #define RR_QROW 512   // query rows of at most this many floats are staged in LDS
 
-// ref_dist64 with the query row in LDS (read at the same address by every
-// lane) and the candidate row fetched 16 floats at a time, so each batch
-// costs one memory round trip; same operations in the same order. (32
-// floats per batch need more registers than six waves per SIMD leave and
-// spill: measured slower at N = 20,000 x d = 128 and N = 50,000 x d = 784.)
-__device__ __forceinline__ double rr_dist64(const float* qs, const float* __restrict__ b, int d4,
-                                         int metric, double rna, double rnb) {
+// ref_dist64 for the re-rank: the query row qa (read at the same address by
+// every lane: LDS when staged there, else global memory through the scalar
+// path) against the lane's candidate row b, fetched 8 floats at a time with
+// the next 8 in flight while these are folded (two buffers of 8: the
+// register cost of one 16-float batch, which six waves per SIMD allow; 32
+// floats spill). Same operations in the same order as ref_dist64.
+__device__ __forceinline__ void rr_fold4(double& s, const float4 x, const float4 y, int metric) {
+  if (metric == 0) {
+    double t;
+    t = (double)x.x - (double)y.x; s = s + t * t;
+    t = (double)x.y - (double)y.y; s = s + t * t;
+    t = (double)x.z - (double)y.z; s = s + t * t;
+    t = (double)x.w - (double)y.w; s = s + t * t;
+  } else {
+    s = s + (double)x.x * (double)y.x;
+    s = s + (double)x.y * (double)y.y;
+    s = s + (double)x.z * (double)y.z;
+    s = s + (double)x.w * (double)y.w;
+  }
+}
+// The candidate row is read through a global-address-space pointer
+// (global_load rather than flat_load: the row pointer comes from the
+// descriptor, whose address space the compiler cannot infer); the query
+// pointer keeps the address space it is called with (LDS or global).
+typedef __attribute__((address_space(1))) const float gfloat;
+typedef __attribute__((address_space(1))) const float4 gfloat4;
+__device__ __forceinline__ float4 ld4(gfloat* p) { return *(gfloat4*)p; }
+__device__ __forceinline__ float4 ld4(const float* p) { return *(const float4*)p; }
+// Query sources of the fold: f32 values read at the same address by every
+// lane, in LDS (staged) or in global memory.
+struct QueryL {
+  const float* q;
+  __device__ __forceinline__ void fold(double& s, int f, const float4 y, int metric) const {
+    rr_fold4(s, *(const float4*)(q + f), y, metric);
+  }
+};
+struct QueryG {
+  gfloat* q;
+  __device__ __forceinline__ void fold(double& s, int f, const float4 y, int metric) const {
+    rr_fold4(s, ld4(q + f), y, metric);
+  }
+};
+template <class Q>
+__device__ __forceinline__ double rr_dist64(const Q qa, const float* __restrict__ bp, int d4,
+                                            int metric, double rna, double rnb) {
+  gfloat* b = (gfloat*)bp;
   double s = 0.0;
+  const int d8 = d4 & ~7;
+  float4 y0[2], y1[2];
+  if (d8 > 0) {
+    y0[0] = ld4(b);
+    y0[1] = ld4(b + 4);
+  }
   int f = 0;
-  for (; f + 16 <= d4; f += 16) {
-    float4 y[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) y[u] = *(const float4*)(b + f + 4 * u);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const float4 x = *(const float4*)(qs + f + 4 * u);
-      if (metric == 0) {
-        double t;
-        t = (double)x.x - (double)y[u].x; s = s + t * t;
-        t = (double)x.y - (double)y[u].y; s = s + t * t;
-        t = (double)x.z - (double)y[u].z; s = s + t * t;
-        t = (double)x.w - (double)y[u].w; s = s + t * t;
-      } else {
-        s = s + (double)x.x * (double)y[u].x;
-        s = s + (double)x.y * (double)y[u].y;
-        s = s + (double)x.z * (double)y[u].z;
-        s = s + (double)x.w * (double)y[u].w;
-      }
+  while (f < d8) {
+    if (f + 8 < d8) {
+      y1[0] = ld4(b + f + 8);
+      y1[1] = ld4(b + f + 12);
     }
-  }
-  for (; f < d4; f += 4) {
-    const float4 x = *(const float4*)(qs + f);
-    const float4 y = *(const float4*)(b + f);
-    if (metric == 0) {
-      double t;
-      t = (double)x.x - (double)y.x; s = s + t * t;
-      t = (double)x.y - (double)y.y; s = s + t * t;
-      t = (double)x.z - (double)y.z; s = s + t * t;
-      t = (double)x.w - (double)y.w; s = s + t * t;
-    } else {
-      s = s + (double)x.x * (double)y.x;
-      s = s + (double)x.y * (double)y.y;
-      s = s + (double)x.z * (double)y.z;
-      s = s + (double)x.w * (double)y.w;
+    qa.fold(s, f, y0[0], metric);
+    qa.fold(s, f + 4, y0[1], metric);
+    f += 8;
+    if (f >= d8) break;
+    if (f + 8 < d8) {
+      y0[0] = ld4(b + f + 8);
+      y0[1] = ld4(b + f + 12);
     }
+    qa.fold(s, f, y1[0], metric);
+    qa.fold(s, f + 4, y1[1], metric);
+    f += 8;
   }
+  if (f < d4) qa.fold(s, f, ld4(b + f), metric);  // d4 % 8 == 4
   if (metric == 0) return sqrt(s);
   return f64max0(1.0 - s / (rna * rnb));
 }
@@ -454,9 +480,17 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   const int64_t i = grow - D.row_off;
   const int k = D.k, L = D.nlist, CO = D.cout, d = D.d, kp = D.kp, metric = D.metric;
   const int d4 = (d + 3) & ~3;
-  uint32_t* oidx = D.knn_idx + i * k;
-  float* odst = D.knn_dist + i * k;
-  const double rna = metric == 1 ? D.rnorm[i] : 0.0;
+  // outputs and row data through global-address-space pointers (global
+  // rather than flat accesses, which would also count in lgkmcnt)
+  __attribute__((address_space(1))) uint32_t* oidx =
+      (__attribute__((address_space(1))) uint32_t*)(D.knn_idx + i * k);
+  __attribute__((address_space(1))) float* odst =
+      (__attribute__((address_space(1))) float*)(D.knn_dist + i * k);
+  __attribute__((address_space(1))) double* od64 =
+      (__attribute__((address_space(1))) double*)(D.knn_dist64 ? D.knn_dist64 + i * k : nullptr);
+  __attribute__((address_space(1))) uint32_t* gflag = (__attribute__((address_space(1))) uint32_t*)flag;
+  typedef __attribute__((address_space(1))) const double gf64;
+  const double rna = metric == 1 ? ((gf64*)D.rnorm)[i] : 0.0;
 
   if (metric == 1 && rna == 0.0) {
     // zero query: every cosine distance is max(NaN, 0) = 0, so the list is
@@ -464,16 +498,87 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     for (int t = lane; t < k; t += 64) {
       oidx[t] = (uint32_t)t;
       odst[t] = 0.0f;
-      if (D.knn_dist64) D.knn_dist64[i * k + t] = 0.0;
+      if (od64) od64[t] = 0.0;
     }
-    if (lane == 0) flag[grow] = 0;
+    if (lane == 0) gflag[grow] = 0;
     return;
   }
 
+  // loads that do not depend on the keys go out first, so their round trips
+  // overlap the list counts' and the keys': the row's bound statistics and
+  // the dataset's maxima, and the query row staged in LDS
+  float st[HM_NSTAT], mx[HM_NSTAT];
+  {
+    gfloat4* sp = (gfloat4*)(D.rstat + i * HM_NSTAT);
+    gfloat4* mp = (gfloat4*)D.smax;
+#pragma unroll
+    for (int h = 0; h < HM_NSTAT / 4; ++h) {
+      const float4 a = sp[h], b = mp[h];
+      st[4 * h] = a.x; st[4 * h + 1] = a.y; st[4 * h + 2] = a.z; st[4 * h + 3] = a.w;
+      mx[4 * h] = b.x; mx[4 * h + 1] = b.y; mx[4 * h + 2] = b.z; mx[4 * h + 3] = b.w;
+    }
+  }
+  uint32_t* cl = (uint32_t*)lds;  // <= 512 plausible candidate columns
+  uint32_t* xd = cl + 512;        // their exact distances (f32 bits)
+  float* qs = (float*)(xd + 512);  // the query row (d4 <= RR_QROW)
+  const float* qa = D.xr + (size_t)i * kp;
+  const bool qlds = d4 <= RR_QROW;
+  if (qlds)
+    for (int f = 4 * lane; f < d4; f += 256) *(float4*)(qs + f) = ld4((gfloat*)qa + f);
+
+  // ---- error bound of the candidate keys for this row ----
+  const double u = 0x1p-24;
+  double E;
+  double dotmax;
+  if (mode == 2) {
+    const double gam = 2.0 * 3.0 * kp * u;
+    const double dot_err = (double)st[ST_LN] * mx[ST_LN] + (double)st[ST_HLN] * mx[ST_RN] +
+                           (double)st[ST_RN] * mx[ST_XO];
+    const double acc_err = gam * ((double)st[ST_HN] * mx[ST_HN] + (double)st[ST_HN] * mx[ST_LN] +
+                                  (double)st[ST_LN] * mx[ST_HN]);
+    E = dot_err + acc_err;
+    dotmax = (double)st[ST_XO] * mx[ST_XO] * 1.01;
+  } else if (mode == 1) {
+    // f32 operand w: ||w|| = XO (rounded up), its rounding error
+    // ||w_exact - w|| <= u/(1-u) ||w|| componentwise (whatever the job's
+    // primary mode stored in the XN/EN slots)
+    const double gam = 2.0 * kp * u;
+    const double xn = (double)st[ST_XO], mxn = (double)mx[ST_XO];
+    const double en = xn * u * 1.0001, men = mxn * u * 1.0001;
+    const double dot_err = en * (mxn + men) + xn * men;
+    const double acc_err = gam * xn * mxn;
+    E = dot_err + acc_err;
+    dotmax = xn * mxn * (1.0 + gam);
+  } else {
+    const double gam = 2.0 * kp * u;
+    const double dot_err = (double)st[ST_EN] * ((double)mx[ST_XN] + mx[ST_EN]) +
+                           (double)st[ST_XN] * mx[ST_EN];
+    const double acc_err = gam * (double)st[ST_XN] * mx[ST_XN];
+    E = dot_err + acc_err;
+    dotmax = (double)st[ST_XN] * mx[ST_XN] * (1.0 + gam);
+  }
+  const double eps_exact = (d + 8) * 0x1p-52;  // f64 fold of the exact distance
+  double nzi = 0.0;
+  if (metric == 0) {
+    nzi = (double)st[ST_NZ];
+    E = 2.0 * E + (double)st[ST_NZ] * 0x1p-23 + (double)mx[ST_NZ] * 0x1p-23 +
+        u * ((double)mx[ST_NZ] + 2.0 * dotmax);
+    // the sweep's accumulators start at -||x_j||^2 / 2: every accumulation
+    // step (at most one per product, three products per element for bf16x3)
+    // also rounds at that magnitude
+    E += (mode == 2 ? 3.0 : 1.0) * 2.0 * (kp + 1) * u * (double)mx[ST_NZ];
+    E = E * (1.0 + 1e-3) + 1e-30;
+  } else {
+    E = E * (1.0 + 1e-3) + 1e-12;
+  }
+
+
   // ---- 1. the stored keys of all lists: slot e = lane + 64 m of the packed
   // order (list after list) holds list l = the last with off[l] <= e ----
-  const uint64_t* src = D.cand + (size_t)i * L * CO;
-  const uint32_t* cnt = D.ccnt + (size_t)i * L;
+  typedef __attribute__((address_space(1))) const uint64_t gu64;
+  typedef __attribute__((address_space(1))) const uint32_t gu32;
+  gu64* src = (gu64*)(D.cand + (size_t)i * L * CO);
+  gu32* cnt = (gu32*)(D.ccnt + (size_t)i * L);
   int off[RR_LISTS + 1];  // uniform
   off[0] = 0;
 #pragma unroll
@@ -481,13 +586,13 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   const int nvalid = off[RR_LISTS];  // <= L * CO <= 512
   uint64_t taumin = ~0ull;
   for (int s = 0; s < L; ++s) {
-    const uint64_t t = D.ctau[(size_t)i * L + s];
+    const uint64_t t = ((gu64*)D.ctau)[(size_t)i * L + s];
     taumin = t < taumin ? t : taumin;
   }
   if (nvalid < k) {
     // cannot happen for n >= k (the lists keep min(C, columns) keys);
     // leave the row to the exact scan rather than decode an empty slot
-    if (lane == 0) flag[grow] = 1;
+    if (lane == 0) gflag[grow] = 1;
     return;
   }
   // the sweep stores (key' bits, column) with key' = key / s; kh holds the
@@ -536,54 +641,6 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     if (c >= k) bhi = mid; else blo = mid + 1;
   }
 
-  // ---- error bound of the candidate keys for this row ----
-  const float* st = D.rstat + i * HM_NSTAT;
-  const float* mx = D.smax;
-  const double u = 0x1p-24;
-  double E;
-  double dotmax;
-  if (mode == 2) {
-    const double gam = 2.0 * 3.0 * kp * u;
-    const double dot_err = (double)st[ST_LN] * mx[ST_LN] + (double)st[ST_HLN] * mx[ST_RN] +
-                           (double)st[ST_RN] * mx[ST_XO];
-    const double acc_err = gam * ((double)st[ST_HN] * mx[ST_HN] + (double)st[ST_HN] * mx[ST_LN] +
-                                  (double)st[ST_LN] * mx[ST_HN]);
-    E = dot_err + acc_err;
-    dotmax = (double)st[ST_XO] * mx[ST_XO] * 1.01;
-  } else if (mode == 1) {
-    // f32 operand w: ||w|| = XO (rounded up), its rounding error
-    // ||w_exact - w|| <= u/(1-u) ||w|| componentwise (whatever the job's
-    // primary mode stored in the XN/EN slots)
-    const double gam = 2.0 * kp * u;
-    const double xn = (double)st[ST_XO], mxn = (double)mx[ST_XO];
-    const double en = xn * u * 1.0001, men = mxn * u * 1.0001;
-    const double dot_err = en * (mxn + men) + xn * men;
-    const double acc_err = gam * xn * mxn;
-    E = dot_err + acc_err;
-    dotmax = xn * mxn * (1.0 + gam);
-  } else {
-    const double gam = 2.0 * kp * u;
-    const double dot_err = (double)st[ST_EN] * ((double)mx[ST_XN] + mx[ST_EN]) +
-                           (double)st[ST_XN] * mx[ST_EN];
-    const double acc_err = gam * (double)st[ST_XN] * mx[ST_XN];
-    E = dot_err + acc_err;
-    dotmax = (double)st[ST_XN] * mx[ST_XN] * (1.0 + gam);
-  }
-  const double eps_exact = (d + 8) * 0x1p-52;  // f64 fold of the exact distance
-  double nzi = 0.0;
-  if (metric == 0) {
-    nzi = (double)st[ST_NZ];
-    E = 2.0 * E + (double)st[ST_NZ] * 0x1p-23 + (double)mx[ST_NZ] * 0x1p-23 +
-        u * ((double)mx[ST_NZ] + 2.0 * dotmax);
-    // the sweep's accumulators start at -||x_j||^2 / 2: every accumulation
-    // step (at most one per product, three products per element for bf16x3)
-    // also rounds at that magnitude
-    E += (mode == 2 ? 3.0 : 1.0) * 2.0 * (kp + 1) * u * (double)mx[ST_NZ];
-    E = E * (1.0 + 1e-3) + 1e-30;
-  } else {
-    E = E * (1.0 + 1e-3) + 1e-12;
-  }
-
   // ---- prune: candidates that cannot beat the k key-best ones ----
   const double kk = (double)unordered_f32(bhi);  // >= the k-th smallest key
   double thr;  // a candidate with lower bound >= thr is strictly beyond the top-k
@@ -598,9 +655,6 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     thr = (double)next_up(f32_round_up(ub));
   }
   // ---- 3. the plausible candidates, compacted (column list in lds) ----
-  uint32_t* cl = (uint32_t*)lds;  // <= 512 columns
-  uint32_t* xd = cl + 512;        // their exact distances (f32 bits)
-  float* qs = (float*)(xd + 512); // the query row (d4 <= RR_QROW)
   int np = 0;
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
@@ -620,14 +674,10 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   }
   if (np < k) {
     // cannot happen (the k key-best candidates pass)
-    if (lane == 0) flag[grow] = 1;
+    if (lane == 0) gflag[grow] = 1;
     return;
   }
-  const float* qa = D.xr + (size_t)i * kp;
-  const bool qlds = d4 <= RR_QROW;
-  if (qlds)
-    for (int f = 4 * lane; f < d4; f += 256) *(float4*)(qs + f) = *(const float4*)(qa + f);
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the query row is in LDS
 
   // ---- 4. exact distances, one lane per candidate (the f64 value of the
   // lane's first candidate is kept for the FP64 calibration) ----
@@ -639,15 +689,16 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     if ((int64_t)j >= D.n) {
       badid = true;  // never produced by the sweep; guard the row read
     } else if ((int64_t)j != i) {
-      const double rnb = metric == 1 ? D.rnorm[j] : 0.0;
-      dist = qlds ? rr_dist64(qs, D.xr + (size_t)j * kp, d4, metric, rna, rnb)
-                  : ref_dist64(qa, D.xr + (size_t)j * kp, d4, metric, rna, rnb);
+      const double rnb = metric == 1 ? ((gf64*)D.rnorm)[j] : 0.0;
+      // two inlined copies, so the query reads keep their address space
+      dist = qlds ? rr_dist64(QueryL{qs}, D.xr + (size_t)j * kp, d4, metric, rna, rnb)
+                  : rr_dist64(QueryG{(gfloat*)qa}, D.xr + (size_t)j * kp, d4, metric, rna, rnb);
     }
     if (e == lane) first64 = dist;
     xd[e] = __float_as_uint((float)dist);
   }
   if (__ballot(badid) != 0ull) {
-    if (lane == 0) flag[grow] = 1;  // recomputed by the exact scan
+    if (lane == 0) gflag[grow] = 1;  // recomputed by the exact scan
     return;
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -663,8 +714,7 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     if (e < np && rk < k) {
       oidx[rk] = (uint32_t)mine;
       odst[rk] = __uint_as_float((uint32_t)(mine >> 32));
-      if (D.knn_dist64)
-        D.knn_dist64[i * k + rk] = m == 0 ? first64 : pair_dist64(D, i, (uint32_t)mine);
+      if (od64) od64[rk] = m == 0 ? first64 : pair_dist64(D, i, (uint32_t)mine);
       if (rk == k - 1) kth = mine;
     }
   }
@@ -688,7 +738,7 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
       cert = (1.0 + tf - E - eps_exact) >= nu;
     }
   }
-  if (lane == 0) flag[grow] = cert ? 0u : 1u;
+  if (lane == 0) gflag[grow] = cert ? 0u : 1u;
 }
 
 // All rows (rlist == nullptr: rows 0 .. total_rows-1, one per wave), or the
@@ -701,7 +751,7 @@ __global__ __launch_bounds__(256, 6) void rerank_kernel(const DsDesc* __restrict
                                                         const uint32_t* __restrict__ rlist,
                                                         const int* __restrict__ rcount) {
   __shared__ uint64_t cs[4][768];
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: row data in SGPRs
   const int64_t cnt = rlist ? (int64_t)*rcount : total_rows;
   for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < cnt; p += (int64_t)gridDim.x * 4) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // previous row's LDS use is over
