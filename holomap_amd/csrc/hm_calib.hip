@@ -150,10 +150,6 @@ __device__ __forceinline__ double directed_edge64(const DsDesc& D, int64_t i, in
   return keep ? fabs((double)wf - w) / w : 0.0;
 }
 
-// Relative error of an f32 value against the f64 value it rounds (0 for 0).
-__device__ __forceinline__ double rel_err(float f, double v) {
-  return v != 0.0 ? fabs((double)f - v) / fabs(v) : 0.0;
-}
 
 // The calibration kernels also write the row's directed edges when key1 is
 // set (the symmetrisation follows): a row's sigma is final here except for
@@ -567,48 +563,15 @@ HM_CALIB_G_INST(32)
 HM_CALIB_G_INST(64)
 #undef HM_CALIB_G_INST
 
-// Applies the global-mean floor to rows with rho == 0. One workgroup per
-// dataset; the mean is the reference's left-to-right f64 fold, computed by
-// one lane and only if the dataset has such a row.
+// Applies the global-mean floor to rows with rho == 0 (calib_fixup_ds), one
+// workgroup per dataset, when no symmetrisation follows (else its last
+// kernel carries these workgroups).
 __global__ __launch_bounds__(256) void calib_fixup_kernel(const DsDesc* __restrict__ ds, int mode,
                                                           float* __restrict__ sigma,
                                                           double* __restrict__ sigma64,
                                                           const float* __restrict__ rho,
                                                           uint32_t* __restrict__ zany) {
-  if (!zany[blockIdx.x]) return;  // no row of this dataset needs the floor
-  const DsDesc& D = ds[blockIdx.x];
-  __shared__ double mean_s;
-  if (threadIdx.x == 0) {
-    const int64_t m = D.n * D.k;
-    double s = 0.0;
-    if (D.knn_dist64)
-      for (int64_t t = 0; t < m; ++t) s = s + D.knn_dist64[t];
-    else
-      for (int64_t t = 0; t < m; ++t) s = s + (double)D.knn_dist[t];
-    mean_s = s / (double)m;
-  }
-  __syncthreads();
-  const double mean_all = mean_s;
-  for (int64_t i = threadIdx.x; i < D.n; i += 256) {
-    const int64_t g = D.row_off + i;
-    if (rho[g] > 0.0f) continue;  // rows with a non-zero distance have their floor
-    if (mode == CALIB_REF_F32) {
-      const float fl = 1e-3f * (float)mean_all;
-      float s = sigma[g];
-      if (s < fl) s = fl;
-      sigma[g] = s;
-      if (sigma64) sigma64[g] = (double)s;
-    } else {
-      double s = sigma64 ? sigma64[g] : (double)sigma[g];
-      const double fl = 1e-3 * mean_all;
-      if (s < fl) s = fl;
-      if (sigma64) sigma64[g] = s;
-      sigma[g] = (float)s;
-      if (D.err3) D.err3[i * 3 + 1] = (float)rel_err((float)s, s);
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) zany[blockIdx.x] = 0u;  // cleared for the next run
+  calib_fixup_ds(ds[blockIdx.x], (int)blockIdx.x, mode, sigma, sigma64, rho, zany);
 }
 
 // Directed membership strengths (src/fuzzy.rs:31-48), one thread per edge,

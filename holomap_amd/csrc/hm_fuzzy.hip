@@ -192,7 +192,10 @@ __global__ __launch_bounds__(256) void union_count_kernel(
 //  * threads [R*G, R*G + E): one per in-edge slot p < off[R] (in-edge
 //    i -> r) without the out-edge r -> i (entry (r, i), v = 0, vt = w),
 //    counting against the row's out-edges and its bucket, so rows of any
-//    in-degree spread over as many threads as they have in-edges.
+//    in-degree spread over as many threads as they have in-edges;
+//  * workgroups from fix_wg on (when zany is set): the calibration's
+//    global-mean floor of dataset blockIdx.x - fix_wg (calib_fixup_ds), one
+//    launch fewer than a kernel of its own.
 template <int G>
 __global__ __launch_bounds__(256) void union_write_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_rows, int64_t total_edges,
@@ -202,7 +205,14 @@ __global__ __launch_bounds__(256) void union_write_kernel(
     const uint32_t* __restrict__ ai, const float* __restrict__ av,
     const uint64_t* __restrict__ indptr_g, const int64_t* __restrict__ iptr_off,
     uint32_t sentinel, uint64_t cap, uint32_t* __restrict__ rows, uint32_t* __restrict__ cols,
-    float* __restrict__ vals, uint64_t* __restrict__ indptr_l) {
+    float* __restrict__ vals, uint64_t* __restrict__ indptr_l, uint32_t fix_wg, int calib_mode,
+    float* __restrict__ sigma, double* __restrict__ sigma64, const float* __restrict__ rho,
+    uint32_t* __restrict__ zany) {
+  if (zany && blockIdx.x >= fix_wg) {  // uniform over the workgroup
+    const int dsi = (int)(blockIdx.x - fix_wg);
+    calib_fixup_ds(ds[dsi], dsi, calib_mode, sigma, sigma64, rho, zany);
+    return;
+  }
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gt < total_rows * G) {
     const int64_t r = gt / G;
@@ -310,7 +320,7 @@ __global__ __launch_bounds__(256) void union_write_kernel(
       const DsDesc*, int, int64_t, int64_t, const uint32_t*, const float*, const uint32_t*,     \
       const uint32_t*, const uint32_t*, const float*, const uint32_t*, const float*,            \
       const uint64_t*, const int64_t*, uint32_t, uint64_t, uint32_t*, uint32_t*, float*,        \
-      uint64_t*);
+      uint64_t*, uint32_t, int, float*, double*, const float*, uint32_t*);
 HM_UNION_INST(16)
 HM_UNION_INST(32)
 HM_UNION_INST(64)

@@ -156,6 +156,58 @@ enum : int { CALIB_REF_F32 = 0, CALIB_F64 = 1 };
 __device__ __forceinline__ uint32_t f2u(float x) { return __float_as_uint(x); }
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 
+// Relative error of an f32 value against the f64 value it rounds (0 for 0).
+__device__ __forceinline__ double rel_err(float f, double v) {
+  return v != 0.0 ? fabs((double)f - v) / fabs(v) : 0.0;
+}
+
+// The calibration's global-mean floor for one dataset (src/smooth_knn.rs:
+// 70-75), by one 256-thread workgroup: sigma = max(sigma, 1e-3 * mean of all
+// kNN distances) for the rows with rho == 0, the mean being the reference's
+// left-to-right f64 fold by one lane, computed only if the dataset flagged
+// such a row (zany, cleared here for the next run). The rows' directed
+// weights do not depend on it (their distances all equal rho = 0, so every
+// strength is exp(0) = 1), which lets it run after the symmetrisation.
+__device__ __forceinline__ void calib_fixup_ds(const DsDesc& D, int dsi, int mode,
+                                               float* __restrict__ sigma,
+                                               double* __restrict__ sigma64,
+                                               const float* __restrict__ rho,
+                                               uint32_t* __restrict__ zany) {
+  if (!zany[dsi]) return;  // no row of this dataset needs the floor (uniform)
+  __shared__ double mean_s;
+  if (threadIdx.x == 0) {
+    const int64_t m = D.n * D.k;
+    double s = 0.0;
+    if (D.knn_dist64)
+      for (int64_t t = 0; t < m; ++t) s = s + D.knn_dist64[t];
+    else
+      for (int64_t t = 0; t < m; ++t) s = s + (double)D.knn_dist[t];
+    mean_s = s / (double)m;
+  }
+  __syncthreads();
+  const double mean_all = mean_s;
+  for (int64_t i = threadIdx.x; i < D.n; i += 256) {
+    const int64_t g = D.row_off + i;
+    if (rho[g] > 0.0f) continue;  // rows with a non-zero distance have their floor
+    if (mode == CALIB_REF_F32) {
+      const float fl = 1e-3f * (float)mean_all;
+      float s = sigma[g];
+      if (s < fl) s = fl;
+      sigma[g] = s;
+      if (sigma64) sigma64[g] = (double)s;
+    } else {
+      double s = sigma64 ? sigma64[g] : (double)sigma[g];
+      const double fl = 1e-3 * mean_all;
+      if (s < fl) s = fl;
+      if (sigma64) sigma64[g] = s;
+      sigma[g] = (float)s;
+      if (D.err3) D.err3[i * 3 + 1] = (float)rel_err((float)s, s);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) zany[dsi] = 0u;  // cleared for the next run
+}
+
 // Order-preserving map of a float to uint32 (total order, -0 < +0).
 __device__ __forceinline__ uint32_t ordered_u32(float x) {
   uint32_t u = __float_as_uint(x);

@@ -403,8 +403,11 @@ __device__ __forceinline__ void rr_fold4(double& s, const float4 x, const float4
 // descriptor, whose address space the compiler cannot infer); the query
 // pointer keeps the address space it is called with (LDS or global).
 typedef __attribute__((address_space(1))) const float gfloat;
-typedef __attribute__((address_space(1))) const float4 gfloat4;
-__device__ __forceinline__ float4 ld4(gfloat* p) { return *(gfloat4*)p; }
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld4(gfloat* p) {
+  const f4v v = *(__attribute__((address_space(1))) const f4v*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
 __device__ __forceinline__ float4 ld4(const float* p) { return *(const float4*)p; }
 // Query sources of the fold: f32 values read at the same address by every
 // lane, in LDS (staged) or in global memory.
@@ -509,11 +512,11 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   // the dataset's maxima, and the query row staged in LDS
   float st[HM_NSTAT], mx[HM_NSTAT];
   {
-    gfloat4* sp = (gfloat4*)(D.rstat + i * HM_NSTAT);
-    gfloat4* mp = (gfloat4*)D.smax;
+    gfloat* sp = (gfloat*)(D.rstat + i * HM_NSTAT);
+    gfloat* mp = (gfloat*)D.smax;
 #pragma unroll
     for (int h = 0; h < HM_NSTAT / 4; ++h) {
-      const float4 a = sp[h], b = mp[h];
+      const float4 a = ld4(sp + 4 * h), b = ld4(mp + 4 * h);
       st[4 * h] = a.x; st[4 * h + 1] = a.y; st[4 * h + 2] = a.z; st[4 * h + 3] = a.w;
       mx[4 * h] = b.x; mx[4 * h + 1] = b.y; mx[4 * h + 2] = b.z; mx[4 * h + 3] = b.w;
     }

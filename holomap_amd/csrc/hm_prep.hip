@@ -4,26 +4,20 @@
 //   * rownorm_kernel  (thread per row): the reference's sequential f64 fold
 //     sum_f x_f^2 (src/metric.rs:61-66 keeps na/nb as such folds), giving the
 //     cosine denominator factor sqrt(na) bit-exactly and the Euclidean key
-//     bias ||x||^2 rounded to f32;
+//     bias ||x||^2 rounded to f32 (a thread per row: every lane of a 16-lane
+//     group folding its row serially inside prep_kernel measured 4x slower);
 //   * prep_kernel     (16 lanes per row): a zero-padded f32 copy (n_pad x kp),
 //     the candidate-selection operand (euclid: the copy itself; cosine: the
-//     row scaled to unit length), its bf16 (RNE) image, and the statistics
-//     the certification bound needs (representation error norms);
-//   * statmax_kernel: per-dataset maxima of the statistics (max is exact and
-//     order-free): prep_kernel's workgroups write the maxima of their rows,
-//     one workgroup per dataset reduces them.
+//     row scaled to unit length), its bf16 (RNE) image, the statistics the
+//     certification bound needs (representation error norms), and their
+//     per-dataset maxima: each workgroup folds the maxima of its rows into
+//     smax by integer atomic maxima of the bit patterns (the statistics are
+//     non-negative floats, whose patterns order as their values; max is
+//     exact and order-free, so the result is the same on every run).
+//     rownorm_kernel clears smax first.
 #include "hm_internal.hpp"
 
 namespace hm {
-
-__device__ __forceinline__ int find_ds(const DsDesc* ds, int nds, int64_t grow) {
-  int lo = 0, hi = nds - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (ds[mid].row_off <= grow) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
   uint32_t u = __float_as_uint(f);
@@ -54,11 +48,23 @@ __device__ __forceinline__ double row_sum15(double x) {
   return x;
 }
 
+__device__ __forceinline__ int find_ds(const DsDesc* ds, int nds, int64_t grow) {
+  int lo = 0, hi = nds - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (ds[mid].row_off <= grow) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Thread per global row: sequential f64 fold of x_f^2 in feature order.
-// zero_g (optional): a per-row count cleared for a later kernel of the run.
+// zero_g (optional): a per-row count cleared for a later kernel of the run;
+// smax_g: the nds x HM_NSTAT statistics maxima, cleared for prep_kernel.
 __global__ void rownorm_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
-                               float* __restrict__ nz_g, uint32_t* __restrict__ zero_g) {
+                               float* __restrict__ nz_g, uint32_t* __restrict__ zero_g,
+                               uint32_t* __restrict__ smax_g) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < (int64_t)nds * HM_NSTAT) smax_g[g] = 0u;
   if (g >= total_rows) return;
   if (zero_g) zero_g[g] = 0u;
   const int di = find_ds(ds, nds, g);
@@ -92,55 +98,61 @@ __global__ void rownorm_kernel(const DsDesc* __restrict__ ds, int nds, int64_t t
   const_cast<double*>(D.rnorm)[i] = sqrt(s);
 }
 
-__device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int nds, int64_t pr,
+__device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int di, int64_t pr,
                                          const int64_t* __restrict__ pad_off,
-                                         const float* __restrict__ nz_g, int mode_bits,
-                                         int lane, float (&stout)[HM_NSTAT]);
+                                         const float* __restrict__ nz_g, int mode_bits, int lane,
+                                         float (&stout)[HM_NSTAT]);
 
 // 16 lanes (one DPP row) per (dataset, padded row), four rows per wave, so
 // a wave overlaps the memory round trips of four rows. Writes xr, op32
-// (cosine), op16 (+ op16lo), colbias and rstat. mode_bits: 1 = produce op16
-// (bf16 sweep), 2 = bf16x3 stats. A workgroup's 16 rows belong to one
-// dataset (padded row counts are multiples of 128); it also writes the
-// maxima of their statistics to part[blockIdx.x] (statmax_kernel reduces
-// them per dataset).
+// (cosine), op16 (+ op16lo), colbias and rstat from the row norms
+// rownorm_kernel left. mode_bits: 1 = produce op16 (bf16 sweep), 2 = bf16x3 stats. A
+// workgroup's 16 rows belong to one dataset (padded row counts are
+// multiples of 128); it folds the maxima of their statistics into the
+// dataset's smax (atomic maxima of the non-negative values' bit patterns).
 __global__ __launch_bounds__(256) void prep_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_pad_rows,
     const int64_t* __restrict__ pad_off /* nds+1 prefix of n_pad */,
-    const float* __restrict__ nz_g, int mode_bits, float* __restrict__ part) {
+    const float* __restrict__ nz_g, int mode_bits) {
   __shared__ float red[16][HM_NSTAT];
+  __shared__ int di_s;
+  if (threadIdx.x == 0) {  // the workgroup's dataset, by padded-row prefix
+    const int64_t pr0 = (int64_t)blockIdx.x * 16;
+    int lo = 0, hi = nds - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pad_off[mid] <= pr0) lo = mid; else hi = mid - 1;
+    }
+    di_s = lo;
+  }
+  __syncthreads();
+  const int di = di_s;
   const int64_t pr = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
   float st[HM_NSTAT];
 #pragma unroll
   for (int s = 0; s < HM_NSTAT; ++s) st[s] = 0.0f;
   if (pr < total_pad_rows)  // (always: the grid covers whole 16-row groups)
-    prep_row(ds, nds, pr, pad_off, nz_g, mode_bits, (int)(threadIdx.x & 15), st);
+    prep_row(ds, di, pr, pad_off, nz_g, mode_bits, (int)(threadIdx.x & 15), st);
   if ((threadIdx.x & 15) == 15) {
 #pragma unroll
     for (int s = 0; s < HM_NSTAT; ++s) red[threadIdx.x >> 4][s] = st[s];
   }
   __syncthreads();
   if (threadIdx.x < HM_NSTAT) {
-    float m = 0.0f;
+    float m = 0.0f;  // fmaxf: a NaN never enters (none arises from finite rows)
 #pragma unroll
     for (int r = 0; r < 16; ++r) m = fmaxf(m, red[r][threadIdx.x]);
-    part[(size_t)blockIdx.x * HM_NSTAT + threadIdx.x] = m;
+    atomicMax((uint32_t*)const_cast<float*>(ds[di].smax) + threadIdx.x, __float_as_uint(m));
   }
 }
 
-// One padded row by one wave (prep_kernel's body).
-__device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int nds, int64_t pr,
+// One padded row of dataset di by 16 lanes (prep_kernel's body).
+__device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int di, int64_t pr,
                                          const int64_t* __restrict__ pad_off,
-                                         const float* __restrict__ nz_g, int mode_bits,
-                                         int lane, float (&stout)[HM_NSTAT]) {
-  // dataset by padded-row prefix
-  int lo = 0, hi = nds - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (pad_off[mid] <= pr) lo = mid; else hi = mid - 1;
-  }
-  const DsDesc& D = ds[lo];
-  const int64_t i = pr - pad_off[lo];
+                                         const float* __restrict__ nz_g, int mode_bits, int lane,
+                                         float (&stout)[HM_NSTAT]) {
+  const DsDesc& D = ds[di];
+  const int64_t i = pr - pad_off[di];
   const int kp = D.kp, d = D.d;
   float* xr = const_cast<float*>(D.xr) + i * kp;
   float* op32 = const_cast<float*>(D.op32) + i * kp;
@@ -240,36 +252,6 @@ __device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int nds,
       for (int k = 0; k < HM_NSTAT; ++k) stout[k] = st[k];
     }
   }
-}
-
-// Per-dataset maxima of the row statistics (max is exact and order-free):
-// one workgroup per dataset reduces the partial maxima prep_kernel wrote for
-// the dataset's 16-row groups, pad_off[ds] / 16 .. pad_off[ds + 1] / 16.
-__global__ __launch_bounds__(256) void statmax_kernel(const DsDesc* __restrict__ ds,
-                                                      const float* __restrict__ part,
-                                                      const int64_t* __restrict__ pad_off) {
-  __shared__ float red[256][HM_NSTAT];
-  const DsDesc& D = ds[blockIdx.x];
-  const int64_t g0 = pad_off[blockIdx.x] / 16, g1 = pad_off[blockIdx.x + 1] / 16;
-  float m[HM_NSTAT];
-#pragma unroll
-  for (int s = 0; s < HM_NSTAT; ++s) m[s] = 0.0f;
-  for (int64_t g = g0 + threadIdx.x; g < g1; g += 256) {
-#pragma unroll
-    for (int s = 0; s < HM_NSTAT; ++s) m[s] = fmaxf(m[s], part[g * HM_NSTAT + s]);
-  }
-#pragma unroll
-  for (int s = 0; s < HM_NSTAT; ++s) red[threadIdx.x][s] = m[s];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-#pragma unroll
-      for (int s = 0; s < HM_NSTAT; ++s)
-        red[threadIdx.x][s] = fmaxf(red[threadIdx.x][s], red[threadIdx.x + w][s]);
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x < HM_NSTAT) const_cast<float*>(D.smax)[threadIdx.x] = red[0][threadIdx.x];
 }
 
 }  // namespace hm

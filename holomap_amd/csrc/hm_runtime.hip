@@ -17,9 +17,8 @@
 
 namespace hm {
 // kernels (defined in the other translation units)
-__global__ void rownorm_kernel(const DsDesc*, int, int64_t, float*, uint32_t*);
-__global__ void prep_kernel(const DsDesc*, int, int64_t, const int64_t*, const float*, int, float*);
-__global__ void statmax_kernel(const DsDesc*, const float*, const int64_t*);
+__global__ void rownorm_kernel(const DsDesc*, int, int64_t, float*, uint32_t*, uint32_t*);
+__global__ void prep_kernel(const DsDesc*, int, int64_t, const int64_t*, const float*, int);
 template <int MODE, int PASS, int CW>
 __global__ void gram_sweep_kernel(const DsDesc*, const WorkItem*, const int*, int, const uint32_t*,
                                   int);
@@ -106,7 +105,8 @@ __global__ void union_write_kernel(const DsDesc*, int, int64_t, int64_t, const u
                                    const float*, const uint32_t*, const uint32_t*,
                                    const uint32_t*, const float*, const uint32_t*, const float*,
                                    const uint64_t*, const int64_t*, uint32_t, uint64_t, uint32_t*,
-                                   uint32_t*, float*, uint64_t*);
+                                   uint32_t*, float*, uint64_t*, uint32_t, int, float*, double*,
+                                   const float*, uint32_t*);
 }  // namespace hm
 
 using namespace hm;
@@ -216,7 +216,7 @@ struct hm_job {
   DsDesc* d_desc = nullptr;
   float *x = nullptr, *xr = nullptr, *op32c = nullptr, *colbias = nullptr, *rstat = nullptr,
         *thr = nullptr,
-        *smax = nullptr, *statpart = nullptr, *nz = nullptr, *knn_dist = nullptr, *sigma = nullptr, *rho = nullptr,
+        *smax = nullptr, *nz = nullptr, *knn_dist = nullptr, *sigma = nullptr, *rho = nullptr,
         *wval = nullptr, *vals = nullptr, *in_w = nullptr;
   uint16_t *op16 = nullptr, *op16lo = nullptr;
   double *rnorm = nullptr, *sigma64 = nullptr, *target = nullptr, *dist64 = nullptr;
@@ -431,8 +431,7 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
     int64_t kp16_max = 0;
     for (auto& h : P.dsh) kp16_max = std::max<int64_t>(kp16_max, h.kp16);
     b += kWideSlackRows * kp16_max * 2;  // rows the wide collect pass may read past the end
-    b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + P.total_pad / 16 * HM_NSTAT * 4 +
-         R * 12;
+    b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + R * 12;
     b += P.sum_cand * 8 + R * 2 * kSplitCap * 12 + R * kSplitCap * 4;  // cand, ctau+ccnt, thr
     b += 520 * 8;                                                          // counter sums
     b += R * 12 + P.fb_cap * 16 + R * P.xs * P.kmax * 8;
@@ -635,14 +634,14 @@ hm_status enqueue_pipeline(hm_job* J) {
   if (J->stages & HM_STAGE_KNN) {
     // ---- prep ----
     // (also clears the in-degree counts of the symmetrisation to follow)
-    hipLaunchKernelGGL(rownorm_kernel, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, J->d_desc,
-                       nds, R, J->nz, (J->stages & HM_STAGE_FUZZY) ? J->inptr : nullptr);
     const int mode_bits = (J->mode == 0 ? 1 : 0) | 2;
+    hipLaunchKernelGGL(rownorm_kernel,
+                       dim3(grid_for(std::max<int64_t>(R, (int64_t)nds * HM_NSTAT), 256)), dim3(256),
+                       0, J->stream, J->d_desc, nds, R, J->nz,
+                       (J->stages & HM_STAGE_FUZZY) ? J->inptr : nullptr, (uint32_t*)J->smax);
     hipLaunchKernelGGL(prep_kernel, dim3(grid_for(J->total_pad, 16)), dim3(256), 0, J->stream,
                        J->d_desc, nds, J->total_pad, (const int64_t*)J->pad_off,
-                       (const float*)J->nz, mode_bits, J->statpart);
-    hipLaunchKernelGGL(statmax_kernel, dim3(nds), dim3(256), 0, J->stream, J->d_desc,
-                       (const float*)J->statpart, (const int64_t*)J->pad_off);
+                       (const float*)J->nz, mode_bits);
     HM_LAUNCH(ctx);
     mark(J, 1);
     // ---- candidate sweep + re-rank ----
@@ -772,8 +771,11 @@ hm_status enqueue_pipeline(hm_job* J) {
     }
 #undef HM_CALIB_G
 #undef HM_CALIB_T
-    hipLaunchKernelGGL(calib_fixup_kernel, dim3(nds), dim3(256), 0, J->stream, J->d_desc,
-                       J->opt.calib, J->sigma, J->sigma64, (const float*)J->rho, J->zany);
+    // the global-mean floor (it leaves the directed weights unchanged) rides
+    // on the symmetrisation's last kernel when one follows
+    if (!(J->stages & HM_STAGE_FUZZY))
+      hipLaunchKernelGGL(calib_fixup_kernel, dim3(nds), dim3(256), 0, J->stream, J->d_desc,
+                         J->opt.calib, J->sigma, J->sigma64, (const float*)J->rho, J->zany);
     HM_LAUNCH(ctx);
   }
   mark(J, 5);
@@ -810,14 +812,18 @@ hm_status enqueue_pipeline(hm_job* J) {
                      J->stream, J->d_desc, nds, R, (const uint32_t*)J->key1,                    \
                      (const uint32_t*)J->inptr, (const uint32_t*)in_src, (const float*)in_w,    \
                      sent, J->cntu, ai, av)
+    // workgroups past the entries' carry the calibration's global-mean floor
+    const bool fix = (J->stages & HM_STAGE_CALIB) != 0;
+    const unsigned wgrid = grid_for(R * GU + E, 256);
 #define HM_UNION_WRITE(GS)                                                                     \
-  hipLaunchKernelGGL((union_write_kernel<GS>), dim3(grid_for(R * GS + E, 256)), dim3(256), 0,   \
+  hipLaunchKernelGGL((union_write_kernel<GS>), dim3(wgrid + (fix ? (unsigned)nds : 0u)), dim3(256), 0, \
                      J->stream, J->d_desc, nds, R, E, (const uint32_t*)J->key1,                 \
                      (const float*)J->wval, (const uint32_t*)J->inptr, (const uint32_t*)in_dst, \
                      (const uint32_t*)in_src, (const float*)in_w, (const uint32_t*)ai,          \
                      (const float*)av, (const uint64_t*)J->indptr_g,                            \
                      (const int64_t*)J->iptr_off, sent, (uint64_t)(2 * E), J->rows, J->cols,    \
-                     J->vals, J->indptr_l)
+                     J->vals, J->indptr_l, wgrid, J->opt.calib, J->sigma, J->sigma64,           \
+                     (const float*)J->rho, fix ? J->zany : nullptr)
     if (GU == 16) HM_UNION_COUNT(16); else if (GU == 32) HM_UNION_COUNT(32); else HM_UNION_COUNT(64);
     HM_LAUNCH(ctx);
     // (indptr_g[R] is the total: J->total_nnz points there)
@@ -1123,7 +1129,6 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
     HM_A(J->colbias, P.total_pad);
     HM_A(J->rstat, R * HM_NSTAT);
     HM_A(J->smax, (int64_t)nds * HM_NSTAT);
-    HM_A(J->statpart, P.total_pad / 16 * HM_NSTAT);
     HM_A(J->nz, R);
     HM_A(J->rnorm, R);
     // sized for any sweep configuration (see plan_sweep)
@@ -1159,7 +1164,7 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
     HM_A(J->dist64, E);
     HM_A(J->err3, R * 3);
   }
-  HM_A(J->zany, nds);  // per-dataset flags, kept clear between runs by calib_fixup_kernel
+  HM_A(J->zany, nds);  // per-dataset flags, kept clear between runs by calib_fixup_ds
   // (a fill kernel: the scratch array keeps the last call's contents)
   if (J->op16) {  // zero bf16 operand rows past the last dataset (read, never used)
     const int64_t words = kWideSlackRows * kp16_max / 2;
