@@ -823,13 +823,13 @@ __device__ __forceinline__ void qreg_mma(floatx16 (&acc)[4], const uint8_t* cand
 }
 
 template <int PASS>
-__global__ __launch_bounds__(QregCfg::NW * 64, 2) void gram_sweep_qreg_kernel(
+__global__ __launch_bounds__(QregCfg::NW * 64, QregCfg::PER) void gram_sweep_qreg_kernel(
     const DsDesc* __restrict__ ds_arr, const WorkItem* __restrict__ items,
     const int* __restrict__ nitems_dev, int nitems_host, const uint32_t* __restrict__ qlist,
     int sample) {
   using Cfg = QregCfg;
   constexpr int NI = Cfg::NI, STAGE = Cfg::STAGE, QW = Cfg::QW, VM = Cfg::VM;
-  static_assert(Cfg::NSTAGE == 4, "ring slot (g + 2) % 4, three stages ahead");
+  constexpr int NS = Cfg::NSTAGE;
   __shared__ __attribute__((aligned(1024))) uint8_t smem[Cfg::LDS];
 
   const int tid = threadIdx.x, lane = tid & 63;
@@ -838,9 +838,9 @@ __global__ __launch_bounds__(QregCfg::NW * 64, 2) void gram_sweep_qreg_kernel(
   const int nitems = nitems_dev ? *nitems_dev : nitems_host;
   const int cb_even = ((lane & 7) ^ (lane >> 3)) * 16;  // copy side, even 8-row group
   const int swz = row_swz(q);                           // read side
-  // stage g lives in ring slot (g + 2) % 4: stages 0 and 1 are copied into
-  // slots 2 and 3 while the query rows occupy slots 0 and 1
-  auto slot = [](int g) { return (g + 2) & 3; };
+  // stage g lives in ring slot (g + 2) % NS: stages 0 .. NS - 3 are copied
+  // into slots 2 .. NS - 1 while the query rows occupy slots 0 and 1
+  auto slot = [](int g) { return (g + 2) % NS; };
 
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const WorkItem W = items[it];
@@ -909,10 +909,11 @@ __global__ __launch_bounds__(QregCfg::NW * 64, 2) void gram_sweep_qreg_kernel(
     if (wave < 2)
       __builtin_amdgcn_global_load_lds(0 < blim ? bsrc : &kBiasOut,
                                        (lds_void_t*)(smem + Cfg::BIAS0 + wave * 256), 4, 0, 0);
-    // stages 0 and 1 (the block's candidate slice and the next block's biases)
+    // stages 0 .. NS - 3 (the block's candidate slice and the next block's
+    // biases)
     int bi = 0, ci = 0;  // block and slice of the next stage to issue
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < NS - 2; ++g) {
       if (g < total) {
         qres_issue<Cfg>(smem + slot(g) * STAGE, csrc, bsrc, blim, bi * blkbytes + ci * 128,
                         (int64_t)(bi + 1) * bcols, wave);
@@ -920,9 +921,12 @@ __global__ __launch_bounds__(QregCfg::NW * 64, 2) void gram_sweep_qreg_kernel(
       }
     }
     // the query rows and first biases were issued before the stages
-    if (total >= 2) vm_wait<2 * VM>();
-    else if (total == 1) vm_wait<VM>();
-    else vm_wait<0>();
+    {
+      const int pre = total < NS - 2 ? total : NS - 2;
+      if (pre >= 2) vm_wait<2 * VM>();
+      else if (pre == 1) vm_wait<VM>();
+      else vm_wait<0>();
+    }
     HM_LDS_BARRIER();  // query rows and the first block's biases are in LDS
     bf16x8_t qreg[2][4];
 #pragma unroll
@@ -936,8 +940,8 @@ __global__ __launch_bounds__(QregCfg::NW * 64, 2) void gram_sweep_qreg_kernel(
     floatx16 acc[4];
     sweep_acc_start(acc, smem + Cfg::BIAS0, half);
     HM_LDS_BARRIER();  // every wave holds its queries: slots 0 and 1 are free
-    if (2 < total) {
-      qres_issue<Cfg>(smem + slot(2) * STAGE, csrc, bsrc, blim, bi * blkbytes + ci * 128,
+    if (NS - 2 < total) {
+      qres_issue<Cfg>(smem + slot(NS - 2) * STAGE, csrc, bsrc, blim, bi * blkbytes + ci * 128,
                       (int64_t)(bi + 1) * bcols, wave);
       if (++ci == nchunk) { ci = 0; ++bi; }
     }
@@ -948,17 +952,17 @@ __global__ __launch_bounds__(QregCfg::NW * 64, 2) void gram_sweep_qreg_kernel(
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         if (c < nchunk) {
-          // this wave's copies of stage g have landed (stages g+1 and g+2
-          // were issued after it); one barrier publishes stage g and retires
-          // stage g-1, whose slot gets stage g+3
-          const int after = total - 1 - g < 2 ? total - 1 - g : 2;
+          // this wave's copies of stage g have landed (stages g+1 ..
+          // g+NS-2 were issued after it); one barrier publishes stage g and
+          // retires stage g-1, whose slot gets stage g+NS-1
+          const int after = total - 1 - g < NS - 2 ? total - 1 - g : NS - 2;
           if (after >= 2) vm_wait<2 * VM>();
           else if (after == 1) vm_wait<VM>();
           else vm_wait<0>();
           HM_LDS_BARRIER();
-          if (g + 3 < total) {
-            qres_issue<Cfg>(smem + slot(g + 3) * STAGE, csrc, bsrc, blim, bi * blkbytes + ci * 128,
-                            (int64_t)(bi + 1) * bcols, wave);
+          if (g + NS - 1 < total) {
+            qres_issue<Cfg>(smem + slot(g + NS - 1) * STAGE, csrc, bsrc, blim,
+                            bi * blkbytes + ci * 128, (int64_t)(bi + 1) * bcols, wave);
             if (++ci == nchunk) { ci = 0; ++bi; }
           }
           qreg_mma(acc, smem + slot(g) * STAGE, qreg[c], q, half, swz);

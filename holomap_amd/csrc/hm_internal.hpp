@@ -61,6 +61,13 @@ struct DsDesc {
 __host__ __device__ constexpr int sweep_rows(int mode) { return mode == 2 ? 128 : 256; }
 // Threads per sweep workgroup: one wave per 32 query rows.
 __host__ __device__ constexpr int sweep_threads(int mode) { return sweep_rows(mode) / 32 * 64; }
+// Ring slots and workgroups per CU of the register-query sweep (a 3-slot
+// ring at three workgroups per CU leaves 168 VGPRs and spills: measured
+// slower on N = 20,000 x d = 128).
+#ifndef HM_QREG_NSTAGE
+#define HM_QREG_NSTAGE 4
+#define HM_QREG_PER 2
+#endif
 // bf16 operand rows (kp16 * 2 bytes) up to this length are swept with the
 // item's query rows resident in LDS (gram_sweep_qres_kernel).
 constexpr int kQresRowBytes = 384;
@@ -102,24 +109,28 @@ using QresWide = QresCfgT<8, 3, kQresRowBytes>;
 static_assert(QresWide::LDS <= 160 * 1024, "LDS");
 
 // bf16 sweep with the query operand in registers (gram_sweep_qreg_kernel),
-// for rows of at most 256 bytes: two 128-query workgroups per CU, a 4-slot
-// ring of candidate stages (three in flight while one is computed); the
-// item's query rows pass through ring slots 0 and 1 on their way to the
-// registers.
-struct QregCfg {
+// for rows of at most 256 bytes: PER 128-query workgroups per CU, each with
+// a ring of NSTAGE candidate stages (NSTAGE - 1 in flight while one is
+// computed); the item's query rows pass through ring slots 0 and 1 on their
+// way to the registers.
+template <int NSTAGE_, int PER_>
+struct QregCfgT {
   static constexpr int NW = 4, QW = 32 * NW;
   static constexpr int BIAS = 128 * 128;             // after the candidate rows
   static constexpr int STAGE = BIAS + NW / 2 * 512;  // + NW/2 copies of the biases
-  static constexpr int NSTAGE = 4;
+  static constexpr int NSTAGE = NSTAGE_;
+  static constexpr int PER = PER_;                   // workgroups per CU
   static constexpr int NI = 16 / NW;                 // candidate copy groups per wave
   static constexpr int VM = NI + 1;                  // vmcnt entries per stage and wave
   static constexpr int BIAS0 = NSTAGE * STAGE;       // the item's first block's biases
   static constexpr int LDS = BIAS0 + 512;
   static constexpr int QMAXB = kQresPairRowBytes;    // row bytes held
+  static_assert(NSTAGE >= 3 && NSTAGE <= 4, "waits cover at most two later stages");
   static_assert(QMAXB / 128 * QW * 128 <= 2 * STAGE, "query rows fit ring slots 0 and 1");
   static_assert(QW / 8 == 4 * NW, "four query copy groups per wave and slice");
+  static_assert(PER * LDS <= 160 * 1024, "PER workgroups per CU");
 };
-static_assert(2 * QregCfg::LDS <= 160 * 1024, "two workgroups per CU");
+using QregCfg = QregCfgT<HM_QREG_NSTAGE, HM_QREG_PER>;
 static_assert(QresWide::NW * 64 == sweep_threads(0), "the runtime launches sweep_threads(0)");
 
 

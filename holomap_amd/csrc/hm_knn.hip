@@ -470,9 +470,18 @@ __device__ __forceinline__ double rr_dist64(const Q qa, const float* __restrict_
 // Sorting all keys and scoring the plausible prefix would score a subset of
 // these candidates; the extra ones have lower bounds above that prefix's
 // threshold, so they cannot enter the top k: the output is the same.
+// An uncertified row: its flag, and (nflag set) a count of such rows that
+// lets the list builders skip their scan when it is zero (an atomic per
+// flagged row, which is rare; the count itself is all that is read).
+__device__ __forceinline__ void flag_row(__attribute__((address_space(1))) uint32_t* gflag,
+                                         int64_t grow, uint32_t* __restrict__ nflag) {
+  gflag[grow] = 1u;
+  if (nflag) atomicAdd(nflag, 1u);
+}
+
 __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, int nds,
                                            int64_t grow, int mode, uint32_t* __restrict__ flag,
-                                           uint64_t* lds) {
+                                           uint32_t* __restrict__ nflag, uint64_t* lds) {
   const int lane = threadIdx.x & 63;
   int lo = 0, hi = nds - 1;
   while (lo < hi) {
@@ -595,7 +604,7 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   if (nvalid < k) {
     // cannot happen for n >= k (the lists keep min(C, columns) keys);
     // leave the row to the exact scan rather than decode an empty slot
-    if (lane == 0) gflag[grow] = 1;
+    if (lane == 0) flag_row(gflag, grow, nflag);
     return;
   }
   // the sweep stores (key' bits, column) with key' = key / s; kh holds the
@@ -677,7 +686,7 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   }
   if (np < k) {
     // cannot happen (the k key-best candidates pass)
-    if (lane == 0) gflag[grow] = 1;
+    if (lane == 0) flag_row(gflag, grow, nflag);
     return;
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the query row is in LDS
@@ -701,7 +710,7 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
     xd[e] = __float_as_uint((float)dist);
   }
   if (__ballot(badid) != 0ull) {
-    if (lane == 0) gflag[grow] = 1;  // recomputed by the exact scan
+    if (lane == 0) flag_row(gflag, grow, nflag);  // recomputed by the exact scan
     return;
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -741,7 +750,10 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
       cert = (1.0 + tf - E - eps_exact) >= nu;
     }
   }
-  if (lane == 0) gflag[grow] = cert ? 0u : 1u;
+  if (lane == 0) {
+    if (cert) gflag[grow] = 0u;
+    else flag_row(gflag, grow, nflag);
+  }
 }
 
 // All rows (rlist == nullptr: rows 0 .. total_rows-1, one per wave), or the
@@ -752,13 +764,14 @@ __global__ __launch_bounds__(256, 6) void rerank_kernel(const DsDesc* __restrict
                                                         int64_t total_rows, int mode,
                                                         uint32_t* __restrict__ flag,
                                                         const uint32_t* __restrict__ rlist,
-                                                        const int* __restrict__ rcount) {
+                                                        const int* __restrict__ rcount,
+                                                        uint32_t* __restrict__ nflag) {
   __shared__ uint64_t cs[4][768];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: row data in SGPRs
   const int64_t cnt = rlist ? (int64_t)*rcount : total_rows;
   for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < cnt; p += (int64_t)gridDim.x * 4) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // previous row's LDS use is over
-    rerank_row(ds_arr, nds, rlist ? (int64_t)rlist[p] : p, mode, flag, cs[wave]);
+    rerank_row(ds_arr, nds, rlist ? (int64_t)rlist[p] : p, mode, flag, nflag, cs[wave]);
   }
 }
 
@@ -957,10 +970,21 @@ __global__ __launch_bounds__(256) void fallback_items_kernel(
 // Flag compaction fused with the work list it feeds (flag arrays of at most
 // kScanSmall entries: one workgroup of 512 threads compacts, then builds the
 // items from the list, which the barrier has made visible to it).
+// nflag: the number of flagged rows (a count the re-rank keeps): none, the
+// usual case, skips the scan.
 __global__ __launch_bounds__(512) void flag_sweep_items_kernel(
     const uint32_t* __restrict__ flag, int64_t n, uint32_t* __restrict__ list,
     int* __restrict__ count, const DsDesc* __restrict__ ds_arr, int nds, int qw, int cus,
-    double rate, WorkItem* __restrict__ items, int* __restrict__ nitems, int* __restrict__ nrows) {
+    double rate, WorkItem* __restrict__ items, int* __restrict__ nitems, int* __restrict__ nrows,
+    const uint32_t* __restrict__ nflag) {
+  if (*nflag == 0u) {  // uniform
+    if (threadIdx.x == 0) {
+      *count = 0;
+      *nitems = 0;
+      *nrows = 0;
+    }
+    return;
+  }
   scan_small_body<uint32_t, 512, 32>(flag, n, nullptr, nullptr, list, count);
   __syncthreads();
   sweep_items_block<512>(ds_arr, nds, list, count, qw, cus, rate, items, nitems, nrows);
@@ -969,7 +993,15 @@ __global__ __launch_bounds__(512) void flag_fallback_items_kernel(
     const uint32_t* __restrict__ flag, int64_t n, uint32_t* __restrict__ list,
     int* __restrict__ count, const DsDesc* __restrict__ ds_arr, int nds, int qb, int xs,
     int xs_cap, int64_t item_cap, int64_t row_cap, int target, WorkItem* __restrict__ items,
-    int* __restrict__ nitems, int* __restrict__ xs_out) {
+    int* __restrict__ nitems, int* __restrict__ xs_out, const uint32_t* __restrict__ nflag) {
+  if (nflag && *nflag == 0u) {  // uniform: no flagged row
+    if (threadIdx.x == 0) {
+      *count = 0;
+      *nitems = 0;
+      *xs_out = xs;
+    }
+    return;
+  }
   scan_small_body<uint32_t, 512, 32>(flag, n, nullptr, nullptr, list, count);
   __syncthreads();
   fallback_items_block<512>(ds_arr, nds, list, count, qb, xs, xs_cap, item_cap, row_cap, target,

@@ -59,12 +59,13 @@ __device__ __forceinline__ int find_ds(const DsDesc* ds, int nds, int64_t grow) 
 
 // Thread per global row: sequential f64 fold of x_f^2 in feature order.
 // zero_g (optional): a per-row count cleared for a later kernel of the run;
-// smax_g: the nds x HM_NSTAT statistics maxima, cleared for prep_kernel.
+// smax_g: the nds x HM_NSTAT statistics maxima, cleared for prep_kernel,
+// and the word after them (the re-rank's count of flagged rows).
 __global__ void rownorm_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
                                float* __restrict__ nz_g, uint32_t* __restrict__ zero_g,
                                uint32_t* __restrict__ smax_g) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < (int64_t)nds * HM_NSTAT) smax_g[g] = 0u;
+  if (g <= (int64_t)nds * HM_NSTAT) smax_g[g] = 0u;
   if (g >= total_rows) return;
   if (zero_g) zero_g[g] = 0u;
   const int di = find_ds(ds, nds, g);
@@ -142,7 +143,12 @@ __global__ __launch_bounds__(256) void prep_kernel(
     float m = 0.0f;  // fmaxf: a NaN never enters (none arises from finite rows)
 #pragma unroll
     for (int r = 0; r < 16; ++r) m = fmaxf(m, red[r][threadIdx.x]);
-    atomicMax((uint32_t*)const_cast<float*>(ds[di].smax) + threadIdx.x, __float_as_uint(m));
+    // every workgroup of the dataset updates the same eight words, where
+    // atomics serialise: read the current maximum (coherently; it only
+    // grows during the run) and update only when this one exceeds it
+    uint32_t* p = (uint32_t*)const_cast<float*>(ds[di].smax) + threadIdx.x;
+    const uint32_t cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (__float_as_uint(m) > cur) atomicMax(p, __float_as_uint(m));
   }
 }
 

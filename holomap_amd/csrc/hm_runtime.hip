@@ -36,16 +36,17 @@ __global__ void exact_group_merge_kernel(const DsDesc*, int, const uint32_t*, co
 __global__ void exact_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t,
                                    const int*, int, int, int, const uint64_t*);
 __global__ void rerank_kernel(const DsDesc*, int, int64_t, int, uint32_t*, const uint32_t*,
-                              const int*);
+                              const int*, uint32_t*);
 __global__ void sweep_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int, int,
                                    double, WorkItem*, int*, int*);
 __global__ void fallback_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int, int,
                                       int, int64_t, int64_t, int, WorkItem*, int*, int*);
 __global__ void flag_sweep_items_kernel(const uint32_t*, int64_t, uint32_t*, int*, const DsDesc*,
-                                        int, int, int, double, WorkItem*, int*, int*);
+                                        int, int, int, double, WorkItem*, int*, int*,
+                                        const uint32_t*);
 __global__ void flag_fallback_items_kernel(const uint32_t*, int64_t, uint32_t*, int*,
                                            const DsDesc*, int, int, int, int, int64_t, int64_t,
-                                           int, WorkItem*, int*, int*);
+                                           int, WorkItem*, int*, int*, const uint32_t*);
 __global__ void list_fallback_items_kernel(const uint32_t*, uint32_t*, int*, const DsDesc*, int,
                                            int, int, int, int64_t, int64_t, int, WorkItem*, int*,
                                            int*);
@@ -229,6 +230,7 @@ struct hm_job {
            *zany = nullptr, *key1 = nullptr, *erank = nullptr, *k1s = nullptr, *p1s = nullptr,
            *tmpk = nullptr, *tmpv = nullptr, *inptr = nullptr, *rows = nullptr, *cols = nullptr,
            *sc32 = nullptr;
+  uint32_t* nflag = nullptr;  // rows the primary re-rank flagged (the word after smax)
   int *count = nullptr, *nitems_fb = nullptr, *xs_fb = nullptr;
   WorkItem *items_l1 = nullptr, *items_fb = nullptr, *items_l2 = nullptr;
   int* nitems_l2 = nullptr;
@@ -537,7 +539,7 @@ int sweep_list_of(int mode, int kp16) {
   return 2;
 }
 constexpr int kListRows[3] = {128, 256, 0};  // query rows per item (0: sweep_rows(mode))
-constexpr int kListPerCu[3] = {2, 1, 1};     // workgroups per CU
+constexpr int kListPerCu[3] = {QregCfg::PER, 1, 1};  // workgroups per CU
 
 
 // Passes of the candidate sweep in `mode` over `items` (a host count, or a
@@ -600,7 +602,7 @@ hm_status launch_sweep_l1(hm_job* J) {
     HM_LAUNCH(ctx);                                                                            \
   } while (0)
     if (n0 > 0) {
-      const unsigned grid = (unsigned)sweep_grid(ctx, 0, n0, kListPerCu[0]);
+      const unsigned grid = (unsigned)sweep_grid(ctx, 0, n0, QregCfg::PER);
       if (pass == 0)
         hipLaunchKernelGGL(gram_sweep_qreg_kernel<0>, dim3(grid), dim3(QregCfg::NW * 64), 0,
                            J->stream, J->d_desc, it, nullptr, n0, nullptr, 1);
@@ -635,9 +637,10 @@ hm_status enqueue_pipeline(hm_job* J) {
     // ---- prep ----
     // (also clears the in-degree counts of the symmetrisation to follow)
     const int mode_bits = (J->mode == 0 ? 1 : 0) | 2;
+    // (rownorm also clears the statistics maxima and the flagged-row count)
     hipLaunchKernelGGL(rownorm_kernel,
-                       dim3(grid_for(std::max<int64_t>(R, (int64_t)nds * HM_NSTAT), 256)), dim3(256),
-                       0, J->stream, J->d_desc, nds, R, J->nz,
+                       dim3(grid_for(std::max<int64_t>(R, (int64_t)nds * HM_NSTAT + 1), 256)),
+                       dim3(256), 0, J->stream, J->d_desc, nds, R, J->nz,
                        (J->stages & HM_STAGE_FUZZY) ? J->inptr : nullptr, (uint32_t*)J->smax);
     hipLaunchKernelGGL(prep_kernel, dim3(grid_for(J->total_pad, 16)), dim3(256), 0, J->stream,
                        J->d_desc, nds, J->total_pad, (const int64_t*)J->pad_off,
@@ -651,7 +654,8 @@ hm_status enqueue_pipeline(hm_job* J) {
       mark(J, 2);
       hipLaunchKernelGGL(rerank_kernel, dim3(std::min<int64_t>(grid_for(R, 4), 6 * (int64_t)cus)),
                          dim3(256), 0, J->stream, J->d_desc,
-                         nds, R, J->mode, J->flag, (const uint32_t*)nullptr, (const int*)nullptr);
+                         nds, R, J->mode, J->flag, (const uint32_t*)nullptr, (const int*)nullptr,
+                         J->nflag);
       HM_LAUNCH(ctx);
       mark(J, 3);
       if (J->esc_mode >= 0) {
@@ -665,7 +669,7 @@ hm_status enqueue_pipeline(hm_job* J) {
           hipLaunchKernelGGL(flag_sweep_items_kernel, dim3(1), dim3(kScanSmallThreads), 0,
                              J->stream, (const uint32_t*)J->flag, R, J->list, J->count, J->d_desc,
                              nds, sweep_rows(em), cus, rate, J->items_l2, J->nitems_l2,
-                             J->nitems_l2 + 1);
+                             J->nitems_l2 + 1, (const uint32_t*)J->nflag);
         } else {
           HM_TRY(compact_flags(J, J->flag, R, J->list, J->count));
           hipLaunchKernelGGL(sweep_items_kernel, dim3(1), dim3(256), 0, J->stream, J->d_desc, nds,
@@ -676,7 +680,8 @@ hm_status enqueue_pipeline(hm_job* J) {
         HM_TRY(launch_sweep(J, em, J->items_l2, J->nitems_l2, 0, J->list, (unsigned)cus, 0, 2));
         hipLaunchKernelGGL(rerank_kernel, dim3(std::max<int64_t>(1, std::min<int64_t>(cdiv(R, 4), 8 * cus))),
                            dim3(256), 0, J->stream, J->d_desc, nds, R, em, J->flag,
-                           (const uint32_t*)J->list, (const int*)(J->nitems_l2 + 1));
+                           (const uint32_t*)J->list, (const int*)(J->nitems_l2 + 1),
+                           (uint32_t*)nullptr);
         HM_LAUNCH(ctx);
       }
     } else {
@@ -703,7 +708,8 @@ hm_status enqueue_pipeline(hm_job* J) {
       hipLaunchKernelGGL(flag_fallback_items_kernel, dim3(1), dim3(kScanSmallThreads), 0,
                          J->stream, (const uint32_t*)J->flag, R, J->list, J->count, J->d_desc, nds,
                          qb, J->xs, xs_cap, (int64_t)J->fb_cap, R, (int)gx, J->items_fb,
-                         J->nitems_fb, J->xs_fb);
+                         J->nitems_fb, J->xs_fb,
+                         J->mode >= 0 ? (const uint32_t*)J->nflag : (const uint32_t*)nullptr);
     } else {
       HM_TRY(compact_flags(J, J->flag, R, J->list, J->count));
       hipLaunchKernelGGL(fallback_items_kernel, dim3(1), dim3(256), 0, J->stream, J->d_desc, nds,
@@ -1128,7 +1134,8 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
     if (J->mode == 2 || J->esc_mode == 2) HM_A(J->op16lo, P.sum_x16);
     HM_A(J->colbias, P.total_pad);
     HM_A(J->rstat, R * HM_NSTAT);
-    HM_A(J->smax, (int64_t)nds * HM_NSTAT);
+    HM_A(J->smax, (int64_t)nds * HM_NSTAT + 1);  // + the flagged-row count
+    J->nflag = (uint32_t*)(J->smax + (int64_t)nds * HM_NSTAT);
     HM_A(J->nz, R);
     HM_A(J->rnorm, R);
     // sized for any sweep configuration (see plan_sweep)
