@@ -79,6 +79,9 @@ constexpr int kQresPairRowBytes = 256;
 // the exact scan); the exact scan and its merges serve k <= kMaxK.
 constexpr int kSweepMaxK = 64;
 constexpr int kMaxK = 1024;
+// Bucket slots a workgroup of the symmetrisation's write kernel stages in
+// LDS at a time.
+constexpr int kSymChunk = 1024;
 // Queries per wave of the exact-scan shape for a job whose largest k is k
 // (exact_scan_kernel<QPW, QCAP>, QCAP >= k + 64; items hold 4 * QPW queries).
 __host__ __device__ constexpr int xs_qpw_for_k(int k) { return k <= 64 ? 8 : (k <= 448 ? 4 : 1); }

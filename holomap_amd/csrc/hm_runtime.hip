@@ -94,20 +94,16 @@ __global__ void sched_scatter_kernel(const uint32_t*, const uint32_t*, const flo
                                      uint32_t*, double*);
 __global__ void nonfinite_min_kernel(const uint64_t*, int, uint64_t*);
 __global__ void l2_normalise_kernel(const float*, int64_t, int, float*);
-__global__ void transpose_scatter_kernel(const DsDesc*, int, int64_t, const uint32_t*,
-                                         const uint32_t*, const float*, uint32_t, const uint32_t*,
-                                         uint32_t*, uint32_t*, float*);
 template <int G>
-__global__ void union_count_kernel(const DsDesc*, int, int64_t, const uint32_t*, const uint32_t*,
-                                   const uint32_t*, const float*, uint32_t, uint64_t*, uint32_t*,
-                                   float*);
-template <int G>
-__global__ void union_write_kernel(const DsDesc*, int, int64_t, int64_t, const uint32_t*,
-                                   const float*, const uint32_t*, const uint32_t*,
-                                   const uint32_t*, const float*, const uint32_t*, const float*,
-                                   const uint64_t*, const int64_t*, uint32_t, uint64_t, uint32_t*,
-                                   uint32_t*, float*, uint64_t*, uint32_t, int, float*, double*,
-                                   const float*, uint32_t*);
+__global__ void sym_scatter_kernel(const DsDesc*, int, int64_t, int64_t, const uint32_t*,
+                                   const uint32_t*, const float*, const uint32_t*, uint32_t*,
+                                   uint32_t*, float*, float*, uint64_t*, uint32_t*);
+__global__ void sym_write_kernel(const DsDesc*, int, int64_t, int64_t, const uint32_t*,
+                                 const float*, const float*, const uint32_t*, const uint32_t*,
+                                 const uint32_t*, const float*, const uint64_t*, const int64_t*,
+                                 uint64_t, uint32_t*, uint32_t*, float*, uint64_t*, uint32_t,
+                                 uint32_t, uint32_t*, int, float*, double*, const float*,
+                                 uint32_t*);
 }  // namespace hm
 
 using namespace hm;
@@ -228,9 +224,10 @@ struct hm_job {
   uint32_t *ccnt = nullptr;
   uint32_t *knn_idx = nullptr, *flag = nullptr, *flagpos = nullptr, *list = nullptr,
            *zany = nullptr, *key1 = nullptr, *erank = nullptr, *k1s = nullptr, *p1s = nullptr,
-           *tmpk = nullptr, *tmpv = nullptr, *inptr = nullptr, *rows = nullptr, *cols = nullptr,
+           *tmpv = nullptr, *inptr = nullptr, *rows = nullptr, *cols = nullptr,
            *sc32 = nullptr;
   uint32_t* nflag = nullptr;  // rows the primary re-rank flagged (the word after smax)
+  uint32_t* serr = nullptr;   // set by a symmetrisation kernel that met inconsistent counts
   int *count = nullptr, *nitems_fb = nullptr, *xs_fb = nullptr;
   WorkItem *items_l1 = nullptr, *items_fb = nullptr, *items_l2 = nullptr;
   int* nitems_l2 = nullptr;
@@ -442,9 +439,9 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
   b += R * 20 + nds * 12;                            // sigma, rho, sigma64; flags, targets
   if ((stages & HM_STAGE_KNN) && opt.calib == HM_CALIB_F64) b += E * 8 + R * 12;  // dist64, err3
   if (stages & HM_STAGE_FUZZY) {
-    // key1 erank wval in_dst in_src in_w ai av; inptr + indptr_g (its last
+    // key1 erank wval in_dst in_key in_w revw; inptr + indptr_g (its last
     // entry the total); union sizes; rows cols vals; indptr_l; iptr_off
-    b += E * 4 * 8 + (R + 1) * 12 + R * 8 + 2 * E * 12 + (R + nds) * 8 + nds * 8;
+    b += E * 4 * 7 + (R + 1) * 12 + R * 8 + 2 * E * 12 + (R + nds) * 8 + nds * 8 + 4;
   }
   b += 2 * (cdiv(std::max<int64_t>(R + 1, E + 1), kScanTile) + 2) * 8;
   return b + (1u << 20);
@@ -483,6 +480,18 @@ hm_status dalloc(hm_job* J, T** p, int64_t count) {
     hm_status s_ = (x);           \
     if (s_ != HM_OK) return s_;   \
   } while (0)
+
+// The symmetrisation's kernels check the bucket counts they are given (a
+// slot or row id out of range is skipped and flagged, never written): a
+// flag means the graph arrays are not the union, reported by every reader.
+static hm_status sym_check(hm_job* J) {
+  if (!(J->stages & HM_STAGE_FUZZY) || !J->serr) return HM_OK;
+  uint32_t e = 0;
+  HM_HIP(J->ctx, hipMemcpyAsync(&e, J->serr, 4, hipMemcpyDeviceToHost, J->stream));
+  HM_HIP(J->ctx, hipStreamSynchronize(J->stream));
+  if (e) return fail(J->ctx, HM_EDEVICE, "symmetrisation: inconsistent in-degree counts");
+  return HM_OK;
+}
 
 // ---- device building blocks ----
 
@@ -798,46 +807,36 @@ hm_status enqueue_pipeline(hm_job* J) {
     }
     // transpose: in-degree counts -> bucket offsets (in place; inptr[R] =
     // the kept edges), then every kept edge to its slot of its destination's
-    // bucket
+    // bucket, marked mutual or not, with the rows' union sizes (a group of
+    // G >= k lanes per row); the sizes' scan; the entries and row pointers
     HM_TRY(scan_excl<uint32_t>(J, J->inptr, R, J->inptr, J->inptr + R, J->sc32));
     uint32_t* in_dst = J->k1s;
-    uint32_t* in_src = J->p1s;
+    uint32_t* in_key = J->p1s;
     float* in_w = J->in_w;
-    hipLaunchKernelGGL(transpose_scatter_kernel, dim3(grid_for(E, 256)), dim3(256), 0, J->stream,
-                       J->d_desc, nds, E, (const uint32_t*)J->key1, (const uint32_t*)J->erank,
-                       (const float*)J->wval, sent, (const uint32_t*)J->inptr, in_dst, in_src,
-                       in_w);
-    HM_LAUNCH(ctx);
-    // a group of G >= k lanes per row: per out-edge in-edge counts and
-    // reverse weights, the union sizes and their scan, then the entries
+    float* revw = (float*)J->tmpv;
     const int GU = J->kmax <= 16 ? 16 : J->kmax <= 32 ? 32 : 64;
-    uint32_t* ai = J->tmpk;
-    float* av = (float*)J->tmpv;
-#define HM_UNION_COUNT(GS)                                                                     \
-  hipLaunchKernelGGL((union_count_kernel<GS>), dim3(grid_for(R * GS, 256)), dim3(256), 0,       \
-                     J->stream, J->d_desc, nds, R, (const uint32_t*)J->key1,                    \
-                     (const uint32_t*)J->inptr, (const uint32_t*)in_src, (const float*)in_w,    \
-                     sent, J->cntu, ai, av)
-    // workgroups past the entries' carry the calibration's global-mean floor
-    const bool fix = (J->stages & HM_STAGE_CALIB) != 0;
-    const unsigned wgrid = grid_for(R * GU + E, 256);
-#define HM_UNION_WRITE(GS)                                                                     \
-  hipLaunchKernelGGL((union_write_kernel<GS>), dim3(wgrid + (fix ? (unsigned)nds : 0u)), dim3(256), 0, \
-                     J->stream, J->d_desc, nds, R, E, (const uint32_t*)J->key1,                 \
-                     (const float*)J->wval, (const uint32_t*)J->inptr, (const uint32_t*)in_dst, \
-                     (const uint32_t*)in_src, (const float*)in_w, (const uint32_t*)ai,          \
-                     (const float*)av, (const uint64_t*)J->indptr_g,                            \
-                     (const int64_t*)J->iptr_off, sent, (uint64_t)(2 * E), J->rows, J->cols,    \
-                     J->vals, J->indptr_l, wgrid, J->opt.calib, J->sigma, J->sigma64,           \
-                     (const float*)J->rho, fix ? J->zany : nullptr)
-    if (GU == 16) HM_UNION_COUNT(16); else if (GU == 32) HM_UNION_COUNT(32); else HM_UNION_COUNT(64);
+#define HM_SYM_SCATTER(GS)                                                                      \
+  hipLaunchKernelGGL((sym_scatter_kernel<GS>), dim3(grid_for(R * GS, 256)), dim3(256), 0,        \
+                     J->stream, J->d_desc, nds, R, E, (const uint32_t*)J->key1,                  \
+                     (const uint32_t*)J->erank, (const float*)J->wval,                           \
+                     (const uint32_t*)J->inptr, in_dst, in_key, in_w, revw, J->cntu, J->serr)
+    if (GU == 16) HM_SYM_SCATTER(16); else if (GU == 32) HM_SYM_SCATTER(32); else HM_SYM_SCATTER(64);
+#undef HM_SYM_SCATTER
     HM_LAUNCH(ctx);
     // (indptr_g[R] is the total: J->total_nnz points there)
     HM_TRY(scan_excl<uint64_t>(J, J->cntu, R, J->indptr_g, J->indptr_g + R, J->sc64));
-    if (GU == 16) HM_UNION_WRITE(16); else if (GU == 32) HM_UNION_WRITE(32); else HM_UNION_WRITE(64);
+    // workgroups past the entries' carry the calibration's global-mean floor
+    const bool fix = (J->stages & HM_STAGE_CALIB) != 0;
+    const uint32_t nrt = (uint32_t)cdiv(R, 256), nslot = (uint32_t)cdiv(E, 256);
+    hipLaunchKernelGGL(sym_write_kernel, dim3(nrt + 2 * nslot + (fix ? (uint32_t)nds : 0u)),
+                       dim3(256), 0, J->stream, J->d_desc, nds, R, E, (const uint32_t*)J->key1,
+                       (const float*)J->wval, (const float*)revw, (const uint32_t*)J->inptr,
+                       (const uint32_t*)in_dst, (const uint32_t*)in_key, (const float*)in_w,
+                       (const uint64_t*)J->indptr_g, (const int64_t*)J->iptr_off,
+                       (uint64_t)(2 * E), J->rows, J->cols, J->vals, J->indptr_l, nrt, nslot,
+                       J->serr, J->opt.calib, J->sigma, J->sigma64, (const float*)J->rho,
+                       fix ? J->zany : nullptr);
     HM_LAUNCH(ctx);
-#undef HM_UNION_COUNT
-#undef HM_UNION_WRITE
   }
   mark(J, 6);
   return HM_OK;
@@ -1185,7 +1184,10 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
   if (stages & HM_STAGE_FUZZY) {
     HM_A(J->key1, E); HM_A(J->in_w, E); HM_A(J->erank, E); HM_A(J->wval, E);
     HM_A(J->k1s, E); HM_A(J->p1s, E);
-    HM_A(J->tmpk, E); HM_A(J->tmpv, E);
+    HM_A(J->tmpv, E);
+    HM_A(J->serr, 1);
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, J->stream, J->serr, (int64_t)1, 0u);
+    if (hipGetLastError() != hipSuccess) return bail(fail(ctx, HM_EDEVICE, "flag clear failed"));
     HM_A(J->inptr, R + 1);
     HM_A(J->cntu, R); HM_A(J->indptr_g, R + 1);
     J->total_nnz = J->indptr_g + R;
@@ -1510,6 +1512,7 @@ hm_status hm_job_counters(hm_job* J, uint64_t* c) {
     }
   }
   if (J->stages & HM_STAGE_FUZZY) {
+    HM_TRY(sym_check(J));
     uint64_t nnz = 0;
     HM_HIP(J->ctx, hipMemcpy(&nnz, J->total_nnz, sizeof(uint64_t), hipMemcpyDeviceToHost));
     c[HM_C_NNZ] = nnz;
@@ -1558,6 +1561,7 @@ hm_status hm_job_download(hm_job* J, uint32_t ds, hm_graph_out* o) {
   HM_TRY(fp64_error(J, ds, o->err64));
   o->nnz = 0;
   if (J->stages & HM_STAGE_FUZZY) {
+    HM_TRY(sym_check(J));
     uint64_t se[2];
     HM_HIP(ctx, hipMemcpy(se, J->indptr_g + h.row_off, 8, hipMemcpyDeviceToHost));
     if (ds + 1 < (uint32_t)J->nds)
@@ -1619,7 +1623,7 @@ hm_status hm_job_device_outputs(hm_job* J, uint32_t ds, const uint32_t** idx, co
   if (dist) *dist = J->knn_dist + h.edge_off;
   uint64_t s0 = 0;
   if ((rows || cols || vals) && (J->stages & HM_STAGE_FUZZY)) {
-    HM_HIP(J->ctx, hipStreamSynchronize(J->stream));
+    HM_TRY(sym_check(J));
     HM_HIP(J->ctx, hipMemcpy(&s0, J->indptr_g + h.row_off, 8, hipMemcpyDeviceToHost));
   }
   if (rows) *rows = J->rows ? J->rows + s0 : nullptr;
@@ -2103,7 +2107,7 @@ hm_status hm_schedule_edges(hm_ctx* ctx, const uint32_t* rows, const uint32_t* c
 // graph arrays.
 static hm_status job_csr(hm_job* J, uint32_t ds, const uint64_t** indptr, uint64_t* s0) {
   if (!(J->stages & HM_STAGE_FUZZY)) return fail(J->ctx, HM_EINVAL, "the job builds no graph");
-  HM_HIP(J->ctx, hipStreamSynchronize(J->stream));
+  HM_TRY(sym_check(J));
   HM_HIP(J->ctx, hipMemcpy(s0, J->indptr_g + J->dsh[ds].row_off, 8, hipMemcpyDeviceToHost));
   *indptr = J->indptr_l + J->iptr_off_h[ds];
   return HM_OK;
