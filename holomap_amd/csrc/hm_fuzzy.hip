@@ -281,6 +281,7 @@ __global__ __launch_bounds__(256) void sym_write_kernel(
     bool act = false;
     uint32_t c = 0, la = 0;
     int64_t r = 0, roff = 0, lo = 0, hi = 0;
+    uint64_t base = 0;
     float va = 0.0f, vt = 0.0f;
     if (e < E) {
       c = key1[e];
@@ -291,6 +292,7 @@ __global__ __launch_bounds__(256) void sym_write_kernel(
         r = roff + (int64_t)((uint32_t)(e - D.edge_off) / (uint32_t)D.k);
         lo = off[r];
         hi = off[r + 1];
+        base = indptr_g[r];
         va = wval[e];
         vt = revw[e];
         la = less_in_row(key1, D.edge_off + (r - roff) * D.k, D.k, c);
@@ -301,7 +303,7 @@ __global__ __launch_bounds__(256) void sym_write_kernel(
     if (!act) lo = hi = L;
     const uint32_t li = span_count(in_key, L, H, lo, hi, c, lo, sk);
     if (!act) return;
-    const uint64_t pos = indptr_g[r] + la + li;
+    const uint64_t pos = base + la + li;
     if (pos >= cap) return;
     rows[pos] = (uint32_t)(r - roff);
     cols[pos] = (uint32_t)((int64_t)c - roff);
@@ -322,6 +324,7 @@ __global__ __launch_bounds__(256) void sym_write_kernel(
   bool act = false;
   uint32_t src = 0xffffffffu, la = 0;
   int64_t r = 0, roff = 0, lo = 0, hi = 0;
+  uint64_t base = 0;
   float w = 0.0f;
   if (p < pe) {
     src = in_key[p];
@@ -337,6 +340,7 @@ __global__ __launch_bounds__(256) void sym_write_kernel(
       roff = D.row_off;
       lo = off[r];
       hi = off[r + 1];
+      base = indptr_g[r];
       la = less_in_row(key1, D.edge_off + (r - roff) * D.k, D.k, src);
     }
   }
@@ -345,7 +349,7 @@ __global__ __launch_bounds__(256) void sym_write_kernel(
   if (!act) lo = hi = L;
   const uint32_t li = span_count(in_key, L, H, lo, hi, src, p, sk);
   if (!act) return;
-  const uint64_t pos = indptr_g[r] + la + li;
+  const uint64_t pos = base + la + li;
   if (pos >= cap) return;
   const float v = 0.0f;
   rows[pos] = (uint32_t)(r - roff);

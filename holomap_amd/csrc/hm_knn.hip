@@ -6,8 +6,8 @@
 //    and each row keeps its k smallest (f32 dist, idx) keys -- the order of
 //    src/knn.rs:48, since dist >= +0 makes the f32 bit pattern monotone.
 //    Serves the exact precision mode and the fallback for uncertified rows.
-//  * exact_group_merge_kernel, exact_merge_kernel -- merge the per-split
-//    top-k lists of a row (two levels when there are many splits).
+//  * exact_merge_kernel -- merges the per-split top-k lists of a row (two
+//    levels when there are many splits).
 //  * rerank_kernel      -- after the MFMA candidate sweep: recomputes the
 //    plausible candidates exactly, takes the top-k and certifies that no
 //    column outside the candidate set can enter it.
@@ -278,56 +278,21 @@ HM_XS_INST(1, 2048)
 #undef HM_XS_INST
 
 // Merging the xs sorted per-split lists of every listed query (or local
-// row in identity mode) into its knn row, in two levels so a wave never
-// sorts more than 1024 keys: with xs * k > 1024, exact_group_merge_kernel
-// first reduces each group of G consecutive splits (G * k <= 1024) to its k
-// smallest keys, written over the group's first list; exact_merge_kernel
-// then merges the remaining lists (every G-th, or all when xs * k <= 1024).
-__global__ __launch_bounds__(256) void exact_group_merge_kernel(
-    const DsDesc* __restrict__ ds_arr, int nds, const uint32_t* __restrict__ qlist,
-    const int* __restrict__ count_dev, int64_t count_host, const int* __restrict__ xs_dev,
-    int xs_host, int kmax, int G, uint64_t* __restrict__ xpart) {
-  __shared__ uint64_t s[4][1024];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t count = count_dev ? (int64_t)*count_dev : count_host;
-  const int xs = xs_dev ? *xs_dev : xs_host;
-  const int ngrp = (xs + G - 1) / G;
-  if (xs * kmax <= 1024) return;  // a single merge suffices
-  for (int64_t u = (int64_t)blockIdx.x * 4 + wave; u < count * ngrp; u += (int64_t)gridDim.x * 4) {
-    const int64_t p = u / ngrp;
-    const int grp = (int)(u - p * ngrp);
-    const int64_t grow = qlist ? (int64_t)qlist[p] : p;
-    int lo = 0, hi = nds - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (ds_arr[mid].row_off <= grow) lo = mid; else hi = mid - 1;
-    }
-    const int k = ds_arr[lo].k;
-    const int s0 = grp * G, ns = xs - s0 < G ? xs - s0 : G;
-    uint64_t* src = xpart + ((size_t)p * xs + s0) * kmax;
-    uint64_t* a = s[wave];
-    int len = 64;
-    while (len < ns * k) len <<= 1;
-    for (int e = lane; e < len; e += 64) {
-      const int sp = e / k, t = e % k;
-      a[e] = (sp < ns) ? src[(size_t)sp * kmax + t] : ~0ull;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    wave_sort_lds(a, len);
-    for (int t = lane; t < k; t += 64) src[t] = a[t];
-  }
-}
-
+// row in identity mode) into its knn row, one wave per row, in two levels so
+// a wave never sorts more than 1024 keys: with xs * k > 1024 each group of G
+// consecutive splits (G * k <= 1024) is first reduced to its k smallest
+// keys, gathered in a second LDS area (at most G groups: G * k <= 1024
+// keys), which is then sorted; else all the lists are sorted at once.
 __global__ __launch_bounds__(256) void exact_merge_kernel(
     const DsDesc* __restrict__ ds_arr, int nds, const uint32_t* __restrict__ qlist,
     const int* __restrict__ count_dev, int64_t count_host, const int* __restrict__ xs_dev,
     int xs_host, int kmax, int G, const uint64_t* __restrict__ xpart) {
-  __shared__ uint64_t s[4][1024];
+  __shared__ uint64_t s[4][2][1024];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t count = count_dev ? (int64_t)*count_dev : count_host;
   const int xs = xs_dev ? *xs_dev : xs_host;
-  const int stride = xs * kmax <= 1024 ? 1 : G;  // split slots holding a list to merge
-  const int nl = (xs + stride - 1) / stride;
+  const bool two = xs * kmax > 1024;
+  const int ngrp = two ? (xs + G - 1) / G : 1;
   for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < count; p += (int64_t)gridDim.x * 4) {
     const int64_t grow = qlist ? (int64_t)qlist[p] : p;
     int lo = 0, hi = nds - 1;
@@ -339,17 +304,42 @@ __global__ __launch_bounds__(256) void exact_merge_kernel(
     const int64_t row = grow - D.row_off;
     const int k = D.k;
     const uint64_t* src = xpart + (size_t)p * xs * kmax;
-    uint64_t* a = s[wave];
-    int len = 64;
-    while (len < nl * k) len <<= 1;
-    for (int e = lane; e < len; e += 64) {
-      const int l = e / k, t = e % k;
-      a[e] = (l < nl) ? src[(size_t)l * stride * kmax + t] : ~0ull;
+    uint64_t* a = s[wave][0];
+    uint64_t* m = s[wave][1];
+    // the lists of splits [s0, s0 + ns) into a[0 .. len), padded
+    auto gather = [&](int s0, int ns) {
+      int len = 64;
+      while (len < ns * k) len <<= 1;
+      for (int e = lane; e < len; e += 64) {
+        const int sp = e / k, t = e % k;
+        a[e] = (sp < ns) ? src[(size_t)(s0 + sp) * kmax + t] : ~0ull;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      return len;
+    };
+    uint64_t* fin = a;
+    int nm = xs * k;
+    if (two) {
+      for (int g = 0; g < ngrp; ++g) {
+        const int s0 = g * G;
+        wave_sort_lds(a, gather(s0, xs - s0 < G ? xs - s0 : G));
+        for (int t = lane; t < k; t += 64) m[g * k + t] = a[t];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+      fin = m;
+      nm = ngrp * k;
+    } else {
+      gather(0, xs);
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    wave_sort_lds(a, len);
+    int len = 64;
+    while (len < nm) len <<= 1;
+    if (two) {
+      for (int e = nm + lane; e < len; e += 64) m[e] = ~0ull;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    wave_sort_lds(fin, len);
     for (int t = lane; t < k; t += 64) {
-      const uint64_t key = a[t];
+      const uint64_t key = fin[t];
       D.knn_idx[row * k + t] = (uint32_t)key;
       D.knn_dist[row * k + t] = __uint_as_float((uint32_t)(key >> 32));
       if (D.knn_dist64) D.knn_dist64[row * k + t] = pair_dist64(D, row, (uint32_t)key);

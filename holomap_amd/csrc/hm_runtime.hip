@@ -31,8 +31,6 @@ __global__ void gram_sweep_qreg_kernel(const DsDesc*, const WorkItem*, const int
 template <int QPW, int QCAP>
 __global__ void exact_scan_kernel(const DsDesc*, const WorkItem*, const int*, int,
                                   const uint32_t*, const int*, int, int, uint64_t*);
-__global__ void exact_group_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t,
-                                         const int*, int, int, int, uint64_t*);
 __global__ void exact_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t,
                                    const int*, int, int, int, const uint64_t*);
 __global__ void rerank_kernel(const DsDesc*, int, int64_t, int, uint32_t*, const uint32_t*,
@@ -734,9 +732,6 @@ hm_status enqueue_pipeline(hm_job* J) {
     else HM_XSCAN(1, 2048);
 #undef HM_XSCAN
     const unsigned gm = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(R, 4), 8 * cus));
-    hipLaunchKernelGGL(exact_group_merge_kernel, dim3(gm), dim3(256), 0, J->stream, J->d_desc,
-                       nds, (const uint32_t*)J->list, (const int*)J->count, (int64_t)0,
-                       (const int*)J->xs_fb, J->xs, J->kmax, G, J->xpart);
     hipLaunchKernelGGL(exact_merge_kernel, dim3(gm), dim3(256), 0, J->stream, J->d_desc, nds,
                        (const uint32_t*)J->list, (const int*)J->count, (int64_t)0,
                        (const int*)J->xs_fb, J->xs, J->kmax, G, (const uint64_t*)J->xpart);
