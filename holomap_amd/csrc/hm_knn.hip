@@ -283,12 +283,18 @@ HM_XS_INST(1, 2048)
 // consecutive splits (G * k <= 1024) is first reduced to its k smallest
 // keys, gathered in a second LDS area (at most G groups: G * k <= 1024
 // keys), which is then sorted; else all the lists are sorted at once.
+// The last kernel of the kNN stage: workgroup 0 also clears clr[0 .. nclr)
+// (the statistics maxima and the flagged-row count, which the next run's
+// prep and re-rank accumulate into; their readers have run).
 __global__ __launch_bounds__(256) void exact_merge_kernel(
     const DsDesc* __restrict__ ds_arr, int nds, const uint32_t* __restrict__ qlist,
     const int* __restrict__ count_dev, int64_t count_host, const int* __restrict__ xs_dev,
-    int xs_host, int kmax, int G, const uint64_t* __restrict__ xpart) {
+    int xs_host, int kmax, int G, const uint64_t* __restrict__ xpart, uint32_t* __restrict__ clr,
+    int nclr) {
   __shared__ uint64_t s[4][2][1024];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (blockIdx.x == 0 && clr)
+    for (int t = threadIdx.x; t < nclr; t += 256) clr[t] = 0u;
   const int64_t count = count_dev ? (int64_t)*count_dev : count_host;
   const int xs = xs_dev ? *xs_dev : xs_host;
   const bool two = xs * kmax > 1024;

@@ -1,20 +1,21 @@
 // Data preparation for the candidate sweep.
 //
 // For every row of every dataset of a batch:
-//   * rownorm_kernel  (thread per row): the reference's sequential f64 fold
-//     sum_f x_f^2 (src/metric.rs:61-66 keeps na/nb as such folds), giving the
-//     cosine denominator factor sqrt(na) bit-exactly and the Euclidean key
-//     bias ||x||^2 rounded to f32 (a thread per row: every lane of a 16-lane
-//     group folding its row serially inside prep_kernel measured 4x slower);
-//   * prep_kernel     (16 lanes per row): a zero-padded f32 copy (n_pad x kp),
-//     the candidate-selection operand (euclid: the copy itself; cosine: the
-//     row scaled to unit length), its bf16 (RNE) image, the statistics the
-//     certification bound needs (representation error norms), and their
-//     per-dataset maxima: each workgroup folds the maxima of its rows into
-//     smax by integer atomic maxima of the bit patterns (the statistics are
-//     non-negative floats, whose patterns order as their values; max is
-//     exact and order-free, so the result is the same on every run).
-//     rownorm_kernel clears smax first.
+//   * prep_kernel     (16 lanes per row): the row's norm, the reference's
+//     sequential f64 fold sum_f x_f^2 (src/metric.rs:61-66 keeps na/nb as
+//     such folds), every lane of the row folding the 16 values of a slice
+//     in order from DPP broadcasts: the cosine denominator factor sqrt(na)
+//     bit-exactly and the Euclidean key bias ||x||^2 rounded to f32; a
+//     zero-padded f32 copy (n_pad x kp), the candidate-selection operand
+//     (euclid: the copy itself; cosine: the row scaled to unit length), its
+//     bf16 (RNE) image, the statistics the certification bound needs
+//     (representation error norms), and their per-dataset maxima: each
+//     workgroup folds the maxima of its rows into smax by integer atomic
+//     maxima of the bit patterns (the statistics are non-negative floats,
+//     whose patterns order as their values; max is exact and order-free, so
+//     the result is the same on every run). smax is clear when prep runs:
+//     zero-filled when the job is made, cleared again by the kNN stage's
+//     last kernel after the re-ranks have read it.
 #include "hm_internal.hpp"
 
 namespace hm {
@@ -61,60 +62,23 @@ __device__ __forceinline__ int find_ds(const DsDesc* ds, int nds, int64_t grow) 
 // zero_g (optional): a per-row count cleared for a later kernel of the run;
 // smax_g: the nds x HM_NSTAT statistics maxima, cleared for prep_kernel,
 // and the word after them (the re-rank's count of flagged rows).
-__global__ void rownorm_kernel(const DsDesc* __restrict__ ds, int nds, int64_t total_rows,
-                               float* __restrict__ nz_g, uint32_t* __restrict__ zero_g,
-                               uint32_t* __restrict__ smax_g) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g <= (int64_t)nds * HM_NSTAT) smax_g[g] = 0u;
-  if (g >= total_rows) return;
-  if (zero_g) zero_g[g] = 0u;
-  const int di = find_ds(ds, nds, g);
-  const DsDesc& D = ds[di];
-  const int64_t i = g - D.row_off;
-  const int d = D.d;
-  const float* x = D.x + i * d;
-  double s = 0.0;
-  int f = 0;
-  if ((d & 3) == 0 && (((uintptr_t)D.x) & 15) == 0) {
-    // 16-byte rows: 8 float4 loads in flight per batch, folded in order
-    for (; f + 32 <= d; f += 32) {
-      float4 b4[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) b4[u] = *(const float4*)(x + f + 4 * u);
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        double v;
-        v = (double)b4[u].x; s = s + v * v;
-        v = (double)b4[u].y; s = s + v * v;
-        v = (double)b4[u].z; s = s + v * v;
-        v = (double)b4[u].w; s = s + v * v;
-      }
-    }
-  }
-  for (; f < d; ++f) {
-    const double v = (double)x[f];
-    s = s + v * v;
-  }
-  nz_g[g] = (float)s;
-  const_cast<double*>(D.rnorm)[i] = sqrt(s);
-}
-
 __device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int di, int64_t pr,
                                          const int64_t* __restrict__ pad_off,
-                                         const float* __restrict__ nz_g, int mode_bits, int lane,
+                                         uint32_t* __restrict__ zero_g, int mode_bits, int lane,
                                          float (&stout)[HM_NSTAT]);
 
 // 16 lanes (one DPP row) per (dataset, padded row), four rows per wave, so
-// a wave overlaps the memory round trips of four rows. Writes xr, op32
-// (cosine), op16 (+ op16lo), colbias and rstat from the row norms
-// rownorm_kernel left. mode_bits: 1 = produce op16 (bf16 sweep), 2 = bf16x3 stats. A
+// a wave overlaps the memory round trips of four rows. Writes rnorm, xr,
+// op32 (cosine), op16 (+ op16lo), colbias and rstat, and clears the row's
+// in-degree count zero_g[row] when a symmetrisation follows (null
+// otherwise). mode_bits: 1 = produce op16 (bf16 sweep), 2 = bf16x3 stats. A
 // workgroup's 16 rows belong to one dataset (padded row counts are
 // multiples of 128); it folds the maxima of their statistics into the
 // dataset's smax (atomic maxima of the non-negative values' bit patterns).
 __global__ __launch_bounds__(256) void prep_kernel(
     const DsDesc* __restrict__ ds, int nds, int64_t total_pad_rows,
     const int64_t* __restrict__ pad_off /* nds+1 prefix of n_pad */,
-    const float* __restrict__ nz_g, int mode_bits) {
+    uint32_t* __restrict__ zero_g, int mode_bits) {
   __shared__ float red[16][HM_NSTAT];
   __shared__ int di_s;
   if (threadIdx.x == 0) {  // the workgroup's dataset, by padded-row prefix
@@ -133,7 +97,7 @@ __global__ __launch_bounds__(256) void prep_kernel(
 #pragma unroll
   for (int s = 0; s < HM_NSTAT; ++s) st[s] = 0.0f;
   if (pr < total_pad_rows)  // (always: the grid covers whole 16-row groups)
-    prep_row(ds, di, pr, pad_off, nz_g, mode_bits, (int)(threadIdx.x & 15), st);
+    prep_row(ds, di, pr, pad_off, zero_g, mode_bits, (int)(threadIdx.x & 15), st);
   if ((threadIdx.x & 15) == 15) {
 #pragma unroll
     for (int s = 0; s < HM_NSTAT; ++s) red[threadIdx.x >> 4][s] = st[s];
@@ -152,10 +116,25 @@ __global__ __launch_bounds__(256) void prep_kernel(
   }
 }
 
+// s + v_0^2 + v_1^2 + ... + v_15^2 in f64, left to right (v_u: lane u of
+// the calling lane's 16-lane DPP row, by a row broadcast): every lane of the
+// row returns the same sum.
+template <int U = 0>
+__device__ __forceinline__ double row_fold_sq(float v, double s) {
+  if constexpr (U < 16) {
+    const float vu =
+        __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + U, 0xf, 0xf, false));
+    const double t = (double)vu;
+    return row_fold_sq<U + 1>(v, s + t * t);
+  } else {
+    return s;
+  }
+}
+
 // One padded row of dataset di by 16 lanes (prep_kernel's body).
 __device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int di, int64_t pr,
                                          const int64_t* __restrict__ pad_off,
-                                         const float* __restrict__ nz_g, int mode_bits, int lane,
+                                         uint32_t* __restrict__ zero_g, int mode_bits, int lane,
                                          float (&stout)[HM_NSTAT]) {
   const DsDesc& D = ds[di];
   const int64_t i = pr - pad_off[di];
@@ -170,11 +149,19 @@ __device__ __forceinline__ void prep_row(const DsDesc* __restrict__ ds, int di, 
   }
   const bool cosine = D.metric == 1;
   const bool real = i < D.n;
-  double rn = 0.0;
-  float nz = 0.0f;
-  if (real) {
-    rn = D.rnorm[i];
-    nz = nz_g[D.row_off + i];
+  // the norm fold, d values in order (padding lanes add +0.0, which leaves
+  // the non-negative partial sums as they are); the loop is uniform over the
+  // workgroup (one dataset)
+  double s = 0.0;
+  for (int f0 = 0; f0 < d; f0 += 16) {
+    const int f = f0 + lane;
+    s = row_fold_sq((real && f < d) ? D.x[i * d + f] : 0.0f, s);
+  }
+  const double rn = real ? sqrt(s) : 0.0;
+  const float nz = real ? (float)s : 0.0f;
+  if (real && lane == 0) {
+    const_cast<double*>(D.rnorm)[i] = rn;
+    if (zero_g) zero_g[D.row_off + i] = 0u;
   }
   const bool zero_row = real && rn == 0.0;
   // statistics accumulated per lane, reduced by a fixed butterfly

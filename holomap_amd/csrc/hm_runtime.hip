@@ -17,8 +17,7 @@
 
 namespace hm {
 // kernels (defined in the other translation units)
-__global__ void rownorm_kernel(const DsDesc*, int, int64_t, float*, uint32_t*, uint32_t*);
-__global__ void prep_kernel(const DsDesc*, int, int64_t, const int64_t*, const float*, int);
+__global__ void prep_kernel(const DsDesc*, int, int64_t, const int64_t*, uint32_t*, int);
 template <int MODE, int PASS, int CW>
 __global__ void gram_sweep_kernel(const DsDesc*, const WorkItem*, const int*, int, const uint32_t*,
                                   int);
@@ -32,7 +31,7 @@ template <int QPW, int QCAP>
 __global__ void exact_scan_kernel(const DsDesc*, const WorkItem*, const int*, int,
                                   const uint32_t*, const int*, int, int, uint64_t*);
 __global__ void exact_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t,
-                                   const int*, int, int, int, const uint64_t*);
+                                   const int*, int, int, int, const uint64_t*, uint32_t*, int);
 __global__ void rerank_kernel(const DsDesc*, int, int64_t, int, uint32_t*, const uint32_t*,
                               const int*, uint32_t*);
 __global__ void sweep_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int, int,
@@ -211,7 +210,7 @@ struct hm_job {
   DsDesc* d_desc = nullptr;
   float *x = nullptr, *xr = nullptr, *op32c = nullptr, *colbias = nullptr, *rstat = nullptr,
         *thr = nullptr,
-        *smax = nullptr, *nz = nullptr, *knn_dist = nullptr, *sigma = nullptr, *rho = nullptr,
+        *smax = nullptr, *knn_dist = nullptr, *sigma = nullptr, *rho = nullptr,
         *wval = nullptr, *vals = nullptr, *in_w = nullptr;
   uint16_t *op16 = nullptr, *op16lo = nullptr;
   double *rnorm = nullptr, *sigma64 = nullptr, *target = nullptr, *dist64 = nullptr;
@@ -428,7 +427,7 @@ uint64_t plan_bytes(const Plan& P, const hm_options& opt, uint32_t stages) {
     int64_t kp16_max = 0;
     for (auto& h : P.dsh) kp16_max = std::max<int64_t>(kp16_max, h.kp16);
     b += kWideSlackRows * kp16_max * 2;  // rows the wide collect pass may read past the end
-    b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + R * 12;
+    b += P.total_pad * 4 + R * HM_NSTAT * 4 + nds * HM_NSTAT * 4 + R * 8;
     b += P.sum_cand * 8 + R * 2 * kSplitCap * 12 + R * kSplitCap * 4;  // cand, ctau+ccnt, thr
     b += 520 * 8;                                                          // counter sums
     b += R * 12 + P.fb_cap * 16 + R * P.xs * P.kmax * 8;
@@ -642,16 +641,13 @@ hm_status enqueue_pipeline(hm_job* J) {
   mark(J, 0);
   if (J->stages & HM_STAGE_KNN) {
     // ---- prep ----
-    // (also clears the in-degree counts of the symmetrisation to follow)
+    // (also clears the in-degree counts of the symmetrisation to follow; the
+    // statistics maxima and the flagged-row count are clear: zero-filled at
+    // creation, cleared by the stage's last kernel)
     const int mode_bits = (J->mode == 0 ? 1 : 0) | 2;
-    // (rownorm also clears the statistics maxima and the flagged-row count)
-    hipLaunchKernelGGL(rownorm_kernel,
-                       dim3(grid_for(std::max<int64_t>(R, (int64_t)nds * HM_NSTAT + 1), 256)),
-                       dim3(256), 0, J->stream, J->d_desc, nds, R, J->nz,
-                       (J->stages & HM_STAGE_FUZZY) ? J->inptr : nullptr, (uint32_t*)J->smax);
     hipLaunchKernelGGL(prep_kernel, dim3(grid_for(J->total_pad, 16)), dim3(256), 0, J->stream,
                        J->d_desc, nds, J->total_pad, (const int64_t*)J->pad_off,
-                       (const float*)J->nz, mode_bits);
+                       (J->stages & HM_STAGE_FUZZY) ? J->inptr : nullptr, mode_bits);
     HM_LAUNCH(ctx);
     mark(J, 1);
     // ---- candidate sweep + re-rank ----
@@ -734,7 +730,8 @@ hm_status enqueue_pipeline(hm_job* J) {
     const unsigned gm = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(R, 4), 8 * cus));
     hipLaunchKernelGGL(exact_merge_kernel, dim3(gm), dim3(256), 0, J->stream, J->d_desc, nds,
                        (const uint32_t*)J->list, (const int*)J->count, (int64_t)0,
-                       (const int*)J->xs_fb, J->xs, J->kmax, G, (const uint64_t*)J->xpart);
+                       (const int*)J->xs_fb, J->xs, J->kmax, G, (const uint64_t*)J->xpart,
+                       (uint32_t*)J->smax, nds * HM_NSTAT + 1);
     HM_LAUNCH(ctx);
     mark(J, 4);
   } else {
@@ -754,7 +751,7 @@ hm_status enqueue_pipeline(hm_job* J) {
     const bool edges = (J->stages & HM_STAGE_FUZZY) != 0;
     const uint32_t sent = (uint32_t)R;
     uint32_t* k1 = edges ? J->key1 : nullptr;
-    if (edges && !(J->stages & HM_STAGE_KNN)) {  // else cleared by rownorm_kernel
+    if (edges && !(J->stages & HM_STAGE_KNN)) {  // else cleared by prep_kernel
       hipLaunchKernelGGL(fill_u32, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, J->inptr, R, 0u);
       HM_LAUNCH(ctx);
     }
@@ -792,7 +789,7 @@ hm_status enqueue_pipeline(hm_job* J) {
   if (J->stages & HM_STAGE_FUZZY) {
     const uint32_t sent = (uint32_t)R;
     if (!(J->stages & HM_STAGE_CALIB)) {  // else written by the calibration
-      if (!(J->stages & HM_STAGE_KNN))  // else cleared by rownorm_kernel
+      if (!(J->stages & HM_STAGE_KNN))  // else cleared by prep_kernel
         hipLaunchKernelGGL(fill_u32, dim3(grid_for(R, 256)), dim3(256), 0, J->stream, J->inptr, R,
                            0u);
       hipLaunchKernelGGL(directed_kernel, dim3(grid_for(E, 256)), dim3(256), 0, J->stream,
@@ -1130,7 +1127,9 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
     HM_A(J->rstat, R * HM_NSTAT);
     HM_A(J->smax, (int64_t)nds * HM_NSTAT + 1);  // + the flagged-row count
     J->nflag = (uint32_t*)(J->smax + (int64_t)nds * HM_NSTAT);
-    HM_A(J->nz, R);
+    // clear for the first run (each run's kNN stage clears it for the next)
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, J->stream, (uint32_t*)J->smax,
+                       (int64_t)nds * HM_NSTAT + 1, 0u);
     HM_A(J->rnorm, R);
     // sized for any sweep configuration (see plan_sweep)
     HM_A(J->cand, std::max<int64_t>(P.sum_cand, 1));
