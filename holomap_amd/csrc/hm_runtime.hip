@@ -61,7 +61,7 @@ __global__ void directed_kernel(const DsDesc*, int, int64_t, const float*, const
 __global__ void expf_kernel(const float*, float*, int64_t);
 template <typename T>
 __global__ void scan_tile_sums(const T*, int64_t, T*);
-template <typename T, int NT, int MS>
+template <typename T, int NT, int MS, bool LIST>
 __global__ void scan_small(const T*, int64_t, T*, T*, uint32_t*, int*);
 template <typename T>
 __global__ void scan_sums(T*, int64_t, T*);
@@ -157,7 +157,10 @@ namespace {
 constexpr int kScanTile = 4096;
 // Arrays scanned by one workgroup (scan_small: 512 threads x kScanSmallQuads
 // quads of 4 elements held per lane): 32 quads for 32-bit values, 16 for
-// 64-bit ones.
+// 64-bit ones. The plain scans (scan_excl, scan_u32_excl) run the same
+// capacity on 1024 threads with half the quads per lane (half the serial
+// steps); the flag compactions keep 512 threads, whose item building
+// follows in the same workgroup.
 constexpr int kScanSmallThreads = 512;
 template <typename T>
 constexpr int kScanSmallQuads = sizeof(T) == 4 ? 32 : 16;
@@ -165,6 +168,12 @@ template <typename T>
 constexpr int64_t kScanSmall = (int64_t)kScanSmallThreads * 4 * kScanSmallQuads<T>;
 static_assert(kScanSmallThreads == 512 && kScanSmallQuads<uint32_t> == 32,
               "flag_*_items_kernel compact with scan_small_body<uint32_t, 512, 32>");
+constexpr int kScanWideThreads = 1024;
+template <typename T>
+constexpr int kScanWideQuads = kScanSmallQuads<T> / 2;
+static_assert((int64_t)kScanWideThreads * 4 * kScanWideQuads<uint32_t> == kScanSmall<uint32_t> &&
+                  (int64_t)kScanWideThreads * 4 * kScanWideQuads<uint64_t> == kScanSmall<uint64_t>,
+              "the wide scans hold the same arrays");
 
 struct DsHost {
   int64_t n, n_pad, row_off, edge_off;
@@ -498,8 +507,8 @@ template <typename T>
 hm_status scan_excl(hm_job* J, const T* in, int64_t n, T* out, T* total, T* tmp) {
   if (n <= 0) return HM_OK;
   if (n <= kScanSmall<T>) {
-    constexpr int NT = kScanSmallThreads, MS = kScanSmallQuads<T>;
-    hipLaunchKernelGGL((scan_small<T, NT, MS>), dim3(1), dim3(NT), 0, J->stream, in, n, out, total,
+    constexpr int NT = kScanWideThreads, MS = kScanWideQuads<T>;
+    hipLaunchKernelGGL((scan_small<T, NT, MS, false>), dim3(1), dim3(NT), 0, J->stream, in, n, out, total,
                        (uint32_t*)nullptr, (int*)nullptr);
     HM_LAUNCH(J->ctx);
     return HM_OK;
@@ -516,7 +525,7 @@ hm_status scan_excl(hm_job* J, const T* in, int64_t n, T* out, T* total, T* tmp)
 // list[0 .. count) = indices i < n with flag[i] set, in order; count on the device.
 hm_status compact_flags(hm_job* J, const uint32_t* flag, int64_t n, uint32_t* list, int* count) {
   if (n <= kScanSmall<uint32_t>) {
-    hipLaunchKernelGGL((scan_small<uint32_t, kScanSmallThreads, kScanSmallQuads<uint32_t>>), dim3(1),
+    hipLaunchKernelGGL((scan_small<uint32_t, kScanSmallThreads, kScanSmallQuads<uint32_t>, true>), dim3(1),
                        dim3(kScanSmallThreads), 0, J->stream, flag, n,
                        (uint32_t*)nullptr, (uint32_t*)nullptr, list, count);
   } else {
@@ -1837,8 +1846,8 @@ namespace {
 hm_status scan_u32_excl(hm_ctx* ctx, hipStream_t st, const uint32_t* in, int64_t n,
                         uint32_t* out, uint32_t* total, uint32_t* tmp) {
   if (n <= kScanSmall<uint32_t>) {
-    constexpr int NT = kScanSmallThreads, MS = kScanSmallQuads<uint32_t>;
-    hipLaunchKernelGGL((scan_small<uint32_t, NT, MS>), dim3(1), dim3(NT), 0, st, in, n, out, total,
+    constexpr int NT = kScanWideThreads, MS = kScanWideQuads<uint32_t>;
+    hipLaunchKernelGGL((scan_small<uint32_t, NT, MS, false>), dim3(1), dim3(NT), 0, st, in, n, out, total,
                        (uint32_t*)nullptr, (int*)nullptr);
   } else {
     const int64_t nt = cdiv(n, kScanTile);

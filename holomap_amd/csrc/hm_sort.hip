@@ -81,18 +81,23 @@ __global__ __launch_bounds__(256) void scan_tiles(const T* __restrict__ in, int6
   }
 }
 
-template <typename T, int NT, int MS>
+// LIST: the flag-compaction form may be asked for (list non-null); without
+// it the kernel is a plain scan and keeps no registers for the flags.
+template <typename T, int NT, int MS, bool LIST>
 __global__ __launch_bounds__(NT) void scan_small(const T* __restrict__ in, int64_t n,
                                                  T* __restrict__ out, T* __restrict__ total,
                                                  uint32_t* __restrict__ list,
                                                  int* __restrict__ count) {
-  scan_small_body<T, NT, MS>(in, n, out, total, list, count);
+  scan_small_body<T, NT, MS>(in, n, out, total, LIST ? list : nullptr, LIST ? count : nullptr);
 }
 
-template __global__ void scan_small<uint32_t, 512, 32>(const uint32_t*, int64_t, uint32_t*,
-                                                       uint32_t*, uint32_t*, int*);
-template __global__ void scan_small<uint64_t, 512, 16>(const uint64_t*, int64_t, uint64_t*, uint64_t*,
-                                                   uint32_t*, int*);
+template __global__ void scan_small<uint32_t, 512, 32, true>(const uint32_t*, int64_t, uint32_t*,
+                                                             uint32_t*, uint32_t*, int*);
+template __global__ void scan_small<uint32_t, 1024, 16, false>(const uint32_t*, int64_t,
+                                                               uint32_t*, uint32_t*, uint32_t*,
+                                                               int*);
+template __global__ void scan_small<uint64_t, 1024, 8, false>(const uint64_t*, int64_t, uint64_t*,
+                                                              uint64_t*, uint32_t*, int*);
 template __global__ void scan_tile_sums<uint32_t>(const uint32_t*, int64_t, uint32_t*);
 template __global__ void scan_tile_sums<uint64_t>(const uint64_t*, int64_t, uint64_t*);
 template __global__ void scan_sums<uint32_t>(uint32_t*, int64_t, uint32_t*);
