@@ -1137,8 +1137,8 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
     HM_A(J->smax, (int64_t)nds * HM_NSTAT + 1);  // + the flagged-row count
     J->nflag = (uint32_t*)(J->smax + (int64_t)nds * HM_NSTAT);
     // clear for the first run (each run's kNN stage clears it for the next)
-    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, J->stream, (uint32_t*)J->smax,
-                       (int64_t)nds * HM_NSTAT + 1, 0u);
+    hipLaunchKernelGGL(fill_u32, dim3(grid_for((int64_t)nds * HM_NSTAT + 1, 256)), dim3(256), 0,
+                       J->stream, (uint32_t*)J->smax, (int64_t)nds * HM_NSTAT + 1, 0u);
     HM_A(J->rnorm, R);
     // sized for any sweep configuration (see plan_sweep)
     HM_A(J->cand, std::max<int64_t>(P.sum_cand, 1));
