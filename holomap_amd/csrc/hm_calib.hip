@@ -346,16 +346,18 @@ __device__ __forceinline__ double grp_lane_f64(double x) {
     return __shfl(x, U, G);
   }
 }
-// psum + e_1 + e_2 + ... + e_{k-1}, left to right (e_t: lane t's value); the
-// slots t >= k add +0.0, which leaves the non-negative partial sums as they
-// are, so the result is the reference's fold bit for bit.
+// psum + e_1 + e_2 + ... + e_{k-1}, left to right (e_t: lane t's value).
+// The lanes t >= k hold +0.0 (the callers compute terms only in lanes 1 ..
+// k-1), and adding +0.0 leaves the non-negative partial sums as they are, so
+// every lane is added unmasked and the result is the reference's fold bit
+// for bit (one broadcast and one add per term).
 template <int G, int U = 1, typename T>
 __device__ __forceinline__ T grp_fold(T e, int k, T psum) {
   if constexpr (U < G) {
     T v;
     if constexpr (sizeof(T) == 4) v = grp_lane_f32<G, U>(e);
     else v = grp_lane_f64<G, U>(e);
-    psum += U < k ? v : (T)0;
+    psum += v;
     return grp_fold<G, U + 1>(e, k, psum);
   } else {
     return psum;
@@ -393,7 +395,7 @@ __device__ __forceinline__ T row_fold(T v, int k, int base, T psum) {
     T w;
     if constexpr (sizeof(T) == 4) w = grp_lane_f32<16, U>(v);
     else w = grp_lane_f64<16, U>(v);
-    psum += base + U < k ? w : (T)0;
+    psum += w;  // (+0.0 from the lanes past k, as in grp_fold)
     return row_fold<U + 1>(v, k, base, psum);
   } else {
     return psum;
