@@ -712,13 +712,28 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
   // ---- 5. the k smallest exact keys by rank ----
+  // (the lists padded to a multiple of 4 with keys above every real one, so
+  // the counts read 4 keys per 16-byte LDS read; np <= 512 slots)
+  const int np4 = (np + 3) & ~3;
+  if (lane < np4 - np) {
+    xd[np + lane] = 0xffffffffu;
+    cl[np + lane] = 0xffffffffu;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const uint4* xd4 = (const uint4*)xd;
+  const uint4* cl4 = (const uint4*)cl;
   uint64_t kth = 0;  // the k-th smallest exact key, found by the lane that holds it
   for (int m = 0; 64 * m < np; ++m) {
     const int e = lane + 64 * m;
     const uint64_t mine = e < np ? dist_key(__uint_as_float(xd[e]), cl[e]) : ~0ull;
     int rk = 0;
-    for (int f = 0; f < np; ++f)  // same address in every lane: broadcasts
-      rk += dist_key(__uint_as_float(xd[f]), cl[f]) < mine ? 1 : 0;
+    for (int f = 0; f < np4; f += 4) {  // same address in every lane: broadcasts
+      const uint4 xv = xd4[f >> 2], cv = cl4[f >> 2];
+      rk += (((uint64_t)xv.x << 32) | cv.x) < mine ? 1 : 0;
+      rk += (((uint64_t)xv.y << 32) | cv.y) < mine ? 1 : 0;
+      rk += (((uint64_t)xv.z << 32) | cv.z) < mine ? 1 : 0;
+      rk += (((uint64_t)xv.w << 32) | cv.w) < mine ? 1 : 0;
+    }
     if (e < np && rk < k) {
       oidx[rk] = (uint32_t)mine;
       odst[rk] = __uint_as_float((uint32_t)(mine >> 32));
