@@ -111,6 +111,8 @@ __global__ __launch_bounds__(256) void sym_scatter_kernel(
   const DsDesc& D = ds[ds_of_row(ds, nds, i)];
   const int k = D.k;
   const int64_t e0 = D.edge_off + (i - D.row_off) * k;
+  // the row's bucket size (read up front: it is only needed at the end)
+  const uint32_t si = t == 0 ? off[i + 1] - off[i] : 0u;
   uint32_t na = 0, nm = 0;
   for (int s = t; s < k; s += G) {
     const int64_t e = e0 + s;
@@ -138,7 +140,7 @@ __global__ __launch_bounds__(256) void sym_scatter_kernel(
     na += __shfl_xor(na, o, G);
     nm += __shfl_xor(nm, o, G);
   }
-  if (t == 0) cnt[i] = (uint64_t)na + (off[i + 1] - off[i]) - nm;
+  if (t == 0) cnt[i] = (uint64_t)na + si - nm;
 }
 
 // The end H of a workgroup's bucket span [L, H), checked against the kept

@@ -409,21 +409,47 @@ __device__ __forceinline__ float4 ld4(const float* p) { return *(const float4*)p
 // lane, in LDS (staged) or in global memory.
 struct QueryL {
   const float* q;
-  __device__ __forceinline__ void fold(double& s, int f, const float4 y, int metric) const {
-    rr_fold4(s, *(const float4*)(q + f), y, metric);
-  }
+  __device__ __forceinline__ float4 x4(int f) const { return *(const float4*)(q + f); }
 };
 struct QueryG {
   gfloat* q;
-  __device__ __forceinline__ void fold(double& s, int f, const float4 y, int metric) const {
-    rr_fold4(s, ld4(q + f), y, metric);
-  }
+  __device__ __forceinline__ float4 x4(int f) const { return ld4(q + f); }
 };
-template <class Q>
-__device__ __forceinline__ double rr_dist64(const Q qa, const float* __restrict__ bp, int d4,
-                                            int metric, double rna, double rnb) {
-  gfloat* b = (gfloat*)bp;
+// Accumulators of the fold: the reference's sequential one, or four partial
+// sums of fused multiply-adds over the same f64 differences (the fast path:
+// a quarter of the dependent chain and one f64 operation fewer per element;
+// its distance from the reference's sum is bounded in rr_end).
+struct AccExact {
   double s = 0.0;
+  __device__ __forceinline__ void add4(const float4 x, const float4 y, int metric) {
+    rr_fold4(s, x, y, metric);
+  }
+  __device__ __forceinline__ double sum() const { return s; }
+};
+struct AccFast {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  __device__ __forceinline__ void add4(const float4 x, const float4 y, int metric) {
+    if (metric == 0) {
+      const double t0 = (double)x.x - (double)y.x, t1 = (double)x.y - (double)y.y;
+      const double t2 = (double)x.z - (double)y.z, t3 = (double)x.w - (double)y.w;
+      a0 = __builtin_fma(t0, t0, a0);
+      a1 = __builtin_fma(t1, t1, a1);
+      a2 = __builtin_fma(t2, t2, a2);
+      a3 = __builtin_fma(t3, t3, a3);
+    } else {
+      a0 = __builtin_fma((double)x.x, (double)y.x, a0);
+      a1 = __builtin_fma((double)x.y, (double)y.y, a1);
+      a2 = __builtin_fma((double)x.z, (double)y.z, a2);
+      a3 = __builtin_fma((double)x.w, (double)y.w, a3);
+    }
+  }
+  __device__ __forceinline__ double sum() const { return (a0 + a1) + (a2 + a3); }
+};
+template <class A, class Q>
+__device__ __forceinline__ double rr_sum(const Q qa, const float* __restrict__ bp, int d4,
+                                         int metric) {
+  gfloat* b = (gfloat*)bp;
+  A acc;
   const int d8 = d4 & ~7;
   float4 y0[2], y1[2];
   if (d8 > 0) {
@@ -436,21 +462,71 @@ __device__ __forceinline__ double rr_dist64(const Q qa, const float* __restrict_
       y1[0] = ld4(b + f + 8);
       y1[1] = ld4(b + f + 12);
     }
-    qa.fold(s, f, y0[0], metric);
-    qa.fold(s, f + 4, y0[1], metric);
+    acc.add4(qa.x4(f), y0[0], metric);
+    acc.add4(qa.x4(f + 4), y0[1], metric);
     f += 8;
     if (f >= d8) break;
     if (f + 8 < d8) {
       y0[0] = ld4(b + f + 8);
       y0[1] = ld4(b + f + 12);
     }
-    qa.fold(s, f, y1[0], metric);
-    qa.fold(s, f + 4, y1[1], metric);
+    acc.add4(qa.x4(f), y1[0], metric);
+    acc.add4(qa.x4(f + 4), y1[1], metric);
     f += 8;
   }
-  if (f < d4) qa.fold(s, f, ld4(b + f), metric);  // d4 % 8 == 4
+  if (f < d4) acc.add4(qa.x4(f), ld4(b + f), metric);  // d4 % 8 == 4
+  return acc.sum();
+}
+template <class Q>
+__device__ __forceinline__ double rr_dist64(const Q qa, const float* __restrict__ bp, int d4,
+                                            int metric, double rna, double rnb) {
+  const double s = rr_sum<AccExact>(qa, bp, d4, metric);
   if (metric == 0) return sqrt(s);
   return f64max0(1.0 - s / (rna * rnb));
+}
+// The fast sum of one candidate row, spread over the G = 2^lg lanes of a
+// group: lane g folds the row's 16-byte chunks g, g + G, ... (of nch) into
+// four fma accumulators, and the group adds its lanes' sums by a butterfly
+// (each pair adds the same two values, so every lane ends with the same
+// sum). The query chunks come from Q.
+template <class Q>
+__device__ __forceinline__ double rr_group_sum(const Q qa, gfloat* b, int g, int lg, int nch,
+                                               int metric) {
+  AccFast acc;
+  const int G = 1 << lg;
+  int f = g;
+  for (; f + G < nch; f += 2 * G) {
+    const float4 y0 = ld4(b + 4 * f), y1 = ld4(b + 4 * (f + G));
+    acc.add4(qa.x4(4 * f), y0, metric);
+    acc.add4(qa.x4(4 * (f + G)), y1, metric);
+  }
+  for (; f < nch; f += G) acc.add4(qa.x4(4 * f), ld4(b + 4 * f), metric);
+  return acc.sum();
+}
+__device__ __forceinline__ double rr_group_reduce(double s, int lg) {
+  for (int o = 1; o < (1 << lg); o <<= 1)
+    s += __longlong_as_double((long long)shfl_xor_u64((uint64_t)__double_as_longlong(s), o));
+  return s;
+}
+// One end of the interval of f32 distances a fast sum s can stand for:
+// hi = false the low end, true the high end. Every term of a sum reaches it
+// through at most m roundings (its product or fma and the additions after
+// it): m <= d4 for the reference's sequential fold, m <= d4/4 + 6 for the
+// group sum (a chain of at most nch fmas, two additions of the
+// accumulators, at most four butterfly levels). The two sums are thus at
+// most (1.25 d4 + 6) u apart, u = 2^-53, relative to sum |p_f| <= s / (1 -
+// m u) (Euclidean, p_f = t_f^2 >= 0 over the same differences t_f) or
+// sum |x_f y_f| <= ||x|| ||y|| <= rna rnb (1 + (d4/2 + 2) u) (cosine).
+// del = (3 d4 + 16) u also covers the roundings of the ends themselves. The
+// f32 distance is a non-decreasing function of the sum (Euclidean: the f64
+// sqrt, then f32) or a non-increasing one (cosine: 1 - s / r, max with 0,
+// then f32), all steps correctly rounded, so where both ends give the same
+// f32 value the reference's sum gives it too.
+__device__ __forceinline__ float rr_end(double s, bool hi, int d4, int metric, double r) {
+  const double del = (3.0 * d4 + 16.0) * 0x1p-53;
+  if (metric == 0) return (float)sqrt(s * (hi ? 1.0 + del : 1.0 - del));
+  const double err = del * r;
+  return (float)f64max0(1.0 - (hi ? s - err : s + err) / r);
 }
 
 // One row (global id grow) by one wave; lds: the wave's 6 KB LDS area.
@@ -687,23 +763,59 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the query row is in LDS
 
-  // ---- 4. exact distances, one lane per candidate (the f64 value of the
-  // lane's first candidate is kept for the FP64 calibration) ----
+  // ---- 4. exact distances of the candidates. With the FP64 outputs: the
+  // reference's fold, one lane per candidate (the lane's first candidate's
+  // f64 value kept for them). Without: G = 2^lg lanes per candidate, 64 / G
+  // candidates per pass, so each load instruction covers whole cache lines
+  // of a few rows (one lane per candidate touched 64 rows' lines per
+  // instruction, more than the CU's L1 keeps for its waves); the group's
+  // fast sum decides the f32 distance (rr_end), and the rare undecided
+  // candidates take the reference's fold in one lane. ----
   bool badid = false;
   double first64 = 0.0;
-  for (int e = lane; e < np; e += 64) {
-    const uint32_t j = cl[e];
-    double dist = 0.0;
-    if ((int64_t)j >= D.n) {
-      badid = true;  // never produced by the sweep; guard the row read
-    } else if ((int64_t)j != i) {
-      const double rnb = metric == 1 ? ((gf64*)D.rnorm)[j] : 0.0;
-      // two inlined copies, so the query reads keep their address space
-      dist = qlds ? rr_dist64(QueryL{qs}, D.xr + (size_t)j * kp, d4, metric, rna, rnb)
-                  : rr_dist64(QueryG{(gfloat*)qa}, D.xr + (size_t)j * kp, d4, metric, rna, rnb);
+  if (od64) {
+    for (int e = lane; e < np; e += 64) {
+      const uint32_t j = cl[e];
+      double dist = 0.0;
+      if ((int64_t)j >= D.n) {
+        badid = true;  // never produced by the sweep; guard the row read
+      } else if ((int64_t)j != i) {
+        const double rnb = metric == 1 ? ((gf64*)D.rnorm)[j] : 0.0;
+        const float* bp = D.xr + (size_t)j * kp;
+        dist = qlds ? rr_dist64(QueryL{qs}, bp, d4, metric, rna, rnb)
+                    : rr_dist64(QueryG{(gfloat*)qa}, bp, d4, metric, rna, rnb);
+      }
+      if (e == lane) first64 = dist;
+      xd[e] = __float_as_uint((float)dist);
     }
-    if (e == lane) first64 = dist;
-    xd[e] = __float_as_uint((float)dist);
+  }
+  const int nch = d4 >> 2;  // 16-byte chunks per row
+  const int lg = nch >= 64 ? 4 : nch >= 32 ? 3 : nch >= 16 ? 2 : nch >= 8 ? 1 : 0;  // uniform
+  const int g = lane & ((1 << lg) - 1);
+  for (int c0 = 0; !od64 && c0 < np; c0 += 64 >> lg) {
+    const int c = c0 + (lane >> lg);
+    const bool valid = c < np;
+    const uint32_t j = valid ? cl[c] : 0u;
+    if (valid && (int64_t)j >= D.n) badid = true;  // never produced by the sweep
+    const bool live = valid && (int64_t)j < D.n && (int64_t)j != i;  // same in a group
+    gfloat* bp = (gfloat*)(D.xr + (size_t)j * kp);
+    double sum = 0.0, rnb = 0.0;
+    if (live) {
+      if (metric == 1) rnb = ((gf64*)D.rnorm)[j];
+      // two inlined copies, so the query reads keep their address space
+      sum = qlds ? rr_group_sum(QueryL{qs}, bp, g, lg, nch, metric)
+                 : rr_group_sum(QueryG{(gfloat*)qa}, bp, g, lg, nch, metric);
+    }
+    sum = rr_group_reduce(sum, lg);
+    // the interval's two ends: in lanes g = 0 and 1, or both in the lane
+    const double r = rna * rnb;
+    const float e0 = rr_end(sum, lg > 0 && (g & 1), d4, metric, r);
+    const float e1 = lg > 0 ? __shfl_xor(e0, 1, 64) : rr_end(sum, true, d4, metric, r);
+    float d32 = e0;
+    if (live && g == 0 && e0 != e1)
+      d32 = (float)(qlds ? rr_dist64(QueryL{qs}, (const float*)bp, d4, metric, rna, rnb)
+                         : rr_dist64(QueryG{(gfloat*)qa}, (const float*)bp, d4, metric, rna, rnb));
+    if (valid && g == 0) xd[c] = __float_as_uint(live ? d32 : 0.0f);
   }
   if (__ballot(badid) != 0ull) {
     if (lane == 0) flag_row(gflag, grow, nflag);  // recomputed by the exact scan
