@@ -373,6 +373,9 @@ __device__ __forceinline__ float f32_round_up(double v) {
 
 #define RR_LISTS 10   // lists per row at most (two per column split)
 #define RR_QROW 512   // query rows of at most this many floats are staged in LDS
+// LDS words per wave: candidate columns and distances (512 each, 32-bit),
+// the query row, 64 sums (f64); six four-wave workgroups fill a CU's LDS
+#define RR_LDS ((512 + 512 + RR_QROW) / 2 + 64)
 
 // ref_dist64 for the re-rank: the query row qa (read at the same address by
 // every lane: LDS when staged there, else global memory through the scalar
@@ -503,9 +506,15 @@ __device__ __forceinline__ double rr_group_sum(const Q qa, gfloat* b, int g, int
   for (; f < nch; f += G) acc.add4(qa.x4(4 * f), ld4(b + 4 * f), metric);
   return acc.sum();
 }
-__device__ __forceinline__ double rr_group_reduce(double s, int lg) {
-  for (int o = 1; o < (1 << lg); o <<= 1)
-    s += __longlong_as_double((long long)shfl_xor_u64((uint64_t)__double_as_longlong(s), o));
+template <int O>
+__device__ __forceinline__ double rr_xor_add(double s) {
+  return s + __longlong_as_double((long long)shfl_xor_u64((uint64_t)__double_as_longlong(s), O));
+}
+__device__ __forceinline__ double rr_group_reduce(double s, int lg) {  // lg uniform
+  if (lg >= 1) s = rr_xor_add<1>(s);
+  if (lg >= 2) s = rr_xor_add<2>(s);
+  if (lg >= 3) s = rr_xor_add<4>(s);
+  if (lg >= 4) s = rr_xor_add<8>(s);
   return s;
 }
 // One end of the interval of f32 distances a fast sum s can stand for:
@@ -605,6 +614,7 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   uint32_t* cl = (uint32_t*)lds;  // <= 512 plausible candidate columns
   uint32_t* xd = cl + 512;        // their exact distances (f32 bits)
   float* qs = (float*)(xd + 512);  // the query row (d4 <= RR_QROW)
+  double* ss = (double*)(qs + RR_QROW);  // fast sums of 64 candidates
   const float* qa = D.xr + (size_t)i * kp;
   const bool qlds = d4 <= RR_QROW;
   if (qlds)
@@ -768,9 +778,10 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   // f64 value kept for them). Without: G = 2^lg lanes per candidate, 64 / G
   // candidates per pass, so each load instruction covers whole cache lines
   // of a few rows (one lane per candidate touched 64 rows' lines per
-  // instruction, more than the CU's L1 keeps for its waves); the group's
-  // fast sum decides the f32 distance (rr_end), and the rare undecided
-  // candidates take the reference's fold in one lane. ----
+  // instruction, more than the CU's L1 keeps for its waves); the groups'
+  // fast sums of 64 candidates are staged in LDS, and one lane per candidate
+  // decides its f32 distance from its sum (rr_end) or, in the rare cases
+  // the sum leaves open, by the reference's fold. ----
   bool badid = false;
   double first64 = 0.0;
   if (od64) {
@@ -792,30 +803,44 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   const int nch = d4 >> 2;  // 16-byte chunks per row
   const int lg = nch >= 64 ? 4 : nch >= 32 ? 3 : nch >= 16 ? 2 : nch >= 8 ? 1 : 0;  // uniform
   const int g = lane & ((1 << lg) - 1);
-  for (int c0 = 0; !od64 && c0 < np; c0 += 64 >> lg) {
-    const int c = c0 + (lane >> lg);
-    const bool valid = c < np;
-    const uint32_t j = valid ? cl[c] : 0u;
-    if (valid && (int64_t)j >= D.n) badid = true;  // never produced by the sweep
-    const bool live = valid && (int64_t)j < D.n && (int64_t)j != i;  // same in a group
-    gfloat* bp = (gfloat*)(D.xr + (size_t)j * kp);
-    double sum = 0.0, rnb = 0.0;
-    if (live) {
-      if (metric == 1) rnb = ((gf64*)D.rnorm)[j];
-      // two inlined copies, so the query reads keep their address space
-      sum = qlds ? rr_group_sum(QueryL{qs}, bp, g, lg, nch, metric)
-                 : rr_group_sum(QueryG{(gfloat*)qa}, bp, g, lg, nch, metric);
+  for (int c00 = 0; !od64 && c00 < np; c00 += 64) {
+    // the sums of candidates [c00, c00 + 64) into ss, by lane groups ...
+    const int cend = np - c00 < 64 ? np : c00 + 64;
+    for (int c0 = c00; c0 < cend; c0 += 64 >> lg) {
+      const int c = c0 + (lane >> lg);
+      const uint32_t j = c < cend ? cl[c] : 0u;
+      const bool live = c < cend && (int64_t)j < D.n && (int64_t)j != i;  // same in a group
+      double sum = 0.0;
+      if (live) {
+        gfloat* bp = (gfloat*)(D.xr + (size_t)j * kp);
+        // two inlined copies, so the query reads keep their address space
+        sum = qlds ? rr_group_sum(QueryL{qs}, bp, g, lg, nch, metric)
+                   : rr_group_sum(QueryG{(gfloat*)qa}, bp, g, lg, nch, metric);
+      }
+      sum = rr_group_reduce(sum, lg);
+      if (c < cend && g == 0) ss[c - c00] = sum;
     }
-    sum = rr_group_reduce(sum, lg);
-    // the interval's two ends: in lanes g = 0 and 1, or both in the lane
-    const double r = rna * rnb;
-    const float e0 = rr_end(sum, lg > 0 && (g & 1), d4, metric, r);
-    const float e1 = lg > 0 ? __shfl_xor(e0, 1, 64) : rr_end(sum, true, d4, metric, r);
-    float d32 = e0;
-    if (live && g == 0 && e0 != e1)
-      d32 = (float)(qlds ? rr_dist64(QueryL{qs}, (const float*)bp, d4, metric, rna, rnb)
-                         : rr_dist64(QueryG{(gfloat*)qa}, (const float*)bp, d4, metric, rna, rnb));
-    if (valid && g == 0) xd[c] = __float_as_uint(live ? d32 : 0.0f);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // ... then each decided in the lane of its own: both ends of the
+    // interval, and the reference's fold where they differ (rare)
+    const int c = c00 + lane;
+    if (c < cend) {
+      const uint32_t j = cl[c];
+      if ((int64_t)j >= D.n) badid = true;  // never produced by the sweep
+      float d32 = 0.0f;
+      if ((int64_t)j < D.n && (int64_t)j != i) {
+        const double rnb = metric == 1 ? ((gf64*)D.rnorm)[j] : 0.0, r = rna * rnb;
+        const double sum = ss[lane];
+        d32 = rr_end(sum, false, d4, metric, r);
+        if (d32 != rr_end(sum, true, d4, metric, r)) {
+          const float* bp = D.xr + (size_t)j * kp;
+          d32 = (float)(qlds ? rr_dist64(QueryL{qs}, bp, d4, metric, rna, rnb)
+                             : rr_dist64(QueryG{(gfloat*)qa}, bp, d4, metric, rna, rnb));
+        }
+      }
+      xd[c] = __float_as_uint(d32);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // ss is reused
   }
   if (__ballot(badid) != 0ull) {
     if (lane == 0) flag_row(gflag, grow, nflag);  // recomputed by the exact scan
@@ -889,7 +914,7 @@ __global__ __launch_bounds__(256, 6) void rerank_kernel(const DsDesc* __restrict
                                                         const uint32_t* __restrict__ rlist,
                                                         const int* __restrict__ rcount,
                                                         uint32_t* __restrict__ nflag) {
-  __shared__ uint64_t cs[4][768];
+  __shared__ uint64_t cs[4][RR_LDS];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: row data in SGPRs
   const int64_t cnt = rlist ? (int64_t)*rcount : total_rows;
   for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < cnt; p += (int64_t)gridDim.x * 4) {

@@ -582,6 +582,32 @@ def test_long_rows_rerank(ctx, shape, metric):
 # finite inputs whose squares leave the f32 range: |x| ~ 1e19 overflows the
 # f32 keys and bias (||x||^2 ~ 1e40), |x| ~ 1e-20 underflows the products;
 # the reference computes both in f64 (its distances stay finite)
+def test_rerank_distances_at_f32_rounding_midpoints(ctx):
+    """Candidate distances whose f64 value lies within 2^-44 of a midpoint
+    between two f32 values: the re-rank's fast sums cannot decide their f32
+    rounding, so these take the reference's sequential fold."""
+    rng = np.random.default_rng(29)
+    ncl, per, d, k = 40, 24, 128, 15
+    x = np.zeros((ncl * per, d), np.float32)
+    near = 0
+    for c in range(ncl):
+        rows = slice(c * per, (c + 1) * per)
+        x[rows, 2:] = rng.integers(-64, 65, size=d - 2) * 8.0  # far-apart centres
+        b = rng.uniform(0.5, 2.0, per - 1).astype(np.float32).astype(np.float64)
+        ulp = np.spacing(b.astype(np.float32)).astype(np.float64)
+        m = b + ulp / 2                      # midpoints between b and the next f32
+        v1 = np.sqrt(m * m - b * b).astype(np.float32)
+        x[c * per + 1:(c + 1) * per, 0] = b.astype(np.float32)
+        x[c * per + 1:(c + 1) * per, 1] = v1
+        dist = np.sqrt(b * b + v1.astype(np.float64) ** 2)   # from the centre row
+        near += int(np.sum(np.abs(dist - m) < 2.0 ** -44 * m))
+    assert near >= ncl * (per - 1) // 2   # most distances to a centre are undecided
+    ref = oracle_graph_mt(x, d, k, "euclidean")
+    for prec in ["bf16", "fp32"]:
+        res = H.knn_graph(x, d, k, "euclidean", precision=prec, ctx=ctx)
+        assert_graph_equal(res, *ref)
+
+
 @pytest.mark.parametrize("scale", [1e19, 1e-20, "mixed"])
 @pytest.mark.parametrize("metric", ["euclidean", "cosine"])
 def test_extreme_magnitudes(ctx, scale, metric):
