@@ -383,7 +383,8 @@ __device__ __forceinline__ float f32_round_up(double v) {
 // the next 8 in flight while these are folded (two buffers of 8: the
 // register cost of one 16-float batch, which six waves per SIMD allow; 32
 // floats spill). Same operations in the same order as ref_dist64.
-__device__ __forceinline__ void rr_fold4(double& s, const float4 x, const float4 y, int metric) {
+template <class X>  // float4, or double4 holding f32 values
+__device__ __forceinline__ void rr_fold4(double& s, const X x, const float4 y, int metric) {
   if (metric == 0) {
     double t;
     t = (double)x.x - (double)y.x; s = s + t * t;
@@ -418,20 +419,31 @@ struct QueryG {
   gfloat* q;
   __device__ __forceinline__ float4 x4(int f) const { return ld4(q + f); }
 };
+// the query's f32 values staged in LDS as f64 (exact), saving a conversion
+// per element and candidate
+struct QueryL64 {
+  const double* q;
+  __device__ __forceinline__ double4 x4(int f) const {
+    const double2 a = *(const double2*)(q + f), b = *(const double2*)(q + f + 2);
+    return make_double4(a.x, a.y, b.x, b.y);
+  }
+};
 // Accumulators of the fold: the reference's sequential one, or four partial
 // sums of fused multiply-adds over the same f64 differences (the fast path:
 // a quarter of the dependent chain and one f64 operation fewer per element;
 // its distance from the reference's sum is bounded in rr_end).
 struct AccExact {
   double s = 0.0;
-  __device__ __forceinline__ void add4(const float4 x, const float4 y, int metric) {
+  template <class X>
+  __device__ __forceinline__ void add4(const X x, const float4 y, int metric) {
     rr_fold4(s, x, y, metric);
   }
   __device__ __forceinline__ double sum() const { return s; }
 };
 struct AccFast {
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  __device__ __forceinline__ void add4(const float4 x, const float4 y, int metric) {
+  template <class X>
+  __device__ __forceinline__ void add4(const X x, const float4 y, int metric) {
     if (metric == 0) {
       const double t0 = (double)x.x - (double)y.x, t1 = (double)x.y - (double)y.y;
       const double t2 = (double)x.z - (double)y.z, t3 = (double)x.w - (double)y.w;
@@ -617,8 +629,19 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   double* ss = (double*)(qs + RR_QROW);  // fast sums of 64 candidates
   const float* qa = D.xr + (size_t)i * kp;
   const bool qlds = d4 <= RR_QROW;
-  if (qlds)
+  // without the FP64 outputs a query of at most RR_QROW / 2 floats is
+  // staged as f64 (the fast sums and their fallback fold read it)
+  double* qd = (double*)qs;
+  const bool q64 = !od64 && d4 <= RR_QROW / 2;
+  if (q64) {
+    for (int f = 4 * lane; f < d4; f += 256) {
+      const float4 v = ld4((gfloat*)qa + f);
+      *(double2*)(qd + f) = make_double2(v.x, v.y);
+      *(double2*)(qd + f + 2) = make_double2(v.z, v.w);
+    }
+  } else if (qlds) {
     for (int f = 4 * lane; f < d4; f += 256) *(float4*)(qs + f) = ld4((gfloat*)qa + f);
+  }
 
   // ---- error bound of the candidate keys for this row ----
   const double u = 0x1p-24;
@@ -813,9 +836,10 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
       double sum = 0.0;
       if (live) {
         gfloat* bp = (gfloat*)(D.xr + (size_t)j * kp);
-        // two inlined copies, so the query reads keep their address space
-        sum = qlds ? rr_group_sum(QueryL{qs}, bp, g, lg, nch, metric)
-                   : rr_group_sum(QueryG{(gfloat*)qa}, bp, g, lg, nch, metric);
+        // inlined copies, so the query reads keep their address space
+        sum = q64    ? rr_group_sum(QueryL64{qd}, bp, g, lg, nch, metric)
+              : qlds ? rr_group_sum(QueryL{qs}, bp, g, lg, nch, metric)
+                     : rr_group_sum(QueryG{(gfloat*)qa}, bp, g, lg, nch, metric);
       }
       sum = rr_group_reduce(sum, lg);
       if (c < cend && g == 0) ss[c - c00] = sum;
@@ -834,8 +858,9 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
         d32 = rr_end(sum, false, d4, metric, r);
         if (d32 != rr_end(sum, true, d4, metric, r)) {
           const float* bp = D.xr + (size_t)j * kp;
-          d32 = (float)(qlds ? rr_dist64(QueryL{qs}, bp, d4, metric, rna, rnb)
-                             : rr_dist64(QueryG{(gfloat*)qa}, bp, d4, metric, rna, rnb));
+          d32 = (float)(q64    ? rr_dist64(QueryL64{qd}, bp, d4, metric, rna, rnb)
+                        : qlds ? rr_dist64(QueryL{qs}, bp, d4, metric, rna, rnb)
+                               : rr_dist64(QueryG{(gfloat*)qa}, bp, d4, metric, rna, rnb));
         }
       }
       xd[c] = __float_as_uint(d32);
@@ -914,7 +939,7 @@ __global__ __launch_bounds__(256, 6) void rerank_kernel(const DsDesc* __restrict
                                                         const uint32_t* __restrict__ rlist,
                                                         const int* __restrict__ rcount,
                                                         uint32_t* __restrict__ nflag) {
-  __shared__ uint64_t cs[4][RR_LDS];
+  __shared__ __attribute__((aligned(16))) uint64_t cs[4][RR_LDS];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: row data in SGPRs
   const int64_t cnt = rlist ? (int64_t)*rcount : total_rows;
   for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < cnt; p += (int64_t)gridDim.x * 4) {
