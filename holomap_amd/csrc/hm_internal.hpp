@@ -160,6 +160,22 @@ struct WorkItem {
   int32_t qcnt;    // number of queries in the block
 };
 
+// The exact scan's work list, built by the last workgroup of the escalation
+// re-rank (rerank_kernel<true>): the escalated rows still flagged, compacted
+// in place in list / count, and their work items (items, nitems, xs_out: the
+// column splits); done counts the finished workgroups (zero between runs).
+struct FbTail {
+  uint32_t* list;
+  int* count;
+  int qb, xs, xs_cap;
+  int64_t item_cap, row_cap;
+  int target;
+  WorkItem* items;
+  int* nitems;
+  int* xs_out;
+  uint32_t* done;
+};
+
 // Precision / mode codes (mirrored in the public header).
 enum : int { PREC_EXACT = 0, PREC_FP32 = 1, PREC_BF16 = 2, PREC_BF16X3 = 3 };
 enum : int { CALIB_REF_F32 = 0, CALIB_F64 = 1 };

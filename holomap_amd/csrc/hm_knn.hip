@@ -933,18 +933,62 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
 // *rcount rows listed in rlist (global ids), grid-stride.
 // 6 KB of LDS and at most 80 VGPRs per wave: six waves per SIMD keep
 // enough rows in flight to cover their dependent memory round trips.
+template <int NT, int PER>
+__device__ void list_compact_block(const uint32_t* __restrict__ flag, uint32_t* __restrict__ list,
+                                   int* __restrict__ count);
+template <int NT>
+__device__ void fallback_items_block(const DsDesc* __restrict__ ds_arr, int nds,
+                                     const uint32_t* __restrict__ list,
+                                     const int* __restrict__ count, int qb, int xs, int xs_cap,
+                                     int64_t item_cap, int64_t row_cap, int target,
+                                     WorkItem* __restrict__ items, int* __restrict__ nitems,
+                                     int* __restrict__ xs_out);
+template <bool TAIL>
 __global__ __launch_bounds__(256, 6) void rerank_kernel(const DsDesc* __restrict__ ds_arr, int nds,
                                                         int64_t total_rows, int mode,
                                                         uint32_t* __restrict__ flag,
                                                         const uint32_t* __restrict__ rlist,
                                                         const int* __restrict__ rcount,
-                                                        uint32_t* __restrict__ nflag) {
+                                                        uint32_t* __restrict__ nflag,
+                                                        const FbTail tail) {
   __shared__ __attribute__((aligned(16))) uint64_t cs[4][RR_LDS];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: row data in SGPRs
   const int64_t cnt = rlist ? (int64_t)*rcount : total_rows;
   for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < cnt; p += (int64_t)gridDim.x * 4) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // previous row's LDS use is over
     rerank_row(ds_arr, nds, rlist ? (int64_t)rlist[p] : p, mode, flag, nflag, cs[wave]);
+  }
+  if constexpr (TAIL) {
+    // The last workgroup with rows to finish builds the exact scan's work
+    // list (it waits for nobody: it is last by its ticket; workgroups
+    // without rows write and read nothing, so take none, and with no rows
+    // at all workgroup 0 builds it). The barrier's release waits for the
+    // workgroup's stores; the ticket's thread then releases them at agent
+    // scope and the last workgroup acquires them all before reading the
+    // flags. (A flag read stale would still hold the 1 the escalated rows
+    // start with: the row would be recomputed exactly.)
+    __shared__ int last;
+    const int64_t nwork = cnt < 4 * (int64_t)gridDim.x ? (cnt + 3) / 4 : (int64_t)gridDim.x;
+    if ((int64_t)blockIdx.x >= (nwork > 0 ? nwork : 1)) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (nwork > 0) {
+        __threadfence();
+        last = atomicAdd(tail.done, 1u) == (uint32_t)nwork - 1u ? 1 : 0;
+      } else {
+        last = 1;
+      }
+    }
+    __syncthreads();
+    if (last) {  // uniform
+      __threadfence();
+      list_compact_block<256, 16>(flag, tail.list, tail.count);
+      __syncthreads();
+      fallback_items_block<256>(ds_arr, nds, tail.list, tail.count, tail.qb, tail.xs,
+                                tail.xs_cap, tail.item_cap, tail.row_cap, tail.target, tail.items,
+                                tail.nitems, tail.xs_out);
+      if (threadIdx.x == 0 && nwork > 0) *tail.done = 0u;  // the next run's count starts at zero
+    }
   }
 }
 
@@ -1217,15 +1261,10 @@ __device__ void list_compact_block(const uint32_t* __restrict__ flag, uint32_t* 
   __syncthreads();
   if (threadIdx.x == 0) *count = (int)base;
 }
-__global__ __launch_bounds__(512) void list_fallback_items_kernel(
-    const uint32_t* __restrict__ flag, uint32_t* __restrict__ list, int* __restrict__ count,
-    const DsDesc* __restrict__ ds_arr, int nds, int qb, int xs, int xs_cap, int64_t item_cap,
-    int64_t row_cap, int target, WorkItem* __restrict__ items, int* __restrict__ nitems,
-    int* __restrict__ xs_out) {
-  list_compact_block<512, 8>(flag, list, count);
-  __syncthreads();
-  fallback_items_block<512>(ds_arr, nds, list, count, qb, xs, xs_cap, item_cap, row_cap, target,
-                            items, nitems, xs_out);
-}
+
+template __global__ void rerank_kernel<false>(const DsDesc*, int, int64_t, int, uint32_t*,
+                                              const uint32_t*, const int*, uint32_t*, FbTail);
+template __global__ void rerank_kernel<true>(const DsDesc*, int, int64_t, int, uint32_t*,
+                                             const uint32_t*, const int*, uint32_t*, FbTail);
 
 }  // namespace hm

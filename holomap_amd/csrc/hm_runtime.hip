@@ -32,8 +32,9 @@ __global__ void exact_scan_kernel(const DsDesc*, const WorkItem*, const int*, in
                                   const uint32_t*, const int*, int, int, uint64_t*);
 __global__ void exact_merge_kernel(const DsDesc*, int, const uint32_t*, const int*, int64_t,
                                    const int*, int, int, int, const uint64_t*, uint32_t*, int);
+template <bool TAIL>
 __global__ void rerank_kernel(const DsDesc*, int, int64_t, int, uint32_t*, const uint32_t*,
-                              const int*, uint32_t*);
+                              const int*, uint32_t*, FbTail);
 __global__ void sweep_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int, int,
                                    double, WorkItem*, int*, int*);
 __global__ void fallback_items_kernel(const DsDesc*, int, const uint32_t*, const int*, int, int,
@@ -44,9 +45,6 @@ __global__ void flag_sweep_items_kernel(const uint32_t*, int64_t, uint32_t*, int
 __global__ void flag_fallback_items_kernel(const uint32_t*, int64_t, uint32_t*, int*,
                                            const DsDesc*, int, int, int, int, int64_t, int64_t,
                                            int, WorkItem*, int*, int*, const uint32_t*);
-__global__ void list_fallback_items_kernel(const uint32_t*, uint32_t*, int*, const DsDesc*, int,
-                                           int, int, int, int64_t, int64_t, int, WorkItem*, int*,
-                                           int*);
 template <int KM>
 __global__ void smooth_knn_kernel(const DsDesc*, int, int64_t, const double*, int, float*, float*,
                                   double*, uint32_t*, uint32_t, uint32_t*, uint32_t*, float*,
@@ -659,15 +657,23 @@ hm_status enqueue_pipeline(hm_job* J) {
                        (J->stages & HM_STAGE_FUZZY) ? J->inptr : nullptr, mode_bits);
     HM_LAUNCH(ctx);
     mark(J, 1);
+    // the exact scan's configuration (its work list may be built by the
+    // escalation re-rank's last workgroup)
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(J->fb_cap, 2 * cus));
+    // merge groups: G splits of k keys fit one 1024-key sort; two merge
+    // levels cover up to G * G splits
+    const int G = merge_group(J->kmax);
+    const int xs_cap = std::max(J->xs, std::min(512, G * G));
+    const int qpw = xs_qpw_for_k(J->kmax), qb = 4 * qpw;
     // ---- candidate sweep + re-rank ----
     if (J->mode >= 0) {
       // grid and item order go together (see the item permutation in hm_job_create)
       HM_TRY(launch_sweep_l1(J));
       mark(J, 2);
-      hipLaunchKernelGGL(rerank_kernel, dim3(std::min<int64_t>(grid_for(R, 4), 6 * (int64_t)cus)),
-                         dim3(256), 0, J->stream, J->d_desc,
-                         nds, R, J->mode, J->flag, (const uint32_t*)nullptr, (const int*)nullptr,
-                         J->nflag);
+      hipLaunchKernelGGL(rerank_kernel<false>,
+                         dim3(std::min<int64_t>(grid_for(R, 4), 6 * (int64_t)cus)), dim3(256), 0,
+                         J->stream, J->d_desc, nds, R, J->mode, J->flag, (const uint32_t*)nullptr,
+                         (const int*)nullptr, J->nflag, FbTail{});
       HM_LAUNCH(ctx);
       mark(J, 3);
       if (J->esc_mode >= 0) {
@@ -690,10 +696,15 @@ hm_status enqueue_pipeline(hm_job* J) {
         }
         HM_LAUNCH(ctx);
         HM_TRY(launch_sweep(J, em, J->items_l2, J->nitems_l2, 0, J->list, (unsigned)cus, 0, 2));
-        hipLaunchKernelGGL(rerank_kernel, dim3(std::max<int64_t>(1, std::min<int64_t>(cdiv(R, 4), 8 * cus))),
+        // its last workgroup also compacts the list to the rows still
+        // flagged and builds the exact scan's items
+        const FbTail tail{J->list, J->count, qb, J->xs, xs_cap, (int64_t)J->fb_cap, R, (int)gx,
+                          J->items_fb, J->nitems_fb, J->xs_fb, (uint32_t*)(J->nitems_l2 + 2)};
+        hipLaunchKernelGGL(rerank_kernel<true>,
+                           dim3(std::max<int64_t>(1, std::min<int64_t>(cdiv(R, 4), 4 * cus))),
                            dim3(256), 0, J->stream, J->d_desc, nds, R, em, J->flag,
                            (const uint32_t*)J->list, (const int*)(J->nitems_l2 + 1),
-                           (uint32_t*)nullptr);
+                           (uint32_t*)nullptr, tail);
         HM_LAUNCH(ctx);
       }
     } else {
@@ -704,29 +715,21 @@ hm_status enqueue_pipeline(hm_job* J) {
       mark(J, 3);
     }
     // ---- exact scan of the uncertified rows ----
-    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(J->fb_cap, 2 * cus));
-    // merge groups: G splits of k keys fit one 1024-key sort; two merge
-    // levels cover up to G * G splits
-    const int G = merge_group(J->kmax);
-    const int xs_cap = std::max(J->xs, std::min(512, G * G));
-    const int qpw = xs_qpw_for_k(J->kmax), qb = 4 * qpw;
-    if (J->mode >= 0 && J->esc_mode >= 0) {
-      // the rows still flagged are among the escalated ones: compact that list
-      hipLaunchKernelGGL(list_fallback_items_kernel, dim3(1), dim3(512), 0, J->stream,
-                         (const uint32_t*)J->flag, J->list, J->count, J->d_desc, nds, qb, J->xs,
-                         xs_cap, (int64_t)J->fb_cap, R, (int)gx, J->items_fb, J->nitems_fb,
-                         J->xs_fb);
-    } else if (R <= kScanSmall<uint32_t>) {  // compaction and items in one launch
-      hipLaunchKernelGGL(flag_fallback_items_kernel, dim3(1), dim3(kScanSmallThreads), 0,
-                         J->stream, (const uint32_t*)J->flag, R, J->list, J->count, J->d_desc, nds,
-                         qb, J->xs, xs_cap, (int64_t)J->fb_cap, R, (int)gx, J->items_fb,
-                         J->nitems_fb, J->xs_fb,
-                         J->mode >= 0 ? (const uint32_t*)J->nflag : (const uint32_t*)nullptr);
-    } else {
-      HM_TRY(compact_flags(J, J->flag, R, J->list, J->count));
-      hipLaunchKernelGGL(fallback_items_kernel, dim3(1), dim3(256), 0, J->stream, J->d_desc, nds,
-                         (const uint32_t*)J->list, (const int*)J->count, qb, J->xs, xs_cap,
-                         (int64_t)J->fb_cap, R, (int)gx, J->items_fb, J->nitems_fb, J->xs_fb);
+    // (after an escalation its work list is built: the rows still flagged
+    // are among the escalated ones)
+    if (!(J->mode >= 0 && J->esc_mode >= 0)) {
+      if (R <= kScanSmall<uint32_t>) {  // compaction and items in one launch
+        hipLaunchKernelGGL(flag_fallback_items_kernel, dim3(1), dim3(kScanSmallThreads), 0,
+                           J->stream, (const uint32_t*)J->flag, R, J->list, J->count, J->d_desc,
+                           nds, qb, J->xs, xs_cap, (int64_t)J->fb_cap, R, (int)gx, J->items_fb,
+                           J->nitems_fb, J->xs_fb,
+                           J->mode >= 0 ? (const uint32_t*)J->nflag : (const uint32_t*)nullptr);
+      } else {
+        HM_TRY(compact_flags(J, J->flag, R, J->list, J->count));
+        hipLaunchKernelGGL(fallback_items_kernel, dim3(1), dim3(256), 0, J->stream, J->d_desc,
+                           nds, (const uint32_t*)J->list, (const int*)J->count, qb, J->xs, xs_cap,
+                           (int64_t)J->fb_cap, R, (int)gx, J->items_fb, J->nitems_fb, J->xs_fb);
+      }
     }
 #define HM_XSCAN(QPW, QCAP)                                                                    \
   hipLaunchKernelGGL((exact_scan_kernel<QPW, QCAP>), dim3(gx), dim3(256), 0, J->stream,         \
@@ -1156,7 +1159,9 @@ static hm_status job_create(hm_ctx* ctx, uint32_t nds, const uint64_t* n, const 
       for (auto& h : P.dsh) cap += cdiv(h.n, sweep_rows(J->esc_mode)) * kSplitCap;
       J->cap_l2 = cap;
       HM_A(J->items_l2, std::max<int64_t>(cap, 1));
-      HM_A(J->nitems_l2, 2);  // items, rows escalated
+      HM_A(J->nitems_l2, 3);  // items, rows escalated, the escalation re-rank's finished workgroups
+      hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, J->stream, (uint32_t*)J->nitems_l2,
+                         (int64_t)3, 0u);
     }
     HM_A(J->xs_fb, 1);
     HM_A(J->items_fb, std::max<int64_t>(P.fb_cap, 1));
