@@ -516,7 +516,10 @@ def main():
                              "entry": "hm_knn_graph (host buffers in and out, per-call job)"}
     if not args.no_fp32 and args.precision != "fp32":
         log(f"{args.workload}: fp32 path")
-        dt32, st32, c32, res32 = run_path(ctx, x, d, k, metric, "fp32", max(3, args.steps // 2), 2)
+        # same margin and calibration mode as the line's own run, so the
+        # graphs compare like for like
+        dt32, st32, c32, res32 = run_path(ctx, x, d, k, metric, "fp32", max(3, args.steps // 2), 2,
+                                          margin=args.margin, calib=args.calib)
         same = bool(np.array_equal(res32.knn.indices, res.knn.indices) and
                     np.array_equal(res32.graph.vals.view(np.uint32), res.graph.vals.view(np.uint32)))
         out["fp32_path"] = {"ms_per_step": dt32 * 1e3, "pts_per_s": n / dt32,
