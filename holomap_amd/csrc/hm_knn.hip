@@ -873,41 +873,125 @@ __device__ __forceinline__ void rerank_row(const DsDesc* __restrict__ ds_arr, in
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
-  // ---- 5. the k smallest exact keys by rank ----
-  // (the lists padded to a multiple of 4 with keys above every real one, so
-  // the counts read 4 keys per 16-byte LDS read; np <= 512 slots)
-  const int np4 = (np + 3) & ~3;
-  if (lane < np4 - np) {
-    xd[np + lane] = 0xffffffffu;
-    cl[np + lane] = 0xffffffffu;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  const uint4* xd4 = (const uint4*)xd;
-  const uint4* cl4 = (const uint4*)cl;
-  uint64_t kth = 0;  // the k-th smallest exact key, found by the lane that holds it
-  for (int m = 0; 64 * m < np; ++m) {
-    const int e = lane + 64 * m;
-    const uint64_t mine = e < np ? dist_key(__uint_as_float(xd[e]), cl[e]) : ~0ull;
-    int rk = 0;
-    for (int f = 0; f < np4; f += 4) {  // same address in every lane: broadcasts
-      const uint4 xv = xd4[f >> 2], cv = cl4[f >> 2];
-      rk += (((uint64_t)xv.x << 32) | cv.x) < mine ? 1 : 0;
-      rk += (((uint64_t)xv.y << 32) | cv.y) < mine ? 1 : 0;
-      rk += (((uint64_t)xv.z << 32) | cv.z) < mine ? 1 : 0;
-      rk += (((uint64_t)xv.w << 32) | cv.w) < mine ? 1 : 0;
+  // ---- 5. the k smallest exact keys ----
+  uint64_t kth = 0;  // the k-th smallest exact key (its distance bits decide below)
+  if (np <= 64 || k > 64) {
+    // by rank counts (the lists padded to a multiple of 4 with keys above
+    // every real one, so the counts read 4 keys per 16-byte LDS read; np <=
+    // 512 slots); kth found by the lane that holds it
+    const int np4 = (np + 3) & ~3;
+    if (lane < np4 - np) {
+      xd[np + lane] = 0xffffffffu;
+      cl[np + lane] = 0xffffffffu;
     }
-    if (e < np && rk < k) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint4* xd4 = (const uint4*)xd;
+    const uint4* cl4 = (const uint4*)cl;
+    for (int m = 0; 64 * m < np; ++m) {
+      const int e = lane + 64 * m;
+      const uint64_t mine = e < np ? dist_key(__uint_as_float(xd[e]), cl[e]) : ~0ull;
+      int rk = 0;
+      for (int f = 0; f < np4; f += 4) {  // same address in every lane: broadcasts
+        const uint4 xv = xd4[f >> 2], cv = cl4[f >> 2];
+        rk += (((uint64_t)xv.x << 32) | cv.x) < mine ? 1 : 0;
+        rk += (((uint64_t)xv.y << 32) | cv.y) < mine ? 1 : 0;
+        rk += (((uint64_t)xv.z << 32) | cv.z) < mine ? 1 : 0;
+        rk += (((uint64_t)xv.w << 32) | cv.w) < mine ? 1 : 0;
+      }
+      if (e < np && rk < k) {
+        oidx[rk] = (uint32_t)mine;
+        odst[rk] = __uint_as_float((uint32_t)(mine >> 32));
+        if (od64) od64[rk] = m == 0 ? first64 : pair_dist64(D, i, (uint32_t)mine);
+        if (rk == k - 1) kth = mine;
+      }
+    }
+    // the one lane holding the k-th key: reduce by max (the others hold 0)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t t = shfl_xor_u64(kth, o);
+      kth = t > kth ? t : kth;
+    }
+  } else {
+    // more than 64 candidates: the k-th smallest distance dk by bisection on
+    // the distance bits (ballot counts; the bits of non-negative floats
+    // order as the values, as the keys compare them), then, if more than
+    // the k - #(dist < dk) still needed share dk, the largest column among
+    // them to take, by bisection on the columns; the k keys at or below
+    // (dk, that column) are compacted into ss and ranked among themselves
+    // (k counts of k instead of np of np)
+    const int mr = (np + 63) >> 6;  // rounds in use (uniform, <= 8)
+    uint32_t xv[8];
+    uint32_t dlo = 0xffffffffu, dhi = 0u;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int e = lane + 64 * m;
+      xv[m] = (m < mr && e < np) ? xd[e] : 0xffffffffu;
+      if (m < mr && e < np) {
+        dlo = xv[m] < dlo ? xv[m] : dlo;
+        dhi = xv[m] > dhi ? xv[m] : dhi;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint32_t a = (uint32_t)__shfl_xor((int)dlo, o, 64), b = (uint32_t)__shfl_xor((int)dhi, o, 64);
+      dlo = a < dlo ? a : dlo;
+      dhi = b > dhi ? b : dhi;
+    }
+    auto count_le = [&](uint32_t v) {  // #(dist <= v); slots past np hold ~0 and are not counted
+      int c = 0;
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+        if (m < mr) c += __popcll(__ballot(lane + 64 * m < np && xv[m] <= v));
+      return c;
+    };
+    while (dlo < dhi) {  // uniform; #(dist <= dhi) >= k throughout
+      const uint32_t mid = dlo + ((dhi - dlo) >> 1);
+      if (count_le(mid) >= k) dhi = mid; else dlo = mid + 1;
+    }
+    const uint32_t dk = dlo;
+    const int nlt = dk == 0u ? 0 : count_le(dk - 1u);
+    const int need = k - nlt, neq = count_le(dk) - nlt;  // 1 <= need <= neq
+    uint32_t cmax = 0xffffffffu;
+    if (neq > need) {
+      uint32_t clo = 0u, chi = 0xffffffffu;
+      while (clo < chi) {  // uniform: the least column bound taking `need` of the ties
+        const uint32_t mid = clo + ((chi - clo) >> 1);
+        int c = 0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+          if (m < mr) {
+            const int e = lane + 64 * m;
+            c += __popcll(__ballot(e < np && xv[m] == dk && cl[e] <= mid));
+          }
+        if (c >= need) chi = mid; else clo = mid + 1;
+      }
+      cmax = clo;
+    }
+    kth = ((uint64_t)dk << 32) | cmax;
+    uint64_t* wk = (uint64_t*)ss;  // the k <= 64 winners
+    int nw = 0;
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      if (m < mr) {
+        const int e = lane + 64 * m;
+        const uint64_t key = e < np ? dist_key(__uint_as_float(xv[m]), cl[e]) : ~0ull;
+        const bool win = e < np && key <= kth;
+        const uint64_t bal = __ballot(win);
+        const int pos = nw + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+        if (win && pos < 64) wk[pos] = key;  // (keys are distinct: nw == k <= 64)
+        nw += __popcll(bal);
+      }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < nw && lane < k) {  // nw == k
+      const uint64_t mine = wk[lane];
+      int rk = 0;
+      for (int f = 0; f < k; ++f) rk += wk[f] < mine ? 1 : 0;  // broadcasts
+      if (rk >= k) rk = k - 1;  // (unreachable with distinct keys)
       oidx[rk] = (uint32_t)mine;
       odst[rk] = __uint_as_float((uint32_t)(mine >> 32));
-      if (od64) od64[rk] = m == 0 ? first64 : pair_dist64(D, i, (uint32_t)mine);
-      if (rk == k - 1) kth = mine;
+      if (od64) od64[rk] = pair_dist64(D, i, (uint32_t)mine);
     }
-  }
-  // the one lane holding the k-th key: reduce by max (the others hold 0)
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const uint64_t t = shfl_xor_u64(kth, o);
-    kth = t > kth ? t : kth;
   }
 
   // ---- certification against every column outside the candidate set ----
