@@ -582,6 +582,27 @@ def test_long_rows_rerank(ctx, shape, metric):
 # finite inputs whose squares leave the f32 range: |x| ~ 1e19 overflows the
 # f32 keys and bias (||x||^2 ~ 1e40), |x| ~ 1e-20 underflows the products;
 # the reference computes both in f64 (its distances stay finite)
+@pytest.mark.parametrize("k", [15, 30])
+def test_rerank_many_candidates_with_tied_distances(ctx, k):
+    """Rows with more than 64 plausible candidates, most of them tied: 120
+    identical rows (distance 0 to each other, the k lowest column indices
+    win the ties) and a 70-row shell at one distance from a centre row."""
+    rng = np.random.default_rng(31)
+    n, d = 700, 16
+    x = rng.normal(size=(n, d)).astype(np.float32) * 4.0
+    x[100:220] = x[7]                    # 121 identical rows
+    c = x[300].copy()
+    for r in range(301, 371):            # 70 rows at distance 0.5 from row 300
+        v = np.zeros(d, np.float32)
+        v[r % d] = 0.5 if (r // d) % 2 == 0 else -0.5
+        x[r] = c + v
+    for metric in ["euclidean", "cosine"]:
+        ref = oracle_graph_mt(x, d, k, metric)
+        for prec in ["bf16", "fp32"]:
+            res = H.knn_graph(x, d, k, metric, precision=prec, ctx=ctx)
+            assert_graph_equal(res, *ref)
+
+
 def test_rerank_distances_at_f32_rounding_midpoints(ctx):
     """Candidate distances whose f64 value lies within 2^-44 of a midpoint
     between two f32 values: the re-rank's fast sums cannot decide their f32
